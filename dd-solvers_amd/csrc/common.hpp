@@ -1,0 +1,117 @@
+// Shared device/host helpers for the dd_solvers HIP library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "dd_solvers_hip.h"
+
+namespace dds {
+
+// ---------------------------------------------------------------------------------------
+// Error reporting: every entry point returns a dds_status; the message of the last failure
+// on the calling thread is kept for dds_last_error().
+// ---------------------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+
+#define DDS_REQUIRE(cond, ...)                \
+  do {                                        \
+    if (!(cond)) {                            \
+      ::dds::set_error(__VA_ARGS__);          \
+      return DDS_EINVAL;                      \
+    }                                         \
+  } while (0)
+
+#define DDS_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      ::dds::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),       \
+                       __FILE__, __LINE__);                                        \
+      return DDS_EHIP;                                                             \
+    }                                                                              \
+  } while (0)
+
+#define DDS_LAUNCHED()                                                             \
+  do {                                                                             \
+    hipError_t e_ = hipGetLastError();                                             \
+    if (e_ != hipSuccess) {                                                        \
+      ::dds::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_),  \
+                       __FILE__, __LINE__);                                        \
+      return DDS_EHIP;                                                             \
+    }                                                                              \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------------------
+// Storage types. fp16 is clang's _Float16; bf16 is kept as raw 16-bit words and widened
+// by a shift (exact).
+// ---------------------------------------------------------------------------------------
+struct bf16_t {
+  uint16_t bits;
+};
+
+__device__ __forceinline__ float widen(bf16_t v) {
+  return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+}
+__device__ __forceinline__ float widen(_Float16 v) { return static_cast<float>(v); }
+__device__ __forceinline__ float widen(float v) { return v; }
+__device__ __forceinline__ double widen(double v) { return v; }
+
+template <typename T>
+struct Widened {
+  using type = float;
+};
+template <>
+struct Widened<double> {
+  using type = double;
+};
+
+// 16-byte vector of storage elements (one global_load_dwordx4 per lane).
+template <typename T>
+struct alignas(16) Vec16 {
+  static constexpr int N = 16 / sizeof(T);
+  T v[N];
+};
+
+template <typename T>
+__device__ __forceinline__ Vec16<T> load16(const T* p) {
+  return *reinterpret_cast<const Vec16<T>*>(p);
+}
+
+// ---------------------------------------------------------------------------------------
+// Wave64 / block reductions (fixed order: butterfly within a wave, then waves in order).
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over the block; the result is valid in every thread. `scratch` holds >= nwaves T.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* scratch) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) scratch[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int i = 0; i < nw; ++i) s += scratch[i];
+  return s;
+}
+
+// Maximum number of per-block partial sums a reduction-producing kernel writes.
+constexpr int kMaxPartials = DDS_MAX_PARTIALS;
+
+}  // namespace dds

@@ -1,0 +1,252 @@
+// Sparse matrix-vector products: BSR (column-major blocks) and CSR, with an optional
+// x . y partial-sum epilogue so the PCG step gets p . q without another pass.
+#include "common.hpp"
+
+namespace dds {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void write_partial(double dot, double* partials, double* scratch) {
+  const double t = block_sum(dot, scratch);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+// ---------------------------------------------------------------------------------------
+// CSR: G lanes per row (grid-stride over rows), strided partial sums then a butterfly.
+// Accumulates in double whatever the value type.
+// ---------------------------------------------------------------------------------------
+template <typename T, typename I, int G>
+__global__ __launch_bounds__(kBlock) void csr_kernel(int64_t n_rows, const I* __restrict__ row_ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const T* __restrict__ val,
+                                                      const T* __restrict__ x, T* __restrict__ y,
+                                                      double* __restrict__ partials,
+                                                      const dds_pcg_state* __restrict__ gate) {
+  __shared__ double scratch[kBlock / 64];
+  if (gate != nullptr && gate->done) return;
+  const int lg = threadIdx.x & (G - 1);
+  const int64_t group = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G;
+  const int64_t n_groups = (int64_t)gridDim.x * kBlock / G;
+  double dot = 0.0;
+  for (int64_t row = group; row < n_rows; row += n_groups) {
+    const int64_t s = row_ptr[row];
+    const int64_t e = row_ptr[row + 1];
+    double acc = 0.0;
+    for (int64_t p = s + lg; p < e; p += G)
+      acc += static_cast<double>(val[p]) * static_cast<double>(x[col[p]]);
+    acc = group_sum<G>(acc);
+    if (lg == 0) {
+      y[row] = static_cast<T>(acc);
+      dot += acc * static_cast<double>(x[row]);
+    }
+  }
+  if (partials != nullptr) write_partial(dot, partials, scratch);
+}
+
+// ---------------------------------------------------------------------------------------
+// BSR, general block size: one lane per scalar row (the row-per-thread order of the
+// reference's RPT-I kernel: blocks of the row in stored order, then c ascending).
+// ---------------------------------------------------------------------------------------
+template <typename T, typename I>
+__global__ __launch_bounds__(kBlock) void bsr_row_kernel(int64_t n_block_rows, int b,
+                                                          const I* __restrict__ crow,
+                                                          const int32_t* __restrict__ col,
+                                                          const T* __restrict__ val,
+                                                          const T* __restrict__ x,
+                                                          T* __restrict__ y,
+                                                          double* __restrict__ partials,
+                                                          const dds_pcg_state* __restrict__ gate) {
+  __shared__ double scratch[kBlock / 64];
+  if (gate != nullptr && gate->done) return;
+  const int64_t n_rows = n_block_rows * b;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const int64_t bb = (int64_t)b * b;
+  double dot = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < n_rows; row += stride) {
+    const int64_t br = row / b;
+    const int r = (int)(row - br * b);
+    double acc = 0.0;
+    for (int64_t blk = crow[br]; blk < crow[br + 1]; ++blk) {
+      const T* v = val + blk * bb + r;
+      const int64_t off = (int64_t)col[blk] * b;
+      for (int c = 0; c < b; ++c)
+        acc += static_cast<double>(v[c * b]) * static_cast<double>(x[off + c]);
+    }
+    y[row] = static_cast<T>(acc);
+    dot += acc * static_cast<double>(x[row]);
+  }
+  if (partials != nullptr) write_partial(dot, partials, scratch);
+}
+
+// ---------------------------------------------------------------------------------------
+// BSR with a power-of-two block size B: G lanes stream the block row's contiguous value
+// segment (nnzb_row * B * B elements), so consecutive lanes read consecutive values. With
+// column-major blocks element q of the segment sits in row q % B, which is lane % B for
+// every lane because G is a multiple of B; lanes with equal lane % B are summed by a
+// butterfly over xor offsets G/2 .. B, leaving row r's result in lane r.
+// ---------------------------------------------------------------------------------------
+template <typename T, typename I, int B, int G>
+__global__ __launch_bounds__(kBlock) void bsr_seg_kernel(int64_t n_block_rows,
+                                                          const I* __restrict__ crow,
+                                                          const int32_t* __restrict__ col,
+                                                          const T* __restrict__ val,
+                                                          const T* __restrict__ x,
+                                                          T* __restrict__ y,
+                                                          double* __restrict__ partials,
+                                                          const dds_pcg_state* __restrict__ gate) {
+  static_assert((B & (B - 1)) == 0 && G % B == 0 && G <= 64, "bad BSR tiling");
+  constexpr int BB = B * B;
+  __shared__ double scratch[kBlock / 64];
+  if (gate != nullptr && gate->done) return;
+  const int lg = threadIdx.x & (G - 1);
+  const int64_t group = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G;
+  const int64_t n_groups = (int64_t)gridDim.x * kBlock / G;
+  double dot = 0.0;
+  for (int64_t br = group; br < n_block_rows; br += n_groups) {
+    const int64_t q0 = (int64_t)crow[br] * BB;
+    const int64_t q1 = (int64_t)crow[br + 1] * BB;
+    double acc = 0.0;
+    for (int64_t q = q0 + lg; q < q1; q += G) {
+      const int64_t blk = q / BB;
+      const int c = (int)((q / B) & (B - 1));
+      acc += static_cast<double>(val[q]) * static_cast<double>(x[(int64_t)col[blk] * B + c]);
+    }
+#pragma unroll
+    for (int off = G / 2; off >= B; off >>= 1) acc += __shfl_xor(acc, off, G);
+    if (lg < B) {
+      const int64_t row = br * B + lg;
+      y[row] = static_cast<T>(acc);
+      dot += acc * static_cast<double>(x[row]);
+    }
+  }
+  if (partials != nullptr) write_partial(dot, partials, scratch);
+}
+
+int64_t grid_for(int64_t work_items, bool capped) {
+  int64_t g = ceil_div(std::max<int64_t>(work_items, 1), kBlock);
+  const int64_t cap = capped ? kMaxPartials : (int64_t)1 << 20;
+  return std::min(g, cap);
+}
+
+template <typename T, typename I>
+int launch_csr(int64_t n_rows, const void* row_ptr, const int32_t* col, const void* values,
+               const void* x, void* y, int lanes, double* partials, int* n_partials,
+               const dds_pcg_state* gate, hipStream_t s) {
+  const I* rp = static_cast<const I*>(row_ptr);
+  const T* v = static_cast<const T*>(values);
+  const T* xv = static_cast<const T*>(x);
+  T* yv = static_cast<T*>(y);
+  const int64_t grid = grid_for(n_rows * lanes, partials != nullptr);
+  if (n_partials) *n_partials = partials ? (int)grid : 0;
+#define DDS_CSR(G)                                                                            \
+  hipLaunchKernelGGL((csr_kernel<T, I, G>), dim3((unsigned)grid), dim3(kBlock), 0, s, n_rows, \
+                     rp, col, v, xv, yv, partials, gate)
+  switch (lanes) {
+    case 1: DDS_CSR(1); break;
+    case 2: DDS_CSR(2); break;
+    case 4: DDS_CSR(4); break;
+    case 8: DDS_CSR(8); break;
+    case 16: DDS_CSR(16); break;
+    case 32: DDS_CSR(32); break;
+    default: DDS_CSR(64); break;
+  }
+#undef DDS_CSR
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+template <typename T, typename I>
+int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, const void* values,
+               const void* x, void* y, double* partials, int* n_partials,
+               const dds_pcg_state* gate, hipStream_t s) {
+  const I* cr = static_cast<const I*>(crow);
+  const T* v = static_cast<const T*>(values);
+  const T* xv = static_cast<const T*>(x);
+  T* yv = static_cast<T*>(y);
+  const bool capped = partials != nullptr;
+  int64_t grid;
+#define DDS_SEG(B, G)                                                                    \
+  do {                                                                                   \
+    grid = grid_for(nbr * G, capped);                                                    \
+    hipLaunchKernelGGL((bsr_seg_kernel<T, I, B, G>), dim3((unsigned)grid), dim3(kBlock), \
+                       0, s, nbr, cr, col, v, xv, yv, partials, gate);                   \
+  } while (0)
+  switch (b) {
+    case 2: DDS_SEG(2, 16); break;
+    case 4: DDS_SEG(4, 32); break;
+    case 8: DDS_SEG(8, 64); break;
+    default:
+      grid = grid_for(nbr * b, capped);
+      hipLaunchKernelGGL((bsr_row_kernel<T, I>), dim3((unsigned)grid), dim3(kBlock), 0, s, nbr,
+                         (int)b, cr, col, v, xv, yv, partials, gate);
+  }
+#undef DDS_SEG
+  if (n_partials) *n_partials = partials ? (int)grid : 0;
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+}  // namespace
+}  // namespace dds
+
+using namespace dds;
+
+extern "C" int dds_csrmv(int val_dtype, int idx_dtype, int64_t n_rows, const void* row_ptr,
+                         const int32_t* col, const void* values, const void* x, void* y,
+                         int lanes, double* dot_partials, int* n_partials,
+                         const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(n_rows >= 0, "dds_csrmv: negative size");
+  if (n_partials) *n_partials = 0;
+  if (n_rows == 0) return DDS_OK;
+  DDS_REQUIRE(row_ptr && x && y, "dds_csrmv: null pointer");
+  DDS_REQUIRE(lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0,
+              "dds_csrmv: lanes must be a power of two in [1, 64] (got %d)", lanes);
+  hipStream_t s = as_stream(stream);
+  if (val_dtype == DDS_F64 && idx_dtype == DDS_I64)
+    return launch_csr<double, int64_t>(n_rows, row_ptr, col, values, x, y, lanes, dot_partials,
+                                       n_partials, gate, s);
+  if (val_dtype == DDS_F64 && idx_dtype == DDS_I32)
+    return launch_csr<double, int32_t>(n_rows, row_ptr, col, values, x, y, lanes, dot_partials,
+                                       n_partials, gate, s);
+  if (val_dtype == DDS_F32 && idx_dtype == DDS_I64)
+    return launch_csr<float, int64_t>(n_rows, row_ptr, col, values, x, y, lanes, dot_partials,
+                                      n_partials, gate, s);
+  if (val_dtype == DDS_F32 && idx_dtype == DDS_I32)
+    return launch_csr<float, int32_t>(n_rows, row_ptr, col, values, x, y, lanes, dot_partials,
+                                      n_partials, gate, s);
+  set_error("dds_csrmv: unsupported dtypes (values %d, row_ptr %d)", val_dtype, idx_dtype);
+  return DDS_EUNSUPPORTED;
+}
+
+extern "C" int dds_bsrmv(int val_dtype, int idx_dtype, int64_t n_block_rows, int64_t block_size,
+                         const void* crow, const int32_t* col, const void* values, const void* x,
+                         void* y, double* dot_partials, int* n_partials,
+                         const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(n_block_rows >= 0 && block_size >= 1, "dds_bsrmv: bad sizes");
+  if (n_partials) *n_partials = 0;
+  if (n_block_rows == 0) return DDS_OK;
+  DDS_REQUIRE(crow && x && y, "dds_bsrmv: null pointer");
+  hipStream_t s = as_stream(stream);
+  if (val_dtype == DDS_F64 && idx_dtype == DDS_I64)
+    return launch_bsr<double, int64_t>(n_block_rows, block_size, crow, col, values, x, y,
+                                       dot_partials, n_partials, gate, s);
+  if (val_dtype == DDS_F64 && idx_dtype == DDS_I32)
+    return launch_bsr<double, int32_t>(n_block_rows, block_size, crow, col, values, x, y,
+                                       dot_partials, n_partials, gate, s);
+  if (val_dtype == DDS_F32 && idx_dtype == DDS_I64)
+    return launch_bsr<float, int64_t>(n_block_rows, block_size, crow, col, values, x, y,
+                                      dot_partials, n_partials, gate, s);
+  if (val_dtype == DDS_F32 && idx_dtype == DDS_I32)
+    return launch_bsr<float, int32_t>(n_block_rows, block_size, crow, col, values, x, y,
+                                      dot_partials, n_partials, gate, s);
+  set_error("dds_bsrmv: unsupported dtypes (values %d, crow %d)", val_dtype, idx_dtype);
+  return DDS_EUNSUPPORTED;
+}

@@ -1,0 +1,16 @@
+"""dd_solvers for MI355X: Additive/Hybrid Schwarz-preconditioned CG on hand-written HIP
+kernels (gfx950), with the Python API of mgr0dzicki/dd-solvers
+(src/dd_solvers/dd_solvers/__init__.py: `from .solvers import *`).
+
+Importing the package loads libdd_solvers_hip.so and fails if it is not built.
+"""
+
+from . import _lib  # noqa: F401  (fails loudly when the HIP library is missing)
+from .solvers import *  # noqa: F401,F403
+from .solvers import __all__ as _solvers_all
+from .bsr import FastBSR
+from .csr import csr_to_bsr, gather_csr, sort_csr
+from .gemv import gemv_strided_batched
+
+__all__ = [*_solvers_all, "FastBSR", "csr_to_bsr", "gather_csr", "sort_csr",
+           "gemv_strided_batched"]

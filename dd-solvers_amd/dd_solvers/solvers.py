@@ -1,0 +1,1106 @@
+"""Solvers: CG, Additive/Hybrid Schwarz preconditioners, local and coarse solvers.
+
+Drop-in for src/dd_solvers/dd_solvers/solvers.py: the same classes, constructor arguments,
+`setup` / `solve` / `destroy` contract, metadata keys and exceptions. The arithmetic runs in
+the HIP kernels of csrc/ through the C ABI (_lib); PyTorch only provides device memory and
+the current stream.
+
+What changes underneath (the contract does not):
+  * CG keeps every scalar and the norm test on the device (dds_pcg_state) and reads the
+    state back every `check_every` iterations instead of every iteration; kernels enqueued
+    after convergence are no-ops, so iterates, norms and counts are those of the
+    reference's loop (solvers.py:228-297). Optionally the iteration block is captured in a
+    hipGraph and replayed.
+  * The coarse solver slot gets `DenseCholesky`, a hand-written dense Cholesky factor and
+    triangular solve, instead of cuDSS.
+  * Subdomains may be uneven and overlapping (keyword `decomposition=` of `setup`); the
+    reference's equal, contiguous, non-overlapping case is the default.
+"""
+
+from __future__ import annotations
+
+import gc
+from abc import ABC, abstractmethod
+from dataclasses import dataclass
+from typing import Any, NamedTuple, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .bsr import FastBSR
+from .csr import csr_rows, csr_to_bsr, csr_to_torch, inverse_permutation, permute_csr_tensor
+from .csr import asm_permutation  # noqa: F401  (re-exported)
+from .gemv import gemv_strided_batched
+
+__all__ = [
+    "Metadata",
+    "ConvergenceNotAchieved",
+    "collect",
+    "timeit",
+    "SparseSolver",
+    "DenseCholesky",
+    "CG",
+    "DenseBatchSolver",
+    "CSRBatch",
+    "SparseBatchSolver",
+    "Inv",
+    "LU",
+    "Cholesky",
+    "SchwarzOperator",
+    "AdditiveSchwarz",
+    "HybridSchwarz",
+    "csr_to_torch",
+    "permute_csr_tensor",
+    "inverse_permutation",
+    "asm_permutation",
+]
+
+Metadata = dict[str, Any]
+
+
+class ConvergenceNotAchieved(Exception):
+    """Raised by CG when the iteration limit is reached (solvers.py:28-37)."""
+
+    def __init__(self, metadata: Metadata):
+        self.metadata = metadata
+        if "iterations" not in self.metadata:
+            msg = "Convergence not achieved."
+        else:
+            msg = f"Convergence not achieved in {self.metadata['iterations']} iterations."
+        super().__init__(msg)
+
+
+def collect():
+    """Collect garbage and return cached device memory (solvers.py:40-45)."""
+    gc.collect()
+    if torch.cuda.is_available():
+        torch.cuda.empty_cache()
+
+
+def timeit(fun, warmup_iters=10, iters=10, repetitions=20):
+    """Minimum over repetitions of the mean event-timed duration in ms (solvers.py:48-65)."""
+    times = []
+    for _ in range(repetitions):
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        for _ in range(warmup_iters):
+            fun()
+        start.record()
+        for _ in range(iters):
+            fun()
+        end.record()
+        torch.cuda.synchronize()
+        times.append(start.elapsed_time(end) / iters)
+    return min(times)
+
+
+class SparseSolver(ABC):
+    @abstractmethod
+    def setup(self, matrix: torch.Tensor, *args, **kwargs) -> Metadata | None:
+        """Set the solver up for the given matrix (sparse CSR)."""
+
+    @abstractmethod
+    def solve(self, rhs: torch.Tensor) -> Tuple[torch.Tensor, Metadata]:
+        pass
+
+    @abstractmethod
+    def destroy(self) -> None:
+        pass
+
+
+def _dense_from(matrix: torch.Tensor) -> torch.Tensor:
+    if matrix.layout in (torch.sparse_csr, torch.sparse_coo, torch.sparse_bsr, torch.sparse_csc):
+        return matrix.to_dense()
+    return matrix
+
+
+# =======================================================================================
+# Coarse solver
+# =======================================================================================
+
+
+class DenseCholesky(SparseSolver):
+    """Coarse-space slot: dense Cholesky factorisation and triangular solves in HIP.
+
+    Replaces the reference's `CUDSS()` (solvers.py:85-113), an SPD sparse Cholesky of the
+    coarse matrix. The coarse matrices of these solvers are small and dense-ish (n_c up
+    to a few thousand), so the factor is stored dense (row-major, lower triangle) and
+    computed by a blocked right-looking Cholesky (dds_chol_factor).
+
+    mode "trsv"    each solve runs forward and backward substitution in one workgroup
+                   (dds_chol_solve); the least traffic per solve for small n.
+    mode "inverse" setup also forms A^-1 = L^-T L^-1 (dds_chol_inverse) and each solve is
+                   one dense symmetric matvec over all CUs (dds_dense_symv); for large n.
+    mode "auto"    "trsv" below `inverse_threshold` unknowns, else "inverse".
+    """
+
+    def __init__(self, mode: str = "auto", inverse_threshold: int = 1024):
+        if mode not in ("auto", "trsv", "inverse"):
+            raise ValueError(f"unknown mode {mode!r}")
+        self.mode = mode
+        self.inverse_threshold = inverse_threshold
+        self.L = None
+        self.Ainv = None
+
+    def __str__(self):
+        return "DenseCholesky"
+
+    def setup(self, matrix: torch.Tensor, *args, **kwargs) -> None:
+        dense = _dense_from(matrix)
+        _lib.require_device(dense)
+        if dense.dtype not in (torch.float32, torch.float64):
+            raise ValueError(f"unsupported coarse dtype {dense.dtype}")
+        n = dense.shape[0]
+        if dense.shape != (n, n):
+            raise ValueError("coarse matrix must be square")
+        self.dtype = dense.dtype
+        self.n = n
+        self.L = dense.contiguous().clone()
+        info = torch.zeros(1, dtype=torch.int32, device=dense.device)
+        with torch.cuda.device(dense.device):
+            _lib.call("dds_chol_factor", _lib.dtype_code(self.dtype), n, self.L.data_ptr(), n,
+                      info.data_ptr(), _lib.stream())
+            bad = int(info.item())
+            if bad:
+                raise np.linalg.LinAlgError(
+                    f"coarse matrix is not positive definite (pivot {bad - 1})")
+            mode = self.mode
+            if mode == "auto":
+                mode = "inverse" if n >= self.inverse_threshold else "trsv"
+            if mode == "trsv" and n > 6144:
+                mode = "inverse"
+            self.active_mode = mode
+            if mode == "inverse":
+                self.Ainv = torch.empty_like(self.L)
+                _lib.call("dds_chol_inverse", _lib.dtype_code(self.dtype), n, self.L.data_ptr(),
+                          n, self.Ainv.data_ptr(), n, _lib.stream())
+            else:
+                self.Ainv = None
+
+    def solve(self, rhs: torch.Tensor) -> Tuple[torch.Tensor, Metadata]:
+        if rhs.dtype != self.dtype:
+            raise AssertionError(f"rhs dtype {rhs.dtype} != coarse dtype {self.dtype}")
+        _lib.require_device(rhs)
+        rhs = rhs.contiguous()
+        y = torch.empty_like(rhs)
+        with torch.cuda.device(rhs.device):
+            self._enqueue_solve(rhs, y, None)
+        return y, {}
+
+    def _enqueue_solve(self, x, y, gate) -> None:
+        dt = _lib.dtype_code(self.dtype)
+        if self.Ainv is not None:
+            _lib.call("dds_dense_symv", dt, self.n, self.Ainv.data_ptr(), self.n, x.data_ptr(),
+                      y.data_ptr(), _lib.ptr(gate), _lib.stream())
+        else:
+            _lib.call("dds_chol_solve", dt, self.n, self.L.data_ptr(), self.n, x.data_ptr(),
+                      y.data_ptr(), _lib.ptr(gate), _lib.stream())
+
+    @property
+    def bytes_per_solve(self) -> int:
+        e = 8 if self.dtype == torch.float64 else 4
+        if self.Ainv is not None:
+            return e * self.n * self.n + 2 * e * self.n
+        return e * self.n * (self.n + 1) + 4 * e * self.n  # two sweeps of the triangle
+
+    def destroy(self):
+        self.L = None
+        self.Ainv = None
+
+
+# =======================================================================================
+# Operators
+# =======================================================================================
+
+
+class _CSROperator:
+    """y = A x for a torch sparse CSR matrix (dds_csrmv); used by CG(bsr_matmul=False)
+    and for R0A in HybridSchwarz."""
+
+    def __init__(self, matrix: torch.Tensor):
+        if matrix.layout is not torch.sparse_csr:
+            matrix = matrix.to_sparse_csr()
+        vals = matrix.values()
+        _lib.require_device(vals)
+        if vals.dtype not in (torch.float64, torch.float32):
+            raise ValueError(f"unsupported CSR dtype {vals.dtype}")
+        self.shape = matrix.shape
+        self.dtype = vals.dtype
+        self.device = vals.device
+        crow = matrix.crow_indices()
+        self.crow = (crow if crow.dtype in (torch.int32, torch.int64) else crow.to(torch.int64)).contiguous()
+        self.col = matrix.col_indices().to(torch.int32).contiguous()
+        self.values = vals.contiguous()
+        self.nnz = int(self.col.shape[0])
+        avg = self.nnz / max(self.shape[0], 1)
+        lanes = 1
+        while lanes < 64 and lanes < avg:
+            lanes *= 2
+        self.lanes = lanes
+
+    def __matmul__(self, x):
+        y = torch.empty(self.shape[0], dtype=self.dtype, device=self.device)
+        with torch.cuda.device(self.device):
+            self._enqueue_matvec(x.contiguous(), y, None, None)
+        return y
+
+    def _enqueue_matvec(self, x, y, partials, gate) -> int:
+        return _lib.call_partials(
+            "dds_csrmv", _lib.dtype_code(self.dtype), _lib.dtype_code(self.crow.dtype),
+            self.shape[0], self.crow.data_ptr(), self.col.data_ptr(), self.values.data_ptr(),
+            x.data_ptr(), y.data_ptr(), self.lanes, _lib.ptr(partials), _lib.PARTIALS_OUT,
+            _lib.ptr(gate), _lib.stream())
+
+    @property
+    def bytes_per_matvec(self) -> int:
+        e = self.values.element_size()
+        return (self.nnz * (e + 4) + self.crow.element_size() * (self.shape[0] + 1)
+                + e * (self.shape[0] + self.shape[1]))
+
+
+class _TorchOperator:
+    """Fallback operator for matrices the kernels do not take (dense, other layouts)."""
+
+    def __init__(self, matrix):
+        self.m = matrix
+        self.shape = matrix.shape
+
+    def __matmul__(self, x):
+        return self.m @ x
+
+
+# =======================================================================================
+# CG
+# =======================================================================================
+
+_STATE_DTYPE = np.dtype([("atol", "f8"), ("rho", "f8"), ("rho_prev", "f8"), ("pq", "f8"),
+                         ("alpha", "f8"), ("beta", "f8"), ("rr", "f8"), ("it", "i4"),
+                         ("done", "i4"), ("maxiter", "i4"), ("pad", "i4")])
+assert _STATE_DTYPE.itemsize == 72
+
+
+def _state_tensor(atol: float, maxiter: int, device) -> torch.Tensor:
+    s = np.zeros(1, dtype=_STATE_DTYPE)
+    s["atol"] = atol
+    s["maxiter"] = maxiter
+    return torch.from_numpy(s.view(np.uint8).copy()).to(device)
+
+
+def _read_state(t: torch.Tensor) -> np.void:
+    return t.cpu().numpy().view(_STATE_DTYPE)[0]
+
+
+class _PCGBuffers:
+    """Device buffers of one CG solve (reused across solves of the same size)."""
+
+    def __init__(self, n: int, maxiter: int, dtype, device):
+        self.n = n
+        self.maxiter = maxiter
+        self.r = torch.empty(n, dtype=dtype, device=device)
+        self.z = torch.empty(n, dtype=dtype, device=device)
+        self.p = torch.empty(n, dtype=dtype, device=device)
+        self.q = torch.empty(n, dtype=dtype, device=device)
+        self.partials = torch.zeros(_lib.MAX_PARTIALS, dtype=torch.float64, device=device)
+        self.log = torch.zeros((3, maxiter + 1), dtype=torch.float64, device=device)
+        self.state = _state_tensor(0.0, maxiter, device)
+        self.norm_out = torch.zeros(1, dtype=torch.float64, device=device)
+        self.x = torch.zeros(n, dtype=dtype, device=device)
+        self.b = torch.zeros(n, dtype=dtype, device=device)
+        for v in (self.r, self.z, self.p, self.q):
+            v.zero_()
+
+
+class CG(SparseSolver):
+    """Preconditioned conjugate gradients (solvers.py:192-303).
+
+    Extra keyword arguments (no counterpart in the reference):
+      check_every  iterations enqueued between two reads of the device-side state.
+      use_graph    capture one block of `check_every` iterations in a hipGraph once and
+                   replay it (single-RHS fast path only).
+    """
+
+    def __init__(self, preconditioner: SparseSolver | None = None, rtol: float = 1e-7,
+                 atol: float = 0, maxiter: int | None = 1000, bsr_matmul: bool = True,
+                 estimate_cond: bool = False, check_every: int = 8, use_graph: bool = False):
+        self.preconditioner = preconditioner
+        self.rtol = rtol
+        self.atol = atol
+        self.maxiter = maxiter
+        self.bsr_matmul = bsr_matmul
+        self.estimate_cond = estimate_cond
+        self.check_every = max(1, int(check_every))
+        self.use_graph = use_graph
+        self.matrix = None
+        self._buf = None
+        self._graph = None
+
+    def __str__(self):
+        return f"CG({self.preconditioner or ''}, bsr_matmul={self.bsr_matmul})"
+
+    def setup(self, matrix: torch.Tensor, *args, **kwargs) -> Metadata | None:
+        meta = None
+        if self.preconditioner is not None:
+            meta = self.preconditioner.setup(matrix, *args, **kwargs)
+        if self.bsr_matmul:
+            if "block_size" in kwargs:
+                block_size = kwargs["block_size"]
+            else:
+                raise ValueError("Unable to determine block size for BSR matrix multiplication.")
+            self.matrix = FastBSR(csr_to_bsr(matrix, block_size=block_size))
+        elif matrix.layout is torch.sparse_csr and matrix.values().is_cuda and \
+                matrix.values().dtype in (torch.float32, torch.float64):
+            self.matrix = _CSROperator(matrix)
+        else:
+            self.matrix = _TorchOperator(matrix)
+        self._buf = None
+        self._graph = None
+        return meta
+
+    # -- helpers ----------------------------------------------------------------------
+    def _fast(self) -> bool:
+        ok_op = isinstance(self.matrix, (FastBSR, _CSROperator)) and \
+            self.matrix.dtype == torch.float64
+        ok_pc = self.preconditioner is None or getattr(self.preconditioner, "_device_apply", False)
+        return ok_op and ok_pc
+
+    def _norm(self, v: torch.Tensor, buf: _PCGBuffers) -> float:
+        np_ = _lib.call_partials("dds_dot_partials", v.shape[0], v.data_ptr(), v.data_ptr(),
+                                 buf.partials.data_ptr(), _lib.PARTIALS_OUT, None, _lib.stream())
+        _lib.call("dds_sum_partials", buf.partials.data_ptr(), np_, buf.norm_out.data_ptr(),
+                  _lib.stream())
+        return float(buf.norm_out.item()) ** 0.5
+
+    def _fin(self, op: int, count: int, buf: _PCGBuffers) -> None:
+        lp = buf.log.data_ptr()
+        stride = buf.log.stride(0) * 8
+        _lib.call("dds_pcg_finalize", op, buf.partials.data_ptr(), count, buf.state.data_ptr(),
+                  lp, lp + stride, lp + 2 * stride, _lib.stream())
+
+    def _enqueue_iteration(self, x: torch.Tensor, buf: _PCGBuffers) -> None:
+        st = buf.state.data_ptr()
+        n = buf.n
+        s = _lib.stream()
+        if self.preconditioner is not None:
+            cnt = self.preconditioner._enqueue_apply(buf.r, buf.z, buf.partials, buf.state)
+            z = buf.z
+        else:
+            cnt = _lib.call_partials("dds_dot_partials", n, buf.r.data_ptr(), buf.r.data_ptr(),
+                                     buf.partials.data_ptr(), _lib.PARTIALS_OUT, st, s)
+            z = buf.r
+        self._fin(_lib.FIN_RHO, cnt, buf)
+        _lib.call("dds_pcg_update_p", n, z.data_ptr(), buf.p.data_ptr(), st, s)
+        cnt = self.matrix._enqueue_matvec(buf.p, buf.q, buf.partials, buf.state)
+        self._fin(_lib.FIN_ALPHA, cnt, buf)
+        cnt = _lib.call_partials("dds_pcg_update_xr", n, buf.p.data_ptr(), buf.q.data_ptr(),
+                                 x.data_ptr(), buf.r.data_ptr(), buf.partials.data_ptr(),
+                                 _lib.PARTIALS_OUT, st, s)
+        self._fin(_lib.FIN_NEXT, cnt, buf)
+
+    def _metadata(self, norms, n_it, buf, alphas=None, betas=None) -> Metadata:
+        meta = {
+            "iterations": len(norms) - 1,
+            "residual norms": norms,
+            "preconditioner metadata": [{} for _ in range(len(norms) - 1)],
+        }
+        if self.estimate_cond:
+            m = n_it
+            T = np.zeros((m, m))
+            for i in range(m):
+                T[i, i] = 1.0 / alphas[i]
+                if i > 0:
+                    T[i, i] += betas[i] / alphas[i - 1]
+                    T[i - 1, i] = T[i, i - 1] = betas[i] ** 0.5 / alphas[i - 1]
+            if m:
+                sv = np.linalg.svd(T, compute_uv=False)
+                meta["condition number estimate"] = float(sv[0] / sv[-1])
+            else:
+                meta["condition number estimate"] = float("nan")
+        return meta
+
+    # -- solve ------------------------------------------------------------------------
+    def solve(self, rhs: torch.Tensor, x0=None) -> Tuple[torch.Tensor, Metadata]:
+        if self.matrix is None:
+            raise RuntimeError("CG.solve called before setup")
+        _lib.require_device(rhs)
+        with torch.cuda.device(rhs.device):
+            if self._fast() and rhs.dtype == torch.float64 and rhs.dim() == 1:
+                return self._solve_device(rhs.contiguous(), x0)
+            return self._solve_generic(rhs, x0)
+
+    def _solve_device(self, rhs, x0):
+        n = rhs.shape[0]
+        # BSR with b not dividing n pads the operator; the PCG vectors carry a zero tail
+        # (empty padded rows keep it zero through every update).
+        n_pad = max(n, int(self.matrix.shape[0]))
+        maxiter = self.maxiter or n * 10
+        buf = self._buf
+        if buf is None or buf.n != n_pad or buf.maxiter != maxiter:
+            buf = self._buf = _PCGBuffers(n_pad, maxiter, rhs.dtype, rhs.device)
+            self._graph = None
+        s = _lib.stream()
+        b = rhs
+        if n_pad > n:
+            buf.b[:n] = rhs
+            b = buf.b
+        atol = max(self.atol, self.rtol * self._norm(b, buf))  # solvers.py:229
+        buf.state.copy_(_state_tensor(atol, maxiter, rhs.device))
+        # working x: buf.x (the address a captured graph holds), copied in and out
+        x = buf.x
+        if x0 is not None:
+            x[:n] = x0
+        else:
+            x.zero_()
+        if self.preconditioner is not None and hasattr(self.preconditioner, "T0"):
+            x[:n] += self.preconditioner.T0(rhs)
+        self.matrix._enqueue_matvec(x, buf.q, None, None)
+        cnt = _lib.call_partials("dds_pcg_residual", n_pad, b.data_ptr(), buf.q.data_ptr(),
+                                 buf.r.data_ptr(), buf.partials.data_ptr(), _lib.PARTIALS_OUT,
+                                 None, s)
+        self._fin(_lib.FIN_CHECK, cnt, buf)
+        done_flag = buf.state[60:64].view(torch.int32)
+        enqueued = 0
+        if self.use_graph:
+            if self._graph is None:
+                self._capture(x, buf)
+            while enqueued < maxiter:
+                self._graph.replay()
+                enqueued += self.check_every
+                if int(done_flag.item()):
+                    break
+        else:
+            while enqueued < maxiter:
+                k = min(self.check_every, maxiter - enqueued)
+                for _ in range(k):
+                    self._enqueue_iteration(x, buf)
+                enqueued += k
+                if int(done_flag.item()):
+                    break
+        stv = _read_state(buf.state)
+        it, done = int(stv["it"]), int(stv["done"])
+        n_norms = it + 1 if done == 1 else it
+        log = buf.log[:, :max(n_norms, 1)].cpu().numpy()
+        norms = [float(v) for v in log[0, :n_norms]]
+        meta = self._metadata(norms, it, buf,
+                              log[1] if self.estimate_cond else None,
+                              log[2] if self.estimate_cond else None)
+        if x0 is not None:  # the reference mutates and returns x0 (solvers.py:232, :266)
+            x0.copy_(x[:n])
+            out = x0
+        else:
+            out = x[:n].clone()
+        if n_norms == maxiter:  # the loop index reached maxiter - 1 (solvers.py:294-295)
+            raise ConvergenceNotAchieved(meta)
+        return out, meta
+
+    def _capture(self, x, buf):
+        """Capture `check_every` iterations on a side stream into a hipGraph."""
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        # Capture with the solve already converged would record nothing different: every
+        # kernel checks the gate at run time, not at capture time.
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(g, stream=side):
+                for _ in range(self.check_every):
+                    self._enqueue_iteration(x, buf)
+        torch.cuda.current_stream().wait_stream(side)
+        self._graph = g
+
+    def _solve_generic(self, rhs, x0):
+        """Reference control flow with a host check per iteration, for preconditioners and
+        operators outside this package (solvers.py:228-297)."""
+        atol = max(self.atol, self.rtol * float(torch.linalg.norm(rhs).item()))
+        n = len(rhs)
+        maxiter = self.maxiter or n * 10
+        x = x0 if x0 is not None else torch.zeros_like(rhs)
+        if self.preconditioner is not None and hasattr(self.preconditioner, "T0"):
+            x += self.preconditioner.T0(rhs)
+        r = rhs - self.matrix @ x
+        rho_prev, p = None, None
+        norms, pmeta, alphas, betas = [], [], [], []
+        i = -1
+        for i in range(maxiter):
+            res = float(torch.linalg.norm(r).item())
+            norms.append(res)
+            if res < atol:
+                break
+            z, m = self.preconditioner.solve(r) if self.preconditioner is not None else (r, {})
+            pmeta.append(m)
+            rho = torch.dot(r, z)
+            if i > 0:
+                beta = rho / rho_prev
+                p.mul_(beta).add_(z)
+            else:
+                p = z.clone()
+            q = self.matrix @ p
+            alpha = rho / torch.dot(p, q)
+            x.add_(alpha * p)
+            r.sub_(alpha * q)
+            rho_prev = rho
+            if self.estimate_cond:
+                betas.append(float(beta) if i > 0 else 0.0)
+                alphas.append(float(alpha))
+        meta = self._metadata(norms, len(alphas), None, alphas, betas)
+        meta["preconditioner metadata"] = pmeta
+        if i + 1 == maxiter:
+            raise ConvergenceNotAchieved(meta)
+        return x, meta
+
+    def destroy(self) -> None:
+        if self.preconditioner is not None:
+            self.preconditioner.destroy()
+            self.preconditioner = None
+        self.matrix = None
+        self._buf = None
+        self._graph = None
+
+
+# =======================================================================================
+# Local solvers
+# =======================================================================================
+
+
+class DenseBatchSolver(ABC):
+    @abstractmethod
+    def setup(self, matrix_batch: torch.Tensor):
+        """matrix_batch has shape (batch_size, n, n)."""
+
+    @abstractmethod
+    def solve(self, rhs_batch: torch.Tensor) -> torch.Tensor:
+        pass
+
+
+class CSRBatch(NamedTuple):
+    nrows: torch.Tensor
+    nnz: torch.Tensor
+    values: torch.Tensor
+    col_indices: torch.Tensor
+    row_offsets: torch.Tensor
+
+
+class SparseBatchSolver(ABC):
+    @abstractmethod
+    def setup(self, matrix_batch: CSRBatch):
+        pass
+
+    @abstractmethod
+    def solve(self, rhs_batch: torch.Tensor) -> torch.Tensor:
+        pass
+
+
+class Inv(DenseBatchSolver):
+    """Explicit local inverses applied by a batched GEMV (solvers.py:343-359).
+
+    The inverse is computed in the dtype of the matrices handed to `setup` (the
+    preconditioner precision) and stored in `precision`. Extra keywords (no counterpart
+    in the reference; they matter inside the Schwarz preconditioners):
+      accumulate  accumulator dtype of the local apply (default: the vector dtype, as the
+                  reference); torch.float32 gives FP32 accumulation for FP64 vectors.
+      scaled      store A_i^-1 / max|A_i^-1| per subdomain and rescale the result (keeps
+                  FP16 storage in range).
+    """
+
+    def __init__(self, precision: torch.dtype | None = None, accumulate: torch.dtype | None = None,
+                 scaled: bool = False):
+        self.precision = precision
+        self.accumulate = accumulate
+        self.scaled = scaled
+
+    def __str__(self):
+        return f"Inv({self.precision or ''})"
+
+    def setup(self, matrix_batch: torch.Tensor):
+        inv = torch.linalg.inv(matrix_batch).to(self.precision or matrix_batch.dtype)
+        # column-major storage, as gemv_strided_batched requires
+        self.matrix_inv = inv.transpose(1, 2).contiguous().transpose(1, 2)
+
+    def solve(self, rhs_batch: torch.Tensor) -> torch.Tensor:
+        return gemv_strided_batched(self.matrix_inv,
+                                    rhs_batch.reshape(self.matrix_inv.shape[0], -1))
+
+
+class LU(DenseBatchSolver):
+    """Local solves by batched LU without pivoting (solvers.py:362-373), torch.linalg."""
+
+    def __str__(self):
+        return "LU"
+
+    def setup(self, matrix_batch: torch.Tensor):
+        self.matrix_lu = torch.linalg.lu_factor(matrix_batch, pivot=False)
+        self.n_solvers = matrix_batch.shape[0]
+
+    def solve(self, rhs_batch: torch.Tensor) -> torch.Tensor:
+        return torch.linalg.lu_solve(*self.matrix_lu, rhs_batch.reshape(self.n_solvers, -1, 1))
+
+
+class Cholesky(DenseBatchSolver):
+    """Local solves by batched Cholesky (solvers.py:376-387), torch.linalg."""
+
+    def __str__(self):
+        return "Cholesky"
+
+    def setup(self, matrix_batch: torch.Tensor):
+        self.matrix_cholesky = torch.linalg.cholesky(matrix_batch)
+
+    def solve(self, rhs_batch: torch.Tensor) -> torch.Tensor:
+        return torch.cholesky_solve(rhs_batch.reshape(self.matrix_cholesky.shape[0], -1, 1),
+                                    self.matrix_cholesky)
+
+
+# =======================================================================================
+# Schwarz preconditioners
+# =======================================================================================
+
+
+@dataclass
+class _LocalStore:
+    """Local inverses packed for dds_local_apply (device tensors)."""
+
+    mats: torch.Tensor      # flat storage, each A_i^-1 column-major with leading dim ld_i
+    mat_off: torch.Tensor   # (n_s,) int64 element offsets
+    ld: torch.Tensor        # (n_s,) int32
+    scale: torch.Tensor | None  # (n_s,) float32
+    tiles: torch.Tensor     # (n_tiles, 2) int32: (subdomain, first row)
+    n_tiles: int
+    max_k: int
+    mat_dtype: torch.dtype
+    acc_dtype: torch.dtype
+    bytes_matrices: int
+
+
+def _tiles_for(sizes: np.ndarray, rows: int) -> np.ndarray:
+    t = []
+    for s, k in enumerate(sizes):
+        for r0 in range(0, int(k), rows):
+            t.append((s, r0))
+    return np.asarray(t, dtype=np.int32).reshape(-1, 2)
+
+
+class SchwarzOperator(SparseSolver):
+    """Common setup of the Schwarz preconditioners (solvers.py:554-771).
+
+    setup(matrix, dofs_per_solver, solvers_per_coarse, *, decomposition=None, waves=None)
+
+    With `decomposition=None` the unknowns are already numbered so that solver i owns
+    [i*k, (i+1)*k) and coarse cell c owns solvers_per_coarse[c] consecutive solvers (the
+    reference's case). A `Decomposition` (dd_solvers.problems) or a mapping with
+    owner_ptr / sub_ptr / sub_idx / solvers_per_coarse describes uneven, overlapping
+    subdomains; `dofs_per_solver` is then ignored.
+    """
+
+    def __init__(self, preconditioner_precision: torch.dtype, local_solver, coarse_solver,
+                 collect_timings: bool = False):
+        self.preconditioner_precision = preconditioner_precision
+        self.local_solver = local_solver
+        self.coarse_solver = coarse_solver
+        self.collect_timings = collect_timings
+        self.waves = None
+
+    # -- setup ------------------------------------------------------------------------
+    def setup(self, matrix: torch.Tensor, dofs_per_solver: int | None = None,
+              solvers_per_coarse: torch.Tensor | None = None, *args, decomposition=None,
+              waves: int | None = None, **kwargs):
+        if matrix.layout is not torch.sparse_csr:
+            raise AssertionError("SchwarzOperator.setup expects a sparse CSR matrix")
+        _lib.require_device(matrix.values())
+        self.device = matrix.device
+        self.dtype_in = matrix.dtype
+        pi = self.preconditioner_precision or matrix.dtype
+        if pi not in (torch.float64, torch.float32):
+            raise ValueError(f"preconditioner precision must be float64 or float32, not {pi}")
+        self.pi = pi
+        self.waves = waves or self.waves
+        n = matrix.shape[0]
+        self.n = n
+        if self.collect_timings:
+            events = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            events[0].record()
+
+        if decomposition is None:
+            if dofs_per_solver is None or solvers_per_coarse is None:
+                raise ValueError("setup needs dofs_per_solver and solvers_per_coarse "
+                                 "(or decomposition=)")
+            spc = torch.as_tensor(solvers_per_coarse).to("cpu", torch.int64).numpy()
+            k = int(dofs_per_solver)
+            n_s = int(spc.sum())
+            if n_s * k != n:
+                raise ValueError(f"{n_s} solvers x {k} dofs != {n} unknowns")
+            owner_ptr = np.arange(0, n + 1, k, dtype=np.int64)
+            sub_ptr = owner_ptr.copy()
+            sub_idx = None
+        else:
+            get = (lambda name: getattr(decomposition, name)) if not isinstance(decomposition, dict) \
+                else (lambda name: decomposition[name])
+            owner_ptr = np.asarray(get("owner_ptr"), dtype=np.int64)
+            sub_ptr = np.asarray(get("sub_ptr"), dtype=np.int64)
+            sub_idx = np.asarray(get("sub_idx"), dtype=np.int64)
+            try:
+                spc = get("solvers_per_coarse")
+            except (AttributeError, KeyError):
+                spc = None
+            n_s = owner_ptr.shape[0] - 1
+            spc = np.ones(n_s, dtype=np.int64) if spc is None else np.asarray(spc, dtype=np.int64)
+            if owner_ptr[-1] != n or sub_idx.shape[0] != sub_ptr[-1]:
+                raise ValueError("decomposition does not match the matrix size")
+            if sub_idx.shape[0] == n and np.array_equal(sub_ptr, owner_ptr) and \
+                    np.array_equal(sub_idx, np.arange(n)):
+                sub_idx = None  # non-overlapping and contiguous
+        self.n_solvers = n_s
+        self.n_coarse = int(spc.shape[0])
+        self.solvers_per_coarse = torch.as_tensor(spc, device=self.device)
+        scan = np.concatenate([[0], np.cumsum(spc)])
+        self.solvers_per_coarse_scan = torch.as_tensor(scan.astype(np.int32), device=self.device)
+        self.dofs_per_solver = int(dofs_per_solver) if decomposition is None else None
+        coarse_ptr = owner_ptr[scan]
+        sizes = np.diff(sub_ptr)
+        self._host = dict(owner_ptr=owner_ptr, sub_ptr=sub_ptr, sub_idx=sub_idx,
+                          coarse_ptr=coarse_ptr, sizes=sizes)
+        dev = self.device
+        self.sub_ptr = torch.as_tensor(sub_ptr, device=dev)
+        self.sub_idx = None if sub_idx is None else torch.as_tensor(sub_idx.astype(np.int32), device=dev)
+        self.coarse_ptr = torch.as_tensor(coarse_ptr, device=dev)
+        cell = np.repeat(np.arange(self.n_coarse, dtype=np.int32), np.diff(coarse_ptr))
+        self.cell = torch.as_tensor(cell, device=dev)
+        if sub_idx is not None:
+            order = np.argsort(sub_idx, kind="stable")
+            counts = np.bincount(sub_idx, minlength=n)
+            if counts.min() < 1:
+                raise ValueError("every unknown must belong to at least one subdomain")
+            self.dof_ptr = torch.as_tensor(np.concatenate([[0], np.cumsum(counts)]), device=dev)
+            self.dof_pos = torch.as_tensor(order.astype(np.int32), device=dev)
+            self.Y = torch.empty(int(sub_ptr[-1]), dtype=pi, device=dev)
+        else:
+            self.dof_ptr = self.dof_pos = self.Y = None
+
+        if self.collect_timings:
+            events[1].record()
+        # Coarse matrix A_C = R_C A R_C^T (solvers.py:721-771), R0A = R_C A (:890-906).
+        crow, col, val = matrix.crow_indices(), matrix.col_indices(), matrix.values()
+        rows = csr_rows(crow)
+        col64 = col.to(torch.int64)
+        cell_t = self.cell.to(torch.int64)
+        AC = self._segment_sum_dense(cell_t[rows] * self.n_coarse + cell_t[col64], val,
+                                     self.n_coarse * self.n_coarse)
+        AC = AC.view(self.n_coarse, self.n_coarse)
+        collect()
+        self.coarse_solver.setup(AC.to(pi).to_sparse_csr())
+        if self.collect_timings:
+            events[2].record()
+        self._setup_local(matrix, rows, col64, val, sizes, sub_idx)
+        if self.collect_timings:
+            events[3].record()
+        self._setup_extra(matrix, rows, col64, val)
+        self._cast_buf = None
+        if self.collect_timings:
+            torch.cuda.synchronize()
+            x_c = torch.rand(self.n_coarse, device=dev, dtype=pi)
+            coarse_time = timeit(lambda: self.coarse_solver.solve(x_c))
+            x = torch.rand(n, device=dev, dtype=pi)
+            local_time = timeit(lambda: self._local_only(x))
+            return {
+                "preprocessing time": events[0].elapsed_time(events[1]),
+                "coarse solver setup time": events[1].elapsed_time(events[2]),
+                "local solver setup time": events[2].elapsed_time(events[3]),
+                "coarse solver solve time": coarse_time,
+                "local solver solve time": local_time,
+            }
+        return None
+
+    def _setup_extra(self, matrix, rows, col64, val):
+        pass
+
+    @staticmethod
+    def _segment_sum_dense(key: torch.Tensor, val: torch.Tensor, size: int) -> torch.Tensor:
+        """out[key] = sum of val over equal keys, deterministic (sort + segment_reduce)."""
+        order = torch.sort(key, stable=True).indices
+        ks = key[order]
+        uk, counts = torch.unique_consecutive(ks, return_counts=True)
+        sums = torch.segment_reduce(val[order].to(torch.float64), "sum", lengths=counts)
+        out = torch.zeros(size, dtype=torch.float64, device=val.device)
+        out[uk] = sums
+        return out
+
+    def _setup_local(self, matrix, rows, col64, val, sizes, sub_idx):
+        """Extract A_i = R_i A R_i^T, invert in the preconditioner precision and pack the
+        inverses for dds_local_apply (solvers.py:645-658, :343-353)."""
+        dev = self.device
+        pi = self.pi
+        n = self.n
+        n_s = self.n_solvers
+        k_max = int(sizes.max())
+        ls = self.local_solver
+        if not isinstance(ls, DenseBatchSolver):
+            raise ValueError(f"Unknown local solver type: {type(ls)}")
+        sub_ptr = self.sub_ptr
+        if sub_idx is None:
+            # contiguous ranges: subdomain of an unknown by search in sub_ptr
+            sub_r = torch.searchsorted(sub_ptr, rows, right=True) - 1
+            sub_c = torch.searchsorted(sub_ptr, col64, right=True) - 1
+            keep = sub_r == sub_c
+            s = sub_r[keep]
+            a = rows[keep] - sub_ptr[s]
+            b = col64[keep] - sub_ptr[s]
+            v = val[keep]
+        else:
+            # every membership (s, a) of the row, then the column looked up in S_s
+            dof_ptr = self.dof_ptr
+            mult = dof_ptr[1:] - dof_ptr[:-1]
+            cnt = mult[rows]
+            e = torch.repeat_interleave(torch.arange(rows.shape[0], device=dev), cnt)
+            first = torch.cumsum(cnt, 0) - cnt
+            t = torch.arange(e.shape[0], device=dev) - first[e]
+            pos = self.dof_pos.to(torch.int64)[dof_ptr[rows[e]] + t]
+            s = torch.searchsorted(sub_ptr, pos, right=True) - 1
+            a = pos - sub_ptr[s]
+            keys = torch.repeat_interleave(torch.arange(n_s, device=dev),
+                                           sub_ptr[1:] - sub_ptr[:-1]) * n + self.sub_idx.to(torch.int64)
+            qk = s * n + col64[e]
+            q = torch.searchsorted(keys, qk).clamp_(max=keys.shape[0] - 1)
+            found = keys[q] == qk
+            s, a, b, v = s[found], a[found], (q - sub_ptr[s])[found], val[e][found]
+            del e, first, t, pos, keys, qk, q, found, cnt, mult
+        if not isinstance(ls, Inv):
+            # LU / Cholesky / user solvers: their own batched setup and solve on the
+            # reference's layout (equal, contiguous, non-overlapping subdomains).
+            if sub_idx is not None or not np.all(sizes == sizes[0]):
+                raise ValueError(f"{ls} needs equal, non-overlapping subdomains; use Inv()")
+            k = int(sizes[0])
+            batch = torch.zeros(n_s * k * k, dtype=pi, device=dev)
+            batch[(s * k + a) * k + b] = v.to(pi)
+            ls.setup(batch.view(n_s, k, k))
+            self.local = None
+            self._generic_local = True
+            collect()
+            return
+        self._generic_local = False
+        store_dtype = ls.precision or pi
+        acc = getattr(ls, "accumulate", None) or pi
+        if store_dtype == torch.float64 and acc != torch.float64:
+            raise ValueError("float64 local inverses need a float64 accumulator")
+        width = _lib._lib.dds_local_vec_width(_lib.dtype_code(store_dtype))
+        rows_per_tile = _lib._lib.dds_local_tile_rows(_lib.dtype_code(store_dtype))
+        ld = ((sizes + width - 1) // width) * width
+        mat_off = np.concatenate([[0], np.cumsum(ld * sizes)]).astype(np.int64)
+        total = int(mat_off[-1])
+        mats = torch.zeros(total, dtype=store_dtype, device=dev)
+        scaled = bool(getattr(ls, "scaled", False))
+        scale = torch.ones(n_s, dtype=torch.float32, device=dev) if scaled else None
+        # Invert in chunks of subdomains (bounded temporary memory).
+        chunk = max(1, int(2 ** 31 // max(k_max * k_max * 8, 1)))
+        flat_key = (s * k_max + a) * k_max + b
+        order = torch.sort(flat_key).indices
+        flat_key, v = flat_key[order], v[order]
+        for c0 in range(0, n_s, chunk):
+            c1 = min(n_s, c0 + chunk)
+            lo = int(torch.searchsorted(flat_key, torch.tensor(c0 * k_max * k_max, device=dev)))
+            hi = int(torch.searchsorted(flat_key, torch.tensor(c1 * k_max * k_max, device=dev)))
+            blk = torch.zeros((c1 - c0) * k_max * k_max, dtype=pi, device=dev)
+            blk[flat_key[lo:hi] - c0 * k_max * k_max] = v[lo:hi].to(pi)
+            blk = blk.view(c1 - c0, k_max, k_max)
+            kk = torch.as_tensor(sizes[c0:c1], device=dev)
+            ar = torch.arange(k_max, device=dev)
+            pad = ar[None, :] >= kk[:, None]
+            blk.diagonal(dim1=1, dim2=2)[pad] = 1.0  # identity padding
+            inv = torch.linalg.inv(blk)
+            for j in range(c1 - c0):
+                i = c0 + j
+                k = int(sizes[i])
+                m = inv[j, :k, :k]
+                if scaled:
+                    sc = m.abs().max().to(torch.float32)
+                    scale[i] = sc
+                    m = m / sc.to(m.dtype)
+                # column-major with padded leading dimension: element (a, b) at b*ld + a
+                col_major = torch.zeros((k, int(ld[i])), dtype=pi, device=dev)
+                col_major[:, :k] = m.transpose(0, 1)
+                mats[int(mat_off[i]):int(mat_off[i + 1])] = col_major.reshape(-1).to(store_dtype)
+            del blk, inv
+        tiles = _tiles_for(sizes, rows_per_tile)
+        self.local = _LocalStore(
+            mats=mats, mat_off=torch.as_tensor(mat_off[:-1], device=dev),
+            ld=torch.as_tensor(ld.astype(np.int32), device=dev), scale=scale,
+            tiles=torch.as_tensor(tiles, device=dev), n_tiles=int(tiles.shape[0]),
+            max_k=k_max, mat_dtype=store_dtype, acc_dtype=acc,
+            bytes_matrices=int(total * mats.element_size()))
+        collect()
+
+    # -- apply pieces ---------------------------------------------------------------------
+    @property
+    def _device_apply(self) -> bool:
+        """True when every step of the apply is a gated HIP kernel (CG's device loop)."""
+        return (not getattr(self, "_generic_local", True)
+                and hasattr(self.coarse_solver, "_enqueue_solve"))
+
+    def _enqueue_local(self, x, Y, gate) -> None:
+        if self._generic_local:
+            k = self.n // self.n_solvers
+            Y[:self.n].copy_(self.local_solver.solve(x[:self.n].reshape(self.n_solvers, k)).reshape(-1))
+            return
+        L = self.local
+        _lib.call("dds_local_apply", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
+                  _lib.dtype_code(L.acc_dtype), L.n_tiles, L.tiles.data_ptr(),
+                  self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), L.max_k,
+                  L.mat_off.data_ptr(), L.ld.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(),
+                  x.data_ptr(), Y.data_ptr(), self.waves or 4, _lib.ptr(gate), _lib.stream())
+
+    def _enqueue_overlap_sum(self, Y, z, yc, sign, r, partials, gate) -> int:
+        overl = self.dof_ptr is not None and Y is self.Y
+        return _lib.call_partials(
+            "dds_overlap_sum", _lib.dtype_code(self.pi), self.n,
+            _lib.ptr(self.dof_ptr) if overl else None, _lib.ptr(self.dof_pos) if overl else None,
+            Y.data_ptr(), _lib.ptr(self.cell) if yc is not None else None, _lib.ptr(yc),
+            float(sign), z.data_ptr(), _lib.ptr(r), _lib.ptr(partials), _lib.PARTIALS_OUT,
+            _lib.ptr(gate), _lib.stream())
+
+    def _local_only(self, x):
+        y = torch.empty_like(x)
+        if self.Y is None:
+            self._enqueue_local(x, y, None)
+        else:
+            self._enqueue_local(x, self.Y, None)
+            self._enqueue_overlap_sum(self.Y, y, None, 0.0, None, None, None)
+        return y
+
+    def _coarse_enqueue(self, xc, yc, gate) -> None:
+        cs = self.coarse_solver
+        if hasattr(cs, "_enqueue_solve"):
+            cs._enqueue_solve(xc, yc, gate)
+        else:
+            out, _ = cs.solve(xc)
+            yc.copy_(out)
+
+    def _buffers(self):
+        if getattr(self, "_xc", None) is None or self._xc.dtype != self.pi:
+            self._xc = torch.empty(self.n_coarse, dtype=self.pi, device=self.device)
+            self._yc = torch.empty(self.n_coarse, dtype=self.pi, device=self.device)
+            self._ybuf = torch.empty(self.n, dtype=self.pi, device=self.device)
+        return self._xc, self._yc, self._ybuf
+
+    def _enqueue_apply(self, r, z, partials, gate) -> int:
+        """z = M r on the current stream (every kernel gated); per-block partials of r . z
+        are written to `partials`; returns their count. r, z are float64."""
+        if self.pi == torch.float64:
+            return self._enqueue_pi(r, z, r, partials, gate)
+        if self._cast_buf is None:
+            self._cast_buf = (torch.empty(self.n, dtype=self.pi, device=self.device),
+                              torch.empty(self.n, dtype=self.pi, device=self.device))
+        r32, z32 = self._cast_buf
+        s = _lib.stream()
+        _lib.call("dds_cast", _lib.F64, _lib.dtype_code(self.pi), self.n, r.data_ptr(),
+                  r32.data_ptr(), _lib.ptr(gate), s)
+        self._enqueue_pi(r32, z32, None, None, gate)
+        _lib.call("dds_cast", _lib.dtype_code(self.pi), _lib.F64, self.n, z32.data_ptr(),
+                  z.data_ptr(), _lib.ptr(gate), s)
+        return _lib.call_partials("dds_dot_partials", self.n, r.data_ptr(), z.data_ptr(),
+                                  partials.data_ptr(), _lib.PARTIALS_OUT, _lib.ptr(gate), s)
+
+    def solve(self, rhs: torch.Tensor) -> Tuple[torch.Tensor, Metadata]:
+        _lib.require_device(rhs)
+        with torch.cuda.device(rhs.device):
+            x = rhs.to(self.pi).contiguous()
+            y = torch.empty_like(x)
+            self._enqueue_pi(x, y, None, None, None)
+        return y.to(rhs.dtype), {}
+
+    def destroy(self) -> None:
+        self.coarse_solver.destroy()
+        self.local = None
+
+    @property
+    def bytes_per_apply(self) -> int:
+        """Algorithmic bytes of one application (matrices, vectors, coarse)."""
+        e = 8 if self.pi == torch.float64 else 4
+        sizes = self._host["sizes"]
+        nk = int(sizes.sum())
+        b = self.local.bytes_matrices + e * 2 * nk
+        if self.sub_idx is not None:
+            b += 4 * nk + e * 2 * nk + 4 * nk + 8 * (self.n + 1)  # gather idx, Y, dof_pos/ptr
+        b += e * 2 * self.n  # r read by the restriction/sum, z written
+        b += getattr(self.coarse_solver, "bytes_per_solve", 0)
+        return b
+
+
+class AdditiveSchwarz(SchwarzOperator):
+    """M r = sum_i R_i^T A_i^-1 R_i r + R_C^T A_C^-1 R_C r (solvers.py:774-826)."""
+
+    def __init__(self, preconditioner_precision: torch.dtype, local_solver, coarse_solver,
+                 number_of_dofs_per_coarse_is_const: bool = False, collect_timings: bool = False):
+        super().__init__(preconditioner_precision, local_solver, coarse_solver, collect_timings)
+        self.number_of_dofs_per_coarse_is_const = number_of_dofs_per_coarse_is_const
+
+    def __str__(self):
+        return (f"AdditiveSchwarz({self.preconditioner_precision}, {self.local_solver}, "
+                f"{self.coarse_solver}, const_dofs_per_coarse={self.number_of_dofs_per_coarse_is_const})")
+
+    def setup(self, matrix, dofs_per_solver=None, solvers_per_coarse=None, *args, **kwargs):
+        timings = super().setup(matrix, dofs_per_solver, solvers_per_coarse, *args, **kwargs)
+        if self.number_of_dofs_per_coarse_is_const and self.dofs_per_solver is not None:
+            self.dofs_per_coarse = self.dofs_per_solver * int(self.solvers_per_coarse[0].item())
+        return timings
+
+    def _enqueue_pi(self, x, z, r, partials, gate) -> int:
+        xc, yc, _ = self._buffers()
+        s = _lib.stream()
+        Y = z if self.Y is None else self.Y
+        self._enqueue_local(x, Y, gate)
+        _lib.call("dds_restrict", _lib.dtype_code(self.pi), self.n_coarse,
+                  self.coarse_ptr.data_ptr(), x.data_ptr(), xc.data_ptr(), _lib.ptr(gate), s)
+        self._coarse_enqueue(xc, yc, gate)
+        return self._enqueue_overlap_sum(Y, z, yc, 1.0, r, partials, gate)
+
+
+class HybridSchwarz(SchwarzOperator):
+    """M r = y - R_C^T A_C^-1 (R0A y), y = sum_i R_i^T A_i^-1 R_i r; T0 b = R_C^T A_C^-1 R_C b
+    (solvers.py:829-906)."""
+
+    def __init__(self, preconditioner_precision: torch.dtype, local_solver, coarse_solver,
+                 collect_timings: bool = False):
+        super().__init__(preconditioner_precision, local_solver, coarse_solver, collect_timings)
+
+    def __str__(self):
+        return f"HybridSchwarz({self.preconditioner_precision}, {self.local_solver}, {self.coarse_solver})"
+
+    def _setup_extra(self, matrix, rows, col64, val):
+        # R0A = R_C A: rows of each coarse cell merged, duplicate columns summed (:890-906)
+        cell_t = self.cell.to(torch.int64)
+        key = cell_t[rows] * self.n + col64
+        order = torch.sort(key, stable=True).indices
+        ks = key[order]
+        uk, counts = torch.unique_consecutive(ks, return_counts=True)
+        sums = torch.segment_reduce(val[order].to(torch.float64), "sum", lengths=counts)
+        r_of = uk // self.n
+        crow = torch.zeros(self.n_coarse + 1, dtype=torch.int64, device=self.device)
+        crow[1:] = torch.cumsum(torch.bincount(r_of, minlength=self.n_coarse), 0)
+        R0A = torch.sparse_csr_tensor(crow, (uk % self.n).to(torch.int32), sums.to(self.pi),
+                                      size=(self.n_coarse, self.n))
+        self.R0A = R0A
+        self._r0a = _CSROperator(R0A)
+
+    def T0(self, rhs: torch.Tensor) -> torch.Tensor:
+        with torch.cuda.device(rhs.device):
+            x = rhs.to(self.pi).contiguous()
+            xc, yc, _ = self._buffers()
+            s = _lib.stream()
+            _lib.call("dds_restrict", _lib.dtype_code(self.pi), self.n_coarse,
+                      self.coarse_ptr.data_ptr(), x.data_ptr(), xc.data_ptr(), None, s)
+            self._coarse_enqueue(xc, yc, None)
+            zero = torch.zeros_like(x)
+            y = torch.empty_like(x)
+            self._enqueue_overlap_sum(zero, y, yc, 1.0, None, None, None)
+        return y.to(rhs.dtype)
+
+    def _enqueue_pi(self, x, z, r, partials, gate) -> int:
+        xc, yc, ybuf = self._buffers()
+        if self.Y is None:
+            self._enqueue_local(x, ybuf, gate)
+        else:
+            self._enqueue_local(x, self.Y, gate)
+            self._enqueue_overlap_sum(self.Y, ybuf, None, 0.0, None, None, gate)
+        self._r0a._enqueue_matvec(ybuf, xc, None, gate)
+        self._coarse_enqueue(xc, yc, gate)
+        return self._enqueue_overlap_sum(ybuf, z, yc, -1.0, r, partials, gate)
+
+    @property
+    def bytes_per_apply(self) -> int:
+        return super().bytes_per_apply + self._r0a.bytes_per_matvec

@@ -1,0 +1,197 @@
+/*
+ * dd_solvers_hip.h — C ABI of the MI355X (gfx950) Schwarz-PCG kernels.
+ *
+ * Every entry point takes plain device pointers, 64-bit sizes, dtype codes and an explicit
+ * hipStream_t (passed as void*), enqueues its kernels on that stream, and returns a
+ * dds_status (0 on success). Arguments are validated on the host before anything is
+ * launched; the message of the last failure on the calling thread is returned by
+ * dds_last_error(). No entry point allocates device memory: callers (the Python package
+ * uses the PyTorch caching allocator) pass every buffer in, which keeps the hot path legal
+ * under hipGraph stream capture.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository root):
+ *   dds_gemv_strided_batched  dd_solvers_gemv::gemv_strided_batched
+ *                             (src/dd_solvers/cpp/gemv.cpp:9-57, gemv_kernels.cu:31-153)
+ *   dds_bsrmv                 FastBSR RPT-I numba kernel (src/dd_solvers/dd_solvers/bsr.py:40-76)
+ *   dds_csrmv                 torch sparse-CSR @ dense in CG(bsr_matmul=False) and
+ *                             HybridSchwarz (src/dd_solvers/dd_solvers/solvers.py:226,235,264,880)
+ *   dds_local_apply           Inv.solve -> gemv_strided_batched (solvers.py:355-359),
+ *                             generalised to uneven and overlapping subdomains
+ *   dds_overlap_sum           y += repeat_interleave(...) prolongation (solvers.py:821-825,
+ *                             :882-887) fused with the overlapped sum_i R_i^T y_i
+ *   dds_restrict              x.reshape(n_s,-1).sum(1) + segment_reduce (solvers.py:817-820)
+ *   dds_chol_factor           cudssExecute(ANALYSIS|FACTORIZATION) of the coarse matrix
+ *                             (src/dd_solvers/cpp/cudss.cpp:80-105), as a dense FP64 Cholesky
+ *   dds_chol_solve            cudssExecute(SOLVE) (cudss.cpp:64-78)
+ *   dds_chol_inverse, dds_dense_symv
+ *                             coarse solve for large coarse spaces (explicit A_C^-1 from the
+ *                             Cholesky factor, applied as one dense symmetric matvec)
+ *   dds_pcg_*                 the CG.solve loop body (solvers.py:228-297): torch.linalg.norm,
+ *                             torch.dot, fused_add / fused_sub / fused_mul_add (:170-189)
+ *   dds_cast                  rhs.to(preconditioner_precision) / y.to(rhs.dtype)
+ *                             (solvers.py:813, :826, :876, :888)
+ */
+#ifndef DD_SOLVERS_HIP_H
+#define DD_SOLVERS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDS_ABI_VERSION 1
+/* Upper bound on the per-block partial sums written by one reduction-producing launch. */
+#define DDS_MAX_PARTIALS 2048
+
+typedef enum dds_dtype {
+  DDS_F64 = 0,
+  DDS_F32 = 1,
+  DDS_F16 = 2,
+  DDS_BF16 = 3,
+  DDS_I32 = 4,
+  DDS_I64 = 5
+} dds_dtype;
+
+typedef enum dds_status {
+  DDS_OK = 0,
+  DDS_EINVAL = 1,      /* bad argument (shape, dtype, pointer, size) */
+  DDS_EHIP = 2,        /* HIP runtime error */
+  DDS_EUNSUPPORTED = 3 /* valid but not implemented combination */
+} dds_status;
+
+/*
+ * Device-resident PCG scalars (one per solve). The loop in CG.solve keeps alpha, beta and
+ * rho on the device and reads ||r|| back every iteration (solvers.py:244); here the norm
+ * test also runs on the device, so the host reads `done` only when it wants to.
+ *   it      index of the next norm check (= number of completed updates)
+ *   done    0 running, 1 converged (||r|| < atol at check `it`), 2 maxiter updates done
+ * Gated kernels (those taking `const dds_pcg_state* gate`) return immediately when
+ * gate->done != 0, so enqueuing iterations past convergence changes nothing.
+ */
+typedef struct dds_pcg_state {
+  double atol;     /* max(atol, rtol * ||b||)                       solvers.py:229 */
+  double rho;      /* r_i . z_i                                      :256 */
+  double rho_prev; /* r_{i-1} . z_{i-1}                              :268 */
+  double pq;       /* p_i . q_i                                      :265 */
+  double alpha;    /* rho / pq                                       :265 */
+  double beta;     /* rho / rho_prev                                 :259 */
+  double rr;       /* ||r||^2 at the last check */
+  int32_t it;
+  int32_t done;
+  int32_t maxiter;
+  int32_t pad_;
+} dds_pcg_state;
+
+/* finalize ops (dds_pcg_finalize) */
+#define DDS_FIN_CHECK 0 /* rr = sum; norms[it] = sqrt(rr); done = 1 if sqrt(rr) < atol */
+#define DDS_FIN_RHO 1   /* rho = sum; beta = rho / rho_prev (it > 0) */
+#define DDS_FIN_ALPHA 2 /* pq = sum; alpha = rho / pq; alphas[it] = alpha; betas[it] = beta */
+#define DDS_FIN_NEXT 3  /* rho_prev = rho; it += 1; done = 2 if it == maxiter, else FIN_CHECK */
+
+int dds_abi_version(void);
+const char* dds_last_error(void);
+
+/* ---- batched dense GEMV (drop-in op) ---------------------------------------------------
+ * out[b, r] = sum_c mat[b, r, c] * vec[b, c] for b < n, with mat column-major
+ * (element (b, r, c) at b*k*k + c*k + r), accumulated in the vector dtype.
+ * mat_dtype in {F64, F32, F16, BF16}; vec_dtype in {F64, F32}; F64 matrices need F64
+ * vectors (gemv.cpp:34-38). use_lds stages vec[b, :] in LDS (the reference's shared-memory
+ * variant, gemv_kernels.cu:59-93). */
+int dds_gemv_strided_batched(int mat_dtype, int vec_dtype, const void* mat, const void* vec,
+                             void* out, int64_t n, int64_t k, int use_lds, void* stream);
+
+/* ---- SpMV --------------------------------------------------------------------------------
+ * BSR with b x b blocks stored column-major inside each block (element (r, c) of block j
+ * at values[j*b*b + c*b + r], the cusparse csr2gebsr layout, cusparse.cpp:30, 81-83).
+ * val_dtype in {F64, F32}; idx_dtype in {I32, I64} is the dtype of crow; col is int32.
+ * If dot_partials != NULL the launch also writes per-block partial sums of x . y and
+ * stores their count in *n_partials (host int). gate may be NULL. */
+int dds_bsrmv(int val_dtype, int idx_dtype, int64_t n_block_rows, int64_t block_size,
+              const void* crow, const int32_t* col, const void* values, const void* x, void* y,
+              double* dot_partials, int* n_partials, const dds_pcg_state* gate, void* stream);
+
+/* CSR y = A x; `lanes` lanes cooperate on one row (a power of two in [1, 64]; the Python
+ * layer picks the power of two nearest above nnz / n_rows). Partials and gate as for
+ * dds_bsrmv. */
+int dds_csrmv(int val_dtype, int idx_dtype, int64_t n_rows, const void* row_ptr,
+              const int32_t* col, const void* values, const void* x, void* y, int lanes,
+              double* dot_partials, int* n_partials, const dds_pcg_state* gate, void* stream);
+
+/* ---- local inverses --------------------------------------------------------------------
+ * Y[sub_ptr[s] + a] = scale[s] * sum_b Ainv_s[a, b] * x[g_s(b)] for every subdomain s,
+ * where g_s(b) = sub_idx[sub_ptr[s] + b] (or sub_ptr[s] + b when sub_idx == NULL) and
+ * Ainv_s is stored column-major with leading dimension ld[s] at mats + mat_off[s]
+ * (elements). ld[s] must be a multiple of dds_local_vec_width(mat_dtype) and >= k_s, the
+ * padding rows must be finite, and mat_off[s] a multiple of the same width.
+ * `tiles` lists (s, row0) pairs, row0 a multiple of dds_local_tile_rows(mat_dtype); one
+ * workgroup per tile. vec_dtype (x and Y) in {F64, F32}; acc_dtype in {F64, F32};
+ * scale may be NULL. max_k bounds every k_s (sizes the LDS stage of x). */
+int dds_local_vec_width(int mat_dtype);
+int dds_local_tile_rows(int mat_dtype);
+int dds_local_apply(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_tiles,
+                    const int32_t* tiles, const int64_t* sub_ptr, const int32_t* sub_idx,
+                    int64_t max_k, const int64_t* mat_off, const int32_t* ld,
+                    const float* scale, const void* mats, const void* x, void* Y,
+                    int waves, const dds_pcg_state* gate, void* stream);
+
+/* z[j] = sum_{p in [dof_ptr[j], dof_ptr[j+1])} Y[dof_pos[p]]   (dof_ptr == NULL: z[j] = Y[j])
+ *        + coarse_sign * yc[cell[j]]                           (yc != NULL)
+ * in dtype (F64 or F32). If r != NULL (F64 only) per-block partials of r . z are written.
+ * z may alias Y when dof_ptr == NULL. */
+int dds_overlap_sum(int dtype, int64_t n, const int64_t* dof_ptr, const int32_t* dof_pos,
+                    const void* Y, const int32_t* cell, const void* yc, double coarse_sign,
+                    void* z, const double* r, double* dot_partials, int* n_partials,
+                    const dds_pcg_state* gate, void* stream);
+
+/* xc[c] = sum_{j in [coarse_ptr[c], coarse_ptr[c+1])} x[j], one workgroup per cell. */
+int dds_restrict(int dtype, int64_t n_c, const int64_t* coarse_ptr, const void* x, void* xc,
+                 const dds_pcg_state* gate, void* stream);
+
+/* ---- dense coarse solver ---------------------------------------------------------------
+ * In-place Cholesky A = L L^T of an n x n row-major SPD matrix (lower triangle read and
+ * overwritten by L; the strict upper triangle is left as is). *info (device int) is 0 on
+ * success, else 1 + the index of the first non-positive pivot. F64 and F32. */
+int dds_chol_factor(int dtype, int64_t n, void* a, int64_t lda, int* info, void* stream);
+/* y = (L L^T)^{-1} x by forward then backward substitution in one workgroup. */
+int dds_chol_solve(int dtype, int64_t n, const void* L, int64_t ldl, const void* x, void* y,
+                   const dds_pcg_state* gate, void* stream);
+/* Ainv = (L L^T)^{-1} as a full symmetric n x n row-major matrix. */
+int dds_chol_inverse(int dtype, int64_t n, const void* L, int64_t ldl, void* ainv,
+                     int64_t ldi, void* stream);
+/* y = S x for a dense symmetric row-major S (one wavefront per row). */
+int dds_dense_symv(int dtype, int64_t n, const void* s, int64_t lds, const void* x, void* y,
+                   const dds_pcg_state* gate, void* stream);
+
+/* ---- PCG vector kernels (F64) ----------------------------------------------------------
+ * Each writes per-block partial sums where stated (count returned in *n_partials). */
+/* r = b - q; partials of r . r */
+int dds_pcg_residual(int64_t n, const double* b, const double* q, double* r,
+                     double* partials, int* n_partials, const dds_pcg_state* gate,
+                     void* stream);
+/* p = z (st->it == 0) else p = st->beta * p + z */
+int dds_pcg_update_p(int64_t n, const double* z, double* p, const dds_pcg_state* st,
+                     void* stream);
+/* x += alpha p; r -= alpha q; partials of r . r */
+int dds_pcg_update_xr(int64_t n, const double* p, const double* q, double* x, double* r,
+                      double* partials, int* n_partials, const dds_pcg_state* st,
+                      void* stream);
+/* partials of a . b */
+int dds_dot_partials(int64_t n, const double* a, const double* b, double* partials,
+                     int* n_partials, const dds_pcg_state* gate, void* stream);
+/* One-workgroup reduction of `count` partials (fixed order) and the scalar update `op`
+ * (DDS_FIN_*). norms/alphas/betas hold maxiter + 1 doubles. */
+int dds_pcg_finalize(int op, const double* partials, int count, dds_pcg_state* st,
+                     double* norms, double* alphas, double* betas, void* stream);
+/* Plain sum of `count` partials into *out (device double), for dot products outside PCG. */
+int dds_sum_partials(const double* partials, int count, double* out, void* stream);
+
+/* dst = (dst_dtype) src, F64 <-> F32 <-> F64 */
+int dds_cast(int src_dtype, int dst_dtype, int64_t n, const void* src, void* dst,
+             const dds_pcg_state* gate, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DD_SOLVERS_HIP_H */

@@ -1,0 +1,187 @@
+"""GPU parity of the individual HIP kernels against the CPU oracle (through the C ABI)."""
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sps
+import torch
+
+from helpers import bsr8
+from oracle import kernels as K
+from oracle.schwarz import SchwarzOracle
+
+import dd_solvers as dds
+from dd_solvers import problems as P
+from dd_solvers.bsr import FastBSR
+from dd_solvers.csr import csr_to_bsr
+from dd_solvers.gemv import gemv_strided_batched
+
+pytestmark = pytest.mark.gpu
+dev = torch.device("cuda")
+
+DTYPE_PAIRS = [
+    (torch.float16, torch.float32), (torch.float16, torch.float64),
+    (torch.bfloat16, torch.float32), (torch.bfloat16, torch.float64),
+    (torch.float32, torch.float32), (torch.float32, torch.float64),
+    (torch.float64, torch.float64),
+]
+
+
+@pytest.mark.parametrize("mat_dtype,vec_dtype", DTYPE_PAIRS)
+@pytest.mark.parametrize("k", [1, 2, 5, 10, 20, 50, 100, 200])
+@pytest.mark.parametrize("use_shared_memory", [True, False])
+def test_gemv_strided_batched(mat_dtype, vec_dtype, k, use_shared_memory):
+    # the 112 cases of src/dd_solvers/tests/test_gemv.py, checked against the oracle
+    n = 7
+    g = torch.Generator().manual_seed(42)
+    A = torch.randn((n, k, k), generator=g, dtype=torch.float32).to(mat_dtype)
+    A = A.permute(0, 2, 1).contiguous().permute(0, 2, 1).to(dev)  # column-major
+    b = torch.randn((n, k), generator=g, dtype=torch.float32).to(vec_dtype).to(dev)
+    out = gemv_strided_batched(A, b, use_shared_memory=use_shared_memory)
+    ref = K.gemv_strided_batched(A.to(torch.float64).cpu().numpy(), b.cpu().numpy())
+    assert out.dtype == vec_dtype
+    tol = 1e-12 if vec_dtype == torch.float64 else 1e-5
+    assert np.allclose(out.cpu().numpy(), ref, rtol=tol, atol=tol)
+
+
+def test_gemv_rejects_bad_inputs():
+    A = torch.randn(3, 4, 4, dtype=torch.float64, device=dev)  # row-major
+    with pytest.raises(RuntimeError, match="column-major"):
+        gemv_strided_batched(A, torch.randn(3, 4, dtype=torch.float64, device=dev))
+    Ac = A.transpose(1, 2).contiguous().transpose(1, 2)
+    with pytest.raises(RuntimeError):
+        gemv_strided_batched(Ac, torch.randn(3, 4, dtype=torch.float32, device=dev))
+    with pytest.raises(RuntimeError, match="compatible"):
+        gemv_strided_batched(Ac, torch.randn(3, 5, dtype=torch.float64, device=dev))
+
+
+def _bsr_tensor(crow, col, vals, dtype):
+    b = vals.shape[1]
+    nbr = crow.shape[0] - 1
+    return torch.sparse_bsr_tensor(torch.as_tensor(crow), torch.as_tensor(col),
+                                   torch.as_tensor(vals, dtype=dtype), size=(nbr * b, nbr * b)).to(dev)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_bsr_fixture(dtype):
+    crow, col, vals = bsr8()  # test_bsr.py:9-138 (b = 3: the row-per-lane kernel)
+    M = _bsr_tensor(crow, col, vals, dtype)
+    x = torch.randn(M.shape[0], dtype=dtype, device=dev)
+    y = FastBSR(M) @ x
+    ref = K.bsrmv(crow, col, vals.astype(np.float64 if dtype == torch.float64 else np.float32),
+                  x.cpu().numpy().astype(np.float64))
+    assert np.allclose(y.cpu().numpy(), ref, atol=1e-12 if dtype == torch.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("b", [1, 2, 3, 4, 8])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_bsr_random(b, dtype):
+    # test_bsr.py:161-193: 1000 block rows, 3 random blocks per row (duplicates allowed)
+    rng = np.random.default_rng(b)
+    nbr = 1000
+    crow = np.arange(0, 3 * nbr + 1, 3)
+    col = rng.integers(0, nbr, size=3 * nbr)
+    vals = rng.standard_normal((3 * nbr, b, b))
+    M = _bsr_tensor(crow, col, vals, dtype)
+    x = torch.as_tensor(rng.standard_normal(nbr * b), dtype=dtype, device=dev)
+    y = FastBSR(M) @ x
+    ref = K.bsrmv(crow, col, vals.astype(np.float32 if dtype == torch.float32 else np.float64),
+                  x.cpu().numpy().astype(np.float64))
+    assert np.allclose(y.cpu().numpy(), ref, atol=1e-12 if dtype == torch.float64 else 1e-4)
+
+
+@pytest.mark.parametrize("b", [2, 4, 8])
+def test_csr_to_bsr_then_spmv_q1(b):
+    A = P.assemble_q1_3d(16, P.lognormal_coefficients(16 ** 3, 1).reshape(16, 16, 16))
+    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr), torch.as_tensor(A.indices),
+                                 torch.as_tensor(A.data), A.shape).to(dev)
+    B = csr_to_bsr(At, b)
+    assert B.values().stride() == (b * b, 1, b)
+    x = np.random.default_rng(0).standard_normal(A.shape[0])
+    xp = np.zeros(B.shape[0])
+    xp[:A.shape[0]] = x
+    y = FastBSR(B) @ torch.as_tensor(xp, device=dev)
+    assert np.allclose(y.cpu().numpy()[:A.shape[0]], A @ x, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("kind", ["q1_3d", "p1_2d"])
+def test_csrmv(kind):
+    pb = P.make_problem(kind, 24 if kind == "q1_3d" else 64, box=4, overlap=0, lognormal=True)
+    A = pb.A
+    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr), torch.as_tensor(A.indices),
+                                 torch.as_tensor(A.data), A.shape).to(dev)
+    op = dds.solvers._CSROperator(At)
+    x = np.random.default_rng(1).standard_normal(A.shape[0])
+    y = op @ torch.as_tensor(x, device=dev)
+    assert np.allclose(y.cpu().numpy(), A @ x, rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------------------
+# coarse solver
+# ---------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("value_type", [torch.float32, torch.float64])
+@pytest.mark.parametrize("index_type", [torch.int32, torch.int64])
+@pytest.mark.parametrize("mode", ["trsv", "inverse"])
+def test_dense_cholesky_known_answers(value_type, index_type, mode):
+    # src/dd_solvers/tests/test_cudss.py:34-58
+    A = torch.sparse_csr_tensor(torch.tensor([0, 2, 3, 5], dtype=index_type),
+                                torch.tensor([0, 2, 1, 0, 2], dtype=index_type),
+                                torch.tensor([2, 2, 2, 2, 3], dtype=value_type), size=(3, 3)).to(dev)
+    s = dds.DenseCholesky(mode=mode)
+    s.setup(A)
+    for b, x in (([2, -2, 2], [1, -1, 0]), ([4, 4, 5], [1, 2, 1])):
+        xh, _ = s.solve(torch.tensor(b, dtype=value_type, device=dev))
+        assert xh.dtype == value_type
+        assert torch.allclose(xh.cpu(), torch.tensor(x, dtype=value_type), atol=1e-6)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 300, 1000, 2100])
+@pytest.mark.parametrize("mode", ["trsv", "inverse"])
+def test_dense_cholesky_random(n, mode):
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, n))
+    A = B @ B.T / n + np.eye(n)
+    b = rng.standard_normal(n)
+    s = dds.DenseCholesky(mode=mode)
+    s.setup(torch.as_tensor(A, device=dev).to_sparse_csr())
+    L = np.tril(s.L.cpu().numpy())
+    assert np.allclose(L, K.cholesky_lower(A), rtol=1e-11, atol=1e-11)
+    x, _ = s.solve(torch.as_tensor(b, device=dev))
+    xr = sla.cho_solve(sla.cho_factor(A, lower=True), b)
+    assert np.linalg.norm(x.cpu().numpy() - xr) <= 1e-11 * np.linalg.norm(xr)
+
+
+def test_dense_cholesky_not_spd():
+    A = torch.tensor([[1.0, 2.0], [2.0, 1.0]], dtype=torch.float64, device=dev)
+    with pytest.raises(np.linalg.LinAlgError):
+        dds.DenseCholesky().setup(A)
+
+
+# ---------------------------------------------------------------------------------------
+# local inverses (ragged, overlapping)
+# ---------------------------------------------------------------------------------------
+
+def _torch_csr(A):
+    return torch.sparse_csr_tensor(torch.as_tensor(A.indptr), torch.as_tensor(A.indices),
+                                   torch.as_tensor(A.data), A.shape).to(dev)
+
+
+@pytest.mark.parametrize("storage", [torch.float64, torch.float32, torch.float16, torch.bfloat16])
+@pytest.mark.parametrize("overlap", [0, 1])
+@pytest.mark.parametrize("waves", [1, 4, 8])
+def test_local_apply_matches_oracle(storage, overlap, waves):
+    pb = P.make_problem("q1_3d", 14, box=4, overlap=overlap, seed=2, lognormal=True)
+    d = pb.decomposition
+    scaled = storage == torch.float16
+    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(storage, scaled=scaled), dds.DenseCholesky())
+    M.setup(_torch_csr(pb.A), decomposition=d, waves=waves)
+    sname = {torch.float64: "float64", torch.float32: "float32", torch.float16: "float16",
+             torch.bfloat16: "bfloat16"}[storage]
+    O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, storage=sname, scaled=scaled)
+    r = np.random.default_rng(0).standard_normal(pb.A.shape[0])
+    y = M._local_only(torch.as_tensor(r, device=dev)).cpu().numpy()
+    yo = O.local(r)
+    rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    tol = {torch.float64: 1e-12, torch.float32: 1e-6, torch.float16: 2e-3, torch.bfloat16: 1e-2}[storage]
+    assert rel < tol, rel

@@ -1,0 +1,219 @@
+"""CPU-side checks: the C-ABI library loads and exports every declared symbol, host-side
+argument validation, the setup utilities (on CPU tensors), and the problem generators."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import torch
+
+from helpers import dg24
+
+import dd_solvers as dds
+from dd_solvers import _lib
+from dd_solvers import problems as P
+from dd_solvers.csr import asm_permutation, csr_to_bsr, gather_csr, inverse_permutation, \
+    permute_csr_tensor, sort_csr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(dds_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    declared = _declared_symbols()
+    assert "dds_local_apply" in declared and "dds_chol_factor" in declared
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.EXPORTED) >= declared
+
+
+def test_abi_version():
+    assert _lib._lib.dds_abi_version() == _lib.ABI_VERSION
+
+
+def test_host_validation_without_gpu():
+    # Arguments are validated before any launch, so these return errors with no device.
+    with pytest.raises(_lib.HipError, match="waves"):
+        _lib.call("dds_local_apply", _lib.F64, _lib.F64, _lib.F64, 1, 1, 1, None, 8, 1, 1,
+                  None, 1, 1, 1, 3, None, None)
+    with pytest.raises(_lib.HipError, match="Double-Float"):
+        _lib.call("dds_gemv_strided_batched", _lib.F64, _lib.F32, 1, 1, 1, 2, 2, 0, None)
+    with pytest.raises(_lib.HipError, match="lanes"):
+        _lib.call("dds_csrmv", _lib.F64, _lib.I64, 4, 1, 1, 1, 1, 1, 3, None, None, None, None)
+    with pytest.raises(_lib.HipError, match="bad op"):
+        _lib.call("dds_pcg_finalize", 9, 1, 1, 1, 1, 1, 1, None)
+    with pytest.raises(_lib.HipError, match="exceeds"):
+        _lib.call("dds_chol_solve", _lib.F64, 7000, 1, 7000, 1, 1, None, None)
+    with pytest.raises(_lib.HipError, match="unsupported"):
+        _lib.call("dds_cast", _lib.F16, _lib.F64, 4, 1, 1, None, None)
+
+
+def test_cpu_tensors_rejected():
+    A = torch.eye(3, dtype=torch.float64).to_sparse_csr()
+    s = dds.CG(bsr_matmul=False)
+    s.setup(A)
+    with pytest.raises(ValueError, match="HIP device"):
+        s.solve(torch.ones(3, dtype=torch.float64))
+    with pytest.raises(ValueError, match="HIP device"):
+        dds.gemv_strided_batched(torch.zeros(1, 2, 2, dtype=torch.float64).transpose(1, 2),
+                                 torch.zeros(1, 2, dtype=torch.float64))
+
+
+# ---- setup utilities (known answers of src/dd_solvers/tests/test_bsr.py / test_cusparse.py)
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_csr_to_bsr_known_answer(dtype):
+    A = torch.sparse_csr_tensor(torch.tensor([0, 2, 3, 5, 9]), torch.tensor([0, 1, 1, 0, 2, 0, 1, 2, 3]),
+                                torch.tensor([1, 2, 3, 4, 5, 6, 7, 8, 9], dtype=dtype))
+    B = csr_to_bsr(A, 2)
+    assert B.layout == torch.sparse_bsr and B.dtype == dtype
+    assert B.crow_indices().tolist() == [0, 1, 3]
+    assert B.col_indices().tolist() == [0, 0, 1]
+    assert B.values().tolist() == [[[1, 2], [0, 3]], [[4, 0], [6, 7]], [[5, 0], [8, 9]]]
+    assert B.values().stride() == (4, 1, 2)
+
+
+@pytest.mark.parametrize("ind_dtype", [torch.int32, torch.int64])
+@pytest.mark.parametrize("num_parts", [1, 2, 3, 4, None])
+def test_sort_csr_known_answer(ind_dtype, num_parts):
+    M = torch.sparse_csr_tensor(torch.tensor([0, 2, 3, 5, 9], dtype=ind_dtype),
+                                torch.tensor([1, 0, 1, 0, 2, 2, 3, 1, 0], dtype=ind_dtype),
+                                torch.tensor([2, 1, 3, 4, 5, 8, 9, 7, 6], dtype=torch.float64))
+    S = sort_csr(M, num_parts=num_parts)
+    assert torch.equal(S.crow_indices(), torch.tensor([0, 2, 3, 5, 9], dtype=torch.int32))
+    assert torch.equal(S.col_indices(), torch.tensor([0, 1, 1, 0, 2, 0, 1, 2, 3], dtype=torch.int32))
+    assert S.values().tolist() == [1, 2, 3, 4, 5, 6, 7, 8, 9]
+
+
+def test_gather_csr_known_answers():
+    G = gather_csr(torch.sparse_csr_tensor(torch.tensor([0, 3, 6], dtype=torch.int32),
+                                           torch.tensor([0, 2, 2, 0, 0, 1], dtype=torch.int32),
+                                           torch.tensor([1.0, 2, 3, 4, 5, 6])))
+    assert G.crow_indices().tolist() == [0, 2, 4] and G.col_indices().tolist() == [0, 2, 0, 1]
+    assert G.values().tolist() == [1.0, 5.0, 9.0, 6.0]
+    G = gather_csr(torch.sparse_csr_tensor(torch.tensor([0, 2, 2, 5], dtype=torch.int32),
+                                           torch.tensor([0, 1, 0, 0, 1], dtype=torch.int32),
+                                           torch.tensor([1.0, 2, 3, 4, 5])))
+    assert G.crow_indices().tolist() == [0, 2, 2, 4] and G.col_indices().tolist() == [0, 1, 0, 1]
+    assert G.values().tolist() == [1.0, 2.0, 7.0, 5.0]
+
+
+def test_asm_permutation_and_permute_dg24():
+    A, b, f2s, s2c, k = dg24()
+    f2d = torch.arange(24).reshape(8, 3)
+    perm = asm_permutation(torch.as_tensor(f2s), torch.as_tensor(s2c), f2d)
+    # restated: elements grouped by solver (stable), solvers grouped by coarse cell (stable)
+    n_s = s2c.shape[0]
+    sd = f2d.numpy()[np.argsort(f2s, kind="stable")].reshape(n_s, -1)
+    ref = sd[np.argsort(s2c, kind="stable")].reshape(-1)
+    assert perm.tolist() == ref.tolist()
+    inv = inverse_permutation(perm)
+    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr), torch.as_tensor(A.indices),
+                                 torch.as_tensor(A.data), A.shape)
+    Ap = permute_csr_tensor(At, inv)
+    assert np.allclose(Ap.to_dense().numpy(), A.toarray()[ref][:, ref])
+
+
+# ---- problem generators ------------------------------------------------------------------
+
+def test_q1_stencil_closed_form():
+    K = P.q1_reference_stiffness()
+    for a in range(8):
+        for b in range(8):
+            dist = bin(a ^ b).count("1")
+            assert np.isclose(K[a, b], P.Q1_REF_STIFFNESS_BY_DISTANCE[dist])
+    N = 6
+    A = P.assemble_q1_3d(N)
+    m = N - 1
+    c = (2 * m + 2) * m + 2  # an interior node away from the boundary
+    row = A.getrow(c).toarray().ravel()
+    h = 1.0 / N
+    assert np.isclose(row[c], 8 * h / 3)
+    assert np.isclose(row[c + 1], 0.0)                  # face neighbour
+    assert np.isclose(row[c + 1 + m], -h / 6)           # edge neighbour
+    assert np.isclose(row[c + 1 + m + m * m], -h / 12)  # corner neighbour
+    assert abs(row.sum()) < 1e-14
+
+
+def test_q1_structured_matches_element_assembly():
+    N = 5
+    kappa = P.lognormal_coefficients(N ** 3, 9).reshape(N, N, N)
+    A = P.assemble_q1_3d(N, kappa)
+    Kref = P.q1_reference_stiffness()
+    dofs = P.q1_element_nodes(N)
+    n = (N - 1) ** 3
+    B = np.zeros((n, n))
+    for e in range(N ** 3):
+        Ke = kappa.reshape(-1)[e] * Kref / N
+        for a in range(8):
+            for bb in range(8):
+                if dofs[e, a] >= 0 and dofs[e, bb] >= 0:
+                    B[dofs[e, a], dofs[e, bb]] += Ke[a, bb]
+    assert np.allclose(A.toarray(), B, atol=1e-14)
+
+
+def test_p1_5_point_stencil():
+    A, *_ = P.assemble_p1_2d(8)
+    m = 7
+    c = 3 * m + 3
+    row = A.getrow(c).toarray().ravel()
+    assert np.isclose(row[c], 4.0)
+    for d in (1, -1, m, -m):
+        assert np.isclose(row[c + d], -1.0)
+    assert abs(row[c + m + 1]) < 1e-14 and abs(row[c - m - 1]) < 1e-14
+    assert np.isclose(row.sum(), 0.0)
+
+
+def test_sizes_of_the_configurations():
+    pb = P.make_problem("p1_2d", 128, box=8, overlap=2)
+    d = pb.decomposition
+    assert pb.A.shape[0] == 16129 and pb.A.nnz == 111889
+    assert d.n_subdomains == 256 and d.sizes.min() == 81 and d.sizes.max() == 144
+    d3 = P.box_decomposition(63, 3, 8, 1)
+    assert d3.n_subdomains == 512 and d3.sizes.min() == 512 and d3.sizes.max() == 1000
+    assert int(d3.sizes.sum()) == 456533
+
+
+@pytest.mark.parametrize("overlap", [0, 1, 2])
+def test_decomposition_invariants(overlap):
+    pb = P.make_problem("q1_3d", 10, box=3, overlap=overlap, seed=1, lognormal=True)
+    d = pb.decomposition
+    n = pb.A.shape[0]
+    assert np.array_equal(np.sort(d.perm), np.arange(n))
+    seen = np.zeros(n, dtype=int)
+    for i in range(d.n_subdomains):
+        s = d.subdomain(i)
+        assert np.all(np.diff(s) > 0)
+        core = np.arange(d.owner_ptr[i], d.owner_ptr[i + 1])
+        assert np.isin(core, s).all()
+        seen[s] += 1
+    assert seen.min() >= 1
+    if overlap == 0:
+        assert seen.max() == 1
+    # the permuted matrix is the original one renumbered
+    A0 = pb.A0.toarray()
+    assert np.allclose(pb.A.toarray(), A0[d.perm][:, d.perm])
+
+
+def test_kmeans_decomposition_uneven():
+    A, xy, tris, n2d = P.assemble_p1_2d(64, jitter=0.2, seed=3)
+    d = P.kmeans_decomposition(A, xy, tris, n2d, 16, overlap=2, seed=3)
+    assert d.n_subdomains == 16
+    assert d.sizes.max() > d.sizes.min()
+    seen = np.zeros(A.shape[0], dtype=int)
+    for i in range(d.n_subdomains):
+        seen[d.subdomain(i)] += 1
+    assert seen.min() >= 1
