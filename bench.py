@@ -122,19 +122,28 @@ def cpu_baseline(pb, iterations, budget_s=20.0):
 
     t_spmv = best(lambda: A @ x)
     sizes = d.sizes
-    sample = rng.choice(d.n_subdomains, size=min(8, d.n_subdomains), replace=False)
-    t_inv = 0.0
-    t_loc = 0.0
-    for i in sample:
-        idx = d.subdomain(int(i))
-        Ai = A[idx][:, idx].toarray()
-        t = time.perf_counter()
-        Ainv = np.linalg.inv(Ai)
-        t_inv += time.perf_counter() - t
-        xi = x[idx]
-        t_loc += best(lambda: Ainv @ xi)
-    scale = float((sizes.astype(np.float64) ** 2).sum() / (sizes[sample].astype(np.float64) ** 2).sum())
+    # Local inverse matvecs: a sample of subdomains, with as many distinct matrices as it
+    # takes to exceed the host caches (>= 1 GiB), so the timing streams from DRAM as the
+    # full set of inverses would. A matvec's time does not depend on the values, so the
+    # sample's matrices are random with the sampled subdomains' sizes.
+    sample = rng.choice(d.n_subdomains, size=min(d.n_subdomains, 256), replace=False)
+    ks = sizes[sample].astype(np.int64)
+    need = int(np.searchsorted(np.cumsum(ks ** 2 * 8), 1 << 30)) + 1
+    ks = ks[:max(1, min(need, ks.shape[0]))]
+    mats = [rng.standard_normal((int(k), int(k))) for k in ks]
+    vecs = [rng.standard_normal(int(k)) for k in ks]
+    t = time.perf_counter()
+    np.linalg.inv(mats[0])
+    t_inv = time.perf_counter() - t
+
+    def local_pass():
+        for Mi, vi in zip(mats, vecs):
+            Mi @ vi
+
+    t_loc = best(local_pass, reps=2)
+    scale = float((sizes.astype(np.float64) ** 2).sum() / (ks.astype(np.float64) ** 2).sum())
     t_local = t_loc * scale
+    del mats
     # coarse: dense SPD of size n_c
     nc = d.n_subdomains
     B = rng.standard_normal((nc, nc))
@@ -163,11 +172,12 @@ def cpu_baseline(pb, iterations, budget_s=20.0):
         "cores": int(threads),
         "kind": "port",
         "sample": (f"oracle FP64 components on this host: full scipy CSR SpMV ({t_spmv * 1e3:.1f} ms), "
-                   f"dense inverse matvecs of {len(sample)}/{d.n_subdomains} subdomains scaled by "
-                   f"sum k_i^2 ({t_local * 1e3:.1f} ms/iter), cho_solve n_c={nc} "
+                   f"dense FP64 inverse matvecs over {len(ks)} matrices of the sampled "
+                   f"subdomain sizes ({(ks ** 2).sum() * 8 / 2 ** 30:.2f} GiB, out of cache) "
+                   f"scaled by sum k_i^2 ({t_local * 1e3:.1f} ms/iter), cho_solve n_c={nc} "
                    f"({t_coarse * 1e3:.2f} ms), vector ops ({t_vec * 1e3:.1f} ms); per-iteration "
-                   f"sum x {iterations} iterations; local inverse setup excluded "
-                   f"({t_inv:.1f} s for the sample); cpu_count={os.cpu_count()}"),
+                   f"sum x {iterations} iterations; setup excluded (one {int(ks[0])}^2 inverse "
+                   f"takes {t_inv * 1e3:.0f} ms); cpu_count={os.cpu_count()}"),
     }
 
 
@@ -180,6 +190,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay captured iteration blocks")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -198,7 +209,7 @@ def main():
     b = torch.as_tensor(pb.b, device=dev)
 
     # ---- plain FP64 (the measured line) ------------------------------------------------
-    solver = make_solver(pb, torch.float64, dev)
+    solver = make_solver(pb, torch.float64, dev, use_graph=args.graph)
     x, meta, total, times = time_solves(solver, b, args.steps, args.warmup)
     iters = meta["iterations"]
     tts = total / args.steps
@@ -214,7 +225,10 @@ def main():
     xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
     Y = M.Y if M.Y is not None else torch.empty_like(xv)
     t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
-    alg_bytes = int((sizes ** 2).sum() * 8 + sizes.sum() * (8 + 4 + 8))
+    # algorithmic bytes of one launch: the matrices (lower triangle for the symmetric
+    # layout, whole matrices for "full"), the gathered x (value + index) and the result
+    alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4)
+                    + (Y.numel() if M.Y is not None else sizes.sum()) * 8)
     gbs = alg_bytes / (t_loc * 1e-3) / 1e9
     log(f"local apply: {t_loc * 1e3:.1f} us, {alg_bytes / 1e9:.3f} GB -> {gbs:.0f} GB/s "
         f"({gbs / HBM_PEAK_GBS * 100:.1f}% of HBM peak)")
@@ -238,7 +252,7 @@ def main():
         for name, st, scaled in (("fp32", torch.float32, False), ("fp16", torch.float16, True),
                                  ("bf16", torch.bfloat16, False)):
             try:
-                s2 = make_solver(pb, st, dev, scaled=scaled)
+                s2 = make_solver(pb, st, dev, scaled=scaled, use_graph=args.graph)
                 x2, m2, tot2, t2 = time_solves(s2, b, args.steps, args.warmup)
                 e2 = float(np.linalg.norm(x2.cpu().numpy() - xh) / np.linalg.norm(xh))
                 r2 = float(np.linalg.norm(pb.b - pb.A @ x2.cpu().numpy()) / np.linalg.norm(pb.b))
@@ -284,7 +298,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "local_apply_kernel<double,double,double,4>",
+            "kernel": ("sym_apply_kernel<double,double,double>" if L.layout == "sym"
+                       else "local_apply_kernel<double,double,double,4>"),
+            "stored_matrix_bytes": int(L.bytes_matrices),
             "achieved": round(gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

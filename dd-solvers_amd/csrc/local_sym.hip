@@ -1,0 +1,233 @@
+// Local-inverse apply on symmetric, tile-packed storage: each subdomain inverse
+// A_s^-1 (symmetric) is stored as its lower triangle of 64 x 64 tiles, so every
+// off-diagonal tile is read from HBM once and used twice (y_I += T x_J and y_J += T^T x_I).
+//
+// Storage of subdomain s (k = k_s unknowns, T = ceil(k/64) tile rows, last block rL rows):
+//   tile (I, J), J <= I, R_I = 64 rows (rL for the last tile row), chunks of V = 16/sizeof
+//   elements: chunk j of row r holds columns [jV, jV+V) at tile_off + (j*R_I + r)*V, so a
+//   wave loads one chunk of all its rows as one coalesced 1 KiB (64 x 16 B) access.
+//   tile_off(I, J) = 4096*(I(I+1)/2 + J)                         for I < T-1
+//                  = 4096*(T-1)T/2 + 64*rL*J                     for I = T-1
+//   Off-diagonal tiles have 64 chunk columns' worth of data (NCH = 64/V chunks); the
+//   diagonal tiles are stored in full (both triangles), the last one with ceil(rL/V)
+//   chunks. Padding entries are zero.
+//
+// Work item (s, p): one wavefront handles tile rows I1 = p and I2 = T-1-p (one row if they
+// coincide), so every item touches T+1 tiles. It writes a partial result vector Yp[s][p]
+// (length 64*T, blocks 0..I2 written); the partial vectors of a subdomain are added later
+// in item order (dds_overlap_sum with the store's lists), which keeps the result
+// deterministic without atomics.
+#include "common.hpp"
+
+namespace dds {
+namespace {
+
+template <typename TM>
+struct SymShape {
+  static constexpr int V = 16 / (int)sizeof(TM);  // elements per 16-byte chunk
+  static constexpr int NCH = 64 / V;              // chunks per 64-wide tile row
+  static constexpr int NH = NCH / 2;              // chunks per column half (32 columns)
+  static constexpr int G = NH >= 16 ? 1 : 16 / NH;
+};
+
+// LDS position (in chunks) of chunk j (of a column half) of tile row r: XOR swizzle so that
+// the chunk writes (lane = row) and the column reads (lane = column) avoid bank conflicts.
+template <typename TM>
+__device__ __forceinline__ int sym_slot(int r, int j) {
+  using S = SymShape<TM>;
+  return r * S::NH + (j ^ ((r / S::G) % S::NH));
+}
+
+// One 64 x 64 tile (R rows, nch chunk columns) at tp, processed by column halves:
+//   row     += sum_c T[lane][c] * xjs[c]                     (lane = row)
+//   returns sum_r T[r][c] * xis[r] for column c = lane          (if transposed)
+// Each half (32 columns) is loaded as NH coalesced 1 KiB accesses; for the transposed
+// product it is staged in LDS, lanes 0-31 sum rows 0-31 and lanes 32-63 rows 32-63 of
+// column lane % 32, and one cross-lane exchange adds the two row halves.
+template <typename TM, typename TA>
+__device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool transposed,
+                                         const TA* xjs, const TA* xis, TM* lds, int lane,
+                                         TA& row) {
+  using S = SymShape<TM>;
+  TA col[2] = {0, 0};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h * S::NH >= nch) break;
+    Vec16<TM> c[S::NH];
+#pragma unroll
+    for (int j = 0; j < S::NH; ++j) {
+      const int jj = h * S::NH + j;
+      if (jj < nch && lane < R) {
+        c[j] = load16(tp + ((int64_t)jj * R + lane) * S::V);
+      } else {
+#pragma unroll
+        for (int e = 0; e < S::V; ++e) c[j].v[e] = TM{};
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < S::NH; ++j) {
+      const int jj = h * S::NH + j;
+      if (jj < nch) {
+#pragma unroll
+        for (int e = 0; e < S::V; ++e)
+          row += static_cast<TA>(widen(c[j].v[e])) * xjs[jj * S::V + e];
+      }
+    }
+    if (transposed) {
+      __syncthreads();  // previous readers of lds are done
+#pragma unroll
+      for (int j = 0; j < S::NH; ++j)
+        *reinterpret_cast<Vec16<TM>*>(lds + sym_slot<TM>(lane, j) * S::V) = c[j];
+      __syncthreads();
+      const int cc = lane & 31;
+      const int jc = cc / S::V;
+      const int ec = cc % S::V;
+      const int r0 = lane & 32;
+      const int nr = R - r0 < 32 ? R - r0 : 32;
+      TA acc = 0;
+      int t = 0;
+      for (; t + 3 < nr; t += 4) {
+        const TA a0 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec]));
+        const TA a1 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 1, jc) * S::V + ec]));
+        const TA a2 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 2, jc) * S::V + ec]));
+        const TA a3 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 3, jc) * S::V + ec]));
+        acc += a0 * xis[r0 + t];
+        acc += a1 * xis[r0 + t + 1];
+        acc += a2 * xis[r0 + t + 2];
+        acc += a3 * xis[r0 + t + 3];
+      }
+      for (; t < nr; ++t)
+        acc += static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec])) * xis[r0 + t];
+      acc += __shfl_xor(acc, 32, 64);
+      col[h] = acc;
+    }
+  }
+  return lane < 32 ? col[0] : col[1];
+}
+
+template <typename TM, typename TV, typename TA>
+__global__ __launch_bounds__(64) void sym_apply_kernel(
+    const int32_t* __restrict__ items, const int64_t* __restrict__ sub_ptr,
+    const int32_t* __restrict__ sub_idx, const int64_t* __restrict__ mat_off,
+    const int64_t* __restrict__ ypos, const float* __restrict__ scale,
+    const TM* __restrict__ mats, const TV* __restrict__ x, TV* __restrict__ Yp,
+    const dds_pcg_state* __restrict__ gate) {
+  using S = SymShape<TM>;
+  if (gate != nullptr && gate->done) return;
+  __shared__ __align__(16) TM lds_tile[64 * 32];
+  __shared__ __align__(16) TA xjs[64];
+  __shared__ __align__(16) TA xi1s[64];
+  __shared__ __align__(16) TA xi2s[64];
+
+  const int lane = threadIdx.x;
+  const int s = items[2 * blockIdx.x];
+  const int p = items[2 * blockIdx.x + 1];
+  const int64_t base = sub_ptr[s];
+  const int k = (int)(sub_ptr[s + 1] - base);
+  const int T = (k + 63) / 64;
+  const int rL = k - 64 * (T - 1);
+  const int I1 = p;
+  const int I2 = T - 1 - p;
+  const bool two = I1 != I2;
+  const TM* M = mats + mat_off[s];
+  TV* Y = Yp + ypos[s] + (int64_t)p * 64 * T;
+  const TA sc = scale != nullptr ? static_cast<TA>(scale[s]) : static_cast<TA>(1);
+
+  auto gather = [&](int J) -> TA {
+    const int a = 64 * J + lane;
+    if (a >= k) return static_cast<TA>(0);
+    const int64_t g = sub_idx != nullptr ? (int64_t)sub_idx[base + a] : base + a;
+    return static_cast<TA>(x[g]);
+  };
+  auto tile_off = [&](int I, int J) -> int64_t {
+    if (I < T - 1) return 4096 * ((int64_t)I * (I + 1) / 2 + J);
+    return 4096 * ((int64_t)(T - 1) * T / 2) + (int64_t)64 * rL * J;
+  };
+  auto rows_of = [&](int I) { return I < T - 1 ? 64 : rL; };
+  auto chunks_of = [&](int J) { return J < T - 1 ? S::NCH : (rL + S::V - 1) / S::V; };
+  auto store = [&](int J, TA v) {
+    if (64 * J + lane < k) Y[64 * J + lane] = static_cast<TV>(v * sc);
+  };
+
+  xi1s[lane] = gather(I1);
+  xi2s[lane] = gather(I2);
+  TA row1 = 0, row2 = 0, hold1 = 0;
+  TA xj = gather(0);
+  for (int J = 0; J <= I2; ++J) {
+    const int nch = chunks_of(J);
+    __syncthreads();  // readers of xjs from the previous J are done
+    xjs[lane] = xj;
+    if (J < I2) xj = gather(J + 1);
+    __syncthreads();
+    TA contrib = tile_apply<TM, TA>(M + tile_off(I2, J), rows_of(I2), nch, J < I2, xjs, xi2s,
+                                    lds_tile, lane, row2);
+    if (two && J <= I1)
+      contrib += tile_apply<TM, TA>(M + tile_off(I1, J), rows_of(I1), nch, J < I1, xjs, xi1s,
+                                    lds_tile, lane, row1);
+    if (J < I2) {
+      if (two && J == I1)
+        hold1 = contrib;
+      else
+        store(J, contrib);
+    }
+  }
+  if (two) store(I1, row1 + hold1);
+  store(I2, row2);
+}
+
+template <typename TM, typename TV, typename TA>
+int launch_sym(int64_t n_items, const int32_t* items, const int64_t* sub_ptr,
+               const int32_t* sub_idx, const int64_t* mat_off, const int64_t* ypos,
+               const float* scale, const void* mats, const void* x, void* Yp,
+               const dds_pcg_state* gate, hipStream_t s) {
+  if (n_items == 0) return DDS_OK;
+  hipLaunchKernelGGL((sym_apply_kernel<TM, TV, TA>), dim3((unsigned)n_items), dim3(64), 0, s,
+                     items, sub_ptr, sub_idx, mat_off, ypos, scale,
+                     static_cast<const TM*>(mats), static_cast<const TV*>(x),
+                     static_cast<TV*>(Yp), gate);
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+}  // namespace
+}  // namespace dds
+
+using namespace dds;
+
+extern "C" int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_items,
+                                   const int32_t* items, const int64_t* sub_ptr,
+                                   const int32_t* sub_idx, const int64_t* mat_off,
+                                   const int64_t* ypos, const float* scale, const void* mats,
+                                   const void* x, void* Yp, const dds_pcg_state* gate,
+                                   void* stream) {
+  DDS_REQUIRE(n_items >= 0 && n_items <= INT32_MAX, "dds_local_apply_sym: bad item count");
+  if (n_items > 0)
+    DDS_REQUIRE(items && sub_ptr && mat_off && ypos && mats && x && Yp,
+                "dds_local_apply_sym: null pointer");
+  hipStream_t s = as_stream(stream);
+#define DDS_ARGS n_items, items, sub_ptr, sub_idx, mat_off, ypos, scale, mats, x, Yp, gate, s
+  if (vec_dtype == DDS_F64 && acc_dtype == DDS_F64) {
+    switch (mat_dtype) {
+      case DDS_F64: return launch_sym<double, double, double>(DDS_ARGS);
+      case DDS_F32: return launch_sym<float, double, double>(DDS_ARGS);
+      case DDS_F16: return launch_sym<_Float16, double, double>(DDS_ARGS);
+      case DDS_BF16: return launch_sym<bf16_t, double, double>(DDS_ARGS);
+    }
+  } else if (vec_dtype == DDS_F64 && acc_dtype == DDS_F32) {
+    switch (mat_dtype) {
+      case DDS_F32: return launch_sym<float, double, float>(DDS_ARGS);
+      case DDS_F16: return launch_sym<_Float16, double, float>(DDS_ARGS);
+      case DDS_BF16: return launch_sym<bf16_t, double, float>(DDS_ARGS);
+    }
+  } else if (vec_dtype == DDS_F32 && acc_dtype == DDS_F32) {
+    switch (mat_dtype) {
+      case DDS_F32: return launch_sym<float, float, float>(DDS_ARGS);
+      case DDS_F16: return launch_sym<_Float16, float, float>(DDS_ARGS);
+      case DDS_BF16: return launch_sym<bf16_t, float, float>(DDS_ARGS);
+    }
+  }
+#undef DDS_ARGS
+  set_error("dds_local_apply_sym: unsupported dtypes (mat %d, vec %d, acc %d)", mat_dtype,
+            vec_dtype, acc_dtype);
+  return DDS_EUNSUPPORTED;
+}

@@ -128,14 +128,18 @@ class DenseCholesky(SparseSolver):
     to a few thousand), so the factor is stored dense (row-major, lower triangle) and
     computed by a blocked right-looking Cholesky (dds_chol_factor).
 
-    mode "trsv"    each solve runs forward and backward substitution in one workgroup
-                   (dds_chol_solve); the least traffic per solve for small n.
-    mode "inverse" setup also forms A^-1 = L^-T L^-1 (dds_chol_inverse) and each solve is
-                   one dense symmetric matvec over all CUs (dds_dense_symv); for large n.
-    mode "auto"    "trsv" below `inverse_threshold` unknowns, else "inverse".
+    mode "trsv"    each solve runs forward and backward substitution with L in one
+                   workgroup (dds_chol_solve): the least traffic, but a sequential chain
+                   of 2 * n / 64 dependent steps on one CU.
+    mode "inverse" setup also forms A^-1 = L^-T L^-1 by triangular solves with the factor
+                   (dds_chol_inverse) and each solve is one dense symmetric matvec spread
+                   over all CUs (dds_dense_symv).
+    mode "auto"    "inverse" from `inverse_threshold` unknowns up, else "trsv". On MI355X
+                   the matvec wins from tiny sizes on (n = 512: a few us against ~0.3 ms
+                   for the one-workgroup substitution), so the default threshold is 32.
     """
 
-    def __init__(self, mode: str = "auto", inverse_threshold: int = 1024):
+    def __init__(self, mode: str = "auto", inverse_threshold: int = 32):
         if mode not in ("auto", "trsv", "inverse"):
             raise ValueError(f"unknown mode {mode!r}")
         self.mode = mode
@@ -599,13 +603,19 @@ class Inv(DenseBatchSolver):
                   reference); torch.float32 gives FP32 accumulation for FP64 vectors.
       scaled      store A_i^-1 / max|A_i^-1| per subdomain and rescale the result (keeps
                   FP16 storage in range).
+      layout      "sym" (default): the lower triangle in 64 x 64 tiles, each off-diagonal
+                  tile read once and applied twice (dds_local_apply_sym); "full": the
+                  whole matrix column-major (dds_local_apply).
     """
 
     def __init__(self, precision: torch.dtype | None = None, accumulate: torch.dtype | None = None,
-                 scaled: bool = False):
+                 scaled: bool = False, layout: str = "sym"):
+        if layout not in ("sym", "full"):
+            raise ValueError(f"unknown layout {layout!r}")
         self.precision = precision
         self.accumulate = accumulate
         self.scaled = scaled
+        self.layout = layout
 
     def __str__(self):
         return f"Inv({self.precision or ''})"
@@ -667,6 +677,64 @@ class _LocalStore:
     mat_dtype: torch.dtype
     acc_dtype: torch.dtype
     bytes_matrices: int
+    layout: str = "full"
+    items: torch.Tensor | None = None   # sym: (n_items, 2) int32 (subdomain, tile-row pair)
+    ypos: torch.Tensor | None = None    # sym: (n_s,) int64 offsets of the partial vectors
+    n_items: int = 0
+    bytes_algorithmic: int = 0          # lower triangle (sym) or full matrix, storage dtype
+
+
+_SYM_INDEX_CACHE: dict = {}
+
+
+def _sym_index(k: int, V: int) -> np.ndarray:
+    """Source index (row * k + col, -1 for padding) of every element of the symmetric tile
+    layout of a k x k matrix with 16-byte chunks of V elements (csrc/local_sym.hip)."""
+    key = (k, V)
+    if key in _SYM_INDEX_CACHE:
+        return _SYM_INDEX_CACHE[key]
+    nch_full = 64 // V
+    T = -(-k // 64)
+    rL = k - 64 * (T - 1)
+    parts = []
+    for I in range(T):
+        R = 64 if I < T - 1 else rL
+        for J in range(I + 1):
+            nch = nch_full if J < T - 1 else -(-rL // V)
+            j = np.arange(nch)[:, None, None]
+            r = np.arange(R)[None, :, None]
+            e = np.arange(V)[None, None, :]
+            row = 64 * I + r + 0 * j + 0 * e
+            col = 64 * J + j * V + e + 0 * r
+            parts.append(np.where(col < k, row * k + col, -1).reshape(-1))
+    idx = np.concatenate(parts).astype(np.int64)
+    _SYM_INDEX_CACHE[key] = idx
+    return idx
+
+
+def _sym_plan(sizes: np.ndarray, sub_ptr: np.ndarray, sub_idx, n: int):
+    """Work items, partial-vector offsets and the per-unknown lists that add the partial
+    vectors (in subdomain, then item order) for the symmetric layout."""
+    T = -(-sizes // 64)
+    P = -(-T // 2)
+    ypos = np.concatenate([[0], np.cumsum(P * 64 * T)]).astype(np.int64)
+    items = np.concatenate([np.stack([np.full(int(P[s]), s), np.arange(int(P[s]))], 1)
+                            for s in range(sizes.shape[0])]).astype(np.int32)
+    js, ps = [], []
+    for s in range(sizes.shape[0]):
+        k = int(sizes[s])
+        a = np.arange(k)
+        g = sub_idx[sub_ptr[s]:sub_ptr[s + 1]] if sub_idx is not None else sub_ptr[s] + a
+        blk = a // 64
+        for p in range(int(P[s])):
+            m = blk <= T[s] - 1 - p
+            js.append(g[m])
+            ps.append(ypos[s] + p * 64 * T[s] + a[m])
+    j = np.concatenate(js)
+    pos = np.concatenate(ps)
+    order = np.argsort(j, kind="stable")
+    dof_ptr = np.concatenate([[0], np.cumsum(np.bincount(j, minlength=n))]).astype(np.int64)
+    return items, ypos, dof_ptr, pos[order].astype(np.int32)
 
 
 def _tiles_for(sizes: np.ndarray, rows: int) -> np.ndarray:
@@ -880,8 +948,16 @@ class SchwarzOperator(SparseSolver):
             raise ValueError("float64 local inverses need a float64 accumulator")
         width = _lib._lib.dds_local_vec_width(_lib.dtype_code(store_dtype))
         rows_per_tile = _lib._lib.dds_local_tile_rows(_lib.dtype_code(store_dtype))
-        ld = ((sizes + width - 1) // width) * width
-        mat_off = np.concatenate([[0], np.cumsum(ld * sizes)]).astype(np.int64)
+        layout = getattr(ls, "layout", "full")
+        if layout == "sym":
+            sym_idx = {int(k): _sym_index(int(k), width) for k in np.unique(sizes)}
+            sym_len = np.array([sym_idx[int(k)].shape[0] for k in sizes], dtype=np.int64)
+            ld = sizes.copy()
+            mat_off = np.concatenate([[0], np.cumsum(sym_len)]).astype(np.int64)
+            sym_idx_dev = {}
+        else:
+            ld = ((sizes + width - 1) // width) * width
+            mat_off = np.concatenate([[0], np.cumsum(ld * sizes)]).astype(np.int64)
         total = int(mat_off[-1])
         mats = torch.zeros(total, dtype=store_dtype, device=dev)
         scaled = bool(getattr(ls, "scaled", False))
@@ -911,18 +987,40 @@ class SchwarzOperator(SparseSolver):
                     sc = m.abs().max().to(torch.float32)
                     scale[i] = sc
                     m = m / sc.to(m.dtype)
-                # column-major with padded leading dimension: element (a, b) at b*ld + a
-                col_major = torch.zeros((k, int(ld[i])), dtype=pi, device=dev)
-                col_major[:, :k] = m.transpose(0, 1)
-                mats[int(mat_off[i]):int(mat_off[i + 1])] = col_major.reshape(-1).to(store_dtype)
+                if layout == "sym":
+                    if k not in sym_idx_dev:
+                        sym_idx_dev[k] = torch.as_tensor(sym_idx[k], device=dev)
+                    src = sym_idx_dev[k]
+                    packed = torch.where(src >= 0, m.reshape(-1)[src.clamp(min=0)],
+                                         torch.zeros((), dtype=pi, device=dev))
+                    mats[int(mat_off[i]):int(mat_off[i + 1])] = packed.to(store_dtype)
+                else:
+                    # column-major with padded leading dimension: element (a, b) at b*ld + a
+                    col_major = torch.zeros((k, int(ld[i])), dtype=pi, device=dev)
+                    col_major[:, :k] = m.transpose(0, 1)
+                    mats[int(mat_off[i]):int(mat_off[i + 1])] = col_major.reshape(-1).to(store_dtype)
             del blk, inv
-        tiles = _tiles_for(sizes, rows_per_tile)
+        esz = mats.element_size()
+        if layout == "sym":
+            items, ypos, dptr, dpos = _sym_plan(sizes, self._host["sub_ptr"], sub_idx, n)
+            tiles = np.zeros((0, 2), dtype=np.int32)
+            self.Y = torch.empty(int(ypos[-1]), dtype=pi, device=dev)
+            self.dof_ptr = torch.as_tensor(dptr, device=dev)
+            self.dof_pos = torch.as_tensor(dpos, device=dev)
+            alg = int((sizes.astype(np.int64) * (sizes + 1) // 2).sum()) * esz
+        else:
+            tiles = _tiles_for(sizes, rows_per_tile)
+            items = ypos = None
+            alg = int((sizes.astype(np.int64) ** 2).sum()) * esz
         self.local = _LocalStore(
             mats=mats, mat_off=torch.as_tensor(mat_off[:-1], device=dev),
             ld=torch.as_tensor(ld.astype(np.int32), device=dev), scale=scale,
             tiles=torch.as_tensor(tiles, device=dev), n_tiles=int(tiles.shape[0]),
             max_k=k_max, mat_dtype=store_dtype, acc_dtype=acc,
-            bytes_matrices=int(total * mats.element_size()))
+            bytes_matrices=int(total * esz), layout=layout,
+            items=None if items is None else torch.as_tensor(items, device=dev),
+            ypos=None if ypos is None else torch.as_tensor(ypos[:-1], device=dev),
+            n_items=0 if items is None else int(items.shape[0]), bytes_algorithmic=alg)
         collect()
 
     # -- apply pieces ---------------------------------------------------------------------
@@ -938,6 +1036,13 @@ class SchwarzOperator(SparseSolver):
             Y[:self.n].copy_(self.local_solver.solve(x[:self.n].reshape(self.n_solvers, k)).reshape(-1))
             return
         L = self.local
+        if L.layout == "sym":
+            _lib.call("dds_local_apply_sym", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
+                      _lib.dtype_code(L.acc_dtype), L.n_items, L.items.data_ptr(),
+                      self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), L.mat_off.data_ptr(),
+                      L.ypos.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(), x.data_ptr(),
+                      Y.data_ptr(), _lib.ptr(gate), _lib.stream())
+            return
         _lib.call("dds_local_apply", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
                   _lib.dtype_code(L.acc_dtype), L.n_tiles, L.tiles.data_ptr(),
                   self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), L.max_k,
@@ -1013,9 +1118,11 @@ class SchwarzOperator(SparseSolver):
         e = 8 if self.pi == torch.float64 else 4
         sizes = self._host["sizes"]
         nk = int(sizes.sum())
-        b = self.local.bytes_matrices + e * 2 * nk
+        b = self.local.bytes_algorithmic + e * 2 * nk
         if self.sub_idx is not None:
-            b += 4 * nk + e * 2 * nk + 4 * nk + 8 * (self.n + 1)  # gather idx, Y, dof_pos/ptr
+            b += 4 * nk  # gather indices
+        if self.dof_pos is not None:  # staged partials: written, read back with the lists
+            b += 2 * e * int(self.dof_pos.shape[0]) + 4 * int(self.dof_pos.shape[0]) + 8 * (self.n + 1)
         b += e * 2 * self.n  # r read by the restriction/sum, z written
         b += getattr(self.coarse_solver, "bytes_per_solve", 0)
         return b

@@ -17,6 +17,7 @@
  *                             HybridSchwarz (src/dd_solvers/dd_solvers/solvers.py:226,235,264,880)
  *   dds_local_apply           Inv.solve -> gemv_strided_batched (solvers.py:355-359),
  *                             generalised to uneven and overlapping subdomains
+ *   dds_local_apply_sym       the same on symmetric tile-packed inverses
  *   dds_overlap_sum           y += repeat_interleave(...) prolongation (solvers.py:821-825,
  *                             :882-887) fused with the overlapped sum_i R_i^T y_i
  *   dds_restrict              x.reshape(n_s,-1).sum(1) + segment_reduce (solvers.py:817-820)
@@ -134,6 +135,19 @@ int dds_local_apply(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_tiles
                     int64_t max_k, const int64_t* mat_off, const int32_t* ld,
                     const float* scale, const void* mats, const void* x, void* Y,
                     int waves, const dds_pcg_state* gate, void* stream);
+
+/* Symmetric local inverses stored as the lower triangle of 64 x 64 tiles (layout in
+ * csrc/local_sym.hip): each off-diagonal tile is read once and applied twice. One
+ * wavefront per item (s, p) = (subdomain, pair of tile rows p and T_s-1-p) writes the
+ * partial vector Yp[ypos[s] + p*64*T_s + a] for the first 64*(T_s - p) local rows a; the
+ * partials are summed by dds_overlap_sum with lists built for this layout. Replaces the same
+ * Inv.solve -> gemv_strided_batched step (solvers.py:355-359) as dds_local_apply, with half
+ * the matrix traffic. dtype rules as for dds_local_apply. */
+int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_items,
+                        const int32_t* items, const int64_t* sub_ptr, const int32_t* sub_idx,
+                        const int64_t* mat_off, const int64_t* ypos, const float* scale,
+                        const void* mats, const void* x, void* Yp, const dds_pcg_state* gate,
+                        void* stream);
 
 /* z[j] = sum_{p in [dof_ptr[j], dof_ptr[j+1])} Y[dof_pos[p]]   (dof_ptr == NULL: z[j] = Y[j])
  *        + coarse_sign * yc[cell[j]]                           (yc != NULL)

@@ -169,13 +169,17 @@ def _torch_csr(A):
 
 @pytest.mark.parametrize("storage", [torch.float64, torch.float32, torch.float16, torch.bfloat16])
 @pytest.mark.parametrize("overlap", [0, 1])
-@pytest.mark.parametrize("waves", [1, 4, 8])
-def test_local_apply_matches_oracle(storage, overlap, waves):
-    pb = P.make_problem("q1_3d", 14, box=4, overlap=overlap, seed=2, lognormal=True)
+@pytest.mark.parametrize("layout,waves", [("full", 1), ("full", 4), ("full", 8), ("sym", None)])
+@pytest.mark.parametrize("N,box", [(14, 4), (18, 8)])
+def test_local_apply_matches_oracle(storage, overlap, layout, waves, N, box):
+    # (18, 8): subdomains of 8^3 + overlap -> k up to 1000, several 64-row tiles
+    pb = P.make_problem("q1_3d", N, box=box, overlap=overlap, seed=2, lognormal=True)
     d = pb.decomposition
     scaled = storage == torch.float16
-    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(storage, scaled=scaled), dds.DenseCholesky())
+    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(storage, scaled=scaled, layout=layout),
+                            dds.DenseCholesky())
     M.setup(_torch_csr(pb.A), decomposition=d, waves=waves)
+    assert M.local.layout == layout
     sname = {torch.float64: "float64", torch.float32: "float32", torch.float16: "float16",
              torch.bfloat16: "bfloat16"}[storage]
     O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, storage=sname, scaled=scaled)

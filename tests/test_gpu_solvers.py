@@ -124,7 +124,7 @@ def test_missing_block_size():
 # ---------------------------------------------------------------------------------------
 
 def _parity(pb, kind, storage="float64", rtol=1e-8, check_every=8, use_graph=False,
-            bsr=None, it_tol=1, x_tol=1e-10):
+            bsr=None, it_tol=1, x_tol=1e-10, layout="sym"):
     d = pb.decomposition
     O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind=kind,
                       storage=storage)
@@ -133,7 +133,7 @@ def _parity(pb, kind, storage="float64", rtol=1e-8, check_every=8, use_graph=Fal
     cls = dds.AdditiveSchwarz if kind == "additive" else dds.HybridSchwarz
     tdt = {"float64": torch.float64, "float32": torch.float32, "float16": torch.float16,
            "bfloat16": torch.bfloat16}[storage]
-    s = dds.CG(cls(torch.float64, dds.Inv(tdt), dds.DenseCholesky()), rtol=rtol,
+    s = dds.CG(cls(torch.float64, dds.Inv(tdt, layout=layout), dds.DenseCholesky()), rtol=rtol,
                bsr_matmul=bsr is not None, check_every=check_every, use_graph=use_graph)
     kw = {"block_size": bsr} if bsr else {}
     s.setup(_torch_csr(pb.A), decomposition=d, **kw)
@@ -158,6 +158,11 @@ def test_c1_p1_2d_overlap2(kind):
 def test_q1_3d_lognormal_overlap1(kind):
     pb = P.make_problem("q1_3d", 24, box=8, overlap=1, seed=1, lognormal=True)
     _parity(pb, kind)
+
+
+def test_q1_3d_full_layout_matches_oracle():
+    pb = P.make_problem("q1_3d", 24, box=8, overlap=1, seed=1, lognormal=True)
+    _parity(pb, "hybrid", layout="full")
 
 
 @pytest.mark.parametrize("b", [2, 4, 8])

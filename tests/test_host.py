@@ -217,3 +217,74 @@ def test_kmeans_decomposition_uneven():
     for i in range(d.n_subdomains):
         seen[d.subdomain(i)] += 1
     assert seen.min() >= 1
+
+
+# ---- symmetric tile layout of the local inverses (host-side model of local_sym.hip) ------
+
+def _unpack_tile(packed, k, V, I, J):
+    """Tile (I, J) of the packed layout as a dense (R, 64) block, following the offsets
+    documented in csrc/local_sym.hip."""
+    T = -(-k // 64)
+    rL = k - 64 * (T - 1)
+    R = 64 if I < T - 1 else rL
+    nch = 64 // V if J < T - 1 else -(-rL // V)
+    off = 4096 * (I * (I + 1) // 2 + J) if I < T - 1 else 4096 * ((T - 1) * T // 2) + 64 * rL * J
+    out = np.zeros((R, 64))
+    for j in range(nch):
+        for r in range(R):
+            out[r, j * V:(j + 1) * V] = packed[off + (j * R + r) * V: off + (j * R + r + 1) * V]
+    return out
+
+
+@pytest.mark.parametrize("k", [1, 5, 64, 65, 130, 200])
+@pytest.mark.parametrize("V", [2, 4, 8])
+def test_sym_layout_and_item_plan(k, V):
+    from dd_solvers.solvers import _sym_index, _sym_plan
+
+    rng = np.random.default_rng(k * 10 + V)
+    B = rng.standard_normal((k, k))
+    A = B + B.T
+    idx = _sym_index(k, V)
+    packed = np.where(idx >= 0, A.reshape(-1)[np.maximum(idx, 0)], 0.0)
+    T = -(-k // 64)
+    # every stored tile matches the matrix (lower triangle, full diagonal tiles)
+    for I in range(T):
+        for J in range(I + 1):
+            t = _unpack_tile(packed, k, V, I, J)
+            blk = A[64 * I:64 * I + t.shape[0], 64 * J:64 * J + 64]
+            assert np.array_equal(t[:, :blk.shape[1]], blk)
+            assert not t[:, blk.shape[1]:].any()
+    # work items (pairs of tile rows) + combine lists reproduce A x
+    sizes = np.array([k])
+    items, ypos, dptr, dpos = _sym_plan(sizes, np.array([0, k]), None, k)
+    x = rng.standard_normal(k)
+    xp = np.zeros(64 * T)
+    xp[:k] = x
+    Yp = np.full(int(ypos[-1]), np.nan)
+    for s_, p in items:
+        I1, I2 = p, T - 1 - p
+        y = np.full(64 * T, np.nan)
+        rows = {I1: np.zeros(64), I2: np.zeros(64)}
+        hold = np.zeros(64)
+        for J in range(I2 + 1):
+            contrib = np.zeros(64)
+            for I in sorted({I1, I2}, reverse=True):
+                if J > I:
+                    continue
+                t = np.zeros((64, 64))
+                tt = _unpack_tile(packed, k, V, I, J)
+                t[:tt.shape[0]] = tt
+                rows[I] += t @ xp[64 * J:64 * J + 64]
+                if J < I:
+                    contrib += t.T @ xp[64 * I:64 * I + 64]
+            if J < I2:
+                if I1 != I2 and J == I1:
+                    hold = contrib
+                else:
+                    y[64 * J:64 * J + 64] = contrib
+        if I1 != I2:
+            y[64 * I1:64 * I1 + 64] = rows[I1] + hold
+        y[64 * I2:64 * I2 + 64] = rows[I2]
+        Yp[ypos[s_] + p * 64 * T: ypos[s_] + (p + 1) * 64 * T] = y
+    z = np.array([Yp[dpos[dptr[j]:dptr[j + 1]]].sum() for j in range(k)])
+    assert np.allclose(z, A @ x, rtol=1e-12, atol=1e-12)
