@@ -191,6 +191,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay captured iteration blocks")
+    ap.add_argument("--profile-kernel", action="store_true",
+                    help="only launch the local-apply kernel 20 times, then a 1 GiB device "
+                         "copy 20 times (calibration for rocprofv3 --pmc byte counters)")
+    ap.add_argument("--storage", default="float64", choices=["float64", "float32", "float16", "bfloat16"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -207,6 +211,29 @@ def main():
     torch.cuda.set_device(dev)
     pb = build_problem(args.N, args.seed)
     b = torch.as_tensor(pb.b, device=dev)
+
+    if args.profile_kernel:
+        st = getattr(torch, args.storage)
+        solver = make_solver(pb, st, dev, scaled=st == torch.float16)
+        M = solver.preconditioner
+        xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
+        Y = M.Y if M.Y is not None else torch.empty_like(xv)
+        for _ in range(20):
+            M._enqueue_local(xv, Y, None)
+        torch.cuda.synchronize()
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        L = M.local
+        sizes = M._host["sizes"].astype(np.int64)
+        log(f"profile-kernel: storage={args.storage} layout={L.layout} "
+            f"algorithmic matrix bytes={L.bytes_algorithmic} stored={L.bytes_matrices} "
+            f"vectors={int(sizes.sum() * 12 + Y.numel() * 8)}; copy 2 x {src.numel() * 8} bytes")
+        print(json.dumps({"profile_kernel": True, "alg_matrix_bytes": L.bytes_algorithmic,
+                          "stored_matrix_bytes": L.bytes_matrices}))
+        return
 
     # ---- plain FP64 (the measured line) ------------------------------------------------
     solver = make_solver(pb, torch.float64, dev, use_graph=args.graph)
@@ -227,8 +254,10 @@ def main():
     t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
     # algorithmic bytes of one launch: the matrices (lower triangle for the symmetric
     # layout, whole matrices for "full"), the gathered x (value + index) and the result
-    alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4)
-                    + (Y.numel() if M.Y is not None else sizes.sum()) * 8)
+    # (sym layout: plus the partial vectors, written by one launch and read by the next)
+    yp = getattr(M, "Yp", None)
+    alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + sizes.sum() * 8
+                    + (2 * yp.numel() * 8 if yp is not None else 0))
     gbs = alg_bytes / (t_loc * 1e-3) / 1e9
     log(f"local apply: {t_loc * 1e3:.1f} us, {alg_bytes / 1e9:.3f} GB -> {gbs:.0f} GB/s "
         f"({gbs / HBM_PEAK_GBS * 100:.1f}% of HBM peak)")
@@ -298,7 +327,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("sym_apply_kernel<double,double,double>" if L.layout == "sym"
+            "kernel": ("sym_apply_kernel<double,double,double> + sym_combine_kernel<double>"
+                       if L.layout == "sym"
                        else "local_apply_kernel<double,double,double,4>"),
             "stored_matrix_bytes": int(L.bytes_matrices),
             "achieved": round(gbs, 1),

@@ -86,6 +86,33 @@ __device__ __forceinline__ Vec16<T> load16(const T* p) {
   return *reinterpret_cast<const Vec16<T>*>(p);
 }
 
+// Streaming loads for data read once per launch (matrix values and indices): the
+// non-temporal hint keeps them from evicting the vectors that are reused from L2 / MALL.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ Vec16<T> load16_nt(const T* p) {
+  const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p));
+  Vec16<T> r;
+  __builtin_memcpy(&r, &v, 16);
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_nt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X. For a grid whose size
+// is a multiple of 8, this renumbers them so that each XCD runs a contiguous range of
+// logical block ids (neighbouring rows share that XCD's L2).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_block_id() {
+  const int64_t b = blockIdx.x;
+  const int64_t per = gridDim.x / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+
 // ---------------------------------------------------------------------------------------
 // Wave64 / block reductions (fixed order: butterfly within a wave, then waves in order).
 // ---------------------------------------------------------------------------------------

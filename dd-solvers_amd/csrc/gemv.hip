@@ -123,10 +123,10 @@ __global__ __launch_bounds__(WAVES * 64) void local_apply_kernel(
     const TM* Ar = mats + mat_off[s] + r;
     int c = w;
     for (; c + 3 * WAVES < k; c += 4 * WAVES) {
-      const Vec16<TM> a0 = load16(Ar + (int64_t)c * ldm);
-      const Vec16<TM> a1 = load16(Ar + (int64_t)(c + WAVES) * ldm);
-      const Vec16<TM> a2 = load16(Ar + (int64_t)(c + 2 * WAVES) * ldm);
-      const Vec16<TM> a3 = load16(Ar + (int64_t)(c + 3 * WAVES) * ldm);
+      const Vec16<TM> a0 = load16_nt(Ar + (int64_t)c * ldm);
+      const Vec16<TM> a1 = load16_nt(Ar + (int64_t)(c + WAVES) * ldm);
+      const Vec16<TM> a2 = load16_nt(Ar + (int64_t)(c + 2 * WAVES) * ldm);
+      const Vec16<TM> a3 = load16_nt(Ar + (int64_t)(c + 3 * WAVES) * ldm);
       const TA x0 = xs[c], x1 = xs[c + WAVES], x2 = xs[c + 2 * WAVES], x3 = xs[c + 3 * WAVES];
 #pragma unroll
       for (int j = 0; j < V; ++j) {
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(WAVES * 64) void local_apply_kernel(
       }
     }
     for (; c < k; c += WAVES) {
-      const Vec16<TM> a0 = load16(Ar + (int64_t)c * ldm);
+      const Vec16<TM> a0 = load16_nt(Ar + (int64_t)c * ldm);
       const TA x0 = xs[c];
 #pragma unroll
       for (int j = 0; j < V; ++j) acc[j] += static_cast<TA>(widen(a0.v[j])) * x0;

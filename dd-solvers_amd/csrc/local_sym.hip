@@ -58,7 +58,7 @@ __device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool tran
     for (int j = 0; j < S::NH; ++j) {
       const int jj = h * S::NH + j;
       if (jj < nch && lane < R) {
-        c[j] = load16(tp + ((int64_t)jj * R + lane) * S::V);
+        c[j] = load16_nt(tp + ((int64_t)jj * R + lane) * S::V);
       } else {
 #pragma unroll
         for (int e = 0; e < S::V; ++e) c[j].v[e] = TM{};
@@ -175,6 +175,29 @@ __global__ __launch_bounds__(64) void sym_apply_kernel(
   store(I2, row2);
 }
 
+// y[sub_ptr[s] + a] = sum over the items p of subdomain s that cover local row a (blocks
+// 0..T-1-p) of Yp[ypos[s] + p*64*T + a], in item order. Grid (ceil(max_k/256), n_s).
+template <typename TV>
+__global__ __launch_bounds__(256) void sym_combine_kernel(const int64_t* __restrict__ sub_ptr,
+                                                          const int64_t* __restrict__ ypos,
+                                                          const TV* __restrict__ Yp,
+                                                          TV* __restrict__ y,
+                                                          const dds_pcg_state* __restrict__ gate) {
+  if (gate != nullptr && gate->done) return;
+  const int s = blockIdx.y;
+  const int a = blockIdx.x * 256 + threadIdx.x;
+  const int64_t base = sub_ptr[s];
+  const int k = (int)(sub_ptr[s + 1] - base);
+  if (a >= k) return;
+  const int T = (k + 63) / 64;
+  const int P = (T + 1) / 2;
+  const int last = min(P - 1, T - 1 - a / 64);
+  const TV* src = Yp + ypos[s] + a;
+  TV acc = 0;
+  for (int p = 0; p <= last; ++p) acc += src[(int64_t)p * 64 * T];
+  y[base + a] = acc;
+}
+
 template <typename TM, typename TV, typename TA>
 int launch_sym(int64_t n_items, const int32_t* items, const int64_t* sub_ptr,
                const int32_t* sub_idx, const int64_t* mat_off, const int64_t* ypos,
@@ -230,4 +253,26 @@ extern "C" int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, 
   set_error("dds_local_apply_sym: unsupported dtypes (mat %d, vec %d, acc %d)", mat_dtype,
             vec_dtype, acc_dtype);
   return DDS_EUNSUPPORTED;
+}
+
+extern "C" int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, const int64_t* sub_ptr,
+                               const int64_t* ypos, const void* Yp, void* y,
+                               const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(n_s >= 0 && n_s <= 65535 && max_k >= 0, "dds_sym_combine: bad sizes");
+  if (n_s == 0 || max_k == 0) return DDS_OK;
+  DDS_REQUIRE(sub_ptr && ypos && Yp && y, "dds_sym_combine: null pointer");
+  const dim3 grid((unsigned)ceil_div(max_k, 256), (unsigned)n_s);
+  hipStream_t s = as_stream(stream);
+  if (dtype == DDS_F64) {
+    hipLaunchKernelGGL(sym_combine_kernel<double>, grid, dim3(256), 0, s, sub_ptr, ypos,
+                       static_cast<const double*>(Yp), static_cast<double*>(y), gate);
+  } else if (dtype == DDS_F32) {
+    hipLaunchKernelGGL(sym_combine_kernel<float>, grid, dim3(256), 0, s, sub_ptr, ypos,
+                       static_cast<const float*>(Yp), static_cast<float*>(y), gate);
+  } else {
+    set_error("dds_sym_combine: unsupported dtype %d", dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
 }

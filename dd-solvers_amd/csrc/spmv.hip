@@ -41,11 +41,11 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(int64_t n_rows, const I* __
     const int64_t e = row_ptr[row + 1];
     double acc = 0.0;
     for (int64_t p = s + lg; p < e; p += G)
-      acc += static_cast<double>(val[p]) * static_cast<double>(x[col[p]]);
+      acc += static_cast<double>(load_nt(val + p)) * static_cast<double>(x[load_nt(col + p)]);
     acc = group_sum<G>(acc);
     if (lg == 0) {
       y[row] = static_cast<T>(acc);
-      dot += acc * static_cast<double>(x[row]);
+      if (partials != nullptr) dot += acc * static_cast<double>(x[row]);
     }
   }
   if (partials != nullptr) write_partial(dot, partials, scratch);
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void bsr_row_kernel(int64_t n_block_rows, i
         acc += static_cast<double>(v[c * b]) * static_cast<double>(x[off + c]);
     }
     y[row] = static_cast<T>(acc);
-    dot += acc * static_cast<double>(x[row]);
+    if (partials != nullptr) dot += acc * static_cast<double>(x[row]);
   }
   if (partials != nullptr) write_partial(dot, partials, scratch);
 }
@@ -124,7 +124,63 @@ __global__ __launch_bounds__(kBlock) void bsr_seg_kernel(int64_t n_block_rows,
     if (lg < B) {
       const int64_t row = br * B + lg;
       y[row] = static_cast<T>(acc);
-      dot += acc * static_cast<double>(x[row]);
+      if (partials != nullptr) dot += acc * static_cast<double>(x[row]);
+    }
+  }
+  if (partials != nullptr) write_partial(dot, partials, scratch);
+}
+
+// ---------------------------------------------------------------------------------------
+// Sliced ELLPACK with 64-row slices (SELL-64): slice q holds rows 64q .. 64q+63 as
+// width_q columns stored column by column (entry j of row 64q + l at
+// slice_ptr[q] + 64*j + l), padded with (col = own row, value 0). One wavefront per slice,
+// lane = row: every value and index load is one coalesced 512 B / 256 B access, no row
+// pointers are read, and the padding costs nothing but its bytes.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void sell_kernel(int64_t n_rows, int64_t n_slices,
+                                                      const int64_t* __restrict__ slice_ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const T* __restrict__ val,
+                                                      const T* __restrict__ x, T* __restrict__ y,
+                                                      double* __restrict__ partials,
+                                                      const dds_pcg_state* __restrict__ gate) {
+  __shared__ double scratch[kBlock / 64];
+  if (gate != nullptr && gate->done) return;
+  const int lane = threadIdx.x & 63;
+  // contiguous slice ranges per wave, contiguous wave ranges per XCD (grid % 8 == 0)
+  const int64_t wave = xcd_block_id() * (kBlock / 64) + (threadIdx.x >> 6);
+  const int64_t n_waves = (int64_t)gridDim.x * (kBlock / 64);
+  const int64_t per = (n_slices + n_waves - 1) / n_waves;
+  const int64_t q0 = wave * per;
+  const int64_t q1 = q0 + per < n_slices ? q0 + per : n_slices;
+  double dot = 0.0;
+  for (int64_t q = q0; q < q1; ++q) {
+    const int64_t s0 = slice_ptr[q];
+    const int w = (int)((slice_ptr[q + 1] - s0) >> 6);
+    const T* v = val + s0 + lane;
+    const int32_t* c = col + s0 + lane;
+    double acc = 0.0;
+    int j = 0;
+    for (; j + 7 < w; j += 8) {
+      int32_t cc[8];
+      T vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cc[u] = load_nt(c + 64 * (j + u));
+#pragma unroll
+      for (int u = 0; u < 8; ++u) vv[u] = load_nt(v + 64 * (j + u));
+      T xx[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xx[u] = x[cc[u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += static_cast<double>(vv[u]) * static_cast<double>(xx[u]);
+    }
+    for (; j < w; ++j)
+      acc += static_cast<double>(load_nt(v + 64 * j)) * static_cast<double>(x[load_nt(c + 64 * j)]);
+    const int64_t row = q * 64 + lane;
+    if (row < n_rows) {
+      y[row] = static_cast<T>(acc);
+      if (partials != nullptr) dot += acc * static_cast<double>(x[row]);
     }
   }
   if (partials != nullptr) write_partial(dot, partials, scratch);
@@ -249,4 +305,37 @@ extern "C" int dds_bsrmv(int val_dtype, int idx_dtype, int64_t n_block_rows, int
                                       dot_partials, n_partials, gate, s);
   set_error("dds_bsrmv: unsupported dtypes (values %d, crow %d)", val_dtype, idx_dtype);
   return DDS_EUNSUPPORTED;
+}
+
+extern "C" int dds_sellmv(int val_dtype, int64_t n_rows, const int64_t* slice_ptr,
+                          const int32_t* col, const void* values, const void* x, void* y,
+                          double* dot_partials, int* n_partials, const dds_pcg_state* gate,
+                          void* stream) {
+  DDS_REQUIRE(n_rows >= 0, "dds_sellmv: negative size");
+  if (n_partials) *n_partials = 0;
+  if (n_rows == 0) return DDS_OK;
+  DDS_REQUIRE(slice_ptr && col && values && x && y, "dds_sellmv: null pointer");
+  const int64_t n_slices = ceil_div(n_rows, 64);
+  // one slice per wave when there are few, contiguous runs of slices when there are many;
+  // a multiple of the XCD count (so every XCD gets an equal contiguous share), at most the
+  // partial-sum capacity
+  const int64_t waves_per_block = kBlock / 64;
+  const int64_t grid = std::min<int64_t>(
+      kMaxPartials, ceil_div(ceil_div(n_slices, waves_per_block), kXcds) * kXcds);
+  hipStream_t s = as_stream(stream);
+  if (val_dtype == DDS_F64) {
+    hipLaunchKernelGGL(sell_kernel<double>, dim3((unsigned)grid), dim3(kBlock), 0, s, n_rows,
+                       n_slices, slice_ptr, col, static_cast<const double*>(values),
+                       static_cast<const double*>(x), static_cast<double*>(y), dot_partials, gate);
+  } else if (val_dtype == DDS_F32) {
+    hipLaunchKernelGGL(sell_kernel<float>, dim3((unsigned)grid), dim3(kBlock), 0, s, n_rows,
+                       n_slices, slice_ptr, col, static_cast<const float*>(values),
+                       static_cast<const float*>(x), static_cast<float*>(y), dot_partials, gate);
+  } else {
+    set_error("dds_sellmv: unsupported dtype %d", val_dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  if (n_partials) *n_partials = dot_partials ? (int)grid : 0;
+  return DDS_OK;
 }

@@ -57,10 +57,12 @@ _SIGNATURES = {
     "dds_gemv_strided_batched": [_i, _i, _p, _p, _p, _i64, _i64, _i, _p],
     "dds_bsrmv": [_i, _i, _i64, _i64, _p, _p, _p, _p, _p, _p, _ip, _p, _p],
     "dds_csrmv": [_i, _i, _i64, _p, _p, _p, _p, _p, _i, _p, _ip, _p, _p],
+    "dds_sellmv": [_i, _i64, _p, _p, _p, _p, _p, _p, _ip, _p, _p],
     "dds_local_vec_width": [_i],
     "dds_local_tile_rows": [_i],
     "dds_local_apply": [_i, _i, _i, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _i, _p, _p],
     "dds_local_apply_sym": [_i, _i, _i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
+    "dds_sym_combine": [_i, _i64, _i64, _p, _p, _p, _p, _p, _p],
     "dds_overlap_sum": [_i, _i64, _p, _p, _p, _p, _p, _d, _p, _p, _p, _ip, _p, _p],
     "dds_restrict": [_i, _i64, _p, _p, _p, _p, _p],
     "dds_chol_factor": [_i, _i64, _p, _i64, _p, _p],
@@ -74,6 +76,7 @@ _SIGNATURES = {
     "dds_pcg_finalize": [_i, _p, _i, _p, _p, _p, _p, _p],
     "dds_sum_partials": [_p, _i, _p, _p],
     "dds_cast": [_i, _i, _i64, _p, _p, _p, _p],
+    "dds_q1_element_grad": [_i64, _p, _p, _p, _p, _p, _p],
 }
 
 _lib.dds_last_error.restype = ctypes.c_char_p

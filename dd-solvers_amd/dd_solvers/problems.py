@@ -26,6 +26,7 @@ __all__ = [
     "Q1_REF_STIFFNESS_BY_DISTANCE",
     "q1_reference_stiffness",
     "assemble_q1_3d",
+    "q1_assembly_operator",
     "assemble_p1_2d",
     "lognormal_coefficients",
     "uniform_rhs",
@@ -106,6 +107,42 @@ def assemble_q1_3d(N: int, kappa: np.ndarray | None = None) -> sps.csr_matrix:
     A = sps.csr_matrix((data, indices, indptr), shape=(m ** 3, m ** 3))
     A.has_sorted_indices = True
     return A
+
+
+def q1_assembly_operator(N: int) -> sps.csr_matrix:
+    """G with A(k).data == G @ k for the CSR pattern of `assemble_q1_3d(N)`: row t of G
+    holds, for stored entry t, the coefficient h * Khat(dist) of every element that
+    contains both of its nodes (k per element, [ez, ey, ex] order)."""
+    m = N - 1
+    h = 1.0 / N
+    offsets = [(dz, dy, dx) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    jz, jy, jx = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    valid = np.empty((m, m, m, 27), dtype=bool)
+    for t, (dz, dy, dx) in enumerate(offsets):
+        valid[..., t] = ((jz + dz >= 0) & (jz + dz < m) & (jy + dy >= 0) & (jy + dy < m)
+                         & (jx + dx >= 0) & (jx + dx < m))
+    entry = np.cumsum(valid.reshape(-1)) - 1
+    entry = entry.reshape(m, m, m, 27)
+    # element offsets (relative to node index j, element e = j + o) touching node j and j + d
+    el = {0: (0, 1), 1: (1,), -1: (0,)}
+    rows, cols, vals = [], [], []
+    for t, (dz, dy, dx) in enumerate(offsets):
+        mask = valid[..., t]
+        ent = entry[..., t][mask]
+        dist = (dz != 0) + (dy != 0) + (dx != 0)
+        coef = h * Q1_REF_STIFFNESS_BY_DISTANCE[dist]
+        for oz in el[dz]:
+            for oy in el[dy]:
+                for ox in el[dx]:
+                    e = ((jz + oz) * N + (jy + oy)) * N + (jx + ox)
+                    rows.append(ent)
+                    cols.append(e[mask])
+                    vals.append(np.full(ent.shape[0], coef))
+    nnz = int(valid.sum())
+    G = sps.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=(nnz, N ** 3))
+    G.sort_indices()
+    return G
 
 
 def q1_element_nodes(N: int) -> np.ndarray:

@@ -250,6 +250,8 @@ class _CSROperator:
         return y
 
     def _enqueue_matvec(self, x, y, partials, gate) -> int:
+        if partials is not None and self.shape[0] != self.shape[1]:
+            raise ValueError("the x . y epilogue needs a square matrix")
         return _lib.call_partials(
             "dds_csrmv", _lib.dtype_code(self.dtype), _lib.dtype_code(self.crow.dtype),
             self.shape[0], self.crow.data_ptr(), self.col.data_ptr(), self.values.data_ptr(),
@@ -261,6 +263,66 @@ class _CSROperator:
         e = self.values.element_size()
         return (self.nnz * (e + 4) + self.crow.element_size() * (self.shape[0] + 1)
                 + e * (self.shape[0] + self.shape[1]))
+
+
+class _SELLOperator:
+    """y = A x in SELL-64 (dds_sellmv), built from a CSR matrix at setup: rows in slices
+    of 64, each slice padded to its longest row and stored column by column, so a
+    wavefront reads every slot of its 64 rows with coalesced loads. CG(bsr_matmul=False)
+    uses it for square matrices."""
+
+    def __init__(self, matrix: torch.Tensor):
+        if matrix.layout is not torch.sparse_csr:
+            matrix = matrix.to_sparse_csr()
+        vals = matrix.values()
+        _lib.require_device(vals)
+        if vals.dtype not in (torch.float64, torch.float32):
+            raise ValueError(f"unsupported CSR dtype {vals.dtype}")
+        self.shape = matrix.shape
+        self.dtype = vals.dtype
+        self.device = dev = vals.device
+        n = self.shape[0]
+        crow = matrix.crow_indices().to(torch.int64)
+        cnt = crow[1:] - crow[:-1]
+        n_slices = -(-n // 64)
+        padded = torch.zeros(n_slices * 64, dtype=torch.int64, device=dev)
+        padded[:n] = cnt
+        width = padded.view(n_slices, 64).max(dim=1).values
+        slice_ptr = torch.zeros(n_slices + 1, dtype=torch.int64, device=dev)
+        slice_ptr[1:] = torch.cumsum(64 * width, 0)
+        total = int(slice_ptr[-1])
+        slot = torch.arange(total, device=dev)
+        q = torch.searchsorted(slice_ptr, slot, right=True) - 1
+        slot_row = q * 64 + (slot - slice_ptr[q]) % 64
+        col = slot_row.clamp(max=n - 1).to(torch.int32)
+        val = torch.zeros(total, dtype=vals.dtype, device=dev)
+        row = csr_rows(crow)
+        t = torch.arange(row.shape[0], device=dev) - crow[row]
+        dest = slice_ptr[row // 64] + 64 * t + row % 64
+        col[dest] = matrix.col_indices().to(torch.int32)
+        val[dest] = vals
+        self.slice_ptr, self.col, self.values = slice_ptr, col, val
+        self.nnz = int(vals.shape[0])
+        self.stored = total
+
+    def __matmul__(self, x):
+        y = torch.empty(self.shape[0], dtype=self.dtype, device=self.device)
+        with torch.cuda.device(self.device):
+            self._enqueue_matvec(x.contiguous(), y, None, None)
+        return y
+
+    def _enqueue_matvec(self, x, y, partials, gate) -> int:
+        return _lib.call_partials(
+            "dds_sellmv", _lib.dtype_code(self.dtype), self.shape[0], self.slice_ptr.data_ptr(),
+            self.col.data_ptr(), self.values.data_ptr(), x.data_ptr(), y.data_ptr(),
+            _lib.ptr(partials), _lib.PARTIALS_OUT, _lib.ptr(gate), _lib.stream())
+
+    @property
+    def bytes_per_matvec(self) -> int:
+        """CSR-equivalent (useful) bytes: values + column indices of the stored entries,
+        row pointers, x and y; SELL padding is not credited."""
+        e = self.values.element_size()
+        return self.nnz * (e + 4) + 8 * (self.shape[0] + 1) + e * (self.shape[0] + self.shape[1])
 
 
 class _TorchOperator:
@@ -322,11 +384,13 @@ class CG(SparseSolver):
       check_every  iterations enqueued between two reads of the device-side state.
       use_graph    capture one block of `check_every` iterations in a hipGraph once and
                    replay it (single-RHS fast path only).
+      spmv_format  "sell" (default) or "csr": the operator format for bsr_matmul=False.
     """
 
     def __init__(self, preconditioner: SparseSolver | None = None, rtol: float = 1e-7,
                  atol: float = 0, maxiter: int | None = 1000, bsr_matmul: bool = True,
-                 estimate_cond: bool = False, check_every: int = 8, use_graph: bool = False):
+                 estimate_cond: bool = False, check_every: int = 8, use_graph: bool = False,
+                 spmv_format: str = "sell"):
         self.preconditioner = preconditioner
         self.rtol = rtol
         self.atol = atol
@@ -335,6 +399,9 @@ class CG(SparseSolver):
         self.estimate_cond = estimate_cond
         self.check_every = max(1, int(check_every))
         self.use_graph = use_graph
+        if spmv_format not in ("sell", "csr"):
+            raise ValueError(f"unknown spmv_format {spmv_format!r}")
+        self.spmv_format = spmv_format
         self.matrix = None
         self._buf = None
         self._graph = None
@@ -354,7 +421,7 @@ class CG(SparseSolver):
             self.matrix = FastBSR(csr_to_bsr(matrix, block_size=block_size))
         elif matrix.layout is torch.sparse_csr and matrix.values().is_cuda and \
                 matrix.values().dtype in (torch.float32, torch.float64):
-            self.matrix = _CSROperator(matrix)
+            self.matrix = _SELLOperator(matrix) if self.spmv_format == "sell" else _CSROperator(matrix)
         else:
             self.matrix = _TorchOperator(matrix)
         self._buf = None
@@ -363,7 +430,7 @@ class CG(SparseSolver):
 
     # -- helpers ----------------------------------------------------------------------
     def _fast(self) -> bool:
-        ok_op = isinstance(self.matrix, (FastBSR, _CSROperator)) and \
+        ok_op = isinstance(self.matrix, (FastBSR, _CSROperator, _SELLOperator)) and \
             self.matrix.dtype == torch.float64
         ok_pc = self.preconditioner is None or getattr(self.preconditioner, "_device_apply", False)
         return ok_op and ok_pc
@@ -712,29 +779,16 @@ def _sym_index(k: int, V: int) -> np.ndarray:
     return idx
 
 
-def _sym_plan(sizes: np.ndarray, sub_ptr: np.ndarray, sub_idx, n: int):
-    """Work items, partial-vector offsets and the per-unknown lists that add the partial
-    vectors (in subdomain, then item order) for the symmetric layout."""
+def _sym_plan(sizes: np.ndarray):
+    """Work items (subdomain, tile-row pair) and the offsets of their partial vectors for
+    the symmetric layout: subdomain s has P_s = ceil(T_s / 2) items, item p covering tile
+    rows p and T_s-1-p and writing a 64*T_s-long partial vector."""
     T = -(-sizes // 64)
     P = -(-T // 2)
     ypos = np.concatenate([[0], np.cumsum(P * 64 * T)]).astype(np.int64)
     items = np.concatenate([np.stack([np.full(int(P[s]), s), np.arange(int(P[s]))], 1)
                             for s in range(sizes.shape[0])]).astype(np.int32)
-    js, ps = [], []
-    for s in range(sizes.shape[0]):
-        k = int(sizes[s])
-        a = np.arange(k)
-        g = sub_idx[sub_ptr[s]:sub_ptr[s + 1]] if sub_idx is not None else sub_ptr[s] + a
-        blk = a // 64
-        for p in range(int(P[s])):
-            m = blk <= T[s] - 1 - p
-            js.append(g[m])
-            ps.append(ypos[s] + p * 64 * T[s] + a[m])
-    j = np.concatenate(js)
-    pos = np.concatenate(ps)
-    order = np.argsort(j, kind="stable")
-    dof_ptr = np.concatenate([[0], np.cumsum(np.bincount(j, minlength=n))]).astype(np.int64)
-    return items, ypos, dof_ptr, pos[order].astype(np.int32)
+    return items, ypos
 
 
 def _tiles_for(sizes: np.ndarray, rows: int) -> np.ndarray:
@@ -1002,11 +1056,9 @@ class SchwarzOperator(SparseSolver):
             del blk, inv
         esz = mats.element_size()
         if layout == "sym":
-            items, ypos, dptr, dpos = _sym_plan(sizes, self._host["sub_ptr"], sub_idx, n)
+            items, ypos = _sym_plan(sizes)
             tiles = np.zeros((0, 2), dtype=np.int32)
-            self.Y = torch.empty(int(ypos[-1]), dtype=pi, device=dev)
-            self.dof_ptr = torch.as_tensor(dptr, device=dev)
-            self.dof_pos = torch.as_tensor(dpos, device=dev)
+            self.Yp = torch.empty(int(ypos[-1]), dtype=pi, device=dev)
             alg = int((sizes.astype(np.int64) * (sizes + 1) // 2).sum()) * esz
         else:
             tiles = _tiles_for(sizes, rows_per_tile)
@@ -1037,11 +1089,15 @@ class SchwarzOperator(SparseSolver):
             return
         L = self.local
         if L.layout == "sym":
+            s = _lib.stream()
             _lib.call("dds_local_apply_sym", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
                       _lib.dtype_code(L.acc_dtype), L.n_items, L.items.data_ptr(),
                       self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), L.mat_off.data_ptr(),
                       L.ypos.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(), x.data_ptr(),
-                      Y.data_ptr(), _lib.ptr(gate), _lib.stream())
+                      self.Yp.data_ptr(), _lib.ptr(gate), s)
+            _lib.call("dds_sym_combine", _lib.dtype_code(self.pi), self.n_solvers, L.max_k,
+                      self.sub_ptr.data_ptr(), L.ypos.data_ptr(), self.Yp.data_ptr(),
+                      Y.data_ptr(), _lib.ptr(gate), s)
             return
         _lib.call("dds_local_apply", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
                   _lib.dtype_code(L.acc_dtype), L.n_tiles, L.tiles.data_ptr(),
@@ -1121,8 +1177,10 @@ class SchwarzOperator(SparseSolver):
         b = self.local.bytes_algorithmic + e * 2 * nk
         if self.sub_idx is not None:
             b += 4 * nk  # gather indices
-        if self.dof_pos is not None:  # staged partials: written, read back with the lists
-            b += 2 * e * int(self.dof_pos.shape[0]) + 4 * int(self.dof_pos.shape[0]) + 8 * (self.n + 1)
+        if getattr(self, "Yp", None) is not None:  # partial vectors: written, read back
+            b += 2 * e * int(self.Yp.shape[0])
+        if self.dof_pos is not None:  # overlapped sum: staged Y written and read with lists
+            b += 2 * e * nk + 4 * nk + 8 * (self.n + 1)
         b += e * 2 * self.n  # r read by the restriction/sum, z written
         b += getattr(self.coarse_solver, "bytes_per_solve", 0)
         return b

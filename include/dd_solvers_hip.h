@@ -13,7 +13,7 @@
  *   dds_gemv_strided_batched  dd_solvers_gemv::gemv_strided_batched
  *                             (src/dd_solvers/cpp/gemv.cpp:9-57, gemv_kernels.cu:31-153)
  *   dds_bsrmv                 FastBSR RPT-I numba kernel (src/dd_solvers/dd_solvers/bsr.py:40-76)
- *   dds_csrmv                 torch sparse-CSR @ dense in CG(bsr_matmul=False) and
+ *   dds_csrmv, dds_sellmv     torch sparse-CSR @ dense in CG(bsr_matmul=False) and
  *                             HybridSchwarz (src/dd_solvers/dd_solvers/solvers.py:226,235,264,880)
  *   dds_local_apply           Inv.solve -> gemv_strided_batched (solvers.py:355-359),
  *                             generalised to uneven and overlapping subdomains
@@ -107,7 +107,8 @@ int dds_gemv_strided_batched(int mat_dtype, int vec_dtype, const void* mat, cons
  * at values[j*b*b + c*b + r], the cusparse csr2gebsr layout, cusparse.cpp:30, 81-83).
  * val_dtype in {F64, F32}; idx_dtype in {I32, I64} is the dtype of crow; col is int32.
  * If dot_partials != NULL the launch also writes per-block partial sums of x . y and
- * stores their count in *n_partials (host int). gate may be NULL. */
+ * stores their count in *n_partials (host int); x is then read at the row indices too, so
+ * the matrix must be square (len(x) >= rows). gate may be NULL. */
 int dds_bsrmv(int val_dtype, int idx_dtype, int64_t n_block_rows, int64_t block_size,
               const void* crow, const int32_t* col, const void* values, const void* x, void* y,
               double* dot_partials, int* n_partials, const dds_pcg_state* gate, void* stream);
@@ -118,6 +119,14 @@ int dds_bsrmv(int val_dtype, int idx_dtype, int64_t n_block_rows, int64_t block_
 int dds_csrmv(int val_dtype, int idx_dtype, int64_t n_rows, const void* row_ptr,
               const int32_t* col, const void* values, const void* x, void* y, int lanes,
               double* dot_partials, int* n_partials, const dds_pcg_state* gate, void* stream);
+
+/* SELL-64 y = A x: rows in slices of 64; entry j of row 64q + l at slice_ptr[q] + 64*j + l
+ * (slice_ptr in elements, multiples of 64), padding entries (col = own row, value 0).
+ * The format CG uses for bsr_matmul=False; built from CSR at setup. Partials and gate as
+ * for dds_bsrmv. */
+int dds_sellmv(int val_dtype, int64_t n_rows, const int64_t* slice_ptr, const int32_t* col,
+               const void* values, const void* x, void* y, double* dot_partials,
+               int* n_partials, const dds_pcg_state* gate, void* stream);
 
 /* ---- local inverses --------------------------------------------------------------------
  * Y[sub_ptr[s] + a] = scale[s] * sum_b Ainv_s[a, b] * x[g_s(b)] for every subdomain s,
@@ -140,7 +149,7 @@ int dds_local_apply(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_tiles
  * csrc/local_sym.hip): each off-diagonal tile is read once and applied twice. One
  * wavefront per item (s, p) = (subdomain, pair of tile rows p and T_s-1-p) writes the
  * partial vector Yp[ypos[s] + p*64*T_s + a] for the first 64*(T_s - p) local rows a; the
- * partials are summed by dds_overlap_sum with lists built for this layout. Replaces the same
+ * partials are summed by dds_sym_combine. Replaces the same
  * Inv.solve -> gemv_strided_batched step (solvers.py:355-359) as dds_local_apply, with half
  * the matrix traffic. dtype rules as for dds_local_apply. */
 int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_items,
@@ -148,6 +157,13 @@ int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_i
                         const int64_t* mat_off, const int64_t* ypos, const float* scale,
                         const void* mats, const void* x, void* Yp, const dds_pcg_state* gate,
                         void* stream);
+
+/* y[sub_ptr[s] + a] = sum of the partial vectors of dds_local_apply_sym that cover local
+ * row a of subdomain s (items p = 0 .. T_s-1-a/64), in item order: the result indexed like
+ * the output of dds_local_apply. One lane per (s, a), coalesced; n_s <= 65535. */
+int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, const int64_t* sub_ptr,
+                    const int64_t* ypos, const void* Yp, void* y, const dds_pcg_state* gate,
+                    void* stream);
 
 /* z[j] = sum_{p in [dof_ptr[j], dof_ptr[j+1])} Y[dof_pos[p]]   (dof_ptr == NULL: z[j] = Y[j])
  *        + coarse_sign * yc[cell[j]]                           (yc != NULL)
@@ -199,6 +215,14 @@ int dds_pcg_finalize(int op, const double* partials, int count, dds_pcg_state* s
                      double* norms, double* alphas, double* betas, void* stream);
 /* Plain sum of `count` partials into *out (device double), for dot products outside PCG. */
 int dds_sum_partials(const double* partials, int count, double* out, void* stream);
+
+/* ---- differentiation -------------------------------------------------------------------
+ * grad[e] = -(1/N) exp(theta[e]) lam_e^T Khat u_e for the N^3 Q1 elements of the unit cube:
+ * the coefficient gradient of J(u(theta)) given the adjoint solution lam of A lam = dJ/du.
+ * lam, u hold the (N-1)^3 interior unknowns; map[i] (optional) is the storage position of
+ * natural (x-fastest) interior node i. No counterpart in the reference (it has no autograd). */
+int dds_q1_element_grad(int64_t N, const double* theta, const double* lam, const double* u,
+                        const int32_t* map, double* grad, void* stream);
 
 /* dst = (dst_dtype) src, F64 <-> F32 <-> F64 */
 int dds_cast(int src_dtype, int dst_dtype, int64_t n, const void* src, void* dst,

@@ -189,3 +189,38 @@ def test_local_apply_matches_oracle(storage, overlap, layout, waves, N, box):
     rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
     tol = {torch.float64: 1e-12, torch.float32: 1e-6, torch.float16: 2e-3, torch.bfloat16: 1e-2}[storage]
     assert rel < tol, rel
+
+
+@pytest.mark.parametrize("kind,N", [("q1_3d", 24), ("p1_2d", 64), ("q1_3d", 5)])
+@pytest.mark.parametrize("fmt", ["sell", "csr"])
+def test_spmv_formats_with_dot(kind, N, fmt):
+    pb = P.make_problem(kind, N, box=4, overlap=0, lognormal=True)
+    A = pb.A
+    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr), torch.as_tensor(A.indices),
+                                 torch.as_tensor(A.data), A.shape).to(dev)
+    op = dds.solvers._SELLOperator(At) if fmt == "sell" else dds.solvers._CSROperator(At)
+    x = np.random.default_rng(1).standard_normal(A.shape[0])
+    xt = torch.as_tensor(x, device=dev)
+    y = torch.empty_like(xt)
+    part = torch.zeros(2048, dtype=torch.float64, device=dev)
+    cnt = op._enqueue_matvec(xt, y, part, None)
+    torch.cuda.synchronize()
+    assert np.allclose(y.cpu().numpy(), A @ x, rtol=1e-13, atol=1e-13)
+    assert np.isclose(part[:cnt].sum().item(), x @ (A @ x), rtol=1e-12)
+
+
+def test_csrmv_tall_matrix():
+    """Rows > columns (the Q1 assembly operator): y = G k must not touch k past its end."""
+    N = 16
+    G = P.q1_assembly_operator(N)
+    Gt = torch.sparse_csr_tensor(torch.as_tensor(G.indptr.astype(np.int64)),
+                                 torch.as_tensor(G.indices.astype(np.int32)),
+                                 torch.as_tensor(G.data), G.shape).to(dev)
+    op = dds.solvers._CSROperator(Gt)
+    k = np.random.default_rng(0).random(N ** 3)
+    y = op @ torch.as_tensor(k, device=dev)
+    torch.cuda.synchronize()
+    assert np.allclose(y.cpu().numpy(), G @ k, rtol=1e-14, atol=1e-15)
+    with pytest.raises(ValueError):
+        op._enqueue_matvec(torch.as_tensor(k, device=dev), y,
+                           torch.zeros(2048, dtype=torch.float64, device=dev), None)

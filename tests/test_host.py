@@ -254,9 +254,9 @@ def test_sym_layout_and_item_plan(k, V):
             blk = A[64 * I:64 * I + t.shape[0], 64 * J:64 * J + 64]
             assert np.array_equal(t[:, :blk.shape[1]], blk)
             assert not t[:, blk.shape[1]:].any()
-    # work items (pairs of tile rows) + combine lists reproduce A x
+    # work items (pairs of tile rows) + the structured combine reproduce A x
     sizes = np.array([k])
-    items, ypos, dptr, dpos = _sym_plan(sizes, np.array([0, k]), None, k)
+    items, ypos = _sym_plan(sizes)
     x = rng.standard_normal(k)
     xp = np.zeros(64 * T)
     xp[:k] = x
@@ -286,5 +286,9 @@ def test_sym_layout_and_item_plan(k, V):
             y[64 * I1:64 * I1 + 64] = rows[I1] + hold
         y[64 * I2:64 * I2 + 64] = rows[I2]
         Yp[ypos[s_] + p * 64 * T: ypos[s_] + (p + 1) * 64 * T] = y
-    z = np.array([Yp[dpos[dptr[j]:dptr[j + 1]]].sum() for j in range(k)])
+    # combine (sym_combine_kernel): row a sums items p = 0 .. min(P-1, T-1-a//64)
+    P = -(-T // 2)
+    z = np.array([sum(Yp[ypos[0] + p * 64 * T + a] for p in range(min(P - 1, T - 1 - a // 64) + 1))
+                  for a in range(k)])
+    assert not np.isnan(z).any()
     assert np.allclose(z, A @ x, rtol=1e-12, atol=1e-12)

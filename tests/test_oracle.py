@@ -283,3 +283,43 @@ def test_cho_solve_matches_dense_inverse():
     assert np.allclose(L @ L.T, A, atol=1e-10)
     assert np.allclose(sla.solve_triangular(L.T, sla.solve_triangular(L, x, lower=True)),
                        K.spd_solve(A, x), rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------------------
+# adjoint gradient
+# ---------------------------------------------------------------------------------------
+
+def _c5_inputs(N, seed=0):
+    theta = 0.5 * np.random.default_rng([seed, 1]).standard_normal(N ** 3)
+    theta_obs = 0.5 * np.random.default_rng([seed + 1, 1]).standard_normal(N ** 3)
+    f = P.uniform_rhs((N - 1) ** 3, seed)
+    return theta, theta_obs, f
+
+
+def test_element_gradient_matches_brute_force():
+    from oracle.adjoint import element_gradient
+    N = 4
+    theta, _, f = _c5_inputs(N)
+    kappa = np.exp(theta).reshape(N, N, N)
+    rng = np.random.default_rng(5)
+    lam = rng.standard_normal((N - 1) ** 3)
+    u = rng.standard_normal((N - 1) ** 3)
+    g = element_gradient(N, kappa, lam, u)
+    for e in (0, 7, 21, N ** 3 - 1):
+        d = np.zeros(N ** 3)
+        d[e] = kappa.reshape(-1)[e]  # dA/dtheta_e = k_e * dA/dk_e, A linear in k
+        dA = P.assemble_q1_3d(N, d.reshape(N, N, N)).toarray()
+        assert np.isclose(g[e], -lam @ dA @ u, rtol=1e-12, atol=1e-14)
+
+
+def test_adjoint_gradient_matches_central_differences():
+    from oracle.adjoint import fd_directional, objective_and_gradient
+    N = 8
+    theta, theta_obs, f = _c5_inputs(N)
+    import scipy.sparse.linalg as spla
+    u_obs = spla.spsolve(P.assemble_q1_3d(N, np.exp(theta_obs).reshape(N, N, N)).tocsc(), f)
+    J, g, u, lam = objective_and_gradient(N, theta, f, u_obs)
+    dirs = np.random.default_rng(7).standard_normal((32, N ** 3))
+    fd = fd_directional(N, theta, f, u_obs, dirs, eps=1e-6)
+    ad = dirs @ g
+    assert np.all(np.abs(fd - ad) <= 1e-6 * np.abs(ad).max())
