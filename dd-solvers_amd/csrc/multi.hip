@@ -1,0 +1,692 @@
+// Multi-right-hand-side kernels: blocks of 16 systems solved together.
+//
+// Vectors are (n, 16) row-major FP64 (the 16 values of one unknown are contiguous, so a
+// gather of one unknown is one 128-byte access). Each column has its own dds_pcg_state;
+// update kernels skip converged columns, and every gated kernel returns at once when all
+// 16 are done. Reductions write per-block partials laid out [block][16].
+//
+// The local-inverse apply is a batched GEMM Y_s = A_s^-1 X_s (k_s x k_s times k_s x 16) on
+// MFMA: A_s^-1 in FP16 or BF16, X_s split into 2 (FP16) or 3 (BF16) low-precision terms
+// after per-(subdomain, column) scaling, FP32 accumulation, result scaled back to FP64.
+#include "common.hpp"
+
+namespace dds {
+namespace {
+
+constexpr int M = 16;  // right-hand sides per block
+constexpr int kBlock = 256;
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bool all_done(const dds_pcg_state* st) {
+  if (st == nullptr) return false;
+  bool done = true;
+#pragma unroll
+  for (int j = 0; j < M; ++j) done = done && st[j].done != 0;
+  return done;
+}
+
+// Sum of M values per lane over the block into partials[blockIdx.x * M + j].
+__device__ __forceinline__ void write_partials_m(const double (&v)[M], double* partials,
+                                                 double* scratch /* (kBlock/64) * M */) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  double r[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) r[j] = wave_sum(v[j]);
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) scratch[w * M + j] = r[j];
+  }
+  __syncthreads();
+  if (threadIdx.x < M) {
+    double s = 0.0;
+    for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) s += scratch[ww * M + threadIdx.x];
+    partials[(int64_t)blockIdx.x * M + threadIdx.x] = s;
+  }
+}
+
+int64_t vgrid(int64_t rows) {
+  return std::min<int64_t>(ceil_div(std::max<int64_t>(rows, 1), kBlock), kMaxPartials);
+}
+
+// ---------------------------------------------------------------------------------------
+// PCG vector kernels (one thread per row, 16 columns)
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void residual_m_kernel(int64_t n, const double* __restrict__ b,
+                                                            const double* __restrict__ q,
+                                                            double* __restrict__ r,
+                                                            double* __restrict__ partials) {
+  __shared__ double scratch[(kBlock / 64) * M];
+  double acc[M] = {};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const double v = b[i * M + j] - q[i * M + j];
+      r[i * M + j] = v;
+      acc[j] += v * v;
+    }
+  }
+  write_partials_m(acc, partials, scratch);
+}
+
+__global__ __launch_bounds__(kBlock) void update_p_m_kernel(int64_t n, const double* __restrict__ z,
+                                                            double* __restrict__ p,
+                                                            const dds_pcg_state* __restrict__ st) {
+  if (all_done(st)) return;
+  bool active[M], first[M];
+  double beta[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    active[j] = st[j].done == 0;
+    first[j] = st[j].it == 0;
+    beta[j] = st[j].beta;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      if (active[j]) p[i * M + j] = first[j] ? z[i * M + j] : p[i * M + j] * beta[j] + z[i * M + j];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void update_xr_m_kernel(int64_t n, const double* __restrict__ p,
+                                                             const double* __restrict__ q,
+                                                             double* __restrict__ x,
+                                                             double* __restrict__ r,
+                                                             double* __restrict__ partials,
+                                                             const dds_pcg_state* __restrict__ st) {
+  __shared__ double scratch[(kBlock / 64) * M];
+  if (all_done(st)) return;
+  bool active[M];
+  double alpha[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    active[j] = st[j].done == 0;
+    alpha[j] = st[j].alpha;
+  }
+  double acc[M] = {};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      double v = r[i * M + j];
+      if (active[j]) {
+        x[i * M + j] += alpha[j] * p[i * M + j];
+        v -= alpha[j] * q[i * M + j];
+        r[i * M + j] = v;
+      }
+      acc[j] += v * v;
+    }
+  }
+  write_partials_m(acc, partials, scratch);
+}
+
+__global__ __launch_bounds__(kBlock) void dot_m_kernel(int64_t n, const double* __restrict__ a,
+                                                       const double* __restrict__ b,
+                                                       double* __restrict__ partials,
+                                                       const dds_pcg_state* __restrict__ st) {
+  __shared__ double scratch[(kBlock / 64) * M];
+  if (all_done(st)) return;
+  double acc[M] = {};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) acc[j] += a[i * M + j] * b[i * M + j];
+  }
+  write_partials_m(acc, partials, scratch);
+}
+
+// One wavefront per column: fixed-order sum of partials[i][j], then the scalar update.
+__global__ __launch_bounds__(M * 64) void finalize_m_kernel(int op, const double* __restrict__ partials,
+                                                            int count, dds_pcg_state* st,
+                                                            double* norms, double* alphas,
+                                                            double* betas) {
+  const int j = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  dds_pcg_state* s = st + j;
+  if (s->done) return;
+  double acc = 0.0;
+  for (int i = lane; i < count; i += 64) acc += partials[(int64_t)i * M + j];
+  const double sum = wave_sum(acc);
+  if (lane != 0) return;
+  auto check = [&](double rr) {
+    s->rr = rr;
+    const double res = sqrt(rr);
+    norms[(int64_t)s->it * M + j] = res;
+    if (res < s->atol) s->done = 1;
+  };
+  switch (op) {
+    case DDS_FIN_CHECK: check(sum); break;
+    case DDS_FIN_RHO:
+      s->rho = sum;
+      if (s->it > 0) s->beta = sum / s->rho_prev;
+      break;
+    case DDS_FIN_ALPHA:
+      s->pq = sum;
+      s->alpha = s->rho / sum;
+      alphas[(int64_t)s->it * M + j] = s->alpha;
+      betas[(int64_t)s->it * M + j] = s->it > 0 ? s->beta : 0.0;
+      break;
+    case DDS_FIN_NEXT:
+      s->rho_prev = s->rho;
+      s->it += 1;
+      if (s->it >= s->maxiter) s->done = 2;
+      else check(sum);
+      break;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Operators
+// ---------------------------------------------------------------------------------------
+
+// SELL-64 times 16 vectors; optional per-column partials of x . y.
+__global__ __launch_bounds__(kBlock) void sell_m_kernel(int64_t n_rows, int64_t n_slices,
+                                                        const int64_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const double* __restrict__ val,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ y,
+                                                        double* __restrict__ partials,
+                                                        const dds_pcg_state* __restrict__ st) {
+  __shared__ double scratch[(kBlock / 64) * M];
+  if (all_done(st)) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t n_waves = (int64_t)gridDim.x * (kBlock / 64);
+  double dot[M] = {};
+  for (int64_t q = wave; q < n_slices; q += n_waves) {
+    const int64_t s0 = slice_ptr[q];
+    const int w = (int)((slice_ptr[q + 1] - s0) >> 6);
+    double acc[M] = {};
+    for (int j = 0; j < w; ++j) {
+      const double v = load_nt(val + s0 + 64 * j + lane);
+      const int64_t c = load_nt(col + s0 + 64 * j + lane);
+      const double* xc = x + c * M;
+#pragma unroll
+      for (int u = 0; u < M; u += 2) {
+        const double2 xv = *reinterpret_cast<const double2*>(xc + u);
+        acc[u] += v * xv.x;
+        acc[u + 1] += v * xv.y;
+      }
+    }
+    const int64_t row = q * 64 + lane;
+    if (row < n_rows) {
+#pragma unroll
+      for (int u = 0; u < M; ++u) {
+        y[row * M + u] = acc[u];
+        if (partials != nullptr) dot[u] += acc[u] * x[row * M + u];
+      }
+    }
+  }
+  if (partials != nullptr) write_partials_m(dot, partials, scratch);
+}
+
+// CSR (long rows, e.g. R0A) times 16 vectors: one wavefront per row.
+__global__ __launch_bounds__(kBlock) void csr_rows_m_kernel(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                            const int32_t* __restrict__ col,
+                                                            const double* __restrict__ val,
+                                                            const double* __restrict__ x,
+                                                            double* __restrict__ y,
+                                                            const dds_pcg_state* __restrict__ st) {
+  if (all_done(st)) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (row >= n_rows) return;
+  double acc[M] = {};
+  for (int64_t p = row_ptr[row] + lane; p < row_ptr[row + 1]; p += 64) {
+    const double v = val[p];
+    const double* xc = x + (int64_t)col[p] * M;
+#pragma unroll
+    for (int u = 0; u < M; ++u) acc[u] += v * xc[u];
+  }
+#pragma unroll
+  for (int u = 0; u < M; ++u) acc[u] = wave_sum(acc[u]);
+  if (lane < M) {
+    double out = 0.0;
+#pragma unroll
+    for (int u = 0; u < M; ++u) out = lane == u ? acc[u] : out;
+    y[row * M + lane] = out;
+  }
+}
+
+// Y = S X for a dense symmetric n x n S: one wavefront per row.
+__global__ __launch_bounds__(kBlock) void dense_symm_m_kernel(int64_t n, const double* __restrict__ S,
+                                                              const double* __restrict__ x,
+                                                              double* __restrict__ y,
+                                                              const dds_pcg_state* __restrict__ st) {
+  if (all_done(st)) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (row >= n) return;
+  double acc[M] = {};
+  for (int64_t c = lane; c < n; c += 64) {
+    const double s = S[row * n + c];
+#pragma unroll
+    for (int u = 0; u < M; ++u) acc[u] += s * x[c * M + u];
+  }
+#pragma unroll
+  for (int u = 0; u < M; ++u) acc[u] = wave_sum(acc[u]);
+  if (lane < M) {
+    double out = 0.0;
+#pragma unroll
+    for (int u = 0; u < M; ++u) out = lane == u ? acc[u] : out;
+    y[row * M + lane] = out;
+  }
+}
+
+// xc[c][:] = sum of x[j][:] over the unknowns j of coarse cell c (one block per cell).
+__global__ __launch_bounds__(kBlock) void restrict_m_kernel(const int64_t* __restrict__ coarse_ptr,
+                                                            const double* __restrict__ x,
+                                                            double* __restrict__ xc,
+                                                            const dds_pcg_state* __restrict__ st) {
+  __shared__ double scratch[(kBlock / 64) * M];
+  if (all_done(st)) return;
+  const int64_t s = coarse_ptr[blockIdx.x];
+  const int64_t e = coarse_ptr[blockIdx.x + 1];
+  double acc[M] = {};
+  for (int64_t j = s + threadIdx.x; j < e; j += kBlock) {
+#pragma unroll
+    for (int u = 0; u < M; ++u) acc[u] += x[j * M + u];
+  }
+  write_partials_m(acc, xc, scratch);  // block c writes xc[c * M + u]
+}
+
+// z[j][:] = sum_p Y[dof_pos[p]][:] (or Y[j][:]) + sign * yc[cell[j]][:], with per-column
+// partials of r . z.
+__global__ __launch_bounds__(kBlock) void overlap_sum_m_kernel(
+    int64_t n, const int64_t* __restrict__ dof_ptr, const int32_t* __restrict__ dof_pos,
+    const double* Y, const int32_t* __restrict__ cell, const double* __restrict__ yc, double sign,
+    double* z, const double* __restrict__ r, double* __restrict__ partials,
+    const dds_pcg_state* __restrict__ st) {
+  __shared__ double scratch[(kBlock / 64) * M];
+  if (all_done(st)) return;
+  double acc[M] = {};
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+    double v[M];
+    if (dof_ptr != nullptr) {
+#pragma unroll
+      for (int u = 0; u < M; ++u) v[u] = 0.0;
+      for (int64_t p = dof_ptr[j]; p < dof_ptr[j + 1]; ++p) {
+        const double* yp = Y + (int64_t)dof_pos[p] * M;
+#pragma unroll
+        for (int u = 0; u < M; ++u) v[u] += yp[u];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < M; ++u) v[u] = Y[j * M + u];
+    }
+    if (yc != nullptr) {
+      const double* c = yc + (int64_t)cell[j] * M;
+#pragma unroll
+      for (int u = 0; u < M; ++u) v[u] += sign * c[u];
+    }
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+      z[j * M + u] = v[u];
+      if (r != nullptr) acc[u] += r[j * M + u] * v[u];
+    }
+  }
+  if (partials != nullptr) write_partials_m(acc, partials, scratch);
+}
+
+// ---------------------------------------------------------------------------------------
+// MFMA local apply
+// ---------------------------------------------------------------------------------------
+// A store of subdomain s (k = k_s, RB = ceil(k/16) row blocks, KP = ceil(k/32) column-block
+// pairs): for row block R and pair K2 the 64 lanes' 8 elements are at
+//   a_off[s] + ((R * KP + K2) * 64 + l) * 8,
+// lane l holding A[16R + l%16][32K2 + 4(l/16) + i] (i < 4) then the same 16 columns on
+// (the A operands of two 16x16x16 MFMAs). Padding is zero.
+// X pack of subdomain s: for pair K2 and term t (< NT) at b_off[s] + ((K2 * NT + t) * 64 + l) * 8,
+// lane l holding term t of X[32K2 + 4(l/16) + i][l%16] / sigma[s][l%16] (i < 4), then the
+// same 16 rows on (the B operands).
+
+template <typename TM>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<_Float16> {
+  static constexpr int NT = 2;  // hi, lo
+  typedef _Float16 elem;
+  __device__ static __forceinline__ elem cvt(float v) { return static_cast<_Float16>(v); }
+  __device__ static __forceinline__ float back(elem v) { return static_cast<float>(v); }
+};
+template <>
+struct MfmaTraits<bf16_t> {
+  static constexpr int NT = 3;  // hi, mid, lo
+  typedef bf16_t elem;
+  __device__ static __forceinline__ elem cvt(float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return bf16_t{static_cast<uint16_t>(u >> 16)};
+  }
+  __device__ static __forceinline__ float back(elem v) { return widen(v); }
+};
+
+// One workgroup per subdomain: gather X_s (k x 16) into LDS-free registers row by row,
+// column maxima by a block reduction, then the split terms in B-operand order.
+template <typename TM>
+__global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restrict__ sub_ptr,
+                                                        const int32_t* __restrict__ sub_idx,
+                                                        const int64_t* __restrict__ b_off,
+                                                        const double* __restrict__ x,
+                                                        TM* __restrict__ bp,
+                                                        float* __restrict__ sigma,
+                                                        const dds_pcg_state* __restrict__ st) {
+  using Tr = MfmaTraits<TM>;
+  __shared__ float smax[(kBlock / 64) * M];
+  __shared__ float sig[M];
+  if (all_done(st)) return;
+  const int s = blockIdx.x;
+  const int64_t base = sub_ptr[s];
+  const int k = (int)(sub_ptr[s + 1] - base);
+  auto row_of = [&](int a) -> int64_t { return sub_idx != nullptr ? (int64_t)sub_idx[base + a] : base + a; };
+  float mx[M] = {};
+  for (int a = threadIdx.x; a < k; a += kBlock) {
+    const double* xr = x + row_of(a) * M;
+#pragma unroll
+    for (int u = 0; u < M; ++u) mx[u] = fmaxf(mx[u], fabsf((float)xr[u]));
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int u = 0; u < M; ++u) {
+    float v = mx[u];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    if (lane == 0) smax[w * M + u] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < M) {
+    float v = 0.f;
+    for (int ww = 0; ww < kBlock / 64; ++ww) v = fmaxf(v, smax[ww * M + threadIdx.x]);
+    v = v > 0.f ? v : 1.f;
+    sig[threadIdx.x] = v;
+    sigma[(int64_t)s * M + threadIdx.x] = v;
+  }
+  __syncthreads();
+  // element (row a, column c): pair K2 = a/32, within pair t2 = (a%32)/16, lane l with
+  // l%16 = c and 4(l/16) + i = a%16
+  const int KP = (k + 31) / 32;
+  const int64_t total = (int64_t)KP * 32 * M;
+  for (int64_t e = threadIdx.x; e < total; e += kBlock) {
+    const int a = (int)(e / M);
+    const int c = (int)(e % M);
+    float v = 0.f;
+    double vd = 0.0;
+    if (a < k) {
+      vd = x[row_of(a) * M + c] / (double)sig[c];
+      v = (float)vd;
+    }
+    const int K2 = a / 32, t2 = (a % 32) / 16, r16 = a % 16;
+    const int l = (r16 / 4) * 16 + c, i = r16 % 4;
+    double rem = vd;
+#pragma unroll
+    for (int t = 0; t < Tr::NT; ++t) {
+      const typename Tr::elem q = Tr::cvt((float)rem);
+      rem -= (double)Tr::back(q);
+      bp[b_off[s] + ((int64_t)(K2 * Tr::NT + t) * 64 + l) * 8 + t2 * 4 + i] = q;
+    }
+    (void)v;
+  }
+}
+
+template <typename TM>
+__device__ __forceinline__ float4_t mfma16(const TM* a, const TM* b, float4_t c);
+template <>
+__device__ __forceinline__ float4_t mfma16<_Float16>(const _Float16* a, const _Float16* b, float4_t c) {
+  half4_t av, bv;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    av[i] = a[i];
+    bv[i] = b[i];
+  }
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ float4_t mfma16<bf16_t>(const bf16_t* a, const bf16_t* b, float4_t c) {
+  bf16x4_t av, bv;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    av[i] = (short)a[i].bits;
+    bv[i] = (short)b[i].bits;
+  }
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, c, 0, 0, 0);
+}
+
+// Persistent: work item i = (s, g) covers row blocks 4g .. 4g+3 of subdomain s; wave w of
+// block b takes items b*W + w, b*W + w + gridDim*W, ...
+template <typename TM>
+__global__ void mfma_apply_kernel(int64_t n_items, const int32_t* __restrict__ items,
+                                  const int64_t* __restrict__ sub_ptr, const int64_t* __restrict__ a_off,
+                                  const int64_t* __restrict__ b_off, const float* __restrict__ scale,
+                                  const float* __restrict__ sigma, const TM* __restrict__ A,
+                                  const TM* __restrict__ bp, double* __restrict__ Y,
+                                  const dds_pcg_state* __restrict__ st) {
+  using Tr = MfmaTraits<TM>;
+  constexpr int NT = Tr::NT;
+  if (all_done(st)) return;
+  const int lane = threadIdx.x & 63;
+  const int W = blockDim.x >> 6;
+  const int64_t stride = (int64_t)gridDim.x * W;
+  for (int64_t it = (int64_t)blockIdx.x * W + (threadIdx.x >> 6); it < n_items; it += stride) {
+    const int s = items[2 * it];
+    const int g = items[2 * it + 1];
+    const int64_t base = sub_ptr[s];
+    const int k = (int)(sub_ptr[s + 1] - base);
+    const int RB = (k + 15) / 16;
+    const int KP = (k + 31) / 32;
+    const int R0 = 4 * g;
+    const int nrb = RB - R0 < 4 ? RB - R0 : 4;
+    float4_t c[4][NT];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) c[r][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const TM* As = A + a_off[s];
+    const TM* Bs = bp + b_off[s];
+    for (int K2 = 0; K2 < KP; ++K2) {
+      Vec16<TM> bv[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r < nrb) {
+          const Vec16<TM> av = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            c[r][t] = mfma16<TM>(av.v, bv[t].v, c[r][t]);
+            c[r][t] = mfma16<TM>(av.v + 4, bv[t].v + 4, c[r][t]);
+          }
+        }
+      }
+    }
+    // D tile: lane l holds rows 4(l/16) + i, column l % 16
+    const int col = lane % 16;
+    const double sg = (double)sigma[(int64_t)s * M + col] * (scale != nullptr ? (double)scale[s] : 1.0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r < nrb) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * (R0 + r) + 4 * (lane / 16) + i;
+          double v = 0.0;
+#pragma unroll
+          for (int t = NT - 1; t >= 0; --t) v += (double)c[r][t][i];
+          if (row < k) Y[(base + row) * M + col] = v * sg;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+}  // namespace dds
+
+using namespace dds;
+
+#define DDS_GATE_M(st) const dds_pcg_state* gate_ = (st)
+
+extern "C" int dds_pcg_residual_m(int64_t n, const double* b, const double* q, double* r,
+                                  double* partials, int* n_partials, void* stream) {
+  DDS_REQUIRE(n >= 0 && b && q && r && partials, "dds_pcg_residual_m: bad arguments");
+  const int64_t g = vgrid(n);
+  hipLaunchKernelGGL(residual_m_kernel, dim3((unsigned)g), dim3(kBlock), 0, as_stream(stream), n, b,
+                     q, r, partials);
+  DDS_LAUNCHED();
+  if (n_partials) *n_partials = (int)g;
+  return DDS_OK;
+}
+
+extern "C" int dds_pcg_update_p_m(int64_t n, const double* z, double* p,
+                                  const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n >= 0 && z && p && st, "dds_pcg_update_p_m: bad arguments");
+  hipLaunchKernelGGL(update_p_m_kernel, dim3((unsigned)vgrid(n)), dim3(kBlock), 0,
+                     as_stream(stream), n, z, p, st);
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_pcg_update_xr_m(int64_t n, const double* p, const double* q, double* x,
+                                   double* r, double* partials, int* n_partials,
+                                   const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n >= 0 && p && q && x && r && partials && st, "dds_pcg_update_xr_m: bad arguments");
+  const int64_t g = vgrid(n);
+  hipLaunchKernelGGL(update_xr_m_kernel, dim3((unsigned)g), dim3(kBlock), 0, as_stream(stream), n,
+                     p, q, x, r, partials, st);
+  DDS_LAUNCHED();
+  if (n_partials) *n_partials = (int)g;
+  return DDS_OK;
+}
+
+extern "C" int dds_dot_m(int64_t n, const double* a, const double* b, double* partials,
+                         int* n_partials, const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n >= 0 && a && b && partials, "dds_dot_m: bad arguments");
+  const int64_t g = vgrid(n);
+  hipLaunchKernelGGL(dot_m_kernel, dim3((unsigned)g), dim3(kBlock), 0, as_stream(stream), n, a, b,
+                     partials, st);
+  DDS_LAUNCHED();
+  if (n_partials) *n_partials = (int)g;
+  return DDS_OK;
+}
+
+extern "C" int dds_pcg_finalize_m(int op, const double* partials, int count, dds_pcg_state* st,
+                                  double* norms, double* alphas, double* betas, void* stream) {
+  DDS_REQUIRE(op >= DDS_FIN_CHECK && op <= DDS_FIN_NEXT, "dds_pcg_finalize_m: bad op %d", op);
+  DDS_REQUIRE(count >= 0 && count <= kMaxPartials, "dds_pcg_finalize_m: bad count %d", count);
+  DDS_REQUIRE(st && norms && alphas && betas && (partials || count == 0),
+              "dds_pcg_finalize_m: null pointer");
+  hipLaunchKernelGGL(finalize_m_kernel, dim3(1), dim3(M * 64), 0, as_stream(stream), op, partials,
+                     count, st, norms, alphas, betas);
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_sellmm(int64_t n_rows, const int64_t* slice_ptr, const int32_t* col,
+                          const double* values, const double* x, double* y, double* partials,
+                          int* n_partials, const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n_rows >= 0 && slice_ptr && col && values && x && y, "dds_sellmm: bad arguments");
+  const int64_t n_slices = ceil_div(n_rows, 64);
+  const int64_t grid = std::min<int64_t>(kMaxPartials, ceil_div(n_slices, kBlock / 64));
+  hipLaunchKernelGGL(sell_m_kernel, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(kBlock), 0,
+                     as_stream(stream), n_rows, n_slices, slice_ptr, col, values, x, y, partials, st);
+  DDS_LAUNCHED();
+  if (n_partials) *n_partials = partials ? (int)std::max<int64_t>(grid, 1) : 0;
+  return DDS_OK;
+}
+
+extern "C" int dds_csrmm_rows(int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                              const double* values, const double* x, double* y,
+                              const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n_rows >= 0 && row_ptr && col && values && x && y, "dds_csrmm_rows: bad arguments");
+  if (n_rows == 0) return DDS_OK;
+  hipLaunchKernelGGL(csr_rows_m_kernel, dim3((unsigned)ceil_div(n_rows, kBlock / 64)), dim3(kBlock),
+                     0, as_stream(stream), n_rows, row_ptr, col, values, x, y, st);
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_dense_symm_m(int64_t n, const double* S, const double* x, double* y,
+                                const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n >= 0 && S && x && y, "dds_dense_symm_m: bad arguments");
+  if (n == 0) return DDS_OK;
+  hipLaunchKernelGGL(dense_symm_m_kernel, dim3((unsigned)ceil_div(n, kBlock / 64)), dim3(kBlock), 0,
+                     as_stream(stream), n, S, x, y, st);
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_restrict_m(int64_t n_c, const int64_t* coarse_ptr, const double* x, double* xc,
+                              const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n_c >= 0 && n_c <= INT32_MAX && coarse_ptr && x && xc, "dds_restrict_m: bad arguments");
+  if (n_c == 0) return DDS_OK;
+  hipLaunchKernelGGL(restrict_m_kernel, dim3((unsigned)n_c), dim3(kBlock), 0, as_stream(stream),
+                     coarse_ptr, x, xc, st);
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_overlap_sum_m(int64_t n, const int64_t* dof_ptr, const int32_t* dof_pos,
+                                 const double* Y, const int32_t* cell, const double* yc,
+                                 double sign, double* z, const double* r, double* partials,
+                                 int* n_partials, const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n >= 0 && Y && z, "dds_overlap_sum_m: bad arguments");
+  DDS_REQUIRE(!dof_ptr || dof_pos, "dds_overlap_sum_m: dof_ptr without dof_pos");
+  DDS_REQUIRE(!yc || cell, "dds_overlap_sum_m: yc without cell");
+  DDS_REQUIRE(!partials || r, "dds_overlap_sum_m: partials need r");
+  const int64_t g = vgrid(n);
+  hipLaunchKernelGGL(overlap_sum_m_kernel, dim3((unsigned)g), dim3(kBlock), 0, as_stream(stream), n,
+                     dof_ptr, dof_pos, Y, cell, yc, sign, z, r, partials, st);
+  DDS_LAUNCHED();
+  if (n_partials) *n_partials = partials ? (int)g : 0;
+  return DDS_OK;
+}
+
+extern "C" int dds_mfma_pack_x(int mat_dtype, int64_t n_s, const int64_t* sub_ptr,
+                               const int32_t* sub_idx, const int64_t* b_off, const double* x,
+                               void* bp, float* sigma, const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n_s >= 0 && n_s <= INT32_MAX && sub_ptr && b_off && x && bp && sigma,
+              "dds_mfma_pack_x: bad arguments");
+  if (n_s == 0) return DDS_OK;
+  hipStream_t s = as_stream(stream);
+  if (mat_dtype == DDS_F16) {
+    hipLaunchKernelGGL(pack_x_kernel<_Float16>, dim3((unsigned)n_s), dim3(kBlock), 0, s, sub_ptr,
+                       sub_idx, b_off, x, static_cast<_Float16*>(bp), sigma, st);
+  } else if (mat_dtype == DDS_BF16) {
+    hipLaunchKernelGGL(pack_x_kernel<bf16_t>, dim3((unsigned)n_s), dim3(kBlock), 0, s, sub_ptr,
+                       sub_idx, b_off, x, static_cast<bf16_t*>(bp), sigma, st);
+  } else {
+    set_error("dds_mfma_pack_x: storage must be F16 or BF16 (got %d)", mat_dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_mfma_apply(int mat_dtype, int64_t n_items, const int32_t* items,
+                              const int64_t* sub_ptr, const int64_t* a_off, const int64_t* b_off,
+                              const float* scale, const float* sigma, const void* A,
+                              const void* bp, double* Y, int grid, int waves,
+                              const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n_items >= 0 && items && sub_ptr && a_off && b_off && sigma && A && bp && Y,
+              "dds_mfma_apply: bad arguments");
+  DDS_REQUIRE(waves == 1 || waves == 2 || waves == 4 || waves == 8,
+              "dds_mfma_apply: waves must be 1, 2, 4 or 8 (got %d)", waves);
+  if (n_items == 0) return DDS_OK;
+  const int64_t full = ceil_div(n_items, waves);
+  const int64_t g = grid > 0 ? std::min<int64_t>(grid, full) : full;
+  hipStream_t s = as_stream(stream);
+  if (mat_dtype == DDS_F16) {
+    hipLaunchKernelGGL(mfma_apply_kernel<_Float16>, dim3((unsigned)g), dim3(64 * waves), 0, s,
+                       n_items, items, sub_ptr, a_off, b_off, scale, sigma,
+                       static_cast<const _Float16*>(A), static_cast<const _Float16*>(bp), Y, st);
+  } else if (mat_dtype == DDS_BF16) {
+    hipLaunchKernelGGL(mfma_apply_kernel<bf16_t>, dim3((unsigned)g), dim3(64 * waves), 0, s,
+                       n_items, items, sub_ptr, a_off, b_off, scale, sigma,
+                       static_cast<const bf16_t*>(A), static_cast<const bf16_t*>(bp), Y, st);
+  } else {
+    set_error("dds_mfma_apply: storage must be F16 or BF16 (got %d)", mat_dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}

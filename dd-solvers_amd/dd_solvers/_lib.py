@@ -26,6 +26,7 @@ F64, F32, F16, BF16, I32, I64 = 0, 1, 2, 3, 4, 5
 # finalize ops
 FIN_CHECK, FIN_RHO, FIN_ALPHA, FIN_NEXT = 0, 1, 2, 3
 MAX_PARTIALS = 2048
+NRHS = 16  # right-hand sides per block in the multi-RHS kernels
 ABI_VERSION = 1
 
 _TORCH_TO_DDS = {
@@ -77,6 +78,18 @@ _SIGNATURES = {
     "dds_sum_partials": [_p, _i, _p, _p],
     "dds_cast": [_i, _i, _i64, _p, _p, _p, _p],
     "dds_q1_element_grad": [_i64, _p, _p, _p, _p, _p, _p],
+    "dds_pcg_residual_m": [_i64, _p, _p, _p, _p, _ip, _p],
+    "dds_pcg_update_p_m": [_i64, _p, _p, _p, _p],
+    "dds_pcg_update_xr_m": [_i64, _p, _p, _p, _p, _p, _ip, _p, _p],
+    "dds_dot_m": [_i64, _p, _p, _p, _ip, _p, _p],
+    "dds_pcg_finalize_m": [_i, _p, _i, _p, _p, _p, _p, _p],
+    "dds_sellmm": [_i64, _p, _p, _p, _p, _p, _p, _ip, _p, _p],
+    "dds_csrmm_rows": [_i64, _p, _p, _p, _p, _p, _p, _p],
+    "dds_dense_symm_m": [_i64, _p, _p, _p, _p, _p],
+    "dds_restrict_m": [_i64, _p, _p, _p, _p, _p],
+    "dds_overlap_sum_m": [_i64, _p, _p, _p, _p, _p, _d, _p, _p, _p, _ip, _p, _p],
+    "dds_mfma_pack_x": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
+    "dds_mfma_apply": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p],
 }
 
 _lib.dds_last_error.restype = ctypes.c_char_p

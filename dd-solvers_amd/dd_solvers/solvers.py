@@ -175,12 +175,10 @@ class DenseCholesky(SparseSolver):
             if mode == "trsv" and n > 6144:
                 mode = "inverse"
             self.active_mode = mode
-            if mode == "inverse":
-                self.Ainv = torch.empty_like(self.L)
-                _lib.call("dds_chol_inverse", _lib.dtype_code(self.dtype), n, self.L.data_ptr(),
-                          n, self.Ainv.data_ptr(), n, _lib.stream())
-            else:
-                self.Ainv = None
+            # the explicit inverse serves "inverse" single solves and every block solve
+            self.Ainv = torch.empty_like(self.L)
+            _lib.call("dds_chol_inverse", _lib.dtype_code(self.dtype), n, self.L.data_ptr(),
+                      n, self.Ainv.data_ptr(), n, _lib.stream())
 
     def solve(self, rhs: torch.Tensor) -> Tuple[torch.Tensor, Metadata]:
         if rhs.dtype != self.dtype:
@@ -192,9 +190,16 @@ class DenseCholesky(SparseSolver):
             self._enqueue_solve(rhs, y, None)
         return y, {}
 
+    def _enqueue_solve_multi(self, x, y, st) -> None:
+        """Y = A^-1 X for 16 columns (FP64)."""
+        if self.dtype != torch.float64:
+            raise ValueError("block coarse solves run in float64")
+        _lib.call("dds_dense_symm_m", self.n, self.Ainv.data_ptr(), x.data_ptr(), y.data_ptr(),
+                  _lib.ptr(st), _lib.stream())
+
     def _enqueue_solve(self, x, y, gate) -> None:
         dt = _lib.dtype_code(self.dtype)
-        if self.Ainv is not None:
+        if self.active_mode == "inverse":
             _lib.call("dds_dense_symv", dt, self.n, self.Ainv.data_ptr(), self.n, x.data_ptr(),
                       y.data_ptr(), _lib.ptr(gate), _lib.stream())
         else:
@@ -204,7 +209,7 @@ class DenseCholesky(SparseSolver):
     @property
     def bytes_per_solve(self) -> int:
         e = 8 if self.dtype == torch.float64 else 4
-        if self.Ainv is not None:
+        if self.active_mode == "inverse":
             return e * self.n * self.n + 2 * e * self.n
         return e * self.n * (self.n + 1) + 4 * e * self.n  # two sweeps of the triangle
 
@@ -316,6 +321,15 @@ class _SELLOperator:
             "dds_sellmv", _lib.dtype_code(self.dtype), self.shape[0], self.slice_ptr.data_ptr(),
             self.col.data_ptr(), self.values.data_ptr(), x.data_ptr(), y.data_ptr(),
             _lib.ptr(partials), _lib.PARTIALS_OUT, _lib.ptr(gate), _lib.stream())
+
+    def _enqueue_matmat(self, X, Y, partials, st) -> int:
+        """Y = A X for (n, 16) blocks; with `partials` also per-column X . Y sums."""
+        if self.dtype != torch.float64:
+            raise ValueError("block products run in float64")
+        return _lib.call_partials(
+            "dds_sellmm", self.shape[0], self.slice_ptr.data_ptr(), self.col.data_ptr(),
+            self.values.data_ptr(), X.data_ptr(), Y.data_ptr(), _lib.ptr(partials),
+            _lib.PARTIALS_OUT, _lib.ptr(st), _lib.stream())
 
     @property
     def bytes_per_matvec(self) -> int:
@@ -491,13 +505,152 @@ class CG(SparseSolver):
 
     # -- solve ------------------------------------------------------------------------
     def solve(self, rhs: torch.Tensor, x0=None) -> Tuple[torch.Tensor, Metadata]:
+        """Solve A x = rhs. A 2-D rhs (n, m) is a block of m independent systems solved
+        together (each column with its own scalars and stopping test); the metadata then
+        holds per-column lists."""
         if self.matrix is None:
             raise RuntimeError("CG.solve called before setup")
         _lib.require_device(rhs)
         with torch.cuda.device(rhs.device):
+            pc = self.preconditioner
+            block_pc = pc is not None and getattr(pc, "_block_apply", False)
+            if rhs.dim() == 2 or (block_pc and rhs.dim() == 1 and rhs.dtype == torch.float64):
+                if not (self._fast_multi() and rhs.dtype == torch.float64):
+                    raise ValueError("blocks of right-hand sides need float64 data, a SELL or CSR "
+                                     "operator (bsr_matmul=False) and Inv(..., layout='mfma')")
+                if rhs.dim() == 1:
+                    x, meta = self._solve_block(rhs[:, None], None if x0 is None else x0[:, None])
+                    meta1 = {k: (v[0] if isinstance(v, list) else v) for k, v in meta.items()}
+                    if x0 is not None:
+                        x0.copy_(x[:, 0])
+                        return x0, meta1
+                    return x[:, 0].clone(), meta1
+                return self._solve_block(rhs, x0)
             if self._fast() and rhs.dtype == torch.float64 and rhs.dim() == 1:
                 return self._solve_device(rhs.contiguous(), x0)
             return self._solve_generic(rhs, x0)
+
+    # -- blocks of 16 right-hand sides ------------------------------------------------
+    def _fast_multi(self) -> bool:
+        pc = self.preconditioner
+        return isinstance(self.matrix, _SELLOperator) and (pc is None or getattr(pc, "_block_apply", False))
+
+    def _solve_block(self, rhs, x0):
+        n, m = rhs.shape
+        M = _lib.NRHS
+        xs, metas = [], []
+        for c0 in range(0, m, M):
+            w = min(M, m - c0)
+            xb, mb = self._solve_block16(rhs[:, c0:c0 + w], None if x0 is None else x0[:, c0:c0 + w], w)
+            xs.append(xb)
+            metas.append(mb)
+        x = torch.cat(xs, dim=1) if len(xs) > 1 else xs[0]
+        meta = {k: sum((mm[k] for mm in metas), []) for k in metas[0]}
+        if x0 is not None:
+            x0.copy_(x)
+            x = x0
+        if any(len(norms) == self._block_maxiter for norms in meta["residual norms"]):
+            raise ConvergenceNotAchieved(meta)
+        return x, meta
+
+    def _solve_block16(self, rhs, x0, w):
+        n = rhs.shape[0]
+        M = _lib.NRHS
+        dev = rhs.device
+        maxiter = self.maxiter or n * 10
+        self._block_maxiter = maxiter
+        key = (n, maxiter)
+        buf = getattr(self, "_bbuf", None)
+        if buf is None or buf["key"] != key:
+            z = lambda: torch.zeros((n, M), dtype=torch.float64, device=dev)  # noqa: E731
+            buf = self._bbuf = dict(key=key, x=z(), r=z(), zz=z(), p=z(), q=z(), b=z(),
+                                    partials=torch.zeros(_lib.MAX_PARTIALS * M, dtype=torch.float64, device=dev),
+                                    log=torch.zeros((3, maxiter + 1, M), dtype=torch.float64, device=dev),
+                                    state=torch.zeros(M * 72, dtype=torch.uint8, device=dev))
+            self._bgraph = None
+        B, X = buf["b"], buf["x"]
+        B.zero_()
+        B[:, :w] = rhs
+        norms_b = torch.linalg.norm(B, dim=0).cpu().numpy()  # one host read, as solvers.py:229
+        st = np.zeros(M, dtype=_STATE_DTYPE)
+        st["atol"] = np.maximum(self.atol, self.rtol * norms_b)
+        st["maxiter"] = maxiter
+        st["done"][w:] = 3  # padding columns never run
+        buf["state"].copy_(torch.from_numpy(st.view(np.uint8).copy()).to(dev))
+        X.zero_()
+        if x0 is not None:
+            X[:, :w] = x0
+        pc = self.preconditioner
+        if pc is not None and hasattr(pc, "T0_multi"):
+            X += pc.T0_multi(B)
+        s = _lib.stream()
+        self.matrix._enqueue_matmat(X, buf["q"], None, None)
+        cnt = _lib.call_partials("dds_pcg_residual_m", n, B.data_ptr(), buf["q"].data_ptr(),
+                                 buf["r"].data_ptr(), buf["partials"].data_ptr(), _lib.PARTIALS_OUT, s)
+        self._fin_m(_lib.FIN_CHECK, cnt, buf)
+        done_view = buf["state"].view(M, 72)[:, 60:64]
+        enqueued = 0
+        while enqueued < maxiter:
+            if self.use_graph:
+                if self._bgraph is None:
+                    self._bgraph = self._capture_fn(lambda: [self._block_iteration(buf) for _ in range(self.check_every)])
+                self._bgraph.replay()
+                enqueued += self.check_every
+            else:
+                k = min(self.check_every, maxiter - enqueued)
+                for _ in range(k):
+                    self._block_iteration(buf)
+                enqueued += k
+            if bool((done_view.contiguous().view(torch.int32) != 0).all().item()):
+                break
+        stv = buf["state"].cpu().numpy().view(_STATE_DTYPE)
+        log = buf["log"].cpu().numpy()
+        meta = {"iterations": [], "residual norms": [], "preconditioner metadata": []}
+        for j in range(w):
+            it, done = int(stv["it"][j]), int(stv["done"][j])
+            nn = it + 1 if done == 1 else it
+            meta["iterations"].append(nn - 1)
+            meta["residual norms"].append([float(v) for v in log[0, :nn, j]])
+            meta["preconditioner metadata"].append([{} for _ in range(max(nn - 1, 0))])
+        return X[:, :w].clone(), meta
+
+    def _fin_m(self, op, count, buf):
+        lp = buf["log"].data_ptr()
+        stride = buf["log"].stride(0) * 8
+        _lib.call("dds_pcg_finalize_m", op, buf["partials"].data_ptr(), count,
+                  buf["state"].data_ptr(), lp, lp + stride, lp + 2 * stride, _lib.stream())
+
+    def _block_iteration(self, buf):
+        n = buf["x"].shape[0]
+        s = _lib.stream()
+        st = buf["state"].data_ptr()
+        pc = self.preconditioner
+        if pc is not None:
+            cnt = pc._enqueue_apply_multi(buf["r"], buf["zz"], buf["partials"], buf["state"])
+            z = buf["zz"]
+        else:
+            cnt = _lib.call_partials("dds_dot_m", n, buf["r"].data_ptr(), buf["r"].data_ptr(),
+                                     buf["partials"].data_ptr(), _lib.PARTIALS_OUT, st, s)
+            z = buf["r"]
+        self._fin_m(_lib.FIN_RHO, cnt, buf)
+        _lib.call("dds_pcg_update_p_m", n, z.data_ptr(), buf["p"].data_ptr(), st, s)
+        cnt = self.matrix._enqueue_matmat(buf["p"], buf["q"], buf["partials"], buf["state"])
+        self._fin_m(_lib.FIN_ALPHA, cnt, buf)
+        cnt = _lib.call_partials("dds_pcg_update_xr_m", n, buf["p"].data_ptr(), buf["q"].data_ptr(),
+                                 buf["x"].data_ptr(), buf["r"].data_ptr(), buf["partials"].data_ptr(),
+                                 _lib.PARTIALS_OUT, st, s)
+        self._fin_m(_lib.FIN_NEXT, cnt, buf)
+
+    @staticmethod
+    def _capture_fn(fn):
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(g, stream=side):
+                fn()
+        torch.cuda.current_stream().wait_stream(side)
+        return g
 
     def _solve_device(self, rhs, x0):
         n = rhs.shape[0]
@@ -672,17 +825,25 @@ class Inv(DenseBatchSolver):
                   FP16 storage in range).
       layout      "sym" (default): the lower triangle in 64 x 64 tiles, each off-diagonal
                   tile read once and applied twice (dds_local_apply_sym); "full": the
-                  whole matrix column-major (dds_local_apply).
+                  whole matrix column-major (dds_local_apply); "mfma" (FP16/BF16): the
+                  whole matrix in MFMA operand order, applied to blocks of 16 right-hand
+                  sides as a batched GEMM on the matrix cores (dds_mfma_apply; a single
+                  right-hand side runs as a block of one).
+      autotune    with layout="mfma": time the apply's launch shapes once after setup and
+                  keep the fastest (dd_solvers.tuning.autotune_mfma).
     """
 
     def __init__(self, precision: torch.dtype | None = None, accumulate: torch.dtype | None = None,
-                 scaled: bool = False, layout: str = "sym"):
-        if layout not in ("sym", "full"):
+                 scaled: bool = False, layout: str = "sym", autotune: bool = False):
+        if layout not in ("sym", "full", "mfma"):
             raise ValueError(f"unknown layout {layout!r}")
+        if layout == "mfma" and precision not in (torch.float16, torch.bfloat16):
+            raise ValueError("layout='mfma' stores the inverses in float16 or bfloat16")
         self.precision = precision
         self.accumulate = accumulate
         self.scaled = scaled
         self.layout = layout
+        self.autotune = autotune
 
     def __str__(self):
         return f"Inv({self.precision or ''})"
@@ -789,6 +950,27 @@ def _sym_plan(sizes: np.ndarray):
     items = np.concatenate([np.stack([np.full(int(P[s]), s), np.arange(int(P[s]))], 1)
                             for s in range(sizes.shape[0])]).astype(np.int32)
     return items, ypos
+
+
+_MFMA_INDEX_CACHE: dict = {}
+
+
+def _mfma_index(k: int) -> np.ndarray:
+    """Source index (row * k + col, -1 for padding) of every element of the MFMA A-operand
+    layout of a k x k matrix (csrc/multi.hip): [row block R][column pair K2][lane l][8],
+    lane l holding row 16R + l%16, columns 32K2 + 4(l/16) + i then 32K2 + 16 + 4(l/16) + i."""
+    if k in _MFMA_INDEX_CACHE:
+        return _MFMA_INDEX_CACHE[k]
+    RB, KP = -(-k // 16), -(-k // 32)
+    R = np.arange(RB)[:, None, None, None]
+    K2 = np.arange(KP)[None, :, None, None]
+    lane = np.arange(64)[None, None, :, None]
+    e = np.arange(8)[None, None, None, :]
+    row = 16 * R + lane % 16 + 0 * K2 + 0 * e
+    col = 32 * K2 + 16 * (e // 4) + 4 * (lane // 16) + e % 4 + 0 * R
+    idx = np.where((row < k) & (col < k), row * k + col, -1).reshape(-1).astype(np.int64)
+    _MFMA_INDEX_CACHE[k] = idx
+    return idx
 
 
 def _tiles_for(sizes: np.ndarray, rows: int) -> np.ndarray:
@@ -914,6 +1096,9 @@ class SchwarzOperator(SparseSolver):
             events[3].record()
         self._setup_extra(matrix, rows, col64, val)
         self._cast_buf = None
+        if self._mfma is not None and getattr(self.local_solver, "autotune", False):
+            from .tuning import autotune_mfma
+            self.tuning = autotune_mfma(self)
         if self.collect_timings:
             torch.cuda.synchronize()
             x_c = torch.rand(self.n_coarse, device=dev, dtype=pi)
@@ -1003,7 +1188,15 @@ class SchwarzOperator(SparseSolver):
         width = _lib._lib.dds_local_vec_width(_lib.dtype_code(store_dtype))
         rows_per_tile = _lib._lib.dds_local_tile_rows(_lib.dtype_code(store_dtype))
         layout = getattr(ls, "layout", "full")
-        if layout == "sym":
+        if layout == "mfma":
+            if store_dtype not in (torch.float16, torch.bfloat16):
+                raise ValueError("layout='mfma' needs float16 or bfloat16 storage")
+            mf_idx = {int(k): _mfma_index(int(k)) for k in np.unique(sizes)}
+            ld = sizes.copy()
+            mat_off = np.concatenate([[0], np.cumsum([mf_idx[int(k)].shape[0] for k in sizes])]).astype(np.int64)
+            mf_idx_dev = {}
+            acc = torch.float32
+        elif layout == "sym":
             sym_idx = {int(k): _sym_index(int(k), width) for k in np.unique(sizes)}
             sym_len = np.array([sym_idx[int(k)].shape[0] for k in sizes], dtype=np.int64)
             ld = sizes.copy()
@@ -1041,7 +1234,14 @@ class SchwarzOperator(SparseSolver):
                     sc = m.abs().max().to(torch.float32)
                     scale[i] = sc
                     m = m / sc.to(m.dtype)
-                if layout == "sym":
+                if layout == "mfma":
+                    if k not in mf_idx_dev:
+                        mf_idx_dev[k] = torch.as_tensor(mf_idx[k], device=dev)
+                    src = mf_idx_dev[k]
+                    packed = torch.where(src >= 0, m.reshape(-1)[src.clamp(min=0)],
+                                         torch.zeros((), dtype=pi, device=dev))
+                    mats[int(mat_off[i]):int(mat_off[i + 1])] = packed.to(store_dtype)
+                elif layout == "sym":
                     if k not in sym_idx_dev:
                         sym_idx_dev[k] = torch.as_tensor(sym_idx[k], device=dev)
                     src = sym_idx_dev[k]
@@ -1055,7 +1255,24 @@ class SchwarzOperator(SparseSolver):
                     mats[int(mat_off[i]):int(mat_off[i + 1])] = col_major.reshape(-1).to(store_dtype)
             del blk, inv
         esz = mats.element_size()
-        if layout == "sym":
+        if layout == "mfma":
+            NT = 2 if store_dtype == torch.float16 else 3
+            RB = -(-sizes // 16)
+            KP = -(-sizes // 32)
+            groups = -(-RB // 4)
+            items = np.concatenate([np.stack([np.full(int(groups[s_]), s_), np.arange(int(groups[s_]))], 1)
+                                    for s_ in range(n_s)]).astype(np.int32)
+            b_off = np.concatenate([[0], np.cumsum(KP * NT * 512)]).astype(np.int64)
+            tiles = np.zeros((0, 2), dtype=np.int32)
+            ypos = None
+            self._mfma = dict(
+                b_off=torch.as_tensor(b_off[:-1], device=dev),
+                bp=torch.empty(int(b_off[-1]), dtype=store_dtype, device=dev),
+                sigma=torch.empty((n_s, _lib.NRHS), dtype=torch.float32, device=dev),
+                Ym=torch.empty((int(sizes.sum()), _lib.NRHS), dtype=torch.float64, device=dev),
+                grid=0, waves=4)
+            alg = int((sizes.astype(np.int64) ** 2).sum()) * esz
+        elif layout == "sym":
             items, ypos = _sym_plan(sizes)
             tiles = np.zeros((0, 2), dtype=np.int32)
             self.Yp = torch.empty(int(ypos[-1]), dtype=pi, device=dev)
@@ -1064,6 +1281,8 @@ class SchwarzOperator(SparseSolver):
             tiles = _tiles_for(sizes, rows_per_tile)
             items = ypos = None
             alg = int((sizes.astype(np.int64) ** 2).sum()) * esz
+        if layout != "mfma":
+            self._mfma = None
         self.local = _LocalStore(
             mats=mats, mat_off=torch.as_tensor(mat_off[:-1], device=dev),
             ld=torch.as_tensor(ld.astype(np.int32), device=dev), scale=scale,
@@ -1088,6 +1307,9 @@ class SchwarzOperator(SparseSolver):
             Y[:self.n].copy_(self.local_solver.solve(x[:self.n].reshape(self.n_solvers, k)).reshape(-1))
             return
         L = self.local
+        if L.layout == "mfma":
+            raise RuntimeError("layout='mfma' applies blocks of right-hand sides (2-D); "
+                               "CG routes single vectors through a block of one")
         if L.layout == "sym":
             s = _lib.stream()
             _lib.call("dds_local_apply_sym", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
@@ -1115,6 +1337,10 @@ class SchwarzOperator(SparseSolver):
             _lib.ptr(gate), _lib.stream())
 
     def _local_only(self, x):
+        if self._mfma is not None:
+            X = torch.zeros((self.n, _lib.NRHS), dtype=torch.float64, device=x.device)
+            X[:, 0] = x
+            return self._local_multi_into(X, None)[:, 0].to(x.dtype)
         y = torch.empty_like(x)
         if self.Y is None:
             self._enqueue_local(x, y, None)
@@ -1156,9 +1382,72 @@ class SchwarzOperator(SparseSolver):
         return _lib.call_partials("dds_dot_partials", self.n, r.data_ptr(), z.data_ptr(),
                                   partials.data_ptr(), _lib.PARTIALS_OUT, _lib.ptr(gate), s)
 
+    # -- blocks of 16 right-hand sides ---------------------------------------------------
+    @property
+    def _block_apply(self) -> bool:
+        return self._mfma is not None and self.pi == torch.float64 and \
+            hasattr(self.coarse_solver, "_enqueue_solve_multi")
+
+    def _enqueue_local_multi(self, X, Y, st) -> None:
+        """Y[sub_ptr[s] + a][:] = (A_s^-1 X_s)[a][:] for (n, 16) X (MFMA)."""
+        L, mf = self.local, self._mfma
+        s = _lib.stream()
+        dt = _lib.dtype_code(L.mat_dtype)
+        _lib.call("dds_mfma_pack_x", dt, self.n_solvers, self.sub_ptr.data_ptr(),
+                  _lib.ptr(self.sub_idx), mf["b_off"].data_ptr(), X.data_ptr(),
+                  mf["bp"].data_ptr(), mf["sigma"].data_ptr(), _lib.ptr(st), s)
+        _lib.call("dds_mfma_apply", dt, L.n_items, L.items.data_ptr(), self.sub_ptr.data_ptr(),
+                  L.mat_off.data_ptr(), mf["b_off"].data_ptr(), _lib.ptr(L.scale),
+                  mf["sigma"].data_ptr(), L.mats.data_ptr(), mf["bp"].data_ptr(), Y.data_ptr(),
+                  int(mf["grid"]), int(mf["waves"]), _lib.ptr(st), s)
+
+    def _buffers_multi(self):
+        if getattr(self, "_mbuf", None) is None:
+            M = _lib.NRHS
+            self._mbuf = dict(xc=torch.empty((self.n_coarse, M), dtype=torch.float64, device=self.device),
+                              yc=torch.empty((self.n_coarse, M), dtype=torch.float64, device=self.device),
+                              y=torch.empty((self.n, M), dtype=torch.float64, device=self.device))
+        return self._mbuf
+
+    def _overlap_sum_multi(self, Y, z, yc, sign, r, partials, st) -> int:
+        overl = self.dof_ptr is not None and Y is self._mfma["Ym"]
+        return _lib.call_partials(
+            "dds_overlap_sum_m", self.n, _lib.ptr(self.dof_ptr) if overl else None,
+            _lib.ptr(self.dof_pos) if overl else None, Y.data_ptr(),
+            _lib.ptr(self.cell) if yc is not None else None, _lib.ptr(yc), float(sign),
+            z.data_ptr(), _lib.ptr(r), _lib.ptr(partials), _lib.PARTIALS_OUT, _lib.ptr(st),
+            _lib.stream())
+
+    def _local_multi_into(self, X, st):
+        """Local part for a block: returns the tensor holding it in unknown order."""
+        Ym = self._mfma["Ym"]
+        self._enqueue_local_multi(X, Ym, st)
+        if self.dof_ptr is None:  # non-overlapping: Ym is already in unknown order
+            return Ym
+        y = self._buffers_multi()["y"]
+        self._overlap_sum_multi(Ym, y, None, 0.0, None, None, st)
+        return y
+
+    def _enqueue_apply_multi(self, R, Z, partials, st) -> int:
+        """Z = M R for an (n_pad, 16) block, gated per column; per-column partials of R . Z."""
+        return self._enqueue_multi(R, Z, R, partials, st)
+
     def solve(self, rhs: torch.Tensor) -> Tuple[torch.Tensor, Metadata]:
         _lib.require_device(rhs)
         with torch.cuda.device(rhs.device):
+            if rhs.dim() == 2:
+                if not self._block_apply:
+                    raise ValueError("2-D right-hand sides need Inv(..., layout='mfma')")
+                out = torch.empty_like(rhs, dtype=torch.float64)
+                M = _lib.NRHS
+                for c0 in range(0, rhs.shape[1], M):
+                    blk = torch.zeros((self.n, M), dtype=torch.float64, device=rhs.device)
+                    w = min(M, rhs.shape[1] - c0)
+                    blk[:, :w] = rhs[:, c0:c0 + w]
+                    z = torch.empty_like(blk)
+                    self._enqueue_multi(blk, z, None, None, None)
+                    out[:, c0:c0 + w] = z[:, :w]
+                return out.to(rhs.dtype), {}
             x = rhs.to(self.pi).contiguous()
             y = torch.empty_like(x)
             self._enqueue_pi(x, y, None, None, None)
@@ -1215,6 +1504,20 @@ class AdditiveSchwarz(SchwarzOperator):
         return self._enqueue_overlap_sum(Y, z, yc, 1.0, r, partials, gate)
 
 
+def _additive_multi(self, X, Z, R, partials, st) -> int:
+    mb = self._buffers_multi()
+    s = _lib.stream()
+    Ym = self._mfma["Ym"]
+    self._enqueue_local_multi(X, Ym, st)
+    _lib.call("dds_restrict_m", self.n_coarse, self.coarse_ptr.data_ptr(), X.data_ptr(),
+              mb["xc"].data_ptr(), _lib.ptr(st), s)
+    self.coarse_solver._enqueue_solve_multi(mb["xc"], mb["yc"], st)
+    return self._overlap_sum_multi(Ym, Z, mb["yc"], 1.0, R, partials, st)
+
+
+AdditiveSchwarz._enqueue_multi = _additive_multi
+
+
 class HybridSchwarz(SchwarzOperator):
     """M r = y - R_C^T A_C^-1 (R0A y), y = sum_i R_i^T A_i^-1 R_i r; T0 b = R_C^T A_C^-1 R_C b
     (solvers.py:829-906)."""
@@ -1254,6 +1557,29 @@ class HybridSchwarz(SchwarzOperator):
             y = torch.empty_like(x)
             self._enqueue_overlap_sum(zero, y, yc, 1.0, None, None, None)
         return y.to(rhs.dtype)
+
+    def _enqueue_multi(self, X, Z, R, partials, st) -> int:
+        mb = self._buffers_multi()
+        y = self._local_multi_into(X, st)
+        _lib.call("dds_csrmm_rows", self.n_coarse, self._r0a.crow.data_ptr(),
+                  self._r0a.col.data_ptr(), self._r0a.values.data_ptr(), y.data_ptr(),
+                  mb["xc"].data_ptr(), _lib.ptr(st), _lib.stream())
+        self.coarse_solver._enqueue_solve_multi(mb["xc"], mb["yc"], st)
+        return self._overlap_sum_multi(y, Z, mb["yc"], -1.0, R, partials, st)
+
+    def T0_multi(self, B: torch.Tensor) -> torch.Tensor:
+        """T0 for an (n, 16) block: R_C^T A_C^-1 R_C B."""
+        mb = self._buffers_multi()
+        s = _lib.stream()
+        _lib.call("dds_restrict_m", self.n_coarse, self.coarse_ptr.data_ptr(), B.data_ptr(),
+                  mb["xc"].data_ptr(), None, s)
+        self.coarse_solver._enqueue_solve_multi(mb["xc"], mb["yc"], None)
+        zero = torch.zeros_like(B)
+        out = torch.empty_like(B)
+        _lib.call_partials("dds_overlap_sum_m", self.n, None, None, zero.data_ptr(),
+                           self.cell.data_ptr(), mb["yc"].data_ptr(), 1.0, out.data_ptr(), None,
+                           None, _lib.PARTIALS_OUT, None, s)
+        return out
 
     def _enqueue_pi(self, x, z, r, partials, gate) -> int:
         xc, yc, ybuf = self._buffers()

@@ -224,6 +224,53 @@ int dds_sum_partials(const double* partials, int count, double* out, void* strea
 int dds_q1_element_grad(int64_t N, const double* theta, const double* lam, const double* u,
                         const int32_t* map, double* grad, void* stream);
 
+/* ---- blocks of 16 right-hand sides --------------------------------------------------------
+ * Vectors are (n, 16) row-major FP64; st points to 16 dds_pcg_state (one per column, NULL
+ * for ungated use); partials are laid out [block][16] (count returned in *n_partials).
+ * Update kernels skip converged columns; gated kernels return when all 16 are done.
+ * They replace the same reference steps as their single-vector versions above. */
+int dds_pcg_residual_m(int64_t n, const double* b, const double* q, double* r, double* partials,
+                       int* n_partials, void* stream);
+int dds_pcg_update_p_m(int64_t n, const double* z, double* p, const dds_pcg_state* st,
+                       void* stream);
+int dds_pcg_update_xr_m(int64_t n, const double* p, const double* q, double* x, double* r,
+                        double* partials, int* n_partials, const dds_pcg_state* st, void* stream);
+int dds_dot_m(int64_t n, const double* a, const double* b, double* partials, int* n_partials,
+              const dds_pcg_state* st, void* stream);
+int dds_pcg_finalize_m(int op, const double* partials, int count, dds_pcg_state* st,
+                       double* norms, double* alphas, double* betas, void* stream);
+/* SELL-64 (layout of dds_sellmv) times 16 vectors; optional x . y partials (square only). */
+int dds_sellmm(int64_t n_rows, const int64_t* slice_ptr, const int32_t* col, const double* values,
+               const double* x, double* y, double* partials, int* n_partials,
+               const dds_pcg_state* st, void* stream);
+/* CSR with long rows (R0A) times 16 vectors, one wavefront per row. */
+int dds_csrmm_rows(int64_t n_rows, const int64_t* row_ptr, const int32_t* col,
+                   const double* values, const double* x, double* y, const dds_pcg_state* st,
+                   void* stream);
+/* Y = S X, S dense symmetric n x n row-major (the coarse inverse). */
+int dds_dense_symm_m(int64_t n, const double* S, const double* x, double* y,
+                     const dds_pcg_state* st, void* stream);
+int dds_restrict_m(int64_t n_c, const int64_t* coarse_ptr, const double* x, double* xc,
+                   const dds_pcg_state* st, void* stream);
+int dds_overlap_sum_m(int64_t n, const int64_t* dof_ptr, const int32_t* dof_pos, const double* Y,
+                      const int32_t* cell, const double* yc, double sign, double* z,
+                      const double* r, double* partials, int* n_partials,
+                      const dds_pcg_state* st, void* stream);
+/* Local inverses on MFMA (layouts in csrc/multi.hip): dds_mfma_pack_x scales X_s per
+ * (subdomain, column) by its largest magnitude (sigma, n_s x 16 floats) and splits it into
+ * 2 FP16 or 3 BF16 terms in B-operand order; dds_mfma_apply computes
+ * Y[sub_ptr[s] + a][:] = scale[s] * sigma[s][:] * (A_s^-1 X_s)[a][:] with
+ * v_mfma_f32_16x16x16 (FP32 accumulation), one wavefront per item (s, 64-row group),
+ * `grid` persistent workgroups of `waves` wavefronts (grid 0: one per item group). Replace
+ * Inv.solve -> gemv_strided_batched (solvers.py:355-359) for stacked right-hand sides. */
+int dds_mfma_pack_x(int mat_dtype, int64_t n_s, const int64_t* sub_ptr, const int32_t* sub_idx,
+                    const int64_t* b_off, const double* x, void* bp, float* sigma,
+                    const dds_pcg_state* st, void* stream);
+int dds_mfma_apply(int mat_dtype, int64_t n_items, const int32_t* items, const int64_t* sub_ptr,
+                   const int64_t* a_off, const int64_t* b_off, const float* scale,
+                   const float* sigma, const void* A, const void* bp, double* Y, int grid,
+                   int waves, const dds_pcg_state* st, void* stream);
+
 /* dst = (dst_dtype) src, F64 <-> F32 <-> F64 */
 int dds_cast(int src_dtype, int dst_dtype, int64_t n, const void* src, void* dst,
              const dds_pcg_state* gate, void* stream);

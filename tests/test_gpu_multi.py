@@ -1,0 +1,209 @@
+"""GPU parity of the batched form: blocks of 16 right-hand sides, uneven subdomains, the
+local inverses applied as an FP16/BF16 batched GEMM on the matrix cores.
+
+Bars: one preconditioner application within 1e-5 (norm-wise, per column) of the oracle's
+FP64 product with the same quantised, scaled inverses (the block path splits each column
+into 2 FP16 or 3 BF16 terms and accumulates in FP32); per-column CG iteration counts within
++-2 of the oracle's CG with those inverses, solutions within 1e-6, and every column's true
+FP64 residual below rtol; a column solved alone equals the same column of a block bit for
+bit (columns never mix); graph replay equals eager execution bit for bit; every autotuner
+launch shape gives the same bits.
+"""
+
+import functools
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import torch
+
+from helpers import dg24_permuted
+from oracle.pcg import cg as oracle_cg
+from oracle.schwarz import SchwarzOracle
+
+import dd_solvers as dds
+from dd_solvers import problems as P
+from dd_solvers.tuning import autotune_mfma, mfma_candidates
+
+pytestmark = pytest.mark.gpu
+dev = torch.device("cuda")
+TD = {"float16": torch.float16, "bfloat16": torch.bfloat16}
+
+
+def _torch_csr(A):
+    return torch.sparse_csr_tensor(torch.as_tensor(A.indptr), torch.as_tensor(A.indices),
+                                   torch.as_tensor(A.data), A.shape).to(device=dev, dtype=torch.float64)
+
+
+@functools.lru_cache(maxsize=None)
+def _kmeans(nrhs=16):
+    # jittered P1 on 96^2, 12 k-means subdomains (uneven, ~600-1000 unknowns), overlap 2
+    return P.make_problem("p1_2d_kmeans", 96, overlap=2, seed=5, nrhs=nrhs, n_clusters=12, jitter=0.2)
+
+
+@functools.lru_cache(maxsize=None)
+def _q1_nonoverlap():
+    return P.make_problem("q1_3d", 17, box=4, overlap=0, seed=2, lognormal=True, nrhs=16)
+
+
+def _oracle(pb, kind, storage):
+    d = pb.decomposition
+    return SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind=kind,
+                         storage=storage, scaled=True)
+
+
+def _pc(kind, storage, **kw):
+    cls = dds.AdditiveSchwarz if kind == "additive" else dds.HybridSchwarz
+    return cls(torch.float64, dds.Inv(TD[storage], scaled=True, layout="mfma", **kw), dds.DenseCholesky())
+
+
+def _colrel(a, b):
+    return np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-300)
+
+
+@pytest.mark.parametrize("storage", ["float16", "bfloat16"])
+@pytest.mark.parametrize("kind", ["additive", "hybrid"])
+def test_block_apply_matches_oracle(kind, storage):
+    pb = _kmeans()
+    pc = _pc(kind, storage)
+    pc.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+    R = np.random.default_rng(11).uniform(-1, 1, (pb.A.shape[0], 16))
+    R[:, 3] *= 1e-6  # a column far below the others: per-column scaling keeps its accuracy
+    Z, _ = pc.solve(torch.as_tensor(R, device=dev))
+    O = _oracle(pb, kind, storage)
+    Zo = np.stack([O.apply(R[:, j]) for j in range(16)], axis=1)
+    rel = _colrel(Z.cpu().numpy(), Zo)
+    assert rel.max() <= 1e-5, rel
+
+
+def test_block_apply_small_subdomains_dg24():
+    """Subdomains of 3 unknowns: one 16-row block, one 32-column pair, mostly padding."""
+    A, b, perm, Ap, k, spc = dg24_permuted()
+    n = Ap.shape[0]
+    ptr = np.arange(0, n + 1, k)
+    pc = _pc("hybrid", "float16")
+    pc.setup(_torch_csr(Ap), dofs_per_solver=k, solvers_per_coarse=torch.as_tensor(spc, device=dev))
+    R = np.random.default_rng(1).uniform(-1, 1, (n, 16))
+    Z, _ = pc.solve(torch.as_tensor(R, device=dev))
+    O = SchwarzOracle(Ap, ptr, ptr, np.arange(n), spc, kind="hybrid", storage="float16", scaled=True)
+    Zo = np.stack([O.apply(R[:, j]) for j in range(16)], axis=1)
+    assert _colrel(Z.cpu().numpy(), Zo).max() <= 1e-5
+
+
+def _block_parity(pb, kind, storage, m, rtol=1e-8, **cg_kw):
+    O = _oracle(pb, kind, storage)
+    s = dds.CG(_pc(kind, storage), rtol=rtol, bsr_matmul=False, **cg_kw)
+    s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+    B = np.ascontiguousarray(pb.b[:, :m])
+    X, meta = s.solve(torch.as_tensor(B, device=dev))
+    X = X.cpu().numpy()
+    assert X.shape == (pb.A.shape[0], m)
+    assert len(meta["iterations"]) == m
+    for j in range(m):
+        xo, mo = oracle_cg(lambda v: pb.A @ v, B[:, j], precond=O.apply,
+                           T0=O.T0 if kind == "hybrid" else None, rtol=rtol)
+        assert abs(meta["iterations"][j] - mo["iterations"]) <= 2, (j, meta["iterations"][j], mo["iterations"])
+        err = np.linalg.norm(X[:, j] - xo) / np.linalg.norm(xo)
+        assert err <= 1e-6, (j, err)
+        res = np.linalg.norm(B[:, j] - pb.A @ X[:, j]) / np.linalg.norm(B[:, j])
+        assert res < rtol * 1.01, (j, res)
+        assert len(meta["residual norms"][j]) == meta["iterations"][j] + 1
+    return X, meta, s
+
+
+@pytest.mark.parametrize("storage", ["float16", "bfloat16"])
+def test_block_cg_hybrid_kmeans(storage):
+    # 13 of 16 columns: three padding columns must stay inert
+    _block_parity(_kmeans(), "hybrid", storage, 13)
+
+
+def test_block_cg_additive_two_blocks():
+    # 20 columns: a full block of 16, then a block of 4
+    pb = _kmeans(nrhs=20)
+    _block_parity(pb, "additive", "float16", 20)
+
+
+def test_block_cg_nonoverlapping_boxes():
+    _block_parity(_q1_nonoverlap(), "hybrid", "bfloat16", 16)
+
+
+def test_single_column_equals_block_column():
+    pb = _kmeans()
+    s = dds.CG(_pc("hybrid", "float16"), rtol=1e-8, bsr_matmul=False)
+    s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+    B = torch.as_tensor(np.ascontiguousarray(pb.b), device=dev)
+    X, meta = s.solve(B)
+    x1, m1 = s.solve(B[:, 5].contiguous())
+    assert x1.dim() == 1
+    assert m1["iterations"] == meta["iterations"][5]
+    assert torch.equal(x1, X[:, 5])
+
+
+def test_block_graph_replay_matches_eager():
+    pb = _kmeans()
+    B = torch.as_tensor(np.ascontiguousarray(pb.b), device=dev)
+    out = []
+    for use_graph in (False, True):
+        s = dds.CG(_pc("hybrid", "float16"), rtol=1e-8, bsr_matmul=False, check_every=4,
+                   use_graph=use_graph)
+        s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+        X1, m1 = s.solve(B)
+        X2, m2 = s.solve(B)  # second replay of the same graph
+        assert torch.equal(X1, X2) and m1["iterations"] == m2["iterations"]
+        out.append((X1, m1))
+    assert out[0][1]["iterations"] == out[1][1]["iterations"]
+    assert torch.equal(out[0][0], out[1][0])
+
+
+def test_autotuner_shapes_are_bitwise_equal():
+    pb = _kmeans()
+    pc = _pc("additive", "float16")
+    pc.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+    X = torch.as_tensor(np.random.default_rng(3).uniform(-1, 1, (pb.A.shape[0], 16)), device=dev)
+    mf = pc._mfma
+    n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
+    ref = None
+    for grid, waves in mfma_candidates(pc.local.n_items, n_cu):
+        mf["grid"], mf["waves"] = grid, waves
+        Y = pc._local_multi_into(X, None).clone()
+        if ref is None:
+            ref = Y
+        assert torch.equal(Y, ref), (grid, waves)
+    best = autotune_mfma(pc)
+    assert (best["grid"], best["waves"]) in best["timings"]
+    assert (mf["grid"], mf["waves"]) == (best["grid"], best["waves"])
+    assert torch.equal(pc._local_multi_into(X, None), ref)
+
+
+def test_autotune_flag_in_setup():
+    pb = _kmeans()
+    s = dds.CG(_pc("hybrid", "bfloat16", autotune=True), rtol=1e-8, bsr_matmul=False)
+    s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+    assert s.preconditioner.tuning["waves"] in (1, 2, 4, 8)
+    X, meta = s.solve(torch.as_tensor(np.ascontiguousarray(pb.b[:, :4]), device=dev))
+    A = pb.A
+    for j in range(4):
+        assert np.linalg.norm(pb.b[:, j] - A @ X[:, j].cpu().numpy()) < 1.01e-8 * np.linalg.norm(pb.b[:, j])
+
+
+def test_block_requires_mfma_layout():
+    pb = _kmeans()
+    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(torch.float16), dds.DenseCholesky()),
+               bsr_matmul=False)
+    s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+    with pytest.raises(ValueError):
+        s.solve(torch.as_tensor(np.ascontiguousarray(pb.b[:, :2]), device=dev))
+
+
+def test_block_sellmm_and_dots_match_host():
+    """SpMM through the SELL kernel, with the fused per-column p . q partials."""
+    pb = _kmeans()
+    s = dds.CG(None, bsr_matmul=False)
+    s.setup(_torch_csr(pb.A))
+    n = pb.A.shape[0]
+    X = np.random.default_rng(4).uniform(-1, 1, (n, 16))
+    Xt = torch.as_tensor(X, device=dev)
+    Y = torch.empty_like(Xt)
+    s.matrix._enqueue_matmat(Xt, Y, None, None)
+    ref = sps.csr_matrix(pb.A) @ X
+    assert np.allclose(Y.cpu().numpy(), ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
