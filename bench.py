@@ -6,9 +6,14 @@ unknown (seed 0), Hybrid Schwarz with 512 boxes of 8^3 elements and overlap 1, F
 inverses, dense FP64 Cholesky coarse solve (n_c = 512), FP64 PCG to rtol 1e-8, x0 = 0.
 
 One step = one complete solve (time to solution), inputs already resident in HBM, setup
-excluded (the reference's protocol). Prints one JSON line.
+excluded (the reference's protocol). Prints one JSON line: `value` is the plain FP64
+time to solution; beside it the same solve with FP32/FP16/BF16 local inverses, the
+batched form (16 right-hand sides, FP16/BF16 MFMA local inverses), the uneven-subdomain
+batching workload (jittered 2D P1 512^2, 256 k-means subdomains, overlap 2, 16
+right-hand sides) and the 128^3 comparison of plain FP64 against the mixed and batched
+forms.
 
-    python bench.py [--steps K] [--warmup W] [--N 64] [--no-cpu] [--no-extra]
+    python bench.py [--steps K] [--warmup W] [--N 64] [--no-cpu] [--no-extra] [--no-target]
 """
 
 from __future__ import annotations
@@ -24,6 +29,15 @@ sys.path.insert(0, os.path.join(ROOT, "dd-solvers_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+_TNAME = {}
+
+
+def _tnames():
+    import torch
+    _TNAME.update({torch.float64: "double", torch.float32: "float", torch.float16: "_Float16",
+                   torch.bfloat16: "bf16_t"})
 
 
 def log(msg: str) -> None:
@@ -56,6 +70,158 @@ def make_solver(pb, storage, dev, bsr=None, scaled=False, use_graph=False):
     torch.cuda.synchronize()
     log(f"setup storage={storage} bsr={bsr}: {time.perf_counter() - t:.1f}s")
     return s
+
+
+def make_block_solver(pb, storage, dev, kind="hybrid", use_graph=True, autotune=True):
+    """CG over blocks of 16 right-hand sides, local inverses as an MFMA batched GEMM."""
+    import torch
+    import dd_solvers as dds
+
+    A = torch.sparse_csr_tensor(torch.as_tensor(pb.A.indptr), torch.as_tensor(pb.A.indices),
+                                torch.as_tensor(pb.A.data), pb.A.shape).to(dev)
+    cls = dds.HybridSchwarz if kind == "hybrid" else dds.AdditiveSchwarz
+    s = dds.CG(cls(torch.float64, dds.Inv(storage, scaled=True, layout="mfma", autotune=autotune),
+                   dds.DenseCholesky()),
+               rtol=1e-8, maxiter=2000, bsr_matmul=False, check_every=8, use_graph=use_graph)
+    t = time.perf_counter()
+    s.setup(A, decomposition=pb.decomposition)
+    torch.cuda.synchronize()
+    tn = getattr(s.preconditioner, "tuning", None)
+    log(f"setup block storage={storage}: {time.perf_counter() - t:.1f}s"
+        + (f", tuned grid={tn['grid']} waves={tn['waves']} ({tn['ms'] * 1e3:.1f} us)" if tn else ""))
+    return s
+
+
+def mfma_kernel_stats(M):
+    """Event-timed average of one dds_mfma_apply launch, with its algorithmic bytes
+    (inverses + packed right-hand sides + FP64 result) and executed MFMA flops."""
+    import numpy as np
+    import torch
+    from dd_solvers import _lib
+
+    L, mf = M.local, M._mfma
+    sizes = M._host["sizes"].astype(np.int64)
+    X = torch.rand((M.n, _lib.NRHS), dtype=torch.float64, device=M.device) - 0.5
+    M._enqueue_local_multi(X, mf["Ym"], None)
+    dt = _lib.dtype_code(L.mat_dtype)
+    s = _lib.stream()
+
+    def launch():
+        _lib.call("dds_mfma_apply", dt, L.n_items, L.items.data_ptr(), M.sub_ptr.data_ptr(),
+                  L.mat_off.data_ptr(), mf["b_off"].data_ptr(), _lib.ptr(L.scale),
+                  mf["sigma"].data_ptr(), L.mats.data_ptr(), mf["bp"].data_ptr(),
+                  mf["Ym"].data_ptr(), int(mf["grid"]), int(mf["waves"]), None, s)
+
+    ms = kernel_time_ms(launch)
+    nt = 2 if L.mat_dtype == torch.float16 else 3
+    esz = L.mats.element_size()
+    alg = int(L.bytes_algorithmic + mf["bp"].numel() * esz + sizes.sum() * 16 * 8)
+    flops = float(2 * (sizes ** 2).sum() * 16 * nt)
+    return {"kernel": f"mfma_apply_kernel<{'_Float16' if nt == 2 else 'bf16_t'}>",
+            "avg_launch_us": round(ms * 1e3, 2), "alg_bytes_per_launch": alg,
+            "GBs": round(alg / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "mfma_TFLOPs": round(flops / (ms * 1e-3) / 1e12, 1),
+            "grid": int(mf["grid"]), "waves": int(mf["waves"])}
+
+
+def batched_extra(pb, storage, dev, steps, warmup, seed, kind="hybrid", fp64_tts=None):
+    """Solve 16 right-hand sides together (FP16/BF16 MFMA local inverses, FP64 Krylov);
+    FP64-verified residual of every column."""
+    import numpy as np
+    import torch
+    from dd_solvers import problems as P
+
+    Bh = pb.b if (pb.b.ndim == 2 and pb.b.shape[1] == 16) else \
+        P.uniform_rhs(pb.A.shape[0], seed + 100, 16)[pb.decomposition.perm]
+    Bh = np.ascontiguousarray(Bh)
+    B = torch.as_tensor(Bh, device=dev)
+    s = make_block_solver(pb, storage, dev, kind=kind)
+    X, meta, total, times = time_solves(s, B, steps, warmup)
+    Xh = X.cpu().numpy()
+    res = np.linalg.norm(Bh - pb.A @ Xh, axis=0) / np.linalg.norm(Bh, axis=0)
+    tts = total / steps
+    out = {"tts_s_16rhs": round(tts, 5), "per_rhs_s": round(tts / 16, 6),
+           "iterations_min_max": [int(min(meta["iterations"])), int(max(meta["iterations"]))],
+           "max_true_rel_residual": float(res.max()),
+           "mfma": mfma_kernel_stats(s.preconditioner)}
+    if fp64_tts:
+        out["per_rhs_ratio_vs_fp64"] = round(tts / 16 / fp64_tts, 4)
+    log(f"batched {storage} ({kind}): 16 RHS in {tts * 1e3:.2f} ms, iterations "
+        f"{out['iterations_min_max']}, max residual {res.max():.1e}, mfma {out['mfma']}")
+    del s
+    torch.cuda.empty_cache()
+    return out
+
+
+def uneven_extra(dev, steps, warmup, seed):
+    """Batching on uneven subdomains: jittered 2D P1 on 512^2, 256 k-means subdomains,
+    overlap 2, 16 right-hand sides; beside the FP64 single-right-hand-side solve."""
+    import numpy as np
+    import torch
+    from dd_solvers import problems as P
+
+    t = time.perf_counter()
+    pb = P.make_problem("p1_2d_kmeans", 512, overlap=2, seed=seed, nrhs=16, n_clusters=256, jitter=0.2)
+    sizes = pb.decomposition.sizes
+    log(f"built P1 512^2 k-means: n={pb.A.shape[0]} subdomains={sizes.shape[0]} "
+        f"k in [{sizes.min()}, {sizes.max()}] ({time.perf_counter() - t:.1f}s)")
+    s = make_solver(pb, torch.float64, dev)
+    b0 = torch.as_tensor(np.ascontiguousarray(pb.b[:, 0]), device=dev)
+    _, m0, tot0, _ = time_solves(s, b0, steps, warmup)
+    fp64 = tot0 / steps
+    log(f"uneven FP64 single RHS: {fp64 * 1e3:.2f} ms, {m0['iterations']} iterations")
+    del s
+    torch.cuda.empty_cache()
+    out = {"workload": ("2D P1 512^2 (n=261121), nodes jittered U(-0.2h, 0.2h), 256 k-means "
+                        "subdomains overlap 2, Hybrid Schwarz, 16 right-hand sides, rtol 1e-8"),
+           "subdomain_sizes_min_max": [int(sizes.min()), int(sizes.max())],
+           "fp64_single_rhs_tts_s": round(fp64, 5), "fp64_iterations": int(m0["iterations"])}
+    for name, st in (("fp16", torch.float16), ("bf16", torch.bfloat16)):
+        out[f"batched_{name}"] = batched_extra(pb, st, dev, steps, warmup, seed, fp64_tts=fp64)
+    return out
+
+
+def target_extra(dev, steps, warmup, seed):
+    """The 128^3 comparison: plain FP64 against FP16 local inverses (one right-hand side)
+    and against the batched FP16 MFMA form (16 right-hand sides), plus SpMV bandwidth."""
+    import numpy as np
+    import torch
+
+    pb = build_problem(128, seed)
+    b = torch.as_tensor(pb.b, device=dev)
+    s = make_solver(pb, torch.float64, dev)
+    x, m, tot, _ = time_solves(s, b, steps, warmup)
+    fp64 = tot / steps
+    xh = x.cpu().numpy()
+    op = s.matrix
+    xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
+    q = torch.empty_like(xv)
+    t_spmv = kernel_time_ms(lambda: op._enqueue_matvec(xv, q, None, None))
+    spmv_gbs = op.bytes_per_matvec / (t_spmv * 1e-3) / 1e9
+    log(f"128^3 FP64 TTS {fp64 * 1e3:.2f} ms, {m['iterations']} it; SpMV {t_spmv * 1e3:.1f} us "
+        f"{spmv_gbs:.0f} GB/s")
+    del s
+    torch.cuda.empty_cache()
+    s = make_solver(pb, torch.float16, dev, scaled=True)
+    x2, m2, tot2, _ = time_solves(s, b, steps, warmup)
+    x2h = x2.cpu().numpy()
+    mixed = {"tts_s": round(tot2 / steps, 5), "iterations": int(m2["iterations"]),
+             "ratio_vs_fp64": round(tot2 / steps / fp64, 4),
+             "rel_err_vs_fp64": float(np.linalg.norm(x2h - xh) / np.linalg.norm(xh)),
+             "true_rel_residual": float(np.linalg.norm(pb.b - pb.A @ x2h) / np.linalg.norm(pb.b))}
+    log(f"128^3 fp16 TTS {tot2 / steps * 1e3:.2f} ms ({mixed['ratio_vs_fp64']}x)")
+    del s
+    torch.cuda.empty_cache()
+    out = {"workload": ("3D Q1 Poisson 128^3 (n=2048383), lognormal k, Hybrid Schwarz 4096 "
+                        "boxes of 8^3 elements overlap 1, rtol 1e-8"),
+           "fp64_tts_s": round(fp64, 5), "fp64_iterations": int(m["iterations"]),
+           "fp64_true_rel_residual": float(np.linalg.norm(pb.b - pb.A @ xh) / np.linalg.norm(pb.b)),
+           "spmv": {"us": round(t_spmv * 1e3, 2), "bytes": int(op.bytes_per_matvec),
+                    "GBs": round(spmv_gbs, 1), "hbm_frac": round(spmv_gbs / HBM_PEAK_GBS, 4)},
+           "mixed_fp16_single_rhs": mixed,
+           "batched_fp16": batched_extra(pb, torch.float16, dev, steps, warmup, seed, fp64_tts=fp64)}
+    return out
 
 
 def time_solves(solver, b, steps, warmup):
@@ -94,90 +260,57 @@ def kernel_time_ms(fn, reps=20):
 
 
 def cpu_baseline(pb, iterations, budget_s=20.0):
-    """The oracle's FP64 PCG components timed on this host on a bounded sample, scaled to
-    one full solve: scipy CSR SpMV (whole matrix), numpy dense local inverse matvecs on a
-    sample of subdomains (scaled by sum k_i^2), scipy cho_solve of the coarse system, and
-    the PCG vector updates."""
+    """The oracle (a numpy/scipy restatement of the reference's FP64 Hybrid Schwarz PCG)
+    run on this host on the full workload: its setup (local inverses, coarse Cholesky) is
+    not timed, as on the GPU; its PCG is timed for as many iterations as fit in about
+    `budget_s` seconds (all of them when they fit), and the time to solution is the
+    per-iteration time times the GPU solve's iteration count plus the start-up work."""
     import numpy as np
-    import scipy.linalg as sla
+
+    from oracle.pcg import ConvergenceNotAchieved, cg
+    from oracle.schwarz import SchwarzOracle
 
     try:
         from threadpoolctl import threadpool_info
         threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
     except Exception:  # pragma: no cover
         threads = 1
-    A = pb.A
     d = pb.decomposition
-    n = A.shape[0]
-    rng = np.random.default_rng(0)
-    x = rng.standard_normal(n)
-
-    def best(f, reps=3):
-        ts = []
-        for _ in range(reps):
-            t = time.perf_counter()
-            f()
-            ts.append(time.perf_counter() - t)
-        return min(ts)
-
-    t_spmv = best(lambda: A @ x)
-    sizes = d.sizes
-    # Local inverse matvecs: a sample of subdomains, with as many distinct matrices as it
-    # takes to exceed the host caches (>= 1 GiB), so the timing streams from DRAM as the
-    # full set of inverses would. A matvec's time does not depend on the values, so the
-    # sample's matrices are random with the sampled subdomains' sizes.
-    sample = rng.choice(d.n_subdomains, size=min(d.n_subdomains, 256), replace=False)
-    ks = sizes[sample].astype(np.int64)
-    need = int(np.searchsorted(np.cumsum(ks ** 2 * 8), 1 << 30)) + 1
-    ks = ks[:max(1, min(need, ks.shape[0]))]
-    mats = [rng.standard_normal((int(k), int(k))) for k in ks]
-    vecs = [rng.standard_normal(int(k)) for k in ks]
+    A = pb.A
     t = time.perf_counter()
-    np.linalg.inv(mats[0])
-    t_inv = time.perf_counter() - t
-
-    def local_pass():
-        for Mi, vi in zip(mats, vecs):
-            Mi @ vi
-
-    t_loc = best(local_pass, reps=2)
-    scale = float((sizes.astype(np.float64) ** 2).sum() / (ks.astype(np.float64) ** 2).sum())
-    t_local = t_loc * scale
-    del mats
-    # coarse: dense SPD of size n_c
-    nc = d.n_subdomains
-    B = rng.standard_normal((nc, nc))
-    AC = B @ B.T + nc * np.eye(nc)
-    cf = sla.cho_factor(AC, lower=True)
-    xc = rng.standard_normal(nc)
-    t_coarse = best(lambda: sla.cho_solve(cf, xc))
-    p = rng.standard_normal(n)
-    r = rng.standard_normal(n)
-
-    def vec():
-        np.dot(r, p)
-        p2 = p * 0.5 + r
-        r2 = r - 0.1 * p2
-        np.linalg.norm(r2)
-        np.dot(p2, r2)
-        return p2
-
-    t_vec = best(vec)
-    # Hybrid: SpMV (A p) + R0A matvec (~ n entries) + local + coarse + vectors per iteration
-    t_iter = t_spmv + t_local + t_coarse + t_vec + t_spmv / 20
-    tts = t_iter * iterations + t_spmv + t_coarse
+    O = SchwarzOracle(A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid")
+    t_setup = time.perf_counter() - t
+    b = np.asarray(pb.b, dtype=np.float64)
+    # start-up (T0 b, first residual) and one iteration's cost, to size the sample
+    t = time.perf_counter()
+    x0 = O.T0(b)
+    r = b - A @ x0
+    t_start = time.perf_counter() - t
+    t = time.perf_counter()
+    z = O.apply(r)
+    A @ z
+    t_one = time.perf_counter() - t
+    k = int(max(1, min(iterations, budget_s // max(t_one, 1e-9))))
+    t = time.perf_counter()
+    try:
+        _, meta = cg(lambda v: A @ v, b, precond=O.apply, T0=O.T0, rtol=1e-8, maxiter=k)
+        done = meta["iterations"]
+    except ConvergenceNotAchieved as e:
+        done = e.metadata["iterations"] + 1
+    t_run = time.perf_counter() - t
+    per_iter = max(t_run - t_start, 0.0) / max(done, 1)
+    tts = t_start + per_iter * iterations
+    del O
     return {
         "value": round(tts, 4),
         "unit": "s",
         "cores": int(threads),
         "kind": "port",
-        "sample": (f"oracle FP64 components on this host: full scipy CSR SpMV ({t_spmv * 1e3:.1f} ms), "
-                   f"dense FP64 inverse matvecs over {len(ks)} matrices of the sampled "
-                   f"subdomain sizes ({(ks ** 2).sum() * 8 / 2 ** 30:.2f} GiB, out of cache) "
-                   f"scaled by sum k_i^2 ({t_local * 1e3:.1f} ms/iter), cho_solve n_c={nc} "
-                   f"({t_coarse * 1e3:.2f} ms), vector ops ({t_vec * 1e3:.1f} ms); per-iteration "
-                   f"sum x {iterations} iterations; setup excluded (one {int(ks[0])}^2 inverse "
-                   f"takes {t_inv * 1e3:.0f} ms); cpu_count={os.cpu_count()}"),
+        "sample": (f"oracle FP64 Hybrid Schwarz PCG (numpy/scipy, {threads} BLAS threads) on the "
+                   f"full workload: {done} of {iterations} iterations timed ({t_run:.1f} s, "
+                   f"{per_iter * 1e3:.1f} ms/iteration), time to solution = start-up + "
+                   f"{iterations} x per-iteration; setup excluded ({t_setup:.1f} s); "
+                   f"cpu_count={os.cpu_count()}"),
     }
 
 
@@ -189,7 +322,8 @@ def main():
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the FP64 line and its kernels")
+    ap.add_argument("--no-target", action="store_true", help="skip the 128^3 comparison")
     ap.add_argument("--graph", action="store_true", help="replay captured iteration blocks")
     ap.add_argument("--profile-kernel", action="store_true",
                     help="only launch the local-apply kernel 20 times, then a 1 GiB device "
@@ -209,6 +343,7 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    _tnames()
     pb = build_problem(args.N, args.seed)
     b = torch.as_tensor(pb.b, device=dev)
 
@@ -251,22 +386,30 @@ def main():
     sizes = M._host["sizes"].astype(np.int64)
     xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
     Y = M.Y if M.Y is not None else torch.empty_like(xv)
-    t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
-    # algorithmic bytes of one launch: the matrices (lower triangle for the symmetric
-    # layout, whole matrices for "full"), the gathered x (value + index) and the result
-    # (sym layout: plus the partial vectors, written by one launch and read by the next)
     yp = getattr(M, "Yp", None)
-    alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + sizes.sum() * 8
-                    + (2 * yp.numel() * 8 if yp is not None else 0))
+    if L.layout == "sym":
+        # sym_apply_kernel alone: reads the packed lower triangles and the gathered x
+        # (value + index), writes the partial vectors; sym_combine_kernel reads those back
+        # and writes the subdomain-ordered result
+        t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None))
+        t_comb = kernel_time_ms(lambda: M._enqueue_sym_combine(Y, None))
+        alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + yp.numel() * 8)
+        comb_bytes = int(yp.numel() * 8 + sizes.sum() * 8)
+        kname = f"sym_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
+    else:
+        t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
+        t_comb, comb_bytes = 0.0, 0
+        alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + sizes.sum() * 8)
+        kname = f"local_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
     gbs = alg_bytes / (t_loc * 1e-3) / 1e9
-    log(f"local apply: {t_loc * 1e3:.1f} us, {alg_bytes / 1e9:.3f} GB -> {gbs:.0f} GB/s "
-        f"({gbs / HBM_PEAK_GBS * 100:.1f}% of HBM peak)")
-    # SpMV alone (CSR), for the record
+    log(f"local apply ({kname}): {t_loc * 1e3:.1f} us, {alg_bytes / 1e9:.3f} GB -> {gbs:.0f} GB/s "
+        f"({gbs / HBM_PEAK_GBS * 100:.1f}% of HBM peak); combine {t_comb * 1e3:.1f} us")
+    # SpMV alone, for the record
     op = solver.matrix
     q = torch.empty_like(xv)
     t_spmv = kernel_time_ms(lambda: op._enqueue_matvec(xv, q, None, None))
     spmv_bytes = op.bytes_per_matvec
-    log(f"SpMV CSR: {t_spmv * 1e3:.1f} us, {spmv_bytes / 1e6:.1f} MB -> "
+    log(f"SpMV ({type(op).__name__}): {t_spmv * 1e3:.1f} us, {spmv_bytes / 1e6:.1f} MB -> "
         f"{spmv_bytes / (t_spmv * 1e-3) / 1e9:.0f} GB/s")
     # whole-solve algorithmic bytes
     per_iter = M.bytes_per_apply + spmv_bytes + 9 * 8 * pb.A.shape[0]
@@ -295,6 +438,21 @@ def main():
                 torch.cuda.empty_cache()
             except Exception as e:  # report, keep the measured line
                 extra[f"mixed_{name}"] = {"error": str(e)[:200]}
+        for name, st in (("fp16", torch.float16), ("bf16", torch.bfloat16)):
+            try:
+                extra[f"batched_{name}"] = batched_extra(pb, st, dev, args.steps, args.warmup,
+                                                         args.seed, fp64_tts=tts)
+            except Exception as e:
+                extra[f"batched_{name}"] = {"error": str(e)[:200]}
+        try:
+            extra["batching_uneven"] = uneven_extra(dev, args.steps, args.warmup, args.seed)
+        except Exception as e:
+            extra["batching_uneven"] = {"error": str(e)[:200]}
+        if not args.no_target and args.N != 128:
+            try:
+                extra["target_128"] = target_extra(dev, min(args.steps, 3), args.warmup, args.seed)
+            except Exception as e:
+                extra["target_128"] = {"error": str(e)[:200]}
 
     cpu = None
     if not args.no_cpu:
@@ -318,7 +476,7 @@ def main():
         "config": {
             "workload": f"3D Q1 Poisson {args.N}^3 (n={pb.A.shape[0]}), lognormal k=exp(N(0,1)), "
                         f"Hybrid Schwarz 8^3-element boxes overlap 1 ({pb.decomposition.n_subdomains} "
-                        "subdomains), FP64 inverses, dense FP64 Cholesky coarse, CSR SpMV, rtol 1e-8",
+                        "subdomains), FP64 inverses, dense FP64 Cholesky coarse, SELL-64 SpMV, rtol 1e-8",
             "n": int(pb.A.shape[0]),
             "nnz": int(pb.A.nnz),
             "iterations": int(iters),
@@ -327,9 +485,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("sym_apply_kernel<double,double,double> + sym_combine_kernel<double>"
-                       if L.layout == "sym"
-                       else "local_apply_kernel<double,double,double,4>"),
+            "kernel": kname,
+            "combine_kernel_us": round(t_comb * 1e3, 2),
+            "combine_alg_bytes": comb_bytes,
             "stored_matrix_bytes": int(L.bytes_matrices),
             "achieved": round(gbs, 1),
             "peak": HBM_PEAK_GBS,

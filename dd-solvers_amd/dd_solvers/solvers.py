@@ -1311,21 +1311,30 @@ class SchwarzOperator(SparseSolver):
             raise RuntimeError("layout='mfma' applies blocks of right-hand sides (2-D); "
                                "CG routes single vectors through a block of one")
         if L.layout == "sym":
-            s = _lib.stream()
-            _lib.call("dds_local_apply_sym", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
-                      _lib.dtype_code(L.acc_dtype), L.n_items, L.items.data_ptr(),
-                      self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), L.mat_off.data_ptr(),
-                      L.ypos.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(), x.data_ptr(),
-                      self.Yp.data_ptr(), _lib.ptr(gate), s)
-            _lib.call("dds_sym_combine", _lib.dtype_code(self.pi), self.n_solvers, L.max_k,
-                      self.sub_ptr.data_ptr(), L.ypos.data_ptr(), self.Yp.data_ptr(),
-                      Y.data_ptr(), _lib.ptr(gate), s)
+            self._enqueue_sym_apply(x, gate)
+            self._enqueue_sym_combine(Y, gate)
             return
         _lib.call("dds_local_apply", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
                   _lib.dtype_code(L.acc_dtype), L.n_tiles, L.tiles.data_ptr(),
                   self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), L.max_k,
                   L.mat_off.data_ptr(), L.ld.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(),
                   x.data_ptr(), Y.data_ptr(), self.waves or 4, _lib.ptr(gate), _lib.stream())
+
+    def _enqueue_sym_apply(self, x, gate) -> None:
+        """Partial products of every (subdomain, tile-row pair) item into Yp."""
+        L = self.local
+        _lib.call("dds_local_apply_sym", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
+                  _lib.dtype_code(L.acc_dtype), L.n_items, L.items.data_ptr(),
+                  self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), L.mat_off.data_ptr(),
+                  L.ypos.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(), x.data_ptr(),
+                  self.Yp.data_ptr(), _lib.ptr(gate), _lib.stream())
+
+    def _enqueue_sym_combine(self, Y, gate) -> None:
+        """Y[sub_ptr[s] + a] = sum of the partial vectors of subdomain s at row a."""
+        L = self.local
+        _lib.call("dds_sym_combine", _lib.dtype_code(self.pi), self.n_solvers, L.max_k,
+                  self.sub_ptr.data_ptr(), L.ypos.data_ptr(), self.Yp.data_ptr(),
+                  Y.data_ptr(), _lib.ptr(gate), _lib.stream())
 
     def _enqueue_overlap_sum(self, Y, z, yc, sign, r, partials, gate) -> int:
         overl = self.dof_ptr is not None and Y is self.Y
