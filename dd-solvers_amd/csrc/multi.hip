@@ -28,66 +28,91 @@ __device__ __forceinline__ bool all_done(const dds_pcg_state* st) {
   return done;
 }
 
-// Sum of M values per lane over the block into partials[blockIdx.x * M + j].
-__device__ __forceinline__ void write_partials_m(const double (&v)[M], double* partials,
-                                                 double* scratch /* (kBlock/64) * M */) {
+// Threads own column pairs: in a grid-stride loop over the (n, 16) array taken as n * 8
+// pairs of doubles, thread t handles pairs t, t + stride, ... (stride a multiple of 8), so
+// its pair is always t % 8 = threadIdx.x % 8 and a wavefront reads or writes 1 KB of
+// contiguous memory per instruction.
+constexpr int kPairs = M / 2;
+
+__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+__device__ __forceinline__ void st2(double* p, double2 v) { *reinterpret_cast<double2*>(p) = v; }
+
+// Block sums per column of the pair each thread owns (pair threadIdx.x % 8) into
+// out[blockIdx.x * M + j]; fixed order, so deterministic.
+__device__ __forceinline__ void write_partials_pairs(double2 v, double* out,
+                                                     double* scratch /* (kBlock/64) * M */) {
+#pragma unroll
+  for (int off = kPairs; off < 64; off <<= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  double r[M];
-#pragma unroll
-  for (int j = 0; j < M; ++j) r[j] = wave_sum(v[j]);
-  if (lane == 0) {
-#pragma unroll
-    for (int j = 0; j < M; ++j) scratch[w * M + j] = r[j];
+  if (lane < kPairs) {
+    scratch[w * M + 2 * lane] = v.x;
+    scratch[w * M + 2 * lane + 1] = v.y;
   }
   __syncthreads();
   if (threadIdx.x < M) {
     double s = 0.0;
     for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) s += scratch[ww * M + threadIdx.x];
-    partials[(int64_t)blockIdx.x * M + threadIdx.x] = s;
+    out[(int64_t)blockIdx.x * M + threadIdx.x] = s;
   }
 }
 
+// Sum over the 8 lanes groups of a wavefront that share a pair (lanes l, l ^ 8, ..., l ^ 56).
+__device__ __forceinline__ double2 wave_pair_sum(double2 v) {
+#pragma unroll
+  for (int off = kPairs; off < 64; off <<= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
+  return v;
+}
+
 int64_t vgrid(int64_t rows) {
-  return std::min<int64_t>(ceil_div(std::max<int64_t>(rows, 1), kBlock), kMaxPartials);
+  return std::min<int64_t>(ceil_div(std::max<int64_t>(rows * kPairs, 1), kBlock), kMaxPartials);
 }
 
 // ---------------------------------------------------------------------------------------
-// PCG vector kernels (one thread per row, 16 columns)
+// PCG vector kernels
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void residual_m_kernel(int64_t n, const double* __restrict__ b,
                                                             const double* __restrict__ q,
                                                             double* __restrict__ r,
                                                             double* __restrict__ partials) {
   __shared__ double scratch[(kBlock / 64) * M];
-  double acc[M] = {};
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-      const double v = b[i * M + j] - q[i * M + j];
-      r[i * M + j] = v;
-      acc[j] += v * v;
-    }
+  double2 acc = {0.0, 0.0};
+  const int64_t total = n * kPairs;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+    const double2 bv = ld2(b + 2 * t), qv = ld2(q + 2 * t);
+    const double2 v = {bv.x - qv.x, bv.y - qv.y};
+    st2(r + 2 * t, v);
+    acc.x += v.x * v.x;
+    acc.y += v.y * v.y;
   }
-  write_partials_m(acc, partials, scratch);
+  write_partials_pairs(acc, partials, scratch);
 }
 
 __global__ __launch_bounds__(kBlock) void update_p_m_kernel(int64_t n, const double* __restrict__ z,
                                                             double* __restrict__ p,
                                                             const dds_pcg_state* __restrict__ st) {
   if (all_done(st)) return;
-  bool active[M], first[M];
-  double beta[M];
-#pragma unroll
-  for (int j = 0; j < M; ++j) {
-    active[j] = st[j].done == 0;
-    first[j] = st[j].it == 0;
-    beta[j] = st[j].beta;
-  }
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-      if (active[j]) p[i * M + j] = first[j] ? z[i * M + j] : p[i * M + j] * beta[j] + z[i * M + j];
+  const int c = 2 * (threadIdx.x % kPairs);
+  const bool a0 = st[c].done == 0, a1 = st[c + 1].done == 0;
+  if (!a0 && !a1) return;
+  const bool f0 = st[c].it == 0, f1 = st[c + 1].it == 0;
+  const double b0 = st[c].beta, b1 = st[c + 1].beta;
+  const bool need_p = !(a0 && a1 && f0 && f1);
+  const int64_t total = n * kPairs;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+    const double2 zv = ld2(z + 2 * t);
+    double2 pv = {0.0, 0.0};
+    if (need_p) pv = ld2(p + 2 * t);
+    double2 o;
+    o.x = a0 ? (f0 ? zv.x : pv.x * b0 + zv.x) : pv.x;
+    o.y = a1 ? (f1 ? zv.y : pv.y * b1 + zv.y) : pv.y;
+    st2(p + 2 * t, o);
   }
 }
 
@@ -99,27 +124,30 @@ __global__ __launch_bounds__(kBlock) void update_xr_m_kernel(int64_t n, const do
                                                              const dds_pcg_state* __restrict__ st) {
   __shared__ double scratch[(kBlock / 64) * M];
   if (all_done(st)) return;
-  bool active[M];
-  double alpha[M];
-#pragma unroll
-  for (int j = 0; j < M; ++j) {
-    active[j] = st[j].done == 0;
-    alpha[j] = st[j].alpha;
-  }
-  double acc[M] = {};
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-      double v = r[i * M + j];
-      if (active[j]) {
-        x[i * M + j] += alpha[j] * p[i * M + j];
-        v -= alpha[j] * q[i * M + j];
-        r[i * M + j] = v;
+  const int c = 2 * (threadIdx.x % kPairs);
+  const bool a0 = st[c].done == 0, a1 = st[c + 1].done == 0;
+  const double al0 = a0 ? st[c].alpha : 0.0, al1 = a1 ? st[c + 1].alpha : 0.0;
+  double2 acc = {0.0, 0.0};
+  if (a0 || a1) {
+    const int64_t total = n * kPairs;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+      const double2 pv = ld2(p + 2 * t), qv = ld2(q + 2 * t);
+      double2 xv = ld2(x + 2 * t), rv = ld2(r + 2 * t);
+      if (a0) {
+        xv.x += al0 * pv.x;
+        rv.x -= al0 * qv.x;
       }
-      acc[j] += v * v;
+      if (a1) {
+        xv.y += al1 * pv.y;
+        rv.y -= al1 * qv.y;
+      }
+      st2(x + 2 * t, xv);
+      st2(r + 2 * t, rv);
+      acc.x += rv.x * rv.x;
+      acc.y += rv.y * rv.y;
     }
   }
-  write_partials_m(acc, partials, scratch);
+  write_partials_pairs(acc, partials, scratch);
 }
 
 __global__ __launch_bounds__(kBlock) void dot_m_kernel(int64_t n, const double* __restrict__ a,
@@ -128,12 +156,14 @@ __global__ __launch_bounds__(kBlock) void dot_m_kernel(int64_t n, const double* 
                                                        const dds_pcg_state* __restrict__ st) {
   __shared__ double scratch[(kBlock / 64) * M];
   if (all_done(st)) return;
-  double acc[M] = {};
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-#pragma unroll
-    for (int j = 0; j < M; ++j) acc[j] += a[i * M + j] * b[i * M + j];
+  double2 acc = {0.0, 0.0};
+  const int64_t total = n * kPairs;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+    const double2 av = ld2(a + 2 * t), bv = ld2(b + 2 * t);
+    acc.x += av.x * bv.x;
+    acc.y += av.y * bv.y;
   }
-  write_partials_m(acc, partials, scratch);
+  write_partials_pairs(acc, partials, scratch);
 }
 
 // One wavefront per column: fixed-order sum of partials[i][j], then the scalar update.
@@ -180,7 +210,9 @@ __global__ __launch_bounds__(M * 64) void finalize_m_kernel(int op, const double
 // Operators
 // ---------------------------------------------------------------------------------------
 
-// SELL-64 times 16 vectors; optional per-column partials of x . y.
+// SELL-64 times 16 vectors; optional per-column partials of x . y. One wavefront per
+// slice: lane l owns column pair l % 8 of the rows 8u + l / 8 (u < 8), so each gather of
+// x reads 8 whole 128-byte rows per instruction.
 __global__ __launch_bounds__(kBlock) void sell_m_kernel(int64_t n_rows, int64_t n_slices,
                                                         const int64_t* __restrict__ slice_ptr,
                                                         const int32_t* __restrict__ col,
@@ -192,37 +224,47 @@ __global__ __launch_bounds__(kBlock) void sell_m_kernel(int64_t n_rows, int64_t 
   __shared__ double scratch[(kBlock / 64) * M];
   if (all_done(st)) return;
   const int lane = threadIdx.x & 63;
+  const int cp = lane & (kPairs - 1);
+  const int rg = lane >> 3;
   const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int64_t n_waves = (int64_t)gridDim.x * (kBlock / 64);
-  double dot[M] = {};
+  double2 dot = {0.0, 0.0};
   for (int64_t q = wave; q < n_slices; q += n_waves) {
     const int64_t s0 = slice_ptr[q];
     const int w = (int)((slice_ptr[q + 1] - s0) >> 6);
-    double acc[M] = {};
-    for (int j = 0; j < w; ++j) {
-      const double v = load_nt(val + s0 + 64 * j + lane);
-      const int64_t c = load_nt(col + s0 + 64 * j + lane);
-      const double* xc = x + c * M;
+    double2 acc[8];
 #pragma unroll
-      for (int u = 0; u < M; u += 2) {
-        const double2 xv = *reinterpret_cast<const double2*>(xc + u);
-        acc[u] += v * xv.x;
-        acc[u + 1] += v * xv.y;
+    for (int u = 0; u < 8; ++u) acc[u] = double2{0.0, 0.0};
+    for (int j = 0; j < w; ++j) {
+      const double* vj = val + s0 + 64 * j;
+      const int32_t* cj = col + s0 + 64 * j;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = 8 * u + rg;
+        const double v = vj[r];
+        const double2 xv = ld2(x + (int64_t)cj[r] * M + 2 * cp);
+        acc[u].x += v * xv.x;
+        acc[u].y += v * xv.y;
       }
     }
-    const int64_t row = q * 64 + lane;
-    if (row < n_rows) {
 #pragma unroll
-      for (int u = 0; u < M; ++u) {
-        y[row * M + u] = acc[u];
-        if (partials != nullptr) dot[u] += acc[u] * x[row * M + u];
+    for (int u = 0; u < 8; ++u) {
+      const int64_t row = q * 64 + 8 * u + rg;
+      if (row < n_rows) {
+        st2(y + row * M + 2 * cp, acc[u]);
+        if (partials != nullptr) {
+          const double2 xr = ld2(x + row * M + 2 * cp);
+          dot.x += acc[u].x * xr.x;
+          dot.y += acc[u].y * xr.y;
+        }
       }
     }
   }
-  if (partials != nullptr) write_partials_m(dot, partials, scratch);
+  if (partials != nullptr) write_partials_pairs(dot, partials, scratch);
 }
 
-// CSR (long rows, e.g. R0A) times 16 vectors: one wavefront per row.
+// CSR (long rows, e.g. R0A) times 16 vectors: one wavefront per row, lane groups of 8
+// over the row's entries.
 __global__ __launch_bounds__(kBlock) void csr_rows_m_kernel(int64_t n_rows, const int64_t* __restrict__ row_ptr,
                                                             const int32_t* __restrict__ col,
                                                             const double* __restrict__ val,
@@ -231,23 +273,19 @@ __global__ __launch_bounds__(kBlock) void csr_rows_m_kernel(int64_t n_rows, cons
                                                             const dds_pcg_state* __restrict__ st) {
   if (all_done(st)) return;
   const int lane = threadIdx.x & 63;
+  const int cp = lane & (kPairs - 1);
+  const int g = lane >> 3;
   const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (row >= n_rows) return;
-  double acc[M] = {};
-  for (int64_t p = row_ptr[row] + lane; p < row_ptr[row + 1]; p += 64) {
+  double2 acc = {0.0, 0.0};
+  for (int64_t p = row_ptr[row] + g; p < row_ptr[row + 1]; p += 8) {
     const double v = val[p];
-    const double* xc = x + (int64_t)col[p] * M;
-#pragma unroll
-    for (int u = 0; u < M; ++u) acc[u] += v * xc[u];
+    const double2 xv = ld2(x + (int64_t)col[p] * M + 2 * cp);
+    acc.x += v * xv.x;
+    acc.y += v * xv.y;
   }
-#pragma unroll
-  for (int u = 0; u < M; ++u) acc[u] = wave_sum(acc[u]);
-  if (lane < M) {
-    double out = 0.0;
-#pragma unroll
-    for (int u = 0; u < M; ++u) out = lane == u ? acc[u] : out;
-    y[row * M + lane] = out;
-  }
+  acc = wave_pair_sum(acc);
+  if (lane < kPairs) st2(y + row * M + 2 * cp, acc);
 }
 
 // Y = S X for a dense symmetric n x n S: one wavefront per row.
@@ -257,22 +295,19 @@ __global__ __launch_bounds__(kBlock) void dense_symm_m_kernel(int64_t n, const d
                                                               const dds_pcg_state* __restrict__ st) {
   if (all_done(st)) return;
   const int lane = threadIdx.x & 63;
+  const int cp = lane & (kPairs - 1);
+  const int g = lane >> 3;
   const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (row >= n) return;
-  double acc[M] = {};
-  for (int64_t c = lane; c < n; c += 64) {
+  double2 acc = {0.0, 0.0};
+  for (int64_t c = g; c < n; c += 8) {
     const double s = S[row * n + c];
-#pragma unroll
-    for (int u = 0; u < M; ++u) acc[u] += s * x[c * M + u];
+    const double2 xv = ld2(x + c * M + 2 * cp);
+    acc.x += s * xv.x;
+    acc.y += s * xv.y;
   }
-#pragma unroll
-  for (int u = 0; u < M; ++u) acc[u] = wave_sum(acc[u]);
-  if (lane < M) {
-    double out = 0.0;
-#pragma unroll
-    for (int u = 0; u < M; ++u) out = lane == u ? acc[u] : out;
-    y[row * M + lane] = out;
-  }
+  acc = wave_pair_sum(acc);
+  if (lane < kPairs) st2(y + row * M + 2 * cp, acc);
 }
 
 // xc[c][:] = sum of x[j][:] over the unknowns j of coarse cell c (one block per cell).
@@ -284,50 +319,54 @@ __global__ __launch_bounds__(kBlock) void restrict_m_kernel(const int64_t* __res
   if (all_done(st)) return;
   const int64_t s = coarse_ptr[blockIdx.x];
   const int64_t e = coarse_ptr[blockIdx.x + 1];
-  double acc[M] = {};
-  for (int64_t j = s + threadIdx.x; j < e; j += kBlock) {
-#pragma unroll
-    for (int u = 0; u < M; ++u) acc[u] += x[j * M + u];
+  const int cp = threadIdx.x & (kPairs - 1);
+  double2 acc = {0.0, 0.0};
+  for (int64_t j = s + (threadIdx.x >> 3); j < e; j += kBlock / kPairs) {
+    const double2 v = ld2(x + j * M + 2 * cp);
+    acc.x += v.x;
+    acc.y += v.y;
   }
-  write_partials_m(acc, xc, scratch);  // block c writes xc[c * M + u]
+  write_partials_pairs(acc, xc, scratch);  // block c writes xc[c * M + j]
 }
 
 // z[j][:] = sum_p Y[dof_pos[p]][:] (or Y[j][:]) + sign * yc[cell[j]][:], with per-column
 // partials of r . z.
 __global__ __launch_bounds__(kBlock) void overlap_sum_m_kernel(
     int64_t n, const int64_t* __restrict__ dof_ptr, const int32_t* __restrict__ dof_pos,
-    const double* Y, const int32_t* __restrict__ cell, const double* __restrict__ yc, double sign,
-    double* z, const double* __restrict__ r, double* __restrict__ partials,
+    const double* __restrict__ Y, const int32_t* __restrict__ cell, const double* __restrict__ yc,
+    double sign, double* z, const double* __restrict__ r, double* __restrict__ partials,
     const dds_pcg_state* __restrict__ st) {
   __shared__ double scratch[(kBlock / 64) * M];
   if (all_done(st)) return;
-  double acc[M] = {};
-  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
-    double v[M];
+  const int cp = threadIdx.x & (kPairs - 1);
+  double2 acc = {0.0, 0.0};
+  const int64_t total = n * kPairs;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+    const int64_t j = t >> 3;
+    double2 v;
     if (dof_ptr != nullptr) {
-#pragma unroll
-      for (int u = 0; u < M; ++u) v[u] = 0.0;
+      v = double2{0.0, 0.0};
       for (int64_t p = dof_ptr[j]; p < dof_ptr[j + 1]; ++p) {
-        const double* yp = Y + (int64_t)dof_pos[p] * M;
-#pragma unroll
-        for (int u = 0; u < M; ++u) v[u] += yp[u];
+        const double2 yv = ld2(Y + (int64_t)dof_pos[p] * M + 2 * cp);
+        v.x += yv.x;
+        v.y += yv.y;
       }
     } else {
-#pragma unroll
-      for (int u = 0; u < M; ++u) v[u] = Y[j * M + u];
+      v = ld2(Y + 2 * t);
     }
     if (yc != nullptr) {
-      const double* c = yc + (int64_t)cell[j] * M;
-#pragma unroll
-      for (int u = 0; u < M; ++u) v[u] += sign * c[u];
+      const double2 c = ld2(yc + (int64_t)cell[j] * M + 2 * cp);
+      v.x += sign * c.x;
+      v.y += sign * c.y;
     }
-#pragma unroll
-    for (int u = 0; u < M; ++u) {
-      z[j * M + u] = v[u];
-      if (r != nullptr) acc[u] += r[j * M + u] * v[u];
+    st2(z + 2 * t, v);
+    if (r != nullptr) {
+      const double2 rv = ld2(r + 2 * t);
+      acc.x += rv.x * v.x;
+      acc.y += rv.y * v.y;
     }
   }
-  if (partials != nullptr) write_partials_m(acc, partials, scratch);
+  if (partials != nullptr) write_partials_pairs(acc, partials, scratch);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -363,8 +402,10 @@ struct MfmaTraits<bf16_t> {
   __device__ static __forceinline__ float back(elem v) { return widen(v); }
 };
 
-// One workgroup per subdomain: gather X_s (k x 16) into LDS-free registers row by row,
-// column maxima by a block reduction, then the split terms in B-operand order.
+// One workgroup per subdomain. Pass 1: column maxima of X_s (threads own column pairs of
+// the gathered rows). Pass 2: one thread per B-operand chunk (pair K2, lane l): it gathers
+// the 8 values X[32K2 + 16t2 + 4(l/16) + i][l%16] (t2 < 2, i < 4), scales them, splits
+// each into NT terms and writes NT 16-byte chunks.
 template <typename TM>
 __global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restrict__ sub_ptr,
                                                         const int32_t* __restrict__ sub_idx,
@@ -374,6 +415,7 @@ __global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restric
                                                         float* __restrict__ sigma,
                                                         const dds_pcg_state* __restrict__ st) {
   using Tr = MfmaTraits<TM>;
+  constexpr int NT = Tr::NT;
   __shared__ float smax[(kBlock / 64) * M];
   __shared__ float sig[M];
   if (all_done(st)) return;
@@ -381,19 +423,22 @@ __global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restric
   const int64_t base = sub_ptr[s];
   const int k = (int)(sub_ptr[s + 1] - base);
   auto row_of = [&](int a) -> int64_t { return sub_idx != nullptr ? (int64_t)sub_idx[base + a] : base + a; };
-  float mx[M] = {};
-  for (int a = threadIdx.x; a < k; a += kBlock) {
-    const double* xr = x + row_of(a) * M;
+  const int cp = threadIdx.x & (kPairs - 1);
+  float m0 = 0.f, m1 = 0.f;
+  for (int e = threadIdx.x; e < k * kPairs; e += kBlock) {
+    const double2 v = ld2(x + row_of(e >> 3) * M + 2 * cp);
+    m0 = fmaxf(m0, fabsf((float)v.x));
+    m1 = fmaxf(m1, fabsf((float)v.y));
+  }
 #pragma unroll
-    for (int u = 0; u < M; ++u) mx[u] = fmaxf(mx[u], fabsf((float)xr[u]));
+  for (int off = kPairs; off < 64; off <<= 1) {
+    m0 = fmaxf(m0, __shfl_xor(m0, off, 64));
+    m1 = fmaxf(m1, __shfl_xor(m1, off, 64));
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int u = 0; u < M; ++u) {
-    float v = mx[u];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    if (lane == 0) smax[w * M + u] = v;
+  if (lane < kPairs) {
+    smax[w * M + 2 * lane] = m0;
+    smax[w * M + 2 * lane + 1] = m1;
   }
   __syncthreads();
   if (threadIdx.x < M) {
@@ -404,29 +449,27 @@ __global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restric
     sigma[(int64_t)s * M + threadIdx.x] = v;
   }
   __syncthreads();
-  // element (row a, column c): pair K2 = a/32, within pair t2 = (a%32)/16, lane l with
-  // l%16 = c and 4(l/16) + i = a%16
   const int KP = (k + 31) / 32;
-  const int64_t total = (int64_t)KP * 32 * M;
-  for (int64_t e = threadIdx.x; e < total; e += kBlock) {
-    const int a = (int)(e / M);
-    const int c = (int)(e % M);
-    float v = 0.f;
-    double vd = 0.0;
-    if (a < k) {
-      vd = x[row_of(a) * M + c] / (double)sig[c];
-      v = (float)vd;
-    }
-    const int K2 = a / 32, t2 = (a % 32) / 16, r16 = a % 16;
-    const int l = (r16 / 4) * 16 + c, i = r16 % 4;
-    double rem = vd;
+  TM* out = bp + b_off[s];
+  for (int ch = threadIdx.x; ch < KP * 64; ch += kBlock) {
+    const int K2 = ch >> 6, l = ch & 63;
+    const int c = l & 15, r4 = 4 * (l >> 4);
+    const double inv = 1.0 / (double)sig[c];
+    Vec16<TM> terms[NT];
 #pragma unroll
-    for (int t = 0; t < Tr::NT; ++t) {
-      const typename Tr::elem q = Tr::cvt((float)rem);
-      rem -= (double)Tr::back(q);
-      bp[b_off[s] + ((int64_t)(K2 * Tr::NT + t) * 64 + l) * 8 + t2 * 4 + i] = q;
+    for (int e8 = 0; e8 < 8; ++e8) {
+      const int a = 32 * K2 + 16 * (e8 >> 2) + r4 + (e8 & 3);
+      double rem = a < k ? x[row_of(a) * M + c] * inv : 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const typename Tr::elem q = Tr::cvt((float)rem);
+        rem -= (double)Tr::back(q);
+        terms[t].v[e8] = q;
+      }
     }
-    (void)v;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      *reinterpret_cast<Vec16<TM>*>(out + ((int64_t)(K2 * NT + t) * 64 + l) * 8) = terms[t];
   }
 }
 
