@@ -1,0 +1,177 @@
+// Batched inverse of symmetric positive definite matrices (the local subdomain matrices
+// at preconditioner setup) by blocked Gauss-Jordan elimination without pivoting, in place.
+//
+// Matrix b is row-major k_b x k_b at A + b * stride with leading dimension ld. For each
+// pivot block p (NB rows/columns at p0) with P = A_pp^-1:
+//   A_pj <- P A_pj            (j outside p: the row panel)
+//   A_ij <- A_ij - A_ip A_pj' (i, j outside p, A_pj' the new row panel)
+//   A_ip <- -A_ip P           (i outside p)
+//   A_pp <- P
+// after the last pivot block A holds A^-1. On an SPD matrix every pivot is a Schur
+// complement diagonal and positive; a pivot <= 0 (the matrix is not SPD) sets info[b] = 1
+// and the caller falls back to a pivoting inverse for that batch.
+//
+// One workgroup of 256 threads per matrix; the trailing update runs over 64 x 64 tiles,
+// each thread owning a 4 x 4 micro-tile (rows 4 * (t / 16) + r, columns t % 16 + 16 c),
+// with the tile's pivot columns A_Ip and the row panel staged in LDS.
+#include "common.hpp"
+
+namespace dds {
+namespace {
+
+constexpr int kNB = 32;
+constexpr int kTile = 64;
+constexpr int kThreads = 256;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gj_inverse_kernel(T* __restrict__ A, int64_t stride, int64_t ld,
+                                                              const int32_t* __restrict__ ks,
+                                                              int32_t* __restrict__ info) {
+  __shared__ T P[kNB][kNB + 1];
+  __shared__ T Ti[kTile][kNB + 1];
+  __shared__ T Rt[kNB][kTile + 1];
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  T* a = A + (int64_t)blockIdx.x * stride;
+  const int k = ks[blockIdx.x];
+  if (tid == 0) bad = 0;
+  const int ty = tid >> 4, tx = tid & 15;
+  for (int p0 = 0; p0 < k; p0 += kNB) {
+    const int nb = min(kNB, k - p0);
+    // 1. pivot block into LDS (identity outside nb x nb) and its inverse by scalar sweeps
+    for (int e = tid; e < kNB * kNB; e += kThreads) {
+      const int r = e / kNB, c = e % kNB;
+      P[r][c] = (r < nb && c < nb) ? a[(int64_t)(p0 + r) * ld + p0 + c] : (r == c ? T(1) : T(0));
+    }
+    __syncthreads();
+    for (int q = 0; q < nb; ++q) {
+      const T piv = P[q][q];
+      if (tid == 0 && !(piv > T(0))) bad = 1;
+      const T ipiv = T(1) / piv;
+      T nv[(kNB * kNB) / kThreads];
+#pragma unroll
+      for (int u = 0; u < (kNB * kNB) / kThreads; ++u) {
+        const int e = tid + u * kThreads;
+        const int i = e / kNB, j = e % kNB;
+        T v;
+        if (i == q && j == q) v = ipiv;
+        else if (i == q) v = P[q][j] * ipiv;
+        else if (j == q) v = -P[i][q] * ipiv;
+        else v = P[i][j] - P[i][q] * (P[q][j] * ipiv);
+        nv[u] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < (kNB * kNB) / kThreads; ++u) {
+        const int e = tid + u * kThreads;
+        P[e / kNB][e % kNB] = nv[u];
+      }
+      __syncthreads();
+    }
+    // 2. row panel: A_pj <- P A_pj for the columns outside the pivot block
+    for (int j0 = 0; j0 < k; j0 += kTile) {
+      for (int e = tid; e < kNB * kTile; e += kThreads) {
+        const int q = e / kTile, c = e % kTile;
+        const int j = j0 + c;
+        Rt[q][c] = (q < nb && j < k) ? a[(int64_t)(p0 + q) * ld + j] : T(0);
+      }
+      __syncthreads();
+      for (int e = tid; e < kNB * kTile; e += kThreads) {
+        const int r = e / kTile, c = e % kTile;
+        const int j = j0 + c;
+        if (r < nb && j < k && (j < p0 || j >= p0 + nb)) {
+          T s = T(0);
+          for (int q = 0; q < nb; ++q) s += P[r][q] * Rt[q][c];
+          a[(int64_t)(p0 + r) * ld + j] = s;
+        }
+      }
+      __syncthreads();
+    }
+    // 3. rows outside the pivot block, tile by tile
+    for (int i0 = 0; i0 < k; i0 += kTile) {
+      for (int e = tid; e < kTile * kNB; e += kThreads) {
+        const int r = e / kNB, q = e % kNB;
+        const int i = i0 + r;
+        Ti[r][q] = (q < nb && i < k && (i < p0 || i >= p0 + nb)) ? a[(int64_t)i * ld + p0 + q] : T(0);
+      }
+      __syncthreads();
+      for (int j0 = 0; j0 < k; j0 += kTile) {
+        // Rt: the new row panel, with P in the pivot columns
+        for (int e = tid; e < kNB * kTile; e += kThreads) {
+          const int q = e / kTile, c = e % kTile;
+          const int j = j0 + c;
+          T v = T(0);
+          if (q < nb && j < k) {
+            if (j >= p0 && j < p0 + nb) v = P[q][j - p0];
+            else v = a[(int64_t)(p0 + q) * ld + j];
+          }
+          Rt[q][c] = v;
+        }
+        __syncthreads();
+        T acc[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[r][c] = T(0);
+        for (int q = 0; q < nb; ++q) {
+          T ti[4], rt[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ti[r] = Ti[4 * ty + r][q];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) rt[c] = Rt[q][tx + 16 * c];
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[r][c] += ti[r] * rt[c];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + 4 * ty + r;
+          if (i >= k || (i >= p0 && i < p0 + nb)) continue;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int j = j0 + tx + 16 * c;
+            if (j >= k) continue;
+            T* dst = a + (int64_t)i * ld + j;
+            const bool pivot_col = j >= p0 && j < p0 + nb;
+            *dst = (pivot_col ? T(0) : *dst) - acc[r][c];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // 4. the pivot block itself
+    for (int e = tid; e < kNB * kNB; e += kThreads) {
+      const int r = e / kNB, c = e % kNB;
+      if (r < nb && c < nb) a[(int64_t)(p0 + r) * ld + p0 + c] = P[r][c];
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && info != nullptr) info[blockIdx.x] = bad;
+}
+
+}  // namespace
+}  // namespace dds
+
+using namespace dds;
+
+extern "C" int dds_spd_inverse_batched(int dtype, int64_t n_mats, const int32_t* k, int64_t ld,
+                                       int64_t stride, void* A, int32_t* info, void* stream) {
+  DDS_REQUIRE(n_mats >= 0 && n_mats <= INT32_MAX, "dds_spd_inverse_batched: bad batch size");
+  DDS_REQUIRE(k && A, "dds_spd_inverse_batched: null pointer");
+  DDS_REQUIRE(ld >= 1 && stride >= ld, "dds_spd_inverse_batched: bad ld/stride");
+  if (n_mats == 0) return DDS_OK;
+  hipStream_t s = as_stream(stream);
+  if (dtype == DDS_F64) {
+    hipLaunchKernelGGL(gj_inverse_kernel<double>, dim3((unsigned)n_mats), dim3(kThreads), 0, s,
+                       static_cast<double*>(A), stride, ld, k, info);
+  } else if (dtype == DDS_F32) {
+    hipLaunchKernelGGL(gj_inverse_kernel<float>, dim3((unsigned)n_mats), dim3(kThreads), 0, s,
+                       static_cast<float*>(A), stride, ld, k, info);
+  } else {
+    set_error("dds_spd_inverse_batched: dtype must be F64 or F32 (got %d)", dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}

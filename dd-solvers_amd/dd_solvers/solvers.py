@@ -1000,6 +1000,7 @@ class SchwarzOperator(SparseSolver):
         self.coarse_solver = coarse_solver
         self.collect_timings = collect_timings
         self.waves = None
+        self._mfma = None
 
     # -- setup ------------------------------------------------------------------------
     def setup(self, matrix: torch.Tensor, dofs_per_solver: int | None = None,
@@ -1214,8 +1215,7 @@ class SchwarzOperator(SparseSolver):
         flat_key = (s * k_max + a) * k_max + b
         order = torch.sort(flat_key).indices
         flat_key, v = flat_key[order], v[order]
-        for c0 in range(0, n_s, chunk):
-            c1 = min(n_s, c0 + chunk)
+        def build_chunk(c0, c1):
             lo = int(torch.searchsorted(flat_key, torch.tensor(c0 * k_max * k_max, device=dev)))
             hi = int(torch.searchsorted(flat_key, torch.tensor(c1 * k_max * k_max, device=dev)))
             blk = torch.zeros((c1 - c0) * k_max * k_max, dtype=pi, device=dev)
@@ -1225,7 +1225,23 @@ class SchwarzOperator(SparseSolver):
             ar = torch.arange(k_max, device=dev)
             pad = ar[None, :] >= kk[:, None]
             blk.diagonal(dim1=1, dim2=2)[pad] = 1.0  # identity padding
-            inv = torch.linalg.inv(blk)
+            return blk, kk
+
+        for c0 in range(0, n_s, chunk):
+            c1 = min(n_s, c0 + chunk)
+            blk, kk = build_chunk(c0, c1)
+            # local matrices of an SPD operator are SPD: batched Gauss-Jordan inverse in
+            # place; a chunk holding a matrix with a non-positive pivot is rebuilt and
+            # inverted with pivoting (torch.linalg.inv, as the reference's Inv.setup)
+            info = torch.zeros(c1 - c0, dtype=torch.int32, device=dev)
+            _lib.call("dds_spd_inverse_batched", _lib.dtype_code(pi), c1 - c0,
+                      kk.to(torch.int32).contiguous().data_ptr(), k_max, k_max * k_max,
+                      blk.data_ptr(), info.data_ptr(), _lib.stream())
+            if int(info.max()) == 0:
+                inv = blk
+            else:
+                blk, kk = build_chunk(c0, c1)
+                inv = torch.linalg.inv(blk)
             for j in range(c1 - c0):
                 i = c0 + j
                 k = int(sizes[i])

@@ -216,6 +216,16 @@ int dds_pcg_finalize(int op, const double* partials, int count, dds_pcg_state* s
 /* Plain sum of `count` partials into *out (device double), for dot products outside PCG. */
 int dds_sum_partials(const double* partials, int count, double* out, void* stream);
 
+/* ---- setup -------------------------------------------------------------------------------
+ * In-place inverse of n_mats SPD matrices (dtype DDS_F64 or DDS_F32): matrix b is row-major
+ * k[b] x k[b] at A + b * stride elements, leading dimension ld (k[b] <= ld, stride >=
+ * ld * k[b]); blocked Gauss-Jordan without pivoting, one workgroup per matrix. info[b]
+ * (optional) is set to 1 when a pivot is not positive (the matrix is not SPD; the result is
+ * then unusable). Replaces torch.linalg.inv(matrix_batch) in Inv.setup (solvers.py:350-352)
+ * for the local matrices of the Schwarz preconditioners, which are SPD. */
+int dds_spd_inverse_batched(int dtype, int64_t n_mats, const int32_t* k, int64_t ld,
+                            int64_t stride, void* A, int32_t* info, void* stream);
+
 /* ---- differentiation -------------------------------------------------------------------
  * grad[e] = -(1/N) exp(theta[e]) lam_e^T Khat u_e for the N^3 Q1 elements of the unit cube:
  * the coefficient gradient of J(u(theta)) given the adjoint solution lam of A lam = dJ/du.

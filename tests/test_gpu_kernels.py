@@ -224,3 +224,49 @@ def test_csrmv_tall_matrix():
     with pytest.raises(ValueError):
         op._enqueue_matvec(torch.as_tensor(k, device=dev), y,
                            torch.zeros(2048, dtype=torch.float64, device=dev), None)
+
+
+# ---------------------------------------------------------------------------------------
+# batched SPD inverse (local matrices at setup)
+# ---------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 2e-3)])
+def test_spd_inverse_batched_matches_torch(dtype, tol):
+    from dd_solvers import _lib
+    rng = np.random.default_rng(5)
+    ks = [1, 5, 31, 32, 33, 64, 65, 100, 130, 257]
+    kmax = max(ks)
+    A = np.zeros((len(ks), kmax, kmax))
+    for b, k in enumerate(ks):
+        B = rng.standard_normal((k, k))
+        A[b, :k, :k] = B @ B.T + k * np.eye(k)
+        A[b, k:, k:] = np.eye(kmax - k) * 7.0  # padding outside k x k must stay as it is
+    At = torch.as_tensor(A, dtype=dtype, device=dev).contiguous()
+    orig = At.clone()
+    info = torch.full((len(ks),), -1, dtype=torch.int32, device=dev)
+    kt = torch.as_tensor(ks, dtype=torch.int32, device=dev)
+    _lib.call("dds_spd_inverse_batched", _lib.dtype_code(dtype), len(ks), kt.data_ptr(), kmax,
+              kmax * kmax, At.data_ptr(), info.data_ptr(), _lib.stream())
+    assert info.cpu().tolist() == [0] * len(ks)
+    for b, k in enumerate(ks):
+        ref = np.linalg.inv(A[b, :k, :k])
+        got = At[b, :k, :k].double().cpu().numpy()
+        assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (k, np.abs(got - ref).max())
+        assert torch.equal(At[b, k:, :], orig[b, k:, :]) and torch.equal(At[b, :k, k:], orig[b, :k, k:])
+
+
+def test_spd_inverse_batched_flags_indefinite():
+    from dd_solvers import _lib
+    A = torch.tensor([[[2.0, 0.0], [0.0, -1.0]], [[4.0, 1.0], [1.0, 3.0]]], dtype=torch.float64, device=dev)
+    info = torch.zeros(2, dtype=torch.int32, device=dev)
+    kt = torch.tensor([2, 2], dtype=torch.int32, device=dev)
+    _lib.call("dds_spd_inverse_batched", _lib.F64, 2, kt.data_ptr(), 2, 4, A.data_ptr(),
+              info.data_ptr(), _lib.stream())
+    assert info.cpu().tolist() == [1, 0]
+    assert np.allclose(A[1].cpu().numpy(), np.linalg.inv([[4.0, 1.0], [1.0, 3.0]]), rtol=1e-14)
+
+
+def test_spd_inverse_rejects_bad_arguments():
+    from dd_solvers import _lib
+    with pytest.raises(_lib.HipError):
+        _lib.call("dds_spd_inverse_batched", _lib.F16, 1, 1, 2, 4, 1, None, None)
