@@ -224,6 +224,16 @@ def target_extra(dev, steps, warmup, seed):
     return out
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC
+    measurement (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            return int(json.load(f)[workload]["kernels"][kernel]["hbm_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def time_solves(solver, b, steps, warmup):
     import torch
 
@@ -326,9 +336,11 @@ def main():
     ap.add_argument("--no-target", action="store_true", help="skip the 128^3 comparison")
     ap.add_argument("--graph", action="store_true", help="replay captured iteration blocks")
     ap.add_argument("--profile-kernel", action="store_true",
-                    help="only launch the local-apply kernel 20 times, then a 1 GiB device "
-                         "copy 20 times (calibration for rocprofv3 --pmc byte counters)")
+                    help="after setup only launch the local apply, the SpMV and a 1 GiB device "
+                         "copy, 20 times each (for rocprofv3 --pmc byte counters)")
     ap.add_argument("--storage", default="float64", choices=["float64", "float32", "float16", "bfloat16"])
+    ap.add_argument("--layout", default="sym", choices=["sym", "mfma"],
+                    help="--profile-kernel: the local-inverse layout to launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -348,26 +360,44 @@ def main():
     b = torch.as_tensor(pb.b, device=dev)
 
     if args.profile_kernel:
+        # counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): after setup, only the
+        # measured kernels, 20 launches each, then a 1 GiB device copy 20 times to
+        # calibrate the byte counters
         st = getattr(torch, args.storage)
-        solver = make_solver(pb, st, dev, scaled=st == torch.float16)
+        if args.layout == "mfma":
+            solver = make_block_solver(pb, st, dev, use_graph=False, autotune=False)
+        else:
+            solver = make_solver(pb, st, dev, scaled=st == torch.float16)
         M = solver.preconditioner
+        L = M.local
+        sizes = M._host["sizes"].astype(np.int64)
         xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
-        Y = M.Y if M.Y is not None else torch.empty_like(xv)
+        if args.layout == "mfma":
+            X = torch.rand((pb.A.shape[0], _lib.NRHS), dtype=torch.float64, device=dev)
+            M._enqueue_local_multi(X, M._mfma["Ym"], None)
+            info = mfma_kernel_stats(M)  # 3 + 20 launches, event-timed
+            alg = info["alg_bytes_per_launch"]
+        else:
+            Y = M.Y if M.Y is not None else torch.empty_like(xv)
+            for _ in range(20):
+                M._enqueue_sym_apply(xv, None)
+            for _ in range(20):
+                M._enqueue_sym_combine(Y, None)
+            alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + M.Yp.numel() * 8)
+        q = torch.empty_like(xv)
         for _ in range(20):
-            M._enqueue_local(xv, Y, None)
+            solver.matrix._enqueue_matvec(xv, q, None, None)
         torch.cuda.synchronize()
         src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
         dst = torch.empty_like(src)
         for _ in range(20):
             dst.copy_(src)
         torch.cuda.synchronize()
-        L = M.local
-        sizes = M._host["sizes"].astype(np.int64)
-        log(f"profile-kernel: storage={args.storage} layout={L.layout} "
-            f"algorithmic matrix bytes={L.bytes_algorithmic} stored={L.bytes_matrices} "
-            f"vectors={int(sizes.sum() * 12 + Y.numel() * 8)}; copy 2 x {src.numel() * 8} bytes")
-        print(json.dumps({"profile_kernel": True, "alg_matrix_bytes": L.bytes_algorithmic,
-                          "stored_matrix_bytes": L.bytes_matrices}))
+        out = {"profile_kernel": True, "layout": L.layout, "storage": args.storage,
+               "local_alg_bytes": alg, "stored_matrix_bytes": L.bytes_matrices,
+               "spmv_alg_bytes": solver.matrix.bytes_per_matvec, "copy_bytes_each_way": src.numel() * 8}
+        log(json.dumps(out))
+        print(json.dumps(out))
         return
 
     # ---- plain FP64 (the measured line) ------------------------------------------------
@@ -402,6 +432,7 @@ def main():
         alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + sizes.sum() * 8)
         kname = f"local_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
     gbs = alg_bytes / (t_loc * 1e-3) / 1e9
+    traffic = measured_traffic(f"q1_{args.N}_fp64_{L.layout}", "sym_apply" if L.layout == "sym" else "local_apply")
     log(f"local apply ({kname}): {t_loc * 1e3:.1f} us, {alg_bytes / 1e9:.3f} GB -> {gbs:.0f} GB/s "
         f"({gbs / HBM_PEAK_GBS * 100:.1f}% of HBM peak); combine {t_comb * 1e3:.1f} us")
     # SpMV alone, for the record
@@ -493,7 +524,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_us": round(t_loc * 1e3, 2),
         },
