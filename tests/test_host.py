@@ -292,3 +292,50 @@ def test_sym_layout_and_item_plan(k, V):
                   for a in range(k)])
     assert not np.isnan(z).any()
     assert np.allclose(z, A @ x, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("k", [3, 16, 17, 33, 100])
+def test_mfma_operand_layout_model(k):
+    """The A-operand order written at setup (_mfma_index) and the B-operand order of the X
+    pack, combined with the 16x16x16 MFMA lane rule (A: row l%16, k = 4(l/16)+i; B: column
+    l%16, k = 4(l/16)+i; D: rows 4(l/16)+i, column l%16), reproduce A @ X."""
+    from dd_solvers.solvers import _mfma_index
+    rng = np.random.default_rng(k)
+    A = rng.standard_normal((k, k))
+    X = rng.standard_normal((k, 16))
+    idx = _mfma_index(k)
+    packed = np.where(idx >= 0, A.reshape(-1)[np.maximum(idx, 0)], 0.0)
+    RB, KP = -(-k // 16), -(-k // 32)
+    assert packed.shape[0] == RB * KP * 64 * 8
+    Bp = np.zeros(KP * 64 * 8)
+    for a in range(32 * KP):  # the pack: element (a, c) of X to (K2, lane, slot)
+        for c in range(16):
+            K2, t2, r16 = a // 32, (a % 32) // 16, a % 16
+            lane, i = (r16 // 4) * 16 + c, r16 % 4
+            Bp[(K2 * 64 + lane) * 8 + t2 * 4 + i] = X[a, c] if a < k else 0.0
+    Y = np.zeros((16 * RB, 16))
+    for R in range(RB):
+        D = np.zeros((16, 16))
+        for K2 in range(KP):
+            a0 = (R * KP + K2) * 512
+            a_op = packed[a0:a0 + 512].reshape(64, 8)
+            b_op = Bp[K2 * 512:K2 * 512 + 512].reshape(64, 8)
+            for half in range(2):
+                Am = np.zeros((16, 16))
+                Bm = np.zeros((16, 16))
+                for lane in range(64):
+                    for i in range(4):
+                        Am[lane % 16, 4 * (lane // 16) + i] = a_op[lane, 4 * half + i]
+                        Bm[4 * (lane // 16) + i, lane % 16] = b_op[lane, 4 * half + i]
+                D += Am @ Bm
+        Y[16 * R:16 * R + 16] = D
+    assert np.allclose(Y[:k], A @ X, rtol=1e-12, atol=1e-12)
+
+
+def test_mfma_autotuner_candidates():
+    from dd_solvers.tuning import mfma_candidates
+    c = mfma_candidates(10_000, 256)
+    assert (0, 1) in c and (0, 8) in c
+    assert all(w in (1, 2, 4, 8) for _, w in c)
+    assert all(g == 0 or g * w < 10_000 for g, w in c)
+    assert mfma_candidates(10, 256) == [(0, 1), (0, 2), (0, 4), (0, 8)]
