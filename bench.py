@@ -224,6 +224,36 @@ def target_extra(dev, steps, warmup, seed):
     return out
 
 
+def stream_peaks(dev):
+    """Achievable HBM bandwidth on this GPU: dds_stream read / copy / triad over 1 GiB
+    arrays (beyond the 256 MB MALL), best grid of {1, 2, 4, 8} workgroups per CU, and the
+    reference's own method (a tensor clone, experiments/memory_bandwidth.ipynb)."""
+    import torch
+    from dd_solvers import _lib
+
+    n = 1 << 27  # doubles per array (1 GiB)
+    a = torch.ones(n, dtype=torch.float64, device=dev)
+    b = torch.ones(n, dtype=torch.float64, device=dev)
+    y = torch.empty(n, dtype=torch.float64, device=dev)
+    n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
+    s = _lib.stream()
+    out = {}
+    for op, name, moved in ((0, "read", 8), (1, "copy", 16), (2, "triad", 24)):
+        best = 0.0
+        for per_cu in (1, 2, 4, 8):
+            grid = min(n_cu * per_cu, _lib.MAX_PARTIALS)
+            ms = kernel_time_ms(lambda: _lib.call("dds_stream", op, n, a.data_ptr(), b.data_ptr(),
+                                                  0.5, y.data_ptr(), grid, s), reps=10)
+            best = max(best, moved * n / (ms * 1e-3) / 1e9)
+        out[f"{name}_GBs"] = round(best, 1)
+    ms = kernel_time_ms(lambda: a.clone(), reps=10)
+    out["torch_clone_GBs"] = round(16 * n / (ms * 1e-3) / 1e9, 1)
+    del a, b, y
+    torch.cuda.empty_cache()
+    log(f"stream: {out}")
+    return out
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC
     measurement (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
@@ -485,6 +515,13 @@ def main():
             except Exception as e:
                 extra["target_128"] = {"error": str(e)[:200]}
 
+    stream = None
+    if not args.no_extra:
+        try:
+            stream = stream_peaks(dev)
+        except Exception as e:
+            stream = {"error": str(e)[:200]}
+
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -528,6 +565,7 @@ def main():
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_us": round(t_loc * 1e3, 2),
         },
+        "stream": stream,
         "solve_roofline": {"alg_bytes": int(solve_bytes), "achieved_GBs": round(solve_gbs, 1),
                            "frac": round(solve_gbs / HBM_PEAK_GBS, 4)},
         "spmv": {"us": round(t_spmv * 1e3, 2), "bytes": int(spmv_bytes),

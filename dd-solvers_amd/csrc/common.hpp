@@ -140,5 +140,8 @@ __device__ __forceinline__ T block_sum(T v, T* scratch) {
 
 // Maximum number of per-block partial sums a reduction-producing kernel writes.
 constexpr int kMaxPartials = DDS_MAX_PARTIALS;
+// Grid of the grid-stride vector kernels (their partial counts stay at or below this):
+// 2048 workgroups of 256 fill the 256 CUs 8 deep.
+constexpr int kVecGrid = 2048;
 
 }  // namespace dds

@@ -71,7 +71,7 @@ __device__ __forceinline__ double2 wave_pair_sum(double2 v) {
 }
 
 int64_t vgrid(int64_t rows) {
-  return std::min<int64_t>(ceil_div(std::max<int64_t>(rows * kPairs, 1), kBlock), kMaxPartials);
+  return std::min<int64_t>(ceil_div(std::max<int64_t>(rows * kPairs, 1), kBlock), kVecGrid);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -213,7 +213,9 @@ __global__ __launch_bounds__(M * 64) void finalize_m_kernel(int op, const double
 // SELL-64 times 16 vectors; optional per-column partials of x . y. One wavefront per
 // slice: lane l owns column pair l % 8 of the rows 8u + l / 8 (u < 8), so each gather of
 // x reads 8 whole 128-byte rows per instruction.
-__global__ __launch_bounds__(kBlock) void sell_m_kernel(int64_t n_rows, int64_t n_slices,
+constexpr int kSellWaves = 16;  // wavefronts per SpMM workgroup
+
+__global__ __launch_bounds__(64 * kSellWaves) void sell_m_kernel(int64_t n_rows, int64_t n_slices,
                                                         const int64_t* __restrict__ slice_ptr,
                                                         const int32_t* __restrict__ col,
                                                         const double* __restrict__ val,
@@ -221,13 +223,13 @@ __global__ __launch_bounds__(kBlock) void sell_m_kernel(int64_t n_rows, int64_t 
                                                         double* __restrict__ y,
                                                         double* __restrict__ partials,
                                                         const dds_pcg_state* __restrict__ st) {
-  __shared__ double scratch[(kBlock / 64) * M];
+  __shared__ double scratch[kSellWaves * M];
   if (all_done(st)) return;
   const int lane = threadIdx.x & 63;
   const int cp = lane & (kPairs - 1);
   const int rg = lane >> 3;
-  const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const int64_t n_waves = (int64_t)gridDim.x * (kBlock / 64);
+  const int64_t wave = (int64_t)blockIdx.x * kSellWaves + (threadIdx.x >> 6);
+  const int64_t n_waves = (int64_t)gridDim.x * kSellWaves;
   double2 dot = {0.0, 0.0};
   for (int64_t q = wave; q < n_slices; q += n_waves) {
     const int64_t s0 = slice_ptr[q];
@@ -629,8 +631,8 @@ extern "C" int dds_sellmm(int64_t n_rows, const int64_t* slice_ptr, const int32_
                           int* n_partials, const dds_pcg_state* st, void* stream) {
   DDS_REQUIRE(n_rows >= 0 && slice_ptr && col && values && x && y, "dds_sellmm: bad arguments");
   const int64_t n_slices = ceil_div(n_rows, 64);
-  const int64_t grid = std::min<int64_t>(kMaxPartials, ceil_div(n_slices, kBlock / 64));
-  hipLaunchKernelGGL(sell_m_kernel, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(kBlock), 0,
+  const int64_t grid = std::min<int64_t>(kVecGrid, ceil_div(n_slices, kSellWaves));
+  hipLaunchKernelGGL(sell_m_kernel, dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(64 * kSellWaves), 0,
                      as_stream(stream), n_rows, n_slices, slice_ptr, col, values, x, y, partials, st);
   DDS_LAUNCHED();
   if (n_partials) *n_partials = partials ? (int)std::max<int64_t>(grid, 1) : 0;

@@ -8,7 +8,7 @@ namespace {
 constexpr int kBlock = 256;
 
 int64_t vec_grid(int64_t n) {
-  return std::min<int64_t>(ceil_div(std::max<int64_t>(n, 1), kBlock), kMaxPartials);
+  return std::min<int64_t>(ceil_div(std::max<int64_t>(n, 1), kBlock), kVecGrid);
 }
 
 __device__ __forceinline__ bool gated(const dds_pcg_state* g) { return g != nullptr && g->done; }

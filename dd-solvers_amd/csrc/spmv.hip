@@ -137,20 +137,59 @@ __global__ __launch_bounds__(kBlock) void bsr_seg_kernel(int64_t n_block_rows,
 // lane = row: every value and index load is one coalesced 512 B / 256 B access, no row
 // pointers are read, and the padding costs nothing but its bytes.
 // ---------------------------------------------------------------------------------------
+// SELL-64 with compact column indices: one wavefront per slice (contiguous runs of slices
+// per wave, contiguous runs of waves per XCD). A slice whose slots each span < 65536
+// columns stores 16-bit offsets from a per-slot base (2 bytes per entry instead of 4).
+constexpr int kSellWaves = 4;
+
+template <typename T, typename ColF>
+__device__ __forceinline__ double sell_slice(const T* __restrict__ v, int w, ColF col_of,
+                                             const T* __restrict__ x) {
+  double acc = 0.0;
+  int j = 0;
+  for (; j + 7 < w; j += 8) {
+    int32_t cc[8];
+    T vv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cc[u] = col_of(j + u);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) vv[u] = load_nt(v + 64 * (j + u));
+    T xx[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xx[u] = x[cc[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += static_cast<double>(vv[u]) * static_cast<double>(xx[u]);
+  }
+  for (; j + 3 < w; j += 4) {
+    int32_t cc[4];
+    T vv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) cc[u] = col_of(j + u);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) vv[u] = load_nt(v + 64 * (j + u));
+    T xx[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xx[u] = x[cc[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += static_cast<double>(vv[u]) * static_cast<double>(xx[u]);
+  }
+  for (; j < w; ++j) acc += static_cast<double>(load_nt(v + 64 * j)) * static_cast<double>(x[col_of(j)]);
+  return acc;
+}
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void sell_kernel(int64_t n_rows, int64_t n_slices,
-                                                      const int64_t* __restrict__ slice_ptr,
-                                                      const int32_t* __restrict__ col,
-                                                      const T* __restrict__ val,
-                                                      const T* __restrict__ x, T* __restrict__ y,
-                                                      double* __restrict__ partials,
-                                                      const dds_pcg_state* __restrict__ gate) {
-  __shared__ double scratch[kBlock / 64];
+__global__ __launch_bounds__(64 * kSellWaves) void sell_kernel(
+    int64_t n_rows, int64_t n_slices, const int64_t* __restrict__ slice_ptr,
+    const int64_t* __restrict__ col_ptr, const uint16_t* __restrict__ col16,
+    const int32_t* __restrict__ col32, const int32_t* __restrict__ slot_base,
+    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+    double* __restrict__ partials, const dds_pcg_state* __restrict__ gate) {
+  __shared__ double scratch[kSellWaves];
   if (gate != nullptr && gate->done) return;
   const int lane = threadIdx.x & 63;
   // contiguous slice ranges per wave, contiguous wave ranges per XCD (grid % 8 == 0)
-  const int64_t wave = xcd_block_id() * (kBlock / 64) + (threadIdx.x >> 6);
-  const int64_t n_waves = (int64_t)gridDim.x * (kBlock / 64);
+  const int64_t wave = xcd_block_id() * kSellWaves + (threadIdx.x >> 6);
+  const int64_t n_waves = (int64_t)gridDim.x * kSellWaves;
   const int64_t per = (n_slices + n_waves - 1) / n_waves;
   const int64_t q0 = wave * per;
   const int64_t q1 = q0 + per < n_slices ? q0 + per : n_slices;
@@ -158,25 +197,17 @@ __global__ __launch_bounds__(kBlock) void sell_kernel(int64_t n_rows, int64_t n_
   for (int64_t q = q0; q < q1; ++q) {
     const int64_t s0 = slice_ptr[q];
     const int w = (int)((slice_ptr[q + 1] - s0) >> 6);
-    const T* v = val + s0 + lane;
-    const int32_t* c = col + s0 + lane;
-    double acc = 0.0;
-    int j = 0;
-    for (; j + 7 < w; j += 8) {
-      int32_t cc[8];
-      T vv[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) cc[u] = load_nt(c + 64 * (j + u));
-#pragma unroll
-      for (int u = 0; u < 8; ++u) vv[u] = load_nt(v + 64 * (j + u));
-      T xx[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) xx[u] = x[cc[u]];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc += static_cast<double>(vv[u]) * static_cast<double>(xx[u]);
+    const int64_t cp = col_ptr[q];
+    double acc;
+    if (cp >= 0) {
+      const uint16_t* d = col16 + cp + lane;
+      const int32_t* b = slot_base + (s0 >> 6);
+      acc = sell_slice<T>(val + s0 + lane, w,
+                          [&](int j) { return b[j] + (int32_t)load_nt(d + 64 * j); }, x);
+    } else {
+      const int32_t* c = col32 + (~cp) + lane;
+      acc = sell_slice<T>(val + s0 + lane, w, [&](int j) { return load_nt(c + 64 * j); }, x);
     }
-    for (; j < w; ++j)
-      acc += static_cast<double>(load_nt(v + 64 * j)) * static_cast<double>(x[load_nt(c + 64 * j)]);
     const int64_t row = q * 64 + lane;
     if (row < n_rows) {
       y[row] = static_cast<T>(acc);
@@ -308,28 +339,28 @@ extern "C" int dds_bsrmv(int val_dtype, int idx_dtype, int64_t n_block_rows, int
 }
 
 extern "C" int dds_sellmv(int val_dtype, int64_t n_rows, const int64_t* slice_ptr,
-                          const int32_t* col, const void* values, const void* x, void* y,
+                          const int64_t* col_ptr, const uint16_t* col16, const int32_t* col32,
+                          const int32_t* slot_base, const void* values, const void* x, void* y,
                           double* dot_partials, int* n_partials, const dds_pcg_state* gate,
                           void* stream) {
   DDS_REQUIRE(n_rows >= 0, "dds_sellmv: negative size");
   if (n_partials) *n_partials = 0;
   if (n_rows == 0) return DDS_OK;
-  DDS_REQUIRE(slice_ptr && col && values && x && y, "dds_sellmv: null pointer");
+  DDS_REQUIRE(slice_ptr && col_ptr && slot_base && values && x && y, "dds_sellmv: null pointer");
+  DDS_REQUIRE(col16 || col32, "dds_sellmv: no column indices");
   const int64_t n_slices = ceil_div(n_rows, 64);
-  // one slice per wave when there are few, contiguous runs of slices when there are many;
-  // a multiple of the XCD count (so every XCD gets an equal contiguous share), at most the
-  // partial-sum capacity
-  const int64_t waves_per_block = kBlock / 64;
+  // one slice per wave up to the partial-sum capacity, contiguous runs of slices beyond;
+  // a multiple of the XCD count (so every XCD gets an equal contiguous share)
   const int64_t grid = std::min<int64_t>(
-      kMaxPartials, ceil_div(ceil_div(n_slices, waves_per_block), kXcds) * kXcds);
+      kMaxPartials, ceil_div(ceil_div(n_slices, kSellWaves), kXcds) * kXcds);
   hipStream_t s = as_stream(stream);
   if (val_dtype == DDS_F64) {
-    hipLaunchKernelGGL(sell_kernel<double>, dim3((unsigned)grid), dim3(kBlock), 0, s, n_rows,
-                       n_slices, slice_ptr, col, static_cast<const double*>(values),
+    hipLaunchKernelGGL(sell_kernel<double>, dim3((unsigned)grid), dim3(64 * kSellWaves), 0, s, n_rows,
+                       n_slices, slice_ptr, col_ptr, col16, col32, slot_base, static_cast<const double*>(values),
                        static_cast<const double*>(x), static_cast<double*>(y), dot_partials, gate);
   } else if (val_dtype == DDS_F32) {
-    hipLaunchKernelGGL(sell_kernel<float>, dim3((unsigned)grid), dim3(kBlock), 0, s, n_rows,
-                       n_slices, slice_ptr, col, static_cast<const float*>(values),
+    hipLaunchKernelGGL(sell_kernel<float>, dim3((unsigned)grid), dim3(64 * kSellWaves), 0, s, n_rows,
+                       n_slices, slice_ptr, col_ptr, col16, col32, slot_base, static_cast<const float*>(values),
                        static_cast<const float*>(x), static_cast<float*>(y), dot_partials, gate);
   } else {
     set_error("dds_sellmv: unsupported dtype %d", val_dtype);

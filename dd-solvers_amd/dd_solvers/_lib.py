@@ -25,9 +25,9 @@ _lib = ctypes.CDLL(LIB_PATH)
 F64, F32, F16, BF16, I32, I64 = 0, 1, 2, 3, 4, 5
 # finalize ops
 FIN_CHECK, FIN_RHO, FIN_ALPHA, FIN_NEXT = 0, 1, 2, 3
-MAX_PARTIALS = 2048
+MAX_PARTIALS = 8192
 NRHS = 16  # right-hand sides per block in the multi-RHS kernels
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _TORCH_TO_DDS = {
     torch.float64: F64,
@@ -58,7 +58,7 @@ _SIGNATURES = {
     "dds_gemv_strided_batched": [_i, _i, _p, _p, _p, _i64, _i64, _i, _p],
     "dds_bsrmv": [_i, _i, _i64, _i64, _p, _p, _p, _p, _p, _p, _ip, _p, _p],
     "dds_csrmv": [_i, _i, _i64, _p, _p, _p, _p, _p, _i, _p, _ip, _p, _p],
-    "dds_sellmv": [_i, _i64, _p, _p, _p, _p, _p, _p, _ip, _p, _p],
+    "dds_sellmv": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _ip, _p, _p],
     "dds_local_vec_width": [_i],
     "dds_local_tile_rows": [_i],
     "dds_local_apply": [_i, _i, _i, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _i, _p, _p],
@@ -79,6 +79,7 @@ _SIGNATURES = {
     "dds_cast": [_i, _i, _i64, _p, _p, _p, _p],
     "dds_q1_element_grad": [_i64, _p, _p, _p, _p, _p, _p],
     "dds_spd_inverse_batched": [_i, _i64, _p, _i64, _i64, _p, _p, _p],
+    "dds_stream": [_i, _i64, _p, _p, _d, _p, _i64, _p],
     "dds_pcg_residual_m": [_i64, _p, _p, _p, _p, _ip, _p],
     "dds_pcg_update_p_m": [_i64, _p, _p, _p, _p],
     "dds_pcg_update_xr_m": [_i64, _p, _p, _p, _p, _p, _ip, _p, _p],

@@ -309,6 +309,31 @@ class _SELLOperator:
         self.slice_ptr, self.col, self.values = slice_ptr, col, val
         self.nnz = int(vals.shape[0])
         self.stored = total
+        self._compact_columns(width)
+
+    def _compact_columns(self, width):
+        """Per slot (64 entries) an int32 base and 16-bit offsets when the slot's columns
+        span < 65536; slices with a wider slot keep int32 columns (dds_sellmv)."""
+        dev = self.device
+        n_slices = width.shape[0]
+        C = self.col.view(-1, 64).to(torch.int64)
+        base = C.min(dim=1).values
+        span = C.max(dim=1).values - base
+        slot_slice = torch.repeat_interleave(torch.arange(n_slices, device=dev), width)
+        wide = torch.zeros(n_slices, dtype=torch.int64, device=dev).scatter_reduce_(
+            0, slot_slice, (span > 65535).to(torch.int64), "amax").bool()
+        entries = 64 * width
+        n16 = torch.where(wide, 0, entries)
+        n32 = torch.where(wide, entries, 0)
+        off16 = torch.cumsum(n16, 0) - n16
+        off32 = torch.cumsum(n32, 0) - n32
+        self.col_ptr = torch.where(wide, -1 - off32, off16).contiguous()
+        slot_wide = wide[slot_slice]
+        d = (C[~slot_wide] - base[~slot_wide, None]).reshape(-1)
+        self.col16 = torch.where(d >= 32768, d - 65536, d).to(torch.int16).contiguous()
+        self.col32 = C[slot_wide].reshape(-1).to(torch.int32).contiguous()
+        self.slot_base = torch.where(slot_wide, torch.zeros_like(base), base).to(torch.int32).contiguous()
+        self.wide_fraction = float(wide.float().mean()) if n_slices else 0.0
 
     def __matmul__(self, x):
         y = torch.empty(self.shape[0], dtype=self.dtype, device=self.device)
@@ -319,7 +344,8 @@ class _SELLOperator:
     def _enqueue_matvec(self, x, y, partials, gate) -> int:
         return _lib.call_partials(
             "dds_sellmv", _lib.dtype_code(self.dtype), self.shape[0], self.slice_ptr.data_ptr(),
-            self.col.data_ptr(), self.values.data_ptr(), x.data_ptr(), y.data_ptr(),
+            self.col_ptr.data_ptr(), _lib.ptr(self.col16), _lib.ptr(self.col32),
+            self.slot_base.data_ptr(), self.values.data_ptr(), x.data_ptr(), y.data_ptr(),
             _lib.ptr(partials), _lib.PARTIALS_OUT, _lib.ptr(gate), _lib.stream())
 
     def _enqueue_matmat(self, X, Y, partials, st) -> int:
@@ -330,6 +356,14 @@ class _SELLOperator:
             "dds_sellmm", self.shape[0], self.slice_ptr.data_ptr(), self.col.data_ptr(),
             self.values.data_ptr(), X.data_ptr(), Y.data_ptr(), _lib.ptr(partials),
             _lib.PARTIALS_OUT, _lib.ptr(st), _lib.stream())
+
+    @property
+    def bytes_stored(self) -> int:
+        """Bytes one matvec streams from the stored format (values, compact columns, slot
+        bases, slice pointers) plus x and y."""
+        e = self.values.element_size()
+        return (self.stored * e + 2 * self.col16.numel() + 4 * self.col32.numel()
+                + 4 * self.slot_base.numel() + 16 * self.col_ptr.numel() + e * 2 * self.shape[0])
 
     @property
     def bytes_per_matvec(self) -> int:

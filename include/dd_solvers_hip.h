@@ -41,9 +41,10 @@
 extern "C" {
 #endif
 
-#define DDS_ABI_VERSION 1
-/* Upper bound on the per-block partial sums written by one reduction-producing launch. */
-#define DDS_MAX_PARTIALS 2048
+#define DDS_ABI_VERSION 2
+/* Upper bound on the per-block partial sums written by one reduction-producing launch
+ * (partial buffers hold this many doubles, x16 for blocks of right-hand sides). */
+#define DDS_MAX_PARTIALS 8192
 
 typedef enum dds_dtype {
   DDS_F64 = 0,
@@ -120,11 +121,17 @@ int dds_csrmv(int val_dtype, int idx_dtype, int64_t n_rows, const void* row_ptr,
               const int32_t* col, const void* values, const void* x, void* y, int lanes,
               double* dot_partials, int* n_partials, const dds_pcg_state* gate, void* stream);
 
-/* SELL-64 y = A x: rows in slices of 64; entry j of row 64q + l at slice_ptr[q] + 64*j + l
- * (slice_ptr in elements, multiples of 64), padding entries (col = own row, value 0).
- * The format CG uses for bsr_matmul=False; built from CSR at setup. Partials and gate as
+/* SELL-64 y = A x: rows in slices of 64; entry j of row 64q + l at slot slice_ptr[q]/64 + j,
+ * lane l: values[slice_ptr[q] + 64*j + l] (slice_ptr in elements, multiples of 64), padding
+ * entries (col = own row, value 0). Column indices are compact: for slice q with
+ * col_ptr[q] >= 0 the column is slot_base[slice_ptr[q]/64 + j] + col16[col_ptr[q] + 64*j + l]
+ * (col16 read as unsigned 16-bit), so a slot costs 2 bytes per entry when its 64 columns
+ * lie within a 65536-wide window; otherwise (col_ptr[q] < 0) it is
+ * col32[~col_ptr[q] + 64*j + l]. The format CG uses for bsr_matmul=False (torch sparse-CSR
+ * @ dense in the reference, solvers.py:280); built from CSR at setup. Partials and gate as
  * for dds_bsrmv. */
-int dds_sellmv(int val_dtype, int64_t n_rows, const int64_t* slice_ptr, const int32_t* col,
+int dds_sellmv(int val_dtype, int64_t n_rows, const int64_t* slice_ptr, const int64_t* col_ptr,
+               const uint16_t* col16, const int32_t* col32, const int32_t* slot_base,
                const void* values, const void* x, void* y, double* dot_partials,
                int* n_partials, const dds_pcg_state* gate, void* stream);
 
@@ -226,6 +233,15 @@ int dds_sum_partials(const double* partials, int count, double* out, void* strea
 int dds_spd_inverse_batched(int dtype, int64_t n_mats, const int32_t* k, int64_t ld,
                             int64_t stride, void* A, int32_t* info, void* stream);
 
+/* ---- measurement ---------------------------------------------------------------------------
+ * Streaming bandwidth over n doubles (n even, 16-byte aligned arrays), grid workgroups of 256
+ * (grid <= DDS_MAX_PARTIALS): op 0 = read (y[b] = sum of a over workgroup b's elements; y holds
+ * grid doubles), op 1 = copy (y = a), op 2 = triad (y = a + s * b). The achievable-bandwidth
+ * reference for the memory-bound kernels; it replaces the clone-bandwidth measurement of
+ * experiments/memory_bandwidth.ipynb (cells 2-4). */
+int dds_stream(int op, int64_t n, const double* a, const double* b, double s, double* y,
+               int64_t grid, void* stream);
+
 /* ---- differentiation -------------------------------------------------------------------
  * grad[e] = -(1/N) exp(theta[e]) lam_e^T Khat u_e for the N^3 Q1 elements of the unit cube:
  * the coefficient gradient of J(u(theta)) given the adjoint solution lam of A lam = dJ/du.
@@ -249,7 +265,8 @@ int dds_dot_m(int64_t n, const double* a, const double* b, double* partials, int
               const dds_pcg_state* st, void* stream);
 int dds_pcg_finalize_m(int op, const double* partials, int count, dds_pcg_state* st,
                        double* norms, double* alphas, double* betas, void* stream);
-/* SELL-64 (layout of dds_sellmv) times 16 vectors; optional x . y partials (square only). */
+/* SELL-64 (slices of dds_sellmv, with plain int32 columns col[slice_ptr[q] + 64*j + l]) times
+ * 16 vectors; optional x . y partials (square only). */
 int dds_sellmm(int64_t n_rows, const int64_t* slice_ptr, const int32_t* col, const double* values,
                const double* x, double* y, double* partials, int* n_partials,
                const dds_pcg_state* st, void* stream);

@@ -270,3 +270,56 @@ def test_spd_inverse_rejects_bad_arguments():
     from dd_solvers import _lib
     with pytest.raises(_lib.HipError):
         _lib.call("dds_spd_inverse_batched", _lib.F16, 1, 1, 2, 4, 1, None, None)
+
+
+def test_stream_kernels():
+    from dd_solvers import _lib
+    n = 1 << 16
+    a = torch.rand(n, dtype=torch.float64, device=dev)
+    b = torch.rand(n, dtype=torch.float64, device=dev)
+    y = torch.empty_like(a)
+    s = _lib.stream()
+    _lib.call("dds_stream", 1, n, a.data_ptr(), None, 0.0, y.data_ptr(), 7, s)
+    assert torch.equal(y, a)
+    _lib.call("dds_stream", 2, n, a.data_ptr(), b.data_ptr(), 0.5, y.data_ptr(), 3, s)
+    assert torch.allclose(y, a + 0.5 * b, rtol=1e-15, atol=0)
+    part = torch.empty(5, dtype=torch.float64, device=dev)
+    _lib.call("dds_stream", 0, n, a.data_ptr(), None, 0.0, part.data_ptr(), 5, s)
+    assert abs(part.sum().item() - a.sum().item()) <= 1e-9 * a.sum().item()
+    with pytest.raises(_lib.HipError):
+        _lib.call("dds_stream", 1, n + 1, a.data_ptr(), None, 0.0, y.data_ptr(), 4, s)
+
+
+def test_sell_mixed_compact_and_wide_slices():
+    """Rows of the first half each have one scattered far column (their slots span more
+    than 65535 columns, so those slices keep int32 columns), the rest is banded (16-bit
+    offsets); with the fused x . y partials."""
+    import scipy.sparse as sps
+    from dd_solvers import _lib
+    from dd_solvers.solvers import _SELLOperator
+    n = 150_000
+    rng = np.random.default_rng(2)
+    rows, cols = [], []
+    for off in (-1, 0, 1):
+        r = np.arange(max(0, -off), min(n, n - off))
+        rows.append(r)
+        cols.append(r + off)
+    far = np.arange(n // 2)
+    rows.append(far)
+    cols.append((far * 7919 + 12345) % n)
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    A = sps.csr_matrix((rng.standard_normal(r.shape[0]), (r, c)), shape=(n, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr), torch.as_tensor(A.indices),
+                                 torch.as_tensor(A.data), A.shape).to(dev)
+    op = _SELLOperator(At)
+    assert 0.3 < op.wide_fraction < 0.7
+    x = torch.as_tensor(rng.standard_normal(n), device=dev)
+    y = torch.empty_like(x)
+    partials = torch.zeros(_lib.MAX_PARTIALS, dtype=torch.float64, device=dev)
+    cnt = op._enqueue_matvec(x, y, partials, None)
+    ref = A @ x.cpu().numpy()
+    assert np.allclose(y.cpu().numpy(), ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max())
+    assert np.isclose(partials[:cnt].sum().item(), float(ref @ x.cpu().numpy()), rtol=1e-12)

@@ -339,3 +339,34 @@ def test_mfma_autotuner_candidates():
     assert all(w in (1, 2, 4, 8) for _, w in c)
     assert all(g == 0 or g * w < 10_000 for g, w in c)
     assert mfma_candidates(10, 256) == [(0, 1), (0, 2), (0, 4), (0, 8)]
+
+
+def test_sell_compact_column_format():
+    """The compact SELL columns decode to the int32 columns: 16-bit offsets from a per-slot
+    base where a slot spans < 65536 columns, int32 for the slices holding a wider slot."""
+    import torch
+    from dd_solvers.solvers import _SELLOperator
+    rng = np.random.default_rng(0)
+    width = torch.tensor([3, 1, 4, 2, 5])
+    n_slots = int(width.sum())
+    col = rng.integers(0, 60000, size=(n_slots, 64))
+    col[4] += 200_000          # slot 4 (slice 2) lies far away but narrow: still compact
+    col[5, 7] = 900_000        # slot 5 (slice 2) spans > 65535: slice 2 goes wide
+    col[10, :] = 70_000 + np.arange(64)
+    op = object.__new__(_SELLOperator)
+    op.col = torch.as_tensor(col.reshape(-1), dtype=torch.int32)
+    op.device = torch.device("cpu")
+    op._compact_columns(width)
+    cp = op.col_ptr.numpy()
+    assert (cp < 0).tolist() == [False, False, True, False, False]
+    d16 = op.col16.numpy().view(np.uint16).astype(np.int64)
+    c32 = op.col32.numpy()
+    base = op.slot_base.numpy()
+    slot0 = np.concatenate([[0], np.cumsum(width.numpy())])
+    for q in range(5):
+        for j in range(int(width[q])):
+            slot = slot0[q] + j
+            got = (base[slot] + d16[cp[q] + 64 * j + np.arange(64)]) if cp[q] >= 0 else \
+                c32[~cp[q] + 64 * j + np.arange(64)]
+            assert np.array_equal(got, col[slot]), (q, j)
+    assert d16.shape[0] == 64 * (n_slots - 4) and c32.shape[0] == 64 * 4
