@@ -2,21 +2,24 @@
 // A_s^-1 (symmetric) is stored as its lower triangle of 64 x 64 tiles, so every
 // off-diagonal tile is read from HBM once and used twice (y_I += T x_J and y_J += T^T x_I).
 //
-// Storage of subdomain s (k = k_s unknowns, T = ceil(k/64) tile rows, last block rL rows):
-//   tile (I, J), J <= I, R_I = 64 rows (rL for the last tile row), chunks of V = 16/sizeof
-//   elements: chunk j of row r holds columns [jV, jV+V) at tile_off + (j*R_I + r)*V, so a
-//   wave loads one chunk of all its rows as one coalesced 1 KiB (64 x 16 B) access.
-//   tile_off(I, J) = 4096*(I(I+1)/2 + J)                         for I < T-1
-//                  = 4096*(T-1)T/2 + 64*rL*J                     for I = T-1
-//   Off-diagonal tiles have 64 chunk columns' worth of data (NCH = 64/V chunks); the
-//   diagonal tiles are stored in full (both triangles), the last one with ceil(rL/V)
-//   chunks. Padding entries are zero.
+// Storage of subdomain s (k = k_s unknowns, T = ceil(k/64) tile rows, last block rL rows),
+// in chunks of V = 16/sizeof elements (NCH = 64/V chunk columns per 64-wide tile):
+//   off-diagonal tile (I, J), J < I, R_I rows (64, or rL for the last tile row): chunk j of
+//     row r holds columns [jV, jV+V) at tile_off + (j*R_I + r)*V, so a wave loads one chunk
+//     of all its rows as one coalesced 1 KiB (64 x 16 B) access;
+//   diagonal tile (I, I): only its lower triangle, with the diagonal halved: chunk column j
+//     holds rows r = jV .. R_I-1 at tile_off + V*(j*R_I - V*j*(j-1)/2) + (r - jV)*V, the
+//     entries above the diagonal zero (ceil(rL/V) chunk columns in the last tile). Its row
+//     product gives (L + D/2) x and its transposed product (L^T + D/2) x, which add to A x.
+//   D64 = 2048 + 32 V elements (a full diagonal tile), and
+//   tile_off(I, J) = 4096*I(I-1)/2 + I*D64 + 4096*J                       for I < T-1
+//                  = 4096*(T-1)(T-2)/2 + (T-1)*D64 + 64*rL*J                for I = T-1
+//   Padding entries are zero.
 //
 // Work item (s, p): one wavefront handles tile rows I1 = p and I2 = T-1-p (one row if they
 // coincide), so every item touches T+1 tiles. It writes a partial result vector Yp[s][p]
-// (length 64*T, blocks 0..I2 written); the partial vectors of a subdomain are added later
-// in item order (dds_overlap_sum with the store's lists), which keeps the result
-// deterministic without atomics.
+// (length 64*T, blocks 0..I2 written); sym_combine_kernel adds the partial vectors of a
+// subdomain in item order, which keeps the result deterministic without atomics.
 #include "common.hpp"
 
 namespace dds {
@@ -28,6 +31,7 @@ struct SymShape {
   static constexpr int NCH = 64 / V;              // chunks per 64-wide tile row
   static constexpr int NH = NCH / 2;              // chunks per column half (32 columns)
   static constexpr int G = NH >= 16 ? 1 : 16 / NH;
+  static constexpr int64_t D64 = 2048 + 32 * V;   // elements of a full diagonal tile
 };
 
 // LDS position (in chunks) of chunk j (of a column half) of tile row r: XOR swizzle so that
@@ -40,16 +44,21 @@ __device__ __forceinline__ int sym_slot(int r, int j) {
 
 // One 64 x 64 tile (R rows, nch chunk columns) at tp, processed by column halves:
 //   row     += sum_c T[lane][c] * xjs[c]                     (lane = row)
-//   returns sum_r T[r][c] * xis[r] for column c = lane          (if transposed)
+//   returns sum_r T[r][c] * xis[r] for column c = lane
 // Each half (32 columns) is loaded as NH coalesced 1 KiB accesses; for the transposed
 // product it is staged in LDS, lanes 0-31 sum rows 0-31 and lanes 32-63 rows 32-63 of
 // column lane % 32, and one cross-lane exchange adds the two row halves.
+// diag: a diagonal tile in its packed lower-triangular layout (rows r >= jV of chunk j).
+// Every chunk address is a wave-uniform base plus lane * V. The loads are predicated per
+// lane (exec mask) rather than selected after loading: a select right after a load would
+// make the compiler wait for each load before issuing the next.
 template <typename TM, typename TA>
-__device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool transposed,
+__device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool diag,
                                          const TA* xjs, const TA* xis, TM* lds, int lane,
                                          TA& row) {
   using S = SymShape<TM>;
   TA col[2] = {0, 0};
+  const TM* lp = tp + (int64_t)lane * S::V;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     if (h * S::NH >= nch) break;
@@ -57,8 +66,12 @@ __device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool tran
 #pragma unroll
     for (int j = 0; j < S::NH; ++j) {
       const int jj = h * S::NH + j;
-      if (jj < nch && lane < R) {
-        c[j] = load16_nt(tp + ((int64_t)jj * R + lane) * S::V);
+      if (jj < nch && lane < R && (!diag || lane >= jj * S::V)) {
+        const int64_t ub =
+            diag ? (int64_t)S::V * ((int64_t)jj * R - (int64_t)S::V * jj * (jj - 1) / 2) -
+                       (int64_t)jj * S::V * S::V
+                 : (int64_t)jj * R * S::V;
+        c[j] = load16_nt(lp + ub);
       } else {
 #pragma unroll
         for (int e = 0; e < S::V; ++e) c[j].v[e] = TM{};
@@ -73,7 +86,7 @@ __device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool tran
           row += static_cast<TA>(widen(c[j].v[e])) * xjs[jj * S::V + e];
       }
     }
-    if (transposed) {
+    {
       __syncthreads();  // previous readers of lds are done
 #pragma unroll
       for (int j = 0; j < S::NH; ++j)
@@ -140,8 +153,8 @@ __global__ __launch_bounds__(64) void sym_apply_kernel(
     return static_cast<TA>(x[g]);
   };
   auto tile_off = [&](int I, int J) -> int64_t {
-    if (I < T - 1) return 4096 * ((int64_t)I * (I + 1) / 2 + J);
-    return 4096 * ((int64_t)(T - 1) * T / 2) + (int64_t)64 * rL * J;
+    if (I < T - 1) return 4096 * ((int64_t)I * (I - 1) / 2) + (int64_t)I * S::D64 + 4096 * (int64_t)J;
+    return 4096 * ((int64_t)(T - 1) * (T - 2) / 2) + (int64_t)(T - 1) * S::D64 + (int64_t)64 * rL * J;
   };
   auto rows_of = [&](int I) { return I < T - 1 ? 64 : rL; };
   auto chunks_of = [&](int J) { return J < T - 1 ? S::NCH : (rL + S::V - 1) / S::V; };
@@ -159,11 +172,26 @@ __global__ __launch_bounds__(64) void sym_apply_kernel(
     xjs[lane] = xj;
     if (J < I2) xj = gather(J + 1);
     __syncthreads();
-    TA contrib = tile_apply<TM, TA>(M + tile_off(I2, J), rows_of(I2), nch, J < I2, xjs, xi2s,
-                                    lds_tile, lane, row2);
-    if (two && J <= I1)
-      contrib += tile_apply<TM, TA>(M + tile_off(I1, J), rows_of(I1), nch, J < I1, xjs, xi1s,
-                                    lds_tile, lane, row1);
+    // tile (I2, J), then (I1, J): the transposed product of an off-diagonal tile goes to
+    // block J, that of a diagonal tile to its own block
+    TA contrib = 0;
+    const int n_rows_here = (two && J <= I1) ? 2 : 1;
+#pragma unroll 1
+    for (int w = 0; w < n_rows_here; ++w) {
+      const int I = w == 0 ? I2 : I1;
+      const bool diag = J == I;
+      TA r = w == 0 ? row2 : row1;
+      const TA c = tile_apply<TM, TA>(M + tile_off(I, J), rows_of(I), nch, diag, xjs,
+                                      w == 0 ? xi2s : xi1s, lds_tile, lane, r);
+      if (diag)
+        r += c;
+      else
+        contrib += c;
+      if (w == 0)
+        row2 = r;
+      else
+        row1 = r;
+    }
     if (J < I2) {
       if (two && J == I1)
         hold1 = contrib;

@@ -950,8 +950,11 @@ _SYM_INDEX_CACHE: dict = {}
 
 
 def _sym_index(k: int, V: int) -> np.ndarray:
-    """Source index (row * k + col, -1 for padding) of every element of the symmetric tile
-    layout of a k x k matrix with 16-byte chunks of V elements (csrc/local_sym.hip)."""
+    """Source of every element of the symmetric tile layout of a k x k matrix with 16-byte
+    chunks of V elements (csrc/local_sym.hip): row * k + col, -1 for padding (zero), and
+    -2 - (row * k + col) for a diagonal entry, which is stored halved. Off-diagonal tiles
+    are whole 64-wide chunk columns; a diagonal tile keeps, in chunk column j, only the
+    rows r >= jV, with the entries above the diagonal zero."""
     key = (k, V)
     if key in _SYM_INDEX_CACHE:
         return _SYM_INDEX_CACHE[key]
@@ -963,12 +966,21 @@ def _sym_index(k: int, V: int) -> np.ndarray:
         R = 64 if I < T - 1 else rL
         for J in range(I + 1):
             nch = nch_full if J < T - 1 else -(-rL // V)
-            j = np.arange(nch)[:, None, None]
-            r = np.arange(R)[None, :, None]
-            e = np.arange(V)[None, None, :]
-            row = 64 * I + r + 0 * j + 0 * e
-            col = 64 * J + j * V + e + 0 * r
-            parts.append(np.where(col < k, row * k + col, -1).reshape(-1))
+            if J < I:
+                j = np.arange(nch)[:, None, None]
+                r = np.arange(R)[None, :, None]
+                e = np.arange(V)[None, None, :]
+                row = 64 * I + r + 0 * j + 0 * e
+                col = 64 * J + j * V + e + 0 * r
+                parts.append(np.where(col < k, row * k + col, -1).reshape(-1))
+                continue
+            for j in range(nch):
+                r = np.arange(j * V, R)[:, None]
+                c = j * V + np.arange(V)[None, :]
+                row, col = 64 * I + r, 64 * J + c
+                flat = row * k + col
+                out = np.where((col < k) & (c <= r), flat, -1)
+                parts.append(np.where(c == r, -2 - flat, out).reshape(-1))
     idx = np.concatenate(parts).astype(np.int64)
     _SYM_INDEX_CACHE[key] = idx
     return idx
@@ -983,7 +995,12 @@ def _sym_plan(sizes: np.ndarray):
     ypos = np.concatenate([[0], np.cumsum(P * 64 * T)]).astype(np.int64)
     items = np.concatenate([np.stack([np.full(int(P[s]), s), np.arange(int(P[s]))], 1)
                             for s in range(sizes.shape[0])]).astype(np.int32)
-    return items, ypos
+    # longest items first (tiles touched: T + 1, or T - p for the middle row of an odd T),
+    # so the last workgroups to start are the short ones
+    Ts = T[items[:, 0]]
+    p = items[:, 1]
+    work = np.where(p == Ts - 1 - p, Ts - p, Ts + 1)
+    return items[np.argsort(-work, kind="stable")], ypos
 
 
 _MFMA_INDEX_CACHE: dict = {}
@@ -1295,8 +1312,11 @@ class SchwarzOperator(SparseSolver):
                     if k not in sym_idx_dev:
                         sym_idx_dev[k] = torch.as_tensor(sym_idx[k], device=dev)
                     src = sym_idx_dev[k]
-                    packed = torch.where(src >= 0, m.reshape(-1)[src.clamp(min=0)],
-                                         torch.zeros((), dtype=pi, device=dev))
+                    half = src <= -2
+                    at = torch.where(half, -2 - src, src).clamp(min=0)
+                    vals = m.reshape(-1)[at]
+                    packed = torch.where(src >= 0, vals, torch.where(half, 0.5 * vals,
+                                                                     torch.zeros((), dtype=pi, device=dev)))
                     mats[int(mat_off[i]):int(mat_off[i + 1])] = packed.to(store_dtype)
                 else:
                     # column-major with padded leading dimension: element (a, b) at b*ld + a
@@ -1312,6 +1332,9 @@ class SchwarzOperator(SparseSolver):
             groups = -(-RB // 4)
             items = np.concatenate([np.stack([np.full(int(groups[s_]), s_), np.arange(int(groups[s_]))], 1)
                                     for s_ in range(n_s)]).astype(np.int32)
+            # longest items first: work = row blocks in the group x column pairs
+            work = np.minimum(4, RB[items[:, 0]] - 4 * items[:, 1]) * KP[items[:, 0]]
+            items = items[np.argsort(-work, kind="stable")]
             b_off = np.concatenate([[0], np.cumsum(KP * NT * 512)]).astype(np.int64)
             tiles = np.zeros((0, 2), dtype=np.int32)
             ypos = None

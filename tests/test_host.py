@@ -223,16 +223,26 @@ def test_kmeans_decomposition_uneven():
 
 def _unpack_tile(packed, k, V, I, J):
     """Tile (I, J) of the packed layout as a dense (R, 64) block, following the offsets
-    documented in csrc/local_sym.hip."""
+    documented in csrc/local_sym.hip (diagonal tiles: lower triangle, diagonal halved)."""
     T = -(-k // 64)
     rL = k - 64 * (T - 1)
     R = 64 if I < T - 1 else rL
     nch = 64 // V if J < T - 1 else -(-rL // V)
-    off = 4096 * (I * (I + 1) // 2 + J) if I < T - 1 else 4096 * ((T - 1) * T // 2) + 64 * rL * J
+    D64 = 2048 + 32 * V
+    if I < T - 1:
+        off = 4096 * (I * (I - 1) // 2) + I * D64 + 4096 * J
+    else:
+        off = 4096 * ((T - 1) * (T - 2) // 2) + (T - 1) * D64 + 64 * rL * J
     out = np.zeros((R, 64))
     for j in range(nch):
         for r in range(R):
-            out[r, j * V:(j + 1) * V] = packed[off + (j * R + r) * V: off + (j * R + r + 1) * V]
+            if J < I:
+                o = off + (j * R + r) * V
+            elif r >= j * V:
+                o = off + V * (j * R - V * j * (j - 1) // 2) + (r - j * V) * V
+            else:
+                continue
+            out[r, j * V:(j + 1) * V] = packed[o:o + V]
     return out
 
 
@@ -245,13 +255,24 @@ def test_sym_layout_and_item_plan(k, V):
     B = rng.standard_normal((k, k))
     A = B + B.T
     idx = _sym_index(k, V)
-    packed = np.where(idx >= 0, A.reshape(-1)[np.maximum(idx, 0)], 0.0)
+    flat = A.reshape(-1)
+    packed = np.where(idx >= 0, flat[np.maximum(idx, 0)],
+                      np.where(idx <= -2, 0.5 * flat[np.maximum(-2 - idx, 0)], 0.0))
     T = -(-k // 64)
-    # every stored tile matches the matrix (lower triangle, full diagonal tiles)
+    rL = k - 64 * (T - 1)
+    D64 = 2048 + 32 * V
+    nchL = -(-rL // V)
+    diag_last = V * (nchL * rL - V * nchL * (nchL - 1) // 2)
+    expect = 4096 * ((T - 1) * (T - 2) // 2) + (T - 1) * D64 + 64 * rL * (T - 1) + diag_last
+    assert idx.shape[0] == expect
+    # every stored tile matches the matrix: off-diagonal tiles whole, diagonal tiles as their
+    # lower triangle with the diagonal halved
     for I in range(T):
         for J in range(I + 1):
             t = _unpack_tile(packed, k, V, I, J)
             blk = A[64 * I:64 * I + t.shape[0], 64 * J:64 * J + 64]
+            if J == I:
+                blk = np.tril(blk, -1) + np.diag(np.diag(blk)) / 2
             assert np.array_equal(t[:, :blk.shape[1]], blk)
             assert not t[:, blk.shape[1]:].any()
     # work items (pairs of tile rows) + the structured combine reproduce A x
@@ -277,6 +298,8 @@ def test_sym_layout_and_item_plan(k, V):
                 rows[I] += t @ xp[64 * J:64 * J + 64]
                 if J < I:
                     contrib += t.T @ xp[64 * I:64 * I + 64]
+                else:
+                    rows[I] += t.T @ xp[64 * I:64 * I + 64]
             if J < I2:
                 if I1 != I2 and J == I1:
                     hold = contrib
