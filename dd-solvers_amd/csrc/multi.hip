@@ -166,19 +166,31 @@ __global__ __launch_bounds__(kBlock) void dot_m_kernel(int64_t n, const double* 
   write_partials_pairs(acc, partials, scratch);
 }
 
-// One wavefront per column: fixed-order sum of partials[i][j], then the scalar update.
+// Fixed-order sum of partials[i][j] per column j, then the scalar update of column j.
+// Thread t reads partials[t + 1024 k] (coalesced), so it always sees column t % 16; the
+// four lanes of a wave with one column are added by two butterflies, the 16 waves through
+// LDS in wave order, and thread j finishes column j.
 __global__ __launch_bounds__(M * 64) void finalize_m_kernel(int op, const double* __restrict__ partials,
                                                             int count, dds_pcg_state* st,
                                                             double* norms, double* alphas,
                                                             double* betas) {
-  const int j = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
+  __shared__ double red[M][M];  // [wave][column]
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  double acc = 0.0;
+  const int64_t total = (int64_t)count * M;
+  for (int64_t i = t; i < total; i += M * 64) acc += partials[i];
+  acc += __shfl_xor(acc, 16, 64);
+  acc += __shfl_xor(acc, 32, 64);
+  if (lane < M) red[t >> 6][lane] = acc;
+  __syncthreads();
+  if (t >= M) return;
+  const int j = t;
   dds_pcg_state* s = st + j;
   if (s->done) return;
-  double acc = 0.0;
-  for (int i = lane; i < count; i += 64) acc += partials[(int64_t)i * M + j];
-  const double sum = wave_sum(acc);
-  if (lane != 0) return;
+  double sum = 0.0;
+#pragma unroll
+  for (int w = 0; w < M; ++w) sum += red[w][j];
   auto check = [&](double rr) {
     s->rr = rr;
     const double res = sqrt(rr);
@@ -265,29 +277,30 @@ __global__ __launch_bounds__(64 * kSellWaves) void sell_m_kernel(int64_t n_rows,
   if (partials != nullptr) write_partials_pairs(dot, partials, scratch);
 }
 
-// CSR (long rows, e.g. R0A) times 16 vectors: one wavefront per row, lane groups of 8
-// over the row's entries.
-__global__ __launch_bounds__(kBlock) void csr_rows_m_kernel(int64_t n_rows, const int64_t* __restrict__ row_ptr,
-                                                            const int32_t* __restrict__ col,
-                                                            const double* __restrict__ val,
-                                                            const double* __restrict__ x,
-                                                            double* __restrict__ y,
-                                                            const dds_pcg_state* __restrict__ st) {
+// CSR (long rows, e.g. R0A = R_C A, thousands of entries per row) times 16 vectors: one
+// workgroup of 1024 threads per row, 128 groups of 8 lanes over the row's entries, then a
+// fixed-order reduction (butterflies within waves, waves in order through LDS).
+constexpr int kRowThreads = 1024;
+
+__global__ __launch_bounds__(kRowThreads) void csr_rows_m_kernel(int64_t n_rows, const int64_t* __restrict__ row_ptr,
+                                                                 const int32_t* __restrict__ col,
+                                                                 const double* __restrict__ val,
+                                                                 const double* __restrict__ x,
+                                                                 double* __restrict__ y,
+                                                                 const dds_pcg_state* __restrict__ st) {
+  __shared__ double scratch[(kRowThreads / 64) * M];
   if (all_done(st)) return;
-  const int lane = threadIdx.x & 63;
-  const int cp = lane & (kPairs - 1);
-  const int g = lane >> 3;
-  const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-  if (row >= n_rows) return;
+  const int64_t row = blockIdx.x;
+  const int cp = threadIdx.x & (kPairs - 1);
+  const int g = threadIdx.x >> 3;
   double2 acc = {0.0, 0.0};
-  for (int64_t p = row_ptr[row] + g; p < row_ptr[row + 1]; p += 8) {
+  for (int64_t p = row_ptr[row] + g; p < row_ptr[row + 1]; p += kRowThreads / kPairs) {
     const double v = val[p];
     const double2 xv = ld2(x + (int64_t)col[p] * M + 2 * cp);
     acc.x += v * xv.x;
     acc.y += v * xv.y;
   }
-  acc = wave_pair_sum(acc);
-  if (lane < kPairs) st2(y + row * M + 2 * cp, acc);
+  write_partials_pairs(acc, y, scratch);  // block `row` writes y[row * M + j]
 }
 
 // Y = S X for a dense symmetric n x n S: one wavefront per row.
@@ -408,8 +421,10 @@ struct MfmaTraits<bf16_t> {
 // the gathered rows). Pass 2: one thread per B-operand chunk (pair K2, lane l): it gathers
 // the 8 values X[32K2 + 16t2 + 4(l/16) + i][l%16] (t2 < 2, i < 4), scales them, splits
 // each into NT terms and writes NT 16-byte chunks.
+constexpr int kPackThreads = 1024;
+
 template <typename TM>
-__global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restrict__ sub_ptr,
+__global__ __launch_bounds__(kPackThreads) void pack_x_kernel(const int64_t* __restrict__ sub_ptr,
                                                         const int32_t* __restrict__ sub_idx,
                                                         const int64_t* __restrict__ b_off,
                                                         const double* __restrict__ x,
@@ -418,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restric
                                                         const dds_pcg_state* __restrict__ st) {
   using Tr = MfmaTraits<TM>;
   constexpr int NT = Tr::NT;
-  __shared__ float smax[(kBlock / 64) * M];
+  __shared__ float smax[(kPackThreads / 64) * M];
   __shared__ float sig[M];
   if (all_done(st)) return;
   const int s = blockIdx.x;
@@ -427,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restric
   auto row_of = [&](int a) -> int64_t { return sub_idx != nullptr ? (int64_t)sub_idx[base + a] : base + a; };
   const int cp = threadIdx.x & (kPairs - 1);
   float m0 = 0.f, m1 = 0.f;
-  for (int e = threadIdx.x; e < k * kPairs; e += kBlock) {
+  for (int e = threadIdx.x; e < k * kPairs; e += kPackThreads) {
     const double2 v = ld2(x + row_of(e >> 3) * M + 2 * cp);
     m0 = fmaxf(m0, fabsf((float)v.x));
     m1 = fmaxf(m1, fabsf((float)v.y));
@@ -445,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restric
   __syncthreads();
   if (threadIdx.x < M) {
     float v = 0.f;
-    for (int ww = 0; ww < kBlock / 64; ++ww) v = fmaxf(v, smax[ww * M + threadIdx.x]);
+    for (int ww = 0; ww < kPackThreads / 64; ++ww) v = fmaxf(v, smax[ww * M + threadIdx.x]);
     v = v > 0.f ? v : 1.f;
     sig[threadIdx.x] = v;
     sigma[(int64_t)s * M + threadIdx.x] = v;
@@ -453,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void pack_x_kernel(const int64_t* __restric
   __syncthreads();
   const int KP = (k + 31) / 32;
   TM* out = bp + b_off[s];
-  for (int ch = threadIdx.x; ch < KP * 64; ch += kBlock) {
+  for (int ch = threadIdx.x; ch < KP * 64; ch += kPackThreads) {
     const int K2 = ch >> 6, l = ch & 63;
     const int c = l & 15, r4 = 4 * (l >> 4);
     const double inv = 1.0 / (double)sig[c];
@@ -644,8 +659,9 @@ extern "C" int dds_csrmm_rows(int64_t n_rows, const int64_t* row_ptr, const int3
                               const dds_pcg_state* st, void* stream) {
   DDS_REQUIRE(n_rows >= 0 && row_ptr && col && values && x && y, "dds_csrmm_rows: bad arguments");
   if (n_rows == 0) return DDS_OK;
-  hipLaunchKernelGGL(csr_rows_m_kernel, dim3((unsigned)ceil_div(n_rows, kBlock / 64)), dim3(kBlock),
-                     0, as_stream(stream), n_rows, row_ptr, col, values, x, y, st);
+  DDS_REQUIRE(n_rows <= INT32_MAX, "dds_csrmm_rows: too many rows");
+  hipLaunchKernelGGL(csr_rows_m_kernel, dim3((unsigned)n_rows), dim3(kRowThreads), 0,
+                     as_stream(stream), n_rows, row_ptr, col, values, x, y, st);
   DDS_LAUNCHED();
   return DDS_OK;
 }
@@ -694,10 +710,10 @@ extern "C" int dds_mfma_pack_x(int mat_dtype, int64_t n_s, const int64_t* sub_pt
   if (n_s == 0) return DDS_OK;
   hipStream_t s = as_stream(stream);
   if (mat_dtype == DDS_F16) {
-    hipLaunchKernelGGL(pack_x_kernel<_Float16>, dim3((unsigned)n_s), dim3(kBlock), 0, s, sub_ptr,
+    hipLaunchKernelGGL(pack_x_kernel<_Float16>, dim3((unsigned)n_s), dim3(kPackThreads), 0, s, sub_ptr,
                        sub_idx, b_off, x, static_cast<_Float16*>(bp), sigma, st);
   } else if (mat_dtype == DDS_BF16) {
-    hipLaunchKernelGGL(pack_x_kernel<bf16_t>, dim3((unsigned)n_s), dim3(kBlock), 0, s, sub_ptr,
+    hipLaunchKernelGGL(pack_x_kernel<bf16_t>, dim3((unsigned)n_s), dim3(kPackThreads), 0, s, sub_ptr,
                        sub_idx, b_off, x, static_cast<bf16_t*>(bp), sigma, st);
   } else {
     set_error("dds_mfma_pack_x: storage must be F16 or BF16 (got %d)", mat_dtype);
