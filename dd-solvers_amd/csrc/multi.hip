@@ -544,18 +544,39 @@ __global__ void mfma_apply_kernel(int64_t n_items, const int32_t* __restrict__ i
       for (int t = 0; t < NT; ++t) c[r][t] = float4_t{0.f, 0.f, 0.f, 0.f};
     const TM* As = A + a_off[s];
     const TM* Bs = bp + b_off[s];
-    for (int K2 = 0; K2 < KP; ++K2) {
-      Vec16<TM> bv[NT];
+    if (nrb == 4) {
+      // full group: all operand loads of a step are issued before its MFMAs (no branch
+      // between a load and its use), two steps per iteration
+#pragma unroll 2
+      for (int K2 = 0; K2 < KP; ++K2) {
+        Vec16<TM> bv[NT], av[4];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
+        for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (r < nrb) {
-          const Vec16<TM> av = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
+        for (int r = 0; r < 4; ++r)
+          av[r] = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
           for (int t = 0; t < NT; ++t) {
-            c[r][t] = mfma16<TM>(av.v, bv[t].v, c[r][t]);
-            c[r][t] = mfma16<TM>(av.v + 4, bv[t].v + 4, c[r][t]);
+            c[r][t] = mfma16<TM>(av[r].v, bv[t].v, c[r][t]);
+            c[r][t] = mfma16<TM>(av[r].v + 4, bv[t].v + 4, c[r][t]);
+          }
+      }
+    } else {
+      for (int K2 = 0; K2 < KP; ++K2) {
+        Vec16<TM> bv[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (r < nrb) {
+            const Vec16<TM> av = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              c[r][t] = mfma16<TM>(av.v, bv[t].v, c[r][t]);
+              c[r][t] = mfma16<TM>(av.v + 4, bv[t].v + 4, c[r][t]);
+            }
           }
         }
       }
