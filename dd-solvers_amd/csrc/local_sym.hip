@@ -34,12 +34,14 @@ struct SymShape {
   static constexpr int64_t D64 = 2048 + 32 * V;   // elements of a full diagonal tile
 };
 
-// LDS position (in chunks) of chunk j (of a column half) of tile row r: XOR swizzle so that
-// the chunk writes (lane = row) and the column reads (lane = column) avoid bank conflicts.
+// LDS position (in chunks) of chunk j (of a column half) of tile row r: rows padded by one
+// chunk, so the chunk writes (lane = row) and the column reads (lane = column) spread over
+// the banks, and a lane's column read steps by a constant stride from row to row
+// (immediate offsets, no per-row address arithmetic).
 template <typename TM>
 __device__ __forceinline__ int sym_slot(int r, int j) {
   using S = SymShape<TM>;
-  return r * S::NH + (j ^ ((r / S::G) % S::NH));
+  return r * (S::NH + 1) + j;
 }
 
 // One 64 x 64 tile (R rows, nch chunk columns) at tp, processed by column halves:
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(64) void sym_apply_kernel(
     const dds_pcg_state* __restrict__ gate) {
   using S = SymShape<TM>;
   if (gate != nullptr && gate->done) return;
-  __shared__ __align__(16) TM lds_tile[64 * 32];
+  __shared__ __align__(16) TM lds_tile[64 * (32 + SymShape<TM>::V)];  // padded rows
   __shared__ __align__(16) TA xjs[64];
   __shared__ __align__(16) TA xi1s[64];
   __shared__ __align__(16) TA xi2s[64];
