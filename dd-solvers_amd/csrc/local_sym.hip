@@ -79,15 +79,18 @@ __device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool diag
         for (int e = 0; e < S::V; ++e) c[j].v[e] = TM{};
       }
     }
+    // four independent accumulators (shorter dependent FMA chains), added in a fixed order
+    TA ra[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < S::NH; ++j) {
       const int jj = h * S::NH + j;
       if (jj < nch) {
 #pragma unroll
         for (int e = 0; e < S::V; ++e)
-          row += static_cast<TA>(widen(c[j].v[e])) * xjs[jj * S::V + e];
+          ra[(j * S::V + e) & 3] += static_cast<TA>(widen(c[j].v[e])) * xjs[jj * S::V + e];
       }
     }
+    row += (ra[0] + ra[1]) + (ra[2] + ra[3]);
     {
       __syncthreads();  // previous readers of lds are done
 #pragma unroll
@@ -99,20 +102,21 @@ __device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool diag
       const int ec = cc % S::V;
       const int r0 = lane & 32;
       const int nr = R - r0 < 32 ? R - r0 : 32;
-      TA acc = 0;
+      TA q0 = 0, q1 = 0, q2 = 0, q3 = 0;
       int t = 0;
       for (; t + 3 < nr; t += 4) {
         const TA a0 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec]));
         const TA a1 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 1, jc) * S::V + ec]));
         const TA a2 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 2, jc) * S::V + ec]));
         const TA a3 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 3, jc) * S::V + ec]));
-        acc += a0 * xis[r0 + t];
-        acc += a1 * xis[r0 + t + 1];
-        acc += a2 * xis[r0 + t + 2];
-        acc += a3 * xis[r0 + t + 3];
+        q0 += a0 * xis[r0 + t];
+        q1 += a1 * xis[r0 + t + 1];
+        q2 += a2 * xis[r0 + t + 2];
+        q3 += a3 * xis[r0 + t + 3];
       }
       for (; t < nr; ++t)
-        acc += static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec])) * xis[r0 + t];
+        q0 += static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec])) * xis[r0 + t];
+      TA acc = (q0 + q1) + (q2 + q3);
       acc += __shfl_xor(acc, 32, 64);
       col[h] = acc;
     }
@@ -167,12 +171,22 @@ __global__ __launch_bounds__(64) void sym_apply_kernel(
   xi1s[lane] = gather(I1);
   xi2s[lane] = gather(I2);
   TA row1 = 0, row2 = 0, hold1 = 0;
+  // two-stage gather: the index of block J + 2 and the value of block J + 1 are in flight
+  // while block J is processed, so no wait on an index load precedes the tile loads
+  auto index_of = [&](int J) -> int64_t {
+    const int a = 64 * J + lane;
+    if (J > I2 || a >= k) return -1;
+    return sub_idx != nullptr ? (int64_t)sub_idx[base + a] : base + a;
+  };
   TA xj = gather(0);
+  int64_t g_next = index_of(1);
   for (int J = 0; J <= I2; ++J) {
     const int nch = chunks_of(J);
     __syncthreads();  // readers of xjs from the previous J are done
     xjs[lane] = xj;
-    if (J < I2) xj = gather(J + 1);
+    const int64_t g_cur = g_next;
+    g_next = index_of(J + 2);
+    xj = g_cur >= 0 ? static_cast<TA>(x[g_cur]) : static_cast<TA>(0);
     __syncthreads();
     // tile (I2, J), then (I1, J): the transposed product of an off-diagonal tile goes to
     // block J, that of a diagonal tile to its own block
