@@ -254,6 +254,47 @@ def stream_peaks(dev):
     return out
 
 
+# The reference's own batched-GEMV benchmark (experiments/benchmark_batched_gemv.py: n =
+# 2^27 / k^2 column-major k x k matrices, one vector each) on an A100 80GB: the best of its
+# two kernels, min over repetitions, from results/benchmark_batched_gemv.csv (line numbers).
+REF_GEMV_A100_MS = {
+    (3, "float64", "float64"): (1.077, 35), (10, "float64", "float64"): (0.8179, 183),
+    (20, "float64", "float64"): (0.7372, 393), (48, "float64", "float64"): (0.6623, 981),
+    (128, "float64", "float64"): (0.6129, 2661),
+    (3, "float16", "float64"): (0.6003, 38), (10, "float16", "float64"): (0.3248, 186),
+    (20, "float16", "float64"): (0.2525, 396), (48, "float16", "float64"): (0.2118, 984),
+    (128, "float16", "float64"): (0.1916, 2664),
+    (3, "float32", "float32"): (0.5558, 23), (20, "float32", "float32"): (0.3751, 381),
+    (128, "float32", "float32"): (0.3318, 2649),
+    (3, "float16", "float32"): (0.4034, 26), (20, "float16", "float32"): (0.2212, 384),
+    (128, "float16", "float32"): (0.1834, 2652),
+}
+
+
+def gemv_reference_shapes(dev):
+    """dd_solvers.gemv_strided_batched (the drop-in op) on the reference benchmark's shapes,
+    best of its two kernels (use_shared_memory False / True), beside the A100 times."""
+    import torch
+    import dd_solvers as dds
+
+    out = []
+    for (k, ad, bd), (ref_ms, line) in REF_GEMV_A100_MS.items():
+        n = (1 << 27) // (k * k)
+        A = torch.randn((n, k, k), device=dev, dtype=getattr(torch, ad)).transpose(1, 2).contiguous().transpose(1, 2)
+        b = torch.randn((n, k), device=dev, dtype=getattr(torch, bd))
+        ms = min(kernel_time_ms(lambda: dds.gemv_strided_batched(A, b, use_shared_memory=u), reps=20)
+                 for u in (False, True))
+        nbytes = n * k * k * A.element_size() + 2 * n * k * b.element_size()
+        out.append({"k": k, "A": ad, "b": bd, "ms": round(ms, 4), "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                    "reference_a100_ms": ref_ms, "speedup_vs_a100": round(ref_ms / ms, 2),
+                    "reference_csv_line": line})
+        del A, b
+    torch.cuda.empty_cache()
+    log("gemv vs reference A100: " + ", ".join(f"k={o['k']} {o['A'][:4]}/{o['b'][:4]} {o['ms']} ms "
+                                                f"({o['speedup_vs_a100']}x)" for o in out))
+    return out
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC
     measurement (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
@@ -516,11 +557,16 @@ def main():
                 extra["target_128"] = {"error": str(e)[:200]}
 
     stream = None
+    gemv_ref = None
     if not args.no_extra:
         try:
             stream = stream_peaks(dev)
         except Exception as e:
             stream = {"error": str(e)[:200]}
+        try:
+            gemv_ref = gemv_reference_shapes(dev)
+        except Exception as e:
+            gemv_ref = {"error": str(e)[:200]}
 
     cpu = None
     if not args.no_cpu:
@@ -566,6 +612,7 @@ def main():
             "avg_launch_us": round(t_loc * 1e3, 2),
         },
         "stream": stream,
+        "gemv_reference_benchmark": gemv_ref,
         "solve_roofline": {"alg_bytes": int(solve_bytes), "achieved_GBs": round(solve_gbs, 1),
                            "frac": round(solve_gbs / HBM_PEAK_GBS, 4)},
         "spmv": {"us": round(t_spmv * 1e3, 2), "bytes": int(spmv_bytes),
