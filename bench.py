@@ -295,6 +295,68 @@ def gemv_reference_shapes(dev):
     return out
 
 
+# The reference's BSR SpMV benchmark (results/benchmark_bsrmv.csv, A100 80GB, FP64): 2D
+# DG-SIPG matrices on the uniform mesh S_m (2 * 4^m triangles), block size b = DoFs per
+# element, one block per triangle and per interior edge (each direction); its RPT-I kernel
+# (the FastBSR default backend), min over repetitions (line numbers).
+REF_BSR_A100_MS = {
+    (11, 3): (1.9776, 3), (10, 6): (1.9016, 10), (10, 10): (4.6802, 17), (9, 15): (2.4548, 24),
+    (8, 21): (1.1619, 31),
+}
+
+
+def dg_triangle_bsr(m: int, b: int, dev, seed: int = 0):
+    """A BSR matrix with the block structure of DG on the uniform triangle mesh S_m (N = 2^m
+    squares per side, each cut into a lower and an upper triangle; a block per triangle and
+    per interior edge), random FP64 values. Same block count as the reference's matrices."""
+    import torch
+
+    N = 1 << m
+    ij = torch.arange(N * N, device=dev)
+    i, j = ij // N, ij % N
+    t0 = 2 * ij
+    # lower triangle (bottom, left, diagonal neighbours), upper (diagonal, right, top)
+    c0 = torch.stack([2 * ((i - 1) * N + j) + 1, 2 * (i * N + j - 1) + 1, t0, t0 + 1], 1)
+    v0 = torch.stack([i > 0, j > 0, torch.ones_like(i, dtype=torch.bool), torch.ones_like(i, dtype=torch.bool)], 1)
+    c1 = torch.stack([t0, t0 + 1, 2 * (i * N + j + 1), 2 * ((i + 1) * N + j)], 1)
+    v1 = torch.stack([torch.ones_like(i, dtype=torch.bool), torch.ones_like(i, dtype=torch.bool), j < N - 1, i < N - 1], 1)
+    cols = torch.stack([c0, c1], 1).reshape(-1, 4)   # triangle order t0, t1, t0, t1, ...
+    valid = torch.stack([v0, v1], 1).reshape(-1, 4)
+    counts = valid.sum(1)
+    crow = torch.zeros(cols.shape[0] + 1, dtype=torch.int64, device=dev)
+    crow[1:] = torch.cumsum(counts, 0)
+    col = cols[valid].to(torch.int32)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    vals = torch.rand((col.shape[0], b, b), dtype=torch.float64, device=dev, generator=g)
+    n = cols.shape[0] * b
+    return torch.sparse_bsr_tensor(crow, col, vals, (n, n))
+
+
+def bsr_reference_shapes(dev):
+    """FastBSR (the drop-in class) on the reference's 2D DG benchmark shapes."""
+    import torch
+    import dd_solvers as dds
+
+    out = []
+    for (m, b), (ref_ms, line) in REF_BSR_A100_MS.items():
+        A = dg_triangle_bsr(m, b, dev)
+        op = dds.FastBSR(A)
+        x = torch.rand(A.shape[1], dtype=torch.float64, device=dev)
+        y = torch.empty(A.shape[0], dtype=torch.float64, device=dev)
+        ms = kernel_time_ms(lambda: op._enqueue_matvec(x, y, None, None), reps=20)
+        # the reference's GB/s: values + block columns + block-row pointers + x + y
+        nbytes = op.bytes_per_matvec
+        out.append({"mesh": f"S{m}", "b": b, "dofs": int(A.shape[0]), "nnzb": op.nnzb,
+                    "ms": round(ms, 4), "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                    "reference_a100_ms": ref_ms, "speedup_vs_a100": round(ref_ms / ms, 2),
+                    "reference_csv_line": line})
+        del A, op, x, y
+        torch.cuda.empty_cache()
+    log("BSR vs reference A100: " + ", ".join(f"S{o['mesh'][1:]} b={o['b']} {o['ms']} ms "
+                                               f"({o['GBs']} GB/s, {o['speedup_vs_a100']}x)" for o in out))
+    return out
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC
     measurement (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
@@ -557,7 +619,7 @@ def main():
                 extra["target_128"] = {"error": str(e)[:200]}
 
     stream = None
-    gemv_ref = None
+    gemv_ref = bsr_ref = None
     if not args.no_extra:
         try:
             stream = stream_peaks(dev)
@@ -567,6 +629,10 @@ def main():
             gemv_ref = gemv_reference_shapes(dev)
         except Exception as e:
             gemv_ref = {"error": str(e)[:200]}
+        try:
+            bsr_ref = bsr_reference_shapes(dev)
+        except Exception as e:
+            bsr_ref = {"error": str(e)[:200]}
 
     cpu = None
     if not args.no_cpu:
@@ -613,6 +679,7 @@ def main():
         },
         "stream": stream,
         "gemv_reference_benchmark": gemv_ref,
+        "bsr_reference_benchmark": bsr_ref,
         "solve_roofline": {"alg_bytes": int(solve_bytes), "achieved_GBs": round(solve_gbs, 1),
                            "frac": round(solve_gbs / HBM_PEAK_GBS, 4)},
         "spmv": {"us": round(t_spmv * 1e3, 2), "bytes": int(spmv_bytes),

@@ -87,6 +87,60 @@ __global__ __launch_bounds__(kBlock) void bsr_row_kernel(int64_t n_block_rows, i
 }
 
 // ---------------------------------------------------------------------------------------
+// BSR with a small block size B (3, 5, 6, 7: DG elements): one lane per block. The G lanes
+// of a block row take its blocks lg, lg + G, ...; a lane reads its block's B*B values
+// (contiguous) and the B entries of x it multiplies, keeps B row sums, and the G lanes'
+// sums are combined by a butterfly; lane r mod G writes row r. Far fewer load instructions
+// than one lane per scalar row (which reads one value per instruction per row).
+// ---------------------------------------------------------------------------------------
+template <typename T, typename I, int B, int G>
+__global__ __launch_bounds__(kBlock) void bsr_blk_kernel(int64_t n_block_rows,
+                                                          const I* __restrict__ crow,
+                                                          const int32_t* __restrict__ col,
+                                                          const T* __restrict__ val,
+                                                          const T* __restrict__ x,
+                                                          T* __restrict__ y,
+                                                          double* __restrict__ partials,
+                                                          const dds_pcg_state* __restrict__ gate) {
+  static_assert((G & (G - 1)) == 0 && G <= 64, "bad BSR group");
+  __shared__ double scratch[kBlock / 64];
+  if (gate != nullptr && gate->done) return;
+  const int lg = threadIdx.x & (G - 1);
+  const int64_t group = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G;
+  const int64_t n_groups = (int64_t)gridDim.x * kBlock / G;
+  double dot = 0.0;
+  for (int64_t br = group; br < n_block_rows; br += n_groups) {
+    double acc[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) acc[r] = 0.0;
+    for (int64_t blk = (int64_t)crow[br] + lg; blk < (int64_t)crow[br + 1]; blk += G) {
+      const T* v = val + blk * (B * B);
+      const T* xs = x + (int64_t)col[blk] * B;
+      double xv[B];
+#pragma unroll
+      for (int c = 0; c < B; ++c) xv[c] = static_cast<double>(xs[c]);
+#pragma unroll
+      for (int c = 0; c < B; ++c)
+#pragma unroll
+        for (int r = 0; r < B; ++r) acc[r] += static_cast<double>(v[c * B + r]) * xv[c];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int r = 0; r < B; ++r) acc[r] += __shfl_xor(acc[r], off, G);
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      if ((r & (G - 1)) == lg) {
+        const int64_t row = br * B + r;
+        y[row] = static_cast<T>(acc[r]);
+        if (partials != nullptr) dot += acc[r] * static_cast<double>(x[row]);
+      }
+    }
+  }
+  if (partials != nullptr) write_partial(dot, partials, scratch);
+}
+
+// ---------------------------------------------------------------------------------------
 // BSR with a power-of-two block size B: G lanes stream the block row's contiguous value
 // segment (nnzb_row * B * B elements), so consecutive lanes read consecutive values. With
 // column-major blocks element q of the segment sits in row q % B, which is lane % B for
@@ -266,9 +320,19 @@ int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, con
     hipLaunchKernelGGL((bsr_seg_kernel<T, I, B, G>), dim3((unsigned)grid), dim3(kBlock), \
                        0, s, nbr, cr, col, v, xv, yv, partials, gate);                   \
   } while (0)
+#define DDS_BLK(B, G)                                                                    \
+  do {                                                                                   \
+    grid = grid_for(nbr * G, capped);                                                    \
+    hipLaunchKernelGGL((bsr_blk_kernel<T, I, B, G>), dim3((unsigned)grid), dim3(kBlock), \
+                       0, s, nbr, cr, col, v, xv, yv, partials, gate);                   \
+  } while (0)
   switch (b) {
     case 2: DDS_SEG(2, 16); break;
+    case 3: DDS_BLK(3, 4); break;
     case 4: DDS_SEG(4, 32); break;
+    case 5: DDS_BLK(5, 4); break;
+    case 6: DDS_BLK(6, 8); break;
+    case 7: DDS_BLK(7, 8); break;
     case 8: DDS_SEG(8, 64); break;
     default:
       grid = grid_for(nbr * b, capped);
@@ -276,6 +340,7 @@ int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, con
                          (int)b, cr, col, v, xv, yv, partials, gate);
   }
 #undef DDS_SEG
+#undef DDS_BLK
   if (n_partials) *n_partials = partials ? (int)grid : 0;
   DDS_LAUNCHED();
   return DDS_OK;

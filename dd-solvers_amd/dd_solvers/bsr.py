@@ -4,8 +4,9 @@ Keeps the reference's class name, constructor and `@` operator
 (src/dd_solvers/dd_solvers/bsr.py:15-32). Backends:
   "row_per_thread_irrespective"  the default (bsr.py:17). For b in {2, 4, 8} it runs the
                                  segment kernel (lanes stream the block row's contiguous
-                                 values); other b use one lane per scalar row, the RPT-I
-                                 order (bsr.py:46-66).
+                                 values); for b in {3, 5, 6, 7} one lane per block with a
+                                 butterfly over a block row's lanes; other b use one lane
+                                 per scalar row, the RPT-I order (bsr.py:46-66).
   "torch"                        torch's own BSR @ dense (library comparator).
 """
 

@@ -64,7 +64,7 @@ def _bsr_tensor(crow, col, vals, dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_bsr_fixture(dtype):
-    crow, col, vals = bsr8()  # test_bsr.py:9-138 (b = 3: the row-per-lane kernel)
+    crow, col, vals = bsr8()  # test_bsr.py:9-138 (b = 3: the lane-per-block kernel)
     M = _bsr_tensor(crow, col, vals, dtype)
     x = torch.randn(M.shape[0], dtype=dtype, device=dev)
     y = FastBSR(M) @ x
@@ -73,7 +73,7 @@ def test_bsr_fixture(dtype):
     assert np.allclose(y.cpu().numpy(), ref, atol=1e-12 if dtype == torch.float64 else 1e-5)
 
 
-@pytest.mark.parametrize("b", [1, 2, 3, 4, 8])
+@pytest.mark.parametrize("b", [1, 2, 3, 4, 5, 6, 7, 8, 10])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_bsr_random(b, dtype):
     # test_bsr.py:161-193: 1000 block rows, 3 random blocks per row (duplicates allowed)
