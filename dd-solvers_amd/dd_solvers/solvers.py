@@ -640,12 +640,15 @@ class CG(SparseSolver):
         stv = buf["state"].cpu().numpy().view(_STATE_DTYPE)
         log = buf["log"].cpu().numpy()
         meta = {"iterations": [], "residual norms": [], "preconditioner metadata": []}
+        if self.estimate_cond:
+            meta["condition number estimate"] = []
         for j in range(w):
             it, done = int(stv["it"][j]), int(stv["done"][j])
             nn = it + 1 if done == 1 else it
-            meta["iterations"].append(nn - 1)
-            meta["residual norms"].append([float(v) for v in log[0, :nn, j]])
-            meta["preconditioner metadata"].append([{} for _ in range(max(nn - 1, 0))])
+            mj = self._metadata([float(v) for v in log[0, :nn, j]], it, None,
+                                log[1, :, j], log[2, :, j])
+            for key in meta:
+                meta[key].append(mj[key])
         return X[:, :w].clone(), meta
 
     def _fin_m(self, op, count, buf):

@@ -207,3 +207,39 @@ def test_block_sellmm_and_dots_match_host():
     s.matrix._enqueue_matmat(Xt, Y, None, None)
     ref = sps.csr_matrix(pb.A) @ X
     assert np.allclose(Y.cpu().numpy(), ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+
+
+def test_block_x0_mutated_and_convergence_not_achieved():
+    # Additive: the reference's Hybrid start x = x0 + T0(b) (solvers.py:233-234) leaves a
+    # residual outside the space the balancing preconditioner is SPD on unless x0 = 0
+    pb = _kmeans()
+    s = dds.CG(_pc("additive", "float16"), rtol=1e-8, bsr_matmul=False)
+    s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+    B = torch.as_tensor(np.ascontiguousarray(pb.b[:, :3]), device=dev)
+    X, m = s.solve(B)
+    # a nonzero initial guess is used and, as in the 1-D path, mutated and returned
+    x0 = 0.5 * X.clone()
+    out, m0 = s.solve(B, x0)
+    assert out is x0
+    assert max(m0["iterations"]) <= max(m["iterations"])
+    for j in range(3):
+        r = np.linalg.norm(pb.b[:, j] - pb.A @ x0[:, j].cpu().numpy()) / np.linalg.norm(pb.b[:, j])
+        assert r < 1.01e-8
+    s2 = dds.CG(_pc("hybrid", "float16"), rtol=1e-12, maxiter=3, bsr_matmul=False)
+    s2.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
+    with pytest.raises(dds.ConvergenceNotAchieved) as e:
+        s2.solve(B)
+    assert all(len(r) == 3 for r in e.value.metadata["residual norms"])
+
+
+def test_block_condition_number_estimate():
+    """Per-column Lanczos estimate from the device-side alpha/beta logs: a diagonal SPD
+    matrix with known eigenvalues, unpreconditioned (the reference's known answer,
+    test_solvers.py:179-184, for every column)."""
+    A = sps.diags([0.5, 4.0, 21.0]).tocsr()
+    s = dds.CG(preconditioner=None, rtol=1e-9, estimate_cond=True, bsr_matmul=False)
+    s.setup(_torch_csr(A))
+    B = torch.ones((3, 5), dtype=torch.float64, device=dev)
+    _, meta = s.solve(B)
+    assert len(meta["condition number estimate"]) == 5
+    assert np.allclose(meta["condition number estimate"], 42.0)

@@ -59,6 +59,29 @@ def test_host_validation_without_gpu():
         _lib.call("dds_chol_solve", _lib.F64, 7000, 1, 7000, 1, 1, None, None)
     with pytest.raises(_lib.HipError, match="unsupported"):
         _lib.call("dds_cast", _lib.F16, _lib.F64, 4, 1, 1, None, None)
+    # entry points of the batched, setup and measurement paths
+    with pytest.raises(_lib.HipError, match="dtype must be F64 or F32"):
+        _lib.call("dds_spd_inverse_batched", _lib.F16, 2, 1, 4, 16, 1, None, None)
+    with pytest.raises(_lib.HipError, match="ld/stride"):
+        _lib.call("dds_spd_inverse_batched", _lib.F64, 2, 1, 4, 2, 1, None, None)
+    with pytest.raises(_lib.HipError, match="n must be even"):
+        _lib.call("dds_stream", 1, 7, 1, None, 0.0, 1, 8, None)
+    with pytest.raises(_lib.HipError, match="grid"):
+        _lib.call("dds_stream", 1, 8, 1, None, 0.0, 1, _lib.MAX_PARTIALS + 1, None)
+    with pytest.raises(_lib.HipError, match="op must be"):
+        _lib.call("dds_stream", 5, 8, 1, None, 0.0, 1, 8, None)
+    with pytest.raises(_lib.HipError, match="no column indices"):
+        _lib.call("dds_sellmv", _lib.F64, 64, 1, 1, None, None, 1, 1, 1, 1, None, None, None, None)
+    with pytest.raises(_lib.HipError, match="waves must be"):
+        _lib.call("dds_mfma_apply", _lib.F16, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 0, 3, None, None)
+    with pytest.raises(_lib.HipError, match="F16 or BF16"):
+        _lib.call("dds_mfma_apply", _lib.F64, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 0, 4, None, None)
+    with pytest.raises(_lib.HipError, match="F16 or BF16"):
+        _lib.call("dds_mfma_pack_x", _lib.F32, 4, 1, None, 1, 1, 1, 1, None, None)
+    with pytest.raises(_lib.HipError, match="bad count"):
+        _lib.call("dds_pcg_finalize_m", 0, 1, _lib.MAX_PARTIALS + 1, 1, 1, 1, 1, None)
+    with pytest.raises(_lib.HipError, match="bad grid size"):
+        _lib.call("dds_q1_element_grad", 1, 1, 1, 1, None, 1, None)
 
 
 def test_cpu_tensors_rejected():
