@@ -44,84 +44,100 @@ __device__ __forceinline__ int sym_slot(int r, int j) {
   return r * (S::NH + 1) + j;
 }
 
-// One 64 x 64 tile (R rows, nch chunk columns) at tp, processed by column halves:
-//   row     += sum_c T[lane][c] * xjs[c]                     (lane = row)
-//   returns sum_r T[r][c] * xis[r] for column c = lane
-// Each half (32 columns) is loaded as NH coalesced 1 KiB accesses; for the transposed
-// product it is staged in LDS, lanes 0-31 sum rows 0-31 and lanes 32-63 rows 32-63 of
-// column lane % 32, and one cross-lane exchange adds the two row halves.
-// diag: a diagonal tile in its packed lower-triangular layout (rows r >= jV of chunk j).
-// Every chunk address is a wave-uniform base plus lane * V. The loads are predicated per
-// lane (exec mask) rather than selected after loading: a select right after a load would
-// make the compiler wait for each load before issuing the next.
-template <typename TM, typename TA>
-__device__ __forceinline__ TA tile_apply(const TM* tp, int R, int nch, bool diag,
-                                         const TA* xjs, const TA* xis, TM* lds, int lane,
-                                         TA& row) {
+// Workgroups are one wavefront, and the LDS operations of a wavefront are performed in
+// order, so a wavefront-scope fence (a compiler barrier, no hardware wait) orders the
+// lanes' LDS writes before other lanes' reads. A workgroup barrier would also wait for the
+// global loads in flight, which are the prefetched tile halves.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Issues the NH chunk loads of column half h of the tile at tp (R rows, nch chunk
+// columns; diag: packed lower-triangular layout, rows r >= jV of chunk j) and returns the
+// lane's mask of chunks that hold data. Every load is issued on every path (lanes with
+// nothing to load read the tile's first chunk), so the count of loads in flight is the same
+// on all paths and the compiler waits for exactly the loads a computation needs; the
+// unused values are zeroed where they are consumed, in compute_half.
+template <typename TM>
+__device__ __forceinline__ uint32_t load_half(const TM* tp, int R, int nch, bool diag, int h,
+                                              bool dummy, int lane,
+                                              Vec16<TM> (&c)[SymShape<TM>::NH]) {
   using S = SymShape<TM>;
-  TA col[2] = {0, 0};
-  const TM* lp = tp + (int64_t)lane * S::V;
+  uint32_t m = 0;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    if (h * S::NH >= nch) break;
-    Vec16<TM> c[S::NH];
+  for (int j = 0; j < S::NH; ++j) {
+    const int jj = h * S::NH + j;
+    const bool ok = !dummy && jj < nch && lane < R && (!diag || lane >= jj * S::V);
+    const int64_t ub =
+        diag ? (int64_t)S::V * ((int64_t)jj * R - (int64_t)S::V * jj * (jj - 1) / 2) -
+                   (int64_t)jj * S::V * S::V
+             : (int64_t)jj * R * S::V;
+    c[j] = load16_nt(tp + (ok ? (int64_t)lane * S::V + ub : 0));
+    m |= ok ? (1u << j) : 0u;
+  }
+  return m;
+}
+
+// Column half h (32 columns) of a tile, loaded by load_half with mask m (full: every chunk
+// holds data):
+//   row += sum_c T[lane][c] * xjs[c]                          (lane = row)
+//   returns sum_r T[r][c] * xis[r] for column c = 32 h + lane % 32 (every lane)
+// For the transposed product the half is staged in LDS, lanes 0-31 sum rows 0-31 and lanes
+// 32-63 rows 32-63 of their column, and one cross-lane exchange adds the two row halves.
+template <typename TM, typename TA>
+__device__ __forceinline__ TA compute_half(Vec16<TM> (&c)[SymShape<TM>::NH], uint32_t m,
+                                           bool full, int R, int nch, int h, const TA* xjs,
+                                           const TA* xis, TM* lds, int lane, TA& row) {
+  using S = SymShape<TM>;
+  if (!full) {
 #pragma unroll
-    for (int j = 0; j < S::NH; ++j) {
-      const int jj = h * S::NH + j;
-      if (jj < nch && lane < R && (!diag || lane >= jj * S::V)) {
-        const int64_t ub =
-            diag ? (int64_t)S::V * ((int64_t)jj * R - (int64_t)S::V * jj * (jj - 1) / 2) -
-                       (int64_t)jj * S::V * S::V
-                 : (int64_t)jj * R * S::V;
-        c[j] = load16_nt(lp + ub);
-      } else {
+    for (int j = 0; j < S::NH; ++j)
+      if (!((m >> j) & 1u)) {
 #pragma unroll
         for (int e = 0; e < S::V; ++e) c[j].v[e] = TM{};
       }
-    }
-    // four independent accumulators (shorter dependent FMA chains), added in a fixed order
-    TA ra[4] = {0, 0, 0, 0};
+  }
+  // four independent accumulators (shorter dependent FMA chains), added in a fixed order
+  TA ra[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int j = 0; j < S::NH; ++j) {
-      const int jj = h * S::NH + j;
-      if (jj < nch) {
+  for (int j = 0; j < S::NH; ++j) {
+    const int jj = h * S::NH + j;
+    if (jj < nch) {
 #pragma unroll
-        for (int e = 0; e < S::V; ++e)
-          ra[(j * S::V + e) & 3] += static_cast<TA>(widen(c[j].v[e])) * xjs[jj * S::V + e];
-      }
-    }
-    row += (ra[0] + ra[1]) + (ra[2] + ra[3]);
-    {
-      __syncthreads();  // previous readers of lds are done
-#pragma unroll
-      for (int j = 0; j < S::NH; ++j)
-        *reinterpret_cast<Vec16<TM>*>(lds + sym_slot<TM>(lane, j) * S::V) = c[j];
-      __syncthreads();
-      const int cc = lane & 31;
-      const int jc = cc / S::V;
-      const int ec = cc % S::V;
-      const int r0 = lane & 32;
-      const int nr = R - r0 < 32 ? R - r0 : 32;
-      TA q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-      int t = 0;
-      for (; t + 3 < nr; t += 4) {
-        const TA a0 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec]));
-        const TA a1 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 1, jc) * S::V + ec]));
-        const TA a2 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 2, jc) * S::V + ec]));
-        const TA a3 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 3, jc) * S::V + ec]));
-        q0 += a0 * xis[r0 + t];
-        q1 += a1 * xis[r0 + t + 1];
-        q2 += a2 * xis[r0 + t + 2];
-        q3 += a3 * xis[r0 + t + 3];
-      }
-      for (; t < nr; ++t)
-        q0 += static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec])) * xis[r0 + t];
-      TA acc = (q0 + q1) + (q2 + q3);
-      acc += __shfl_xor(acc, 32, 64);
-      col[h] = acc;
+      for (int e = 0; e < S::V; ++e)
+        ra[(j * S::V + e) & 3] += static_cast<TA>(widen(c[j].v[e])) * xjs[jj * S::V + e];
     }
   }
-  return lane < 32 ? col[0] : col[1];
+  row += (ra[0] + ra[1]) + (ra[2] + ra[3]);
+  wave_lds_fence();  // previous readers of lds are done
+#pragma unroll
+  for (int j = 0; j < S::NH; ++j)
+    *reinterpret_cast<Vec16<TM>*>(lds + sym_slot<TM>(lane, j) * S::V) = c[j];
+  wave_lds_fence();
+  const int cc = lane & 31;
+  const int jc = cc / S::V;
+  const int ec = cc % S::V;
+  const int r0 = lane & 32;
+  const int nr = R - r0 < 32 ? R - r0 : 32;
+  TA q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+  int t = 0;
+  for (; t + 3 < nr; t += 4) {
+    const TA a0 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec]));
+    const TA a1 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 1, jc) * S::V + ec]));
+    const TA a2 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 2, jc) * S::V + ec]));
+    const TA a3 = static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t + 3, jc) * S::V + ec]));
+    q0 += a0 * xis[r0 + t];
+    q1 += a1 * xis[r0 + t + 1];
+    q2 += a2 * xis[r0 + t + 2];
+    q3 += a3 * xis[r0 + t + 3];
+  }
+  for (; t < nr; ++t)
+    q0 += static_cast<TA>(widen(lds[sym_slot<TM>(r0 + t, jc) * S::V + ec])) * xis[r0 + t];
+  TA acc = (q0 + q1) + (q2 + q3);
+  acc += __shfl_xor(acc, 32, 64);
+  return acc;
 }
 
 template <typename TM, typename TV, typename TA>
@@ -171,49 +187,97 @@ __global__ __launch_bounds__(64) void sym_apply_kernel(
   xi1s[lane] = gather(I1);
   xi2s[lane] = gather(I2);
   TA row1 = 0, row2 = 0, hold1 = 0;
-  // two-stage gather: the index of block J + 2 and the value of block J + 1 are in flight
-  // while block J is processed, so no wait on an index load precedes the tile loads
-  auto index_of = [&](int J) -> int64_t {
+  // Two-stage gather: the index of block J + 2 and the value of block J + 1 are in flight
+  // while block J is processed. The index is kept as loaded (32 bits; the local position
+  // when sub_idx is null): widening it right after the load would wait for every load in
+  // flight.
+  auto index_of = [&](int J) -> int32_t {
     const int a = 64 * J + lane;
     if (J > I2 || a >= k) return -1;
-    return sub_idx != nullptr ? (int64_t)sub_idx[base + a] : base + a;
+    return sub_idx != nullptr ? sub_idx[base + a] : a;
   };
   TA xj = gather(0);
-  int64_t g_next = index_of(1);
-  for (int J = 0; J <= I2; ++J) {
-    const int nch = chunks_of(J);
-    __syncthreads();  // readers of xjs from the previous J are done
+  int32_t g_next = index_of(1);
+  int gJ = 1;  // block whose index is in g_next
+  auto start_block = [&]() {  // x of the block about to start into xjs; gather ahead
+    wave_lds_fence();         // readers of xjs from the previous block are done
     xjs[lane] = xj;
-    const int64_t g_cur = g_next;
-    g_next = index_of(J + 2);
-    xj = g_cur >= 0 ? static_cast<TA>(x[g_cur]) : static_cast<TA>(0);
-    __syncthreads();
-    // tile (I2, J), then (I1, J): the transposed product of an off-diagonal tile goes to
-    // block J, that of a diagonal tile to its own block
-    TA contrib = 0;
-    const int n_rows_here = (two && J <= I1) ? 2 : 1;
-#pragma unroll 1
-    for (int w = 0; w < n_rows_here; ++w) {
-      const int I = w == 0 ? I2 : I1;
-      const bool diag = J == I;
-      TA r = w == 0 ? row2 : row1;
-      const TA c = tile_apply<TM, TA>(M + tile_off(I, J), rows_of(I), nch, diag, xjs,
-                                      w == 0 ? xi2s : xi1s, lds_tile, lane, r);
-      if (diag)
-        r += c;
-      else
-        contrib += c;
-      if (w == 0)
-        row2 = r;
-      else
-        row1 = r;
+    const int32_t g_cur = g_next;
+    ++gJ;
+    g_next = index_of(gJ);
+    xj = g_cur >= 0 ? static_cast<TA>(x[sub_idx != nullptr ? (int64_t)g_cur : base + g_cur])
+                    : static_cast<TA>(0);
+    wave_lds_fence();
+  };
+  // Tasks, in order: for J = 0 .. I2, tile (I2, J), then tile (I1, J) if J <= I1, each in one
+  // or two column halves. The loads of the next task are issued before the current one is
+  // computed (two register buffers), so a wave always has a tile half in flight; after the
+  // last task a dummy issue keeps the load count equal on every path.
+  auto halves_of = [&](int J) { return chunks_of(J) > S::NH ? 2 : 1; };
+  auto next_task = [&](int& J, int& w, int& h) -> bool {
+    if (h + 1 < halves_of(J)) {
+      ++h;
+      return true;
     }
-    if (J < I2) {
-      if (two && J == I1)
-        hold1 = contrib;
-      else
-        store(J, contrib);
+    h = 0;
+    if (w == 0 && two && J <= I1) {
+      w = 1;
+      return true;
     }
+    w = 0;
+    ++J;
+    return J <= I2;
+  };
+  auto issue = [&](int J, int w, int h, bool dummy, Vec16<TM> (&c)[S::NH]) -> uint32_t {
+    const int I = w == 0 ? I2 : I1;
+    return load_half<TM>(M + tile_off(I, J), rows_of(I), chunks_of(J), J == I, h, dummy, lane,
+                         c);
+  };
+  int J = 0, w = 0, h = 0;  // current task: tile (w == 0 ? I2 : I1, J), column half h
+  TA contrib = 0, tc0 = 0;
+  Vec16<TM> cA[S::NH], cB[S::NH];
+  uint32_t mA = issue(0, 0, 0, false, cA), mB = 0;
+  start_block();
+  auto step = [&](Vec16<TM> (&cur)[S::NH], uint32_t mc, Vec16<TM> (&nxt)[S::NH],
+                  uint32_t& mn) -> bool {
+    int J2 = J, w2 = w, h2 = h;
+    const bool more = next_task(J2, w2, h2);
+    mn = issue(more ? J2 : J, more ? w2 : w, more ? h2 : h, !more, nxt);
+    const int I = w == 0 ? I2 : I1;
+    const bool diag = J == I;
+    const int R = rows_of(I), nch = chunks_of(J);
+    const bool full = !diag && R == 64 && nch >= (h + 1) * S::NH;
+    TA r = w == 0 ? row2 : row1;
+    const TA col = compute_half<TM, TA>(cur, mc, full, R, nch, h, xjs, w == 0 ? xi2s : xi1s,
+                                        lds_tile, lane, r);
+    if (h == 0) tc0 = col;
+    if (h + 1 == halves_of(J)) {  // tile complete: column lane of block J is tv
+      const TA tv = lane < 32 ? tc0 : (h == 0 ? static_cast<TA>(0) : col);
+      if (diag)  // (L^T + D/2) x_I belongs to the tile's own row block
+        r += tv;
+      else
+        contrib += tv;
+    }
+    if (w == 0)
+      row2 = r;
+    else
+      row1 = r;
+    if (!more || J2 != J) {  // block J is complete
+      if (J < I2) {
+        if (two && J == I1)
+          hold1 = contrib;  // block I1 also receives row1, stored at the end
+        else
+          store(J, contrib);
+      }
+      contrib = 0;
+      if (more) start_block();
+    }
+    J = J2;
+    w = w2;
+    h = h2;
+    return more;
+  };
+  while (step(cA, mA, cB, mB) && step(cB, mB, cA, mA)) {
   }
   if (two) store(I1, row1 + hold1);
   store(I2, row2);
