@@ -52,6 +52,47 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(int64_t n_rows, const I* __
 }
 
 // ---------------------------------------------------------------------------------------
+// CSR with long rows (lanes = BT > 64, e.g. R_C A of a coarse space: few rows of hundreds
+// of entries): one workgroup of BT threads per row (grid-stride over rows), so a row costs
+// one or two dependent load steps instead of nnz/64; the waves' butterfly sums are added in
+// wave order through LDS (deterministic). The x . y partial of a workgroup is written by its
+// first thread.
+// ---------------------------------------------------------------------------------------
+template <typename T, typename I, int BT>
+__global__ __launch_bounds__(BT) void csr_rows_kernel(int64_t n_rows, const I* __restrict__ row_ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const T* __restrict__ val,
+                                                      const T* __restrict__ x, T* __restrict__ y,
+                                                      double* __restrict__ partials,
+                                                      const dds_pcg_state* __restrict__ gate) {
+  constexpr int NW = BT / 64;
+  __shared__ double wsum[2][NW];
+  if (gate != nullptr && gate->done) return;
+  const int wid = threadIdx.x >> 6;
+  double dot = 0.0;
+  int buf = 0;
+  for (int64_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
+    const int64_t s = row_ptr[row];
+    const int64_t e = row_ptr[row + 1];
+    double acc = 0.0;
+    for (int64_t p = s + threadIdx.x; p < e; p += BT)
+      acc += static_cast<double>(load_nt(val + p)) * static_cast<double>(x[load_nt(col + p)]);
+    acc = group_sum<64>(acc);
+    if ((threadIdx.x & 63) == 0) wsum[buf][wid] = acc;
+    __syncthreads();  // (wsum alternates between two buffers, so one barrier per row suffices)
+    if (threadIdx.x == 0) {
+      double r = wsum[buf][0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) r += wsum[buf][w];
+      y[row] = static_cast<T>(r);
+      if (partials != nullptr) dot += r * static_cast<double>(x[row]);
+    }
+    buf ^= 1;
+  }
+  if (partials != nullptr && threadIdx.x == 0) partials[blockIdx.x] = dot;
+}
+
+// ---------------------------------------------------------------------------------------
 // BSR, general block size: one lane per scalar row (the row-per-thread order of the
 // reference's RPT-I kernel: blocks of the row in stored order, then c ascending).
 // ---------------------------------------------------------------------------------------
@@ -285,6 +326,22 @@ int launch_csr(int64_t n_rows, const void* row_ptr, const int32_t* col, const vo
   const T* v = static_cast<const T*>(values);
   const T* xv = static_cast<const T*>(x);
   T* yv = static_cast<T*>(y);
+  if (lanes > 64) {
+    const int64_t grid = std::min<int64_t>(n_rows, partials != nullptr ? kMaxPartials : 1 << 20);
+    if (n_partials) *n_partials = partials ? (int)grid : 0;
+#define DDS_CSR_ROWS(BT)                                                                    \
+  hipLaunchKernelGGL((csr_rows_kernel<T, I, BT>), dim3((unsigned)grid), dim3(BT), 0, s, n_rows, \
+                     rp, col, v, xv, yv, partials, gate)
+    switch (lanes) {
+      case 128: DDS_CSR_ROWS(128); break;
+      case 256: DDS_CSR_ROWS(256); break;
+      case 512: DDS_CSR_ROWS(512); break;
+      default: DDS_CSR_ROWS(1024); break;
+    }
+#undef DDS_CSR_ROWS
+    DDS_LAUNCHED();
+    return DDS_OK;
+  }
   const int64_t grid = grid_for(n_rows * lanes, partials != nullptr);
   if (n_partials) *n_partials = partials ? (int)grid : 0;
 #define DDS_CSR(G)                                                                            \
@@ -359,8 +416,8 @@ extern "C" int dds_csrmv(int val_dtype, int idx_dtype, int64_t n_rows, const voi
   if (n_partials) *n_partials = 0;
   if (n_rows == 0) return DDS_OK;
   DDS_REQUIRE(row_ptr && x && y, "dds_csrmv: null pointer");
-  DDS_REQUIRE(lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0,
-              "dds_csrmv: lanes must be a power of two in [1, 64] (got %d)", lanes);
+  DDS_REQUIRE(lanes >= 1 && lanes <= 1024 && (lanes & (lanes - 1)) == 0,
+              "dds_csrmv: lanes must be a power of two in [1, 1024] (got %d)", lanes);
   hipStream_t s = as_stream(stream);
   if (val_dtype == DDS_F64 && idx_dtype == DDS_I64)
     return launch_csr<double, int64_t>(n_rows, row_ptr, col, values, x, y, lanes, dot_partials,

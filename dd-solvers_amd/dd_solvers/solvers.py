@@ -227,7 +227,7 @@ class _CSROperator:
     """y = A x for a torch sparse CSR matrix (dds_csrmv); used by CG(bsr_matmul=False)
     and for R0A in HybridSchwarz."""
 
-    def __init__(self, matrix: torch.Tensor):
+    def __init__(self, matrix: torch.Tensor, lanes: int | None = None):
         if matrix.layout is not torch.sparse_csr:
             matrix = matrix.to_sparse_csr()
         vals = matrix.values()
@@ -243,10 +243,14 @@ class _CSROperator:
         self.values = vals.contiguous()
         self.nnz = int(self.col.shape[0])
         avg = self.nnz / max(self.shape[0], 1)
-        lanes = 1
-        while lanes < 64 and lanes < avg:
-            lanes *= 2
-        self.lanes = lanes
+        if lanes is None:
+            # one wavefront's lanes per row up to 64 (128 entries on average); longer rows
+            # (R_C A: few rows of hundreds of entries) get a workgroup of 256 per row
+            cap = 256 if avg > 128 else 64
+            lanes = 1
+            while lanes < cap and lanes < avg:
+                lanes *= 2
+        self.lanes = int(lanes)
 
     def __matmul__(self, x):
         y = torch.empty(self.shape[0], dtype=self.dtype, device=self.device)
@@ -276,7 +280,7 @@ class _SELLOperator:
     wavefront reads every slot of its 64 rows with coalesced loads. CG(bsr_matmul=False)
     uses it for square matrices."""
 
-    def __init__(self, matrix: torch.Tensor):
+    def __init__(self, matrix: torch.Tensor, lanes: int | None = None):
         if matrix.layout is not torch.sparse_csr:
             matrix = matrix.to_sparse_csr()
         vals = matrix.values()

@@ -114,9 +114,10 @@ int dds_bsrmv(int val_dtype, int idx_dtype, int64_t n_block_rows, int64_t block_
               const void* crow, const int32_t* col, const void* values, const void* x, void* y,
               double* dot_partials, int* n_partials, const dds_pcg_state* gate, void* stream);
 
-/* CSR y = A x; `lanes` lanes cooperate on one row (a power of two in [1, 64]; the Python
- * layer picks the power of two nearest above nnz / n_rows). Partials and gate as for
- * dds_bsrmv. */
+/* CSR y = A x; `lanes` lanes cooperate on one row (a power of two in [1, 1024]; the Python
+ * layer picks the power of two nearest above nnz / n_rows). Up to 64 the lanes are part of
+ * one wavefront; above 64 a workgroup of `lanes` threads handles each row (long rows, e.g.
+ * R_C A of the hybrid coarse correction). Partials and gate as for dds_bsrmv. */
 int dds_csrmv(int val_dtype, int idx_dtype, int64_t n_rows, const void* row_ptr,
               const int32_t* col, const void* values, const void* x, void* y, int lanes,
               double* dot_partials, int* n_partials, const dds_pcg_state* gate, void* stream);

@@ -209,6 +209,47 @@ def test_spmv_formats_with_dot(kind, N, fmt):
     assert np.isclose(part[:cnt].sum().item(), x @ (A @ x), rtol=1e-12)
 
 
+@pytest.mark.parametrize("lanes", [None, 64, 128, 256, 512, 1024])
+@pytest.mark.parametrize("value_type,index_type", [(torch.float64, torch.int64),
+                                                   (torch.float64, torch.int32),
+                                                   (torch.float32, torch.int64)])
+def test_csrmv_long_rows(lanes, value_type, index_type):
+    """Rows of 0 .. 3000 entries (the shape of R_C A, plus empty rows), every lane setting
+    including the workgroup-per-row kernel, with the x . y epilogue on a square matrix."""
+    rng = np.random.default_rng(7)
+    n = 1500
+    lens = rng.integers(0, 3000, n)
+    lens[::97] = 0
+    rows = np.repeat(np.arange(n), lens)
+    cols = rng.integers(0, n, lens.sum())
+    A = sps.csr_matrix((rng.standard_normal(lens.sum()), (rows, cols)), shape=(n, n))
+    A.sum_duplicates()
+    if value_type == torch.float32:  # the values and x the kernel sees, in float64
+        A.data = A.data.astype(np.float32).astype(np.float64)
+    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr).to(index_type),
+                                 torch.as_tensor(A.indices.astype(np.int32)),
+                                 torch.as_tensor(A.data).to(value_type), A.shape).to(dev)
+    op = dds.solvers._CSROperator(At, lanes=lanes)
+    if lanes is None:
+        assert op.lanes == 256  # average row length > 128
+    xt = torch.as_tensor(rng.standard_normal(n), device=dev).to(value_type)
+    x = xt.double().cpu().numpy()
+    y = torch.empty_like(xt)
+    part = torch.zeros(2048, dtype=torch.float64, device=dev)
+    cnt = op._enqueue_matvec(xt, y, part, None)
+    torch.cuda.synchronize()
+    ref = A @ x  # double accumulation, as in the kernels
+    tol = 1e-13 if value_type == torch.float64 else 1e-6
+    scale = np.abs(A) @ np.abs(x)
+    assert np.all(np.abs(y.double().cpu().numpy() - ref) <= tol * scale + 1e-300)
+    assert abs(part[:cnt].sum().item() - ref @ x) <= 1e-12 * (scale @ np.abs(x))
+    # deterministic: the same launch gives the same bits
+    y2 = torch.empty_like(xt)
+    op._enqueue_matvec(xt, y2, None, None)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y2)
+
+
 def test_csrmv_tall_matrix():
     """Rows > columns (the Q1 assembly operator): y = G k must not touch k past its end."""
     N = 16

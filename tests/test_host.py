@@ -53,6 +53,8 @@ def test_host_validation_without_gpu():
         _lib.call("dds_gemv_strided_batched", _lib.F64, _lib.F32, 1, 1, 1, 2, 2, 0, None)
     with pytest.raises(_lib.HipError, match="lanes"):
         _lib.call("dds_csrmv", _lib.F64, _lib.I64, 4, 1, 1, 1, 1, 1, 3, None, None, None, None)
+    with pytest.raises(_lib.HipError, match="lanes"):
+        _lib.call("dds_csrmv", _lib.F64, _lib.I64, 4, 1, 1, 1, 1, 1, 2048, None, None, None, None)
     with pytest.raises(_lib.HipError, match="bad op"):
         _lib.call("dds_pcg_finalize", 9, 1, 1, 1, 1, 1, 1, None)
     with pytest.raises(_lib.HipError, match="exceeds"):
