@@ -284,7 +284,8 @@ __global__ __launch_bounds__(64) void sym_apply_kernel(
 }
 
 // y[sub_ptr[s] + a] = sum over the items p of subdomain s that cover local row a (blocks
-// 0..T-1-p) of Yp[ypos[s] + p*64*T + a], in item order. Grid (ceil(max_k/256), n_s).
+// 0..T-1-p) of Yp[ypos[s] + p*64*T + a], in item order. Grid (n_s, ceil(max_k/256)):
+// subdomains on x, which has no 65535 limit.
 template <typename TV>
 __global__ __launch_bounds__(256) void sym_combine_kernel(const int64_t* __restrict__ sub_ptr,
                                                           const int64_t* __restrict__ ypos,
@@ -292,8 +293,8 @@ __global__ __launch_bounds__(256) void sym_combine_kernel(const int64_t* __restr
                                                           TV* __restrict__ y,
                                                           const dds_pcg_state* __restrict__ gate) {
   if (gate != nullptr && gate->done) return;
-  const int s = blockIdx.y;
-  const int a = blockIdx.x * 256 + threadIdx.x;
+  const int s = blockIdx.x;
+  const int a = blockIdx.y * 256 + threadIdx.x;
   const int64_t base = sub_ptr[s];
   const int k = (int)(sub_ptr[s + 1] - base);
   if (a >= k) return;
@@ -366,10 +367,11 @@ extern "C" int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, 
 extern "C" int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, const int64_t* sub_ptr,
                                const int64_t* ypos, const void* Yp, void* y,
                                const dds_pcg_state* gate, void* stream) {
-  DDS_REQUIRE(n_s >= 0 && n_s <= 65535 && max_k >= 0, "dds_sym_combine: bad sizes");
+  DDS_REQUIRE(n_s >= 0 && n_s <= INT32_MAX && max_k >= 0 && ceil_div(max_k, 256) <= 65535,
+              "dds_sym_combine: bad sizes");
   if (n_s == 0 || max_k == 0) return DDS_OK;
   DDS_REQUIRE(sub_ptr && ypos && Yp && y, "dds_sym_combine: null pointer");
-  const dim3 grid((unsigned)ceil_div(max_k, 256), (unsigned)n_s);
+  const dim3 grid((unsigned)n_s, (unsigned)ceil_div(max_k, 256));
   hipStream_t s = as_stream(stream);
   if (dtype == DDS_F64) {
     hipLaunchKernelGGL(sym_combine_kernel<double>, grid, dim3(256), 0, s, sub_ptr, ypos,

@@ -1,0 +1,394 @@
+"""Discontinuous Galerkin (SIPG) systems on the reference's uniform triangle meshes.
+
+Host-side problem generation (numpy/scipy) for the DG experiments of the reference, which
+assembles them with DOLFINx (src/experiments/experiments/discretization.py:34-96) on the
+mesh family `UniformMeshes(2, m)` (src/experiments/experiments/meshing.py:197-283).
+
+Mesh S_m. The reference starts from `create_unit_square(1, 1)` (two right triangles split
+by the (0,0)-(1,1) diagonal) and refines it m times with DOLFINx's Plaza refinement, every
+edge marked (meshing.py:62-73, 207-213). For a right isosceles triangle with right-angle
+vertex R and hypotenuse PQ that refinement bisects the hypotenuse at M = (P+Q)/2, then the
+two legs at M1 = (P+R)/2 and M2 = (Q+R)/2, joining each to M: the children are
+(M1; P, M), (M1; R, M), (M2; R, M), (M2; Q, M), again right isosceles with the right angle
+first. So S_m is the 2^m x 2^m grid of squares with diagonals that alternate in a fixed
+pattern (not the one-direction grid that midpoint subdivision would give); from level 1
+on it does not depend on which diagonal S_0 used. Cells are numbered by the recursion
+(the children of cell t are 4t .. 4t+3), so the S_k cell containing a cell of S_m is its
+index shifted right by 2(m - k). Vertex numbering, cell numbering and the local node order
+differ from DOLFINx, which changes no norm and no Schwarz iterate (the DoFs are renumbered
+by `asm_permutation` anyway).
+
+Subdomain and coarse maps. `parent_cells(k)` is the S_k cell containing each cell (the
+reference composes the refinement parent maps, meshing.py:251-256); `quad_cells(k)` is the
+cell of the 2^k x 2^k quadrilateral mesh C_k (meshing.py:266-283).
+
+Space and form (discretization.py:40-96). DG_p with a nodal Lagrange basis: the vertices,
+for p = 2 the edge midpoints, for p = 3 the two Gauss-Lobatto-Legendre points of each edge
+and the centroid (the points of DOLFINx's default "DG" element, which uses GLL-warped
+points; for p <= 3 these are fixed by symmetry). rho and f are interpolated into the same
+space. With h_e the edge length, delta p^2 = delta * p^2 and w = rho+ rho- / (rho+ + rho-),
+
+  a(u, v) = sum_T (rho grad u, grad v)_T
+          + sum_interior (delta p^2 / h_e * w [u], [v])_e + sum_boundary (delta p^2 rho / h_e u, v)_e
+          - sum_interior ([u] n+, w (grad v+ + grad v-))_e - sum_boundary (u n, rho grad v)_e
+          - (the same two terms with u and v exchanged),
+  b_i = (f_h, phi_i).
+
+Every integrand is a polynomial (rho_h and f_h are piecewise polynomials; on an interior
+edge the two traces of rho_h coincide, so w = rho_h / 2), and the rules used here are exact
+for its degree, so the entries equal DOLFINx's up to rounding.
+
+Pins (tests/test_dg.py): the degrees of freedom, matrix nonzeros and ||b|| that the
+reference's runs record for p = 1 (S1-S11) and p = 3 (S1-S10) in
+results/experiment_amgx_pcg.csv (the first residual norm of a CG run from x0 = 0 is ||b||);
+the Hybrid Schwarz initial residual ||b - A T0(b)|| and the first Additive Schwarz
+residuals of the reference's S10 / S8 / S8 runs (results/experiment_precisions.csv); and
+second-order convergence (p = 1) to the test case's exact solution.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Callable
+
+import numpy as np
+import scipy.sparse as sps
+import torch
+
+from .csr import asm_permutation
+
+__all__ = ["UniformTriangleMesh", "DGTestCase", "DGProblem", "continuous_coefficient_2d",
+           "constant_coefficient_2d", "reference_nodes", "assemble_sipg", "dg_experiment"]
+
+
+# ---------------------------------------------------------------------------------------
+# Mesh
+# ---------------------------------------------------------------------------------------
+
+class UniformTriangleMesh:
+    """S_m: the unit square after m Plaza refinements of its two right triangles."""
+
+    def __init__(self, m: int):
+        if m < 0:
+            raise ValueError("m must be >= 0")
+        n = 2 ** m
+        self.m, self.n = m, n
+        # triangles as (right-angle vertex, hypotenuse end, hypotenuse end) on the n-grid;
+        # S_0 = create_unit_square(1, 1): cells (v0, v1, v3) and (v0, v2, v3)
+        T = np.array([[[n, 0], [0, 0], [n, n]], [[0, n], [0, 0], [n, n]]], dtype=np.int64)
+        for _ in range(m):
+            R, P, Q = T[:, 0], T[:, 1], T[:, 2]
+            M, M1, M2 = (P + Q) // 2, (P + R) // 2, (Q + R) // 2
+            T = np.stack([np.stack([M1, P, M], 1), np.stack([M1, R, M], 1),
+                          np.stack([M2, R, M], 1), np.stack([M2, Q, M], 1)], 1).reshape(-1, 3, 2)
+        j, i = np.divmod(np.arange((n + 1) ** 2), n + 1)
+        self.vertices = np.stack([i, j], 1).astype(np.float64) / n
+        self.cells = T[..., 1] * (n + 1) + T[..., 0]
+
+    @property
+    def num_cells(self) -> int:
+        return self.cells.shape[0]
+
+    def centroids(self) -> np.ndarray:
+        return self.vertices[self.cells].mean(axis=1)
+
+    def parent_cells(self, k: int) -> np.ndarray:
+        """Index of the S_k cell that contains each cell (k <= m)."""
+        if not 0 <= k <= self.m:
+            raise ValueError(f"level {k} is not in [0, {self.m}]")
+        return np.arange(self.num_cells, dtype=np.int64) >> (2 * (self.m - k))
+
+    def quad_cells(self, k: int) -> np.ndarray:
+        """Index of the cell of the 2^k x 2^k quadrilateral mesh C_k that contains each cell."""
+        q = 2 ** k
+        c = self.centroids() * q
+        return np.floor(c[:, 1]).astype(np.int64) * q + np.floor(c[:, 0]).astype(np.int64)
+
+    def level_map(self, name: str) -> np.ndarray:
+        """Cell map to a mesh named as in the reference: "S<k>" or "C<k>"."""
+        kind, k = name[0], int(name[1:])
+        if kind == "S":
+            return self.parent_cells(k)
+        if kind == "C":
+            return self.quad_cells(k)
+        raise ValueError(f"unknown mesh id {name!r}")
+
+
+# ---------------------------------------------------------------------------------------
+# Test cases (src/experiments/experiments/problems.py:33-92)
+# ---------------------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class DGTestCase:
+    name: str
+    delta: float
+    rho: Callable[[np.ndarray, np.ndarray], np.ndarray]
+    f: Callable[[np.ndarray, np.ndarray], np.ndarray]
+    solution: Callable[[np.ndarray, np.ndarray], np.ndarray]
+
+
+_pi = np.pi
+
+constant_coefficient_2d = DGTestCase(
+    "constant coefficient 2D", 7.0,
+    lambda x, y: np.ones_like(x),
+    lambda x, y: 2 * _pi ** 2 * np.sin(_pi * x) * np.sin(_pi * y),
+    lambda x, y: np.sin(_pi * x) * np.sin(_pi * y))
+
+continuous_coefficient_2d = DGTestCase(
+    "continuous coefficient 2D", 7.0,
+    lambda x, y: x ** 2 + y ** 2 + 1,
+    lambda x, y: (8 * _pi) * (4 * _pi * (x ** 2 + y ** 2 + 1) * np.sin(4 * _pi * x) * np.sin(4 * _pi * y)
+                              - x * np.cos(4 * _pi * x) * np.sin(4 * _pi * y)
+                              - y * np.sin(4 * _pi * x) * np.cos(4 * _pi * y)),
+    lambda x, y: np.sin(4 * _pi * x) * np.sin(4 * _pi * y))
+
+
+# ---------------------------------------------------------------------------------------
+# Reference element
+# ---------------------------------------------------------------------------------------
+
+def reference_nodes(p: int) -> np.ndarray:
+    """Nodes of the DG_p basis on the triangle (0,0), (1,0), (0,1)."""
+    V = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    if p == 1:
+        return V
+    if p == 2:
+        return np.vstack([V, [[0.5, 0.0], [0.5, 0.5], [0.0, 0.5]]])
+    if p == 3:
+        g = 0.5 - 0.5 / np.sqrt(5.0)  # interior GLL points of [0, 1]: g, 1 - g
+        edges = []
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            for t in (g, 1.0 - g):
+                edges.append(V[a] + t * (V[b] - V[a]))
+        return np.vstack([V, np.array(edges), [[1.0 / 3.0, 1.0 / 3.0]]])
+    raise ValueError(f"DG degree {p} is not supported (1, 2 or 3)")
+
+
+def _monomials(p: int):
+    return [(a, d - a) for d in range(p + 1) for a in range(d, -1, -1)]
+
+
+def _mono_eval(p: int, xi: np.ndarray):
+    """Monomials xi^a eta^b (a + b <= p) and their gradients at points xi (..., 2)."""
+    e = _monomials(p)
+    x, y = xi[..., 0], xi[..., 1]
+    val = np.stack([x ** a * y ** b for a, b in e], -1)
+    dx = np.stack([a * x ** max(a - 1, 0) * y ** b if a else np.zeros_like(x) for a, b in e], -1)
+    dy = np.stack([b * x ** a * y ** max(b - 1, 0) if b else np.zeros_like(x) for a, b in e], -1)
+    return val, np.stack([dx, dy], -1)
+
+
+class _Element:
+    def __init__(self, p: int):
+        self.p = p
+        self.nodes = reference_nodes(p)
+        self.nloc = self.nodes.shape[0]
+        V, _ = _mono_eval(p, self.nodes)
+        self.C = np.linalg.inv(V)  # nodal basis: phi = monomials @ C
+
+    def eval(self, xi: np.ndarray):
+        """Basis values (..., nloc) and reference gradients (..., nloc, 2)."""
+        m, dm = _mono_eval(self.p, xi)
+        return m @ self.C, np.swapaxes(np.swapaxes(dm, -1, -2) @ self.C, -1, -2)
+
+
+def _wsum(w: np.ndarray, Pi: np.ndarray, Pj: np.ndarray) -> np.ndarray:
+    """(e, i, j) = sum_q w[e, q] Pi[e, q, i] Pj[e, q, j]."""
+    return np.matmul(np.swapaxes(w[..., None] * Pi, 1, 2), Pj)
+
+
+def _triangle_rule(degree: int):
+    """Collapsed Gauss-Legendre rule on the reference triangle, exact to `degree`."""
+    ng = degree // 2 + 2
+    x, w = np.polynomial.legendre.leggauss(ng)
+    u, wu = (x + 1) / 2, w / 2
+    U, Vv = np.meshgrid(u, u, indexing="ij")
+    WU, WV = np.meshgrid(wu, wu, indexing="ij")
+    pts = np.stack([U * (1 - Vv), Vv], -1).reshape(-1, 2)
+    wts = (WU * WV * (1 - Vv)).reshape(-1)
+    return pts, wts
+
+
+def _line_rule(degree: int):
+    ng = degree // 2 + 1
+    x, w = np.polynomial.legendre.leggauss(ng)
+    return (x + 1) / 2, w / 2
+
+
+# ---------------------------------------------------------------------------------------
+# Assembly
+# ---------------------------------------------------------------------------------------
+
+@dataclass
+class DGProblem:
+    """A DG system in the mesh numbering: DoF `c * nloc + l` is node l of cell c."""
+
+    mesh: UniformTriangleMesh
+    p: int
+    A: sps.csr_matrix
+    b: np.ndarray
+    fine_to_dofs: np.ndarray      # (num_cells, nloc)
+    node_coords: np.ndarray       # (num_cells, nloc, 2)
+
+    @property
+    def block_size(self) -> int:
+        return self.fine_to_dofs.shape[1]
+
+
+def assemble_sipg(mesh: UniformTriangleMesh, p: int, case: DGTestCase = continuous_coefficient_2d,
+                  chunk: int = 1 << 18) -> DGProblem:
+    """SIPG matrix (CSR, full block pattern) and load vector of `case` on `mesh`."""
+    el = _Element(p)
+    nl = el.nloc
+    nc = mesh.num_cells
+    X = mesh.vertices[mesh.cells]                      # (C, 3, 2)
+    v0 = X[:, 0]
+    Jm = np.stack([X[:, 1] - v0, X[:, 2] - v0], -1)    # columns: J[:, :, 0], J[:, :, 1]
+    det = Jm[:, 0, 0] * Jm[:, 1, 1] - Jm[:, 0, 1] * Jm[:, 1, 0]
+    Jinv = np.stack([np.stack([Jm[:, 1, 1], -Jm[:, 0, 1]], -1),
+                     np.stack([-Jm[:, 1, 0], Jm[:, 0, 0]], -1)], -2) / det[:, None, None]
+    adet = np.abs(det)
+    nodes = v0[:, None, :] + np.einsum("cab,kb->cka", Jm, el.nodes)   # (C, nl, 2)
+    rho_n = case.rho(nodes[..., 0], nodes[..., 1])
+    f_n = case.f(nodes[..., 0], nodes[..., 1])
+    dp2 = case.delta * p * p
+
+    # Cell terms: stiffness with rho_h (degree 3p - 2) and the load (f_h, phi) (degree 2p).
+    qp, qw = _triangle_rule(max(3 * p - 2, 2 * p))
+    Phi, dPhi = el.eval(qp)                            # (nq, nl), (nq, nl, 2)
+    nq = qw.shape[0]
+    Mref = (Phi.T * qw) @ Phi
+    diag = np.zeros((nc, nl, nl))
+    for s in range(0, nc, chunk):
+        e = slice(s, min(s + chunk, nc))
+        G = np.matmul(dPhi[None], Jinv[e][:, None])    # (c, nq, nl, 2) physical gradients
+        S = adet[e, None] * qw[None, :] * (rho_n[e] @ Phi.T)
+        Gt = np.swapaxes(G, 1, 2).reshape(-1, nl, nq * 2)
+        St = np.swapaxes(S[:, :, None, None] * G, 1, 2).reshape(-1, nl, nq * 2)
+        diag[e] = np.matmul(St, np.swapaxes(Gt, 1, 2))
+    b = (adet[:, None] * (f_n @ Mref)).reshape(-1)     # Mref symmetric
+
+    # Edges: local edge l of a cell joins vertices (l, l + 1 mod 3).
+    lv = np.array([[0, 1], [1, 2], [2, 0]])
+    ev = mesh.cells[:, lv]                             # (C, 3, 2)
+    lo, hi = ev.min(-1).reshape(-1), ev.max(-1).reshape(-1)
+    key = lo * mesh.vertices.shape[0] + hi
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    first = np.ones(ks.shape[0], bool)
+    first[1:] = ks[1:] != ks[:-1]
+    starts = np.flatnonzero(first)
+    counts = np.diff(np.append(starts, ks.shape[0]))
+    occ_cell = order // 3
+    occ_edge = order % 3
+    int_s = starts[counts == 2]
+    bnd_s = starts[counts == 1]
+
+    lq, lw = _line_rule(3 * p)
+
+    def side(cells, a, b_, t):
+        """Basis values, physical gradients and rho_h of `cells` at points a + t (b - a)."""
+        x = a[:, None, :] + t[None, :, None] * (b_ - a)[:, None, :]
+        xi = np.matmul(x - v0[cells][:, None, :], np.swapaxes(Jinv[cells], 1, 2))
+        ph, dph = el.eval(xi)
+        g = np.matmul(dph, Jinv[cells][:, None])
+        r = np.matmul(ph, rho_n[cells][:, :, None])[..., 0]
+        return ph, g, r
+
+    def outward(cells, ledge, a, b_):
+        d = b_ - a
+        h = np.hypot(d[:, 0], d[:, 1])
+        n = np.stack([d[:, 1], -d[:, 0]], -1) / h[:, None]
+        opp = X[cells, (ledge + 2) % 3]
+        flip = np.einsum("ea,ea->e", n, opp - a) > 0
+        n[flip] *= -1
+        return n, h
+
+    def normal_derivative(G, n):
+        return np.matmul(G, n[:, None, :, None])[..., 0]
+
+    rows_blk, cols_blk, vals_blk = [], [], []
+    diag_cells, diag_vals = [], []
+    for s in range(0, int_s.shape[0], chunk):
+        st = int_s[s:s + chunk]
+        cp, cm = occ_cell[st], occ_cell[st + 1]
+        le_p = occ_edge[st]
+        a = mesh.vertices[ev[cp, le_p, 0]]
+        b_ = mesh.vertices[ev[cp, le_p, 1]]
+        n, h = outward(cp, le_p, a, b_)
+        P1, G1, r1 = side(cp, a, b_, lq)
+        P2, G2, r2 = side(cm, a, b_, lq)
+        w = r1 * r2 / (r1 + r2)
+        gam = dp2 / h[:, None] * w
+        D1, D2 = normal_derivative(G1, n), normal_derivative(G2, n)
+        W = lw[None, :] * h[:, None]
+        sides = ((cp, P1, D1, 1.0), (cm, P2, D2, -1.0))
+        for ci, Pi, Di, sgi in sides:
+            for cj, Pj, Dj, sgj in sides:
+                B = (_wsum(W * gam * (sgi * sgj), Pi, Pj) - _wsum(W * w * sgj, Di, Pj)
+                     - _wsum(W * w * sgi, Pi, Dj))
+                if ci is cj:
+                    diag_cells.append(ci)
+                    diag_vals.append(B)
+                else:
+                    rows_blk.append(ci)
+                    cols_blk.append(cj)
+                    vals_blk.append(B)
+    for s in range(0, bnd_s.shape[0], chunk):
+        st = bnd_s[s:s + chunk]
+        c, le = occ_cell[st], occ_edge[st]
+        a = mesh.vertices[ev[c, le, 0]]
+        b_ = mesh.vertices[ev[c, le, 1]]
+        n, h = outward(c, le, a, b_)
+        P, G, r = side(c, a, b_, lq)
+        D = normal_derivative(G, n)
+        W = lw[None, :] * h[:, None]
+        diag_cells.append(c)
+        diag_vals.append(_wsum(W * dp2 * r / h[:, None], P, P) - _wsum(W * r, D, P)
+                         - _wsum(W * r, P, D))
+    dc = np.concatenate(diag_cells)
+    diag += np.bincount((dc[:, None] * nl * nl + np.arange(nl * nl)[None, :]).reshape(-1),
+                        weights=np.concatenate(diag_vals).reshape(-1),
+                        minlength=nc * nl * nl).reshape(nc, nl, nl)
+
+    # Block CSR: the diagonal block of every cell and the two couplings of every interior edge.
+    br = np.concatenate([np.arange(nc)] + rows_blk)
+    bc = np.concatenate([np.arange(nc)] + cols_blk)
+    bv = np.concatenate([diag.reshape(nc, nl, nl)] + vals_blk)
+    o = np.lexsort((bc, br))
+    br, bc, bv = br[o], bc[o], bv[o]
+    indptr = np.zeros(nc + 1, np.int64)
+    indptr[1:] = np.cumsum(np.bincount(br, minlength=nc))
+    A = sps.bsr_matrix((bv, bc, indptr), shape=(nc * nl, nc * nl)).tocsr()
+    A.indices = A.indices.astype(np.int32)
+    A.indptr = A.indptr.astype(np.int64)
+    dofs = np.arange(nc * nl, dtype=np.int64).reshape(nc, nl)
+    return DGProblem(mesh, p, A, b, dofs, nodes)
+
+
+def dg_experiment(problem: DGProblem, solvers_m: str, coarse_m: str):
+    """The reference's experiment set-up (experiments.py:189-240) on the host: returns
+    (perm, A_perm, b_perm, dofs_per_solver, solvers_per_coarse) with
+    A_perm = A[perm][:, perm] (scipy CSR) in the numbering of asm_permutation, solvers on
+    mesh `solvers_m` and coarse cells on mesh `coarse_m` ("S<k>" or "C<k>")."""
+    mesh = problem.mesh
+    f2s = mesh.level_map(solvers_m)
+    if solvers_m[0] == "S":
+        s2c = UniformTriangleMesh(int(solvers_m[1:])).level_map(coarse_m)
+    else:
+        # C_k cells map to the coarser C_j by their index on the 2^k grid
+        k, j = int(solvers_m[1:]), int(coarse_m[1:])
+        q = 2 ** k
+        J, I = np.divmod(np.arange(q * q), q)
+        s2c = (J >> (k - j)) * (2 ** j) + (I >> (k - j))
+    perm = asm_permutation(torch.as_tensor(f2s), torch.as_tensor(s2c),
+                           torch.as_tensor(problem.fine_to_dofs)).numpy()
+    A = problem.A[perm][:, perm].tocsr()
+    A.sort_indices()
+    A.indices = A.indices.astype(np.int32)
+    A.indptr = A.indptr.astype(np.int64)
+    n_c = int(s2c.max()) + 1
+    dofs_per_solver = int(np.sum(f2s == 0)) * problem.block_size
+    spc = np.bincount(s2c, minlength=n_c)
+    return perm, A, np.ascontiguousarray(problem.b[perm]), dofs_per_solver, spc

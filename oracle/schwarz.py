@@ -22,11 +22,19 @@ dtype of A_i, A_C, R0A); `storage` is Inv.precision; `accumulate` is the local-a
 accumulator (the reference accumulates in the vector dtype, gemv_kernels.cu:47-54);
 `scaled` stores A_i^-1 / max|A_i^-1| per subdomain (no counterpart in the reference).
 
+`BlockSchwarzOracle` is the reference's own case (equal, contiguous, non-overlapping
+subdomains of k unknowns, coarse cells = runs of `solvers_per_coarse` subdomains,
+solvers.py:554-556, 590-598) in FP64, vectorised over the subdomains and with a sparse LU
+coarse solve, so that the reference's DG runs (10^5 - 10^6 coarse cells) fit on a CPU.
+
 Pins (tests/test_oracle.py): the reference's 24-unknown DG fixture solved to
 ||A x - b|| < 1e-10 (src/dd_solvers/tests/test_solvers.py:153-176); M_add built as an
 explicit dense matrix is SPD; the Hybrid projection P = I - R_C^T A_C^-1 R_C A satisfies
 P^2 = P and A P = P^T A (preprint.pdf p.6-7); one subdomain covering every unknown gives
 M = A^-1 (one CG iteration); single-unknown subdomains without a coarse space give Jacobi.
+BlockSchwarzOracle agrees with SchwarzOracle (tests/test_oracle.py) and, through
+tests/test_dg.py, reproduces the iteration counts and residual histories of the
+reference's published FP64 DG runs (results/experiment_precisions.csv).
 """
 
 from __future__ import annotations
@@ -34,10 +42,11 @@ from __future__ import annotations
 import numpy as np
 import scipy.linalg as sla
 import scipy.sparse as sps
+import scipy.sparse.linalg as spla
 
 from .kernels import quantize
 
-__all__ = ["SchwarzOracle"]
+__all__ = ["SchwarzOracle", "BlockSchwarzOracle"]
 
 _NP = {"float64": np.float64, "float32": np.float32}
 
@@ -126,3 +135,54 @@ class SchwarzOracle:
         """The preconditioner as an explicit n x n matrix (columns = apply(e_j))."""
         E = np.eye(self.n)
         return np.stack([self.apply(E[:, j]) for j in range(self.n)], axis=1)
+
+
+class BlockSchwarzOracle:
+    """FP64 Additive / Hybrid Schwarz for equal, contiguous, non-overlapping subdomains of k
+    unknowns (the reference's layout), vectorised, with A_C factored by sparse LU."""
+
+    def __init__(self, A: sps.csr_matrix, k: int, solvers_per_coarse=None,
+                 kind: str = "additive"):
+        A = sps.csr_matrix(A, dtype=np.float64)
+        n = A.shape[0]
+        if n % k:
+            raise ValueError("the unknowns must split into subdomains of k")
+        self.kind, self.n, self.k = kind, n, k
+        n_s = n // k
+        self.n_s = n_s
+        rows = np.repeat(np.arange(n), np.diff(A.indptr))
+        cols = A.indices.astype(np.int64)
+        own = (rows // k) == (cols // k)
+        D = np.zeros((n_s, k, k))
+        D[rows[own] // k, rows[own] % k, cols[own] % k] = A.data[own]
+        self.inv = np.linalg.inv(D)                 # A_i^-1 (solvers.py:351)
+        spc = np.ones(n_s, np.int64) if solvers_per_coarse is None else np.asarray(
+            solvers_per_coarse, np.int64)
+        self.coarse_ptr = k * np.concatenate([[0], np.cumsum(spc)])
+        n_c = spc.shape[0]
+        cell = np.repeat(np.arange(n_c), np.diff(self.coarse_ptr))
+        RC = sps.csr_matrix((np.ones(n), (cell, np.arange(n))), shape=(n_c, n))
+        self.R0A = (RC @ A).tocsr()                 # R_C A (solvers.py:890-906)
+        self.lu = spla.splu((self.R0A @ RC.T).tocsc())   # A_C = R_C A R_C^T
+
+    def local(self, x: np.ndarray) -> np.ndarray:
+        return np.matmul(self.inv, x.reshape(self.n_s, self.k, 1)).reshape(-1)
+
+    def restrict(self, x: np.ndarray) -> np.ndarray:
+        return np.add.reduceat(x, self.coarse_ptr[:-1])
+
+    def prolong(self, yc: np.ndarray) -> np.ndarray:
+        return np.repeat(yc, np.diff(self.coarse_ptr))
+
+    def coarse_solve(self, xc: np.ndarray) -> np.ndarray:
+        return self.lu.solve(xc)
+
+    def apply(self, r: np.ndarray) -> np.ndarray:
+        r = np.asarray(r, dtype=np.float64)
+        y = self.local(r)
+        if self.kind == "additive":
+            return y + self.prolong(self.coarse_solve(self.restrict(r)))
+        return y - self.prolong(self.coarse_solve(self.R0A @ y))
+
+    def T0(self, b: np.ndarray) -> np.ndarray:
+        return self.prolong(self.coarse_solve(self.restrict(np.asarray(b, np.float64))))

@@ -1,0 +1,60 @@
+"""Extract the reference's published DG results into tests/golden/dg_published.json.
+
+Reads two result files of the reference (text data, not code):
+  results/experiment_amgx_pcg.csv    -> per (p, mesh level): DoFs, matrix nnz and ||b||
+                                        (the first residual norm of CG from x0 = 0)
+  results/experiment_precisions.csv  -> for the FP64 Schwarz runs (preconditioner and
+                                        local-inverse precision float64): iteration
+                                        count and residual-norm history per
+                                        (p, fine, solvers, coarse, Additive/Hybrid)
+
+Run from the repository root in a container that has the reference checked out at
+/root/reference:  python tests/golden/make_dg_published.py
+"""
+
+import ast
+import csv
+import json
+import os
+
+REF = "/root/reference/results"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dg_published.json")
+
+
+def main():
+    csv.field_size_limit(1 << 30)
+    levels = {}
+    with open(os.path.join(REF, "experiment_amgx_pcg.csv")) as fh:
+        for r in csv.DictReader(fh):
+            if r["random rhs"] != "False" or not r["metadata"]:
+                continue
+            md = ast.literal_eval(r["metadata"])
+            if "residual norms" not in md:
+                continue
+            key = f"p{r['p']}_{r['fine m']}"
+            levels.setdefault(key, {"p": int(r["p"]), "level": int(r["fine m"][1:]),
+                                    "dofs": int(r["DoFs"]), "nnz": int(r["matrix nnz"]),
+                                    "b_norm": md["residual norms"][0]})
+    runs = {}
+    with open(os.path.join(REF, "experiment_precisions.csv")) as fh:
+        for r in csv.DictReader(fh):
+            s = r["solver"]
+            if not s.startswith("CG(") or "torch.float64, Inv()" not in s or r["exception"]:
+                continue
+            kind = "additive" if "AdditiveSchwarz" in s else "hybrid"
+            md = ast.literal_eval(r["metadata"])
+            key = f"p{r['p']}_{r['fine m']}_{r['solvers m']}_{r['coarse m']}_{kind}"
+            runs[key] = {"p": int(r["p"]), "fine": r["fine m"], "solvers": r["solvers m"],
+                         "coarse": r["coarse m"], "kind": kind, "rtol": 1e-9,
+                         "iterations": md["iterations"],
+                         "residual_norms": md["residual norms"],
+                         "solver": s}
+    with open(OUT, "w") as fh:
+        json.dump({"source": "mgr0dzicki/dd-solvers results/experiment_amgx_pcg.csv and "
+                             "results/experiment_precisions.csv (rtol 1e-9, FP64 runs)",
+                   "levels": levels, "runs": runs}, fh, indent=1)
+    print(f"wrote {OUT}: {len(levels)} levels, {len(runs)} runs")
+
+
+if __name__ == "__main__":
+    main()

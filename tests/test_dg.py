@@ -1,0 +1,121 @@
+"""DG-SIPG systems of the reference's experiments (dd_solvers.dg) against the reference's
+published results (tests/golden/dg_published.json, extracted from its results/*.csv by
+tests/golden/make_dg_published.py), the SIPG convergence order, and the oracle."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle.pcg import cg
+from oracle.schwarz import BlockSchwarzOracle, SchwarzOracle
+
+from dd_solvers import dg
+
+PUB = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "dg_published.json")))
+
+
+def _nloc(p):
+    return (p + 1) * (p + 2) // 2
+
+
+@pytest.mark.parametrize("p,level", [(1, m) for m in range(1, 9)] + [(3, m) for m in range(1, 7)])
+def test_published_dofs_nnz_and_load_norm(p, level):
+    """DoFs, nonzeros and ||b|| of the reference's DOLFINx assembly, level by level."""
+    ref = PUB["levels"][f"p{p}_S{level}"]
+    pr = dg.assemble_sipg(dg.UniformTriangleMesh(level), p)
+    assert pr.A.shape[0] == ref["dofs"]
+    assert pr.A.nnz == ref["nnz"]
+    nb = float(np.linalg.norm(pr.b))
+    if ref["b_norm"] < 1e-12:   # S1, S2 at p = 1: f vanishes at every vertex
+        assert nb < 1e-12
+    else:
+        assert abs(nb - ref["b_norm"]) <= 1e-13 * ref["b_norm"], (nb, ref["b_norm"])
+    assert abs(pr.A - pr.A.T).max() <= 1e-14 * abs(pr.A).max()
+
+
+def test_published_sizes_of_the_large_levels():
+    """Cells and interior edges of our S_m give the published DoFs and nonzeros of every
+    level the reference ran (nnz = nloc^2 (cells + 2 interior edges))."""
+    for key, ref in PUB["levels"].items():
+        if ref["level"] > 10:
+            continue
+        mesh = dg.UniformTriangleMesh(ref["level"])
+        e = np.sort(mesh.cells[:, [[0, 1], [1, 2], [2, 0]]], axis=-1).reshape(-1, 2)
+        key_ = e[:, 0] * (mesh.n + 1) ** 2 + e[:, 1]
+        _, counts = np.unique(key_, return_counts=True)
+        nl = _nloc(ref["p"])
+        assert nl * mesh.num_cells == ref["dofs"], key
+        assert nl * nl * (mesh.num_cells + 2 * int((counts == 2).sum())) == ref["nnz"], key
+
+
+def test_mesh_hierarchy():
+    """Children lie inside their parents; S_k and C_k maps agree with point location."""
+    fine = dg.UniformTriangleMesh(5)
+    c = fine.centroids()
+    for k in range(0, 5):
+        par = fine.parent_cells(k)
+        coarse = dg.UniformTriangleMesh(k)
+        V = coarse.vertices[coarse.cells[par]]          # the parent triangle of each cell
+        d = lambda a, b, p_: (b[:, 0] - a[:, 0]) * (p_[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (p_[:, 0] - a[:, 0])
+        s = np.stack([d(V[:, 0], V[:, 1], c), d(V[:, 1], V[:, 2], c), d(V[:, 2], V[:, 0], c)], 1)
+        assert np.all((s > 0).all(1) | (s < 0).all(1)), k
+        assert np.array_equal(np.bincount(par), np.full(coarse.num_cells, 4 ** (5 - k)))
+    q = fine.quad_cells(3)
+    assert np.array_equal(q, np.floor(c[:, 1] * 8).astype(int) * 8 + np.floor(c[:, 0] * 8).astype(int))
+
+
+@pytest.mark.parametrize("p,levels,order", [(1, (5, 6, 7), 2), (2, (3, 4, 5), 3), (3, (3, 4, 5), 4)])
+def test_sipg_converges_at_the_expected_order(p, levels, order):
+    """Nodal RMS error against sin(4 pi x) sin(4 pi y) falls as h^(p+1)."""
+    errs = []
+    for m in levels:
+        pr = dg.assemble_sipg(dg.UniformTriangleMesh(m), p)
+        u = spla.spsolve(pr.A.tocsc(), pr.b)
+        xy = pr.node_coords.reshape(-1, 2)
+        ex = dg.continuous_coefficient_2d.solution(xy[:, 0], xy[:, 1])
+        errs.append(float(np.sqrt(np.mean((u - ex) ** 2))))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert rates[-1] > order - 0.3, (errs, rates)
+
+
+@pytest.mark.parametrize("kind", ["additive", "hybrid"])
+@pytest.mark.parametrize("solvers,coarse", [("S2", "S1"), ("S3", "S3"), ("S3", "C1")])
+def test_block_oracle_matches_general_oracle(kind, solvers, coarse):
+    pr = dg.assemble_sipg(dg.UniformTriangleMesh(4), 1)
+    perm, A, b, k, spc = dg.dg_experiment(pr, solvers, coarse)
+    n_s = A.shape[0] // k
+    ptr = np.arange(n_s + 1) * k
+    gen = SchwarzOracle(A, ptr, ptr, np.arange(A.shape[0]), spc, kind=kind)
+    blk = BlockSchwarzOracle(A, k, spc, kind=kind)
+    x = np.random.default_rng(0).standard_normal(A.shape[0])
+    ref = gen.apply(x)
+    assert np.abs(blk.apply(x) - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.allclose(blk.T0(b), gen.T0(b), rtol=1e-12, atol=1e-14 * np.abs(b).max())
+
+
+@pytest.fixture(scope="module")
+def s10_s8_s8():
+    pr = dg.assemble_sipg(dg.UniformTriangleMesh(10), 1)
+    perm, A, b, k, spc = dg.dg_experiment(pr, "S8", "S8")
+    return A, b, BlockSchwarzOracle(A, k, spc)
+
+
+@pytest.mark.parametrize("kind", ["additive", "hybrid"])
+def test_reproduces_published_fp64_runs_s10_s8_s8(s10_s8_s8, kind):
+    """The reference's CG(AdditiveSchwarz / HybridSchwarz(float64, Inv(), CUDSS)) runs on
+    the continuous-coefficient problem, p = 1, S10 fine mesh (6,291,456 DoFs), S8
+    subdomains and coarse cells (131,072 each), rtol 1e-9: iteration count and residual
+    history (results/experiment_precisions.csv)."""
+    A, b, M = s10_s8_s8
+    M.kind = kind
+    ref = PUB["runs"][f"p1_S10_S8_S8_{kind}"]
+    x, md = cg(lambda v: A @ v, b, precond=M.apply, T0=M.T0 if kind == "hybrid" else None,
+               rtol=ref["rtol"], maxiter=2000)
+    assert md["iterations"] == ref["iterations"]
+    got, pub = np.array(md["residual norms"]), np.array(ref["residual_norms"])
+    rel = np.abs(got - pub) / pub
+    assert rel[:10].max() < 1e-11, rel[:10]
+    assert rel.max() < 1e-3, rel.max()
