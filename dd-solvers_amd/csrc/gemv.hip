@@ -10,11 +10,17 @@ namespace {
 // vector type (the semantics of gemv_kernels.cu:31-93). Wave64 blocks; the LDS variant
 // stages each batch's vector once per workgroup.
 // ---------------------------------------------------------------------------------------
+// The Schwarz preconditioners run the same kernels on their local inverses when the
+// subdomains are the reference's (equal, contiguous, non-overlapping): `scale` (optional)
+// multiplies output row block b by scale[b], and `gate` (optional) skips the launch once
+// the PCG state says done. The drop-in op passes neither.
 template <typename TM, typename TV>
 __global__ __launch_bounds__(256) void gemv_rows_kernel(const TM* __restrict__ mat,
                                                         const TV* __restrict__ vec,
                                                         TV* __restrict__ out, int64_t n,
-                                                        int64_t k) {
+                                                        int64_t k, const float* __restrict__ scale,
+                                                        const dds_pcg_state* __restrict__ gate) {
+  if (gate != nullptr && gate->done) return;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (tid >= n * k) return;
   const int64_t b = tid / k;
@@ -23,6 +29,7 @@ __global__ __launch_bounds__(256) void gemv_rows_kernel(const TM* __restrict__ m
   const TV* v = vec + b * k;
   TV acc = 0;
   for (int64_t c = 0; c < k; ++c) acc += static_cast<TV>(widen(M[c * k])) * v[c];
+  if (scale != nullptr) acc *= static_cast<TV>(scale[b]);
   out[b * k + r] = acc;
 }
 
@@ -30,8 +37,11 @@ template <typename TM, typename TV>
 __global__ __launch_bounds__(1024) void gemv_rows_lds_kernel(const TM* __restrict__ mat,
                                                              const TV* __restrict__ vec,
                                                              TV* __restrict__ out, int64_t n,
-                                                             int k, int per_block) {
+                                                             int k, int per_block,
+                                                             const float* __restrict__ scale,
+                                                             const dds_pcg_state* __restrict__ gate) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (gate != nullptr && gate->done) return;
   TV* vs = reinterpret_cast<TV*>(smem_raw);
   const int local = threadIdx.x / k;
   const int r = threadIdx.x - local * k;
@@ -44,12 +54,14 @@ __global__ __launch_bounds__(1024) void gemv_rows_lds_kernel(const TM* __restric
   const TV* v = vs + local * k;
   TV acc = 0;
   for (int c = 0; c < k; ++c) acc += static_cast<TV>(widen(M[(int64_t)c * k])) * v[c];
+  if (scale != nullptr) acc *= static_cast<TV>(scale[b]);
   out[b * k + r] = acc;
 }
 
 template <typename TM, typename TV>
 int launch_gemv(const void* mat, const void* vec, void* out, int64_t n, int64_t k, int use_lds,
-                hipStream_t s) {
+                hipStream_t s, const float* scale = nullptr,
+                const dds_pcg_state* gate = nullptr) {
   if (n == 0 || k == 0) return DDS_OK;
   const TM* m = static_cast<const TM*>(mat);
   const TV* v = static_cast<const TV*>(vec);
@@ -66,11 +78,11 @@ int launch_gemv(const void* mat, const void* vec, void* out, int64_t n, int64_t 
     const int64_t grid = ceil_div(n, per_block);
     const size_t shm = (size_t)per_block * k * sizeof(TV);
     hipLaunchKernelGGL((gemv_rows_lds_kernel<TM, TV>), dim3((unsigned)grid), dim3(block), shm,
-                       s, m, v, o, n, (int)k, per_block);
+                       s, m, v, o, n, (int)k, per_block, scale, gate);
   } else {
     const int64_t grid = ceil_div(n * k, 256);
     hipLaunchKernelGGL((gemv_rows_kernel<TM, TV>), dim3((unsigned)grid), dim3(256), 0, s, m, v,
-                       o, n, k);
+                       o, n, k, scale, gate);
   }
   DDS_LAUNCHED();
   return DDS_OK;
@@ -240,6 +252,33 @@ extern "C" int dds_gemv_strided_batched(int mat_dtype, int vec_dtype, const void
     }
   }
   set_error("gemv_strided_batched: unsupported dtypes (mat %d, vec %d)", mat_dtype, vec_dtype);
+  return DDS_EUNSUPPORTED;
+}
+
+extern "C" int dds_local_apply_batched(int mat_dtype, int vec_dtype, const void* mats,
+                                       const void* x, void* y, int64_t n, int64_t k,
+                                       int use_lds, const float* scale,
+                                       const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(n >= 0 && k >= 0, "dds_local_apply_batched: negative size");
+  DDS_REQUIRE(n * k <= (int64_t)INT32_MAX * 256, "dds_local_apply_batched: n*k too large");
+  DDS_REQUIRE(!use_lds || k <= 1024, "dds_local_apply_batched: LDS variant needs k <= 1024");
+  if (n > 0 && k > 0) DDS_REQUIRE(mats && x && y, "dds_local_apply_batched: null pointer");
+  hipStream_t s = as_stream(stream);
+  if (vec_dtype == DDS_F64) {
+    switch (mat_dtype) {
+      case DDS_F64: return launch_gemv<double, double>(mats, x, y, n, k, use_lds, s, scale, gate);
+      case DDS_F32: return launch_gemv<float, double>(mats, x, y, n, k, use_lds, s, scale, gate);
+      case DDS_F16: return launch_gemv<_Float16, double>(mats, x, y, n, k, use_lds, s, scale, gate);
+      case DDS_BF16: return launch_gemv<bf16_t, double>(mats, x, y, n, k, use_lds, s, scale, gate);
+    }
+  } else if (vec_dtype == DDS_F32) {
+    switch (mat_dtype) {
+      case DDS_F32: return launch_gemv<float, float>(mats, x, y, n, k, use_lds, s, scale, gate);
+      case DDS_F16: return launch_gemv<_Float16, float>(mats, x, y, n, k, use_lds, s, scale, gate);
+      case DDS_BF16: return launch_gemv<bf16_t, float>(mats, x, y, n, k, use_lds, s, scale, gate);
+    }
+  }
+  set_error("dds_local_apply_batched: unsupported dtypes (mat %d, vec %d)", mat_dtype, vec_dtype);
   return DDS_EUNSUPPORTED;
 }
 

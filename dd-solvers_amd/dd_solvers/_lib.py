@@ -62,6 +62,7 @@ _SIGNATURES = {
     "dds_local_vec_width": [_i],
     "dds_local_tile_rows": [_i],
     "dds_local_apply": [_i, _i, _i, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _i, _p, _p],
+    "dds_local_apply_batched": [_i, _i, _p, _p, _p, _i64, _i64, _i, _p, _p, _p],
     "dds_local_apply_sym": [_i, _i, _i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "dds_sym_combine": [_i, _i64, _i64, _p, _p, _p, _p, _p, _p],
     "dds_overlap_sum": [_i, _i64, _p, _p, _p, _p, _p, _d, _p, _p, _p, _ip, _p, _p],
@@ -70,6 +71,8 @@ _SIGNATURES = {
     "dds_chol_solve": [_i, _i64, _p, _i64, _p, _p, _p, _p],
     "dds_chol_inverse": [_i, _i64, _p, _i64, _p, _i64, _p],
     "dds_dense_symv": [_i, _i64, _p, _i64, _p, _p, _p, _p],
+    "dds_nd_forward": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p],
+    "dds_nd_backward": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p],
     "dds_pcg_residual": [_i64, _p, _p, _p, _p, _ip, _p, _p],
     "dds_pcg_update_p": [_i64, _p, _p, _p, _p],
     "dds_pcg_update_xr": [_i64, _p, _p, _p, _p, _p, _ip, _p, _p],

@@ -864,9 +864,15 @@ class Inv(DenseBatchSolver):
                   reference); torch.float32 gives FP32 accumulation for FP64 vectors.
       scaled      store A_i^-1 / max|A_i^-1| per subdomain and rescale the result (keeps
                   FP16 storage in range).
-      layout      "sym" (default): the lower triangle in 64 x 64 tiles, each off-diagonal
-                  tile read once and applied twice (dds_local_apply_sym); "full": the
-                  whole matrix column-major (dds_local_apply); "mfma" (FP16/BF16): the
+      layout      "auto" (default): "batched" for the reference's subdomains (equal,
+                  contiguous, non-overlapping) of at most 64 unknowns with the
+                  accumulator of the vector type, "full" for other subdomains of at most
+                  64 unknowns, else "sym". "batched": the reference's (n, k, k)
+                  column-major array, applied by the gemv_strided_batched kernels
+                  (dds_local_apply_batched). "sym": the lower triangle in 64 x 64 tiles,
+                  each off-diagonal tile read once and applied twice
+                  (dds_local_apply_sym); "full": the whole matrix column-major with a
+                  16-byte leading dimension (dds_local_apply); "mfma" (FP16/BF16): the
                   whole matrix in MFMA operand order, applied to blocks of 16 right-hand
                   sides as a batched GEMM on the matrix cores (dds_mfma_apply; a single
                   right-hand side runs as a block of one).
@@ -875,8 +881,8 @@ class Inv(DenseBatchSolver):
     """
 
     def __init__(self, precision: torch.dtype | None = None, accumulate: torch.dtype | None = None,
-                 scaled: bool = False, layout: str = "sym", autotune: bool = False):
-        if layout not in ("sym", "full", "mfma"):
+                 scaled: bool = False, layout: str = "auto", autotune: bool = False):
+        if layout not in ("auto", "batched", "sym", "full", "mfma"):
             raise ValueError(f"unknown layout {layout!r}")
         if layout == "mfma" and precision not in (torch.float16, torch.bfloat16):
             raise ValueError("layout='mfma' stores the inverses in float16 or bfloat16")
@@ -1000,8 +1006,7 @@ def _sym_plan(sizes: np.ndarray):
     T = -(-sizes // 64)
     P = -(-T // 2)
     ypos = np.concatenate([[0], np.cumsum(P * 64 * T)]).astype(np.int64)
-    items = np.concatenate([np.stack([np.full(int(P[s]), s), np.arange(int(P[s]))], 1)
-                            for s in range(sizes.shape[0])]).astype(np.int32)
+    items = _enumerate(P)
     # longest items first (tiles touched: T + 1, or T - p for the middle row of an odd T),
     # so the last workgroups to start are the short ones
     Ts = T[items[:, 0]]
@@ -1031,12 +1036,19 @@ def _mfma_index(k: int) -> np.ndarray:
     return idx
 
 
+def _enumerate(counts: np.ndarray) -> np.ndarray:
+    """(s, j) for j < counts[s], s in order: an (sum(counts), 2) int32 array."""
+    counts = np.asarray(counts, dtype=np.int64)
+    s = np.repeat(np.arange(counts.shape[0], dtype=np.int64), counts)
+    start = np.cumsum(counts) - counts
+    j = np.arange(s.shape[0], dtype=np.int64) - np.repeat(start, counts)
+    return np.stack([s, j], 1).astype(np.int32).reshape(-1, 2)
+
+
 def _tiles_for(sizes: np.ndarray, rows: int) -> np.ndarray:
-    t = []
-    for s, k in enumerate(sizes):
-        for r0 in range(0, int(k), rows):
-            t.append((s, r0))
-    return np.asarray(t, dtype=np.int32).reshape(-1, 2)
+    t = _enumerate(-(-np.asarray(sizes, dtype=np.int64) // rows))
+    t[:, 1] *= rows
+    return t
 
 
 class SchwarzOperator(SparseSolver):
@@ -1143,11 +1155,10 @@ class SchwarzOperator(SparseSolver):
         rows = csr_rows(crow)
         col64 = col.to(torch.int64)
         cell_t = self.cell.to(torch.int64)
-        AC = self._segment_sum_dense(cell_t[rows] * self.n_coarse + cell_t[col64], val,
-                                     self.n_coarse * self.n_coarse)
-        AC = AC.view(self.n_coarse, self.n_coarse)
+        AC = self._segment_sum_csr(cell_t[rows] * self.n_coarse + cell_t[col64], val,
+                                   self.n_coarse, self.n_coarse)
         collect()
-        self.coarse_solver.setup(AC.to(pi).to_sparse_csr())
+        self.coarse_solver.setup(AC.to(pi))
         if self.collect_timings:
             events[2].record()
         self._setup_local(matrix, rows, col64, val, sizes, sub_idx)
@@ -1177,15 +1188,18 @@ class SchwarzOperator(SparseSolver):
         pass
 
     @staticmethod
-    def _segment_sum_dense(key: torch.Tensor, val: torch.Tensor, size: int) -> torch.Tensor:
-        """out[key] = sum of val over equal keys, deterministic (sort + segment_reduce)."""
+    def _segment_sum_csr(key: torch.Tensor, val: torch.Tensor, n_rows: int,
+                         n_cols: int) -> torch.Tensor:
+        """CSR (n_rows x n_cols, FP64) of the sums of val over equal keys row * n_cols + col,
+        deterministic (sort + segment_reduce); the coarse matrix stays sparse, so coarse
+        spaces of millions of cells fit."""
         order = torch.sort(key, stable=True).indices
         ks = key[order]
         uk, counts = torch.unique_consecutive(ks, return_counts=True)
         sums = torch.segment_reduce(val[order].to(torch.float64), "sum", lengths=counts)
-        out = torch.zeros(size, dtype=torch.float64, device=val.device)
-        out[uk] = sums
-        return out
+        crow = torch.zeros(n_rows + 1, dtype=torch.int64, device=val.device)
+        crow[1:] = torch.cumsum(torch.bincount(uk // n_cols, minlength=n_rows), 0)
+        return torch.sparse_csr_tensor(crow, uk % n_cols, sums, size=(n_rows, n_cols))
 
     def _setup_local(self, matrix, rows, col64, val, sizes, sub_idx):
         """Extract A_i = R_i A R_i^T, invert in the preconditioner precision and pack the
@@ -1247,6 +1261,16 @@ class SchwarzOperator(SparseSolver):
         width = _lib._lib.dds_local_vec_width(_lib.dtype_code(store_dtype))
         rows_per_tile = _lib._lib.dds_local_tile_rows(_lib.dtype_code(store_dtype))
         layout = getattr(ls, "layout", "full")
+        reference_case = sub_idx is None and bool(np.all(sizes == sizes[0]))
+        if layout == "auto":
+            if reference_case and k_max <= 64 and acc == pi:
+                layout = "batched"
+            else:
+                layout = "full" if k_max <= 64 else "sym"
+        if layout == "batched" and not reference_case:
+            raise ValueError("layout='batched' needs equal, contiguous, non-overlapping subdomains")
+        if layout == "batched" and acc != pi:
+            raise ValueError("layout='batched' accumulates in the vector dtype")
         if layout == "mfma":
             if store_dtype not in (torch.float16, torch.bfloat16):
                 raise ValueError("layout='mfma' needs float16 or bfloat16 storage")
@@ -1262,7 +1286,8 @@ class SchwarzOperator(SparseSolver):
             mat_off = np.concatenate([[0], np.cumsum(sym_len)]).astype(np.int64)
             sym_idx_dev = {}
         else:
-            ld = ((sizes + width - 1) // width) * width
+            # "batched": the reference's (n, k, k) column-major array (ld = k)
+            ld = sizes.copy() if layout == "batched" else ((sizes + width - 1) // width) * width
             mat_off = np.concatenate([[0], np.cumsum(ld * sizes)]).astype(np.int64)
         total = int(mat_off[-1])
         mats = torch.zeros(total, dtype=store_dtype, device=dev)
@@ -1300,36 +1325,47 @@ class SchwarzOperator(SparseSolver):
             else:
                 blk, kk = build_chunk(c0, c1)
                 inv = torch.linalg.inv(blk)
-            for j in range(c1 - c0):
-                i = c0 + j
-                k = int(sizes[i])
-                m = inv[j, :k, :k]
+            # pack every subdomain of one size at once
+            ks = sizes[c0:c1]
+            for k in np.unique(ks):
+                k = int(k)
+                js = np.flatnonzero(ks == k)
+                jt = torch.as_tensor(js, device=dev)
+                m = inv[jt, :k, :k]
                 if scaled:
-                    sc = m.abs().max().to(torch.float32)
-                    scale[i] = sc
-                    m = m / sc.to(m.dtype)
+                    sc = m.abs().amax(dim=(1, 2)).to(torch.float32)
+                    scale[jt + c0] = sc
+                    m = m / sc.to(m.dtype)[:, None, None]
                 if layout == "mfma":
                     if k not in mf_idx_dev:
                         mf_idx_dev[k] = torch.as_tensor(mf_idx[k], device=dev)
                     src = mf_idx_dev[k]
-                    packed = torch.where(src >= 0, m.reshape(-1)[src.clamp(min=0)],
+                    packed = torch.where(src >= 0, m.reshape(len(js), -1)[:, src.clamp(min=0)],
                                          torch.zeros((), dtype=pi, device=dev))
-                    mats[int(mat_off[i]):int(mat_off[i + 1])] = packed.to(store_dtype)
                 elif layout == "sym":
                     if k not in sym_idx_dev:
                         sym_idx_dev[k] = torch.as_tensor(sym_idx[k], device=dev)
                     src = sym_idx_dev[k]
                     half = src <= -2
                     at = torch.where(half, -2 - src, src).clamp(min=0)
-                    vals = m.reshape(-1)[at]
+                    vals = m.reshape(len(js), -1)[:, at]
                     packed = torch.where(src >= 0, vals, torch.where(half, 0.5 * vals,
                                                                      torch.zeros((), dtype=pi, device=dev)))
-                    mats[int(mat_off[i]):int(mat_off[i + 1])] = packed.to(store_dtype)
                 else:
                     # column-major with padded leading dimension: element (a, b) at b*ld + a
-                    col_major = torch.zeros((k, int(ld[i])), dtype=pi, device=dev)
-                    col_major[:, :k] = m.transpose(0, 1)
-                    mats[int(mat_off[i]):int(mat_off[i + 1])] = col_major.reshape(-1).to(store_dtype)
+                    ldk = int(ld[c0 + js[0]])
+                    col_major = torch.zeros((len(js), k, ldk), dtype=pi, device=dev)
+                    col_major[:, :, :k] = m.transpose(1, 2)
+                    packed = col_major.reshape(len(js), -1)
+                off = mat_off[c0 + js]
+                L = packed.shape[1]
+                if np.all(np.diff(off) == L):  # consecutive subdomains: one slice
+                    mats[int(off[0]):int(off[0]) + len(js) * L] = packed.reshape(-1).to(store_dtype)
+                else:
+                    dst = torch.as_tensor(off, device=dev)[:, None] + \
+                        torch.arange(L, device=dev)[None, :]
+                    mats[dst.reshape(-1)] = packed.reshape(-1).to(store_dtype)
+                del m, packed
             del blk, inv
         esz = mats.element_size()
         if layout == "mfma":
@@ -1337,8 +1373,7 @@ class SchwarzOperator(SparseSolver):
             RB = -(-sizes // 16)
             KP = -(-sizes // 32)
             groups = -(-RB // 4)
-            items = np.concatenate([np.stack([np.full(int(groups[s_]), s_), np.arange(int(groups[s_]))], 1)
-                                    for s_ in range(n_s)]).astype(np.int32)
+            items = _enumerate(groups)
             # longest items first: work = row blocks in the group x column pairs
             work = np.minimum(4, RB[items[:, 0]] - 4 * items[:, 1]) * KP[items[:, 0]]
             items = items[np.argsort(-work, kind="stable")]
@@ -1393,6 +1428,15 @@ class SchwarzOperator(SparseSolver):
         if L.layout == "sym":
             self._enqueue_sym_apply(x, gate)
             self._enqueue_sym_combine(Y, gate)
+            return
+        if L.layout == "batched":
+            k = L.max_k
+            # LDS staging of x as gemv.py:8-14 chooses it
+            lds = (5 < k) if self.pi == torch.float64 else (15 < k)
+            _lib.call("dds_local_apply_batched", _lib.dtype_code(L.mat_dtype),
+                      _lib.dtype_code(self.pi), L.mats.data_ptr(), x.data_ptr(), Y.data_ptr(),
+                      self.n_solvers, k, int(lds), _lib.ptr(L.scale), _lib.ptr(gate),
+                      _lib.stream())
             return
         _lib.call("dds_local_apply", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
                   _lib.dtype_code(L.acc_dtype), L.n_tiles, L.tiles.data_ptr(),

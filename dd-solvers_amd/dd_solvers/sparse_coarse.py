@@ -1,0 +1,556 @@
+"""Sparse SPD coarse solver: nested-dissection block LDL^T with level-scheduled HIP sweeps.
+
+The reference's coarse slot is `CUDSS()` (src/dd_solvers/dd_solvers/solvers.py:85-113), a
+sparse Cholesky of the coarse matrix A_C. Its experiments use coarse spaces of up to
+millions of cells (one per cell of the coarse mesh), where a dense factor is out of the
+question. `SparseCholesky` fills that slot without cuDSS:
+
+Ordering (host, setup). Nested dissection of the graph of A_C by recursive bisection:
+from a pseudo-peripheral node (the last node of a breadth-first search), the level set of a
+second breadth-first search that splits the nodes most evenly with the fewest nodes is the
+separator; the two sides are dissected in turn down to `leaf_size` nodes, and disconnected
+pieces become sibling subtrees. Tree nodes at depth < d (the largest d whose separators
+hold at most `top_max` nodes together) form one dense top front T, numbered last; the
+subtrees below it are numbered in post-order, so every front's unknowns J are one
+contiguous range and its coupling set S (the unknowns of ancestors that J touches after
+elimination) lies above it.
+
+Factorisation (device, setup). Multifrontal, one batch per height of the tree (torch FP64
+dense algebra on padded fronts): the front of J is assembled from the entries of A_C in
+the columns J and the children's update matrices (added child by child, so the sums are
+deterministic), and
+
+    Finv = F_JJ^-1,   W = F_SJ Finv,   U = F_SS - W F_JS   (to the parent)
+
+which is the block LDL^T factorisation A = L D L^T with L_SJ = W, D_J = F_JJ. The top
+front is the Schur complement of everything below it; its inverse is dense.
+
+Solve (device, every application; HIP kernels dds_nd_forward / dds_nd_backward, one
+launch per height). Forward, leaves first: f = [b_J; 0] + the children's update vectors
+(gathered in child order), then g_J = Finv f_J and u = f_S - W f_J to the parent; the top
+front's g is its solution. Backward, top first: x_J = g_J - W^T x_S. The stacked rows
+[Finv; W] of a front are contiguous, so the forward sweep is a dense matvec split over
+workgroups by rows and the backward sweep one split by columns; both read each factor
+byte once and add in a fixed order.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Tuple
+
+import numpy as np
+import scipy.sparse as sps
+import scipy.sparse.csgraph as csgraph
+import torch
+
+from . import _lib
+from .solvers import SparseSolver
+
+__all__ = ["SparseCholesky", "nested_dissection", "NDPlan"]
+
+# LDS capacity of the sweep kernels, in elements (f_J forward, x_S backward): 60 KiB of FP64
+LDS_CAP = 7680
+FRONT_FIELDS = 8  # j0, nJ, nS, foff, fpos, fu, fsidx, unused
+
+
+# ---------------------------------------------------------------------------------------
+# Ordering
+# ---------------------------------------------------------------------------------------
+
+def _bfs_levels(G: sps.csr_matrix, src: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Breadth-first order from src and the level of each node in that order."""
+    order, pred = csgraph.breadth_first_order(G, src, directed=True, return_predecessors=True)
+    nv = order.shape[0]
+    pos = np.empty(G.shape[0], dtype=np.int64)
+    pos[order] = np.arange(nv)
+    pp = pos[pred[order[1:]]]  # parent positions, non-decreasing in breadth-first order
+    bounds = [0, 1]
+    while bounds[-1] < nv:
+        nxt = 1 + int(np.searchsorted(pp, bounds[-1], side="left"))
+        if nxt <= bounds[-1]:  # cannot happen for a breadth-first order; guard the loop
+            nxt = nv
+        bounds.append(nxt)
+    lev = np.repeat(np.arange(len(bounds) - 1), np.diff(bounds))
+    return order, lev
+
+
+def _pick_level(lev: np.ndarray) -> int:
+    """Separator level: the smallest level set leaving both sides >= 30% of the rest, else
+    the most balanced one."""
+    cnt = np.bincount(lev)
+    n = lev.shape[0]
+    before = np.cumsum(cnt) - cnt
+    after = n - before - cnt
+    cand = np.arange(1, cnt.shape[0] - 1)
+    bal = np.minimum(before[cand], after[cand]) / np.maximum(n - cnt[cand], 1)
+    ok = cand[bal >= 0.3]
+    if ok.size:
+        return int(ok[np.argmin(cnt[ok])])
+    return int(cand[np.argmax(bal)])
+
+
+@dataclass
+class _Node:
+    J: np.ndarray            # original indices of the unknowns eliminated here
+    depth: int
+    parent: int              # -1 for a root
+    children: List[int] = field(default_factory=list)
+
+
+def nested_dissection(G: sps.csr_matrix, leaf_size: int = 32) -> List[_Node]:
+    """Dissection tree of the symmetric graph G (any values; the diagonal is ignored)."""
+    G = sps.csr_matrix(G, copy=True)
+    G.setdiag(0)
+    G.eliminate_zeros()
+    G.data[:] = 1.0
+    nodes: List[_Node] = []
+
+    def add(J, depth, parent):
+        nodes.append(_Node(np.asarray(J, dtype=np.int64), depth, parent))
+        if parent >= 0:
+            nodes[parent].children.append(len(nodes) - 1)
+        return len(nodes) - 1
+
+    stack = [(G, np.arange(G.shape[0], dtype=np.int64), 0, -1)]
+    while stack:
+        H, ids, depth, parent = stack.pop()
+        n = ids.shape[0]
+        if n <= leaf_size:
+            add(ids, depth, parent)
+            continue
+        order, lev = _bfs_levels(H, 0)
+        if order.shape[0] < n:  # disconnected: each piece is its own subtree
+            nc, lab = csgraph.connected_components(H, directed=False)
+            srt = np.argsort(lab, kind="stable")
+            cuts = np.flatnonzero(np.diff(lab[srt])) + 1
+            for sel in reversed(np.split(srt, cuts)):
+                stack.append((H[sel][:, sel], ids[sel], depth, parent))
+            continue
+        order, lev = _bfs_levels(H, int(order[-1]))
+        if lev[-1] < 2:  # no level splits it: keep it whole
+            add(ids, depth, parent)
+            continue
+        L = _pick_level(lev)
+        sep = order[lev == L]
+        A = np.sort(order[lev < L])
+        B = np.sort(order[lev > L])
+        me = add(ids[np.sort(sep)], depth, parent)
+        stack.append((H[B][:, B], ids[B], depth + 1, me))
+        stack.append((H[A][:, A], ids[A], depth + 1, me))
+    return nodes
+
+
+# ---------------------------------------------------------------------------------------
+# Symbolic plan
+# ---------------------------------------------------------------------------------------
+
+class NDPlan:
+    """Numbering, fronts, levels, gather lists and work items of the block LDL^T solve.
+
+    Fronts are numbered by height (leaves first, the top front last). For front t:
+      fronts[t] = (j0, nJ, nS, foff, fpos, fu, fsidx, 0): unknowns [j0, j0+nJ) of the
+      nested-dissection numbering; factor rows [Finv; W] at fac[foff:foff+(nJ+nS)*nJ]
+      (row-major, nJ columns); its nJ+nS positions start at fpos in the gather lists; its
+      update vector at u[fu:fu+nS]; its coupling set at sidx[fsidx:fsidx+nS].
+    perm[i] is the original index of unknown i of the nested-dissection numbering.
+    """
+
+    def __init__(self, A: sps.csr_matrix, leaf_size: int = 32, top_max: int = 4096):
+        A = sps.csr_matrix(A)
+        n = A.shape[0]
+        if A.shape != (n, n):
+            raise ValueError("coarse matrix must be square")
+        self.n = n
+        nodes = nested_dissection(A, leaf_size)
+        depth = np.array([nd.depth for nd in nodes], dtype=np.int64)
+        size = np.array([nd.J.shape[0] for nd in nodes], dtype=np.int64)
+        dmax = int(depth.max()) + 1
+        # largest d whose nodes of depth < d hold at most top_max unknowns
+        d_top = 0
+        while d_top < dmax and int(size[depth <= d_top].sum()) <= top_max:
+            d_top += 1
+        top = depth < d_top
+        self.d_top = d_top
+        # numbering: lower subtrees in post-order, then the top nodes
+        lower_roots = [i for i, nd in enumerate(nodes)
+                       if not top[i] and (nd.parent < 0 or top[nd.parent])]
+        post: List[int] = []
+        for r in lower_roots:
+            st = [(r, False)]
+            while st:
+                i, done = st.pop()
+                if done:
+                    post.append(i)
+                    continue
+                st.append((i, True))
+                for c in reversed(nodes[i].children):
+                    st.append((c, False))
+        top_nodes = [i for i in range(len(nodes)) if top[i]]
+        perm = np.empty(n, dtype=np.int64)
+        j0 = np.empty(len(nodes), dtype=np.int64)
+        pos = 0
+        for i in post + top_nodes:
+            j0[i] = pos
+            perm[pos:pos + size[i]] = nodes[i].J
+            pos += size[i]
+        assert pos == n
+        self.perm = perm
+        iperm = np.empty(n, dtype=np.int64)
+        iperm[perm] = np.arange(n)
+        Ap = A[perm][:, perm].tocsr()
+        Ap.sort_indices()
+        self.Ap = Ap
+        nT = int(size[top].sum())
+        self.nT = nT
+
+        # fronts: lower nodes (post-order), then the top front (all top nodes merged)
+        F = len(post) + (1 if nT else 0)
+        fj0 = np.zeros(F, dtype=np.int64)
+        fnj = np.zeros(F, dtype=np.int64)
+        S: List[np.ndarray] = [None] * F
+        children: List[List[int]] = [[] for _ in range(F)]
+        front_of = np.full(len(nodes), -1, dtype=np.int64)
+        for t, i in enumerate(post):
+            front_of[i] = t
+        if nT:
+            for i in top_nodes:
+                front_of[i] = F - 1
+        for t, i in enumerate(post):
+            fj0[t], fnj[t] = j0[i], size[i]
+            nd = nodes[i]
+            children[t] = [int(front_of[c]) for c in nd.children]
+            if nd.parent >= 0 and top[nd.parent]:
+                children[F - 1].append(t)
+        if nT:
+            fj0[F - 1], fnj[F - 1] = n - nT, nT
+        indptr, indices = Ap.indptr, Ap.indices
+        for t in range(len(post)):
+            a, e = fj0[t], fj0[t] + fnj[t]
+            adj = indices[indptr[a]:indptr[e]]
+            parts = [adj[adj >= e]] + [S[c][S[c] >= e] for c in children[t]]
+            S[t] = np.unique(np.concatenate(parts)).astype(np.int64)
+        if nT:
+            S[F - 1] = np.zeros(0, dtype=np.int64)
+        fns = np.array([s.shape[0] for s in S], dtype=np.int64)
+        # heights: leaves 0, parents above all their children (the top front last)
+        height = np.zeros(F, dtype=np.int64)
+        for t in range(F):  # children precede parents in this numbering
+            if children[t]:
+                height[t] = 1 + max(int(height[c]) for c in children[t])
+        order = np.argsort(height, kind="stable")
+        rank = np.empty(F, dtype=np.int64)
+        rank[order] = np.arange(F)
+        fj0, fnj, fns = fj0[order], fnj[order], fns[order]
+        S = [S[i] for i in order]
+        children = [[int(rank[c]) for c in children[i]] for i in order]
+        height = height[order]
+        self.n_fronts = F
+        self.fj0, self.fnj, self.fns = fj0, fnj, fns
+        self.S, self.children = S, children
+        self.level_ptr = np.concatenate([[0], np.cumsum(np.bincount(height))]).astype(np.int64)
+        if int(fnj.max()) > LDS_CAP or (fns.size and int(fns.max()) > LDS_CAP):
+            raise ValueError(f"a front exceeds {LDS_CAP} unknowns (nJ {int(fnj.max())}, "
+                             f"nS {int(fns.max())}); lower top_max or leaf_size")
+
+        m = fnj + fns
+        self.foff = np.concatenate([[0], np.cumsum(m * fnj)]).astype(np.int64)
+        self.fpos = np.concatenate([[0], np.cumsum(m)]).astype(np.int64)
+        self.fu = np.concatenate([[0], np.cumsum(fns)]).astype(np.int64)
+        self.fac_size = int(self.foff[-1])
+        self.u_size = int(self.fu[-1])
+        self.sidx = (np.concatenate(S) if F else np.zeros(0)).astype(np.int32)
+        # child maps: positions of each child's coupling set in its parent's front
+        cmap: List[np.ndarray] = [None] * F
+        for t in range(F):
+            a, nj = fj0[t], fnj[t]
+            for c in children[t]:
+                sc = S[c]
+                inJ = sc < a + nj
+                p = np.empty(sc.shape[0], dtype=np.int64)
+                p[inJ] = sc[inJ] - a
+                p[~inJ] = nj + np.searchsorted(S[t], sc[~inJ])
+                assert np.all(sc[inJ] >= a)
+                assert np.array_equal(S[t][p[~inJ] - nj], sc[~inJ]), "coupling set not nested"
+                cmap[c] = p
+        self.cmap = cmap
+        # gather lists of the forward sweep: for each front position, the update-vector
+        # entries of the children that land there, in child order
+        tgt, src = [], []
+        for t in range(F):
+            for c in children[t]:
+                tgt.append(self.fpos[t] + cmap[c])
+                src.append(self.fu[c] + np.arange(fns[c], dtype=np.int64))
+        if tgt:
+            tgt = np.concatenate(tgt)
+            src = np.concatenate(src)
+            o = np.argsort(tgt, kind="stable")
+            tgt, src = tgt[o], src[o]
+        else:
+            tgt = src = np.zeros(0, dtype=np.int64)
+        self.gptr = np.searchsorted(tgt, np.arange(self.fpos[-1] + 1)).astype(np.int64)
+        self.gsrc = src.astype(np.int32)
+        fr = np.zeros((F, FRONT_FIELDS), dtype=np.int64)
+        fr[:, 0], fr[:, 1], fr[:, 2] = fj0, fnj, fns
+        fr[:, 3], fr[:, 4], fr[:, 5] = self.foff[:-1], self.fpos[:-1], self.fu[:-1]
+        fr[:, 6] = np.concatenate([[0], np.cumsum(fns)])[:-1]
+        self.fronts = fr
+        self.uoff2 = np.concatenate([[0], np.cumsum(fns * fns)]).astype(np.int64)
+        self._items()
+
+    def _items(self) -> None:
+        """Work items per level: forward (front, row0, rows) splits the stacked rows
+        [Finv; W] (about 256 KB of factor per item), backward (front, col0, cols) splits
+        the columns into groups of up to 64."""
+        self.fwd, self.bwd = [], []
+        for h in range(self.level_ptr.shape[0] - 1):
+            fi, bi = [], []
+            for t in range(self.level_ptr[h], self.level_ptr[h + 1]):
+                nj, ns = int(self.fnj[t]), int(self.fns[t])
+                R = int(max(4, min(64, (32768 // max(nj, 1)) // 4 * 4)))
+                for r0 in range(0, nj + ns, R):
+                    fi.append((t, r0, min(R, nj + ns - r0)))
+                for c0 in range(0, nj, 64):
+                    bi.append((t, c0, min(64, nj - c0)))
+            lf = int(self.fnj[self.level_ptr[h]:self.level_ptr[h + 1]].max())
+            lb = int(self.fns[self.level_ptr[h]:self.level_ptr[h + 1]].max())
+            self.fwd.append((np.asarray(fi, dtype=np.int32).reshape(-1, 3), lf))
+            self.bwd.append((np.asarray(bi, dtype=np.int32).reshape(-1, 3), lb))
+
+    @property
+    def n_levels(self) -> int:
+        return self.level_ptr.shape[0] - 1
+
+    def validate(self) -> None:
+        """Host check of every offset the sweep kernels use (before any launch)."""
+        F, n = self.n_fronts, self.n
+        fr = self.fronts
+        j0, nj, ns, foff, fpos, fu, fs = (fr[:, k] for k in range(7))
+        assert np.all(nj >= 1) and np.all(ns >= 0)
+        assert np.all(j0 >= 0) and np.all(j0 + nj <= n)
+        assert np.all(foff + (nj + ns) * nj <= self.fac_size)
+        assert np.all(fpos + nj + ns <= self.gptr.shape[0] - 1)
+        assert np.all(fu + ns <= self.u_size) and np.all(fs + ns <= self.sidx.shape[0])
+        assert self.gsrc.size == 0 or (self.gsrc.min() >= 0 and self.gsrc.max() < self.u_size)
+        assert self.sidx.size == 0 or (self.sidx.min() >= 0 and self.sidx.max() < n)
+        assert np.array_equal(np.sort(self.perm), np.arange(n))
+        for items, lds in self.fwd:
+            t, r0, nr = items[:, 0], items[:, 1], items[:, 2]
+            assert np.all((t >= 0) & (t < F)) and np.all(nr >= 1)
+            assert np.all(r0 + nr <= nj[t] + ns[t]) and lds <= LDS_CAP and np.all(nj[t] <= lds)
+        for items, lds in self.bwd:
+            t, c0, nc = items[:, 0], items[:, 1], items[:, 2]
+            assert np.all((nc >= 1) & (nc <= 64)) and np.all(c0 + nc <= nj[t])
+            assert lds <= LDS_CAP and np.all(ns[t] <= lds)
+
+
+# ---------------------------------------------------------------------------------------
+# Numeric factorisation (torch, any device)
+# ---------------------------------------------------------------------------------------
+
+def _level_indices(plan: NDPlan, h: int):
+    """Host index arrays of level h: padded batch shape and the scatter/gather indices."""
+    lo, hi = int(plan.level_ptr[h]), int(plan.level_ptr[h + 1])
+    fr = range(lo, hi)
+    nj, ns = plan.fnj[lo:hi], plan.fns[lo:hi]
+    Jm, Sm = int(nj.max()), int(ns.max()) if hi > lo else 0
+    M = Jm + Sm
+    Ap = plan.Ap
+    dA, sA, dI, dU, sU = [], [], [], [], []
+    pF, sF, pW, sW, pUo, sUo = [], [], [], [], [], []
+    for b, t in enumerate(fr):
+        a, J, s = int(plan.fj0[t]), int(plan.fnj[t]), int(plan.fns[t])
+        St = plan.S[t]
+        base = b * M * M
+        # entries of A in the columns J (rows of the symmetric CSR), at (pos(col), pos(row))
+        k0, k1 = Ap.indptr[a], Ap.indptr[a + J]
+        rows = np.repeat(np.arange(a, a + J), np.diff(Ap.indptr[a:a + J + 1]))
+        cols = Ap.indices[k0:k1]
+        keep = cols >= a
+        rows, cols, idx = rows[keep], cols[keep], np.arange(k0, k1)[keep]
+        sp = np.searchsorted(St, cols)
+        pc = np.where(cols < a + J, cols - a, Jm + sp)
+        assert np.array_equal(St[sp[cols >= a + J]], cols[cols >= a + J])
+        dA.append(base + pc * M + (rows - a))
+        sA.append(idx)
+        dI.append(base + np.arange(J, Jm) * (M + 1))
+        # children's update matrices, child rank r in list r
+        for r, c in enumerate(plan.children[t]):
+            p = plan.cmap[c]
+            p = np.where(p < J, p, p - J + Jm)
+            ncs = p.shape[0]
+            while len(dU) <= r:
+                dU.append([])
+                sU.append([])
+            dU[r].append(base + (p[:, None] * M + p[None, :]).ravel())
+            sU[r].append(plan.uoff2[c] + np.arange(ncs * ncs))
+        # packing: Finv (J x J) and W (s x J) into fac; U (s x s) into the update store
+        rr, cc = np.divmod(np.arange(J * J), J)
+        pF.append(plan.foff[t] + np.arange(J * J))
+        sF.append(b * Jm * Jm + rr * Jm + cc)
+        rr, cc = np.divmod(np.arange(s * J), J)
+        pW.append(plan.foff[t] + J * J + np.arange(s * J))
+        sW.append(b * Sm * Jm + rr * Jm + cc)
+        rr, cc = np.divmod(np.arange(s * s), max(s, 1))
+        pUo.append(plan.uoff2[t] + np.arange(s * s))
+        sUo.append(b * Sm * Sm + rr * Sm + cc)
+    cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0, dtype=np.int64)
+    return dict(B=hi - lo, Jm=Jm, Sm=Sm, dA=cat(dA), sA=cat(sA), dI=cat(dI),
+                dU=[cat(x) for x in dU], sU=[cat(x) for x in sU],
+                pF=cat(pF), sF=cat(sF), pW=cat(pW), sW=cat(sW), pU=cat(pUo), sU_out=cat(sUo))
+
+
+def factorize(plan: NDPlan, values: np.ndarray, device, dtype=torch.float64) -> torch.Tensor:
+    """Factor rows [Finv; W] of every front (plan.fronts layout), in `dtype`, on `device`.
+    `values` are the entries of plan.Ap (FP64)."""
+    f64 = torch.float64
+    vals = torch.as_tensor(np.asarray(values, dtype=np.float64), device=device)
+    fac = torch.zeros(plan.fac_size, dtype=f64, device=device)
+    ustore = torch.zeros(int(plan.uoff2[-1]), dtype=f64, device=device)
+    T = lambda a: torch.as_tensor(a, dtype=torch.int64, device=device)
+    for h in range(plan.n_levels):
+        ix = _level_indices(plan, h)
+        B, Jm, Sm = ix["B"], ix["Jm"], ix["Sm"]
+        M = Jm + Sm
+        F = torch.zeros(B * M * M, dtype=f64, device=device)
+        F[T(ix["dA"])] = vals[T(ix["sA"])]
+        F[T(ix["dI"])] = 1.0
+        for d, s in zip(ix["dU"], ix["sU"]):  # one child rank at a time: distinct targets
+            F.index_put_((T(d),), ustore[T(s)], accumulate=True)
+        F = F.view(B, M, M)
+        L, info = torch.linalg.cholesky_ex(F[:, :Jm, :Jm].contiguous())
+        if int(info.max()) != 0:
+            raise np.linalg.LinAlgError("coarse matrix is not positive definite")
+        Finv = torch.cholesky_inverse(L)
+        FSJ = F[:, Jm:, :Jm].contiguous()
+        W = torch.matmul(FSJ, Finv)
+        U = F[:, Jm:, Jm:] - torch.matmul(W, FSJ.transpose(1, 2).contiguous())
+        fac[T(ix["pF"])] = Finv.reshape(-1)[T(ix["sF"])]
+        fac[T(ix["pW"])] = W.reshape(-1)[T(ix["sW"])]
+        ustore[T(ix["pU"])] = U.reshape(-1)[T(ix["sU_out"])]
+        del F, L, Finv, FSJ, W, U
+    return fac.to(dtype)
+
+
+def solve_host(plan: NDPlan, fac: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """The two sweeps of the device solve, restated on the host over the same tables (a
+    check of the plan and the factor; the device path never calls it)."""
+    n = plan.n
+    G = np.zeros(n)
+    X = np.zeros(n)
+    U = np.zeros(plan.u_size)
+    out = np.zeros(n)
+    fr = plan.fronts
+    for h in range(plan.n_levels):
+        for t in range(plan.level_ptr[h], plan.level_ptr[h + 1]):
+            j0, nj, ns, foff, fpos, fu = (int(v) for v in fr[t, :6])
+            f = np.zeros(nj + ns)
+            f[:nj] = b[plan.perm[j0:j0 + nj]]
+            for p in range(nj + ns):
+                for k in range(plan.gptr[fpos + p], plan.gptr[fpos + p + 1]):
+                    f[p] += U[plan.gsrc[k]]
+            R = fac[foff:foff + (nj + ns) * nj].reshape(nj + ns, nj)
+            y = R @ f[:nj]
+            G[j0:j0 + nj] = y[:nj]
+            U[fu:fu + ns] = f[nj:] - y[nj:]
+    for h in reversed(range(plan.n_levels)):
+        for t in range(plan.level_ptr[h], plan.level_ptr[h + 1]):
+            j0, nj, ns, foff, _, _, fs = (int(v) for v in fr[t, :7])
+            W = fac[foff + nj * nj:foff + (nj + ns) * nj].reshape(ns, nj)
+            xs = X[plan.sidx[fs:fs + ns]]
+            x = G[j0:j0 + nj] - W.T @ xs
+            X[j0:j0 + nj] = x
+            out[plan.perm[j0:j0 + nj]] = x
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# Coarse-slot solver
+# ---------------------------------------------------------------------------------------
+
+class SparseCholesky(SparseSolver):
+    """Coarse-space slot for large sparse SPD coarse matrices (any `SparseSolver` slot:
+    setup(csr), solve(v) -> (x, {})); the sparse counterpart of the reference's `CUDSS()`
+    (solvers.py:85-113).
+
+    setup(A): nested-dissection ordering of A's graph and the symbolic plan on the host,
+    the block LDL^T factor on the device (FP64, stored in A's dtype: float64 or float32).
+    solve(b): 2 x (levels) HIP launches, every sum in a fixed order.
+    """
+
+    def __init__(self, leaf_size: int = 32, top_max: int = 4096):
+        self.leaf_size = leaf_size
+        self.top_max = top_max
+        self.plan = None
+        self.active_mode = "nested-dissection"
+
+    def __str__(self):
+        return "SparseCholesky"
+
+    def setup(self, matrix: torch.Tensor, *args, **kwargs) -> None:
+        _lib.require_device(matrix.values() if matrix.layout is torch.sparse_csr else matrix)
+        if matrix.dtype not in (torch.float64, torch.float32):
+            raise ValueError(f"unsupported coarse dtype {matrix.dtype}")
+        dev = matrix.device
+        self.dtype = matrix.dtype
+        if matrix.layout is torch.sparse_csr:
+            A = sps.csr_matrix((matrix.values().cpu().double().numpy(),
+                                matrix.col_indices().cpu().numpy(),
+                                matrix.crow_indices().cpu().numpy()), shape=matrix.shape)
+        else:
+            A = sps.csr_matrix(matrix.cpu().double().numpy())
+        self.n = A.shape[0]
+        plan = NDPlan(A, self.leaf_size, self.top_max)
+        plan.validate()
+        self.plan = plan
+        with torch.cuda.device(dev):
+            self.fac = factorize(plan, plan.Ap.data, dev, self.dtype)
+            I = lambda a, dt=torch.int64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt,
+                                                          device=dev)
+            self._fronts = I(plan.fronts.reshape(-1))
+            self._gptr = I(plan.gptr)
+            self._gsrc = I(plan.gsrc, torch.int32)
+            self._sidx = I(plan.sidx, torch.int32)
+            self._perm = I(plan.perm, torch.int32)
+            self._fwd = [(I(it.reshape(-1), torch.int32), it.shape[0], lds) for it, lds in plan.fwd]
+            self._bwd = [(I(it.reshape(-1), torch.int32), it.shape[0], lds) for it, lds in plan.bwd]
+            self._G = torch.empty(self.n, dtype=self.dtype, device=dev)
+            self._X = torch.empty(self.n, dtype=self.dtype, device=dev)
+            self._U = torch.empty(max(plan.u_size, 1), dtype=self.dtype, device=dev)
+
+    def solve(self, rhs: torch.Tensor):
+        if rhs.dtype != self.dtype:
+            raise AssertionError(f"rhs dtype {rhs.dtype} != coarse dtype {self.dtype}")
+        _lib.require_device(rhs)
+        rhs = rhs.contiguous()
+        y = torch.empty_like(rhs)
+        with torch.cuda.device(rhs.device):
+            self._enqueue_solve(rhs, y, None)
+        return y, {}
+
+    def _enqueue_solve(self, x, y, gate) -> None:
+        dt = _lib.dtype_code(self.dtype)
+        s = _lib.stream()
+        g = _lib.ptr(gate)
+        for items, n_items, lds in self._fwd:
+            _lib.call("dds_nd_forward", dt, n_items, items.data_ptr(), self._fronts.data_ptr(),
+                      self._gptr.data_ptr(), self._gsrc.data_ptr(), self._perm.data_ptr(),
+                      self.fac.data_ptr(), x.data_ptr(), self._G.data_ptr(), self._U.data_ptr(),
+                      lds, g, s)
+        for items, n_items, lds in reversed(self._bwd):
+            _lib.call("dds_nd_backward", dt, n_items, items.data_ptr(), self._fronts.data_ptr(),
+                      self._sidx.data_ptr(), self._perm.data_ptr(), self.fac.data_ptr(),
+                      self._G.data_ptr(), self._X.data_ptr(), y.data_ptr(), lds, g, s)
+
+    def _enqueue_solve_multi(self, x, y, st) -> None:
+        raise ValueError("blocks of right-hand sides need the DenseCholesky coarse solver")
+
+    @property
+    def bytes_per_solve(self) -> int:
+        e = 8 if self.dtype == torch.float64 else 4
+        p = self.plan
+        return e * int(np.sum(p.fnj * p.fnj + 2 * p.fns * p.fnj)) + 4 * e * p.n + 8 * p.u_size
+
+    def destroy(self):
+        self.plan = None
+        self.fac = None

@@ -18,6 +18,7 @@
  *   dds_local_apply           Inv.solve -> gemv_strided_batched (solvers.py:355-359),
  *                             generalised to uneven and overlapping subdomains
  *   dds_local_apply_sym       the same on symmetric tile-packed inverses
+ *   dds_local_apply_batched   the same on the reference's (n, k, k) layout (equal subdomains)
  *   dds_overlap_sum           y += repeat_interleave(...) prolongation (solvers.py:821-825,
  *                             :882-887) fused with the overlapped sum_i R_i^T y_i
  *   dds_restrict              x.reshape(n_s,-1).sum(1) + segment_reduce (solvers.py:817-820)
@@ -27,6 +28,10 @@
  *   dds_chol_inverse, dds_dense_symv
  *                             coarse solve for large coarse spaces (explicit A_C^-1 from the
  *                             Cholesky factor, applied as one dense symmetric matvec)
+ *   dds_nd_forward, dds_nd_backward
+ *                             cudssExecute(SOLVE) of a large sparse coarse matrix
+ *                             (cudss.cpp:64-78, called from CUDSS.solve, solvers.py:106-110):
+ *                             the two sweeps of a nested-dissection block LDL^T factor
  *   dds_pcg_*                 the CG.solve loop body (solvers.py:228-297): torch.linalg.norm,
  *                             torch.dot, fused_add / fused_sub / fused_mul_add (:170-189)
  *   dds_cast                  rhs.to(preconditioner_precision) / y.to(rhs.dtype)
@@ -153,6 +158,14 @@ int dds_local_apply(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_tiles
                     const float* scale, const void* mats, const void* x, void* Y,
                     int waves, const dds_pcg_state* gate, void* stream);
 
+/* The local apply on the reference's own layout (Inv.solve -> gemv_strided_batched,
+ * solvers.py:355-359; gemv_kernels.cu:31-153): n equal, contiguous, non-overlapping
+ * subdomains of k unknowns, inverses (n, k, k) column-major; y[b*k + r] =
+ * scale[b] * sum_c A_b[r][c] x[b*k + c] with the accumulator of the vector type (scale
+ * optional), the drop-in GEMV kernels with a gate. use_lds: stage x in LDS (k <= 1024). */
+int dds_local_apply_batched(int mat_dtype, int vec_dtype, const void* mats, const void* x,
+                            void* y, int64_t n, int64_t k, int use_lds, const float* scale,
+                            const dds_pcg_state* gate, void* stream);
 /* Symmetric local inverses stored as the lower triangle of 64 x 64 tiles (layout in
  * csrc/local_sym.hip): each off-diagonal tile is read once and applied twice. One
  * wavefront per item (s, p) = (subdomain, pair of tile rows p and T_s-1-p) writes the
@@ -200,6 +213,32 @@ int dds_chol_inverse(int dtype, int64_t n, const void* L, int64_t ldl, void* ain
 /* y = S x for a dense symmetric row-major S (one wavefront per row). */
 int dds_dense_symv(int dtype, int64_t n, const void* s, int64_t lds, const void* x, void* y,
                    const dds_pcg_state* gate, void* stream);
+
+/* ---- sparse coarse solver (nested-dissection block LDL^T) ------------------------------
+ * The solve of A x = b with A = L D L^T from a multifrontal factorisation in a
+ * nested-dissection numbering (perm[i] = original index of unknown i). Front t is
+ * fronts[8t .. 8t+7] = (j0, nJ, nS, foff, fpos, fu, fsidx, unused): unknowns [j0, j0+nJ);
+ * its factor rows [F_JJ^-1 (nJ x nJ); W = F_SJ F_JJ^-1 (nS x nJ)] row-major at fac + foff;
+ * front positions fpos .. fpos+nJ+nS-1 of the gather lists (position p adds
+ * u[gsrc[k]] for k in [gptr[fpos+p], gptr[fpos+p+1])); its update vector u + fu and its
+ * coupling set sidx + fsidx (nS each). items are int32 triples, one per workgroup; a launch
+ * covers the fronts of one height of the dissection tree. lds_elems (<= 7680) is the
+ * largest nJ (forward) or nS (backward) of the launch. F64 and F32, sums in FP64, every
+ * sum in a fixed order. No allocation; b, out are in the original numbering.
+ *
+ * Forward (leaves first), item (t, r0, nr): with f_J = b[perm[j0 + .]] + gathered updates,
+ *   rows r < nJ:  g[j0 + r] = (F_JJ^-1 f_J)_r;  rows r >= nJ: u[fu + r - nJ] =
+ *   (gathered updates at r) - (W f_J)_{r - nJ}.                                          */
+int dds_nd_forward(int dtype, int64_t n_items, const int32_t* items, const int64_t* fronts,
+                   const int64_t* gptr, const int32_t* gsrc, const int32_t* perm,
+                   const void* fac, const void* b, void* g, void* u, int64_t lds_elems,
+                   const dds_pcg_state* gate, void* stream);
+/* Backward (top first), item (t, c0, nc <= 64): x_nd[j0 + c] = g[j0 + c] - (W^T x_S)_c with
+ * x_S = x_nd[sidx[fsidx + .]], and out[perm[j0 + c]] = the same value. */
+int dds_nd_backward(int dtype, int64_t n_items, const int32_t* items, const int64_t* fronts,
+                    const int32_t* sidx, const int32_t* perm, const void* fac, const void* g,
+                    void* x_nd, void* out, int64_t lds_elems, const dds_pcg_state* gate,
+                    void* stream);
 
 /* ---- PCG vector kernels (F64) ----------------------------------------------------------
  * Each writes per-block partial sums where stated (count returned in *n_partials). */

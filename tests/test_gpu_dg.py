@@ -108,3 +108,47 @@ def test_dg_full_size_s10(s10_system, kind):
     xs = x[:: max(1, x.shape[0] // 64)]
     assert np.abs(xs - np.array(ref["x_sample"])).max() <= 1e-6 * np.abs(xs).max()
     assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b) * 1.01
+
+
+@pytest.mark.parametrize("storage", [None, torch.float32, torch.float16, torch.bfloat16])
+@pytest.mark.parametrize("pi", [torch.float64, torch.float32])
+@pytest.mark.parametrize("solvers", ["S3", "S4", "S5"])
+def test_batched_local_apply_matches_oracle(storage, pi, solvers):
+    """The reference's layout (equal contiguous subdomains of 48, 12 and 3 unknowns, the
+    (n, k, k) column-major inverses) through dds_local_apply_batched, chosen by
+    layout="auto", against the oracle's local apply with the same precision semantics."""
+    if pi == torch.float32 and storage is None:
+        storage = torch.float32
+    print(f"DG p=1 S5 solvers {solvers} batched local apply pi {pi} storage {storage}", flush=True)
+    pr = dg.assemble_sipg(dg.UniformTriangleMesh(5), 1)
+    perm, A, b, k, spc = dg.dg_experiment(pr, solvers, "S2")
+    M = dds.AdditiveSchwarz(pi, dds.Inv(storage), dds.DenseCholesky())
+    M.setup(_torch_csr(A), dofs_per_solver=k, solvers_per_coarse=torch.as_tensor(spc, device=dev))
+    assert M.local.layout == "batched"
+    n_s = A.shape[0] // k
+    ptr = np.arange(n_s + 1) * k
+    name = {torch.float64: "float64", torch.float32: "float32", torch.float16: "float16",
+            torch.bfloat16: "bfloat16"}
+    O = SchwarzOracle(A, ptr, ptr, np.arange(A.shape[0]), spc, precision=name[pi],
+                      storage=name[storage or pi])
+    r = np.random.default_rng(0).standard_normal(A.shape[0])
+    y = M._local_only(torch.as_tensor(r, device=dev).to(pi)).double().cpu().numpy()
+    yo = O.local(r.astype(np.float64 if pi == torch.float64 else np.float32)).astype(np.float64)
+    rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    # same products, same ascending column order; only the inverses' rounding differs
+    tol = {None: 1e-12, torch.float32: 1e-6, torch.float16: 2e-3, torch.bfloat16: 1e-2}[storage] \
+        if pi == torch.float64 else 1e-3
+    assert rel < tol, rel
+
+
+def test_auto_layout_choice():
+    pr = dg.assemble_sipg(dg.UniformTriangleMesh(5), 1)
+    perm, A, b, k, spc = dg.dg_experiment(pr, "S2", "S1")           # k = 3 * 4^3 = 192
+    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(), dds.DenseCholesky())
+    M.setup(_torch_csr(A), dofs_per_solver=k, solvers_per_coarse=torch.as_tensor(spc, device=dev))
+    assert M.local.layout == "sym"
+    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(accumulate=torch.float32, precision=torch.float32),
+                            dds.DenseCholesky())
+    perm, A, b, k, spc = dg.dg_experiment(pr, "S4", "S1")           # k = 12, FP32 accumulator
+    M.setup(_torch_csr(A), dofs_per_solver=k, solvers_per_coarse=torch.as_tensor(spc, device=dev))
+    assert M.local.layout == "full"

@@ -1,0 +1,147 @@
+"""Sparse coarse solver (dd_solvers.sparse_coarse) on the host: the nested-dissection
+ordering, the symbolic plan and the block LDL^T factor, checked through a host restatement
+of the device sweeps (sparse_coarse.solve_host, same tables) against scipy's sparse direct
+solve (SuperLU) of the same matrix. No GPU is needed: the factorisation runs in torch on
+the CPU."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import scipy.sparse.linalg as spla
+import torch
+
+from dd_solvers import _lib, dg
+from dd_solvers import sparse_coarse as sc
+
+
+def lap2d(nx, ny):
+    T = lambda m: sps.diags([-np.ones(m - 1), 2 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+    return (sps.kron(sps.identity(ny), T(nx)) + sps.kron(T(ny), sps.identity(nx))).tocsr()
+
+
+def dg_coarse(fine, coarse):
+    """A_C = R_C A R_C^T of the reference's DG experiment (p = 1, solvers = coarse mesh)."""
+    pr = dg.assemble_sipg(dg.UniformTriangleMesh(fine), 1)
+    perm, A, b, k, spc = dg.dg_experiment(pr, coarse, coarse)
+    n_c = spc.shape[0]
+    cell = np.repeat(np.arange(n_c), k * spc)
+    R = sps.csr_matrix((np.ones(A.shape[0]), (cell, np.arange(A.shape[0]))), shape=(n_c, A.shape[0]))
+    return (R @ A @ R.T).tocsr()
+
+
+def random_spd_pieces(seed=0):
+    """Block-diagonal SPD matrix of three disconnected random sparse pieces."""
+    rng = np.random.default_rng(seed)
+    blocks = []
+    for m in (37, 90, 5):
+        B = sps.random(m, m, density=0.05, random_state=rng)
+        B = B + B.T
+        blocks.append(B + sps.diags(np.abs(B).sum(1).A1 + 1.0))
+    return sps.block_diag(blocks).tocsr()
+
+
+def _check(A, leaf, top_max, tol=1e-11):
+    plan = sc.NDPlan(A, leaf_size=leaf, top_max=top_max)
+    plan.validate()
+    fac = sc.factorize(plan, plan.Ap.data, "cpu").numpy()
+    b = np.random.default_rng(1).standard_normal(A.shape[0])
+    x = sc.solve_host(plan, fac, b)
+    xr = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xr) <= tol * np.linalg.norm(xr)
+    return plan
+
+
+@pytest.mark.parametrize("A,leaf,top_max", [
+    (lap2d(1, 1), 32, 4096),            # one unknown
+    (lap2d(7, 5), 4, 8),                # several levels, small top
+    (lap2d(40, 33), 16, 300),
+    (lap2d(40, 33), 16, 0),             # no top front: the roots are ordinary fronts
+    (random_spd_pieces(), 8, 16),       # disconnected graph
+])
+def test_sweeps_solve_like_superlu(A, leaf, top_max):
+    _check(A, leaf, top_max)
+
+
+def test_dg_coarse_matrix():
+    """The coarse matrix of the reference's DG experiment, S6 fine / S4 coarse (512 cells),
+    split down to 8-cell leaves under a 64-unknown top front."""
+    A = dg_coarse(6, "S4")
+    plan = _check(A, 8, 64)
+    assert plan.n_levels >= 4 and plan.nT <= 64
+
+
+def test_plan_structure():
+    """Nested dissection: perm is a permutation; every coupling set lies above its front;
+    an edge of A never joins two fronts unless one is an ancestor of the other (so the
+    factor has no fill outside the fronts)."""
+    A = lap2d(30, 21)
+    plan = sc.NDPlan(A, leaf_size=8, top_max=40)
+    assert np.array_equal(np.sort(plan.perm), np.arange(A.shape[0]))
+    owner = np.empty(A.shape[0], dtype=np.int64)
+    for t in range(plan.n_fronts):
+        a, nj = plan.fj0[t], plan.fnj[t]
+        owner[a:a + nj] = t
+        assert np.all(plan.S[t] >= a + nj)
+    parent = np.full(plan.n_fronts, -1)
+    for t in range(plan.n_fronts):
+        for c in plan.children[t]:
+            parent[c] = t
+
+    def ancestors(t):
+        out = set()
+        while parent[t] >= 0:
+            t = parent[t]
+            out.add(t)
+        return out
+
+    Ap = plan.Ap.tocoo()
+    for i, j in zip(Ap.row, Ap.col):
+        ti, tj = owner[i], owner[j]
+        if ti != tj:
+            assert tj in ancestors(ti) or ti in ancestors(tj), (i, j)
+    # heights: children are processed in earlier launches than their parents
+    lev = np.searchsorted(plan.level_ptr, np.arange(plan.n_fronts), side="right") - 1
+    for t in range(plan.n_fronts):
+        for c in plan.children[t]:
+            assert lev[c] < lev[t]
+
+
+def test_planted_mistake_is_caught():
+    """A factor entry off by 1e-6 (relative) is visible to the check above."""
+    A = lap2d(20, 20)
+    plan = sc.NDPlan(A, leaf_size=8, top_max=30)
+    fac = sc.factorize(plan, plan.Ap.data, "cpu").numpy()
+    t = plan.n_fronts // 2
+    j0, nj, ns, foff = (int(v) for v in plan.fronts[t, :4])
+    assert ns > 0
+    fac[foff + nj * nj] *= 1 + 1e-6          # first entry of W
+    b = np.random.default_rng(1).standard_normal(A.shape[0])
+    x = sc.solve_host(plan, fac, b)
+    xr = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xr) > 1e-11 * np.linalg.norm(xr)
+
+
+def test_not_positive_definite():
+    A = lap2d(10, 10) - 3.0 * sps.identity(100)
+    plan = sc.NDPlan(A.tocsr(), leaf_size=8, top_max=20)
+    with pytest.raises(np.linalg.LinAlgError):
+        sc.factorize(plan, plan.Ap.data, "cpu")
+
+
+def test_front_capacity_is_enforced():
+    with pytest.raises(ValueError, match="exceeds"):
+        sc.NDPlan(lap2d(8000, 1), leaf_size=8000, top_max=0)   # one 8000-unknown leaf
+
+
+def test_host_validation_of_the_sweep_entry_points():
+    with pytest.raises(_lib.HipError, match="front of"):
+        _lib.call("dds_nd_forward", _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 7681, None, None)
+    with pytest.raises(_lib.HipError, match="front of"):
+        _lib.call("dds_nd_backward", _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 1, 9000, None, None)
+    with pytest.raises(_lib.HipError, match="unsupported"):
+        _lib.call("dds_nd_forward", _lib.F16, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 8, None, None)
+
+
+def test_cpu_tensors_rejected():
+    with pytest.raises(ValueError, match="HIP device"):
+        sc.SparseCholesky().setup(torch.eye(3, dtype=torch.float64).to_sparse_csr())
