@@ -1,16 +1,17 @@
-// Sparse coarse solve: the two sweeps of a nested-dissection block LDL^T factor
+// Sparse coarse solve: the sweeps of a nested-dissection block LDL^T factor
 // (A = L D L^T with L_SJ = W = F_SJ F_JJ^-1 and D_J = F_JJ per front), one launch per
-// height of the dissection tree.
+// height of the dissection tree below a dense top front.
 //
-// Front t is described by fronts[8t .. 8t+7] = (j0, nJ, nS, foff, fpos, fu, fsidx, -):
-//   unknowns j0 .. j0+nJ-1 of the nested-dissection numbering (perm[i] = original index);
-//   factor rows [Finv (nJ x nJ); W (nS x nJ)] row-major at fac + foff, so row r of the
-//   stack is at fac + foff + r*nJ;
-//   its nJ+nS front positions at gather-list positions fpos .. fpos+nJ+nS-1: position p
-//   receives u[gsrc[k]] for k in [gptr[fpos+p], gptr[fpos+p+1]) (the children's update
-//   vectors, in child order);
-//   its update vector at u + fu (nS), its coupling set (nested-dissection indices of the
-//   ancestors' unknowns it touches) at sidx + fsidx (nS).
+// A front owns the unknowns j0 .. j0+nJ-1 of the nested-dissection numbering
+// (perm[i] = original index) and couples to nS unknowns of its ancestors; its stacked
+// factor [F_JJ^-1 (nJ x nJ); W (nS x nJ)] is column-major at fac + foff, element (r, c) at
+// foff + c*m + r with m = nJ + nS. Data moves between levels only by pushes into fixed
+// buffers, so every kernel reads its inputs with one level of indexing:
+//   forward:  a child leaves its update vector u in its own slot of its parent's front,
+//             slots[wbase + maps[moff + s]] = u_s; a front adds its children's slots in
+//             child order (entries that no child maps stay zero);
+//   backward: a front writes x at each child's coupling set into that child's buffer,
+//             xs[push_dst[k]] = x at front position push_src[k].
 // Every sum runs in a fixed order, so a solve is bitwise reproducible.
 #include "common.hpp"
 
@@ -19,101 +20,92 @@ namespace {
 
 constexpr int kNdThreads = 256;
 constexpr int kNdWaves = kNdThreads / 64;
-constexpr int64_t kNdLdsCap = 7680;  // elements of f_J (forward) or x_S (backward): 60 KiB
+constexpr int64_t kNdLdsCap = 7680;  // dynamic LDS elements per workgroup: 60 KiB of FP64
 
-struct Front {
-  int64_t j0, nJ, nS, foff, fpos, fu, fsidx;
-};
-
-__device__ __forceinline__ Front load_front(const int64_t* __restrict__ fronts, int t) {
-  const int64_t* f = fronts + 8 * (int64_t)t;
-  return Front{f[0], f[1], f[2], f[3], f[4], f[5], f[6]};
-}
-
-// Work item (t, r0, nr): rows r0 .. r0+nr-1 of front t's stacked factor.
-//   f_J = b[perm[j0 + i]] + gathered updates            (every workgroup of the front)
-//   r <  nJ:  g[j0 + r]      = Finv[r] . f_J
-//   r >= nJ:  u[fu + r - nJ] = (gathered updates at r) - W[r - nJ] . f_J
-// Wave w takes rows r0 + w, r0 + w + 4, ...; lanes split a row's columns and a butterfly
-// adds them.
+// Forward item (12 x int64): j0, nJ, nS, foff, soff, n_children, wbase, moff, r0, nr.
+// Thread i takes row r = r0 + i (nr <= 256) of the stacked factor:
+//   f_J = b[perm[j0 + .]] + the children's slots (child order), staged in LDS;
+//   r <  nJ:  g[j0 + r] = sum_c Finv[r][c] f_J[c]                     (c ascending)
+//   r >= nJ:  the parent's slot gets (children's slots at r) - sum_c W[r-nJ][c] f_J[c].
 template <typename T>
 __global__ __launch_bounds__(kNdThreads) void nd_forward_kernel(
-    const int32_t* __restrict__ items, const int64_t* __restrict__ fronts,
-    const int64_t* __restrict__ gptr, const int32_t* __restrict__ gsrc,
-    const int32_t* __restrict__ perm, const T* __restrict__ fac, const T* __restrict__ b,
-    T* __restrict__ g, T* __restrict__ u, const dds_pcg_state* __restrict__ gate) {
+    const int64_t* __restrict__ items, const int32_t* __restrict__ perm,
+    const int32_t* __restrict__ maps, const T* __restrict__ fac, const T* __restrict__ b,
+    T* __restrict__ g, T* __restrict__ slots, const dds_pcg_state* __restrict__ gate) {
   extern __shared__ double f[];
   if (gate != nullptr && gate->done) return;
-  const int t = items[3 * blockIdx.x];
-  const int r0 = items[3 * blockIdx.x + 1];
-  const int nr = items[3 * blockIdx.x + 2];
-  const Front fr = load_front(fronts, t);
-  const int nJ = (int)fr.nJ;
-  for (int i = threadIdx.x; i < nJ; i += kNdThreads) {
-    double v = static_cast<double>(b[perm[fr.j0 + i]]);
-    const int64_t k1 = gptr[fr.fpos + i + 1];
-    for (int64_t k = gptr[fr.fpos + i]; k < k1; ++k) v += static_cast<double>(u[gsrc[k]]);
-    f[i] = v;
+  const int64_t* it = items + 12 * (int64_t)blockIdx.x;
+  const int64_t j0 = it[0], foff = it[3], soff = it[4], wbase = it[6], moff = it[7];
+  const int nJ = (int)it[1], nS = (int)it[2], nch = (int)it[5], r0 = (int)it[8], nr = (int)it[9];
+  const int m = nJ + nS;
+  for (int p = threadIdx.x; p < nJ; p += kNdThreads) {
+    double v = static_cast<double>(b[perm[j0 + p]]);
+    for (int q = 0; q < nch; ++q) v += static_cast<double>(slots[soff + (int64_t)q * m + p]);
+    f[p] = v;
+  }
+  const int r = r0 + (int)threadIdx.x;
+  const bool active = (int)threadIdx.x < nr;
+  double fs = 0.0;  // the children's update at an S row
+  int32_t dst = 0;
+  if (active && r >= nJ) {
+    for (int q = 0; q < nch; ++q) fs += static_cast<double>(slots[soff + (int64_t)q * m + r]);
+    dst = maps[moff + (r - nJ)];
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  for (int r = r0 + w; r < r0 + nr; r += kNdWaves) {
-    const T* row = fac + fr.foff + (int64_t)r * nJ;
-    double a0 = 0.0, a1 = 0.0;
-    int c = lane;
-    for (; c + 64 < nJ; c += 128) {
-      a0 += static_cast<double>(load_nt(row + c)) * f[c];
-      a1 += static_cast<double>(load_nt(row + c + 64)) * f[c + 64];
-    }
-    if (c < nJ) a0 += static_cast<double>(load_nt(row + c)) * f[c];
-    const double acc = wave_sum(a0 + a1);
-    if (lane == 0) {
-      if (r < nJ) {
-        g[fr.j0 + r] = static_cast<T>(acc);
-      } else {
-        double v = 0.0;
-        const int64_t k1 = gptr[fr.fpos + r + 1];
-        for (int64_t k = gptr[fr.fpos + r]; k < k1; ++k) v += static_cast<double>(u[gsrc[k]]);
-        u[fr.fu + (r - nJ)] = static_cast<T>(v - acc);
-      }
-    }
+  if (!active) return;
+  const T* col = fac + foff + r;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int c = 0;
+  for (; c + 3 < nJ; c += 4) {
+    a0 += static_cast<double>(load_nt(col + (int64_t)c * m)) * f[c];
+    a1 += static_cast<double>(load_nt(col + (int64_t)(c + 1) * m)) * f[c + 1];
+    a2 += static_cast<double>(load_nt(col + (int64_t)(c + 2) * m)) * f[c + 2];
+    a3 += static_cast<double>(load_nt(col + (int64_t)(c + 3) * m)) * f[c + 3];
   }
+  for (; c < nJ; ++c) a0 += static_cast<double>(load_nt(col + (int64_t)c * m)) * f[c];
+  const double acc = (a0 + a1) + (a2 + a3);
+  if (r < nJ)
+    g[j0 + r] = static_cast<T>(acc);
+  else
+    slots[wbase + dst] = static_cast<T>(fs - acc);
 }
 
-// Work item (t, c0, nc), nc <= 64: x_J[c] = g[j0 + c] - sum_s W[s][c] x_S[s] for the nc
-// columns c0 .. c0+nc-1 (lane = column), x_S gathered from the ancestors' solution. Wave
-// w sums the rows s = w, w+4, ...; the four wave sums are added in wave order.
+// Backward item (12 x int64): j0, nJ, nS, foff, xoff, c0, nc, k0, k1. Lane l takes column
+// c = c0 + l (nc <= 64); wave w sums the rows s of its quarter of [0, nS):
+//   x_J[c] = g[j0 + c] - sum_s W[s][c] x_S[s]   (x_S = xs[xoff .. xoff+nS), quarters added
+//   in wave order), written to out[perm[j0 + c]]; then the push entries [k0, k1) copy
+//   x_J (sources in [c0, c0+nc)) or x_S (sources >= nJ) into the children's buffers.
 template <typename T>
 __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
-    const int32_t* __restrict__ items, const int64_t* __restrict__ fronts,
-    const int32_t* __restrict__ sidx, const int32_t* __restrict__ perm,
-    const T* __restrict__ fac, const T* __restrict__ g, T* __restrict__ xnd,
+    const int64_t* __restrict__ items, const int32_t* __restrict__ perm,
+    const int64_t* __restrict__ push_dst, const int32_t* __restrict__ push_src,
+    const T* __restrict__ fac, const T* __restrict__ g, T* __restrict__ xs,
     T* __restrict__ out, const dds_pcg_state* __restrict__ gate) {
-  extern __shared__ double xs[];
+  extern __shared__ double xsl[];
   __shared__ double red[kNdWaves][64];
+  __shared__ double xj[64];
   if (gate != nullptr && gate->done) return;
-  const int t = items[3 * blockIdx.x];
-  const int c0 = items[3 * blockIdx.x + 1];
-  const int nc = items[3 * blockIdx.x + 2];
-  const Front fr = load_front(fronts, t);
-  const int nJ = (int)fr.nJ;
-  const int nS = (int)fr.nS;
-  for (int i = threadIdx.x; i < nS; i += kNdThreads)
-    xs[i] = static_cast<double>(xnd[sidx[fr.fsidx + i]]);
+  const int64_t* it = items + 12 * (int64_t)blockIdx.x;
+  const int64_t j0 = it[0], foff = it[3], xoff = it[4], k0 = it[7], k1 = it[8];
+  const int nJ = (int)it[1], nS = (int)it[2], c0 = (int)it[5], nc = (int)it[6];
+  const int m = nJ + nS;
+  for (int s = threadIdx.x; s < nS; s += kNdThreads) xsl[s] = static_cast<double>(xs[xoff + s]);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
+  const int q = (nS + kNdWaves - 1) / kNdWaves;
+  const int s0 = w * q;
+  const int s1 = min(nS, s0 + q);
   double acc = 0.0;
   if (lane < nc) {
-    const T* W = fac + fr.foff + (int64_t)nJ * nJ + c0 + lane;
+    const T* W = fac + foff + (int64_t)(c0 + lane) * m + nJ;
     double a0 = 0.0, a1 = 0.0;
-    int s = w;
-    for (; s + kNdWaves < nS; s += 2 * kNdWaves) {
-      a0 += static_cast<double>(load_nt(W + (int64_t)s * nJ)) * xs[s];
-      a1 += static_cast<double>(load_nt(W + (int64_t)(s + kNdWaves) * nJ)) * xs[s + kNdWaves];
+    int s = s0;
+    for (; s + 1 < s1; s += 2) {
+      a0 += static_cast<double>(load_nt(W + s)) * xsl[s];
+      a1 += static_cast<double>(load_nt(W + s + 1)) * xsl[s + 1];
     }
-    if (s < nS) a0 += static_cast<double>(load_nt(W + (int64_t)s * nJ)) * xs[s];
+    if (s < s1) a0 += static_cast<double>(load_nt(W + s)) * xsl[s];
     acc = a0 + a1;
   }
   red[w][lane] = acc;
@@ -122,35 +114,68 @@ __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
     double sum = 0.0;
 #pragma unroll
     for (int i = 0; i < kNdWaves; ++i) sum += red[i][lane];
-    const int64_t j = fr.j0 + c0 + lane;
+    const int64_t j = j0 + c0 + lane;
     const double x = static_cast<double>(g[j]) - sum;
-    xnd[j] = static_cast<T>(x);
+    xj[lane] = x;
     out[perm[j]] = static_cast<T>(x);
+  }
+  __syncthreads();
+  for (int64_t k = k0 + threadIdx.x; k < k1; k += kNdThreads) {
+    const int src = push_src[k];
+    const double v = src < nJ ? xj[src - c0] : xsl[src - nJ];
+    xs[push_dst[k]] = static_cast<T>(v);
   }
 }
 
+// Top front (unknowns j0 .. j0+nT-1): f = b[perm[j0 + p]] + its children's slots.
 template <typename T>
-int launch_nd_forward(int64_t n_items, const int32_t* items, const int64_t* fronts,
-                      const int64_t* gptr, const int32_t* gsrc, const int32_t* perm,
-                      const void* fac, const void* b, void* g, void* u, int64_t lds,
-                      const dds_pcg_state* gate, hipStream_t s) {
+__global__ __launch_bounds__(kNdThreads) void nd_top_assemble_kernel(
+    int64_t j0, int64_t nT, int64_t soff, int nch, const int32_t* __restrict__ perm,
+    const T* __restrict__ b, const T* __restrict__ slots, T* __restrict__ ftop,
+    const dds_pcg_state* __restrict__ gate) {
+  if (gate != nullptr && gate->done) return;
+  const int64_t p = (int64_t)blockIdx.x * kNdThreads + threadIdx.x;
+  if (p >= nT) return;
+  double v = static_cast<double>(b[perm[j0 + p]]);
+  for (int q = 0; q < nch; ++q) v += static_cast<double>(slots[soff + (int64_t)q * nT + p]);
+  ftop[p] = static_cast<T>(v);
+}
+
+// Top front solution x (from the dense inverse): out[perm[j0 + p]] = x[p], and the
+// pushes of x into the top's children's buffers.
+template <typename T>
+__global__ __launch_bounds__(kNdThreads) void nd_top_push_kernel(
+    int64_t j0, int64_t nT, int64_t n_push, const int32_t* __restrict__ perm,
+    const T* __restrict__ xtop, const int64_t* __restrict__ push_dst,
+    const int32_t* __restrict__ push_src, T* __restrict__ xs, T* __restrict__ out,
+    const dds_pcg_state* __restrict__ gate) {
+  if (gate != nullptr && gate->done) return;
+  const int64_t i = (int64_t)blockIdx.x * kNdThreads + threadIdx.x;
+  if (i < nT) out[perm[j0 + i]] = xtop[i];
+  if (i < n_push) xs[push_dst[i]] = xtop[push_src[i]];
+}
+
+template <typename T>
+int nd_forward(int64_t n_items, const int64_t* items, const int32_t* perm, const int32_t* maps,
+               const void* fac, const void* b, void* g, void* slots, int64_t lds,
+               const dds_pcg_state* gate, hipStream_t s) {
   hipLaunchKernelGGL(nd_forward_kernel<T>, dim3((unsigned)n_items), dim3(kNdThreads),
-                     (size_t)std::max<int64_t>(lds, 1) * sizeof(double), s, items, fronts, gptr,
-                     gsrc, perm, static_cast<const T*>(fac), static_cast<const T*>(b),
-                     static_cast<T*>(g), static_cast<T*>(u), gate);
+                     (size_t)std::max<int64_t>(lds, 1) * sizeof(double), s, items, perm, maps,
+                     static_cast<const T*>(fac), static_cast<const T*>(b), static_cast<T*>(g),
+                     static_cast<T*>(slots), gate);
   DDS_LAUNCHED();
   return DDS_OK;
 }
 
 template <typename T>
-int launch_nd_backward(int64_t n_items, const int32_t* items, const int64_t* fronts,
-                       const int32_t* sidx, const int32_t* perm, const void* fac,
-                       const void* g, void* xnd, void* out, int64_t lds,
-                       const dds_pcg_state* gate, hipStream_t s) {
+int nd_backward(int64_t n_items, const int64_t* items, const int32_t* perm,
+                const int64_t* push_dst, const int32_t* push_src, const void* fac,
+                const void* g, void* xs, void* out, int64_t lds, const dds_pcg_state* gate,
+                hipStream_t s) {
   hipLaunchKernelGGL(nd_backward_kernel<T>, dim3((unsigned)n_items), dim3(kNdThreads),
-                     (size_t)std::max<int64_t>(lds, 1) * sizeof(double), s, items, fronts, sidx,
-                     perm, static_cast<const T*>(fac), static_cast<const T*>(g),
-                     static_cast<T*>(xnd), static_cast<T*>(out), gate);
+                     (size_t)std::max<int64_t>(lds, 1) * sizeof(double), s, items, perm,
+                     push_dst, push_src, static_cast<const T*>(fac), static_cast<const T*>(g),
+                     static_cast<T*>(xs), static_cast<T*>(out), gate);
   DDS_LAUNCHED();
   return DDS_OK;
 }
@@ -160,45 +185,94 @@ int launch_nd_backward(int64_t n_items, const int32_t* items, const int64_t* fro
 
 using namespace dds;
 
-extern "C" int dds_nd_forward(int dtype, int64_t n_items, const int32_t* items,
-                              const int64_t* fronts, const int64_t* gptr, const int32_t* gsrc,
-                              const int32_t* perm, const void* fac, const void* b, void* g,
-                              void* u, int64_t lds_elems, const dds_pcg_state* gate,
-                              void* stream) {
+extern "C" int dds_nd_forward(int dtype, int64_t n_items, const int64_t* items,
+                              const int32_t* perm, const int32_t* maps, const void* fac,
+                              const void* b, void* g, void* slots, int64_t lds_elems,
+                              const dds_pcg_state* gate, void* stream) {
   DDS_REQUIRE(n_items >= 0 && n_items <= INT32_MAX, "dds_nd_forward: bad item count");
   DDS_REQUIRE(lds_elems >= 0 && lds_elems <= kNdLdsCap, "dds_nd_forward: front of %lld > %lld",
               (long long)lds_elems, (long long)kNdLdsCap);
   if (n_items == 0) return DDS_OK;
-  DDS_REQUIRE(items && fronts && gptr && perm && fac && b && g && u,
-              "dds_nd_forward: null pointer");
+  DDS_REQUIRE(items && perm && fac && b && g && slots, "dds_nd_forward: null pointer");
   hipStream_t s = as_stream(stream);
   if (dtype == DDS_F64)
-    return launch_nd_forward<double>(n_items, items, fronts, gptr, gsrc, perm, fac, b, g, u,
-                                     lds_elems, gate, s);
+    return nd_forward<double>(n_items, items, perm, maps, fac, b, g, slots, lds_elems, gate, s);
   if (dtype == DDS_F32)
-    return launch_nd_forward<float>(n_items, items, fronts, gptr, gsrc, perm, fac, b, g, u,
-                                    lds_elems, gate, s);
+    return nd_forward<float>(n_items, items, perm, maps, fac, b, g, slots, lds_elems, gate, s);
   set_error("dds_nd_forward: unsupported dtype %d", dtype);
   return DDS_EUNSUPPORTED;
 }
 
-extern "C" int dds_nd_backward(int dtype, int64_t n_items, const int32_t* items,
-                               const int64_t* fronts, const int32_t* sidx, const int32_t* perm,
-                               const void* fac, const void* g, void* x_nd, void* out,
-                               int64_t lds_elems, const dds_pcg_state* gate, void* stream) {
+extern "C" int dds_nd_backward(int dtype, int64_t n_items, const int64_t* items,
+                               const int32_t* perm, const int64_t* push_dst,
+                               const int32_t* push_src, const void* fac, const void* g,
+                               void* xs, void* out, int64_t lds_elems,
+                               const dds_pcg_state* gate, void* stream) {
   DDS_REQUIRE(n_items >= 0 && n_items <= INT32_MAX, "dds_nd_backward: bad item count");
   DDS_REQUIRE(lds_elems >= 0 && lds_elems <= kNdLdsCap, "dds_nd_backward: front of %lld > %lld",
               (long long)lds_elems, (long long)kNdLdsCap);
   if (n_items == 0) return DDS_OK;
-  DDS_REQUIRE(items && fronts && perm && fac && g && x_nd && out,
-              "dds_nd_backward: null pointer");
+  DDS_REQUIRE(items && perm && fac && g && xs && out, "dds_nd_backward: null pointer");
   hipStream_t s = as_stream(stream);
   if (dtype == DDS_F64)
-    return launch_nd_backward<double>(n_items, items, fronts, sidx, perm, fac, g, x_nd, out,
-                                      lds_elems, gate, s);
+    return nd_backward<double>(n_items, items, perm, push_dst, push_src, fac, g, xs, out,
+                               lds_elems, gate, s);
   if (dtype == DDS_F32)
-    return launch_nd_backward<float>(n_items, items, fronts, sidx, perm, fac, g, x_nd, out,
-                                     lds_elems, gate, s);
+    return nd_backward<float>(n_items, items, perm, push_dst, push_src, fac, g, xs, out,
+                              lds_elems, gate, s);
   set_error("dds_nd_backward: unsupported dtype %d", dtype);
   return DDS_EUNSUPPORTED;
+}
+
+extern "C" int dds_nd_top_assemble(int dtype, int64_t j0, int64_t n_top, int64_t soff,
+                                   int64_t n_children, const int32_t* perm, const void* b,
+                                   const void* slots, void* ftop, const dds_pcg_state* gate,
+                                   void* stream) {
+  DDS_REQUIRE(j0 >= 0 && n_top >= 0 && soff >= 0 && n_children >= 0 && n_children <= INT32_MAX,
+              "dds_nd_top_assemble: bad sizes");
+  if (n_top == 0) return DDS_OK;
+  DDS_REQUIRE(perm && b && ftop && (slots || n_children == 0), "dds_nd_top_assemble: null pointer");
+  hipStream_t s = as_stream(stream);
+  const dim3 grid((unsigned)ceil_div(n_top, kNdThreads));
+  if (dtype == DDS_F64) {
+    hipLaunchKernelGGL(nd_top_assemble_kernel<double>, grid, dim3(kNdThreads), 0, s, j0, n_top,
+                       soff, (int)n_children, perm, static_cast<const double*>(b),
+                       static_cast<const double*>(slots), static_cast<double*>(ftop), gate);
+  } else if (dtype == DDS_F32) {
+    hipLaunchKernelGGL(nd_top_assemble_kernel<float>, grid, dim3(kNdThreads), 0, s, j0, n_top,
+                       soff, (int)n_children, perm, static_cast<const float*>(b),
+                       static_cast<const float*>(slots), static_cast<float*>(ftop), gate);
+  } else {
+    set_error("dds_nd_top_assemble: unsupported dtype %d", dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_nd_top_push(int dtype, int64_t j0, int64_t n_top, int64_t n_push,
+                               const int32_t* perm, const void* xtop, const int64_t* push_dst,
+                               const int32_t* push_src, void* xs, void* out,
+                               const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(j0 >= 0 && n_top >= 0 && n_push >= 0, "dds_nd_top_push: bad sizes");
+  const int64_t n = std::max(n_top, n_push);
+  if (n == 0) return DDS_OK;
+  DDS_REQUIRE(perm && xtop && out && (n_push == 0 || (push_dst && push_src && xs)),
+              "dds_nd_top_push: null pointer");
+  hipStream_t s = as_stream(stream);
+  const dim3 grid((unsigned)ceil_div(n, kNdThreads));
+  if (dtype == DDS_F64) {
+    hipLaunchKernelGGL(nd_top_push_kernel<double>, grid, dim3(kNdThreads), 0, s, j0, n_top,
+                       n_push, perm, static_cast<const double*>(xtop), push_dst, push_src,
+                       static_cast<double*>(xs), static_cast<double*>(out), gate);
+  } else if (dtype == DDS_F32) {
+    hipLaunchKernelGGL(nd_top_push_kernel<float>, grid, dim3(kNdThreads), 0, s, j0, n_top,
+                       n_push, perm, static_cast<const float*>(xtop), push_dst, push_src,
+                       static_cast<float*>(xs), static_cast<float*>(out), gate);
+  } else {
+    set_error("dds_nd_top_push: unsupported dtype %d", dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
 }

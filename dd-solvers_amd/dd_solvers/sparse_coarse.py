@@ -25,13 +25,15 @@ deterministic), and
 which is the block LDL^T factorisation A = L D L^T with L_SJ = W, D_J = F_JJ. The top
 front is the Schur complement of everything below it; its inverse is dense.
 
-Solve (device, every application; HIP kernels dds_nd_forward / dds_nd_backward, one
-launch per height). Forward, leaves first: f = [b_J; 0] + the children's update vectors
-(gathered in child order), then g_J = Finv f_J and u = f_S - W f_J to the parent; the top
-front's g is its solution. Backward, top first: x_J = g_J - W^T x_S. The stacked rows
-[Finv; W] of a front are contiguous, so the forward sweep is a dense matvec split over
-workgroups by rows and the backward sweep one split by columns; both read each factor
-byte once and add in a fixed order.
+Solve (device, every application). Forward, leaves first, one dds_nd_forward launch per
+height: f = [b_J; 0] + the update vectors the children left in the front's slots (added
+in child order), then g_J = Finv f_J and u = f_S - W f_J, which goes into the parent's
+slots. Top: dds_nd_top_assemble, dds_dense_symv with its inverse, dds_nd_top_push.
+Backward, top first, one dds_nd_backward launch per height: x_J = g_J - W^T x_S, then x
+at each child's coupling set is pushed into that child's buffer. The stacked factor
+[Finv; W] of a front is column-major, so the forward sweep gives each thread one row and
+the backward sweep each lane one column, both streaming contiguous memory with
+independent loads; every sum runs in a fixed order.
 """
 
 from __future__ import annotations
@@ -51,7 +53,6 @@ __all__ = ["SparseCholesky", "nested_dissection", "NDPlan"]
 
 # LDS capacity of the sweep kernels, in elements (f_J forward, x_S backward): 60 KiB of FP64
 LDS_CAP = 7680
-FRONT_FIELDS = 8  # j0, nJ, nS, foff, fpos, fu, fsidx, unused
 
 
 # ---------------------------------------------------------------------------------------
@@ -75,16 +76,18 @@ def _bfs_levels(G: sps.csr_matrix, src: int) -> Tuple[np.ndarray, np.ndarray]:
     return order, lev
 
 
-def _pick_level(lev: np.ndarray) -> int:
-    """Separator level: the smallest level set leaving both sides >= 30% of the rest, else
-    the most balanced one."""
+def _pick_level(lev: np.ndarray, balance: float = 0.47) -> int:
+    """Separator level: the smallest level set leaving both sides >= `balance` of the rest,
+    else the most balanced one. Near-even splits keep the tree shallow (its height is the
+    number of launches of each sweep): on the 131,072-cell coarse mesh of the reference's
+    S8 runs, 0.3 gives 14 heights and 0.47 gives 8 for 30% more factor bytes."""
     cnt = np.bincount(lev)
     n = lev.shape[0]
     before = np.cumsum(cnt) - cnt
     after = n - before - cnt
     cand = np.arange(1, cnt.shape[0] - 1)
     bal = np.minimum(before[cand], after[cand]) / np.maximum(n - cnt[cand], 1)
-    ok = cand[bal >= 0.3]
+    ok = cand[bal >= balance]
     if ok.size:
         return int(ok[np.argmin(cnt[ok])])
     return int(cand[np.argmax(bal)])
@@ -146,14 +149,18 @@ def nested_dissection(G: sps.csr_matrix, leaf_size: int = 32) -> List[_Node]:
 # ---------------------------------------------------------------------------------------
 
 class NDPlan:
-    """Numbering, fronts, levels, gather lists and work items of the block LDL^T solve.
+    """Numbering, fronts, levels, data-movement tables and work items of the solve.
 
-    Fronts are numbered by height (leaves first, the top front last). For front t:
-      fronts[t] = (j0, nJ, nS, foff, fpos, fu, fsidx, 0): unknowns [j0, j0+nJ) of the
-      nested-dissection numbering; factor rows [Finv; W] at fac[foff:foff+(nJ+nS)*nJ]
-      (row-major, nJ columns); its nJ+nS positions start at fpos in the gather lists; its
-      update vector at u[fu:fu+nS]; its coupling set at sidx[fsidx:fsidx+nS].
-    perm[i] is the original index of unknown i of the nested-dissection numbering.
+    Fronts are numbered by height (leaves first, the top front last). Front t owns the
+    unknowns [fj0[t], fj0[t] + fnj[t]) of the nested-dissection numbering (perm[i] is the
+    original index of unknown i) and couples to the fns[t] unknowns S[t] of its ancestors.
+    Its stacked factor [Finv (nJ x nJ); W (nS x nJ)] is stored column-major at
+    fac[foff[t]:], element (r, c) at foff + c * m + r with m = nJ + nS (the top front, nS = 0,
+    is its symmetric dense inverse). Between levels data moves by pushes into fixed
+    buffers, so no kernel follows an index chain: forward, a child writes its update
+    vector into its own slot of its parent's front (slots[wbase[c] + maps[moff[c] + i]]);
+    backward, a parent writes x at each child's coupling set into that child's buffer
+    (xs[push_dst[k]] = value at front position push_src[k]).
     """
 
     def __init__(self, A: sps.csr_matrix, leaf_size: int = 32, top_max: int = 4096):
@@ -249,22 +256,22 @@ class NDPlan:
         self.fj0, self.fnj, self.fns = fj0, fnj, fns
         self.S, self.children = S, children
         self.level_ptr = np.concatenate([[0], np.cumsum(np.bincount(height))]).astype(np.int64)
-        if int(fnj.max()) > LDS_CAP or (fns.size and int(fns.max()) > LDS_CAP):
-            raise ValueError(f"a front exceeds {LDS_CAP} unknowns (nJ {int(fnj.max())}, "
-                             f"nS {int(fns.max())}); lower top_max or leaf_size")
+        low = slice(0, F - (1 if nT else 0))  # the top front is solved by dense kernels
+        if F - (1 if nT else 0) and (int(fnj[low].max()) > LDS_CAP or int(fns[low].max()) > LDS_CAP):
+            raise ValueError(f"a front exceeds {LDS_CAP} unknowns (nJ {int(fnj[low].max())}, "
+                             f"nS {int(fns[low].max())}); lower leaf_size or raise top_max")
 
         m = fnj + fns
+        self.m = m
         self.foff = np.concatenate([[0], np.cumsum(m * fnj)]).astype(np.int64)
-        self.fpos = np.concatenate([[0], np.cumsum(m)]).astype(np.int64)
-        self.fu = np.concatenate([[0], np.cumsum(fns)]).astype(np.int64)
         self.fac_size = int(self.foff[-1])
-        self.u_size = int(self.fu[-1])
-        self.sidx = (np.concatenate(S) if F else np.zeros(0)).astype(np.int32)
         # child maps: positions of each child's coupling set in its parent's front
         cmap: List[np.ndarray] = [None] * F
+        parent = np.full(F, -1, dtype=np.int64)
+        rank_in_parent = np.zeros(F, dtype=np.int64)
         for t in range(F):
             a, nj = fj0[t], fnj[t]
-            for c in children[t]:
+            for q, c in enumerate(children[t]):
                 sc = S[c]
                 inJ = sc < a + nj
                 p = np.empty(sc.shape[0], dtype=np.int64)
@@ -273,49 +280,72 @@ class NDPlan:
                 assert np.all(sc[inJ] >= a)
                 assert np.array_equal(S[t][p[~inJ] - nj], sc[~inJ]), "coupling set not nested"
                 cmap[c] = p
-        self.cmap = cmap
-        # gather lists of the forward sweep: for each front position, the update-vector
-        # entries of the children that land there, in child order
-        tgt, src = [], []
+                parent[c] = t
+                rank_in_parent[c] = q
+        self.cmap, self.parent = cmap, parent
+        nch = np.array([len(c) for c in children], dtype=np.int64)
+        # forward: child c adds its update vector into slot rank(c) of its parent's front,
+        # slots[soff[p] + rank * m[p] + position]; positions no child maps stay zero
+        self.soff = np.concatenate([[0], np.cumsum(nch * m)]).astype(np.int64)
+        self.slot_size = int(self.soff[-1])
+        self.wbase = np.where(parent >= 0, self.soff[:-1][np.maximum(parent, 0)] +
+                              rank_in_parent * m[np.maximum(parent, 0)], -1).astype(np.int64)
+        self.moff = np.concatenate([[0], np.cumsum(fns)]).astype(np.int64)
+        self.maps = (np.concatenate([cmap[c] if cmap[c] is not None else np.zeros(0, np.int64)
+                                     for c in range(F)]) if F else np.zeros(0)).astype(np.int32)
+        # backward: x at a front's coupling set is pushed into its own buffer xs[xoff:+nS]
+        # by its parent: push list of parent p = (xoff[c] + i, cmap[c][i]) sorted by source
+        self.xoff = self.moff
+        self.xs_size = int(self.moff[-1])
+        pdst, psrc, pptr = [], [], [0]
         for t in range(F):
-            for c in children[t]:
-                tgt.append(self.fpos[t] + cmap[c])
-                src.append(self.fu[c] + np.arange(fns[c], dtype=np.int64))
-        if tgt:
-            tgt = np.concatenate(tgt)
-            src = np.concatenate(src)
-            o = np.argsort(tgt, kind="stable")
-            tgt, src = tgt[o], src[o]
-        else:
-            tgt = src = np.zeros(0, dtype=np.int64)
-        self.gptr = np.searchsorted(tgt, np.arange(self.fpos[-1] + 1)).astype(np.int64)
-        self.gsrc = src.astype(np.int32)
-        fr = np.zeros((F, FRONT_FIELDS), dtype=np.int64)
-        fr[:, 0], fr[:, 1], fr[:, 2] = fj0, fnj, fns
-        fr[:, 3], fr[:, 4], fr[:, 5] = self.foff[:-1], self.fpos[:-1], self.fu[:-1]
-        fr[:, 6] = np.concatenate([[0], np.cumsum(fns)])[:-1]
-        self.fronts = fr
+            d = [self.xoff[c] + np.arange(fns[c], dtype=np.int64) for c in children[t]]
+            sr = [cmap[c] for c in children[t]]
+            d = np.concatenate(d) if d else np.zeros(0, np.int64)
+            sr = np.concatenate(sr) if sr else np.zeros(0, np.int64)
+            o = np.argsort(sr, kind="stable")
+            pdst.append(d[o])
+            psrc.append(sr[o])
+            pptr.append(pptr[-1] + d.shape[0])
+        self.push_dst = np.concatenate(pdst).astype(np.int64) if F else np.zeros(0, np.int64)
+        self.push_src = np.concatenate(psrc).astype(np.int32) if F else np.zeros(0, np.int32)
+        self.push_ptr = np.asarray(pptr, dtype=np.int64)
+        self.has_top = bool(nT)
+        self.n_lower = F - (1 if nT else 0)
         self.uoff2 = np.concatenate([[0], np.cumsum(fns * fns)]).astype(np.int64)
         self._items()
 
     def _items(self) -> None:
-        """Work items per level: forward (front, row0, rows) splits the stacked rows
-        [Finv; W] (about 256 KB of factor per item), backward (front, col0, cols) splits
-        the columns into groups of up to 64."""
+        """Work items of the lower levels (the top front is solved densely).
+
+        Forward record (12 x int64): j0, nJ, nS, foff, soff, n_children, wbase, moff, r0,
+        nr, 0, 0 -- rows r0 .. r0+nr-1 (nr <= 256, one per thread) of the front's stacked
+        factor. Backward record: j0, nJ, nS, foff, xoff, c0, nc, k0, k1, 0, 0, 0 -- columns
+        c0 .. c0+nc-1 (nc <= 64) and the push-list entries [k0, k1) whose sources are those
+        columns (the last item of a front also pushes the coupling-set values)."""
         self.fwd, self.bwd = [], []
-        for h in range(self.level_ptr.shape[0] - 1):
+        nlev = self.level_ptr.shape[0] - 1 - (1 if self.has_top else 0)
+        for h in range(nlev):
             fi, bi = [], []
             for t in range(self.level_ptr[h], self.level_ptr[h + 1]):
-                nj, ns = int(self.fnj[t]), int(self.fns[t])
-                R = int(max(4, min(64, (32768 // max(nj, 1)) // 4 * 4)))
-                for r0 in range(0, nj + ns, R):
-                    fi.append((t, r0, min(R, nj + ns - r0)))
+                j0, nj, ns, m = int(self.fj0[t]), int(self.fnj[t]), int(self.fns[t]), int(self.m[t])
+                head = [j0, nj, ns, int(self.foff[t])]
+                nch = len(self.children[t])
+                for r0 in range(0, m, 256):
+                    fi.append(head + [int(self.soff[t]), nch, int(self.wbase[t]),
+                                      int(self.moff[t]), r0, min(256, m - r0), 0, 0])
+                k_lo, k_hi = int(self.push_ptr[t]), int(self.push_ptr[t + 1])
+                src = self.push_src[k_lo:k_hi]
                 for c0 in range(0, nj, 64):
-                    bi.append((t, c0, min(64, nj - c0)))
-            lf = int(self.fnj[self.level_ptr[h]:self.level_ptr[h + 1]].max())
-            lb = int(self.fns[self.level_ptr[h]:self.level_ptr[h + 1]].max())
-            self.fwd.append((np.asarray(fi, dtype=np.int32).reshape(-1, 3), lf))
-            self.bwd.append((np.asarray(bi, dtype=np.int32).reshape(-1, 3), lb))
+                    nc = min(64, nj - c0)
+                    k0 = k_lo + int(np.searchsorted(src, c0))
+                    k1 = k_hi if c0 + nc == nj else k_lo + int(np.searchsorted(src, c0 + nc))
+                    bi.append(head + [int(self.xoff[t]), c0, nc, k0, k1, 0, 0, 0])
+            lo, hi = self.level_ptr[h], self.level_ptr[h + 1]
+            lf = int(self.fnj[lo:hi].max())
+            lb = int(self.fns[lo:hi].max())
+            self.fwd.append((np.asarray(fi, dtype=np.int64).reshape(-1, 12), lf))
+            self.bwd.append((np.asarray(bi, dtype=np.int64).reshape(-1, 12), lb))
 
     @property
     def n_levels(self) -> int:
@@ -323,25 +353,41 @@ class NDPlan:
 
     def validate(self) -> None:
         """Host check of every offset the sweep kernels use (before any launch)."""
-        F, n = self.n_fronts, self.n
-        fr = self.fronts
-        j0, nj, ns, foff, fpos, fu, fs = (fr[:, k] for k in range(7))
-        assert np.all(nj >= 1) and np.all(ns >= 0)
-        assert np.all(j0 >= 0) and np.all(j0 + nj <= n)
-        assert np.all(foff + (nj + ns) * nj <= self.fac_size)
-        assert np.all(fpos + nj + ns <= self.gptr.shape[0] - 1)
-        assert np.all(fu + ns <= self.u_size) and np.all(fs + ns <= self.sidx.shape[0])
-        assert self.gsrc.size == 0 or (self.gsrc.min() >= 0 and self.gsrc.max() < self.u_size)
-        assert self.sidx.size == 0 or (self.sidx.min() >= 0 and self.sidx.max() < n)
+        n = self.n
         assert np.array_equal(np.sort(self.perm), np.arange(n))
+        assert self.maps.size == 0 or self.maps.min() >= 0
+        assert self.push_src.size == 0 or self.push_src.min() >= 0
+        assert self.push_dst.size == 0 or (self.push_dst.min() >= 0 and
+                                           self.push_dst.max() < self.xs_size)
+        for t in range(self.n_fronts):  # every map entry lands inside the parent's front
+            p = self.parent[t]
+            if p >= 0:
+                assert self.maps[self.moff[t]:self.moff[t + 1]].max(initial=-1) < self.m[p]
+                assert self.wbase[t] + self.m[p] <= self.slot_size
         for items, lds in self.fwd:
-            t, r0, nr = items[:, 0], items[:, 1], items[:, 2]
-            assert np.all((t >= 0) & (t < F)) and np.all(nr >= 1)
-            assert np.all(r0 + nr <= nj[t] + ns[t]) and lds <= LDS_CAP and np.all(nj[t] <= lds)
+            j0, nj, ns, foff, soff, nch, wb, mo, r0, nr = (items[:, k] for k in range(10))
+            m = nj + ns
+            assert np.all(nj >= 1) and np.all(j0 + nj <= n) and np.all(foff + m * nj <= self.fac_size)
+            assert np.all(soff + nch * m <= self.slot_size)
+            assert np.all((nr >= 1) & (nr <= 256)) and np.all(r0 + nr <= m)
+            assert np.all((wb >= 0) | (ns == 0)) and np.all(mo + ns <= self.maps.shape[0])
+            assert lds <= LDS_CAP and np.all(nj <= lds)
         for items, lds in self.bwd:
-            t, c0, nc = items[:, 0], items[:, 1], items[:, 2]
-            assert np.all((nc >= 1) & (nc <= 64)) and np.all(c0 + nc <= nj[t])
-            assert lds <= LDS_CAP and np.all(ns[t] <= lds)
+            j0, nj, ns, foff, xo, c0, nc, k0, k1 = (items[:, k] for k in range(9))
+            assert np.all((nc >= 1) & (nc <= 64)) and np.all(c0 + nc <= nj)
+            assert np.all(xo + ns <= max(self.xs_size, 1)) and np.all(foff + (nj + ns) * nj <= self.fac_size)
+            assert np.all((0 <= k0) & (k0 <= k1) & (k1 <= self.push_src.shape[0]))
+            for it in items:  # push sources: own columns, or the coupling set (last item)
+                src = self.push_src[it[7]:it[8]]
+                last = it[5] + it[6] == it[1]
+                ok = ((src >= it[5]) & (src < it[5] + it[6])) | (last & (src >= it[1]) & (src < it[1] + it[2]))
+                assert np.all(ok)
+            assert lds <= LDS_CAP and np.all(ns <= lds)
+        if self.has_top:
+            t = self.n_fronts - 1
+            k0, k1 = self.push_ptr[t], self.push_ptr[t + 1]
+            assert self.push_src[k0:k1].max(initial=-1) < self.nT
+            assert self.soff[t] + len(self.children[t]) * self.nT <= self.slot_size
 
 
 # ---------------------------------------------------------------------------------------
@@ -384,12 +430,14 @@ def _level_indices(plan: NDPlan, h: int):
                 sU.append([])
             dU[r].append(base + (p[:, None] * M + p[None, :]).ravel())
             sU[r].append(plan.uoff2[c] + np.arange(ncs * ncs))
-        # packing: Finv (J x J) and W (s x J) into fac; U (s x s) into the update store
+        # packing: the stacked factor [Finv (J x J); W (s x J)] column-major (element
+        # (r, c) at foff + c*(J+s) + r) into fac; U (s x s) into the update store
+        m = J + s
         rr, cc = np.divmod(np.arange(J * J), J)
-        pF.append(plan.foff[t] + np.arange(J * J))
+        pF.append(plan.foff[t] + cc * m + rr)
         sF.append(b * Jm * Jm + rr * Jm + cc)
         rr, cc = np.divmod(np.arange(s * J), J)
-        pW.append(plan.foff[t] + J * J + np.arange(s * J))
+        pW.append(plan.foff[t] + cc * m + J + rr)
         sW.append(b * Sm * Jm + rr * Jm + cc)
         rr, cc = np.divmod(np.arange(s * s), max(s, 1))
         pUo.append(plan.uoff2[t] + np.arange(s * s))
@@ -401,8 +449,8 @@ def _level_indices(plan: NDPlan, h: int):
 
 
 def factorize(plan: NDPlan, values: np.ndarray, device, dtype=torch.float64) -> torch.Tensor:
-    """Factor rows [Finv; W] of every front (plan.fronts layout), in `dtype`, on `device`.
-    `values` are the entries of plan.Ap (FP64)."""
+    """Stacked factors [Finv; W] of every front (column-major at plan.foff), in `dtype`, on
+    `device`. `values` are the entries of plan.Ap (FP64)."""
     f64 = torch.float64
     vals = torch.as_tensor(np.asarray(values, dtype=np.float64), device=device)
     fac = torch.zeros(plan.fac_size, dtype=f64, device=device)
@@ -433,34 +481,50 @@ def factorize(plan: NDPlan, values: np.ndarray, device, dtype=torch.float64) -> 
 
 
 def solve_host(plan: NDPlan, fac: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """The two sweeps of the device solve, restated on the host over the same tables (a
-    check of the plan and the factor; the device path never calls it)."""
+    """The device solve restated on the host over the same tables and buffers (a check of
+    the plan and the factor; the device path never calls it)."""
     n = plan.n
-    G = np.zeros(n)
-    X = np.zeros(n)
-    U = np.zeros(plan.u_size)
+    g = np.zeros(n)
+    slots = np.zeros(max(plan.slot_size, 1))
+    xs = np.zeros(max(plan.xs_size, 1))
     out = np.zeros(n)
-    fr = plan.fronts
-    for h in range(plan.n_levels):
-        for t in range(plan.level_ptr[h], plan.level_ptr[h + 1]):
-            j0, nj, ns, foff, fpos, fu = (int(v) for v in fr[t, :6])
-            f = np.zeros(nj + ns)
-            f[:nj] = b[plan.perm[j0:j0 + nj]]
-            for p in range(nj + ns):
-                for k in range(plan.gptr[fpos + p], plan.gptr[fpos + p + 1]):
-                    f[p] += U[plan.gsrc[k]]
-            R = fac[foff:foff + (nj + ns) * nj].reshape(nj + ns, nj)
-            y = R @ f[:nj]
-            G[j0:j0 + nj] = y[:nj]
-            U[fu:fu + ns] = f[nj:] - y[nj:]
-    for h in reversed(range(plan.n_levels)):
-        for t in range(plan.level_ptr[h], plan.level_ptr[h + 1]):
-            j0, nj, ns, foff, _, _, fs = (int(v) for v in fr[t, :7])
-            W = fac[foff + nj * nj:foff + (nj + ns) * nj].reshape(ns, nj)
-            xs = X[plan.sidx[fs:fs + ns]]
-            x = G[j0:j0 + nj] - W.T @ xs
-            X[j0:j0 + nj] = x
-            out[plan.perm[j0:j0 + nj]] = x
+    for items, _ in plan.fwd:                        # dds_nd_forward, level by level
+        for it in items:
+            j0, nj, ns, foff, soff, nch, wb, mo, r0, nr = (int(v) for v in it[:10])
+            m = nj + ns
+            f = b[plan.perm[j0:j0 + nj]].copy()
+            for q in range(nch):
+                f += slots[soff + q * m:soff + q * m + nj]
+            M = fac[foff:foff + m * nj].reshape(nj, m).T   # column-major (m x nj)
+            for r in range(r0, r0 + nr):
+                acc = M[r] @ f
+                if r < nj:
+                    g[j0 + r] = acc
+                else:
+                    fs = sum(slots[soff + q * m + r] for q in range(nch))
+                    slots[wb + plan.maps[mo + r - nj]] = fs - acc
+    if plan.has_top:                                 # dds_nd_top_assemble, dense_symv, push
+        t = plan.n_fronts - 1
+        j0, nT = int(plan.fj0[t]), plan.nT
+        ft = b[plan.perm[j0:j0 + nT]].copy()
+        for q in range(len(plan.children[t])):
+            ft += slots[plan.soff[t] + q * nT:plan.soff[t] + (q + 1) * nT]
+        xt = fac[plan.foff[t]:plan.foff[t] + nT * nT].reshape(nT, nT) @ ft
+        out[plan.perm[j0:j0 + nT]] = xt
+        k0, k1 = plan.push_ptr[t], plan.push_ptr[t + 1]
+        xs[plan.push_dst[k0:k1]] = xt[plan.push_src[k0:k1]]
+    for items, _ in reversed(plan.bwd):              # dds_nd_backward, top level first
+        for it in items:
+            j0, nj, ns, foff, xo, c0, nc, k0, k1 = (int(v) for v in it[:9])
+            m = nj + ns
+            M = fac[foff:foff + m * nj].reshape(nj, m).T
+            xsl = xs[xo:xo + ns].copy()
+            xj = g[j0 + c0:j0 + c0 + nc] - M[nj:, c0:c0 + nc].T @ xsl
+            out[plan.perm[j0 + c0:j0 + c0 + nc]] = xj
+            src = plan.push_src[k0:k1]
+            val = np.where(src < nj, xj[np.clip(src - c0, 0, nc - 1)],
+                           xsl[np.clip(src - nj, 0, max(ns - 1, 0))] if ns else 0.0)
+            xs[plan.push_dst[k0:k1]] = val
     return out
 
 
@@ -478,7 +542,7 @@ class SparseCholesky(SparseSolver):
     solve(b): 2 x (levels) HIP launches, every sum in a fixed order.
     """
 
-    def __init__(self, leaf_size: int = 32, top_max: int = 4096):
+    def __init__(self, leaf_size: int = 64, top_max: int = 4096):
         self.leaf_size = leaf_size
         self.top_max = top_max
         self.plan = None
@@ -507,16 +571,19 @@ class SparseCholesky(SparseSolver):
             self.fac = factorize(plan, plan.Ap.data, dev, self.dtype)
             I = lambda a, dt=torch.int64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt,
                                                           device=dev)
-            self._fronts = I(plan.fronts.reshape(-1))
-            self._gptr = I(plan.gptr)
-            self._gsrc = I(plan.gsrc, torch.int32)
-            self._sidx = I(plan.sidx, torch.int32)
+            self._maps = I(plan.maps, torch.int32)
             self._perm = I(plan.perm, torch.int32)
-            self._fwd = [(I(it.reshape(-1), torch.int32), it.shape[0], lds) for it, lds in plan.fwd]
-            self._bwd = [(I(it.reshape(-1), torch.int32), it.shape[0], lds) for it, lds in plan.bwd]
-            self._G = torch.empty(self.n, dtype=self.dtype, device=dev)
-            self._X = torch.empty(self.n, dtype=self.dtype, device=dev)
-            self._U = torch.empty(max(plan.u_size, 1), dtype=self.dtype, device=dev)
+            self._push_dst = I(plan.push_dst)
+            self._push_src = I(plan.push_src, torch.int32)
+            self._fwd = [(I(it.reshape(-1)), it.shape[0], lds) for it, lds in plan.fwd]
+            self._bwd = [(I(it.reshape(-1)), it.shape[0], lds) for it, lds in plan.bwd]
+            self._g = torch.empty(self.n, dtype=self.dtype, device=dev)
+            # slot entries no child maps are read as zeros and never written
+            self._slots = torch.zeros(max(plan.slot_size, 1), dtype=self.dtype, device=dev)
+            self._xs = torch.zeros(max(plan.xs_size, 1), dtype=self.dtype, device=dev)
+            if plan.has_top:
+                self._ftop = torch.empty(plan.nT, dtype=self.dtype, device=dev)
+                self._xtop = torch.empty(plan.nT, dtype=self.dtype, device=dev)
 
     def solve(self, rhs: torch.Tensor):
         if rhs.dtype != self.dtype:
@@ -532,15 +599,27 @@ class SparseCholesky(SparseSolver):
         dt = _lib.dtype_code(self.dtype)
         s = _lib.stream()
         g = _lib.ptr(gate)
+        p = self.plan
         for items, n_items, lds in self._fwd:
-            _lib.call("dds_nd_forward", dt, n_items, items.data_ptr(), self._fronts.data_ptr(),
-                      self._gptr.data_ptr(), self._gsrc.data_ptr(), self._perm.data_ptr(),
-                      self.fac.data_ptr(), x.data_ptr(), self._G.data_ptr(), self._U.data_ptr(),
-                      lds, g, s)
+            _lib.call("dds_nd_forward", dt, n_items, items.data_ptr(), self._perm.data_ptr(),
+                      self._maps.data_ptr(), self.fac.data_ptr(), x.data_ptr(),
+                      self._g.data_ptr(), self._slots.data_ptr(), lds, g, s)
+        if p.has_top:
+            t = p.n_fronts - 1
+            j0, nT = int(p.fj0[t]), p.nT
+            _lib.call("dds_nd_top_assemble", dt, j0, nT, int(p.soff[t]), len(p.children[t]),
+                      self._perm.data_ptr(), x.data_ptr(), self._slots.data_ptr(),
+                      self._ftop.data_ptr(), g, s)
+            _lib.call("dds_dense_symv", dt, nT, self.fac[int(p.foff[t]):].data_ptr(), nT,
+                      self._ftop.data_ptr(), self._xtop.data_ptr(), g, s)
+            k0, k1 = int(p.push_ptr[t]), int(p.push_ptr[t + 1])
+            _lib.call("dds_nd_top_push", dt, j0, nT, k1 - k0, self._perm.data_ptr(),
+                      self._xtop.data_ptr(), self._push_dst[k0:].data_ptr(),
+                      self._push_src[k0:].data_ptr(), self._xs.data_ptr(), y.data_ptr(), g, s)
         for items, n_items, lds in reversed(self._bwd):
-            _lib.call("dds_nd_backward", dt, n_items, items.data_ptr(), self._fronts.data_ptr(),
-                      self._sidx.data_ptr(), self._perm.data_ptr(), self.fac.data_ptr(),
-                      self._G.data_ptr(), self._X.data_ptr(), y.data_ptr(), lds, g, s)
+            _lib.call("dds_nd_backward", dt, n_items, items.data_ptr(), self._perm.data_ptr(),
+                      self._push_dst.data_ptr(), self._push_src.data_ptr(), self.fac.data_ptr(),
+                      self._g.data_ptr(), self._xs.data_ptr(), y.data_ptr(), lds, g, s)
 
     def _enqueue_solve_multi(self, x, y, st) -> None:
         raise ValueError("blocks of right-hand sides need the DenseCholesky coarse solver")
@@ -549,7 +628,8 @@ class SparseCholesky(SparseSolver):
     def bytes_per_solve(self) -> int:
         e = 8 if self.dtype == torch.float64 else 4
         p = self.plan
-        return e * int(np.sum(p.fnj * p.fnj + 2 * p.fns * p.fnj)) + 4 * e * p.n + 8 * p.u_size
+        return e * int(np.sum(p.fnj * p.fnj + 2 * p.fns * p.fnj)) + 4 * e * p.n + \
+            4 * e * p.xs_size
 
     def destroy(self):
         self.plan = None

@@ -28,7 +28,7 @@
  *   dds_chol_inverse, dds_dense_symv
  *                             coarse solve for large coarse spaces (explicit A_C^-1 from the
  *                             Cholesky factor, applied as one dense symmetric matvec)
- *   dds_nd_forward, dds_nd_backward
+ *   dds_nd_forward, dds_nd_backward, dds_nd_top_assemble, dds_nd_top_push
  *                             cudssExecute(SOLVE) of a large sparse coarse matrix
  *                             (cudss.cpp:64-78, called from CUDSS.solve, solvers.py:106-110):
  *                             the two sweeps of a nested-dissection block LDL^T factor
@@ -216,29 +216,44 @@ int dds_dense_symv(int dtype, int64_t n, const void* s, int64_t lds, const void*
 
 /* ---- sparse coarse solver (nested-dissection block LDL^T) ------------------------------
  * The solve of A x = b with A = L D L^T from a multifrontal factorisation in a
- * nested-dissection numbering (perm[i] = original index of unknown i). Front t is
- * fronts[8t .. 8t+7] = (j0, nJ, nS, foff, fpos, fu, fsidx, unused): unknowns [j0, j0+nJ);
- * its factor rows [F_JJ^-1 (nJ x nJ); W = F_SJ F_JJ^-1 (nS x nJ)] row-major at fac + foff;
- * front positions fpos .. fpos+nJ+nS-1 of the gather lists (position p adds
- * u[gsrc[k]] for k in [gptr[fpos+p], gptr[fpos+p+1])); its update vector u + fu and its
- * coupling set sidx + fsidx (nS each). items are int32 triples, one per workgroup; a launch
- * covers the fronts of one height of the dissection tree. lds_elems (<= 7680) is the
- * largest nJ (forward) or nS (backward) of the launch. F64 and F32, sums in FP64, every
- * sum in a fixed order. No allocation; b, out are in the original numbering.
+ * nested-dissection numbering (perm[i] = original index of unknown i): one forward and one
+ * backward launch per height of the dissection tree, the dense top front in between
+ * (dds_nd_top_assemble, dds_dense_symv with its inverse, dds_nd_top_push). A front owns
+ * unknowns [j0, j0+nJ), couples to nS unknowns of its ancestors, and stores the stacked
+ * factor [F_JJ^-1 (nJ x nJ); W = F_SJ F_JJ^-1 (nS x nJ)] column-major at fac + foff (element
+ * (r, c) at foff + c*(nJ+nS) + r). Between levels, a child writes its update vector into its
+ * own slot of its parent's front, slots[wbase + maps[moff + s]] (entries no child maps stay
+ * zero), and a parent writes x at a child's coupling set into that child's buffer,
+ * xs[push_dst[k]] = x at front position push_src[k]. Work items are 12-int64 records, one
+ * per workgroup; lds_elems (<= 7680) is the largest nJ (forward) or nS (backward) of the
+ * launch. F64 and F32, sums in FP64 in a fixed order. No allocation; b and out are in the
+ * original numbering.
  *
- * Forward (leaves first), item (t, r0, nr): with f_J = b[perm[j0 + .]] + gathered updates,
- *   rows r < nJ:  g[j0 + r] = (F_JJ^-1 f_J)_r;  rows r >= nJ: u[fu + r - nJ] =
- *   (gathered updates at r) - (W f_J)_{r - nJ}.                                          */
-int dds_nd_forward(int dtype, int64_t n_items, const int32_t* items, const int64_t* fronts,
-                   const int64_t* gptr, const int32_t* gsrc, const int32_t* perm,
-                   const void* fac, const void* b, void* g, void* u, int64_t lds_elems,
-                   const dds_pcg_state* gate, void* stream);
-/* Backward (top first), item (t, c0, nc <= 64): x_nd[j0 + c] = g[j0 + c] - (W^T x_S)_c with
- * x_S = x_nd[sidx[fsidx + .]], and out[perm[j0 + c]] = the same value. */
-int dds_nd_backward(int dtype, int64_t n_items, const int32_t* items, const int64_t* fronts,
-                    const int32_t* sidx, const int32_t* perm, const void* fac, const void* g,
-                    void* x_nd, void* out, int64_t lds_elems, const dds_pcg_state* gate,
-                    void* stream);
+ * Forward record (j0, nJ, nS, foff, soff, n_children, wbase, moff, r0, nr <= 256, -, -):
+ * with f_J = b[perm[j0 + .]] + sum over the children q of slots[soff + q*(nJ+nS) + .], row
+ * r of [r0, r0+nr): r < nJ: g[j0 + r] = (F_JJ^-1 f_J)_r; r >= nJ: slots[wbase +
+ * maps[moff + r - nJ]] = (children's slots at r) - (W f_J)_{r - nJ}. */
+int dds_nd_forward(int dtype, int64_t n_items, const int64_t* items, const int32_t* perm,
+                   const int32_t* maps, const void* fac, const void* b, void* g, void* slots,
+                   int64_t lds_elems, const dds_pcg_state* gate, void* stream);
+/* Backward record (j0, nJ, nS, foff, xoff, c0, nc <= 64, k0, k1, -, -, -): x_J[c] = g[j0 + c]
+ * - (W^T x_S)_c for c in [c0, c0+nc) with x_S = xs[xoff .. xoff+nS), out[perm[j0 + c]] =
+ * x_J[c]; then push entries [k0, k1) (sources in [c0, c0+nc), or >= nJ for the coupling
+ * set). */
+int dds_nd_backward(int dtype, int64_t n_items, const int64_t* items, const int32_t* perm,
+                    const int64_t* push_dst, const int32_t* push_src, const void* fac,
+                    const void* g, void* xs, void* out, int64_t lds_elems,
+                    const dds_pcg_state* gate, void* stream);
+/* Top front, unknowns [j0, j0+n_top): ftop[p] = b[perm[j0 + p]] + sum over its n_children
+ * children q of slots[soff + q*n_top + p]. */
+int dds_nd_top_assemble(int dtype, int64_t j0, int64_t n_top, int64_t soff, int64_t n_children,
+                        const int32_t* perm, const void* b, const void* slots, void* ftop,
+                        const dds_pcg_state* gate, void* stream);
+/* out[perm[j0 + p]] = xtop[p] for p < n_top; xs[push_dst[i]] = xtop[push_src[i]] for
+ * i < n_push. */
+int dds_nd_top_push(int dtype, int64_t j0, int64_t n_top, int64_t n_push, const int32_t* perm,
+                    const void* xtop, const int64_t* push_dst, const int32_t* push_src,
+                    void* xs, void* out, const dds_pcg_state* gate, void* stream);
 
 /* ---- PCG vector kernels (F64) ----------------------------------------------------------
  * Each writes per-block partial sums where stated (count returned in *n_partials). */

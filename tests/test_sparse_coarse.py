@@ -112,9 +112,9 @@ def test_planted_mistake_is_caught():
     plan = sc.NDPlan(A, leaf_size=8, top_max=30)
     fac = sc.factorize(plan, plan.Ap.data, "cpu").numpy()
     t = plan.n_fronts // 2
-    j0, nj, ns, foff = (int(v) for v in plan.fronts[t, :4])
+    nj, ns, foff = int(plan.fnj[t]), int(plan.fns[t]), int(plan.foff[t])
     assert ns > 0
-    fac[foff + nj * nj] *= 1 + 1e-6          # first entry of W
+    fac[foff + nj] *= 1 + 1e-6               # W[0, 0] (column-major stacked factor)
     b = np.random.default_rng(1).standard_normal(A.shape[0])
     x = sc.solve_host(plan, fac, b)
     xr = spla.spsolve(A.tocsc(), b)
@@ -135,11 +135,15 @@ def test_front_capacity_is_enforced():
 
 def test_host_validation_of_the_sweep_entry_points():
     with pytest.raises(_lib.HipError, match="front of"):
-        _lib.call("dds_nd_forward", _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 7681, None, None)
+        _lib.call("dds_nd_forward", _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 7681, None, None)
     with pytest.raises(_lib.HipError, match="front of"):
         _lib.call("dds_nd_backward", _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 1, 9000, None, None)
     with pytest.raises(_lib.HipError, match="unsupported"):
-        _lib.call("dds_nd_forward", _lib.F16, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 8, None, None)
+        _lib.call("dds_nd_forward", _lib.F16, 1, 1, 1, 1, 1, 1, 1, 1, 8, None, None)
+    with pytest.raises(_lib.HipError, match="bad sizes"):
+        _lib.call("dds_nd_top_assemble", _lib.F64, 0, -1, 0, 0, 1, 1, 1, 1, None, None)
+    with pytest.raises(_lib.HipError, match="null pointer"):
+        _lib.call("dds_nd_top_push", _lib.F64, 0, 4, 2, 1, 1, None, None, 1, 1, None, None)
 
 
 def test_cpu_tensors_rejected():
