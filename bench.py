@@ -10,10 +10,12 @@ excluded (the reference's protocol). Prints one JSON line: `value` is the plain 
 time to solution; beside it the same solve with FP32/FP16/BF16 local inverses, the
 batched form (16 right-hand sides, FP16/BF16 MFMA local inverses), the uneven-subdomain
 batching workload (jittered 2D P1 512^2, 256 k-means subdomains, overlap 2, 16
-right-hand sides) and the 128^3 comparison of plain FP64 against the mixed and batched
-forms.
+right-hand sides), the 128^3 comparison of plain FP64 against the mixed and batched
+forms, and the reference's own DG precision-experiment runs (p = 1 on S10, p = 3 on S9)
+beside its published A100 times.
 
     python bench.py [--steps K] [--warmup W] [--N 64] [--no-cpu] [--no-extra] [--no-target]
+                    [--no-reference]
 """
 
 from __future__ import annotations
@@ -357,6 +359,96 @@ def bsr_reference_shapes(dev):
     return out
 
 
+# The reference's precision experiment (results/experiment_precisions.csv, rtol 1e-9, A100)
+# on its own DG systems: (p, fine mesh, subdomain mesh, coarse mesh).
+REF_EXPERIMENTS = [(1, 10, "S8", "S8"), (1, 10, "S9", "S9"), (3, 9, "S8", "S8"), (3, 9, "S7", "S7")]
+REF_SOLVERS = [("hybrid", "float64", None), ("hybrid", "float64", "float16"),
+               ("additive", "float64", None), ("additive", "float32", "float16")]
+
+
+def reference_experiments(dev, steps, warmup):
+    """The reference's DG experiment runs with this build in every slot (the BSR SpMV with
+    block size = DoFs per cell, the local inverses in the reference's precisions, the
+    sparse coarse solver for its CUDSS slot), graph-replayed iteration blocks, min over
+    `steps` solves; beside the reference's published iteration count and best A100 time."""
+    import numpy as np
+    import torch
+    import dd_solvers as dds
+    from dd_solvers import dg
+
+    with open(os.path.join(ROOT, "tests", "golden", "dg_published.json")) as f:
+        pub = json.load(f)["solves"]
+    T = {"float64": torch.float64, "float32": torch.float32, "float16": torch.float16}
+    out = []
+    meshes = {}
+    for p, fine, sm, cm in REF_EXPERIMENTS:
+        t = time.perf_counter()
+        if (p, fine) not in meshes:
+            meshes.clear()
+            meshes[(p, fine)] = dg.assemble_sipg(dg.UniformTriangleMesh(fine), p)
+        pr = meshes[(p, fine)]
+        perm, A, bh, k, spc = dg.dg_experiment(pr, sm, cm)
+        At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr.astype(np.int64)),
+                                     torch.as_tensor(A.indices.astype(np.int64)),
+                                     torch.as_tensor(A.data), A.shape).to(dev)
+        b = torch.as_tensor(bh, device=dev)
+        log(f"DG p={p} S{fine} solvers {sm} coarse {cm}: n={A.shape[0]} k={k} n_c={spc.shape[0]} "
+            f"({time.perf_counter() - t:.1f}s)")
+        for kind, pi, inv in REF_SOLVERS:
+            ref = [r for r in pub if (r["p"], r["fine"], r["solvers"], r["coarse"], r["kind"],
+                                      r["precision"], r["inv"]) == (p, f"S{fine}", sm, cm, kind, pi, inv)]
+            ref = ref[0] if ref else {}
+            row = {"p": p, "fine": f"S{fine}", "solvers": sm, "coarse": cm, "kind": kind,
+                   "precision": pi, "inv": inv, "dofs": int(A.shape[0]), "n_coarse": int(spc.shape[0]),
+                   "reference_solver": ref.get("solver"),
+                   "reference_iterations": ref.get("iterations"),
+                   "reference_a100_ms": None if ref.get("best_ms") is None else round(ref["best_ms"], 2)}
+            try:
+                cls = dds.AdditiveSchwarz if kind == "additive" else dds.HybridSchwarz
+                P = cls(T[pi], dds.Inv(T[inv] if inv else None), dds.SparseCholesky(),
+                        collect_timings=True)
+                s = dds.CG(P, rtol=1e-9, maxiter=2000, use_graph=True)
+                t = time.perf_counter()
+                tm = s.setup(At, dofs_per_solver=k, solvers_per_coarse=torch.as_tensor(spc, device=dev),
+                             block_size=pr.block_size)
+                torch.cuda.synchronize()
+                setup_s = time.perf_counter() - t
+                for _ in range(warmup):
+                    s.solve(b)
+                best = None
+                for _ in range(steps):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    x, md = s.solve(b)
+                    e1.record()
+                    torch.cuda.synchronize()
+                    ms = e0.elapsed_time(e1)
+                    best = ms if best is None else min(best, ms)
+                xh = x.double().cpu().numpy()
+                res = float(np.linalg.norm(A @ xh - bh) / np.linalg.norm(bh))
+                row.update({"iterations": md["iterations"], "ms": round(best, 2),
+                            "true_rel_residual": res, "setup_s": round(setup_s, 1),
+                            "local_solve_ms": round(tm["local solver solve time"], 4),
+                            "coarse_solve_ms": round(tm["coarse solver solve time"], 4),
+                            "local_layout": P.local.layout if P.local is not None else None,
+                            "coarse_levels": P.coarse_solver.plan.n_levels})
+                if row["reference_a100_ms"]:
+                    row["speedup_vs_a100"] = round(row["reference_a100_ms"] / best, 2)
+                log(f"  {kind} {pi}+{inv}: {md['iterations']} it (reference {row['reference_iterations']}), "
+                    f"{best:.1f} ms (A100 {row['reference_a100_ms']} ms), local {row['local_solve_ms']} ms, "
+                    f"coarse {row['coarse_solve_ms']} ms")
+                s.destroy()
+                del s, P
+            except Exception as e:  # record and go on with the others
+                row["error"] = str(e)[:200]
+                log(f"  {kind} {pi}+{inv}: error {e}")
+            torch.cuda.empty_cache()
+            out.append(row)
+        del At, A, b
+    return out
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 PMC
     measurement (2 x FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), or None."""
@@ -467,6 +559,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the FP64 line and its kernels")
     ap.add_argument("--no-target", action="store_true", help="skip the 128^3 comparison")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="skip the reference's DG precision-experiment runs")
     ap.add_argument("--graph", action="store_true", help="replay captured iteration blocks")
     ap.add_argument("--profile-kernel", action="store_true",
                     help="after setup only launch the local apply, the SpMV and a 1 GiB device "
@@ -619,7 +713,12 @@ def main():
                 extra["target_128"] = {"error": str(e)[:200]}
 
     stream = None
-    gemv_ref = bsr_ref = None
+    gemv_ref = bsr_ref = ref_exp = None
+    if not args.no_extra and not args.no_reference:
+        try:
+            ref_exp = reference_experiments(dev, 3, 1)
+        except Exception as e:
+            ref_exp = {"error": str(e)[:200]}
     if not args.no_extra:
         try:
             stream = stream_peaks(dev)
@@ -680,6 +779,7 @@ def main():
         "stream": stream,
         "gemv_reference_benchmark": gemv_ref,
         "bsr_reference_benchmark": bsr_ref,
+        "reference_experiments": ref_exp,
         "solve_roofline": {"alg_bytes": int(solve_bytes), "achieved_GBs": round(solve_gbs, 1),
                            "frac": round(solve_gbs / HBM_PEAK_GBS, 4)},
         "spmv": {"us": round(t_spmv * 1e3, 2), "bytes": int(spmv_bytes),

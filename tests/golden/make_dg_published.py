@@ -6,7 +6,10 @@ Reads two result files of the reference (text data, not code):
   results/experiment_precisions.csv  -> for the FP64 Schwarz runs (preconditioner and
                                         local-inverse precision float64): iteration
                                         count and residual-norm history per
-                                        (p, fine, solvers, coarse, Additive/Hybrid)
+                                        (p, fine, solvers, coarse, Additive/Hybrid);
+                                        for every run ("solves"): the solver's
+                                        precisions, its iteration count (null when it
+                                        raised) and its best solve time on the A100
 
 Run from the repository root in a container that has the reference checked out at
 /root/reference:  python tests/golden/make_dg_published.py
@@ -36,9 +39,22 @@ def main():
                                     "dofs": int(r["DoFs"]), "nnz": int(r["matrix nnz"]),
                                     "b_norm": md["residual norms"][0]})
     runs = {}
+    solves = []
     with open(os.path.join(REF, "experiment_precisions.csv")) as fh:
         for r in csv.DictReader(fh):
             s = r["solver"]
+            if s.startswith("CG(") and "Schwarz(" in s:
+                pi = s.split("Schwarz(")[1].split(",")[0].replace("torch.", "")
+                inv = s.split("Inv(")[1].split(")")[0].replace("torch.", "") or None
+                times = ast.literal_eval(r["solve times"]) if r["solve times"] else []
+                md = ast.literal_eval(r["metadata"]) if r["metadata"] else {}
+                solves.append({"p": int(r["p"]), "fine": r["fine m"], "solvers": r["solvers m"],
+                               "coarse": r["coarse m"],
+                               "kind": "additive" if "AdditiveSchwarz" in s else "hybrid",
+                               "precision": pi, "inv": inv,
+                               "iterations": None if r["exception"] else md.get("iterations"),
+                               "best_ms": min(times) if times and not r["exception"] else None,
+                               "solver": s})
             if not s.startswith("CG(") or "torch.float64, Inv()" not in s or r["exception"]:
                 continue
             kind = "additive" if "AdditiveSchwarz" in s else "hybrid"
@@ -52,8 +68,8 @@ def main():
     with open(OUT, "w") as fh:
         json.dump({"source": "mgr0dzicki/dd-solvers results/experiment_amgx_pcg.csv and "
                              "results/experiment_precisions.csv (rtol 1e-9, FP64 runs)",
-                   "levels": levels, "runs": runs}, fh, indent=1)
-    print(f"wrote {OUT}: {len(levels)} levels, {len(runs)} runs")
+                   "levels": levels, "runs": runs, "solves": solves}, fh, indent=1)
+    print(f"wrote {OUT}: {len(levels)} levels, {len(runs)} runs, {len(solves)} solves")
 
 
 if __name__ == "__main__":
