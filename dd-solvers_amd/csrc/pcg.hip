@@ -80,11 +80,20 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(int64_t n, const double* __
   write_partial(acc, partials, scratch);
 }
 
-// Fixed-order sum of the partials: lane t adds t, t + 256, ... then the block tree.
+// Fixed-order sum of the partials: lane t keeps four sums over t + 256 (4i + q), i = 0, 1,
+// ..., so four independent loads are in flight per step, adds them in order, then the
+// block tree.
 __device__ double sum_partials_block(const double* partials, int count, double* scratch) {
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < count; i += kBlock) acc += partials[i];
-  return block_sum(acc, scratch);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * kBlock < count; i += 4 * kBlock) {
+    a0 += partials[i];
+    a1 += partials[i + kBlock];
+    a2 += partials[i + 2 * kBlock];
+    a3 += partials[i + 3 * kBlock];
+  }
+  for (; i < count; i += kBlock) a0 += partials[i];
+  return block_sum((a0 + a1) + (a2 + a3), scratch);
 }
 
 __device__ __forceinline__ void check_norm(dds_pcg_state* st, double rr, double* norms) {

@@ -46,10 +46,10 @@
 extern "C" {
 #endif
 
-#define DDS_ABI_VERSION 2
+#define DDS_ABI_VERSION 3
 /* Upper bound on the per-block partial sums written by one reduction-producing launch
  * (partial buffers hold this many doubles, x16 for blocks of right-hand sides). */
-#define DDS_MAX_PARTIALS 8192
+#define DDS_MAX_PARTIALS 16384
 
 typedef enum dds_dtype {
   DDS_F64 = 0,
