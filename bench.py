@@ -366,7 +366,7 @@ REF_SOLVERS = [("hybrid", "float64", None), ("hybrid", "float64", "float16"),
                ("additive", "float64", None), ("additive", "float32", "float16")]
 
 
-def reference_experiments(dev, steps, warmup):
+def reference_experiments(dev, steps, warmup, only=None):
     """The reference's DG experiment runs with this build in every slot (the BSR SpMV with
     block size = DoFs per cell, the local inverses in the reference's precisions, the
     sparse coarse solver for its CUDSS slot), graph-replayed iteration blocks, min over
@@ -382,6 +382,8 @@ def reference_experiments(dev, steps, warmup):
     out = []
     meshes = {}
     for p, fine, sm, cm in REF_EXPERIMENTS:
+        if only and f"p{p}_S{fine}_{sm}_{cm}" not in only:
+            continue
         t = time.perf_counter()
         if (p, fine) not in meshes:
             meshes.clear()
@@ -561,6 +563,9 @@ def main():
     ap.add_argument("--no-target", action="store_true", help="skip the 128^3 comparison")
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the reference's DG precision-experiment runs")
+    ap.add_argument("--reference-only", default=None, metavar="RUNS",
+                    help="only the reference's DG runs named here (comma-separated, e.g. "
+                         "p1_S10_S8_S8), printed as one JSON line")
     ap.add_argument("--graph", action="store_true", help="replay captured iteration blocks")
     ap.add_argument("--profile-kernel", action="store_true",
                     help="after setup only launch the local apply, the SpMV and a 1 GiB device "
@@ -583,6 +588,10 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     _tnames()
+    if args.reference_only:
+        runs = reference_experiments(dev, args.steps, args.warmup, args.reference_only.split(","))
+        print(json.dumps({"reference_experiments": runs}), flush=True)
+        return
     pb = build_problem(args.N, args.seed)
     b = torch.as_tensor(pb.b, device=dev)
 

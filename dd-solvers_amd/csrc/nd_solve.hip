@@ -20,6 +20,7 @@ namespace {
 
 constexpr int kNdThreads = 256;
 constexpr int kNdWaves = kNdThreads / 64;
+static_assert(kNdWaves == 4, "the top assembly adds four wave sums");
 constexpr int64_t kNdLdsCap = 7680;  // dynamic LDS elements per workgroup: 60 KiB of FP64
 
 // Forward item (12 x int64): j0, nJ, nS, foff, soff, n_children, wbase, moff, r0, nr.
@@ -127,18 +128,34 @@ __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
   }
 }
 
-// Top front (unknowns j0 .. j0+nT-1): f = b[perm[j0 + p]] + its children's slots.
+// Top front (unknowns j0 .. j0+nT-1): f = b[perm[j0 + p]] + its children's slots. A
+// workgroup takes 64 positions; wave g sums the children q = g, g+4, ... (two loads in
+// flight per step), and the four wave sums are added in wave order.
 template <typename T>
 __global__ __launch_bounds__(kNdThreads) void nd_top_assemble_kernel(
     int64_t j0, int64_t nT, int64_t soff, int nch, const int32_t* __restrict__ perm,
     const T* __restrict__ b, const T* __restrict__ slots, T* __restrict__ ftop,
     const dds_pcg_state* __restrict__ gate) {
+  __shared__ double part[kNdWaves][64];
   if (gate != nullptr && gate->done) return;
-  const int64_t p = (int64_t)blockIdx.x * kNdThreads + threadIdx.x;
-  if (p >= nT) return;
-  double v = static_cast<double>(b[perm[j0 + p]]);
-  for (int q = 0; q < nch; ++q) v += static_cast<double>(slots[soff + (int64_t)q * nT + p]);
-  ftop[p] = static_cast<T>(v);
+  const int pl = threadIdx.x & 63;
+  const int g = threadIdx.x >> 6;
+  const int64_t p = (int64_t)blockIdx.x * 64 + pl;
+  double a0 = 0.0, a1 = 0.0;
+  if (p < nT) {
+    int q = g;
+    for (; q + kNdWaves < nch; q += 2 * kNdWaves) {
+      a0 += static_cast<double>(slots[soff + (int64_t)q * nT + p]);
+      a1 += static_cast<double>(slots[soff + (int64_t)(q + kNdWaves) * nT + p]);
+    }
+    if (q < nch) a0 += static_cast<double>(slots[soff + (int64_t)q * nT + p]);
+  }
+  part[g][pl] = a0 + a1;
+  __syncthreads();
+  if (g == 0 && p < nT) {
+    const double c = (part[0][pl] + part[1][pl]) + (part[2][pl] + part[3][pl]);
+    ftop[p] = static_cast<T>(static_cast<double>(b[perm[j0 + p]]) + c);
+  }
 }
 
 // Top front solution x (from the dense inverse): out[perm[j0 + p]] = x[p], and the
@@ -233,7 +250,7 @@ extern "C" int dds_nd_top_assemble(int dtype, int64_t j0, int64_t n_top, int64_t
   if (n_top == 0) return DDS_OK;
   DDS_REQUIRE(perm && b && ftop && (slots || n_children == 0), "dds_nd_top_assemble: null pointer");
   hipStream_t s = as_stream(stream);
-  const dim3 grid((unsigned)ceil_div(n_top, kNdThreads));
+  const dim3 grid((unsigned)ceil_div(n_top, 64));
   if (dtype == DDS_F64) {
     hipLaunchKernelGGL(nd_top_assemble_kernel<double>, grid, dim3(kNdThreads), 0, s, j0, n_top,
                        soff, (int)n_children, perm, static_cast<const double*>(b),

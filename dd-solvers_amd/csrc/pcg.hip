@@ -184,6 +184,53 @@ __global__ __launch_bounds__(kBlock) void restrict_kernel(const int64_t* __restr
   if (threadIdx.x == 0) xc[blockIdx.x] = static_cast<T>(acc);
 }
 
+// Small cells (the reference's DG coarse spaces: 3-48 unknowns per cell): G lanes per cell,
+// grid-stride over cells; lane t of a group adds t, t + G, ... then a butterfly.
+template <typename T, int G>
+__global__ __launch_bounds__(kBlock) void restrict_group_kernel(int64_t n_c,
+                                                                const int64_t* __restrict__ coarse_ptr,
+                                                                const T* __restrict__ x,
+                                                                T* __restrict__ xc,
+                                                                const dds_pcg_state* __restrict__ gate) {
+  if (gated(gate)) return;
+  const int lg = threadIdx.x & (G - 1);
+  const int64_t n_groups = (int64_t)gridDim.x * (kBlock / G);
+  for (int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G; c < n_c; c += n_groups) {
+    const int64_t s = coarse_ptr[c];
+    const int64_t e = coarse_ptr[c + 1];
+    double acc = 0.0;
+    for (int64_t j = s + lg; j < e; j += G) acc += static_cast<double>(x[j]);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+    if (lg == 0) xc[c] = static_cast<T>(acc);
+  }
+}
+
+template <typename T>
+int launch_restrict(int64_t n_c, const int64_t* coarse_ptr, const void* x, void* xc, int lanes,
+                    const dds_pcg_state* gate, hipStream_t s) {
+  const T* xv = static_cast<const T*>(x);
+  T* xcv = static_cast<T*>(xc);
+  if (lanes == 0) {
+    hipLaunchKernelGGL(restrict_kernel<T>, dim3((unsigned)n_c), dim3(kBlock), 0, s, coarse_ptr,
+                       xv, xcv, gate);
+  } else {
+    const int64_t grid = std::min<int64_t>(ceil_div(n_c * lanes, kBlock), 1 << 20);
+#define DDS_RESTRICT_G(G)                                                                       \
+  hipLaunchKernelGGL((restrict_group_kernel<T, G>), dim3((unsigned)grid), dim3(kBlock), 0, s, \
+                     n_c, coarse_ptr, xv, xcv, gate)
+    switch (lanes) {
+      case 8: DDS_RESTRICT_G(8); break;
+      case 16: DDS_RESTRICT_G(16); break;
+      case 32: DDS_RESTRICT_G(32); break;
+      default: DDS_RESTRICT_G(64); break;
+    }
+#undef DDS_RESTRICT_G
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
 template <typename S, typename D>
 __global__ __launch_bounds__(kBlock) void cast_kernel(int64_t n, const S* __restrict__ src,
                                                       D* __restrict__ dst,
@@ -300,23 +347,17 @@ extern "C" int dds_overlap_sum(int dtype, int64_t n, const int64_t* dof_ptr,
 }
 
 extern "C" int dds_restrict(int dtype, int64_t n_c, const int64_t* coarse_ptr, const void* x,
-                            void* xc, const dds_pcg_state* gate, void* stream) {
+                            void* xc, int lanes, const dds_pcg_state* gate, void* stream) {
   DDS_REQUIRE(n_c >= 0 && n_c <= INT32_MAX, "dds_restrict: bad size");
+  DDS_REQUIRE(lanes == 0 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64,
+              "dds_restrict: lanes must be 0, 8, 16, 32 or 64 (got %d)", lanes);
   if (n_c == 0) return DDS_OK;
   DDS_REQUIRE(coarse_ptr && x && xc, "dds_restrict: null pointer");
   hipStream_t s = as_stream(stream);
-  if (dtype == DDS_F64) {
-    hipLaunchKernelGGL(restrict_kernel<double>, dim3((unsigned)n_c), dim3(kBlock), 0, s,
-                       coarse_ptr, static_cast<const double*>(x), static_cast<double*>(xc), gate);
-  } else if (dtype == DDS_F32) {
-    hipLaunchKernelGGL(restrict_kernel<float>, dim3((unsigned)n_c), dim3(kBlock), 0, s,
-                       coarse_ptr, static_cast<const float*>(x), static_cast<float*>(xc), gate);
-  } else {
-    set_error("dds_restrict: unsupported dtype %d", dtype);
-    return DDS_EUNSUPPORTED;
-  }
-  DDS_LAUNCHED();
-  return DDS_OK;
+  if (dtype == DDS_F64) return launch_restrict<double>(n_c, coarse_ptr, x, xc, lanes, gate, s);
+  if (dtype == DDS_F32) return launch_restrict<float>(n_c, coarse_ptr, x, xc, lanes, gate, s);
+  set_error("dds_restrict: unsupported dtype %d", dtype);
+  return DDS_EUNSUPPORTED;
 }
 
 extern "C" int dds_cast(int src_dtype, int dst_dtype, int64_t n, const void* src, void* dst,

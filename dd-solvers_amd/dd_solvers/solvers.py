@@ -1128,6 +1128,11 @@ class SchwarzOperator(SparseSolver):
         self.solvers_per_coarse_scan = torch.as_tensor(scan.astype(np.int32), device=self.device)
         self.dofs_per_solver = int(dofs_per_solver) if decomposition is None else None
         coarse_ptr = owner_ptr[scan]
+        # lanes per coarse cell of the restriction R_C x: a workgroup for large cells, a
+        # group of 8-64 lanes for the reference's small DG cells
+        avg = n / max(self.n_coarse, 1)
+        self.restrict_lanes = 0 if avg > 256 else 64 if avg > 32 else 32 if avg > 16 else \
+            16 if avg > 8 else 8
         sizes = np.diff(sub_ptr)
         self._host = dict(owner_ptr=owner_ptr, sub_ptr=sub_ptr, sub_idx=sub_idx,
                           coarse_ptr=coarse_ptr, sizes=sizes)
@@ -1632,7 +1637,8 @@ class AdditiveSchwarz(SchwarzOperator):
         Y = z if self.Y is None else self.Y
         self._enqueue_local(x, Y, gate)
         _lib.call("dds_restrict", _lib.dtype_code(self.pi), self.n_coarse,
-                  self.coarse_ptr.data_ptr(), x.data_ptr(), xc.data_ptr(), _lib.ptr(gate), s)
+                  self.coarse_ptr.data_ptr(), x.data_ptr(), xc.data_ptr(), self.restrict_lanes,
+                  _lib.ptr(gate), s)
         self._coarse_enqueue(xc, yc, gate)
         return self._enqueue_overlap_sum(Y, z, yc, 1.0, r, partials, gate)
 
@@ -1684,7 +1690,8 @@ class HybridSchwarz(SchwarzOperator):
             xc, yc, _ = self._buffers()
             s = _lib.stream()
             _lib.call("dds_restrict", _lib.dtype_code(self.pi), self.n_coarse,
-                      self.coarse_ptr.data_ptr(), x.data_ptr(), xc.data_ptr(), None, s)
+                      self.coarse_ptr.data_ptr(), x.data_ptr(), xc.data_ptr(), self.restrict_lanes,
+                      None, s)
             self._coarse_enqueue(xc, yc, None)
             zero = torch.zeros_like(x)
             y = torch.empty_like(x)

@@ -195,9 +195,10 @@ int dds_overlap_sum(int dtype, int64_t n, const int64_t* dof_ptr, const int32_t*
                     void* z, const double* r, double* dot_partials, int* n_partials,
                     const dds_pcg_state* gate, void* stream);
 
-/* xc[c] = sum_{j in [coarse_ptr[c], coarse_ptr[c+1])} x[j], one workgroup per cell. */
+/* xc[c] = sum_{j in [coarse_ptr[c], coarse_ptr[c+1])} x[j]: lanes = 0, one workgroup per
+ * cell (large cells); lanes = 8, 16, 32 or 64, that many lanes per cell (small cells). */
 int dds_restrict(int dtype, int64_t n_c, const int64_t* coarse_ptr, const void* x, void* xc,
-                 const dds_pcg_state* gate, void* stream);
+                 int lanes, const dds_pcg_state* gate, void* stream);
 
 /* ---- dense coarse solver ---------------------------------------------------------------
  * In-place Cholesky A = L L^T of an n x n row-major SPD matrix (lower triangle read and

@@ -11,6 +11,7 @@ from oracle import kernels as K
 from oracle.schwarz import SchwarzOracle
 
 import dd_solvers as dds
+from dd_solvers import _lib
 from dd_solvers import problems as P
 from dd_solvers.bsr import FastBSR
 from dd_solvers.csr import csr_to_bsr
@@ -364,3 +365,24 @@ def test_sell_mixed_compact_and_wide_slices():
     ref = A @ x.cpu().numpy()
     assert np.allclose(y.cpu().numpy(), ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max())
     assert np.isclose(partials[:cnt].sum().item(), float(ref @ x.cpu().numpy()), rtol=1e-12)
+
+
+@pytest.mark.parametrize("lanes", [0, 8, 16, 32, 64])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_restrict_all_lane_counts(lanes, dtype):
+    """R_C x: per-cell sums over cells of 0 to 700 unknowns (empty cells included), every
+    launch shape, against numpy."""
+    rng = np.random.default_rng(lanes)
+    sizes = np.concatenate([rng.integers(0, 12, 3000), rng.integers(40, 700, 50), [0, 1, 3]])
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    x = rng.standard_normal(int(ptr[-1]))
+    xt = torch.as_tensor(x, device=dev).to(dtype)
+    xc = torch.full((sizes.shape[0],), 7.0, dtype=dtype, device=dev)
+    _lib.call("dds_restrict", _lib.dtype_code(dtype), sizes.shape[0],
+              torch.as_tensor(ptr, device=dev).data_ptr(), xt.data_ptr(), xc.data_ptr(), lanes,
+              None, _lib.stream())
+    ref = np.array([x.astype(np.float32 if dtype == torch.float32 else np.float64)[a:b].sum()
+                    for a, b in zip(ptr[:-1], ptr[1:])])
+    got = xc.double().cpu().numpy()
+    tol = 1e-12 if dtype == torch.float64 else 1e-4
+    assert np.allclose(got, ref, rtol=tol, atol=tol)

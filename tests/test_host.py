@@ -82,6 +82,8 @@ def test_host_validation_without_gpu():
         _lib.call("dds_mfma_pack_x", _lib.F32, 4, 1, None, 1, 1, 1, 1, None, None)
     with pytest.raises(_lib.HipError, match="bad count"):
         _lib.call("dds_pcg_finalize_m", 0, 1, _lib.MAX_PARTIALS + 1, 1, 1, 1, 1, None)
+    with pytest.raises(_lib.HipError, match="lanes must be"):
+        _lib.call("dds_restrict", _lib.F64, 4, 1, 1, 1, 4, None, None)
     with pytest.raises(_lib.HipError, match="bad grid size"):
         _lib.call("dds_q1_element_grad", 1, 1, 1, 1, None, 1, None)
 
