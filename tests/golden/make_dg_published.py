@@ -41,6 +41,8 @@ def main():
     runs = {}
     solves = []
     with open(os.path.join(REF, "experiment_precisions.csv")) as fh:
+        columns = next(csv.reader(fh))
+    with open(os.path.join(REF, "experiment_precisions.csv")) as fh:
         for r in csv.DictReader(fh):
             s = r["solver"]
             if s.startswith("CG(") and "Schwarz(" in s:
@@ -68,7 +70,8 @@ def main():
     with open(OUT, "w") as fh:
         json.dump({"source": "mgr0dzicki/dd-solvers results/experiment_amgx_pcg.csv and "
                              "results/experiment_precisions.csv (rtol 1e-9, FP64 runs)",
-                   "levels": levels, "runs": runs, "solves": solves}, fh, indent=1)
+                   "columns": columns, "levels": levels, "runs": runs, "solves": solves}, fh,
+                  indent=1)
     print(f"wrote {OUT}: {len(levels)} levels, {len(runs)} runs, {len(solves)} solves")
 
 
