@@ -619,7 +619,8 @@ def main():
                 M._enqueue_sym_apply(xv, None)
             for _ in range(20):
                 M._enqueue_sym_combine(Y, None)
-            alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + M.Yp.numel() * 8)
+            from dd_solvers.solvers import _sym_written_rows
+            alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + _sym_written_rows(sizes) * 8)
         q = torch.empty_like(xv)
         for _ in range(20):
             solver.matrix._enqueue_matvec(xv, q, None, None)
@@ -659,8 +660,10 @@ def main():
         # and writes the subdomain-ordered result
         t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None))
         t_comb = kernel_time_ms(lambda: M._enqueue_sym_combine(Y, None))
-        alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + yp.numel() * 8)
-        comb_bytes = int(yp.numel() * 8 + sizes.sum() * 8)
+        from dd_solvers.solvers import _sym_written_rows
+        yp_rows = _sym_written_rows(sizes)   # partial-vector entries written (and combined)
+        alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + yp_rows * 8)
+        comb_bytes = int(yp_rows * 8 + sizes.sum() * 8)
         kname = f"sym_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
     else:
         t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))

@@ -1015,6 +1015,16 @@ def _sym_plan(sizes: np.ndarray):
     return items[np.argsort(-work, kind="stable")], ypos
 
 
+def _sym_written_rows(sizes: np.ndarray) -> int:
+    """Entries of the partial vectors that the symmetric apply writes (and its combine
+    reads): item p writes the valid rows of blocks 0 .. T-1-p."""
+    total = 0
+    for k, cnt in zip(*np.unique(np.asarray(sizes, dtype=np.int64), return_counts=True)):
+        k, T = int(k), -(-int(k) // 64)
+        total += int(cnt) * sum(min(64 * (T - p), k) for p in range(-(-T // 2)))
+    return total
+
+
 _MFMA_INDEX_CACHE: dict = {}
 
 
