@@ -56,7 +56,7 @@ def build_problem(N: int, seed: int):
     return pb
 
 
-def make_solver(pb, storage, dev, bsr=None, scaled=False, use_graph=False):
+def make_solver(pb, storage, dev, bsr=None, scaled=False, use_graph=True):
     import torch
     import dd_solvers as dds
 
@@ -566,7 +566,9 @@ def main():
     ap.add_argument("--reference-only", default=None, metavar="RUNS",
                     help="only the reference's DG runs named here (comma-separated, e.g. "
                          "p1_S10_S8_S8), printed as one JSON line")
-    ap.add_argument("--graph", action="store_true", help="replay captured iteration blocks")
+    ap.add_argument("--eager", action="store_true",
+                    help="enqueue every iteration's kernels from Python instead of replaying a "
+                         "captured hipGraph of each block of iterations")
     ap.add_argument("--profile-kernel", action="store_true",
                     help="after setup only launch the local apply, the SpMV and a 1 GiB device "
                          "copy, 20 times each (for rocprofv3 --pmc byte counters)")
@@ -638,7 +640,7 @@ def main():
         return
 
     # ---- plain FP64 (the measured line) ------------------------------------------------
-    solver = make_solver(pb, torch.float64, dev, use_graph=args.graph)
+    solver = make_solver(pb, torch.float64, dev, use_graph=not args.eager)
     x, meta, total, times = time_solves(solver, b, args.steps, args.warmup)
     iters = meta["iterations"]
     tts = total / args.steps
@@ -694,7 +696,7 @@ def main():
         for name, st, scaled in (("fp32", torch.float32, False), ("fp16", torch.float16, True),
                                  ("bf16", torch.bfloat16, False)):
             try:
-                s2 = make_solver(pb, st, dev, scaled=scaled, use_graph=args.graph)
+                s2 = make_solver(pb, st, dev, scaled=scaled, use_graph=not args.eager)
                 x2, m2, tot2, t2 = time_solves(s2, b, args.steps, args.warmup)
                 e2 = float(np.linalg.norm(x2.cpu().numpy() - xh) / np.linalg.norm(xh))
                 r2 = float(np.linalg.norm(pb.b - pb.A @ x2.cpu().numpy()) / np.linalg.norm(pb.b))
