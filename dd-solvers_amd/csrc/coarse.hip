@@ -206,20 +206,48 @@ __global__ __launch_bounds__(kSolveThreads) void chol_solve_kernel(int64_t n, co
   for (int64_t i = threadIdx.x; i < n; i += kSolveThreads) yb[i] = sol[i];
 }
 
-// y = S x, S dense symmetric row-major: one wavefront per row.
+// y = S x, S dense symmetric row-major: one wavefront per row. When the row and x start on
+// 16-byte boundaries each lane reads whole 16-byte chunks of its row (chunks c and c + 64
+// in flight together) and the last n % V elements one per lane; lanes' sums are added by
+// a butterfly.
 template <typename T>
 __global__ __launch_bounds__(256) void dense_symv_kernel(int64_t n, const T* __restrict__ S,
                                                          int64_t lds, const T* __restrict__ x,
                                                          T* __restrict__ y,
                                                          const dds_pcg_state* __restrict__ gate) {
   if (gate != nullptr && gate->done) return;
+  constexpr int V = Vec16<T>::N;
   const int lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n) return;
   const T* Sr = S + row * lds;
-  T acc = 0;
-  for (int64_t j = lane; j < n; j += 64) acc += Sr[j] * x[j];
-  acc = wave_sum(acc);
+  T a0 = 0, a1 = 0;
+  if ((reinterpret_cast<uintptr_t>(Sr) & 15) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+    const int64_t nchunk = n / V;
+    int64_t c = lane;
+    for (; c + 64 < nchunk; c += 128) {
+      const Vec16<T> s0 = load16_nt(Sr + c * V);
+      const Vec16<T> s1 = load16_nt(Sr + (c + 64) * V);
+      const Vec16<T> x0 = load16(x + c * V);
+      const Vec16<T> x1 = load16(x + (c + 64) * V);
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        a0 += s0.v[e] * x0.v[e];
+        a1 += s1.v[e] * x1.v[e];
+      }
+    }
+    if (c < nchunk) {
+      const Vec16<T> s0 = load16_nt(Sr + c * V);
+      const Vec16<T> x0 = load16(x + c * V);
+#pragma unroll
+      for (int e = 0; e < V; ++e) a0 += s0.v[e] * x0.v[e];
+    }
+    const int64_t t = nchunk * V + lane;
+    if (t < n) a0 += Sr[t] * x[t];
+  } else {
+    for (int64_t j = lane; j < n; j += 64) a0 += Sr[j] * x[j];
+  }
+  const T acc = wave_sum(a0 + a1);
   if (lane == 0) y[row] = acc;
 }
 

@@ -71,8 +71,9 @@ __global__ __launch_bounds__(kNdThreads) void nd_forward_kernel(
     slots[wbase + dst] = static_cast<T>(fs - acc);
 }
 
-// Backward item (12 x int64): j0, nJ, nS, foff, xoff, c0, nc, k0, k1. Lane l takes column
-// c = c0 + l (nc <= 64); wave w sums the rows s of its quarter of [0, nS):
+// Backward item (12 x int64): j0, nJ, nS, wtoff, xoff, c0, nc, k0, k1, with W row-major at
+// fac + wtoff. Lane l takes column c = c0 + l (nc <= 64), so a wavefront reads consecutive
+// columns of a row of W; wave w sums the rows s of its quarter of [0, nS):
 //   x_J[c] = g[j0 + c] - sum_s W[s][c] x_S[s]   (x_S = xs[xoff .. xoff+nS), quarters added
 //   in wave order), written to out[perm[j0 + c]]; then the push entries [k0, k1) copy
 //   x_J (sources in [c0, c0+nc)) or x_S (sources >= nJ) into the children's buffers.
@@ -87,9 +88,8 @@ __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
   __shared__ double xj[64];
   if (gate != nullptr && gate->done) return;
   const int64_t* it = items + 12 * (int64_t)blockIdx.x;
-  const int64_t j0 = it[0], foff = it[3], xoff = it[4], k0 = it[7], k1 = it[8];
+  const int64_t j0 = it[0], wtoff = it[3], xoff = it[4], k0 = it[7], k1 = it[8];
   const int nJ = (int)it[1], nS = (int)it[2], c0 = (int)it[5], nc = (int)it[6];
-  const int m = nJ + nS;
   for (int s = threadIdx.x; s < nS; s += kNdThreads) xsl[s] = static_cast<double>(xs[xoff + s]);
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -99,15 +99,17 @@ __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
   const int s1 = min(nS, s0 + q);
   double acc = 0.0;
   if (lane < nc) {
-    const T* W = fac + foff + (int64_t)(c0 + lane) * m + nJ;
-    double a0 = 0.0, a1 = 0.0;
+    const T* W = fac + wtoff + c0 + lane;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int s = s0;
-    for (; s + 1 < s1; s += 2) {
-      a0 += static_cast<double>(load_nt(W + s)) * xsl[s];
-      a1 += static_cast<double>(load_nt(W + s + 1)) * xsl[s + 1];
+    for (; s + 3 < s1; s += 4) {
+      a0 += static_cast<double>(load_nt(W + (int64_t)s * nJ)) * xsl[s];
+      a1 += static_cast<double>(load_nt(W + (int64_t)(s + 1) * nJ)) * xsl[s + 1];
+      a2 += static_cast<double>(load_nt(W + (int64_t)(s + 2) * nJ)) * xsl[s + 2];
+      a3 += static_cast<double>(load_nt(W + (int64_t)(s + 3) * nJ)) * xsl[s + 3];
     }
-    if (s < s1) a0 += static_cast<double>(load_nt(W + s)) * xsl[s];
-    acc = a0 + a1;
+    for (; s < s1; ++s) a0 += static_cast<double>(load_nt(W + (int64_t)s * nJ)) * xsl[s];
+    acc = (a0 + a1) + (a2 + a3);
   }
   red[w][lane] = acc;
   __syncthreads();

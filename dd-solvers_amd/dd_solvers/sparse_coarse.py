@@ -155,7 +155,8 @@ class NDPlan:
     unknowns [fj0[t], fj0[t] + fnj[t]) of the nested-dissection numbering (perm[i] is the
     original index of unknown i) and couples to the fns[t] unknowns S[t] of its ancestors.
     Its stacked factor [Finv (nJ x nJ); W (nS x nJ)] is stored column-major at
-    fac[foff[t]:], element (r, c) at foff + c * m + r with m = nJ + nS (the top front, nS = 0,
+    fac[foff[t]:], element (r, c) at foff + c * m + r with m = nJ + nS, and W again row-major
+    at fac[wtoff[t]:] for the backward sweep (the top front, nS = 0,
     is its symmetric dense inverse). Between levels data moves by pushes into fixed
     buffers, so no kernel follows an index chain: forward, a child writes its update
     vector into its own slot of its parent's front (slots[wbase[c] + maps[moff[c] + i]]);
@@ -264,7 +265,10 @@ class NDPlan:
         m = fnj + fns
         self.m = m
         self.foff = np.concatenate([[0], np.cumsum(m * fnj)]).astype(np.int64)
-        self.fac_size = int(self.foff[-1])
+        # W once more, row-major (element (s, c) at wtoff + s*nJ + c), for the backward sweep:
+        # its lanes take consecutive columns, so this copy makes their loads coalesced
+        self.wtoff = int(self.foff[-1]) + np.concatenate([[0], np.cumsum(fns * fnj)]).astype(np.int64)
+        self.fac_size = int(self.wtoff[-1])
         # child maps: positions of each child's coupling set in its parent's front
         cmap: List[np.ndarray] = [None] * F
         parent = np.full(F, -1, dtype=np.int64)
@@ -320,7 +324,7 @@ class NDPlan:
 
         Forward record (12 x int64): j0, nJ, nS, foff, soff, n_children, wbase, moff, r0,
         nr, 0, 0 -- rows r0 .. r0+nr-1 (nr <= 256, one per thread) of the front's stacked
-        factor. Backward record: j0, nJ, nS, foff, xoff, c0, nc, k0, k1, 0, 0, 0 -- columns
+        factor. Backward record: j0, nJ, nS, wtoff, xoff, c0, nc, k0, k1, 0, 0, 0 -- columns
         c0 .. c0+nc-1 (nc <= 64) and the push-list entries [k0, k1) whose sources are those
         columns (the last item of a front also pushes the coupling-set values)."""
         self.fwd, self.bwd = [], []
@@ -340,7 +344,8 @@ class NDPlan:
                     nc = min(64, nj - c0)
                     k0 = k_lo + int(np.searchsorted(src, c0))
                     k1 = k_hi if c0 + nc == nj else k_lo + int(np.searchsorted(src, c0 + nc))
-                    bi.append(head + [int(self.xoff[t]), c0, nc, k0, k1, 0, 0, 0])
+                    bi.append([j0, nj, ns, int(self.wtoff[t]), int(self.xoff[t]), c0, nc, k0, k1,
+                               0, 0, 0])
             lo, hi = self.level_ptr[h], self.level_ptr[h + 1]
             lf = int(self.fnj[lo:hi].max())
             lb = int(self.fns[lo:hi].max())
@@ -373,9 +378,9 @@ class NDPlan:
             assert np.all((wb >= 0) | (ns == 0)) and np.all(mo + ns <= self.maps.shape[0])
             assert lds <= LDS_CAP and np.all(nj <= lds)
         for items, lds in self.bwd:
-            j0, nj, ns, foff, xo, c0, nc, k0, k1 = (items[:, k] for k in range(9))
+            j0, nj, ns, wt, xo, c0, nc, k0, k1 = (items[:, k] for k in range(9))
             assert np.all((nc >= 1) & (nc <= 64)) and np.all(c0 + nc <= nj)
-            assert np.all(xo + ns <= max(self.xs_size, 1)) and np.all(foff + (nj + ns) * nj <= self.fac_size)
+            assert np.all(xo + ns <= max(self.xs_size, 1)) and np.all(wt + ns * nj <= self.fac_size)
             assert np.all((0 <= k0) & (k0 <= k1) & (k1 <= self.push_src.shape[0]))
             for it in items:  # push sources: own columns, or the coupling set (last item)
                 src = self.push_src[it[7]:it[8]]
@@ -437,8 +442,8 @@ def _level_indices(plan: NDPlan, h: int):
         pF.append(plan.foff[t] + cc * m + rr)
         sF.append(b * Jm * Jm + rr * Jm + cc)
         rr, cc = np.divmod(np.arange(s * J), J)
-        pW.append(plan.foff[t] + cc * m + J + rr)
-        sW.append(b * Sm * Jm + rr * Jm + cc)
+        pW.append(np.concatenate([plan.foff[t] + cc * m + J + rr, plan.wtoff[t] + rr * J + cc]))
+        sW.append(np.tile(b * Sm * Jm + rr * Jm + cc, 2))
         rr, cc = np.divmod(np.arange(s * s), max(s, 1))
         pUo.append(plan.uoff2[t] + np.arange(s * s))
         sUo.append(b * Sm * Sm + rr * Sm + cc)
@@ -515,11 +520,10 @@ def solve_host(plan: NDPlan, fac: np.ndarray, b: np.ndarray) -> np.ndarray:
         xs[plan.push_dst[k0:k1]] = xt[plan.push_src[k0:k1]]
     for items, _ in reversed(plan.bwd):              # dds_nd_backward, top level first
         for it in items:
-            j0, nj, ns, foff, xo, c0, nc, k0, k1 = (int(v) for v in it[:9])
-            m = nj + ns
-            M = fac[foff:foff + m * nj].reshape(nj, m).T
+            j0, nj, ns, wt, xo, c0, nc, k0, k1 = (int(v) for v in it[:9])
+            W = fac[wt:wt + ns * nj].reshape(ns, nj)       # row-major copy of W
             xsl = xs[xo:xo + ns].copy()
-            xj = g[j0 + c0:j0 + c0 + nc] - M[nj:, c0:c0 + nc].T @ xsl
+            xj = g[j0 + c0:j0 + c0 + nc] - W[:, c0:c0 + nc].T @ xsl
             out[plan.perm[j0 + c0:j0 + c0 + nc]] = xj
             src = plan.push_src[k0:k1]
             val = np.where(src < nj, xj[np.clip(src - c0, 0, nc - 1)],
@@ -628,6 +632,7 @@ class SparseCholesky(SparseSolver):
     def bytes_per_solve(self) -> int:
         e = 8 if self.dtype == torch.float64 else 4
         p = self.plan
+        # forward: the stacked factor; backward: the row-major copy of W
         return e * int(np.sum(p.fnj * p.fnj + 2 * p.fns * p.fnj)) + 4 * e * p.n + \
             4 * e * p.xs_size
 

@@ -237,8 +237,9 @@ int dds_dense_symv(int dtype, int64_t n, const void* s, int64_t lds, const void*
 int dds_nd_forward(int dtype, int64_t n_items, const int64_t* items, const int32_t* perm,
                    const int32_t* maps, const void* fac, const void* b, void* g, void* slots,
                    int64_t lds_elems, const dds_pcg_state* gate, void* stream);
-/* Backward record (j0, nJ, nS, foff, xoff, c0, nc <= 64, k0, k1, -, -, -): x_J[c] = g[j0 + c]
- * - (W^T x_S)_c for c in [c0, c0+nc) with x_S = xs[xoff .. xoff+nS), out[perm[j0 + c]] =
+/* Backward record (j0, nJ, nS, wtoff, xoff, c0, nc <= 64, k0, k1, -, -, -), W row-major at
+ * fac + wtoff: x_J[c] = g[j0 + c] - (W^T x_S)_c for c in [c0, c0+nc) with
+ * x_S = xs[xoff .. xoff+nS), out[perm[j0 + c]] =
  * x_J[c]; then push entries [k0, k1) (sources in [c0, c0+nc), or >= nJ for the coupling
  * set). */
 int dds_nd_backward(int dtype, int64_t n_items, const int64_t* items, const int32_t* perm,

@@ -386,3 +386,23 @@ def test_restrict_all_lane_counts(lanes, dtype):
     got = xc.double().cpu().numpy()
     tol = 1e-12 if dtype == torch.float64 else 1e-4
     assert np.allclose(got, ref, rtol=tol, atol=tol)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 127, 128, 129, 1000, 4097])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_dense_symv_all_widths(n, dtype, offset):
+    """y = S x for a dense row-major S: rows on 16-byte boundaries (chunked loads and the
+    element tail) and rows off them (offset start, odd leading dimension)."""
+    rng = np.random.default_rng(n)
+    lda = n + offset
+    S = rng.standard_normal((n, lda))
+    x = rng.standard_normal(n)
+    St = torch.as_tensor(np.concatenate([np.zeros(offset), S.reshape(-1)]), device=dev).to(dtype)
+    xt = torch.as_tensor(x, device=dev).to(dtype)
+    yt = torch.empty(n, dtype=dtype, device=dev)
+    _lib.call("dds_dense_symv", _lib.dtype_code(dtype), n, St[offset:].data_ptr(), lda,
+              xt.data_ptr(), yt.data_ptr(), None, _lib.stream())
+    ref = S[:, :n] @ x
+    tol = 1e-12 if dtype == torch.float64 else 2e-4
+    assert np.allclose(yt.double().cpu().numpy(), ref, rtol=tol, atol=tol * np.sqrt(n))

@@ -130,3 +130,26 @@ def test_published_s10_s8_s8_runs_on_the_gpu(s10_s8_s8, kind):
     assert np.allclose(got[:10], pub[:10], rtol=1e-10, atol=0)
     assert np.max(np.abs(got - pub) / pub) < 1e-3
     assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b) * 1.01
+
+
+def test_graph_replay_matches_eager_with_sparse_coarse():
+    """CG(use_graph=True) captures blocks of iterations holding the sparse coarse sweeps;
+    two graph solves (with eager work between them) equal the eager solve bit for bit."""
+    print("DG S6 S4 S4 hybrid: graph replay vs eager, sparse coarse", flush=True)
+    pr = dg.assemble_sipg(dg.UniformTriangleMesh(6), 1)
+    perm, A, b, k, spc = dg.dg_experiment(pr, "S4", "S4")
+    out = []
+    for graph in (False, True):
+        s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(), dds.SparseCholesky(leaf_size=8, top_max=64)),
+                   rtol=1e-9, maxiter=2000, use_graph=graph)
+        s.setup(_torch_csr(A), dofs_per_solver=k, solvers_per_coarse=torch.as_tensor(spc, device=dev),
+                block_size=3)
+        bt = torch.as_tensor(b, device=dev)
+        for rep in range(2 if graph else 1):
+            x, md = s.solve(bt)
+            torch.cuda.synchronize()
+            out.append((x.cpu().numpy(), md["iterations"]))
+            _ = (bt * 2).sum().item()  # eager work between replays
+    (xe, ie), (xg1, ig1), (xg2, ig2) = out
+    assert ie == ig1 == ig2
+    assert np.array_equal(xe, xg1) and np.array_equal(xe, xg2)
