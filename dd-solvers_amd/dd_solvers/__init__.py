@@ -11,7 +11,7 @@ from .solvers import __all__ as _solvers_all
 from .bsr import FastBSR
 from .csr import csr_to_bsr, gather_csr, sort_csr
 from .gemv import gemv_strided_batched
-from .sparse_coarse import SparseCholesky
+from .sparse_coarse import BatchSparseCholesky, SparseCholesky
 
 __all__ = [*_solvers_all, "FastBSR", "csr_to_bsr", "gather_csr", "sort_csr",
-           "gemv_strided_batched", "SparseCholesky"]
+           "gemv_strided_batched", "SparseCholesky", "BatchSparseCholesky"]

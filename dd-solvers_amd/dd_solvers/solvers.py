@@ -854,6 +854,37 @@ class SparseBatchSolver(ABC):
         pass
 
 
+def local_csr_batch(sub_ptr: torch.Tensor, s: torch.Tensor, a: torch.Tensor,
+                    b: torch.Tensor, v: torch.Tensor, dtype: torch.dtype) -> CSRBatch:
+    """The local matrices A_i as a CSRBatch, the layout of the reference's
+    `_construct_local_solvers_matrices_sparse` (solvers.py:660-719, kernels :484-538).
+
+    Entry e of the matrix lies in subdomain s[e] at local row a[e], local column b[e].
+    Per subdomain i: nrows[i] = k_i (int32), nnz[i] (int64); the values (in `dtype`) and
+    int32 local columns of its rows follow one another, subdomain by subdomain, with the
+    columns of a row ascending; row_offsets holds k_i + 1 int32 offsets per subdomain,
+    counted from the subdomain's first entry (sum(k_i + 1) in all; n + n_s for the
+    reference's equal subdomains). Works on any device (torch only)."""
+    dev = sub_ptr.device
+    sizes = sub_ptr[1:] - sub_ptr[:-1]
+    n_s = sizes.shape[0]
+    nk = int(sub_ptr[-1])
+    k_max = int(sizes.max()) if n_s else 0
+    row = sub_ptr[s] + a  # position in the stacked rows of all subdomains
+    order = torch.sort(row * max(k_max, 1) + b, stable=True).indices
+    counts = torch.bincount(row, minlength=nk)
+    G = torch.zeros(nk + 1, dtype=torch.int64, device=dev)
+    G[1:] = torch.cumsum(counts, 0)
+    starts = torch.cumsum(sizes + 1, 0) - (sizes + 1)
+    q = torch.arange(int(sizes.sum()) + n_s, device=dev)
+    sid = torch.repeat_interleave(torch.arange(n_s, device=dev), sizes + 1)
+    j = q - starts[sid]
+    row_offsets = (G[sub_ptr[sid] + j] - G[sub_ptr[sid]]).to(torch.int32)
+    return CSRBatch(nrows=sizes.to(torch.int32), nnz=G[sub_ptr[1:]] - G[sub_ptr[:-1]],
+                    values=v[order].to(dtype), col_indices=b[order].to(torch.int32),
+                    row_offsets=row_offsets)
+
+
 class Inv(DenseBatchSolver):
     """Explicit local inverses applied by a batched GEMV (solvers.py:343-359).
 
@@ -1225,7 +1256,8 @@ class SchwarzOperator(SparseSolver):
         n_s = self.n_solvers
         k_max = int(sizes.max())
         ls = self.local_solver
-        if not isinstance(ls, DenseBatchSolver):
+        self._sparse_local = False
+        if not isinstance(ls, (DenseBatchSolver, SparseBatchSolver)):
             raise ValueError(f"Unknown local solver type: {type(ls)}")
         sub_ptr = self.sub_ptr
         if sub_idx is None:
@@ -1255,6 +1287,24 @@ class SchwarzOperator(SparseSolver):
             found = keys[q] == qk
             s, a, b, v = s[found], a[found], (q - sub_ptr[s])[found], val[e][found]
             del e, first, t, pos, keys, qk, q, found, cnt, mult
+        if isinstance(ls, SparseBatchSolver):
+            # the reference's sparse local path (solvers.py:610-616): the A_i as a CSRBatch.
+            # A solver with a device sweep (_enqueue_solve, BatchSparseCholesky) stays on the
+            # gated device path, for any decomposition (it gathers overlapped subdomains
+            # itself); other solvers see the reference's (n_s, k) batches of equal,
+            # contiguous subdomains.
+            device_path = hasattr(ls, "_enqueue_solve")
+            if not device_path and (sub_idx is not None or not np.all(sizes == sizes[0])):
+                raise ValueError(f"{ls} needs equal, non-overlapping subdomains")
+            ls.setup(local_csr_batch(sub_ptr, s, a, b, v, pi))
+            del s, a, b, v
+            if device_path and self.sub_idx is not None:
+                ls._gather_from(self.sub_idx)
+            self.local = None
+            self._generic_local = not device_path
+            self._sparse_local = device_path
+            collect()
+            return
         if not isinstance(ls, Inv):
             # LU / Cholesky / user solvers: their own batched setup and solve on the
             # reference's layout (equal, contiguous, non-overlapping subdomains).
@@ -1432,6 +1482,9 @@ class SchwarzOperator(SparseSolver):
                 and hasattr(self.coarse_solver, "_enqueue_solve"))
 
     def _enqueue_local(self, x, Y, gate) -> None:
+        if self._sparse_local:
+            self.local_solver._enqueue_solve(x, Y, gate)
+            return
         if self._generic_local:
             k = self.n // self.n_solvers
             Y[:self.n].copy_(self.local_solver.solve(x[:self.n].reshape(self.n_solvers, k)).reshape(-1))
@@ -1611,7 +1664,10 @@ class SchwarzOperator(SparseSolver):
         e = 8 if self.pi == torch.float64 else 4
         sizes = self._host["sizes"]
         nk = int(sizes.sum())
-        b = self.local.bytes_algorithmic + e * 2 * nk
+        if self.local is not None:
+            b = self.local.bytes_algorithmic + e * 2 * nk
+        else:  # a sparse local factor: its sweeps' bytes, vectors included
+            b = getattr(self.local_solver, "bytes_per_solve", 0)
         if self.sub_idx is not None:
             b += 4 * nk  # gather indices
         if getattr(self, "Yp", None) is not None:  # partial vectors: written, read back

@@ -47,9 +47,9 @@ import scipy.sparse.csgraph as csgraph
 import torch
 
 from . import _lib
-from .solvers import SparseSolver
+from .solvers import CSRBatch, SparseBatchSolver, SparseSolver
 
-__all__ = ["SparseCholesky", "nested_dissection", "NDPlan"]
+__all__ = ["SparseCholesky", "BatchSparseCholesky", "nested_dissection", "NDPlan"]
 
 # LDS capacity of the sweep kernels, in elements (f_J forward, x_S backward): 60 KiB of FP64
 LDS_CAP = 7680
@@ -126,9 +126,12 @@ def nested_dissection(G: sps.csr_matrix, leaf_size: int = 32) -> List[_Node]:
         if order.shape[0] < n:  # disconnected: each piece is its own subtree
             nc, lab = csgraph.connected_components(H, directed=False)
             srt = np.argsort(lab, kind="stable")
-            cuts = np.flatnonzero(np.diff(lab[srt])) + 1
-            for sel in reversed(np.split(srt, cuts)):
-                stack.append((H[sel][:, sel], ids[sel], depth, parent))
+            cuts = np.concatenate([[0], np.flatnonzero(np.diff(lab[srt])) + 1, [n]])
+            # one symmetric permutation, then each piece is a diagonal block (a batch of
+            # thousands of subdomains costs one pass over the graph, not one per piece)
+            Hs = H[srt][:, srt].tocsr()
+            for a, e in reversed(list(zip(cuts[:-1], cuts[1:]))):
+                stack.append((Hs[a:e, a:e], ids[srt[a:e]], depth, parent))
             continue
         order, lev = _bfs_levels(H, int(order[-1]))
         if lev[-1] < 2:  # no level splits it: keep it whole
@@ -165,7 +168,10 @@ class NDPlan:
     """
 
     def __init__(self, A: sps.csr_matrix, leaf_size: int = 32, top_max: int = 4096):
-        A = sps.csr_matrix(A)
+        # stored zeros (the nearest-neighbour couplings of a Q1 stiffness matrix are exact
+        # zeros) leave the pattern: the dissection ignores them, so the fronts must too
+        A = sps.csr_matrix(A, copy=True)
+        A.eliminate_zeros()
         n = A.shape[0]
         if A.shape != (n, n):
             raise ValueError("coarse matrix must be square")
@@ -567,6 +573,10 @@ class SparseCholesky(SparseSolver):
                                 matrix.crow_indices().cpu().numpy()), shape=matrix.shape)
         else:
             A = sps.csr_matrix(matrix.cpu().double().numpy())
+        self._setup_host(A, dev)
+
+    def _setup_host(self, A: sps.csr_matrix, dev) -> None:
+        """Plan on the host, factor and tables on `dev` (self.dtype set by the caller)."""
         self.n = A.shape[0]
         plan = NDPlan(A, self.leaf_size, self.top_max)
         plan.validate()
@@ -577,6 +587,8 @@ class SparseCholesky(SparseSolver):
                                                           device=dev)
             self._maps = I(plan.maps, torch.int32)
             self._perm = I(plan.perm, torch.int32)
+            # where the forward sweep reads b (perm, or a gather composed with it)
+            self._perm_in = self._perm
             self._push_dst = I(plan.push_dst)
             self._push_src = I(plan.push_src, torch.int32)
             self._fwd = [(I(it.reshape(-1)), it.shape[0], lds) for it, lds in plan.fwd]
@@ -605,14 +617,14 @@ class SparseCholesky(SparseSolver):
         g = _lib.ptr(gate)
         p = self.plan
         for items, n_items, lds in self._fwd:
-            _lib.call("dds_nd_forward", dt, n_items, items.data_ptr(), self._perm.data_ptr(),
+            _lib.call("dds_nd_forward", dt, n_items, items.data_ptr(), self._perm_in.data_ptr(),
                       self._maps.data_ptr(), self.fac.data_ptr(), x.data_ptr(),
                       self._g.data_ptr(), self._slots.data_ptr(), lds, g, s)
         if p.has_top:
             t = p.n_fronts - 1
             j0, nT = int(p.fj0[t]), p.nT
             _lib.call("dds_nd_top_assemble", dt, j0, nT, int(p.soff[t]), len(p.children[t]),
-                      self._perm.data_ptr(), x.data_ptr(), self._slots.data_ptr(),
+                      self._perm_in.data_ptr(), x.data_ptr(), self._slots.data_ptr(),
                       self._ftop.data_ptr(), g, s)
             _lib.call("dds_dense_symv", dt, nT, self.fac[int(p.foff[t]):].data_ptr(), nT,
                       self._ftop.data_ptr(), self._xtop.data_ptr(), g, s)
@@ -639,3 +651,87 @@ class SparseCholesky(SparseSolver):
     def destroy(self):
         self.plan = None
         self.fac = None
+
+
+# ---------------------------------------------------------------------------------------
+# Local-solver slot: a batch of sparse SPD matrices
+# ---------------------------------------------------------------------------------------
+
+def csr_batch_to_host(batch: CSRBatch) -> sps.csr_matrix:
+    """The block-diagonal matrix diag(A_0, ..., A_{n_s-1}) of a CSRBatch (FP64, host)."""
+    nrows = batch.nrows.cpu().numpy().astype(np.int64)
+    nnz = batch.nnz.cpu().numpy().astype(np.int64)
+    ro = batch.row_offsets.cpu().numpy().astype(np.int64)
+    cols = batch.col_indices.cpu().numpy().astype(np.int64)
+    vals = batch.values.cpu().double().numpy()
+    n_s = nrows.shape[0]
+    if ro.shape[0] != int(nrows.sum()) + n_s or cols.shape[0] != int(nnz.sum()):
+        raise ValueError("inconsistent CSRBatch sizes")
+    rbase = np.concatenate([[0], np.cumsum(nrows)])
+    ebase = np.concatenate([[0], np.cumsum(nnz)])
+    sid = np.repeat(np.arange(n_s), nrows + 1)
+    start = np.concatenate([[0], np.cumsum(nrows + 1)])[:-1]
+    first = (np.arange(ro.shape[0]) - start[sid]) == 0
+    # global row pointer: each subdomain's offsets shifted by its first entry, dropping the
+    # leading zero of every subdomain but the first
+    indptr = np.concatenate([[0], (ro + ebase[sid])[~first]])
+    col = cols + np.repeat(rbase[:-1], nnz)
+    n = int(rbase[-1])
+    return sps.csr_matrix((vals, col, indptr), shape=(n, n))
+
+
+class BatchSparseCholesky(SparseBatchSolver):
+    """Local-solver slot for sparse subdomain matrices (`SparseBatchSolver`: setup(CSRBatch),
+    solve((n_s, k)) -> flat); the counterpart of the reference's `BatchCUDSS`
+    (solvers.py:390-403), which factorises every A_i with cuDSS's batch API.
+
+    The batch is factorised as the one block-diagonal matrix diag(A_i): nested dissection
+    of each subdomain's graph (its pieces are sibling subtrees, no top front), block LDL^T
+    by height across all subdomains at once, and one forward and one backward sweep launch
+    per height for the whole batch. The factor is stored in the dtype of the batch values
+    (the preconditioner precision). Inside the Schwarz preconditioners the sweeps are gated
+    device launches (graph replay works), and overlapping or uneven subdomains are
+    supported: the forward sweep reads the right-hand side through the subdomain index
+    lists, so no gathered copy is made.
+    """
+
+    def __init__(self, leaf_size: int = 64):
+        self.leaf_size = leaf_size
+        self._inner = None
+
+    def __str__(self):
+        return "BatchSparseCholesky"
+
+    def setup(self, matrix_batch: CSRBatch):
+        vals = matrix_batch.values
+        _lib.require_device(vals)
+        if vals.dtype not in (torch.float64, torch.float32):
+            raise ValueError(f"unsupported local dtype {vals.dtype}")
+        self.n_solvers = int(matrix_batch.nrows.shape[0])
+        inner = SparseCholesky(self.leaf_size, top_max=0)
+        inner.dtype = vals.dtype
+        inner._setup_host(csr_batch_to_host(matrix_batch), vals.device)
+        self._inner = inner
+
+    def _gather_from(self, sub_idx: torch.Tensor) -> None:
+        """Read position p of the stacked right-hand side from x[sub_idx[p]]."""
+        inner = self._inner
+        inner._perm_in = sub_idx.to(torch.int64)[inner._perm.to(torch.int64)].to(torch.int32)
+
+    def solve(self, rhs_batch: torch.Tensor) -> torch.Tensor:
+        x, _ = self._inner.solve(rhs_batch.reshape(-1))
+        return x
+
+    def _enqueue_solve(self, x, y, gate) -> None:
+        self._inner._enqueue_solve(x, y, gate)
+
+    @property
+    def plan(self) -> NDPlan:
+        return self._inner.plan
+
+    @property
+    def bytes_per_solve(self) -> int:
+        return self._inner.bytes_per_solve
+
+    def destroy(self):
+        self._inner = None

@@ -7,6 +7,9 @@ Pins (tests/test_oracle.py):
   sort_csr / gather_csr known answers of src/dd_solvers/tests/test_cusparse.py:8-78
   spd_solve             known answers of src/dd_solvers/tests/test_cudss.py:15-84, and
                         scipy.linalg.cho_solve on random SPD matrices
+  local_csr_batch       scipy's slicing of the diagonal blocks A[ik:(i+1)k, ik:(i+1)k]
+                        (tests/test_sparse_coarse.py), with an entry planted in the
+                        wrong block caught
 """
 
 from __future__ import annotations
@@ -24,6 +27,7 @@ __all__ = [
     "gather_csr",
     "cholesky_lower",
     "spd_solve",
+    "local_csr_batch",
 ]
 
 
@@ -190,3 +194,49 @@ def spd_solve(A: np.ndarray, b: np.ndarray) -> np.ndarray:
     """
     c = sla.cho_factor(np.asarray(A, dtype=np.float64), lower=True)
     return sla.cho_solve(c, np.asarray(b, dtype=np.float64))
+
+
+def local_csr_batch(indptr, indices, data, k: int):
+    """The diagonal k x k blocks of a CSR matrix as the reference's CSRBatch fields
+    (nrows, nnz, values, col_indices, row_offsets).
+
+    Follows SchwarzOperator._construct_local_solvers_matrices_sparse
+    (src/dd_solvers/dd_solvers/solvers.py:660-719) and its two kernels (:484-538): a row
+    keeps its entries in the row's own subdomain, in the row's stored order, and stops at
+    the first column of a later subdomain (the rows are sorted there); row_offsets has
+    k + 1 entries per subdomain, counted from the subdomain's first entry.
+    """
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    data = np.asarray(data)
+    n = indptr.shape[0] - 1
+    n_s = n // k
+    count = np.zeros(n, dtype=np.int64)
+    for row in range(n):
+        for ind in range(indptr[row], indptr[row + 1]):
+            col = indices[ind]
+            if col // k == row // k:
+                count[row] += 1
+            elif col // k > row // k:
+                break
+    nnz = count.reshape(n_s, k).sum(axis=1)
+    scan = np.concatenate([[0], np.cumsum(count)])
+    values = np.empty(int(scan[-1]), dtype=data.dtype)
+    cols = np.empty(int(scan[-1]), dtype=np.int32)
+    row_offsets = np.empty(n + n_s, dtype=np.int32)
+    for row in range(n):
+        solver, solver_row = divmod(row, k)
+        out = scan[row]
+        first = scan[solver * k]
+        if solver_row == 0:
+            row_offsets[solver * (k + 1)] = out - first
+        for ind in range(indptr[row], indptr[row + 1]):
+            col = indices[ind]
+            if col // k == row // k:
+                cols[out] = col % k
+                values[out] = data[ind]
+                out += 1
+            elif col // k > row // k:
+                break
+        row_offsets[solver * (k + 1) + solver_row + 1] = out - first
+    return np.full(n_s, k, dtype=np.int32), nnz, values, cols, row_offsets

@@ -149,3 +149,120 @@ def test_host_validation_of_the_sweep_entry_points():
 def test_cpu_tensors_rejected():
     with pytest.raises(ValueError, match="HIP device"):
         sc.SparseCholesky().setup(torch.eye(3, dtype=torch.float64).to_sparse_csr())
+
+
+# ---------------------------------------------------------------------------------------
+# stored zeros; the local-solver slot (CSRBatch, block-diagonal plans)
+# ---------------------------------------------------------------------------------------
+
+def test_stored_zeros_leave_the_pattern():
+    """A 3D Q1 stiffness matrix stores its nearest-neighbour couplings as exact zeros, and
+    the dissection ignores stored zeros: the fronts must ignore them as well (else a child's
+    coupling set reaches into a sibling subtree). Also a planted zero linking two corners."""
+    from dd_solvers import problems as P
+    pb = P.make_problem("q1_3d", 10, box=5, overlap=0, seed=0, lognormal=True)
+    A = pb.A.tocsr()
+    assert np.count_nonzero(A.data == 0) > 0
+    _check(A, 16, 0)
+    _check(A, 16, 64)
+    C = lap2d(20, 20).tocoo()
+    L = sps.csr_matrix((np.concatenate([C.data, [0.0, 0.0]]),
+                        (np.concatenate([C.row, [0, 399]]), np.concatenate([C.col, [399, 0]]))),
+                       shape=C.shape)
+    assert L[0, 399] == 0 and 399 in L.indices[L.indptr[0]:L.indptr[1]]
+    _check(L, 8, 32)
+
+
+def _sorted_spd(n, seed):
+    rng = np.random.default_rng(seed)
+    B = sps.random(n, n, density=0.15, random_state=rng)
+    A = (B + B.T + sps.diags(np.full(n, float(n)))).tocsr()
+    A.sort_indices()
+    return A
+
+
+def test_local_csr_batch_oracle_pinned():
+    """The oracle's restatement of the reference's CSRBatch construction against scipy's
+    slicing of the diagonal blocks; a column index moved inside a block is caught."""
+    from oracle.kernels import local_csr_batch as ref
+    A, k = _sorted_spd(60, 3), 12
+    nrows, nnz, vals, cols, ro = ref(A.indptr, A.indices, A.data, k)
+    assert np.all(nrows == k) and ro.shape[0] == 60 + 5
+
+    def blocks_match(vals, cols):
+        for i in range(5):
+            e0 = int(nnz[:i].sum())
+            B = sps.csr_matrix((vals[e0:e0 + nnz[i]], cols[e0:e0 + nnz[i]],
+                                ro[i * (k + 1):(i + 1) * (k + 1)]), shape=(k, k))
+            if (B != A[i * k:(i + 1) * k, i * k:(i + 1) * k]).nnz:
+                return False
+        return True
+
+    assert blocks_match(vals, cols)
+    bad = cols.copy()
+    bad[3] = (bad[3] + 1) % k
+    assert not blocks_match(vals, bad)
+
+
+def test_local_csr_batch_matches_reference_layout():
+    """The CSRBatch the Schwarz set-up hands a sparse local solver (built from the matrix
+    entries in any order) equals the reference's layout field by field."""
+    from oracle.kernels import local_csr_batch as ref
+    from dd_solvers.solvers import local_csr_batch
+    A, k = _sorted_spd(72, 4), 8
+    C = A.tocoo()
+    r, c, v = C.row.astype(np.int64), C.col.astype(np.int64), C.data
+    keep = r // k == c // k
+    o = np.random.default_rng(0).permutation(int(keep.sum()))
+    s, a, b = (r // k)[keep][o], (r % k)[keep][o], (c % k)[keep][o]
+    T = lambda x: torch.as_tensor(x)
+    got = local_csr_batch(torch.arange(0, 73, k), T(s), T(a), T(b), T(v[keep][o]), torch.float64)
+    want = ref(A.indptr, A.indices, A.data, k)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.numpy(), w)
+    assert got.nrows.dtype == torch.int32 and got.col_indices.dtype == torch.int32
+    assert got.row_offsets.dtype == torch.int32 and got.nnz.dtype == torch.int64
+
+
+def _overlapped_blocks():
+    from dd_solvers import problems as P
+    pb = P.make_problem("p1_2d", 16, box=4, overlap=2, seed=0)
+    d, A = pb.decomposition, pb.A.tocsr()
+    subs = [A[idx][:, idx].tocsr() for idx in
+            (d.sub_idx[d.sub_ptr[i]:d.sub_ptr[i + 1]] for i in range(d.n_subdomains))]
+    return d, subs
+
+
+def test_local_csr_batch_uneven_overlapping_round_trip():
+    """Uneven, overlapping subdomains: the CSRBatch read back as a block-diagonal matrix is
+    diag(A[S_i][:, S_i])."""
+    from dd_solvers.solvers import local_csr_batch
+    d, subs = _overlapped_blocks()
+    assert len({m.shape[0] for m in subs}) > 1
+    s, a, b, v = [], [], [], []
+    for i, m in enumerate(subs):
+        C = m.tocoo()
+        s.append(np.full(C.nnz, i))
+        a.append(C.row)
+        b.append(C.col)
+        v.append(C.data)
+    T = lambda x: torch.as_tensor(np.concatenate(x).astype(np.int64))
+    batch = local_csr_batch(torch.as_tensor(d.sub_ptr), T(s), T(a), T(b),
+                            torch.as_tensor(np.concatenate(v)), torch.float64)
+    H = sc.csr_batch_to_host(batch)
+    assert (H != sps.block_diag(subs).tocsr()).nnz == 0
+    assert np.array_equal(batch.nrows.numpy(), np.diff(d.sub_ptr))
+
+
+def test_block_diagonal_plan():
+    """The plan of a batch (diag of the subdomain matrices): no top front, every front
+    inside one subdomain, and the sweeps solve each block like SuperLU."""
+    d, subs = _overlapped_blocks()
+    B = sps.block_diag(subs).tocsr()
+    plan = _check(B, 16, 0)
+    assert not plan.has_top
+    block = np.repeat(np.arange(len(subs)), [m.shape[0] for m in subs])
+    for t in range(plan.n_fronts):
+        own = block[plan.perm[plan.fj0[t]:plan.fj0[t] + plan.fnj[t]]]
+        cpl = block[plan.perm[plan.S[t]]]
+        assert np.all(own == own[0]) and np.all(cpl == own[0])
