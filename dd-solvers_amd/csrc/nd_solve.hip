@@ -23,60 +23,88 @@ constexpr int kNdWaves = kNdThreads / 64;
 static_assert(kNdWaves == 4, "the top assembly adds four wave sums");
 constexpr int64_t kNdLdsCap = 7680;  // dynamic LDS elements per workgroup: 60 KiB of FP64
 
-// Forward item (12 x int64): j0, nJ, nS, foff, soff, n_children, wbase, moff, r0, nr.
-// Thread i takes row r = r0 + i (nr <= 256) of the stacked factor:
+// Forward item (12 x int64): j0, nJ, nS, foff, soff, n_children, wbase, moff, r0, nr, R.
+// The item covers rows r0 .. r0+nr-1 (nr <= R) of the stacked factor; R (64, 128 or 256)
+// rows take the 256 threads as R lanes x 256/R column parts: thread t has row r0 + t % R
+// and part t / R, which sums a contiguous quarter, half or all of the columns (four
+// accumulators, c ascending); the parts are added in part order through LDS. Then
 //   f_J = b[perm[j0 + .]] + the children's slots (child order), staged in LDS;
-//   r <  nJ:  g[j0 + r] = sum_c Finv[r][c] f_J[c]                     (c ascending)
+//   r <  nJ:  g[j0 + r] = sum_c Finv[r][c] f_J[c]
 //   r >= nJ:  the parent's slot gets (children's slots at r) - sum_c W[r-nJ][c] f_J[c].
+// Splitting the columns shortens each thread's chain of dependent loads for the wide
+// fronts near the roots, where few workgroups are in flight.
 template <typename T>
 __global__ __launch_bounds__(kNdThreads) void nd_forward_kernel(
     const int64_t* __restrict__ items, const int32_t* __restrict__ perm,
     const int32_t* __restrict__ maps, const T* __restrict__ fac, const T* __restrict__ b,
     T* __restrict__ g, T* __restrict__ slots, const dds_pcg_state* __restrict__ gate) {
   extern __shared__ double f[];
+  __shared__ double red[kNdThreads];
   if (gate != nullptr && gate->done) return;
   const int64_t* it = items + 12 * (int64_t)blockIdx.x;
   const int64_t j0 = it[0], foff = it[3], soff = it[4], wbase = it[6], moff = it[7];
   const int nJ = (int)it[1], nS = (int)it[2], nch = (int)it[5], r0 = (int)it[8], nr = (int)it[9];
+  const int R = (int)it[10];  // 64, 128 or 256 (checked on the host)
+  const int K = kNdThreads / R;
   const int m = nJ + nS;
   for (int p = threadIdx.x; p < nJ; p += kNdThreads) {
     double v = static_cast<double>(b[perm[j0 + p]]);
     for (int q = 0; q < nch; ++q) v += static_cast<double>(slots[soff + (int64_t)q * m + p]);
     f[p] = v;
   }
-  const int r = r0 + (int)threadIdx.x;
-  const bool active = (int)threadIdx.x < nr;
+  const int lr = (int)threadIdx.x & (R - 1);
+  const int part = (int)threadIdx.x / R;
+  const int r = r0 + lr;
+  const bool active = lr < nr;
   double fs = 0.0;  // the children's update at an S row
   int32_t dst = 0;
-  if (active && r >= nJ) {
+  if (part == 0 && active && r >= nJ) {
     for (int q = 0; q < nch; ++q) fs += static_cast<double>(slots[soff + (int64_t)q * m + r]);
     dst = maps[moff + (r - nJ)];
   }
   __syncthreads();
-  if (!active) return;
-  const T* col = fac + foff + r;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int c = 0;
-  for (; c + 3 < nJ; c += 4) {
-    a0 += static_cast<double>(load_nt(col + (int64_t)c * m)) * f[c];
-    a1 += static_cast<double>(load_nt(col + (int64_t)(c + 1) * m)) * f[c + 1];
-    a2 += static_cast<double>(load_nt(col + (int64_t)(c + 2) * m)) * f[c + 2];
-    a3 += static_cast<double>(load_nt(col + (int64_t)(c + 3) * m)) * f[c + 3];
+  double acc = 0.0;
+  if (active) {
+    const int q = (nJ + K - 1) / K;
+    const int cb = part * q;
+    const int ce = min(nJ, cb + q);
+    const T* col = fac + foff + r;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int c = cb;
+    for (; c + 3 < ce; c += 4) {
+      a0 += static_cast<double>(load_nt(col + (int64_t)c * m)) * f[c];
+      a1 += static_cast<double>(load_nt(col + (int64_t)(c + 1) * m)) * f[c + 1];
+      a2 += static_cast<double>(load_nt(col + (int64_t)(c + 2) * m)) * f[c + 2];
+      a3 += static_cast<double>(load_nt(col + (int64_t)(c + 3) * m)) * f[c + 3];
+    }
+    for (; c < ce; ++c) a0 += static_cast<double>(load_nt(col + (int64_t)c * m)) * f[c];
+    acc = (a0 + a1) + (a2 + a3);
   }
-  for (; c < nJ; ++c) a0 += static_cast<double>(load_nt(col + (int64_t)c * m)) * f[c];
-  const double acc = (a0 + a1) + (a2 + a3);
+  if (K > 1) {
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (part == 0) {
+      acc = red[lr];
+      for (int k = 1; k < K; ++k) acc += red[k * R + lr];
+    }
+  }
+  if (part != 0 || !active) return;
   if (r < nJ)
     g[j0 + r] = static_cast<T>(acc);
   else
     slots[wbase + dst] = static_cast<T>(fs - acc);
 }
 
-// Backward item (12 x int64): j0, nJ, nS, wtoff, xoff, c0, nc, k0, k1, with W row-major at
-// fac + wtoff. Lane l takes column c = c0 + l (nc <= 64), so a wavefront reads consecutive
-// columns of a row of W; wave w sums the rows s of its quarter of [0, nS):
-//   x_J[c] = g[j0 + c] - sum_s W[s][c] x_S[s]   (x_S = xs[xoff .. xoff+nS), quarters added
-//   in wave order), written to out[perm[j0 + c]]; then the push entries [k0, k1) copy
-//   x_J (sources in [c0, c0+nc)) or x_S (sources >= nJ) into the children's buffers.
+// Backward item (12 x int64): j0, nJ, nS, wtoff, xoff, c0, nc, k0, k1, C, with W
+// row-major at fac + wtoff. The item covers columns c0 .. c0+nc-1 (nc <= C); C (16, 32 or
+// 64) columns take each wavefront as C lanes x 64/C row groups, so the 256 threads form
+// P = 256/C parts (part = wave * 64/C + lane / C), and part p sums the rows s of the p-th
+// contiguous 1/P of [0, nS) (four accumulators, s ascending); the parts are added in part
+// order through LDS:
+//   x_J[c] = g[j0 + c] - sum_s W[s][c] x_S[s]   (x_S = xs[xoff .. xoff+nS)),
+// written to out[perm[j0 + c]]; then the push entries [k0, k1) copy x_J (sources in
+// [c0, c0+nc)) or x_S (sources >= nJ) into the children's buffers. Narrow fronts use
+// narrow C, so fewer lanes idle.
 template <typename T>
 __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
     const int64_t* __restrict__ items, const int32_t* __restrict__ perm,
@@ -84,22 +112,24 @@ __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
     const T* __restrict__ fac, const T* __restrict__ g, T* __restrict__ xs,
     T* __restrict__ out, const dds_pcg_state* __restrict__ gate) {
   extern __shared__ double xsl[];
-  __shared__ double red[kNdWaves][64];
+  __shared__ double red[kNdThreads];
   __shared__ double xj[64];
   if (gate != nullptr && gate->done) return;
   const int64_t* it = items + 12 * (int64_t)blockIdx.x;
   const int64_t j0 = it[0], wtoff = it[3], xoff = it[4], k0 = it[7], k1 = it[8];
   const int nJ = (int)it[1], nS = (int)it[2], c0 = (int)it[5], nc = (int)it[6];
+  const int C = (int)it[9];  // 16, 32 or 64 (checked on the host)
+  const int P = kNdThreads / C;
   for (int s = threadIdx.x; s < nS; s += kNdThreads) xsl[s] = static_cast<double>(xs[xoff + s]);
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int q = (nS + kNdWaves - 1) / kNdWaves;
-  const int s0 = w * q;
+  const int lc = (int)threadIdx.x & (C - 1);
+  const int part = (int)threadIdx.x / C;
+  const int q = (nS + P - 1) / P;
+  const int s0 = min(nS, part * q);
   const int s1 = min(nS, s0 + q);
   double acc = 0.0;
-  if (lane < nc) {
-    const T* W = fac + wtoff + c0 + lane;
+  if (lc < nc) {
+    const T* W = fac + wtoff + c0 + lc;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int s = s0;
     for (; s + 3 < s1; s += 4) {
@@ -111,15 +141,15 @@ __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
     for (; s < s1; ++s) a0 += static_cast<double>(load_nt(W + (int64_t)s * nJ)) * xsl[s];
     acc = (a0 + a1) + (a2 + a3);
   }
-  red[w][lane] = acc;
+  red[threadIdx.x] = acc;
   __syncthreads();
-  if (w == 0 && lane < nc) {
-    double sum = 0.0;
-#pragma unroll
-    for (int i = 0; i < kNdWaves; ++i) sum += red[i][lane];
-    const int64_t j = j0 + c0 + lane;
+  if ((int)threadIdx.x < nc) {
+    const int c = (int)threadIdx.x;
+    double sum = red[c];
+    for (int p = 1; p < P; ++p) sum += red[p * C + c];
+    const int64_t j = j0 + c0 + c;
     const double x = static_cast<double>(g[j]) - sum;
-    xj[lane] = x;
+    xj[c] = x;
     out[perm[j]] = static_cast<T>(x);
   }
   __syncthreads();

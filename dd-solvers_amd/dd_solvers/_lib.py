@@ -27,7 +27,7 @@ F64, F32, F16, BF16, I32, I64 = 0, 1, 2, 3, 4, 5
 FIN_CHECK, FIN_RHO, FIN_ALPHA, FIN_NEXT = 0, 1, 2, 3
 MAX_PARTIALS = 16384
 NRHS = 16  # right-hand sides per block in the multi-RHS kernels
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _TORCH_TO_DDS = {
     torch.float64: F64,

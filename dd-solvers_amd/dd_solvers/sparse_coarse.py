@@ -329,30 +329,39 @@ class NDPlan:
         """Work items of the lower levels (the top front is solved densely).
 
         Forward record (12 x int64): j0, nJ, nS, foff, soff, n_children, wbase, moff, r0,
-        nr, 0, 0 -- rows r0 .. r0+nr-1 (nr <= 256, one per thread) of the front's stacked
-        factor. Backward record: j0, nJ, nS, wtoff, xoff, c0, nc, k0, k1, 0, 0, 0 -- columns
-        c0 .. c0+nc-1 (nc <= 64) and the push-list entries [k0, k1) whose sources are those
-        columns (the last item of a front also pushes the coupling-set values)."""
+        nr, R, 0 -- rows r0 .. r0+nr-1 (nr <= R) of the front's stacked factor; the 256
+        threads are R rows x 256/R column parts. Backward record: j0, nJ, nS, wtoff, xoff,
+        c0, nc, k0, k1, C, 0, 0 -- columns c0 .. c0+nc-1 (nc <= C), the 256 threads being C
+        columns x 256/C row parts, and the push-list entries [k0, k1) whose sources are
+        those columns (the last item of a front also pushes the coupling-set values).
+        On levels with fewer than 4096 whole-row items (too few workgroups to hide the
+        latency of long per-thread sums), wide fronts split their forward sums into column
+        parts: four from 32 columns on, two from 12. Backward items take 64 columns above
+        32 columns, 32 above 16, else 16 (measured on the 3D box subdomains of the 64^3 Q1
+        problem: leaf levels are fastest unsplit, the levels near the roots split)."""
         self.fwd, self.bwd = [], []
         nlev = self.level_ptr.shape[0] - 1 - (1 if self.has_top else 0)
         for h in range(nlev):
             fi, bi = [], []
-            for t in range(self.level_ptr[h], self.level_ptr[h + 1]):
+            lo, hi = self.level_ptr[h], self.level_ptr[h + 1]
+            split = int(np.sum((self.m[lo:hi] + 255) // 256)) < 4096
+            for t in range(lo, hi):
                 j0, nj, ns, m = int(self.fj0[t]), int(self.fnj[t]), int(self.fns[t]), int(self.m[t])
                 head = [j0, nj, ns, int(self.foff[t])]
                 nch = len(self.children[t])
-                for r0 in range(0, m, 256):
+                R = 256 if not split else 64 if nj >= 32 else 128 if nj >= 12 else 256
+                for r0 in range(0, m, R):
                     fi.append(head + [int(self.soff[t]), nch, int(self.wbase[t]),
-                                      int(self.moff[t]), r0, min(256, m - r0), 0, 0])
+                                      int(self.moff[t]), r0, min(R, m - r0), R, 0])
                 k_lo, k_hi = int(self.push_ptr[t]), int(self.push_ptr[t + 1])
                 src = self.push_src[k_lo:k_hi]
-                for c0 in range(0, nj, 64):
-                    nc = min(64, nj - c0)
+                C = 64 if nj > 32 else 32 if nj > 16 else 16
+                for c0 in range(0, nj, C):
+                    nc = min(C, nj - c0)
                     k0 = k_lo + int(np.searchsorted(src, c0))
                     k1 = k_hi if c0 + nc == nj else k_lo + int(np.searchsorted(src, c0 + nc))
                     bi.append([j0, nj, ns, int(self.wtoff[t]), int(self.xoff[t]), c0, nc, k0, k1,
-                               0, 0, 0])
-            lo, hi = self.level_ptr[h], self.level_ptr[h + 1]
+                               C, 0, 0])
             lf = int(self.fnj[lo:hi].max())
             lb = int(self.fns[lo:hi].max())
             self.fwd.append((np.asarray(fi, dtype=np.int64).reshape(-1, 12), lf))
@@ -376,16 +385,18 @@ class NDPlan:
                 assert self.maps[self.moff[t]:self.moff[t + 1]].max(initial=-1) < self.m[p]
                 assert self.wbase[t] + self.m[p] <= self.slot_size
         for items, lds in self.fwd:
-            j0, nj, ns, foff, soff, nch, wb, mo, r0, nr = (items[:, k] for k in range(10))
+            j0, nj, ns, foff, soff, nch, wb, mo, r0, nr, R = (items[:, k] for k in range(11))
             m = nj + ns
             assert np.all(nj >= 1) and np.all(j0 + nj <= n) and np.all(foff + m * nj <= self.fac_size)
             assert np.all(soff + nch * m <= self.slot_size)
-            assert np.all((nr >= 1) & (nr <= 256)) and np.all(r0 + nr <= m)
+            assert np.all(np.isin(R, (64, 128, 256)))
+            assert np.all((nr >= 1) & (nr <= R)) and np.all(r0 + nr <= m)
             assert np.all((wb >= 0) | (ns == 0)) and np.all(mo + ns <= self.maps.shape[0])
             assert lds <= LDS_CAP and np.all(nj <= lds)
         for items, lds in self.bwd:
-            j0, nj, ns, wt, xo, c0, nc, k0, k1 = (items[:, k] for k in range(9))
-            assert np.all((nc >= 1) & (nc <= 64)) and np.all(c0 + nc <= nj)
+            j0, nj, ns, wt, xo, c0, nc, k0, k1, C = (items[:, k] for k in range(10))
+            assert np.all(np.isin(C, (16, 32, 64)))
+            assert np.all((nc >= 1) & (nc <= C)) and np.all(c0 + nc <= nj)
             assert np.all(xo + ns <= max(self.xs_size, 1)) and np.all(wt + ns * nj <= self.fac_size)
             assert np.all((0 <= k0) & (k0 <= k1) & (k1 <= self.push_src.shape[0]))
             for it in items:  # push sources: own columns, or the coupling set (last item)

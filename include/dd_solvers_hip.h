@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define DDS_ABI_VERSION 3
+#define DDS_ABI_VERSION 4
 /* Upper bound on the per-block partial sums written by one reduction-producing launch
  * (partial buffers hold this many doubles, x16 for blocks of right-hand sides). */
 #define DDS_MAX_PARTIALS 16384
@@ -230,14 +230,16 @@ int dds_dense_symv(int dtype, int64_t n, const void* s, int64_t lds, const void*
  * launch. F64 and F32, sums in FP64 in a fixed order. No allocation; b and out are in the
  * original numbering.
  *
- * Forward record (j0, nJ, nS, foff, soff, n_children, wbase, moff, r0, nr <= 256, -, -):
- * with f_J = b[perm[j0 + .]] + sum over the children q of slots[soff + q*(nJ+nS) + .], row
+ * Forward record (j0, nJ, nS, foff, soff, n_children, wbase, moff, r0, nr, R, -), R in
+ * {64, 128, 256} and nr <= R (the 256 threads are R rows x 256/R column parts): with
+ * f_J = b[perm[j0 + .]] + sum over the children q of slots[soff + q*(nJ+nS) + .], row
  * r of [r0, r0+nr): r < nJ: g[j0 + r] = (F_JJ^-1 f_J)_r; r >= nJ: slots[wbase +
  * maps[moff + r - nJ]] = (children's slots at r) - (W f_J)_{r - nJ}. */
 int dds_nd_forward(int dtype, int64_t n_items, const int64_t* items, const int32_t* perm,
                    const int32_t* maps, const void* fac, const void* b, void* g, void* slots,
                    int64_t lds_elems, const dds_pcg_state* gate, void* stream);
-/* Backward record (j0, nJ, nS, wtoff, xoff, c0, nc <= 64, k0, k1, -, -, -), W row-major at
+/* Backward record (j0, nJ, nS, wtoff, xoff, c0, nc, k0, k1, C, -, -), C in {16, 32, 64}
+ * and nc <= C (the 256 threads are C columns x 256/C row parts), W row-major at
  * fac + wtoff: x_J[c] = g[j0 + c] - (W^T x_S)_c for c in [c0, c0+nc) with
  * x_S = xs[xoff .. xoff+nS), out[perm[j0 + c]] =
  * x_J[c]; then push entries [k0, k1) (sources in [c0, c0+nc), or >= nJ for the coupling
