@@ -619,8 +619,9 @@ def main():
                     help="after setup only launch the local apply, the SpMV and a 1 GiB device "
                          "copy, 20 times each (for rocprofv3 --pmc byte counters)")
     ap.add_argument("--storage", default="float64", choices=["float64", "float32", "float16", "bfloat16"])
-    ap.add_argument("--layout", default="sym", choices=["sym", "mfma"],
-                    help="--profile-kernel: the local-inverse layout to launch")
+    ap.add_argument("--layout", default="sym", choices=["sym", "mfma", "sparse"],
+                    help="--profile-kernel: the local-inverse layout to launch (sparse: the "
+                         "sweeps of the sparse local factors, BatchSparseCholesky)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -648,6 +649,26 @@ def main():
         # measured kernels, 20 launches each, then a 1 GiB device copy 20 times to
         # calibrate the byte counters
         st = getattr(torch, args.storage)
+        if args.layout == "sparse":
+            A = torch.sparse_csr_tensor(torch.as_tensor(pb.A.indptr), torch.as_tensor(pb.A.indices),
+                                        torch.as_tensor(pb.A.data), pb.A.shape).to(dev)
+            solver = dds.CG(dds.HybridSchwarz(torch.float64, dds.BatchSparseCholesky(leaf_size=64),
+                                              dds.DenseCholesky()), rtol=1e-8, bsr_matmul=False)
+            solver.setup(A, decomposition=pb.decomposition)
+            M = solver.preconditioner
+            xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
+            Y = M.Y if M.Y is not None else torch.empty_like(xv)
+            torch.cuda.synchronize()
+            for _ in range(20):
+                M._enqueue_local(xv, Y, None)
+            torch.cuda.synchronize()
+            plan = M.local_solver.plan
+            out = {"profile_kernel": True, "layout": "sparse", "heights": int(plan.n_levels),
+                   "fronts": int(plan.n_fronts),
+                   "factor_bytes": 8 * int(np.sum(plan.fnj * plan.fnj + 2 * plan.fns * plan.fnj))}
+            log(json.dumps(out))
+            print(json.dumps(out))
+            return
         if args.layout == "mfma":
             solver = make_block_solver(pb, st, dev, use_graph=False, autotune=False)
         else:

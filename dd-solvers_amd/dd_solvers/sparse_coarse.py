@@ -33,7 +33,14 @@ Backward, top first, one dds_nd_backward launch per height: x_J = g_J - W^T x_S,
 at each child's coupling set is pushed into that child's buffer. The stacked factor
 [Finv; W] of a front is column-major, so the forward sweep gives each thread one row and
 the backward sweep each lane one column, both streaming contiguous memory with
-independent loads; every sum runs in a fixed order.
+independent loads; on the levels near the roots, where few workgroups are in flight, the
+rows' sums are split into column parts (forward) and a front's columns into narrower
+items with more row parts (backward), added in part order. Every sum runs in a fixed
+order.
+
+`BatchSparseCholesky` puts the same machinery in the local-solver slot: the subdomain
+matrices of a Schwarz preconditioner are factorised as one block-diagonal matrix, so one
+sweep launch per height serves every subdomain.
 """
 
 from __future__ import annotations
