@@ -156,10 +156,11 @@ def batched_extra(pb, storage, dev, steps, warmup, seed, kind="hybrid", fp64_tts
     return out
 
 
-def sparse_local_extra(pb, dev, steps, warmup, xh, fp64_tts, use_graph=True):
-    """The same FP64 Hybrid solve with exact sparse local solves (BatchSparseCholesky:
+def sparse_local_extra(pb, dev, steps, warmup, xh, fp64_tts, use_graph=True, precision=None):
+    """The same FP64 Hybrid solve with sparse local solves (BatchSparseCholesky:
     nested-dissection block LDL^T of every subdomain, one forward and one backward sweep
-    launch per tree height) in place of the dense inverses."""
+    launch per tree height) in place of the dense inverses; the factor in FP64 (exact
+    local solves) or stored in FP32 (`precision`, FP64 vectors and sums)."""
     import numpy as np
     import torch
     import dd_solvers as dds
@@ -167,7 +168,8 @@ def sparse_local_extra(pb, dev, steps, warmup, xh, fp64_tts, use_graph=True):
     A = torch.sparse_csr_tensor(torch.as_tensor(pb.A.indptr), torch.as_tensor(pb.A.indices),
                                 torch.as_tensor(pb.A.data), pb.A.shape).to(dev)
     b = torch.as_tensor(pb.b, device=dev)
-    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.BatchSparseCholesky(leaf_size=64),
+    s = dds.CG(dds.HybridSchwarz(torch.float64,
+                                 dds.BatchSparseCholesky(leaf_size=64, precision=precision),
                                  dds.DenseCholesky()),
                rtol=1e-8, maxiter=2000, bsr_matmul=False, check_every=8, use_graph=use_graph)
     t = time.perf_counter()
@@ -183,7 +185,8 @@ def sparse_local_extra(pb, dev, steps, warmup, xh, fp64_tts, use_graph=True):
     # algorithmic bytes of one local solve: the stacked factor (forward) and the row-major
     # W (backward), the right-hand side gathered (value + index) and the result written
     nk = int(pb.decomposition.sub_ptr[-1])
-    fac_bytes = 8 * int(np.sum(plan.fnj * plan.fnj + 2 * plan.fns * plan.fnj))
+    fac_bytes = (4 if precision == torch.float32 else 8) * \
+        int(np.sum(plan.fnj * plan.fnj + 2 * plan.fns * plan.fnj))
     alg = fac_bytes + nk * (8 + 4) + nk * 8
     xs = x.cpu().numpy()
     out = {"tts_s": round(total / steps, 6), "min_tts_s": round(min(times), 6),
@@ -195,7 +198,7 @@ def sparse_local_extra(pb, dev, steps, warmup, xh, fp64_tts, use_graph=True):
            "local_solve_us": round(t_loc * 1e3, 2), "local_alg_bytes": int(alg),
            "factor_bytes": int(fac_bytes),
            "local_GBs": round(alg / (t_loc * 1e-3) / 1e9, 1)}
-    log(f"sparse local factor: TTS {out['tts_s'] * 1e3:.2f} ms, {out['iterations']} it, local "
+    log(f"sparse local factor ({precision or 'fp64'}): TTS {out['tts_s'] * 1e3:.2f} ms, {out['iterations']} it, local "
         f"solve {out['local_solve_us']} us ({out['local_GBs']} GB/s), err vs inverses "
         f"{out['rel_err_vs_fp64_inverses']:.1e}")
     s.destroy()
@@ -777,12 +780,13 @@ def main():
                 torch.cuda.empty_cache()
             except Exception as e:  # report, keep the measured line
                 extra[f"mixed_{name}"] = {"error": str(e)[:200]}
-        try:
-            extra["sparse_local_fp64"] = sparse_local_extra(pb, dev, args.steps, args.warmup, xh,
-                                                            tts, use_graph=not args.eager)
-        except Exception as e:
-            extra["sparse_local_fp64"] = {"error": str(e)[:200]}
-        torch.cuda.empty_cache()
+        for name, prec in (("sparse_local_fp64", None), ("sparse_local_fp32_factor", torch.float32)):
+            try:
+                extra[name] = sparse_local_extra(pb, dev, args.steps, args.warmup, xh, tts,
+                                                 use_graph=not args.eager, precision=prec)
+            except Exception as e:
+                extra[name] = {"error": str(e)[:200]}
+            torch.cuda.empty_cache()
         for name, st in (("fp16", torch.float16), ("bf16", torch.bfloat16)):
             try:
                 extra[f"batched_{name}"] = batched_extra(pb, st, dev, args.steps, args.warmup,

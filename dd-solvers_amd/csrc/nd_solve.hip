@@ -33,10 +33,10 @@ constexpr int64_t kNdLdsCap = 7680;  // dynamic LDS elements per workgroup: 60 K
 //   r >= nJ:  the parent's slot gets (children's slots at r) - sum_c W[r-nJ][c] f_J[c].
 // Splitting the columns shortens each thread's chain of dependent loads for the wide
 // fronts near the roots, where few workgroups are in flight.
-template <typename T>
+template <typename T, typename F>
 __global__ __launch_bounds__(kNdThreads) void nd_forward_kernel(
     const int64_t* __restrict__ items, const int32_t* __restrict__ perm,
-    const int32_t* __restrict__ maps, const T* __restrict__ fac, const T* __restrict__ b,
+    const int32_t* __restrict__ maps, const F* __restrict__ fac, const T* __restrict__ b,
     T* __restrict__ g, T* __restrict__ slots, const dds_pcg_state* __restrict__ gate) {
   extern __shared__ double f[];
   __shared__ double red[kNdThreads];
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kNdThreads) void nd_forward_kernel(
     const int q = (nJ + K - 1) / K;
     const int cb = part * q;
     const int ce = min(nJ, cb + q);
-    const T* col = fac + foff + r;
+    const F* col = fac + foff + r;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int c = cb;
     for (; c + 3 < ce; c += 4) {
@@ -105,11 +105,11 @@ __global__ __launch_bounds__(kNdThreads) void nd_forward_kernel(
 // written to out[perm[j0 + c]]; then the push entries [k0, k1) copy x_J (sources in
 // [c0, c0+nc)) or x_S (sources >= nJ) into the children's buffers. Narrow fronts use
 // narrow C, so fewer lanes idle.
-template <typename T>
+template <typename T, typename F>
 __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
     const int64_t* __restrict__ items, const int32_t* __restrict__ perm,
     const int64_t* __restrict__ push_dst, const int32_t* __restrict__ push_src,
-    const T* __restrict__ fac, const T* __restrict__ g, T* __restrict__ xs,
+    const F* __restrict__ fac, const T* __restrict__ g, T* __restrict__ xs,
     T* __restrict__ out, const dds_pcg_state* __restrict__ gate) {
   extern __shared__ double xsl[];
   __shared__ double red[kNdThreads];
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
   const int s1 = min(nS, s0 + q);
   double acc = 0.0;
   if (lc < nc) {
-    const T* W = fac + wtoff + c0 + lc;
+    const F* W = fac + wtoff + c0 + lc;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int s = s0;
     for (; s + 3 < s1; s += 4) {
@@ -204,26 +204,26 @@ __global__ __launch_bounds__(kNdThreads) void nd_top_push_kernel(
   if (i < n_push) xs[push_dst[i]] = xtop[push_src[i]];
 }
 
-template <typename T>
+template <typename T, typename F>
 int nd_forward(int64_t n_items, const int64_t* items, const int32_t* perm, const int32_t* maps,
                const void* fac, const void* b, void* g, void* slots, int64_t lds,
                const dds_pcg_state* gate, hipStream_t s) {
-  hipLaunchKernelGGL(nd_forward_kernel<T>, dim3((unsigned)n_items), dim3(kNdThreads),
+  hipLaunchKernelGGL((nd_forward_kernel<T, F>), dim3((unsigned)n_items), dim3(kNdThreads),
                      (size_t)std::max<int64_t>(lds, 1) * sizeof(double), s, items, perm, maps,
-                     static_cast<const T*>(fac), static_cast<const T*>(b), static_cast<T*>(g),
+                     static_cast<const F*>(fac), static_cast<const T*>(b), static_cast<T*>(g),
                      static_cast<T*>(slots), gate);
   DDS_LAUNCHED();
   return DDS_OK;
 }
 
-template <typename T>
+template <typename T, typename F>
 int nd_backward(int64_t n_items, const int64_t* items, const int32_t* perm,
                 const int64_t* push_dst, const int32_t* push_src, const void* fac,
                 const void* g, void* xs, void* out, int64_t lds, const dds_pcg_state* gate,
                 hipStream_t s) {
-  hipLaunchKernelGGL(nd_backward_kernel<T>, dim3((unsigned)n_items), dim3(kNdThreads),
+  hipLaunchKernelGGL((nd_backward_kernel<T, F>), dim3((unsigned)n_items), dim3(kNdThreads),
                      (size_t)std::max<int64_t>(lds, 1) * sizeof(double), s, items, perm,
-                     push_dst, push_src, static_cast<const T*>(fac), static_cast<const T*>(g),
+                     push_dst, push_src, static_cast<const F*>(fac), static_cast<const T*>(g),
                      static_cast<T*>(xs), static_cast<T*>(out), gate);
   DDS_LAUNCHED();
   return DDS_OK;
@@ -234,42 +234,50 @@ int nd_backward(int64_t n_items, const int64_t* items, const int32_t* perm,
 
 using namespace dds;
 
-extern "C" int dds_nd_forward(int dtype, int64_t n_items, const int64_t* items,
-                              const int32_t* perm, const int32_t* maps, const void* fac,
-                              const void* b, void* g, void* slots, int64_t lds_elems,
-                              const dds_pcg_state* gate, void* stream) {
+extern "C" int dds_nd_forward(int dtype, int fac_dtype, int64_t n_items,
+                              const int64_t* items, const int32_t* perm, const int32_t* maps,
+                              const void* fac, const void* b, void* g, void* slots,
+                              int64_t lds_elems, const dds_pcg_state* gate, void* stream) {
   DDS_REQUIRE(n_items >= 0 && n_items <= INT32_MAX, "dds_nd_forward: bad item count");
   DDS_REQUIRE(lds_elems >= 0 && lds_elems <= kNdLdsCap, "dds_nd_forward: front of %lld > %lld",
               (long long)lds_elems, (long long)kNdLdsCap);
   if (n_items == 0) return DDS_OK;
   DDS_REQUIRE(items && perm && fac && b && g && slots, "dds_nd_forward: null pointer");
   hipStream_t s = as_stream(stream);
-  if (dtype == DDS_F64)
-    return nd_forward<double>(n_items, items, perm, maps, fac, b, g, slots, lds_elems, gate, s);
-  if (dtype == DDS_F32)
-    return nd_forward<float>(n_items, items, perm, maps, fac, b, g, slots, lds_elems, gate, s);
-  set_error("dds_nd_forward: unsupported dtype %d", dtype);
+  if (dtype == DDS_F64 && fac_dtype == DDS_F64)
+    return nd_forward<double, double>(n_items, items, perm, maps, fac, b, g, slots, lds_elems,
+                                      gate, s);
+  if (dtype == DDS_F64 && fac_dtype == DDS_F32)
+    return nd_forward<double, float>(n_items, items, perm, maps, fac, b, g, slots, lds_elems,
+                                     gate, s);
+  if (dtype == DDS_F32 && fac_dtype == DDS_F32)
+    return nd_forward<float, float>(n_items, items, perm, maps, fac, b, g, slots, lds_elems,
+                                    gate, s);
+  set_error("dds_nd_forward: unsupported dtypes %d / %d", dtype, fac_dtype);
   return DDS_EUNSUPPORTED;
 }
 
-extern "C" int dds_nd_backward(int dtype, int64_t n_items, const int64_t* items,
-                               const int32_t* perm, const int64_t* push_dst,
-                               const int32_t* push_src, const void* fac, const void* g,
-                               void* xs, void* out, int64_t lds_elems,
-                               const dds_pcg_state* gate, void* stream) {
+extern "C" int dds_nd_backward(int dtype, int fac_dtype, int64_t n_items,
+                               const int64_t* items, const int32_t* perm,
+                               const int64_t* push_dst, const int32_t* push_src,
+                               const void* fac, const void* g, void* xs, void* out,
+                               int64_t lds_elems, const dds_pcg_state* gate, void* stream) {
   DDS_REQUIRE(n_items >= 0 && n_items <= INT32_MAX, "dds_nd_backward: bad item count");
   DDS_REQUIRE(lds_elems >= 0 && lds_elems <= kNdLdsCap, "dds_nd_backward: front of %lld > %lld",
               (long long)lds_elems, (long long)kNdLdsCap);
   if (n_items == 0) return DDS_OK;
   DDS_REQUIRE(items && perm && fac && g && xs && out, "dds_nd_backward: null pointer");
   hipStream_t s = as_stream(stream);
-  if (dtype == DDS_F64)
-    return nd_backward<double>(n_items, items, perm, push_dst, push_src, fac, g, xs, out,
-                               lds_elems, gate, s);
-  if (dtype == DDS_F32)
-    return nd_backward<float>(n_items, items, perm, push_dst, push_src, fac, g, xs, out,
-                              lds_elems, gate, s);
-  set_error("dds_nd_backward: unsupported dtype %d", dtype);
+  if (dtype == DDS_F64 && fac_dtype == DDS_F64)
+    return nd_backward<double, double>(n_items, items, perm, push_dst, push_src, fac, g, xs,
+                                       out, lds_elems, gate, s);
+  if (dtype == DDS_F64 && fac_dtype == DDS_F32)
+    return nd_backward<double, float>(n_items, items, perm, push_dst, push_src, fac, g, xs,
+                                      out, lds_elems, gate, s);
+  if (dtype == DDS_F32 && fac_dtype == DDS_F32)
+    return nd_backward<float, float>(n_items, items, perm, push_dst, push_src, fac, g, xs, out,
+                                     lds_elems, gate, s);
+  set_error("dds_nd_backward: unsupported dtypes %d / %d", dtype, fac_dtype);
   return DDS_EUNSUPPORTED;
 }
 

@@ -71,8 +71,8 @@ _SIGNATURES = {
     "dds_chol_solve": [_i, _i64, _p, _i64, _p, _p, _p, _p],
     "dds_chol_inverse": [_i, _i64, _p, _i64, _p, _i64, _p],
     "dds_dense_symv": [_i, _i64, _p, _i64, _p, _p, _p, _p],
-    "dds_nd_forward": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p],
-    "dds_nd_backward": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p],
+    "dds_nd_forward": [_i, _i, _i64, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p],
+    "dds_nd_backward": [_i, _i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p],
     "dds_nd_top_assemble": [_i, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p],
     "dds_nd_top_push": [_i, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
     "dds_pcg_residual": [_i64, _p, _p, _p, _p, _ip, _p, _p],

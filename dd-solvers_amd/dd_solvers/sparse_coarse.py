@@ -584,7 +584,7 @@ class SparseCholesky(SparseSolver):
         if matrix.dtype not in (torch.float64, torch.float32):
             raise ValueError(f"unsupported coarse dtype {matrix.dtype}")
         dev = matrix.device
-        self.dtype = matrix.dtype
+        self.dtype = self.fac_dtype = matrix.dtype
         if matrix.layout is torch.sparse_csr:
             A = sps.csr_matrix((matrix.values().cpu().double().numpy(),
                                 matrix.col_indices().cpu().numpy(),
@@ -594,13 +594,17 @@ class SparseCholesky(SparseSolver):
         self._setup_host(A, dev)
 
     def _setup_host(self, A: sps.csr_matrix, dev) -> None:
-        """Plan on the host, factor and tables on `dev` (self.dtype set by the caller)."""
+        """Plan on the host, factor and tables on `dev` (self.dtype, the vectors' dtype,
+        and self.fac_dtype, the factor's, set by the caller)."""
         self.n = A.shape[0]
         plan = NDPlan(A, self.leaf_size, self.top_max)
         plan.validate()
+        if plan.has_top and self.fac_dtype != self.dtype:
+            raise ValueError("a factor stored in another dtype than the vectors needs a plan "
+                             "without a top front (top_max=0)")
         self.plan = plan
         with torch.cuda.device(dev):
-            self.fac = factorize(plan, plan.Ap.data, dev, self.dtype)
+            self.fac = factorize(plan, plan.Ap.data, dev, self.fac_dtype)
             I = lambda a, dt=torch.int64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt,
                                                           device=dev)
             self._maps = I(plan.maps, torch.int32)
@@ -631,11 +635,12 @@ class SparseCholesky(SparseSolver):
 
     def _enqueue_solve(self, x, y, gate) -> None:
         dt = _lib.dtype_code(self.dtype)
+        ft = _lib.dtype_code(self.fac_dtype)
         s = _lib.stream()
         g = _lib.ptr(gate)
         p = self.plan
         for items, n_items, lds in self._fwd:
-            _lib.call("dds_nd_forward", dt, n_items, items.data_ptr(), self._perm_in.data_ptr(),
+            _lib.call("dds_nd_forward", dt, ft, n_items, items.data_ptr(), self._perm_in.data_ptr(),
                       self._maps.data_ptr(), self.fac.data_ptr(), x.data_ptr(),
                       self._g.data_ptr(), self._slots.data_ptr(), lds, g, s)
         if p.has_top:
@@ -651,7 +656,7 @@ class SparseCholesky(SparseSolver):
                       self._xtop.data_ptr(), self._push_dst[k0:].data_ptr(),
                       self._push_src[k0:].data_ptr(), self._xs.data_ptr(), y.data_ptr(), g, s)
         for items, n_items, lds in reversed(self._bwd):
-            _lib.call("dds_nd_backward", dt, n_items, items.data_ptr(), self._perm.data_ptr(),
+            _lib.call("dds_nd_backward", dt, ft, n_items, items.data_ptr(), self._perm.data_ptr(),
                       self._push_dst.data_ptr(), self._push_src.data_ptr(), self.fac.data_ptr(),
                       self._g.data_ptr(), self._xs.data_ptr(), y.data_ptr(), lds, g, s)
 
@@ -661,9 +666,10 @@ class SparseCholesky(SparseSolver):
     @property
     def bytes_per_solve(self) -> int:
         e = 8 if self.dtype == torch.float64 else 4
+        f = 8 if self.fac_dtype == torch.float64 else 4
         p = self.plan
         # forward: the stacked factor; backward: the row-major copy of W
-        return e * int(np.sum(p.fnj * p.fnj + 2 * p.fns * p.fnj)) + 4 * e * p.n + \
+        return f * int(np.sum(p.fnj * p.fnj + 2 * p.fns * p.fnj)) + 4 * e * p.n + \
             4 * e * p.xs_size
 
     def destroy(self):
@@ -706,19 +712,25 @@ class BatchSparseCholesky(SparseBatchSolver):
     The batch is factorised as the one block-diagonal matrix diag(A_i): nested dissection
     of each subdomain's graph (its pieces are sibling subtrees, no top front), block LDL^T
     by height across all subdomains at once, and one forward and one backward sweep launch
-    per height for the whole batch. The factor is stored in the dtype of the batch values
-    (the preconditioner precision). Inside the Schwarz preconditioners the sweeps are gated
+    per height for the whole batch. The factor is computed in FP64 and stored in
+    `precision` (default: the dtype of the batch values, the preconditioner precision);
+    `precision=torch.float32` with FP64 values keeps FP64 vectors and sums over an FP32
+    factor (half the bytes; the local solves are then exact to about 1e-7 relative).
+    Inside the Schwarz preconditioners the sweeps are gated
     device launches (graph replay works), and overlapping or uneven subdomains are
     supported: the forward sweep reads the right-hand side through the subdomain index
     lists, so no gathered copy is made.
     """
 
-    def __init__(self, leaf_size: int = 64):
+    def __init__(self, leaf_size: int = 64, precision: torch.dtype | None = None):
+        if precision not in (None, torch.float64, torch.float32):
+            raise ValueError(f"unsupported factor precision {precision}")
         self.leaf_size = leaf_size
+        self.precision = precision
         self._inner = None
 
     def __str__(self):
-        return "BatchSparseCholesky"
+        return f"BatchSparseCholesky({self.precision or ''})"
 
     def setup(self, matrix_batch: CSRBatch):
         vals = matrix_batch.values
@@ -726,8 +738,11 @@ class BatchSparseCholesky(SparseBatchSolver):
         if vals.dtype not in (torch.float64, torch.float32):
             raise ValueError(f"unsupported local dtype {vals.dtype}")
         self.n_solvers = int(matrix_batch.nrows.shape[0])
+        fac_dtype = self.precision or vals.dtype
+        if vals.dtype == torch.float32 and fac_dtype == torch.float64:
+            raise ValueError("an FP64 factor needs FP64 values (the preconditioner precision)")
         inner = SparseCholesky(self.leaf_size, top_max=0)
-        inner.dtype = vals.dtype
+        inner.dtype, inner.fac_dtype = vals.dtype, fac_dtype
         inner._setup_host(csr_batch_to_host(matrix_batch), vals.device)
         self._inner = inner
 

@@ -227,26 +227,29 @@ int dds_dense_symv(int dtype, int64_t n, const void* s, int64_t lds, const void*
  * zero), and a parent writes x at a child's coupling set into that child's buffer,
  * xs[push_dst[k]] = x at front position push_src[k]. Work items are 12-int64 records, one
  * per workgroup; lds_elems (<= 7680) is the largest nJ (forward) or nS (backward) of the
- * launch. F64 and F32, sums in FP64 in a fixed order. No allocation; b and out are in the
- * original numbering.
+ * launch. dtype is that of the vectors and buffers (b, g, slots, xs, out), fac_dtype that of
+ * the factor: F64/F64, F32/F32, or F64 vectors with an F32 factor (a factor stored in FP32,
+ * for plans without a top front). Sums in FP64 in a fixed order. No allocation; b and out
+ * are in the original numbering.
  *
  * Forward record (j0, nJ, nS, foff, soff, n_children, wbase, moff, r0, nr, R, -), R in
  * {64, 128, 256} and nr <= R (the 256 threads are R rows x 256/R column parts): with
  * f_J = b[perm[j0 + .]] + sum over the children q of slots[soff + q*(nJ+nS) + .], row
  * r of [r0, r0+nr): r < nJ: g[j0 + r] = (F_JJ^-1 f_J)_r; r >= nJ: slots[wbase +
  * maps[moff + r - nJ]] = (children's slots at r) - (W f_J)_{r - nJ}. */
-int dds_nd_forward(int dtype, int64_t n_items, const int64_t* items, const int32_t* perm,
-                   const int32_t* maps, const void* fac, const void* b, void* g, void* slots,
-                   int64_t lds_elems, const dds_pcg_state* gate, void* stream);
+int dds_nd_forward(int dtype, int fac_dtype, int64_t n_items, const int64_t* items,
+                   const int32_t* perm, const int32_t* maps, const void* fac, const void* b,
+                   void* g, void* slots, int64_t lds_elems, const dds_pcg_state* gate,
+                   void* stream);
 /* Backward record (j0, nJ, nS, wtoff, xoff, c0, nc, k0, k1, C, -, -), C in {16, 32, 64}
  * and nc <= C (the 256 threads are C columns x 256/C row parts), W row-major at
  * fac + wtoff: x_J[c] = g[j0 + c] - (W^T x_S)_c for c in [c0, c0+nc) with
  * x_S = xs[xoff .. xoff+nS), out[perm[j0 + c]] =
  * x_J[c]; then push entries [k0, k1) (sources in [c0, c0+nc), or >= nJ for the coupling
  * set). */
-int dds_nd_backward(int dtype, int64_t n_items, const int64_t* items, const int32_t* perm,
-                    const int64_t* push_dst, const int32_t* push_src, const void* fac,
-                    const void* g, void* xs, void* out, int64_t lds_elems,
+int dds_nd_backward(int dtype, int fac_dtype, int64_t n_items, const int64_t* items,
+                    const int32_t* perm, const int64_t* push_dst, const int32_t* push_src,
+                    const void* fac, const void* g, void* xs, void* out, int64_t lds_elems,
                     const dds_pcg_state* gate, void* stream);
 /* Top front, unknowns [j0, j0+n_top): ftop[p] = b[perm[j0 + p]] + sum over its n_children
  * children q of slots[soff + q*n_top + p]. */

@@ -35,7 +35,8 @@ def _torch_csr(A, dtype=torch.float64):
                                    torch.as_tensor(A.data), A.shape).to(device=dev, dtype=dtype)
 
 
-def _parity(pb, kind, local, rtol=1e-8, use_graph=False, check_every=8, coarse=None):
+def _parity(pb, kind, local, rtol=1e-8, use_graph=False, check_every=8, coarse=None,
+            it_tol=1, x_tol=1e-10):
     d = pb.decomposition
     O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind=kind,
                       storage="float64")
@@ -46,9 +47,9 @@ def _parity(pb, kind, local, rtol=1e-8, use_graph=False, check_every=8, coarse=N
                bsr_matmul=False, check_every=check_every, use_graph=use_graph)
     s.setup(_torch_csr(pb.A), decomposition=d)
     x, m = s.solve(torch.as_tensor(pb.b, device=dev))
-    assert abs(m["iterations"] - mo["iterations"]) <= 1, (m["iterations"], mo["iterations"])
+    assert abs(m["iterations"] - mo["iterations"]) <= it_tol, (m["iterations"], mo["iterations"])
     err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
-    assert err <= 1e-10, err
+    assert err <= x_tol, err
     res = np.linalg.norm(pb.b - pb.A @ x.cpu().numpy()) / np.linalg.norm(pb.b)
     assert res < rtol * 1.01
     return x, m, s
@@ -77,6 +78,18 @@ def test_sparse_coarse_and_sparse_local():
     pb = P.make_problem("q1_3d", 16, box=4, overlap=1, seed=4, lognormal=True)
     _parity(pb, "hybrid", dds.BatchSparseCholesky(leaf_size=16),
             coarse=dds.SparseCholesky(leaf_size=8, top_max=16))
+
+
+@pytest.mark.parametrize("kind", ["additive", "hybrid"])
+def test_fp32_factor_fp64_vectors(kind):
+    """The factor stored in FP32 (computed in FP64), vectors and sums in FP64: the precision
+    form of the sparse local slot. Bar as for the FP32 inverses: iterations within 2 of the
+    FP64 oracle's, solution within 1e-6, true residual below rtol."""
+    pb = P.make_problem("q1_3d", 24, box=8, overlap=1, seed=1, lognormal=True)
+    _, _, s = _parity(pb, kind, dds.BatchSparseCholesky(precision=torch.float32), it_tol=2,
+                      x_tol=1e-6)
+    ls = s.preconditioner.local_solver
+    assert ls._inner.fac.dtype == torch.float32 and ls._inner.dtype == torch.float64
 
 
 def test_graph_replay_matches_eager():
@@ -172,8 +185,10 @@ def test_user_sparse_batch_solver_needs_reference_subdomains():
         s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
 
 
-@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-5)])
-def test_batch_solver_standalone(dtype, tol):
+@pytest.mark.parametrize("dtype,precision,tol", [(torch.float64, None, 1e-12),
+                                                   (torch.float32, None, 2e-5),
+                                                   (torch.float64, torch.float32, 2e-6)])
+def test_batch_solver_standalone(dtype, precision, tol):
     """setup(CSRBatch) / solve((n_s, k)) -> flat, as BatchCUDSS: against SuperLU per block."""
     rng = np.random.default_rng(7)
     n_s, k = 9, 40
@@ -186,10 +201,20 @@ def test_batch_solver_standalone(dtype, tol):
     fields = ref_csr_batch(Ab.indptr, Ab.indices, Ab.data, k)
     batch = CSRBatch(*(torch.as_tensor(f, device=dev) for f in fields))
     batch = batch._replace(values=batch.values.to(dtype))
-    solver = dds.BatchSparseCholesky(leaf_size=8)
+    solver = dds.BatchSparseCholesky(leaf_size=8, precision=precision)
     solver.setup(batch)
     rhs = rng.standard_normal((n_s, k))
     x = solver.solve(torch.as_tensor(rhs, dtype=dtype, device=dev))
     assert x.shape == (n_s * k,) and x.dtype == dtype
     want = np.concatenate([spla.spsolve(Bk.tocsc(), r) for Bk, r in zip(blocks, rhs)])
     assert np.linalg.norm(x.double().cpu().numpy() - want) <= tol * np.linalg.norm(want)
+
+
+def test_factor_precision_checks():
+    with pytest.raises(ValueError, match="unsupported factor precision"):
+        dds.BatchSparseCholesky(precision=torch.float16)
+    pb = P.make_problem("q1_3d", 16, box=4, overlap=1, seed=4, lognormal=True)
+    s = dds.CG(dds.HybridSchwarz(torch.float32, dds.BatchSparseCholesky(precision=torch.float64),
+                                 dds.DenseCholesky()), bsr_matmul=False)
+    with pytest.raises(ValueError, match="FP64 factor needs FP64 values"):
+        s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)

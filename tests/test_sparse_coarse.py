@@ -135,11 +135,13 @@ def test_front_capacity_is_enforced():
 
 def test_host_validation_of_the_sweep_entry_points():
     with pytest.raises(_lib.HipError, match="front of"):
-        _lib.call("dds_nd_forward", _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 7681, None, None)
+        _lib.call("dds_nd_forward", _lib.F64, _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 7681, None, None)
     with pytest.raises(_lib.HipError, match="front of"):
-        _lib.call("dds_nd_backward", _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 1, 9000, None, None)
+        _lib.call("dds_nd_backward", _lib.F64, _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 1, 9000, None, None)
     with pytest.raises(_lib.HipError, match="unsupported"):
-        _lib.call("dds_nd_forward", _lib.F16, 1, 1, 1, 1, 1, 1, 1, 1, 8, None, None)
+        _lib.call("dds_nd_forward", _lib.F16, _lib.F16, 1, 1, 1, 1, 1, 1, 1, 1, 8, None, None)
+    with pytest.raises(_lib.HipError, match="unsupported"):  # FP32 vectors, FP64 factor
+        _lib.call("dds_nd_backward", _lib.F32, _lib.F64, 1, 1, 1, 1, 1, 1, 1, 1, 1, 8, None, None)
     with pytest.raises(_lib.HipError, match="bad sizes"):
         _lib.call("dds_nd_top_assemble", _lib.F64, 0, -1, 0, 0, 1, 1, 1, 1, None, None)
     with pytest.raises(_lib.HipError, match="null pointer"):
@@ -266,3 +268,12 @@ def test_block_diagonal_plan():
         own = block[plan.perm[plan.fj0[t]:plan.fj0[t] + plan.fnj[t]]]
         cpl = block[plan.perm[plan.S[t]]]
         assert np.all(own == own[0]) and np.all(cpl == own[0])
+
+
+def test_mixed_factor_dtype_needs_no_top_front():
+    """An FP32 factor under FP64 vectors is for plans without a dense top front (the top
+    front's kernels take one dtype); the check comes before any device work."""
+    solver = sc.SparseCholesky(leaf_size=8, top_max=64)
+    solver.dtype, solver.fac_dtype = torch.float64, torch.float32
+    with pytest.raises(ValueError, match="without a top front"):
+        solver._setup_host(lap2d(30, 30), "cpu")
