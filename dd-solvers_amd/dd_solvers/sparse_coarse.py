@@ -341,11 +341,14 @@ class NDPlan:
         c0, nc, k0, k1, C, 0, 0 -- columns c0 .. c0+nc-1 (nc <= C), the 256 threads being C
         columns x 256/C row parts, and the push-list entries [k0, k1) whose sources are
         those columns (the last item of a front also pushes the coupling-set values).
-        On levels with fewer than 4096 whole-row items (too few workgroups to hide the
-        latency of long per-thread sums), wide fronts split their forward sums into column
-        parts: four from 32 columns on, two from 12. Backward items take 64 columns above
-        32 columns, 32 above 16, else 16 (measured on the 3D box subdomains of the 64^3 Q1
-        problem: leaf levels are fastest unsplit, the levels near the roots split)."""
+        Backward items take 64 columns above 32 columns, 32 above 16, else 16. On levels
+        with fewer than 4096 whole-row items (too few workgroups to hide the latency of
+        long per-thread sums), wide fronts split their forward sums into column parts (four
+        from 32 columns on, two from 12), and long coupling sets narrow the backward items
+        (at most 32 columns above 128 coupled unknowns, 16 above 256), so a thread sums at
+        most about 32 rows. Measured: the leaf levels of the 64^3 Q1 boxes are fastest
+        unsplit; the split levels cut the S9 coarse solve of the reference's DG runs
+        (524,288 cells) by 10%."""
         self.fwd, self.bwd = [], []
         nlev = self.level_ptr.shape[0] - 1 - (1 if self.has_top else 0)
         for h in range(nlev):
@@ -363,6 +366,8 @@ class NDPlan:
                 k_lo, k_hi = int(self.push_ptr[t]), int(self.push_ptr[t + 1])
                 src = self.push_src[k_lo:k_hi]
                 C = 64 if nj > 32 else 32 if nj > 16 else 16
+                if split:
+                    C = min(C, 64 if ns <= 128 else 32 if ns <= 256 else 16)
                 for c0 in range(0, nj, C):
                     nc = min(C, nj - c0)
                     k0 = k_lo + int(np.searchsorted(src, c0))
