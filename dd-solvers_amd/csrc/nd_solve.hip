@@ -71,6 +71,19 @@ __global__ __launch_bounds__(kNdThreads) void nd_forward_kernel(
     const F* col = fac + foff + r;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int c = cb;
+    for (; c + 7 < ce; c += 8) {  // eight loads in flight, then the same four sums
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = static_cast<double>(load_nt(col + (int64_t)(c + u) * m));
+      a0 += v[0] * f[c];
+      a1 += v[1] * f[c + 1];
+      a2 += v[2] * f[c + 2];
+      a3 += v[3] * f[c + 3];
+      a0 += v[4] * f[c + 4];
+      a1 += v[5] * f[c + 5];
+      a2 += v[6] * f[c + 6];
+      a3 += v[7] * f[c + 7];
+    }
     for (; c + 3 < ce; c += 4) {
       a0 += static_cast<double>(load_nt(col + (int64_t)c * m)) * f[c];
       a1 += static_cast<double>(load_nt(col + (int64_t)(c + 1) * m)) * f[c + 1];
@@ -132,6 +145,19 @@ __global__ __launch_bounds__(kNdThreads) void nd_backward_kernel(
     const F* W = fac + wtoff + c0 + lc;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int s = s0;
+    for (; s + 7 < s1; s += 8) {  // eight loads in flight, then the same four sums
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = static_cast<double>(load_nt(W + (int64_t)(s + u) * nJ));
+      a0 += v[0] * xsl[s];
+      a1 += v[1] * xsl[s + 1];
+      a2 += v[2] * xsl[s + 2];
+      a3 += v[3] * xsl[s + 3];
+      a0 += v[4] * xsl[s + 4];
+      a1 += v[5] * xsl[s + 5];
+      a2 += v[6] * xsl[s + 6];
+      a3 += v[7] * xsl[s + 7];
+    }
     for (; s + 3 < s1; s += 4) {
       a0 += static_cast<double>(load_nt(W + (int64_t)s * nJ)) * xsl[s];
       a1 += static_cast<double>(load_nt(W + (int64_t)(s + 1) * nJ)) * xsl[s + 1];
