@@ -206,10 +206,14 @@ __global__ __launch_bounds__(kSolveThreads) void chol_solve_kernel(int64_t n, co
   for (int64_t i = threadIdx.x; i < n; i += kSolveThreads) yb[i] = sol[i];
 }
 
-// y = S x, S dense symmetric row-major: one wavefront per row. When the row and x start on
-// 16-byte boundaries each lane reads whole 16-byte chunks of its row (chunks c and c + 64
-// in flight together) and the last n % V elements one per lane; lanes' sums are added by
-// a butterfly.
+// y = S x, S dense symmetric row-major: one wavefront per row. A row that starts off a
+// 16-byte boundary (an odd leading dimension puts every other row there) first takes
+// the h < V elements up to the boundary one per lane; the rest of the row is read in whole
+// 16-byte chunks, eight per lane at a time (chunks c, c + 64, ..., c + 448 in flight
+// together, those past the row end skipped), with x read in 16-byte chunks when it shares
+// that alignment and element by element (cached) when it does not; the last elements
+// follow one per lane. Lanes' sums are added by a butterfly. A row of a few thousand
+// entries is one round trip to memory per lane.
 template <typename T>
 __global__ __launch_bounds__(256) void dense_symv_kernel(int64_t n, const T* __restrict__ S,
                                                          int64_t lds, const T* __restrict__ x,
@@ -222,27 +226,43 @@ __global__ __launch_bounds__(256) void dense_symv_kernel(int64_t n, const T* __r
   if (row >= n) return;
   const T* Sr = S + row * lds;
   T a0 = 0, a1 = 0;
-  if ((reinterpret_cast<uintptr_t>(Sr) & 15) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
-    const int64_t nchunk = n / V;
-    int64_t c = lane;
-    for (; c + 64 < nchunk; c += 128) {
-      const Vec16<T> s0 = load16_nt(Sr + c * V);
-      const Vec16<T> s1 = load16_nt(Sr + (c + 64) * V);
-      const Vec16<T> x0 = load16(x + c * V);
-      const Vec16<T> x1 = load16(x + (c + 64) * V);
+  const uintptr_t mis = reinterpret_cast<uintptr_t>(Sr) & 15;
+  if (mis % sizeof(T) == 0) {
+    // elements before the first 16-byte boundary of the row
+    const int64_t h = mis == 0 ? 0 : std::min<int64_t>(n, (int64_t)((16 - mis) / sizeof(T)));
+    if (lane < h) a0 += Sr[lane] * x[lane];
+    const T* Sa = Sr + h;
+    const T* xa = x + h;
+    const bool xal = (reinterpret_cast<uintptr_t>(xa) & 15) == 0;
+    const int64_t nchunk = (n - h) / V;
+    for (int64_t c0 = lane; c0 < nchunk; c0 += 8 * 64) {
+      Vec16<T> sv[8], xv[8];
 #pragma unroll
-      for (int e = 0; e < V; ++e) {
-        a0 += s0.v[e] * x0.v[e];
-        a1 += s1.v[e] * x1.v[e];
+      for (int u = 0; u < 8; ++u) {
+        const int64_t c = c0 + (int64_t)u * 64;
+        if (c < nchunk) {
+          sv[u] = load16_nt(Sa + c * V);
+          if (xal) {
+            xv[u] = load16(xa + c * V);
+          } else {
+#pragma unroll
+            for (int e = 0; e < V; ++e) xv[u].v[e] = xa[c * V + e];
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < V; ++e) sv[u].v[e] = xv[u].v[e] = T{};
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          a0 += sv[u].v[e] * xv[u].v[e];
+          a1 += sv[u + 1].v[e] * xv[u + 1].v[e];
+        }
       }
     }
-    if (c < nchunk) {
-      const Vec16<T> s0 = load16_nt(Sr + c * V);
-      const Vec16<T> x0 = load16(x + c * V);
-#pragma unroll
-      for (int e = 0; e < V; ++e) a0 += s0.v[e] * x0.v[e];
-    }
-    const int64_t t = nchunk * V + lane;
+    const int64_t t = h + nchunk * V + lane;
     if (t < n) a0 += Sr[t] * x[t];
   } else {
     for (int64_t j = lane; j < n; j += 64) a0 += Sr[j] * x[j];

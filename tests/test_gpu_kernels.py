@@ -390,10 +390,12 @@ def test_restrict_all_lane_counts(lanes, dtype):
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 127, 128, 129, 1000, 4097])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-@pytest.mark.parametrize("offset", [0, 1])
+@pytest.mark.parametrize("offset", [0, 1, 2, 3])
 def test_dense_symv_all_widths(n, dtype, offset):
     """y = S x for a dense row-major S: rows on 16-byte boundaries (chunked loads and the
-    element tail) and rows off them (offset start, odd leading dimension)."""
+    element tail) and rows off them by every element count (offset start and leading
+    dimension n + offset: the head elements up to the boundary, then chunks with x read
+    element-wise)."""
     rng = np.random.default_rng(n)
     lda = n + offset
     S = rng.standard_normal((n, lda))
