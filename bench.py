@@ -410,7 +410,8 @@ def bsr_reference_shapes(dev):
 
 # The reference's precision experiment (results/experiment_precisions.csv, rtol 1e-9, A100)
 # on its own DG systems: (p, fine mesh, subdomain mesh, coarse mesh).
-REF_EXPERIMENTS = [(1, 10, "S8", "S8"), (1, 10, "S9", "S9"), (3, 9, "S8", "S8"), (3, 9, "S7", "S7")]
+REF_EXPERIMENTS = [(1, 10, "S8", "S8"), (1, 10, "S9", "S9"), (3, 9, "S8", "S8"), (3, 9, "S9", "S9"),
+                   (3, 9, "S7", "S7")]
 REF_SOLVERS = [("hybrid", "float64", None), ("hybrid", "float64", "float16"),
                ("additive", "float64", None), ("additive", "float32", "float16")]
 

@@ -150,10 +150,47 @@ __global__ __launch_bounds__(kThreads) void gj_inverse_kernel(T* __restrict__ A,
   if (tid == 0 && info != nullptr) info[blockIdx.x] = bad;
 }
 
+// out[i] = vals[seg_ptr[i]] + vals[seg_ptr[i] + 1] + ... + vals[seg_ptr[i + 1] - 1], added
+// in index order in T (one thread per segment: the sums are the sequential ones, the same
+// on every run).
+template <typename T>
+__global__ __launch_bounds__(256) void segment_sum_kernel(int64_t n_seg,
+                                                          const int64_t* __restrict__ seg_ptr,
+                                                          const T* __restrict__ vals,
+                                                          T* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_seg) return;
+  const int64_t e = seg_ptr[i + 1];
+  T acc = T{};
+  for (int64_t j = seg_ptr[i]; j < e; ++j) acc += vals[j];
+  out[i] = acc;
+}
+
 }  // namespace
 }  // namespace dds
 
 using namespace dds;
+
+extern "C" int dds_segment_sum(int dtype, int64_t n_seg, const int64_t* seg_ptr,
+                               const void* vals, void* out, void* stream) {
+  DDS_REQUIRE(n_seg >= 0 && n_seg <= (int64_t)INT32_MAX * 256, "dds_segment_sum: bad count");
+  if (n_seg == 0) return DDS_OK;
+  DDS_REQUIRE(seg_ptr && vals && out, "dds_segment_sum: null pointer");
+  hipStream_t s = as_stream(stream);
+  const dim3 grid((unsigned)ceil_div(n_seg, 256));
+  if (dtype == DDS_F64) {
+    hipLaunchKernelGGL(segment_sum_kernel<double>, grid, dim3(256), 0, s, n_seg, seg_ptr,
+                       static_cast<const double*>(vals), static_cast<double*>(out));
+  } else if (dtype == DDS_F32) {
+    hipLaunchKernelGGL(segment_sum_kernel<float>, grid, dim3(256), 0, s, n_seg, seg_ptr,
+                       static_cast<const float*>(vals), static_cast<float*>(out));
+  } else {
+    set_error("dds_segment_sum: dtype must be F64 or F32 (got %d)", dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
 
 extern "C" int dds_spd_inverse_batched(int dtype, int64_t n_mats, const int32_t* k, int64_t ld,
                                        int64_t stride, void* A, int32_t* info, void* stream) {

@@ -29,7 +29,8 @@ import torch
 
 from . import _lib
 from .bsr import FastBSR
-from .csr import csr_rows, csr_to_bsr, csr_to_torch, inverse_permutation, permute_csr_tensor
+from .csr import (csr_rows, csr_to_bsr, csr_to_torch, inverse_permutation, permute_csr_tensor,
+                  segment_sum)
 from .csr import asm_permutation  # noqa: F401  (re-exported)
 from .gemv import gemv_strided_batched
 
@@ -1237,12 +1238,12 @@ class SchwarzOperator(SparseSolver):
     def _segment_sum_csr(key: torch.Tensor, val: torch.Tensor, n_rows: int,
                          n_cols: int) -> torch.Tensor:
         """CSR (n_rows x n_cols, FP64) of the sums of val over equal keys row * n_cols + col,
-        deterministic (sort + segment_reduce); the coarse matrix stays sparse, so coarse
+        deterministic (sort + in-order segment sums); the coarse matrix stays sparse, so coarse
         spaces of millions of cells fit."""
         order = torch.sort(key, stable=True).indices
         ks = key[order]
         uk, counts = torch.unique_consecutive(ks, return_counts=True)
-        sums = torch.segment_reduce(val[order].to(torch.float64), "sum", lengths=counts)
+        sums = segment_sum(val[order].to(torch.float64), counts)
         crow = torch.zeros(n_rows + 1, dtype=torch.int64, device=val.device)
         crow[1:] = torch.cumsum(torch.bincount(uk // n_cols, minlength=n_rows), 0)
         return torch.sparse_csr_tensor(crow, uk % n_cols, sums, size=(n_rows, n_cols))
@@ -1741,7 +1742,7 @@ class HybridSchwarz(SchwarzOperator):
         order = torch.sort(key, stable=True).indices
         ks = key[order]
         uk, counts = torch.unique_consecutive(ks, return_counts=True)
-        sums = torch.segment_reduce(val[order].to(torch.float64), "sum", lengths=counts)
+        sums = segment_sum(val[order].to(torch.float64), counts)
         r_of = uk // self.n
         crow = torch.zeros(self.n_coarse + 1, dtype=torch.int64, device=self.device)
         crow[1:] = torch.cumsum(torch.bincount(r_of, minlength=self.n_coarse), 0)

@@ -408,3 +408,40 @@ def test_dense_symv_all_widths(n, dtype, offset):
     ref = S[:, :n] @ x
     tol = 1e-12 if dtype == torch.float64 else 2e-4
     assert np.allclose(yt.double().cpu().numpy(), ref, rtol=tol, atol=tol * np.sqrt(n))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_segment_sum_small_and_empty_runs(dtype):
+    """dds_segment_sum adds each run in index order, in the values' dtype; empty runs give 0."""
+    from dd_solvers.csr import segment_sum
+    rng = np.random.default_rng(5)
+    lengths = rng.integers(0, 9, size=300)
+    vals = rng.standard_normal(int(lengths.sum())).astype(np.float64 if dtype == torch.float64 else np.float32)
+    got = segment_sum(torch.as_tensor(vals, device=dev), torch.as_tensor(lengths, device=dev))
+    want, k = [], 0
+    for ln in lengths:
+        acc = vals.dtype.type(0)
+        for v in vals[k:k + ln]:
+            acc = vals.dtype.type(acc + v)
+        want.append(acc)
+        k += ln
+    assert got.dtype == dtype
+    assert np.array_equal(got.cpu().numpy(), np.array(want, dtype=vals.dtype))
+
+
+def test_segment_sum_beyond_2_pow_24_segments():
+    """More than 2^24 runs (the merge of R_C A for the reference's p = 3 S9 / S9 / S9 runs has
+    21 million): every run's sum equals the in-order sum (built on the host position by
+    position: the j-th entry of every run is added at step j)."""
+    from dd_solvers.csr import segment_sum
+    rng = np.random.default_rng(6)
+    n = (1 << 24) + 4096
+    lengths = rng.integers(1, 8, size=n)
+    vals = rng.standard_normal(int(lengths.sum()))
+    got = segment_sum(torch.as_tensor(vals, device=dev), torch.as_tensor(lengths, device=dev))
+    starts = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+    want = np.zeros(n)
+    for j in range(int(lengths.max())):
+        has = lengths > j
+        want[has] = want[has] + vals[starts[has] + j]
+    assert np.array_equal(got.cpu().numpy(), want)
