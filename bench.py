@@ -412,6 +412,10 @@ def bsr_reference_shapes(dev):
 # on its own DG systems: (p, fine mesh, subdomain mesh, coarse mesh).
 REF_EXPERIMENTS = [(1, 10, "S8", "S8"), (1, 10, "S9", "S9"), (3, 9, "S8", "S8"), (3, 9, "S9", "S9"),
                    (3, 9, "S7", "S7")]
+# run only when named in --reference-only: their host-side set-up (the S11 assembly, the
+# nested-dissection plan of a 2-million-cell coarse space) takes minutes
+REF_EXPERIMENTS_LARGE = [(1, 10, "S10", "S10"), (1, 11, "S9", "S9"), (1, 11, "S10", "S10"),
+                         (1, 11, "S11", "S11")]
 REF_SOLVERS = [("hybrid", "float64", None), ("hybrid", "float64", "float16"),
                ("additive", "float64", None), ("additive", "float32", "float16")]
 
@@ -431,7 +435,7 @@ def reference_experiments(dev, steps, warmup, only=None):
     T = {"float64": torch.float64, "float32": torch.float32, "float16": torch.float16}
     out = []
     meshes = {}
-    for p, fine, sm, cm in REF_EXPERIMENTS:
+    for p, fine, sm, cm in REF_EXPERIMENTS + (REF_EXPERIMENTS_LARGE if only else []):
         if only and f"p{p}_S{fine}_{sm}_{cm}" not in only:
             continue
         t = time.perf_counter()

@@ -48,6 +48,8 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 from typing import List, Tuple
 
+import hashlib
+
 import numpy as np
 import scipy.sparse as sps
 import scipy.sparse.csgraph as csgraph
@@ -565,6 +567,38 @@ def solve_host(plan: NDPlan, fac: np.ndarray, b: np.ndarray) -> np.ndarray:
 # Coarse-slot solver
 # ---------------------------------------------------------------------------------------
 
+# The last plan built, keyed by the pattern it was built for: the ordering and the tables
+# depend only on the pattern of A (without stored zeros), and a run of experiments sets up
+# one coarse matrix after another with the same pattern (other precisions and methods), so
+# the host-side dissection of a coarse space of millions of cells is done once.
+_PLAN_CACHE: dict = {}
+
+
+def plan_for(A: sps.csr_matrix, leaf_size: int, top_max: int):
+    """(plan, values): the plan of A's pattern (built, or the cached one for the same
+    pattern and parameters) and A's entries in the order of the plan's permuted matrix
+    plan.Ap, which `factorize` takes."""
+    A = sps.csr_matrix(A, copy=True)
+    A.eliminate_zeros()
+    A.sort_indices()
+    h = hashlib.blake2b(digest_size=16)
+    h.update(np.ascontiguousarray(A.indptr, dtype=np.int64).tobytes())
+    h.update(np.ascontiguousarray(A.indices, dtype=np.int64).tobytes())
+    key = (A.shape[0], int(leaf_size), int(top_max), h.hexdigest())
+    plan = _PLAN_CACHE.get(key)
+    if plan is None:
+        plan = NDPlan(A, leaf_size, top_max)
+        plan.validate()
+        _PLAN_CACHE.clear()
+        _PLAN_CACHE[key] = plan
+        return plan, plan.Ap.data
+    Ap = A[plan.perm][:, plan.perm].tocsr()
+    Ap.sort_indices()
+    if not (np.array_equal(Ap.indptr, plan.Ap.indptr) and np.array_equal(Ap.indices, plan.Ap.indices)):
+        raise RuntimeError("cached plan does not match the pattern")  # a hash collision
+    return plan, Ap.data
+
+
 class SparseCholesky(SparseSolver):
     """Coarse-space slot for large sparse SPD coarse matrices (any `SparseSolver` slot:
     setup(csr), solve(v) -> (x, {})); the sparse counterpart of the reference's `CUDSS()`
@@ -602,14 +636,13 @@ class SparseCholesky(SparseSolver):
         """Plan on the host, factor and tables on `dev` (self.dtype, the vectors' dtype,
         and self.fac_dtype, the factor's, set by the caller)."""
         self.n = A.shape[0]
-        plan = NDPlan(A, self.leaf_size, self.top_max)
-        plan.validate()
+        plan, values = plan_for(A, self.leaf_size, self.top_max)
         if plan.has_top and self.fac_dtype != self.dtype:
             raise ValueError("a factor stored in another dtype than the vectors needs a plan "
                              "without a top front (top_max=0)")
         self.plan = plan
         with torch.cuda.device(dev):
-            self.fac = factorize(plan, plan.Ap.data, dev, self.fac_dtype)
+            self.fac = factorize(plan, values, dev, self.fac_dtype)
             I = lambda a, dt=torch.int64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt,
                                                           device=dev)
             self._maps = I(plan.maps, torch.int32)

@@ -277,3 +277,25 @@ def test_mixed_factor_dtype_needs_no_top_front():
     solver.dtype, solver.fac_dtype = torch.float64, torch.float32
     with pytest.raises(ValueError, match="without a top front"):
         solver._setup_host(lap2d(30, 30), "cpu")
+
+
+def test_plan_cache_same_pattern_new_values():
+    """A second matrix with the same pattern reuses the plan, and its own values come back
+    in the plan's order: its factor solves it like SuperLU. Another pattern gets a new plan."""
+    A1 = lap2d(30, 24)
+    A2 = A1.copy()
+    rng = np.random.default_rng(3)
+    A2.data = A2.data * (1.0 + 0.1 * rng.random(A2.nnz))
+    A2 = ((A2 + A2.T) * 0.5).tocsr()
+    A2 = A2 + sps.diags(np.abs(A2).sum(1).A1)
+    A2 = A2.tocsr()
+    p1, v1 = sc.plan_for(A1, 16, 64)
+    p2, v2 = sc.plan_for(A2, 16, 64)
+    assert p2 is p1 and not np.array_equal(v1, v2)
+    fac = sc.factorize(p2, v2, "cpu").numpy()
+    b = rng.standard_normal(A2.shape[0])
+    x = sc.solve_host(p2, fac, b)
+    xr = spla.spsolve(A2.tocsc(), b)
+    assert np.linalg.norm(x - xr) <= 1e-11 * np.linalg.norm(xr)
+    p3, _ = sc.plan_for(lap2d(31, 24), 16, 64)
+    assert p3 is not p1
