@@ -151,9 +151,11 @@ __global__ __launch_bounds__(kThreads) void gj_inverse_kernel(T* __restrict__ A,
 }
 
 // out[i] = vals[seg_ptr[i]] + vals[seg_ptr[i] + 1] + ... + vals[seg_ptr[i + 1] - 1], added
-// in index order in T (one thread per segment: the sums are the sequential ones, the same
-// on every run).
-template <typename T>
+// in index order (one thread per segment: the same sums on every run). Plain: in T, the
+// sequential sum. Compensated (Neumaier): the rounding error of each addition is carried in
+// a second term, so a long run whose terms cancel (a coarse-matrix entry sums a whole cell
+// block of the fine matrix, nearly all of it cancelling) keeps about full precision.
+template <typename T, bool kCompensated>
 __global__ __launch_bounds__(256) void segment_sum_kernel(int64_t n_seg,
                                                           const int64_t* __restrict__ seg_ptr,
                                                           const T* __restrict__ vals,
@@ -162,8 +164,18 @@ __global__ __launch_bounds__(256) void segment_sum_kernel(int64_t n_seg,
   if (i >= n_seg) return;
   const int64_t e = seg_ptr[i + 1];
   T acc = T{};
-  for (int64_t j = seg_ptr[i]; j < e; ++j) acc += vals[j];
-  out[i] = acc;
+  T comp = T{};
+  for (int64_t j = seg_ptr[i]; j < e; ++j) {
+    const T v = vals[j];
+    if (kCompensated) {
+      const T t = acc + v;
+      comp += fabs(acc) >= fabs(v) ? (acc - t) + v : (v - t) + acc;
+      acc = t;
+    } else {
+      acc += v;
+    }
+  }
+  out[i] = kCompensated ? acc + comp : acc;
 }
 
 }  // namespace
@@ -171,19 +183,27 @@ __global__ __launch_bounds__(256) void segment_sum_kernel(int64_t n_seg,
 
 using namespace dds;
 
-extern "C" int dds_segment_sum(int dtype, int64_t n_seg, const int64_t* seg_ptr,
+extern "C" int dds_segment_sum(int dtype, int compensated, int64_t n_seg, const int64_t* seg_ptr,
                                const void* vals, void* out, void* stream) {
   DDS_REQUIRE(n_seg >= 0 && n_seg <= (int64_t)INT32_MAX * 256, "dds_segment_sum: bad count");
   if (n_seg == 0) return DDS_OK;
   DDS_REQUIRE(seg_ptr && vals && out, "dds_segment_sum: null pointer");
   hipStream_t s = as_stream(stream);
   const dim3 grid((unsigned)ceil_div(n_seg, 256));
-  if (dtype == DDS_F64) {
-    hipLaunchKernelGGL(segment_sum_kernel<double>, grid, dim3(256), 0, s, n_seg, seg_ptr,
-                       static_cast<const double*>(vals), static_cast<double*>(out));
+  const double* vd = static_cast<const double*>(vals);
+  const float* vf = static_cast<const float*>(vals);
+  if (dtype == DDS_F64 && compensated) {
+    hipLaunchKernelGGL((segment_sum_kernel<double, true>), grid, dim3(256), 0, s, n_seg, seg_ptr,
+                       vd, static_cast<double*>(out));
+  } else if (dtype == DDS_F64) {
+    hipLaunchKernelGGL((segment_sum_kernel<double, false>), grid, dim3(256), 0, s, n_seg,
+                       seg_ptr, vd, static_cast<double*>(out));
+  } else if (dtype == DDS_F32 && compensated) {
+    hipLaunchKernelGGL((segment_sum_kernel<float, true>), grid, dim3(256), 0, s, n_seg, seg_ptr,
+                       vf, static_cast<float*>(out));
   } else if (dtype == DDS_F32) {
-    hipLaunchKernelGGL(segment_sum_kernel<float>, grid, dim3(256), 0, s, n_seg, seg_ptr,
-                       static_cast<const float*>(vals), static_cast<float*>(out));
+    hipLaunchKernelGGL((segment_sum_kernel<float, false>), grid, dim3(256), 0, s, n_seg, seg_ptr,
+                       vf, static_cast<float*>(out));
   } else {
     set_error("dds_segment_sum: dtype must be F64 or F32 (got %d)", dtype);
     return DDS_EUNSUPPORTED;

@@ -27,9 +27,12 @@ __all__ = [
 ]
 
 
-def segment_sum(vals: torch.Tensor, lengths: torch.Tensor) -> torch.Tensor:
+def segment_sum(vals: torch.Tensor, lengths: torch.Tensor,
+                compensated: bool = False) -> torch.Tensor:
     """Sums of the consecutive runs of `vals` with the given lengths, each run added in
-    index order in vals' dtype (float64 or float32). On the GPU this is dds_segment_sum:
+    index order in vals' dtype (float64 or float32); `compensated` carries the rounding error
+    of each addition (Neumaier), for long runs whose terms cancel. On the GPU this is
+    dds_segment_sum:
     torch.segment_reduce on ROCm returned wrong sums for 2^24 of the segments of a
     21-million-segment input (the merge of R_C A for the reference's p = 3 S9 / S9 / S9
     runs), so no setup step relies on it there."""
@@ -41,8 +44,8 @@ def segment_sum(vals: torch.Tensor, lengths: torch.Tensor) -> torch.Tensor:
     vals = vals.contiguous()
     out = torch.empty(n, dtype=vals.dtype, device=vals.device)
     with torch.cuda.device(vals.device):
-        _lib.call("dds_segment_sum", _lib.dtype_code(vals.dtype), n, ptr.data_ptr(),
-                  vals.data_ptr(), out.data_ptr(), _lib.stream())
+        _lib.call("dds_segment_sum", _lib.dtype_code(vals.dtype), int(compensated), n,
+                  ptr.data_ptr(), vals.data_ptr(), out.data_ptr(), _lib.stream())
     return out
 
 

@@ -1243,7 +1243,7 @@ class SchwarzOperator(SparseSolver):
         order = torch.sort(key, stable=True).indices
         ks = key[order]
         uk, counts = torch.unique_consecutive(ks, return_counts=True)
-        sums = segment_sum(val[order].to(torch.float64), counts)
+        sums = segment_sum(val[order].to(torch.float64), counts, compensated=True)
         crow = torch.zeros(n_rows + 1, dtype=torch.int64, device=val.device)
         crow[1:] = torch.cumsum(torch.bincount(uk // n_cols, minlength=n_rows), 0)
         return torch.sparse_csr_tensor(crow, uk % n_cols, sums, size=(n_rows, n_cols))
@@ -1742,7 +1742,7 @@ class HybridSchwarz(SchwarzOperator):
         order = torch.sort(key, stable=True).indices
         ks = key[order]
         uk, counts = torch.unique_consecutive(ks, return_counts=True)
-        sums = segment_sum(val[order].to(torch.float64), counts)
+        sums = segment_sum(val[order].to(torch.float64), counts, compensated=True)
         r_of = uk // self.n
         crow = torch.zeros(self.n_coarse + 1, dtype=torch.int64, device=self.device)
         crow[1:] = torch.cumsum(torch.bincount(r_of, minlength=self.n_coarse), 0)

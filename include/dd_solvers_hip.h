@@ -295,12 +295,13 @@ int dds_sum_partials(const double* partials, int count, double* out, void* strea
 int dds_spd_inverse_batched(int dtype, int64_t n_mats, const int32_t* k, int64_t ld,
                             int64_t stride, void* A, int32_t* info, void* stream);
 /* Segmented sum (dtype DDS_F64 or DDS_F32): out[i] = sum of vals[seg_ptr[i] ..
- * seg_ptr[i+1]) added in index order in dtype, one thread per segment. The duplicate merge
- * of sorted (key, value) lists at setup: the coarse matrix R_C A R_C^T (solvers.py:721-771),
- * R0A = R_C A (solvers.py:890-906) and gather_csr (gather_csr_sum_kernel,
- * cusparse.py:78-158, which also adds each run in order). */
-int dds_segment_sum(int dtype, int64_t n_seg, const int64_t* seg_ptr, const void* vals,
-                    void* out, void* stream);
+ * seg_ptr[i+1]) added in index order in dtype, one thread per segment; compensated != 0
+ * carries each addition's rounding error (Neumaier), for long runs that cancel. The
+ * duplicate merge of sorted (key, value) lists at setup: the coarse matrix R_C A R_C^T
+ * (solvers.py:721-771) and R0A = R_C A (solvers.py:890-906), compensated, and gather_csr
+ * (gather_csr_sum_kernel, cusparse.py:78-158, which adds each run in order), plain. */
+int dds_segment_sum(int dtype, int compensated, int64_t n_seg, const int64_t* seg_ptr,
+                    const void* vals, void* out, void* stream);
 
 /* ---- measurement ---------------------------------------------------------------------------
  * Streaming bandwidth over n doubles (n even, 16-byte aligned arrays), grid workgroups of 256

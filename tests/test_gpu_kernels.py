@@ -445,3 +445,25 @@ def test_segment_sum_beyond_2_pow_24_segments():
         has = lengths > j
         want[has] = want[has] + vals[starts[has] + j]
     assert np.array_equal(got.cpu().numpy(), want)
+
+
+def test_segment_sum_compensated_keeps_cancelled_terms():
+    """Compensated runs keep what a plain in-order sum rounds away: 1e16 + 1 - 1e16 is 0
+    plainly and 1 compensated; on random runs the compensated sums equal the exact (math.fsum)
+    sums to one rounding."""
+    import math
+    from dd_solvers.csr import segment_sum
+    v = torch.tensor([1e16, 1.0, -1e16, 3.0, 1e-20, -3.0], dtype=torch.float64, device=dev)
+    ln = torch.tensor([3, 3], device=dev)
+    assert segment_sum(v, ln).tolist() == [0.0, 0.0]
+    assert segment_sum(v, ln, compensated=True).tolist() == [1.0, 1e-20]
+    rng = np.random.default_rng(8)
+    lengths = rng.integers(1, 2000, size=200)
+    vals = rng.standard_normal(int(lengths.sum())) * 10.0 ** rng.integers(-8, 9, size=int(lengths.sum()))
+    got = segment_sum(torch.as_tensor(vals, device=dev), torch.as_tensor(lengths, device=dev),
+                      compensated=True).cpu().numpy()
+    k = 0
+    for g, ln_ in zip(got, lengths):
+        exact = math.fsum(vals[k:k + ln_])
+        assert abs(g - exact) <= 4.5e-16 * abs(exact) + 1e-300
+        k += ln_
