@@ -38,6 +38,10 @@ def segment_sum(vals: torch.Tensor, lengths: torch.Tensor,
     runs), so no setup step relies on it there."""
     n = int(lengths.shape[0])
     if vals.device.type != "cuda":
+        # host tensors (gather_csr on the CPU): torch's plain sums; the compensated merges
+        # belong to the solvers' set-up, which runs on the device
+        if compensated:
+            raise ValueError("compensated segment sums run on the GPU")
         return torch.segment_reduce(vals, "sum", lengths=lengths)
     ptr = torch.zeros(n + 1, dtype=torch.int64, device=vals.device)
     ptr[1:] = torch.cumsum(lengths.to(torch.int64), 0)
