@@ -415,7 +415,8 @@ REF_EXPERIMENTS = [(1, 10, "S8", "S8"), (1, 10, "S9", "S9"), (3, 9, "S8", "S8"),
 # run only when named in --reference-only: their host-side set-up (the S11 assembly, the
 # nested-dissection plan of a 2-million-cell coarse space) takes minutes
 REF_EXPERIMENTS_LARGE = [(1, 10, "S10", "S10"), (1, 11, "S9", "S9"), (1, 11, "S10", "S10"),
-                         (1, 11, "S11", "S11")]
+                         (1, 11, "S11", "S11"), (5, 8, "S6", "S6"), (5, 8, "S7", "S7"),
+                         (5, 8, "S8", "S8")]
 REF_SOLVERS = [("hybrid", "float64", None), ("hybrid", "float64", "float16"),
                ("additive", "float64", None), ("additive", "float32", "float16")]
 

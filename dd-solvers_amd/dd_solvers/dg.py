@@ -22,11 +22,12 @@ Subdomain and coarse maps. `parent_cells(k)` is the S_k cell containing each cel
 reference composes the refinement parent maps, meshing.py:251-256); `quad_cells(k)` is the
 cell of the 2^k x 2^k quadrilateral mesh C_k (meshing.py:266-283).
 
-Space and form (discretization.py:40-96). DG_p with a nodal Lagrange basis: the vertices,
-for p = 2 the edge midpoints, for p = 3 the two Gauss-Lobatto-Legendre points of each edge
-and the centroid (the points of DOLFINx's default "DG" element, which uses GLL-warped
-points; for p <= 3 these are fixed by symmetry). rho and f are interpolated into the same
-space. With h_e the edge length, delta p^2 = delta * p^2 and w = rho+ rho- / (rho+ + rho-),
+Space and form (discretization.py:40-96). DG_p with a nodal Lagrange basis on the points
+of DOLFINx's default "DG" element (GLL-warped): the vertices, the p - 1 interior
+Gauss-Lobatto-Legendre points of each edge, and interior points placed by a warp blend
+(`reference_nodes`). For p <= 3 the points are fixed by symmetry (for p = 3 the interior
+point is the centroid); for p >= 4 the blend places them, and the published ||b|| of the
+reference's p = 5 runs pins it. rho and f are interpolated into the same space. With h_e the edge length, delta p^2 = delta * p^2 and w = rho+ rho- / (rho+ + rho-),
 
   a(u, v) = sum_T (rho grad u, grad v)_T
           + sum_interior (delta p^2 / h_e * w [u], [v])_e + sum_boundary (delta p^2 rho / h_e u, v)_e
@@ -39,7 +40,7 @@ edge the two traces of rho_h coincide, so w = rho_h / 2), and the rules used her
 for its degree, so the entries equal DOLFINx's up to rounding.
 
 Pins (tests/test_dg.py): the degrees of freedom, matrix nonzeros and ||b|| that the
-reference's runs record for p = 1 (S1-S11) and p = 3 (S1-S10) in
+reference's runs record for p = 1 (S1-S11), p = 3 (S1-S10) and p = 5 (S1-S9) in
 results/experiment_amgx_pcg.csv (the first residual norm of a CG run from x0 = 0 is ||b||);
 the Hybrid Schwarz initial residual ||b - A T0(b)|| and the first Additive Schwarz
 residuals of the reference's S10 / S8 / S8 runs (results/experiment_precisions.csv); and
@@ -148,9 +149,55 @@ continuous_coefficient_2d = DGTestCase(
 # Reference element
 # ---------------------------------------------------------------------------------------
 
+def _gll01(n: int) -> np.ndarray:
+    """The n + 1 Gauss-Lobatto-Legendre points of [0, 1] (ends and the roots of P_n')."""
+    r = np.sort(np.polynomial.legendre.Legendre.basis(n).deriv().roots().real)
+    return np.concatenate([[0.0], (r + 1.0) / 2.0, [1.0]])
+
+
+def _gll_warp(n: int, x: np.ndarray) -> np.ndarray:
+    """w(x) = sum_k (g_k - k/n) l_k(x): the displacement of the equispaced points k/n of
+    [0, 1] to the GLL points g_k, interpolated by the Lagrange polynomials l_k of the
+    equispaced points (zero at both ends)."""
+    g = _gll01(n)
+    e = np.arange(n + 1) / n
+    x = np.asarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    for k in range(1, n):
+        term = np.full_like(x, g[k] - e[k])
+        for m in range(n + 1):
+            if m != k:
+                term = term * (x - e[m]) / (e[k] - e[m])
+        out = out + term
+    return out
+
+
+def _warped_interior(n: int) -> np.ndarray:
+    """Interior points of the degree-n GLL-warped lattice of the triangle (0,0), (1,0),
+    (0,1). The lattice point (x, y) = (i, j)/n, with a = 1 - x - y, moves along the two
+    edge directions by the 1D warp of the barycentric differences, blended so that every
+    edge gets exactly its GLL points:
+        x' = x + x (a W(x - a) + y W(x - y)),   y' = y + y (a W(y - a) + x W(y - x)),
+    W(t) = w(s) / (s (1 - s)) at s = (1 + t) / 2."""
+    r = np.linspace(0.0, 1.0, 2 * n + 1)      # s on a half-step grid: index n + n t
+    W = _gll_warp(n, r)
+    W[1:-1] /= r[1:-1] * (1.0 - r[1:-1])
+    pts = []
+    for j in range(1, n):
+        for i in range(1, n - j):
+            l = n - i - j
+            x, y, a = i / n, j / n, l / n
+            pts.append([x + x * (a * W[n + i - l] + y * W[n + i - j]),
+                        y + y * (a * W[n + j - l] + x * W[n + j - i])])
+    return np.array(pts, dtype=np.float64).reshape(-1, 2)
+
+
 def reference_nodes(p: int) -> np.ndarray:
-    """Nodes of the DG_p basis on the triangle (0,0), (1,0), (0,1)."""
+    """Nodes of the DG_p basis on the triangle (0,0), (1,0), (0,1): vertices, then each
+    edge's interior GLL points, then the interior points."""
     V = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    if p < 1:
+        raise ValueError(f"DG degree {p} is not supported (p >= 1)")
     if p == 1:
         return V
     if p == 2:
@@ -162,7 +209,9 @@ def reference_nodes(p: int) -> np.ndarray:
             for t in (g, 1.0 - g):
                 edges.append(V[a] + t * (V[b] - V[a]))
         return np.vstack([V, np.array(edges), [[1.0 / 3.0, 1.0 / 3.0]]])
-    raise ValueError(f"DG degree {p} is not supported (1, 2 or 3)")
+    g = _gll01(p)[1:-1]
+    edges = [V[a] + t * (V[b] - V[a]) for a, b in ((0, 1), (1, 2), (2, 0)) for t in g]
+    return np.vstack([V, np.array(edges), _warped_interior(p)])
 
 
 def _monomials(p: int):

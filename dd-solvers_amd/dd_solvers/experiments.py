@@ -16,7 +16,7 @@ lines up with the reference's CSVs row for row:
   * the residual norm ||A x - b|| is computed on the host in FP64 (:281-283);
   * a failing run records the exception text and its metadata (:108-139).
 
-What differs: the discretisation is dd_solvers.dg (the reference's S_m meshes, p = 1-3,
+What differs: the discretisation is dd_solvers.dg (the reference's S_m meshes, any p,
 no DOLFINx), so `mesh_family` is `UniformMeshes(2, m)` built from its meshes, and error
 norms come from dg.error_norm (exact solution at quadrature points) instead of a DOLFINx
 form against its DG_{p+2} interpolant.

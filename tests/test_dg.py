@@ -21,9 +21,12 @@ def _nloc(p):
     return (p + 1) * (p + 2) // 2
 
 
-@pytest.mark.parametrize("p,level", [(1, m) for m in range(1, 9)] + [(3, m) for m in range(1, 7)])
+@pytest.mark.parametrize("p,level", [(1, m) for m in range(1, 9)] + [(3, m) for m in range(1, 7)]
+                         + [(5, m) for m in range(1, 6)])
 def test_published_dofs_nnz_and_load_norm(p, level):
-    """DoFs, nonzeros and ||b|| of the reference's DOLFINx assembly, level by level."""
+    """DoFs, nonzeros and ||b|| of the reference's DOLFINx assembly, level by level. For
+    p = 5, ||b|| depends on where the six interior nodes of the element sit, so it pins the
+    warp blend of reference_nodes."""
     ref = PUB["levels"][f"p{p}_S{level}"]
     pr = dg.assemble_sipg(dg.UniformTriangleMesh(level), p)
     assert pr.A.shape[0] == ref["dofs"]
@@ -119,3 +122,34 @@ def test_reproduces_published_fp64_runs_s10_s8_s8(s10_s8_s8, kind):
     rel = np.abs(got - pub) / pub
     assert rel[:10].max() < 1e-11, rel[:10]
     assert rel.max() < 1e-3, rel.max()
+
+
+def test_reference_nodes_gll_on_edges_and_planted_shift():
+    """Every degree's nodes: GLL points on each edge, interior points strictly inside and
+    symmetric under the triangle's reflection x <-> y; p = 3 gives the centroid. A planted
+    shift of one interior p = 5 node moves ||b|| of S2 far off the published value."""
+    from dd_solvers.dg import _gll01
+    for p in range(1, 7):
+        X = dg.reference_nodes(p)
+        assert X.shape == (_nloc(p), 2)
+        g = _gll01(p)[1:-1]
+        e0 = X[3:3 + (p - 1)]
+        assert np.allclose(e0[:, 0], g) and np.allclose(e0[:, 1], 0.0)
+        inner = X[3 + 3 * (p - 1):]
+        assert np.all(inner > 1e-3) and np.all(inner.sum(1) < 1 - 1e-3)
+        mirrored = np.sort(inner[:, ::-1].round(12), axis=0)
+        assert np.allclose(np.sort(inner.round(12), axis=0), mirrored)
+    assert np.allclose(dg.reference_nodes(3)[-1], [1 / 3, 1 / 3])
+    ref = PUB["levels"]["p5_S2"]["b_norm"]
+    orig = dg.reference_nodes
+    try:
+        def shifted(p):
+            X = orig(p).copy()
+            if p == 5:
+                X[-1] += [1e-3, -1e-3]
+            return X
+        dg.reference_nodes = shifted
+        nb = float(np.linalg.norm(dg.assemble_sipg(dg.UniformTriangleMesh(2), 5).b))
+    finally:
+        dg.reference_nodes = orig
+    assert abs(nb - ref) > 1e-8 * ref
