@@ -40,7 +40,8 @@ def _gpu_solve(A, b, k, spc, block, kind, storage=None, rtol=1e-9):
 @pytest.mark.parametrize("kind", ["additive", "hybrid"])
 @pytest.mark.parametrize("p,fine,solvers,coarse", [(1, 6, "S4", "S3"), (1, 6, "S4", "C2"),
                                                    (1, 6, "C4", "C2"), (2, 5, "S3", "S2"),
-                                                   (3, 5, "S3", "S2"), (3, 4, "S4", "S4")])
+                                                   (3, 5, "S3", "S2"), (3, 4, "S4", "S4"),
+                                                   (5, 4, "S3", "S2"), (5, 4, "S4", "S4")])
 def test_dg_solve_matches_oracle(p, fine, solvers, coarse, kind):
     print(f"DG p={p} S{fine} solvers {solvers} coarse {coarse} {kind}", flush=True)
     pr = dg.assemble_sipg(dg.UniformTriangleMesh(fine), p)
