@@ -419,6 +419,14 @@ REF_EXPERIMENTS_LARGE = [(1, 10, "S10", "S10"), (1, 11, "S9", "S9"), (1, 11, "S1
                          (5, 8, "S8", "S8")]
 REF_SOLVERS = [("hybrid", "float64", None), ("hybrid", "float64", "float16"),
                ("additive", "float64", None), ("additive", "float32", "float16")]
+# the reference's parallelism experiment (experiments/parallelism_experiments.py): p = 1 on
+# S11 with subdomains and coarse cells from S_m and C_m, m = 8..11, two preconditioners;
+# run when named "par_p1_S11_<solvers>_<coarse>" in --reference-only
+REF_PARALLELISM = [(sm, cm) for kind in ("S", "C", "mixed") for c in range(8, 12)
+                   for sm_ in range(c, 12)
+                   for sm, cm in [(("C" if kind == "C" else "S") + str(sm_),
+                                   ("S" if kind == "S" else "C") + str(c))]]
+PAR_SOLVERS = [("additive", "float32", "float16"), ("hybrid", "float64", "float16")]
 
 
 def reference_experiments(dev, steps, warmup, only=None):
@@ -436,8 +444,13 @@ def reference_experiments(dev, steps, warmup, only=None):
     T = {"float64": torch.float64, "float32": torch.float32, "float16": torch.float16}
     out = []
     meshes = {}
-    for p, fine, sm, cm in REF_EXPERIMENTS + (REF_EXPERIMENTS_LARGE if only else []):
-        if only and f"p{p}_S{fine}_{sm}_{cm}" not in only:
+    fams = [(p, fine, sm, cm, "precisions", REF_SOLVERS, f"p{p}_S{fine}_{sm}_{cm}")
+            for p, fine, sm, cm in REF_EXPERIMENTS + (REF_EXPERIMENTS_LARGE if only else [])]
+    if only:
+        fams += [(1, 11, sm, cm, "parallelism", PAR_SOLVERS, f"par_p1_S11_{sm}_{cm}")
+                 for sm, cm in REF_PARALLELISM]
+    for p, fine, sm, cm, experiment, solvers, name in fams:
+        if only and name not in only:
             continue
         t = time.perf_counter()
         if (p, fine) not in meshes:
@@ -451,11 +464,13 @@ def reference_experiments(dev, steps, warmup, only=None):
         b = torch.as_tensor(bh, device=dev)
         log(f"DG p={p} S{fine} solvers {sm} coarse {cm}: n={A.shape[0]} k={k} n_c={spc.shape[0]} "
             f"({time.perf_counter() - t:.1f}s)")
-        for kind, pi, inv in REF_SOLVERS:
-            ref = [r for r in pub if (r["p"], r["fine"], r["solvers"], r["coarse"], r["kind"],
-                                      r["precision"], r["inv"]) == (p, f"S{fine}", sm, cm, kind, pi, inv)]
+        for kind, pi, inv in solvers:
+            ref = [r for r in pub if (r["experiment"], r["p"], r["fine"], r["solvers"], r["coarse"],
+                                      r["kind"], r["precision"], r["inv"])
+                   == (experiment, p, f"S{fine}", sm, cm, kind, pi, inv)]
             ref = ref[0] if ref else {}
-            row = {"p": p, "fine": f"S{fine}", "solvers": sm, "coarse": cm, "kind": kind,
+            row = {"experiment": experiment, "p": p, "fine": f"S{fine}", "solvers": sm,
+                   "coarse": cm, "kind": kind,
                    "precision": pi, "inv": inv, "dofs": int(A.shape[0]), "n_coarse": int(spc.shape[0]),
                    "reference_solver": ref.get("solver"),
                    "reference_iterations": ref.get("iterations"),

@@ -10,6 +10,10 @@ Reads two result files of the reference (text data, not code):
                                         for every run ("solves"): the solver's
                                         precisions, its iteration count (null when it
                                         raised) and its best solve time on the A100
+  results/experiment_parallelism.csv -> the same "solves" records for the p = 1 S11 runs
+                                        over subdomain / coarse meshes S8-S11 and C8-C11
+                                        ("experiment": "parallelism"; the precision runs
+                                        carry "experiment": "precisions")
 
 Run from the repository root in a container that has the reference checked out at
 /root/reference:  python tests/golden/make_dg_published.py
@@ -42,21 +46,30 @@ def main():
     solves = []
     with open(os.path.join(REF, "experiment_precisions.csv")) as fh:
         columns = next(csv.reader(fh))
+
+    def solve_record(r, experiment):
+        s = r["solver"]
+        pi = s.split("Schwarz(")[1].split(",")[0].replace("torch.", "")
+        inv = s.split("Inv(")[1].split(")")[0].replace("torch.", "") or None
+        times = ast.literal_eval(r["solve times"]) if r["solve times"] else []
+        md = ast.literal_eval(r["metadata"]) if r["metadata"] else {}
+        return {"experiment": experiment, "p": int(r["p"]), "fine": r["fine m"],
+                "solvers": r["solvers m"], "coarse": r["coarse m"],
+                "kind": "additive" if "AdditiveSchwarz" in s else "hybrid",
+                "precision": pi, "inv": inv,
+                "iterations": None if r["exception"] else md.get("iterations"),
+                "best_ms": min(times) if times and not r["exception"] else None,
+                "solver": s}
+
+    with open(os.path.join(REF, "experiment_parallelism.csv")) as fh:
+        for r in csv.DictReader(fh):
+            if r["solver"].startswith("CG(") and "Schwarz(" in r["solver"]:
+                solves.append(solve_record(r, "parallelism"))
     with open(os.path.join(REF, "experiment_precisions.csv")) as fh:
         for r in csv.DictReader(fh):
             s = r["solver"]
             if s.startswith("CG(") and "Schwarz(" in s:
-                pi = s.split("Schwarz(")[1].split(",")[0].replace("torch.", "")
-                inv = s.split("Inv(")[1].split(")")[0].replace("torch.", "") or None
-                times = ast.literal_eval(r["solve times"]) if r["solve times"] else []
-                md = ast.literal_eval(r["metadata"]) if r["metadata"] else {}
-                solves.append({"p": int(r["p"]), "fine": r["fine m"], "solvers": r["solvers m"],
-                               "coarse": r["coarse m"],
-                               "kind": "additive" if "AdditiveSchwarz" in s else "hybrid",
-                               "precision": pi, "inv": inv,
-                               "iterations": None if r["exception"] else md.get("iterations"),
-                               "best_ms": min(times) if times and not r["exception"] else None,
-                               "solver": s})
+                solves.append(solve_record(r, "precisions"))
             if not s.startswith("CG(") or "torch.float64, Inv()" not in s or r["exception"]:
                 continue
             kind = "additive" if "AdditiveSchwarz" in s else "hybrid"
@@ -68,8 +81,9 @@ def main():
                          "residual_norms": md["residual norms"],
                          "solver": s}
     with open(OUT, "w") as fh:
-        json.dump({"source": "mgr0dzicki/dd-solvers results/experiment_amgx_pcg.csv and "
-                             "results/experiment_precisions.csv (rtol 1e-9, FP64 runs)",
+        json.dump({"source": "mgr0dzicki/dd-solvers results/experiment_amgx_pcg.csv, "
+                             "results/experiment_precisions.csv (rtol 1e-9, FP64 runs) and "
+                             "results/experiment_parallelism.csv",
                    "columns": columns, "levels": levels, "runs": runs, "solves": solves}, fh,
                   indent=1)
     print(f"wrote {OUT}: {len(levels)} levels, {len(runs)} runs, {len(solves)} solves")
