@@ -58,7 +58,8 @@ import torch
 from . import _lib
 from .solvers import CSRBatch, SparseBatchSolver, SparseSolver
 
-__all__ = ["SparseCholesky", "BatchSparseCholesky", "nested_dissection", "NDPlan"]
+__all__ = ["SparseCholesky", "BatchSparseCholesky", "nested_dissection", "NDPlan",
+           "clear_plan_cache"]
 
 # LDS capacity of the sweep kernels, in elements (f_J forward, x_S backward): 60 KiB of FP64
 LDS_CAP = 7680
@@ -572,6 +573,12 @@ def solve_host(plan: NDPlan, fac: np.ndarray, b: np.ndarray) -> np.ndarray:
 # one coarse matrix after another with the same pattern (other precisions and methods), so
 # the host-side dissection of a coarse space of millions of cells is done once.
 _PLAN_CACHE: dict = {}
+
+
+def clear_plan_cache() -> None:
+    """Drop the cached plan (its host tables outlive the solvers that used it; for a coarse
+    space of millions of cells they take gigabytes of host memory)."""
+    _PLAN_CACHE.clear()
 
 
 def plan_for(A: sps.csr_matrix, leaf_size: int, top_max: int):

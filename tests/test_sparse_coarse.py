@@ -299,3 +299,6 @@ def test_plan_cache_same_pattern_new_values():
     assert np.linalg.norm(x - xr) <= 1e-11 * np.linalg.norm(xr)
     p3, _ = sc.plan_for(lap2d(31, 24), 16, 64)
     assert p3 is not p1
+    sc.clear_plan_cache()
+    p4, _ = sc.plan_for(lap2d(31, 24), 16, 64)
+    assert p4 is not p3
