@@ -419,6 +419,12 @@ REF_EXPERIMENTS_LARGE = [(1, 10, "S10", "S10"), (1, 11, "S9", "S9"), (1, 11, "S1
                          (5, 8, "S8", "S8")]
 REF_SOLVERS = [("hybrid", "float64", None), ("hybrid", "float64", "float16"),
                ("additive", "float64", None), ("additive", "float32", "float16")]
+# every configuration of the published precision experiment (--reference-solvers all):
+# preconditioner precision x local-inverse storage, Additive and Hybrid
+REF_SOLVERS_ALL = [(kind, pi, inv) for kind in ("hybrid", "additive")
+                   for pi in ("float64", "float32")
+                   for inv in ((None, "float32", "float16", "bfloat16") if pi == "float64"
+                               else (None, "float16", "bfloat16"))]
 # the reference's parallelism experiment (experiments/parallelism_experiments.py): p = 1 on
 # S11 with subdomains and coarse cells from S_m and C_m, m = 8..11, two preconditioners;
 # run when named "par_p1_S11_<solvers>_<coarse>" in --reference-only
@@ -429,7 +435,7 @@ REF_PARALLELISM = [(sm, cm) for kind in ("S", "C", "mixed") for c in range(8, 12
 PAR_SOLVERS = [("additive", "float32", "float16"), ("hybrid", "float64", "float16")]
 
 
-def reference_experiments(dev, steps, warmup, only=None):
+def reference_experiments(dev, steps, warmup, only=None, all_solvers=False):
     """The reference's DG experiment runs with this build in every slot (the BSR SpMV with
     block size = DoFs per cell, the local inverses in the reference's precisions, the
     sparse coarse solver for its CUDSS slot), graph-replayed iteration blocks, min over
@@ -441,10 +447,12 @@ def reference_experiments(dev, steps, warmup, only=None):
 
     with open(os.path.join(ROOT, "tests", "golden", "dg_published.json")) as f:
         pub = json.load(f)["solves"]
-    T = {"float64": torch.float64, "float32": torch.float32, "float16": torch.float16}
+    T = {"float64": torch.float64, "float32": torch.float32, "float16": torch.float16,
+         "bfloat16": torch.bfloat16}
     out = []
     meshes = {}
-    fams = [(p, fine, sm, cm, "precisions", REF_SOLVERS, f"p{p}_S{fine}_{sm}_{cm}")
+    fams = [(p, fine, sm, cm, "precisions", REF_SOLVERS_ALL if all_solvers else REF_SOLVERS,
+             f"p{p}_S{fine}_{sm}_{cm}")
             for p, fine, sm, cm in REF_EXPERIMENTS + (REF_EXPERIMENTS_LARGE if only else [])]
     if only:
         fams += [(1, 11, sm, cm, "parallelism", PAR_SOLVERS, f"par_p1_S11_{sm}_{cm}")
@@ -514,7 +522,10 @@ def reference_experiments(dev, steps, warmup, only=None):
                 del s, P
             except Exception as e:  # record and go on with the others
                 row["error"] = str(e)[:200]
-                log(f"  {kind} {pi}+{inv}: error {e}")
+                if ref and ref.get("iterations") is None:
+                    row["reference_raised"] = True  # the published run raised as well
+                log(f"  {kind} {pi}+{inv}: error {e} (reference "
+                    f"{'raised too' if row.get('reference_raised') else row['reference_iterations']})")
             torch.cuda.empty_cache()
             out.append(row)
         del At, A, b
@@ -636,6 +647,9 @@ def main():
     ap.add_argument("--reference-only", default=None, metavar="RUNS",
                     help="only the reference's DG runs named here (comma-separated, e.g. "
                          "p1_S10_S8_S8), printed as one JSON line")
+    ap.add_argument("--reference-solvers", default="four", choices=["four", "all"],
+                    help="with --reference-only: the four solver configurations of the default "
+                         "run, or all 14 of the published precision experiment")
     ap.add_argument("--eager", action="store_true",
                     help="enqueue every iteration's kernels from Python instead of replaying a "
                          "captured hipGraph of each block of iterations")
@@ -662,7 +676,8 @@ def main():
     torch.cuda.set_device(dev)
     _tnames()
     if args.reference_only:
-        runs = reference_experiments(dev, args.steps, args.warmup, args.reference_only.split(","))
+        runs = reference_experiments(dev, args.steps, args.warmup, args.reference_only.split(","),
+                                     all_solvers=args.reference_solvers == "all")
         print(json.dumps({"reference_experiments": runs}), flush=True)
         return
     pb = build_problem(args.N, args.seed)

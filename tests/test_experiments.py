@@ -53,3 +53,37 @@ def test_error_norm_of_the_interpolant_is_small():
     e2 = dg.error_norm(pr, u, 2)
     assert 0 < e2 < 5e-3
     assert dg.error_norm(pr, u, 1) <= e2 * 1.0001  # |Omega| = 1: L1 <= L2
+
+
+def test_bench_runs_every_published_family():
+    """bench.py's reference runs name exactly the published families: the precision
+    experiment's 12 (default and on request) and the parallelism experiment's 30, each with
+    the published solvers."""
+    import json
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    pub = json.load(open(os.path.join(root, "tests", "golden", "dg_published.json")))["solves"]
+    prec = {(r["p"], r["fine"], r["solvers"], r["coarse"]) for r in pub if r["experiment"] == "precisions"}
+    ours = {(p, f"S{fine}", sm, cm) for p, fine, sm, cm in bench.REF_EXPERIMENTS + bench.REF_EXPERIMENTS_LARGE}
+    assert ours == prec
+    par = {(r["solvers"], r["coarse"]) for r in pub if r["experiment"] == "parallelism"}
+    assert set(bench.REF_PARALLELISM) == par and len(bench.REF_PARALLELISM) == 30
+    par_solvers = {(r["kind"], r["precision"], r["inv"]) for r in pub if r["experiment"] == "parallelism"}
+    assert par_solvers == set(bench.PAR_SOLVERS)
+    prec_solvers = {(r["kind"], r["precision"], r["inv"]) for r in pub if r["experiment"] == "precisions"}
+    assert set(bench.REF_SOLVERS) <= prec_solvers
+
+
+def test_bench_all_solver_configurations_are_the_published_ones():
+    import json
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    pub = json.load(open(os.path.join(root, "tests", "golden", "dg_published.json")))["solves"]
+    prec_solvers = {(r["kind"], r["precision"], r["inv"]) for r in pub if r["experiment"] == "precisions"}
+    assert set(bench.REF_SOLVERS_ALL) == prec_solvers and len(bench.REF_SOLVERS_ALL) == 14
