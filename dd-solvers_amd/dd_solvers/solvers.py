@@ -1658,6 +1658,10 @@ class SchwarzOperator(SparseSolver):
     def destroy(self) -> None:
         self.coarse_solver.destroy()
         self.local = None
+        # a sparse local solver holds its factor on the device (BatchSparseCholesky.destroy);
+        # the reference's local solvers have no destroy (solvers.py:642-643)
+        if getattr(self, "_sparse_local", False):
+            self.local_solver.destroy()
 
     @property
     def bytes_per_apply(self) -> int:
