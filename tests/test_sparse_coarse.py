@@ -302,3 +302,21 @@ def test_plan_cache_same_pattern_new_values():
     sc.clear_plan_cache()
     p4, _ = sc.plan_for(lap2d(31, 24), 16, 64)
     assert p4 is not p3
+
+
+def test_item_shapes_on_wide_fronts():
+    """A plan with long coupling sets on levels of few workgroups: forward items split into
+    column parts (R < 256) and backward items narrowed to 32 or 16 columns; every item shape
+    is one the kernels take, and the host restatement over those items solves like SuperLU."""
+    I = sps.identity(24)
+    T = sps.diags([-np.ones(23), 2 * np.ones(24), -np.ones(23)], [-1, 0, 1])
+    A = (sps.kron(sps.kron(T, I), I) + sps.kron(sps.kron(I, T), I)
+         + sps.kron(sps.kron(I, I), T)).tocsr()          # 3D Laplacian: wide separators
+    plan = _check(A, 32, 16)
+    R = np.concatenate([it[:, 10] for it, _ in plan.fwd])
+    C = np.concatenate([it[:, 9] for it, _ in plan.bwd])
+    ns_b = np.concatenate([it[:, 2] for it, _ in plan.bwd])
+    assert set(np.unique(R)) <= {64, 128, 256} and (R < 256).any()
+    assert set(np.unique(C)) <= {16, 32, 64}
+    assert (C[ns_b > 256] == 16).all() and (ns_b > 256).any()
+    assert (C[(ns_b > 128) & (ns_b <= 256)] <= 32).all()
