@@ -1238,8 +1238,9 @@ class SchwarzOperator(SparseSolver):
     def _segment_sum_csr(key: torch.Tensor, val: torch.Tensor, n_rows: int,
                          n_cols: int) -> torch.Tensor:
         """CSR (n_rows x n_cols, FP64) of the sums of val over equal keys row * n_cols + col,
-        deterministic (sort + in-order segment sums); the coarse matrix stays sparse, so coarse
-        spaces of millions of cells fit."""
+        deterministic (stable sort, then compensated in-order segment sums: an entry of A_C
+        sums a whole cell block of A, which nearly cancels); the coarse matrix stays sparse, so
+        coarse spaces of millions of cells fit."""
         order = torch.sort(key, stable=True).indices
         ks = key[order]
         uk, counts = torch.unique_consecutive(ks, return_counts=True)
