@@ -1,20 +1,22 @@
 """Time-to-solution benchmark of Schwarz-preconditioned CG on one MI355X.
 
-Workload (default): 3D Poisson -div(k grad u) = f, Q1 hexahedra on a 64^3 grid
-(n = 63^3 = 250,047 unknowns), k = exp(g) per element with g ~ N(0, 1), f ~ U(-1, 1) per
-unknown (seed 0), Hybrid Schwarz with 512 boxes of 8^3 elements and overlap 1, FP64 local
-inverses, dense FP64 Cholesky coarse solve (n_c = 512), FP64 PCG to rtol 1e-8, x0 = 0.
+Workload (default): 3D Poisson -div(k grad u) = f, Q1 hexahedra on a 128^3 grid
+(n = 127^3 = 2,048,383 unknowns), k = exp(g) per element with g ~ N(0, 1), f ~ U(-1, 1) per
+unknown (seed 0), Hybrid Schwarz with 4096 boxes of 8^3 elements and overlap 1, dense FP64
+Cholesky coarse solve (n_c = 4096), FP64 PCG to rtol 1e-8, x0 = 0.
 
 One step = one complete solve (time to solution), inputs already resident in HBM, setup
-excluded (the reference's protocol). Prints one JSON line: `value` is the plain FP64
-time to solution; beside it the same solve with FP32/FP16/BF16 local inverses, the
-batched form (16 right-hand sides, FP16/BF16 MFMA local inverses), the uneven-subdomain
-batching workload (jittered 2D P1 512^2, 256 k-means subdomains, overlap 2, 16
-right-hand sides), the 128^3 comparison of plain FP64 against the mixed and batched
-forms, and the reference's own DG precision-experiment runs (p = 1 on S10, p = 3 on S9)
-beside its published A100 times.
+excluded (the reference's protocol). Prints one JSON line. `value` is the time to solution
+of the mixed-precision form: local inverses stored in FP16 (scaled per subdomain) and
+applied with FP32 accumulation, PCG recurrences, dots, residual check and coarse solve in
+FP64. Beside it: the plain FP64 solve of the same system and the ratio, the FP32 and BF16
+storage forms, the batched form (16 right-hand sides, FP16 MFMA local inverses), the SpMV;
+the 64^3 system (FP64 and mixed, the BSR block-size sweep b = 2, 4, 8, sparse local
+factors, batched forms), the uneven-subdomain batching workload (jittered 2D P1 512^2,
+256 k-means subdomains, overlap 2, 16 right-hand sides), and the reference's own DG
+precision-experiment runs and kernel-benchmark shapes beside its published A100 times.
 
-    python bench.py [--steps K] [--warmup W] [--N 64] [--no-cpu] [--no-extra] [--no-target]
+    python bench.py [--steps K] [--warmup W] [--N 128] [--no-cpu] [--no-extra]
                     [--no-reference]
 """
 
@@ -56,13 +58,13 @@ def build_problem(N: int, seed: int):
     return pb
 
 
-def make_solver(pb, storage, dev, bsr=None, scaled=False, use_graph=True):
+def make_solver(pb, storage, dev, bsr=None, scaled=False, use_graph=True, accumulate=None):
     import torch
     import dd_solvers as dds
 
     A = torch.sparse_csr_tensor(torch.as_tensor(pb.A.indptr), torch.as_tensor(pb.A.indices),
                                 torch.as_tensor(pb.A.data), pb.A.shape).to(dev)
-    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(storage, scaled=scaled),
+    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(storage, scaled=scaled, accumulate=accumulate),
                                  dds.DenseCholesky()),
                rtol=1e-8, maxiter=2000, bsr_matmul=bsr is not None, check_every=8,
                use_graph=use_graph)
@@ -70,7 +72,7 @@ def make_solver(pb, storage, dev, bsr=None, scaled=False, use_graph=True):
     t = time.perf_counter()
     s.setup(A, decomposition=pb.decomposition, **kw)
     torch.cuda.synchronize()
-    log(f"setup storage={storage} bsr={bsr}: {time.perf_counter() - t:.1f}s")
+    log(f"setup storage={storage} accumulate={accumulate} bsr={bsr}: {time.perf_counter() - t:.1f}s")
     return s
 
 
@@ -233,45 +235,172 @@ def uneven_extra(dev, steps, warmup, seed):
     return out
 
 
-def target_extra(dev, steps, warmup, seed):
-    """The 128^3 comparison: plain FP64 against FP16 local inverses (one right-hand side)
-    and against the batched FP16 MFMA form (16 right-hand sides), plus SpMV bandwidth."""
+def local_kernel_stats(M):
+    """The local inverse apply alone (the dominant kernel of a solve): average launch time
+    from HIP events on the launch stream, with its algorithmic bytes per launch. For the
+    symmetric layout that is sym_apply_kernel, which reads the packed lower triangles and
+    the gathered x (value + index) and writes the partial vectors; sym_combine_kernel reads
+    those back and writes the subdomain-ordered result."""
     import numpy as np
     import torch
 
-    pb = build_problem(128, seed)
+    L = M.local
+    sizes = M._host["sizes"].astype(np.int64)
+    xv = torch.randn(M.n, dtype=torch.float64, device=M.device)
+    Y = M.Y if M.Y is not None else torch.empty_like(xv)
+    if L.layout in ("sym", "sym_mfma"):
+        from dd_solvers.solvers import _sym_written_rows
+        t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None))
+        t_comb = kernel_time_ms(lambda: M._enqueue_sym_combine(Y, None))
+        yp_rows = _sym_written_rows(sizes)
+        alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + yp_rows * 8)
+        comb = int(yp_rows * 8 + sizes.sum() * 8)
+        kname = (f"sym_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
+                 if L.layout == "sym" else
+                 f"sym_mfma_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}> (+ sym_xscale_kernel)")
+    else:
+        t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
+        t_comb, comb = 0.0, 0
+        alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + sizes.sum() * 8)
+        kname = f"local_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
+    gbs = alg / (t_loc * 1e-3) / 1e9
+    return {"kernel": kname, "avg_launch_us": round(t_loc * 1e3, 2), "alg_bytes_per_launch": alg,
+            "stored_matrix_bytes": int(L.bytes_matrices), "GBs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "combine_kernel_us": round(t_comb * 1e3, 2),
+            "combine_alg_bytes": comb}
+
+
+def spmv_stats(op, n, dev):
+    import torch
+
+    xv = torch.randn(n, dtype=torch.float64, device=dev)
+    q = torch.empty(op.shape[0], dtype=torch.float64, device=dev)
+    if op.shape[0] > n:
+        xv = torch.cat([xv, torch.zeros(op.shape[0] - n, dtype=torch.float64, device=dev)])
+    t = kernel_time_ms(lambda: op._enqueue_matvec(xv, q, None, None))
+    gbs = op.bytes_per_matvec / (t * 1e-3) / 1e9
+    out = {"format": type(op).__name__, "us": round(t * 1e3, 2), "bytes": int(op.bytes_per_matvec),
+           "GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    phys = getattr(op, "bytes_physical", None)
+    if phys:
+        out["physical_bytes"] = int(phys)
+        out["physical_hbm_frac"] = round(phys / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
+
+
+def solve_record(pb, s, b, steps, warmup, xh=None):
+    """Time `steps` solves (after `warmup`), then the FP64-verified residual and the error
+    against the FP64 solution `xh`."""
+    import numpy as np
+
+    x, meta, total, times = time_solves(s, b, steps, warmup)
+    xs = x.cpu().numpy()
+    out = {"tts_s": round(total / steps, 6), "min_tts_s": round(min(times), 6),
+           "iterations": int(meta["iterations"]),
+           "true_rel_residual": float(np.linalg.norm(pb.b - pb.A @ xs) / np.linalg.norm(pb.b))}
+    if xh is not None:
+        out["rel_err_vs_fp64"] = float(np.linalg.norm(xs - xh) / np.linalg.norm(xh))
+    return out, xs
+
+
+# the precision forms of the mixed-precision workload: (name, storage, scaled, accumulate)
+PRECISION_FORMS = [("fp16", "float16", True, "float32"), ("bf16", "bfloat16", False, "float32"),
+                   ("fp32", "float32", False, "float32"), ("fp16_fp64acc", "float16", True, None)]
+
+
+def precision_forms(pb, dev, steps, warmup, fp64, xh, forms, use_graph=True):
+    """The same solve with low-precision local inverses: storage, accumulation, TTS,
+    iterations (extra vs FP64), error vs the FP64 solve, local-apply kernel stats."""
+    import torch
+
+    out = {}
+    for name, st, scaled, acc in forms:
+        try:
+            s = make_solver(pb, getattr(torch, st), dev, scaled=scaled, use_graph=use_graph,
+                            accumulate=getattr(torch, acc) if acc else None)
+            r, _ = solve_record(pb, s, torch.as_tensor(pb.b, device=dev), steps, warmup, xh)
+            r.update({"storage": st, "scaled": scaled, "accumulate": acc or "float64",
+                      "ratio_vs_fp64": round(r["tts_s"] / fp64["tts_s"], 4),
+                      "extra_iterations_vs_fp64": r["iterations"] - fp64["iterations"],
+                      "local_kernel": local_kernel_stats(s.preconditioner)})
+            log(f"{name}: TTS {r['tts_s'] * 1e3:.2f} ms ({r['ratio_vs_fp64']}x FP64), "
+                f"{r['iterations']} it, err vs FP64 {r['rel_err_vs_fp64']:.1e}, local "
+                f"{r['local_kernel']['avg_launch_us']} us ({r['local_kernel']['hbm_frac'] * 100:.1f}%)")
+            s.destroy()
+            del s
+        except Exception as e:  # report, keep the measured line
+            r = {"error": str(e)[:200]}
+            log(f"{name}: error {e}")
+        torch.cuda.empty_cache()
+        out[name] = r
+    return out
+
+
+def bsr_sweep(pb, dev, steps, warmup, xh, fp64_tts):
+    """The FP64 solve with the BSR SpMV at block sizes 2, 4, 8 (FastBSR, the reference's
+    CG(bsr_matmul=True)): TTS, stored vs useful bytes and SpMV bandwidth. When b does not
+    divide n the operator is padded and the PCG vectors carry a zero tail."""
+    import torch
+
+    out = []
+    for bs in (2, 4, 8):
+        try:
+            s = make_solver(pb, torch.float64, dev, bsr=bs)
+            r, _ = solve_record(pb, s, torch.as_tensor(pb.b, device=dev), steps, warmup, xh)
+            op = s.matrix
+            st = spmv_stats(op, pb.A.shape[0], dev)
+            csr_bytes = 12 * pb.A.nnz + 8 * (pb.A.shape[0] + 1) + 16 * pb.A.shape[0]
+            st["useful_csr_bytes"] = int(csr_bytes)
+            st["useful_GBs"] = round(csr_bytes / (st["us"] * 1e-6) / 1e9, 1)
+            r.update({"b": bs, "n_padded": int(op.shape[0]), "nnzb": int(op.nnzb),
+                      "ratio_vs_sell": round(r["tts_s"] / fp64_tts, 4), "spmv": st})
+            log(f"BSR b={bs}: TTS {r['tts_s'] * 1e3:.2f} ms, {r['iterations']} it, SpMV {st['us']} us "
+                f"({st['GBs']} GB/s stored, {st['useful_GBs']} GB/s useful)")
+            s.destroy()
+            del s
+        except Exception as e:
+            r = {"b": bs, "error": str(e)[:200]}
+            log(f"BSR b={bs}: error {e}")
+        torch.cuda.empty_cache()
+        out.append(r)
+    return out
+
+
+def workload_64(dev, steps, warmup, seed, use_graph=True):
+    """The 64^3 system (n = 250,047, 512 boxes): FP64 and mixed FP16 forms, the BSR
+    block-size sweep, exact sparse local factors, the batched forms."""
+    import torch
+
+    pb = build_problem(64, seed)
     b = torch.as_tensor(pb.b, device=dev)
-    s = make_solver(pb, torch.float64, dev)
-    x, m, tot, _ = time_solves(s, b, steps, warmup)
-    fp64 = tot / steps
-    xh = x.cpu().numpy()
-    op = s.matrix
-    xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
-    q = torch.empty_like(xv)
-    t_spmv = kernel_time_ms(lambda: op._enqueue_matvec(xv, q, None, None))
-    spmv_gbs = op.bytes_per_matvec / (t_spmv * 1e-3) / 1e9
-    log(f"128^3 FP64 TTS {fp64 * 1e3:.2f} ms, {m['iterations']} it; SpMV {t_spmv * 1e3:.1f} us "
-        f"{spmv_gbs:.0f} GB/s")
+    s = make_solver(pb, torch.float64, dev, use_graph=use_graph)
+    fp64, xh = solve_record(pb, s, b, steps, warmup)
+    fp64["local_kernel"] = local_kernel_stats(s.preconditioner)
+    fp64["spmv"] = spmv_stats(s.matrix, pb.A.shape[0], dev)
+    log(f"64^3 FP64 TTS {fp64['tts_s'] * 1e3:.2f} ms, {fp64['iterations']} it, local "
+        f"{fp64['local_kernel']['avg_launch_us']} us")
+    s.destroy()
     del s
     torch.cuda.empty_cache()
-    s = make_solver(pb, torch.float16, dev, scaled=True)
-    x2, m2, tot2, _ = time_solves(s, b, steps, warmup)
-    x2h = x2.cpu().numpy()
-    mixed = {"tts_s": round(tot2 / steps, 5), "iterations": int(m2["iterations"]),
-             "ratio_vs_fp64": round(tot2 / steps / fp64, 4),
-             "rel_err_vs_fp64": float(np.linalg.norm(x2h - xh) / np.linalg.norm(xh)),
-             "true_rel_residual": float(np.linalg.norm(pb.b - pb.A @ x2h) / np.linalg.norm(pb.b))}
-    log(f"128^3 fp16 TTS {tot2 / steps * 1e3:.2f} ms ({mixed['ratio_vs_fp64']}x)")
-    del s
-    torch.cuda.empty_cache()
-    out = {"workload": ("3D Q1 Poisson 128^3 (n=2048383), lognormal k, Hybrid Schwarz 4096 "
-                        "boxes of 8^3 elements overlap 1, rtol 1e-8"),
-           "fp64_tts_s": round(fp64, 5), "fp64_iterations": int(m["iterations"]),
-           "fp64_true_rel_residual": float(np.linalg.norm(pb.b - pb.A @ xh) / np.linalg.norm(pb.b)),
-           "spmv": {"us": round(t_spmv * 1e3, 2), "bytes": int(op.bytes_per_matvec),
-                    "GBs": round(spmv_gbs, 1), "hbm_frac": round(spmv_gbs / HBM_PEAK_GBS, 4)},
-           "mixed_fp16_single_rhs": mixed,
-           "batched_fp16": batched_extra(pb, torch.float16, dev, steps, warmup, seed, fp64_tts=fp64)}
+    out = {"workload": ("3D Q1 Poisson 64^3 (n=250047), lognormal k, Hybrid Schwarz 512 boxes of "
+                        "8^3 elements overlap 1, dense FP64 Cholesky coarse, rtol 1e-8"),
+           "fp64": fp64}
+    out.update(precision_forms(pb, dev, steps, warmup, fp64, xh, PRECISION_FORMS[:1], use_graph))
+    out["bsr_block_sizes"] = bsr_sweep(pb, dev, steps, warmup, xh, fp64["tts_s"])
+    for name, prec in (("sparse_local_fp64", None), ("sparse_local_fp32_factor", torch.float32)):
+        try:
+            out[name] = sparse_local_extra(pb, dev, steps, warmup, xh, fp64["tts_s"],
+                                           use_graph=use_graph, precision=prec)
+        except Exception as e:
+            out[name] = {"error": str(e)[:200]}
+        torch.cuda.empty_cache()
+    for name, st in (("fp16", torch.float16), ("bf16", torch.bfloat16)):
+        try:
+            out[f"batched_{name}"] = batched_extra(pb, st, dev, steps, warmup, seed,
+                                                   fp64_tts=fp64["tts_s"])
+        except Exception as e:
+            out[f"batched_{name}"] = {"error": str(e)[:200]}
+        torch.cuda.empty_cache()
     return out
 
 
@@ -577,15 +706,19 @@ def kernel_time_ms(fn, reps=20):
     return e0.elapsed_time(e1) / reps
 
 
-def cpu_baseline(pb, iterations, budget_s=20.0):
-    """The oracle (a numpy/scipy restatement of the reference's FP64 Hybrid Schwarz PCG)
-    run on this host on the full workload: its setup (local inverses, coarse Cholesky) is
-    not timed, as on the GPU; its PCG is timed for as many iterations as fit in about
-    `budget_s` seconds (all of them when they fit), and the time to solution is the
-    per-iteration time times the GPU solve's iteration count plus the start-up work."""
+def cpu_baseline(pb, iterations, storage="float16", scaled=True, accumulate="float32",
+                 sample_subdomains=128, seed=0):
+    """The oracle (a numpy/scipy restatement of the reference's Hybrid Schwarz PCG) timed on
+    this host on a bounded sample of the workload. Every piece of one PCG iteration is
+    timed with the oracle's own functions: the SpMV (scipy CSR, single-threaded), R0A y,
+    the coarse Cholesky solve, the prolongation and the fused-free vector updates on full
+    vectors; the local inverse apply on `sample_subdomains` seeded random subdomains
+    (their inverses formed as the oracle forms them: FP64 inverse, quantised to `storage`,
+    scaled per subdomain, multiplied in `accumulate`), scaled up by the stored entries of
+    all subdomains over those of the sample. Time to solution = start-up (T0, first
+    residual) + `iterations` x per-iteration; setup is excluded, as on the GPU."""
     import numpy as np
 
-    from oracle.pcg import ConvergenceNotAchieved, cg
     from oracle.schwarz import SchwarzOracle
 
     try:
@@ -595,28 +728,46 @@ def cpu_baseline(pb, iterations, budget_s=20.0):
         threads = 1
     d = pb.decomposition
     A = pb.A
+    n = A.shape[0]
+    sizes = np.diff(d.sub_ptr).astype(np.int64)
+    rng = np.random.default_rng(seed)
+    sample = np.sort(rng.choice(sizes.shape[0], size=min(sample_subdomains, sizes.shape[0]),
+                                replace=False))
     t = time.perf_counter()
-    O = SchwarzOracle(A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid")
+    O = SchwarzOracle(A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid",
+                      storage=storage, scaled=scaled, accumulate=accumulate, local_subset=sample)
     t_setup = time.perf_counter() - t
-    b = np.asarray(pb.b, dtype=np.float64)
-    # start-up (T0 b, first residual) and one iteration's cost, to size the sample
-    t = time.perf_counter()
-    x0 = O.T0(b)
-    r = b - A @ x0
-    t_start = time.perf_counter() - t
-    t = time.perf_counter()
-    z = O.apply(r)
-    A @ z
-    t_one = time.perf_counter() - t
-    k = int(max(1, min(iterations, budget_s // max(t_one, 1e-9))))
-    t = time.perf_counter()
-    try:
-        _, meta = cg(lambda v: A @ v, b, precond=O.apply, T0=O.T0, rtol=1e-8, maxiter=k)
-        done = meta["iterations"]
-    except ConvergenceNotAchieved as e:
-        done = e.metadata["iterations"] + 1
-    t_run = time.perf_counter() - t
-    per_iter = max(t_run - t_start, 0.0) / max(done, 1)
+
+    def best(fn, reps=3):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    v = np.asarray(pb.b, dtype=np.float64)
+    w = v.copy()
+    t_spmv = best(lambda: A @ v)
+    t_local_sample = best(lambda: O.local(v))
+    scale_up = float((sizes.astype(np.float64) ** 2).sum() / (sizes[sample].astype(np.float64) ** 2).sum())
+    t_local = t_local_sample * scale_up
+    zc = O.R0A @ v
+    t_r0a = best(lambda: O.R0A @ v)
+    t_coarse = best(lambda: O.coarse_solve(zc))
+    yc = O.coarse_solve(zc)
+    t_prolong = best(lambda: v - O.prolong(yc))
+
+    def vec_ops():  # one PCG iteration's vector work (oracle/pcg.py): 2 dots, 3 updates, a norm
+        rho = float(v @ w)
+        p = w + 0.5 * v
+        a = rho / float(p @ v)
+        x = v + a * p
+        r = w - a * p
+        return float(np.sqrt(r @ r)) + x[0]
+    t_vec = best(vec_ops)
+    t_start = best(lambda: A @ O.T0(v)) + t_vec
+    per_iter = t_spmv + t_local + t_r0a + t_coarse + t_prolong + t_vec
     tts = t_start + per_iter * iterations
     del O
     return {
@@ -624,12 +775,74 @@ def cpu_baseline(pb, iterations, budget_s=20.0):
         "unit": "s",
         "cores": int(threads),
         "kind": "port",
-        "sample": (f"oracle FP64 Hybrid Schwarz PCG (numpy/scipy, {threads} BLAS threads) on the "
-                   f"full workload: {done} of {iterations} iterations timed ({t_run:.1f} s, "
-                   f"{per_iter * 1e3:.1f} ms/iteration), time to solution = start-up + "
-                   f"{iterations} x per-iteration; setup excluded ({t_setup:.1f} s); "
-                   f"cpu_count={os.cpu_count()}"),
+        "sample": (f"oracle Hybrid Schwarz PCG (numpy/scipy, {threads} BLAS threads) on the full "
+                   f"{n}-unknown system; per iteration: SpMV {t_spmv * 1e3:.1f} ms (scipy, 1 thread), "
+                   f"local inverses ({storage} storage, scaled={scaled}, {accumulate} products) "
+                   f"timed on {sample.shape[0]} of {sizes.shape[0]} subdomains "
+                   f"({t_local_sample * 1e3:.1f} ms) and scaled by stored entries x{scale_up:.1f} "
+                   f"-> {t_local * 1e3:.0f} ms, R0A {t_r0a * 1e3:.1f} ms, coarse Cholesky solve "
+                   f"{t_coarse * 1e3:.1f} ms, prolongation {t_prolong * 1e3:.1f} ms, vectors "
+                   f"{t_vec * 1e3:.1f} ms; time to solution = start-up {t_start * 1e3:.0f} ms + "
+                   f"{iterations} iterations (the GPU solve's count) x {per_iter * 1e3:.0f} ms; "
+                   f"setup excluded ({t_setup:.1f} s for the sample); cpu_count={os.cpu_count()}"),
     }
+
+
+def profile_kernel_only(args, dev):
+    """Counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): after setup, only the
+    measured kernels, 20 launches each, then a 1 GiB device copy 20 times to calibrate the
+    byte counters."""
+    import numpy as np
+    import torch
+    import dd_solvers as dds
+    from dd_solvers import _lib
+
+    pb = build_problem(args.N, args.seed)
+    st = getattr(torch, args.storage)
+    if args.layout == "sparse":
+        A = torch.sparse_csr_tensor(torch.as_tensor(pb.A.indptr), torch.as_tensor(pb.A.indices),
+                                    torch.as_tensor(pb.A.data), pb.A.shape).to(dev)
+        solver = dds.CG(dds.HybridSchwarz(torch.float64, dds.BatchSparseCholesky(leaf_size=64),
+                                          dds.DenseCholesky()), rtol=1e-8, bsr_matmul=False)
+        solver.setup(A, decomposition=pb.decomposition)
+        M = solver.preconditioner
+        xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
+        Y = M.Y if M.Y is not None else torch.empty_like(xv)
+        torch.cuda.synchronize()
+        for _ in range(20):
+            M._enqueue_local(xv, Y, None)
+        torch.cuda.synchronize()
+        plan = M.local_solver.plan
+        out = {"profile_kernel": True, "layout": "sparse", "heights": int(plan.n_levels),
+               "fronts": int(plan.n_fronts),
+               "factor_bytes": 8 * int(np.sum(plan.fnj * plan.fnj + 2 * plan.fns * plan.fnj))}
+        print(json.dumps(out))
+        return
+    if args.layout == "mfma":
+        solver = make_block_solver(pb, st, dev, use_graph=False, autotune=False)
+    else:
+        acc = torch.float32 if st in (torch.float16, torch.bfloat16, torch.float32) else None
+        solver = make_solver(pb, st, dev, scaled=st == torch.float16, accumulate=acc)
+    M = solver.preconditioner
+    L = M.local
+    xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
+    if args.layout == "mfma":
+        info = mfma_kernel_stats(M)  # 3 + 20 launches, event-timed
+    else:
+        info = local_kernel_stats(M)  # 3 + 20 launches of each kernel, event-timed
+    q = torch.empty_like(xv)
+    for _ in range(20):
+        solver.matrix._enqueue_matvec(xv, q, None, None)
+    torch.cuda.synchronize()
+    src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(20):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    out = {"profile_kernel": True, "N": args.N, "layout": L.layout, "storage": args.storage,
+           "local": info, "spmv_alg_bytes": solver.matrix.bytes_per_matvec,
+           "copy_bytes_each_way": src.numel() * 8}
+    print(json.dumps(out))
 
 
 def main():
@@ -637,11 +850,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--N", type=int, default=64)
+    ap.add_argument("--N", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="only the FP64 line and its kernels")
-    ap.add_argument("--no-target", action="store_true", help="skip the 128^3 comparison")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the headline workload (FP64, the mixed forms, the batched form)")
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the reference's DG precision-experiment runs")
     ap.add_argument("--reference-only", default=None, metavar="RUNS",
@@ -656,10 +869,13 @@ def main():
     ap.add_argument("--profile-kernel", action="store_true",
                     help="after setup only launch the local apply, the SpMV and a 1 GiB device "
                          "copy, 20 times each (for rocprofv3 --pmc byte counters)")
-    ap.add_argument("--storage", default="float64", choices=["float64", "float32", "float16", "bfloat16"])
+    ap.add_argument("--storage", default="float16", choices=["float64", "float32", "float16", "bfloat16"])
     ap.add_argument("--layout", default="sym", choices=["sym", "mfma", "sparse"],
                     help="--profile-kernel: the local-inverse layout to launch (sparse: the "
                          "sweeps of the sparse local factors, BatchSparseCholesky)")
+    ap.add_argument("--solve-only", default=None, choices=["fp64", "fp16"],
+                    help="only set up and time this form of the headline workload (for "
+                         "rocprofv3 --kernel-trace runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -670,247 +886,146 @@ def main():
     import torch
 
     import dd_solvers as dds  # noqa: F401
-    from dd_solvers import _lib
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     _tnames()
+    use_graph = not args.eager
     if args.reference_only:
         runs = reference_experiments(dev, args.steps, args.warmup, args.reference_only.split(","),
                                      all_solvers=args.reference_solvers == "all")
         print(json.dumps({"reference_experiments": runs}), flush=True)
         return
-    pb = build_problem(args.N, args.seed)
-    b = torch.as_tensor(pb.b, device=dev)
-
     if args.profile_kernel:
-        # counter runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): after setup, only the
-        # measured kernels, 20 launches each, then a 1 GiB device copy 20 times to
-        # calibrate the byte counters
-        st = getattr(torch, args.storage)
-        if args.layout == "sparse":
-            A = torch.sparse_csr_tensor(torch.as_tensor(pb.A.indptr), torch.as_tensor(pb.A.indices),
-                                        torch.as_tensor(pb.A.data), pb.A.shape).to(dev)
-            solver = dds.CG(dds.HybridSchwarz(torch.float64, dds.BatchSparseCholesky(leaf_size=64),
-                                              dds.DenseCholesky()), rtol=1e-8, bsr_matmul=False)
-            solver.setup(A, decomposition=pb.decomposition)
-            M = solver.preconditioner
-            xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
-            Y = M.Y if M.Y is not None else torch.empty_like(xv)
-            torch.cuda.synchronize()
-            for _ in range(20):
-                M._enqueue_local(xv, Y, None)
-            torch.cuda.synchronize()
-            plan = M.local_solver.plan
-            out = {"profile_kernel": True, "layout": "sparse", "heights": int(plan.n_levels),
-                   "fronts": int(plan.n_fronts),
-                   "factor_bytes": 8 * int(np.sum(plan.fnj * plan.fnj + 2 * plan.fns * plan.fnj))}
-            log(json.dumps(out))
-            print(json.dumps(out))
-            return
-        if args.layout == "mfma":
-            solver = make_block_solver(pb, st, dev, use_graph=False, autotune=False)
-        else:
-            solver = make_solver(pb, st, dev, scaled=st == torch.float16)
-        M = solver.preconditioner
-        L = M.local
-        sizes = M._host["sizes"].astype(np.int64)
-        xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
-        if args.layout == "mfma":
-            X = torch.rand((pb.A.shape[0], _lib.NRHS), dtype=torch.float64, device=dev)
-            M._enqueue_local_multi(X, M._mfma["Ym"], None)
-            info = mfma_kernel_stats(M)  # 3 + 20 launches, event-timed
-            alg = info["alg_bytes_per_launch"]
-        else:
-            Y = M.Y if M.Y is not None else torch.empty_like(xv)
-            for _ in range(20):
-                M._enqueue_sym_apply(xv, None)
-            for _ in range(20):
-                M._enqueue_sym_combine(Y, None)
-            from dd_solvers.solvers import _sym_written_rows
-            alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + _sym_written_rows(sizes) * 8)
-        q = torch.empty_like(xv)
-        for _ in range(20):
-            solver.matrix._enqueue_matvec(xv, q, None, None)
-        torch.cuda.synchronize()
-        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
-        dst = torch.empty_like(src)
-        for _ in range(20):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        out = {"profile_kernel": True, "layout": L.layout, "storage": args.storage,
-               "local_alg_bytes": alg, "stored_matrix_bytes": L.bytes_matrices,
-               "spmv_alg_bytes": solver.matrix.bytes_per_matvec, "copy_bytes_each_way": src.numel() * 8}
-        log(json.dumps(out))
-        print(json.dumps(out))
+        profile_kernel_only(args, dev)
         return
 
-    # ---- plain FP64 (the measured line) ------------------------------------------------
-    solver = make_solver(pb, torch.float64, dev, use_graph=not args.eager)
-    x, meta, total, times = time_solves(solver, b, args.steps, args.warmup)
-    iters = meta["iterations"]
-    tts = total / args.steps
-    xh = x.cpu().numpy()
-    res = float(np.linalg.norm(pb.b - pb.A @ xh) / np.linalg.norm(pb.b))
-    log(f"FP64 TTS {tts * 1e3:.2f} ms (min {min(times) * 1e3:.2f}), {iters} iterations, "
-        f"true rel. residual {res:.2e}")
+    pb = build_problem(args.N, args.seed)
+    b = torch.as_tensor(pb.b, device=dev)
+    n = pb.A.shape[0]
+    if args.solve_only:
+        st = torch.float64 if args.solve_only == "fp64" else torch.float16
+        s = make_solver(pb, st, dev, scaled=st == torch.float16, use_graph=use_graph,
+                        accumulate=torch.float32 if st == torch.float16 else None)
+        r, _ = solve_record(pb, s, b, args.steps, args.warmup)
+        print(json.dumps({"solve_only": args.solve_only, "N": args.N, **r}), flush=True)
+        return
 
-    # ---- dominant kernel: the local inverse apply ---------------------------------------
-    M = solver.preconditioner
-    L = M.local
-    sizes = M._host["sizes"].astype(np.int64)
-    xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
-    Y = M.Y if M.Y is not None else torch.empty_like(xv)
-    yp = getattr(M, "Yp", None)
-    if L.layout == "sym":
-        # sym_apply_kernel alone: reads the packed lower triangles and the gathered x
-        # (value + index), writes the partial vectors; sym_combine_kernel reads those back
-        # and writes the subdomain-ordered result
-        t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None))
-        t_comb = kernel_time_ms(lambda: M._enqueue_sym_combine(Y, None))
-        from dd_solvers.solvers import _sym_written_rows
-        yp_rows = _sym_written_rows(sizes)   # partial-vector entries written (and combined)
-        alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + yp_rows * 8)
-        comb_bytes = int(yp_rows * 8 + sizes.sum() * 8)
-        kname = f"sym_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
-    else:
-        t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
-        t_comb, comb_bytes = 0.0, 0
-        alg_bytes = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + sizes.sum() * 8)
-        kname = f"local_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
-    gbs = alg_bytes / (t_loc * 1e-3) / 1e9
-    traffic = measured_traffic(f"q1_{args.N}_fp64_{L.layout}", "sym_apply" if L.layout == "sym" else "local_apply")
-    log(f"local apply ({kname}): {t_loc * 1e3:.1f} us, {alg_bytes / 1e9:.3f} GB -> {gbs:.0f} GB/s "
-        f"({gbs / HBM_PEAK_GBS * 100:.1f}% of HBM peak); combine {t_comb * 1e3:.1f} us")
-    # SpMV alone, for the record
-    op = solver.matrix
-    q = torch.empty_like(xv)
-    t_spmv = kernel_time_ms(lambda: op._enqueue_matvec(xv, q, None, None))
-    spmv_bytes = op.bytes_per_matvec
-    log(f"SpMV ({type(op).__name__}): {t_spmv * 1e3:.1f} us, {spmv_bytes / 1e6:.1f} MB -> "
-        f"{spmv_bytes / (t_spmv * 1e-3) / 1e9:.0f} GB/s")
-    # whole-solve algorithmic bytes
-    per_iter = M.bytes_per_apply + spmv_bytes + 9 * 8 * pb.A.shape[0]
-    solve_bytes = per_iter * iters + spmv_bytes
-    solve_gbs = solve_bytes / tts / 1e9
+    # ---- plain FP64 on the same system (the denominator of the ratio) ---------------------
+    s = make_solver(pb, torch.float64, dev, use_graph=use_graph)
+    fp64, xh = solve_record(pb, s, b, args.steps, args.warmup)
+    fp64["local_kernel"] = local_kernel_stats(s.preconditioner)
+    spmv = spmv_stats(s.matrix, n, dev)
+    fp64_iter_bytes = s.preconditioner.bytes_per_apply + s.matrix.bytes_per_matvec + 9 * 8 * n
+    log(f"FP64 TTS {fp64['tts_s'] * 1e3:.2f} ms, {fp64['iterations']} it, residual "
+        f"{fp64['true_rel_residual']:.1e}; local {fp64['local_kernel']['avg_launch_us']} us; SpMV "
+        f"{spmv['us']} us ({spmv['hbm_frac'] * 100:.1f}%)")
+    s.destroy()
+    del s
+    torch.cuda.empty_cache()
+
+    # ---- the mixed-precision form (the measured line): FP16 storage, FP32 accumulation ----
+    s = make_solver(pb, torch.float16, dev, scaled=True, use_graph=use_graph, accumulate=torch.float32)
+    mixed, _ = solve_record(pb, s, b, args.steps, args.warmup, xh)
+    M = s.preconditioner
+    kern = local_kernel_stats(M)
+    iter_bytes = M.bytes_per_apply + s.matrix.bytes_per_matvec + 9 * 8 * n
+    solve_bytes = iter_bytes * mixed["iterations"] + s.matrix.bytes_per_matvec
+    mixed.update({"storage": "float16", "scaled": True, "accumulate": "float32",
+                  "ratio_vs_fp64": round(mixed["tts_s"] / fp64["tts_s"], 4),
+                  "extra_iterations_vs_fp64": mixed["iterations"] - fp64["iterations"]})
+    log(f"mixed FP16 TTS {mixed['tts_s'] * 1e3:.2f} ms ({mixed['ratio_vs_fp64']}x FP64), "
+        f"{mixed['iterations']} it, err vs FP64 {mixed['rel_err_vs_fp64']:.1e}; local "
+        f"{kern['avg_launch_us']} us ({kern['hbm_frac'] * 100:.1f}%)")
+    s.destroy()
+    del s, M
+    torch.cuda.empty_cache()
+    forms = precision_forms(pb, dev, args.steps, args.warmup, fp64, xh, PRECISION_FORMS[1:], use_graph)
+    try:
+        batched = batched_extra(pb, torch.float16, dev, min(args.steps, 3), args.warmup, args.seed,
+                                fp64_tts=fp64["tts_s"])
+    except Exception as e:
+        batched = {"error": str(e)[:200]}
+    torch.cuda.empty_cache()
 
     extra = {}
     if not args.no_extra:
-        # mixed precision: FP16 local inverses (per-subdomain scaled), FP64 everything else
-        del solver
-        torch.cuda.empty_cache()
-        for name, st, scaled in (("fp32", torch.float32, False), ("fp16", torch.float16, True),
-                                 ("bf16", torch.bfloat16, False)):
+        for name, fn in (("q1_64", lambda: workload_64(dev, args.steps, args.warmup, args.seed, use_graph)),
+                         ("batching_uneven", lambda: uneven_extra(dev, args.steps, args.warmup, args.seed)),
+                         ("stream", lambda: stream_peaks(dev)),
+                         ("gemv_reference_benchmark", lambda: gemv_reference_shapes(dev)),
+                         ("bsr_reference_benchmark", lambda: bsr_reference_shapes(dev))):
             try:
-                s2 = make_solver(pb, st, dev, scaled=scaled, use_graph=not args.eager)
-                x2, m2, tot2, t2 = time_solves(s2, b, args.steps, args.warmup)
-                e2 = float(np.linalg.norm(x2.cpu().numpy() - xh) / np.linalg.norm(xh))
-                r2 = float(np.linalg.norm(pb.b - pb.A @ x2.cpu().numpy()) / np.linalg.norm(pb.b))
-                extra[f"mixed_{name}"] = {"tts_s": round(tot2 / args.steps, 5),
-                                          "iterations": m2["iterations"],
-                                          "rel_err_vs_fp64": e2, "true_rel_residual": r2,
-                                          "ratio_vs_fp64": round(tot2 / args.steps / tts, 3)}
-                log(f"{name}: TTS {tot2 / args.steps * 1e3:.2f} ms, {m2['iterations']} it, "
-                    f"err vs fp64 {e2:.1e}")
-                del s2
-                torch.cuda.empty_cache()
+                extra[name] = fn()
             except Exception as e:  # report, keep the measured line
-                extra[f"mixed_{name}"] = {"error": str(e)[:200]}
-        for name, prec in (("sparse_local_fp64", None), ("sparse_local_fp32_factor", torch.float32)):
-            try:
-                extra[name] = sparse_local_extra(pb, dev, args.steps, args.warmup, xh, tts,
-                                                 use_graph=not args.eager, precision=prec)
-            except Exception as e:
                 extra[name] = {"error": str(e)[:200]}
             torch.cuda.empty_cache()
-        for name, st in (("fp16", torch.float16), ("bf16", torch.bfloat16)):
+        if not args.no_reference:
             try:
-                extra[f"batched_{name}"] = batched_extra(pb, st, dev, args.steps, args.warmup,
-                                                         args.seed, fp64_tts=tts)
+                extra["reference_experiments"] = reference_experiments(dev, 3, 1)
             except Exception as e:
-                extra[f"batched_{name}"] = {"error": str(e)[:200]}
-        try:
-            extra["batching_uneven"] = uneven_extra(dev, args.steps, args.warmup, args.seed)
-        except Exception as e:
-            extra["batching_uneven"] = {"error": str(e)[:200]}
-        if not args.no_target and args.N != 128:
-            try:
-                extra["target_128"] = target_extra(dev, min(args.steps, 3), args.warmup, args.seed)
-            except Exception as e:
-                extra["target_128"] = {"error": str(e)[:200]}
-
-    stream = None
-    gemv_ref = bsr_ref = ref_exp = None
-    if not args.no_extra and not args.no_reference:
-        try:
-            ref_exp = reference_experiments(dev, 3, 1)
-        except Exception as e:
-            ref_exp = {"error": str(e)[:200]}
-    if not args.no_extra:
-        try:
-            stream = stream_peaks(dev)
-        except Exception as e:
-            stream = {"error": str(e)[:200]}
-        try:
-            gemv_ref = gemv_reference_shapes(dev)
-        except Exception as e:
-            gemv_ref = {"error": str(e)[:200]}
-        try:
-            bsr_ref = bsr_reference_shapes(dev)
-        except Exception as e:
-            bsr_ref = {"error": str(e)[:200]}
+                extra["reference_experiments"] = {"error": str(e)[:200]}
 
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
-        cpu = cpu_baseline(pb, iters)
+        cpu = cpu_baseline(pb, mixed["iterations"], seed=args.seed)
         log(f"cpu baseline {cpu['value']} s ({time.perf_counter() - t:.1f}s to measure)")
 
+    wl = f"q1_{args.N}_float16_sym"
     out = {
         "metric": "PCG time-to-solution @rtol 1e-8 (s), mixed/batched vs plain FP64, %HBM peak",
-        "value": round(tts, 6),
+        "value": mixed["tts_s"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(tts * 1e3, 4),
+        "ms_per_step": round(mixed["tts_s"] * 1e3, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f16 storage, f32 accumulation, f64 Krylov",
         "data": "synthetic",
         "config": {
-            "workload": f"3D Q1 Poisson {args.N}^3 (n={pb.A.shape[0]}), lognormal k=exp(N(0,1)), "
-                        f"Hybrid Schwarz 8^3-element boxes overlap 1 ({pb.decomposition.n_subdomains} "
-                        "subdomains), FP64 inverses, dense FP64 Cholesky coarse, SELL-64 SpMV, rtol 1e-8",
-            "n": int(pb.A.shape[0]),
+            "workload": f"3D Q1 Poisson {args.N}^3 (n={n}), lognormal k=exp(N(0,1)), Hybrid Schwarz "
+                        f"{pb.decomposition.n_subdomains} boxes of 8^3 elements overlap 1, local inverses "
+                        "stored FP16 (per-subdomain scaled) applied with FP32 accumulation, FP64 PCG "
+                        "vectors/dots/residual, dense FP64 Cholesky coarse, SELL-64 SpMV, rtol 1e-8, "
+                        "x0 = 0, one right-hand side",
+            "n": int(n),
             "nnz": int(pb.A.nnz),
-            "iterations": int(iters),
-            "true_rel_residual": res,
-            "min_tts_s": round(min(times), 6),
+            "subdomains": int(pb.decomposition.n_subdomains),
+            "iterations": mixed["iterations"],
+            "true_rel_residual": mixed["true_rel_residual"],
+            "rel_err_vs_fp64": mixed["rel_err_vs_fp64"],
+            "min_tts_s": mixed["min_tts_s"],
+            "fp64_tts_s": fp64["tts_s"],
+            "fp64_iterations": fp64["iterations"],
+            "ratio_vs_fp64": mixed["ratio_vs_fp64"],
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": kname,
-            "combine_kernel_us": round(t_comb * 1e3, 2),
-            "combine_alg_bytes": comb_bytes,
-            "stored_matrix_bytes": int(L.bytes_matrices),
-            "achieved": round(gbs, 1),
+            "kernel": kern["kernel"],
+            "achieved": kern["GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "alg_bytes_per_launch": alg_bytes,
-            "avg_launch_us": round(t_loc * 1e3, 2),
+            "frac": kern["hbm_frac"],
+            "traffic": measured_traffic(wl, "sym_apply"),
+            "alg_bytes_per_launch": kern["alg_bytes_per_launch"],
+            "avg_launch_us": kern["avg_launch_us"],
+            "stored_matrix_bytes": kern["stored_matrix_bytes"],
+            "combine_kernel_us": kern["combine_kernel_us"],
+            "combine_alg_bytes": kern["combine_alg_bytes"],
         },
-        "stream": stream,
-        "gemv_reference_benchmark": gemv_ref,
-        "bsr_reference_benchmark": bsr_ref,
-        "reference_experiments": ref_exp,
-        "solve_roofline": {"alg_bytes": int(solve_bytes), "achieved_GBs": round(solve_gbs, 1),
-                           "frac": round(solve_gbs / HBM_PEAK_GBS, 4)},
-        "spmv": {"us": round(t_spmv * 1e3, 2), "bytes": int(spmv_bytes),
-                 "GBs": round(spmv_bytes / (t_spmv * 1e-3) / 1e9, 1)},
+        "solve_roofline": {"alg_bytes": int(solve_bytes),
+                           "achieved_GBs": round(solve_bytes / mixed["tts_s"] / 1e9, 1),
+                           "frac": round(solve_bytes / mixed["tts_s"] / 1e9 / HBM_PEAK_GBS, 4)},
+        "fp64": {**fp64, "solve_alg_bytes": int(fp64_iter_bytes * fp64["iterations"]),
+                 "traffic": measured_traffic(f"q1_{args.N}_float64_sym", "sym_apply")},
+        "mixed_fp16": mixed,
+        **{f"mixed_{k}": v for k, v in forms.items()},
+        "batched_fp16": batched,
+        "spmv": spmv,
         "cpu_baseline": cpu,
         **extra,
     }

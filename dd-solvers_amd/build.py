@@ -31,6 +31,11 @@ HIP_FLAGS = [
 ]
 
 
+# Per-file extra flags. local_sym.hip: keep MFMA accumulators in VGPRs (the matrix-core
+# symmetric apply reads and resets them between tiles; in AGPRs each would cost copies).
+FILE_FLAGS = {"local_sym.hip": ["-mllvm", "--amdgpu-mfma-vgpr-form"]}
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and os.path.exists(cand):
@@ -62,7 +67,8 @@ def build(arch: str = "gfx950", jobs: int | None = None, verbose: bool = False) 
     for src in srcs:
         obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
         objs.append(obj)
-        cmds.append([cc, f"--offload-arch={arch}", *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC,
+        cmds.append([cc, f"--offload-arch={arch}", *HIP_FLAGS,
+                     *FILE_FLAGS.get(os.path.basename(src), []), "-I", INCLUDE, "-I", CSRC,
                      "-c", src, "-o", obj])
     jobs = jobs or min(8, len(cmds))
     with ThreadPoolExecutor(max_workers=jobs) as ex:

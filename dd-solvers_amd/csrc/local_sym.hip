@@ -307,6 +307,436 @@ __global__ __launch_bounds__(256) void sym_combine_kernel(const int64_t* __restr
   y[base + a] = acc;
 }
 
+// ---------------------------------------------------------------------------------------
+// Symmetric apply on the matrix cores (16-bit storage, FP32 accumulation).
+//
+// Storage of subdomain s: the same lower triangle of 64 x 64 tiles (tile rows I, tiles
+// J = 0..I in order), each tile kept as its non-zero 16 x 32 blocks (Rb = 0..3 row blocks,
+// K2 = 0..1 column halves) of 512 elements: lane l of block (Rb, K2) holds the 8 elements
+//   T[16 Rb + l%16][32 K2 + 8(l/16) + e],  e < 8,
+// the A operand of a v_mfma_f32_16x16x32 (row m = l%16, k = 8(l/16) + e). Off-diagonal
+// tiles keep blocks q = 2 Rb + K2 for Rb < nb (nb = ceil(R/16) for a tile row of R rows);
+// diagonal tiles keep L + D/2 of the tile (zero above the diagonal), without the two zero
+// blocks (Rb < 2, K2 = 1) and, on the last tile, without blocks past its rL columns:
+//   block index (Rb, K2) = Rb + (nk == 2 ? max(Rb - 2, 0) : 0) + K2,  nk = ceil(C/32).
+//   tile_off(I, J) = 512 (8 I(I-1)/2 + 6 I + 8 J)                          for I < T-1
+//                  = 512 (8 (T-1)(T-2)/2 + 6 (T-1) + 2 nb(rL) J)            for I = T-1
+// Padding is zero.
+//
+// x is taken per subdomain relative to sigma_s = 2^e >= max |x| over S_s (sym_xscale_kernel)
+// and split into NT 16-bit terms (FP16: hi + lo; BF16: hi + mid + lo), which keeps about 22
+// (FP16) or 24 (BF16) bits of x / sigma_s.
+//   Row product y_I += T x_J: D = A(T block) x B, B column n < NT holding term n of x_J.
+//   Transposed product y_J += T^T x_I: D = A' x B', A' row m < NT holding term m of x_I, B'
+//   the tile block in column-major order, read back from an LDS copy of the tile with
+//   ds_read_tr16_b64 (in each group of 16 lanes, lane (r, q) names 4 consecutive elements of
+//   row r; lane c receives element c of the 4 rows).
+// Both products accumulate in FP32 in the MFMA; the NT term columns (rows) are added and
+// scaled back by sigma_s * scale[s] when a block of the result is written.
+// ---------------------------------------------------------------------------------------
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef short i16x4_t __attribute__((__vector_size__(4 * sizeof(short))));
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef short i16x8_t __attribute__((ext_vector_type(8)));
+
+template <typename TM>
+struct SymMfma;
+template <>
+struct SymMfma<_Float16> {
+  static constexpr int NT = 2;
+  typedef f16x8_t op_t;
+  __device__ static __forceinline__ f32x4_t mfma(op_t a, op_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+  // term t of v: the t-th 16-bit piece of the split v = hi + lo
+  __device__ static __forceinline__ uint16_t term(float v, int t) {
+    const _Float16 hi = static_cast<_Float16>(v);
+    const _Float16 q = t == 0 ? hi : static_cast<_Float16>(v - static_cast<float>(hi));
+    uint16_t u;
+    __builtin_memcpy(&u, &q, 2);
+    return u;
+  }
+};
+template <>
+struct SymMfma<bf16_t> {
+  static constexpr int NT = 3;
+  typedef bf16x8_t op_t;
+  __device__ static __forceinline__ f32x4_t mfma(op_t a, op_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ uint32_t rne(float v) {  // bf16 bits of v, widened
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u & 0xFFFF0000u;
+  }
+  __device__ static __forceinline__ uint16_t term(float v, int t) {
+    const uint32_t hi = rne(v);
+    if (t == 0) return static_cast<uint16_t>(hi >> 16);
+    const float r1 = v - __uint_as_float(hi);
+    const uint32_t mid = rne(r1);
+    if (t == 1) return static_cast<uint16_t>(mid >> 16);
+    return static_cast<uint16_t>(rne(r1 - __uint_as_float(mid)) >> 16);
+  }
+};
+
+template <typename TM>
+__device__ __forceinline__ typename SymMfma<TM>::op_t as_op(const Vec16<TM>& v) {
+  typename SymMfma<TM>::op_t o;
+  __builtin_memcpy(&o, &v, 16);
+  return o;
+}
+
+// Pack of x per subdomain, once per apply. sigma_s = 2^e with max_{a in S_s} |x[g(a)]| < 2^e
+// (1 when x is zero on S_s; e clamped to the normal float range) goes to xsc[s]; x / sigma_s
+// is split into NT 16-bit terms stored in MFMA operand order: for each 32-row block c of
+// the subdomain, term t of rows 32 c + j (j < 32) at xt[xt_off[s] + (c NT + t) 32 + j]. A
+// lane l with t = l % 16 < NT then loads its operand (rows 32 c + 8 (l/16) + e, e < 8) as
+// 16 contiguous bytes. Rows past k_s keep the zeros written at setup. One wavefront per
+// subdomain.
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void sym_pack_x_kernel(int64_t n_s,
+                                                         const int64_t* __restrict__ sub_ptr,
+                                                         const int32_t* __restrict__ sub_idx,
+                                                         const int64_t* __restrict__ xt_off,
+                                                         const TV* __restrict__ x,
+                                                         float* __restrict__ xsc,
+                                                         uint16_t* __restrict__ xt,
+                                                         const dds_pcg_state* __restrict__ gate) {
+  using S = SymMfma<TM>;
+  constexpr int NT = S::NT;
+  if (gate != nullptr && gate->done) return;
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= n_s) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t base = sub_ptr[s];
+  const int k = (int)(sub_ptr[s + 1] - base);
+  auto at = [&](int a) -> double {
+    const int64_t g = sub_idx != nullptr ? (int64_t)sub_idx[base + a] : base + a;
+    return static_cast<double>(x[g]);
+  };
+  double m = 0.0;
+  for (int a = lane; a < k; a += 64) m = fmax(m, fabs(at(a)));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  int e = 0;
+  if (m > 0.0) frexp(m, &e);
+  e = e < -126 ? -126 : (e > 127 ? 127 : e);
+  const double inv = ldexp(1.0, -e);  // exact
+  if (lane == 0) xsc[s] = ldexpf(1.0f, e);
+  uint16_t* out = xt + xt_off[s];
+  for (int a = lane; a < k; a += 64) {
+    const float v = static_cast<float>(at(a) * inv);
+    const int c = a >> 5, j = a & 31;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) out[(c * NT + t) * 32 + j] = S::term(v, t);
+  }
+}
+
+// Buffer resource (gfx9 family): 64-bit base, byte range; loads past the range return zero.
+__device__ __forceinline__ i32x4_t buffer_rsrc(const void* base, uint32_t bytes) {
+  const uint64_t b = reinterpret_cast<uint64_t>(base);
+  i32x4_t r;
+  r[0] = static_cast<int>(static_cast<uint32_t>(b));
+  r[1] = static_cast<int>(static_cast<uint32_t>(b >> 32));
+  r[2] = static_cast<int>(bytes);
+  r[3] = 0x00020000;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return static_cast<uint32_t>(
+      reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) const void*)(p)));
+}
+
+__device__ __forceinline__ i32x4_t sgpr4(i32x4_t v) {  // wave-uniform: SGPR operands
+  i32x4_t r;
+  r[0] = __builtin_amdgcn_readfirstlane(v[0]);
+  r[1] = __builtin_amdgcn_readfirstlane(v[1]);
+  r[2] = __builtin_amdgcn_readfirstlane(v[2]);
+  r[3] = __builtin_amdgcn_readfirstlane(v[3]);
+  return r;
+}
+
+// Loads straight into LDS (buffer_load ... lds): lane l's bytes land at m0 + l * width. The
+// compiler does not track these loads, so their completion is awaited explicitly with
+// s_waitcnt vmcnt (vector loads complete in issue order). The LDS reads issued before are
+// awaited first (lgkmcnt(0)), so a stage is never overwritten while it is being read. The
+// 16-byte form streams matrix tiles, read once per apply: non-temporal.
+__device__ __forceinline__ void dma16(uint32_t m0, uint32_t voff, i32x4_t rsrc) {
+  m0 = __builtin_amdgcn_readfirstlane(m0);
+  rsrc = sgpr4(rsrc);
+  asm volatile(
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, %0\n\t"
+      "s_nop 0\n\t"
+      "buffer_load_dwordx4 %1, %2, 0 offen nt lds" ::"s"(m0),
+      "v"(voff), "s"(rsrc)
+      : "memory");
+}
+__device__ __forceinline__ void dma4(uint32_t m0, uint32_t voff, i32x4_t rsrc) {
+  m0 = __builtin_amdgcn_readfirstlane(m0);
+  rsrc = sgpr4(rsrc);
+  asm volatile(
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, %0\n\t"
+      "s_nop 0\n\t"
+      "buffer_load_dword %1, %2, 0 offen lds" ::"s"(m0),
+      "v"(voff), "s"(rsrc)
+      : "memory");
+}
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// Workgroups are dispatched round-robin over the XCDs: this numbers the items so that each
+// XCD takes a contiguous range of them (any grid size).
+__device__ __forceinline__ int64_t xcd_item() {
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const int64_t q = G / kXcds, r = G % kXcds, x = b % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+}
+
+// Stages of the tile pipeline per wavefront (8 KiB each): NS - 1 tiles in flight while one
+// is computed.
+#ifndef DDS_SYM_STAGES
+#define DDS_SYM_STAGES 4
+#endif
+constexpr int kSymStages = DDS_SYM_STAGES;
+constexpr uint32_t kOob = 0x80000000u;  // a buffer offset past every range: loads zeros
+// Largest subdomain of the matrix-core apply: its x terms (NT x 128 B per 64-row block) sit
+// in LDS beside the tile stages.
+constexpr int kSymMfmaMaxK = 4096;
+
+__host__ __device__ inline int64_t symm_store_elems(int k) {  // elements of one subdomain
+  const int T = (k + 63) / 64, rL = k - 64 * (T - 1);
+  const int nbL = (rL + 15) / 16, nkL = (rL + 31) / 32;
+  int ndl = 0;
+  for (int Rb = 0; Rb < nbL; ++Rb)
+    for (int K2 = 0; K2 < nkL; ++K2) ndl += (K2 == 1 && Rb < 2) ? 0 : 1;
+  return 512 * (8 * ((int64_t)(T - 1) * (T - 2) / 2) + 6 * (int64_t)(T - 1) +
+                2 * (int64_t)nbL * (T - 1) + ndl);
+}
+
+// Element offset of tile (I, J) in a subdomain's matrix-core symmetric store.
+__device__ __forceinline__ int64_t symm_tile_off(int I, int J, int T, int nbL) {
+  if (I < T - 1) return 512 * (8 * ((int64_t)I * (I - 1) / 2) + 6 * (int64_t)I + 8 * (int64_t)J);
+  return 512 * (8 * ((int64_t)(T - 1) * (T - 2) / 2) + 6 * (int64_t)(T - 1) + 2 * (int64_t)nbL * J);
+}
+
+// The 8 block loads of tile (I, J) into LDS stage `st` (zeros for absent blocks; every block
+// zero when `real` is false, so every path issues the same count of loads).
+__device__ __forceinline__ void symm_issue(uint32_t st, bool real, int I, int J, int T, int rL,
+                                           int nbL, i32x4_t rm, int lane) {
+  const bool diag = J == I;
+  const int nb = I < T - 1 ? 4 : nbL;
+  const int nk = diag && I == T - 1 ? (rL + 31) / 32 : 2;
+  const uint32_t toff = real ? (uint32_t)(symm_tile_off(I, J, T, nbL) * 2) : 0u;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int Rb = q >> 1, K2 = q & 1;
+    const bool ok = real && Rb < nb && K2 < nk && !(diag && K2 == 1 && Rb < 2);
+    const int idx = diag ? Rb + (nk == 2 ? (Rb > 2 ? Rb - 2 : 0) : 0) + K2 : q;
+    dma16(st + (uint32_t)q * 1024u,
+          ok ? toff + (uint32_t)idx * 1024u + (uint32_t)lane * 16u : kOob, rm);
+  }
+}
+
+// Task t of an item (tiles in order: for J = 0..I2, tile (I2, J), then tile (I1, J) when
+// I1 != I2 and J <= I1) into LDS stage t % NS; past the last task, zeros.
+template <int NS>
+__device__ __forceinline__ void symm_issue_task(int t, int ntasks, int I1, int I2, bool two,
+                                                int T, int rL, int nbL, uint32_t stage0,
+                                                i32x4_t rm, int lane) {
+  int J = 0, w = 0;
+  const bool real = t < ntasks;
+  if (real) {
+    const int pairs = two ? I1 + 1 : 0;
+    if (t < 2 * pairs) {
+      J = t >> 1;
+      w = t & 1;
+    } else {
+      J = t - pairs;
+    }
+  }
+  symm_issue(stage0 + (uint32_t)(t % NS) * 8192u, real, w == 0 ? I2 : I1, J, T, rL, nbL, rm, lane);
+}
+
+template <typename TM, typename TV>
+__global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
+    const int32_t* __restrict__ items, const int64_t* __restrict__ sub_ptr,
+    const int64_t* __restrict__ mat_off, const int64_t* __restrict__ ypos,
+    const int64_t* __restrict__ xt_off, const float* __restrict__ scale,
+    const float* __restrict__ xsc, const uint16_t* __restrict__ xt, const TM* __restrict__ mats,
+    TV* __restrict__ Yp, const dds_pcg_state* __restrict__ gate) {
+  using S = SymMfma<TM>;
+  using op_t = typename S::op_t;
+  constexpr int NT = S::NT;
+  constexpr int NS = kSymStages;
+  constexpr int XB = NT * 128;  // bytes of x terms per 64-row block
+  if (gate != nullptr && gate->done) return;
+  extern __shared__ __align__(16) unsigned char sym_smem[];
+  // [NS stages x 8 blocks x 64 lanes x 16 B][16 zero bytes][yl: 64 floats][x terms]
+  unsigned char* stages = sym_smem;
+  unsigned char* zero16 = sym_smem + NS * 8192;
+  float* yl = reinterpret_cast<float*>(zero16 + 16);
+  unsigned char* xtl = zero16 + 16 + 256;
+
+  const int lane = threadIdx.x;
+  const int64_t item = xcd_item();
+  const int s = items[2 * item];
+  const int p = items[2 * item + 1];
+  const int64_t base = sub_ptr[s];
+  const int k = (int)(sub_ptr[s + 1] - base);
+  const int T = (k + 63) / 64;
+  const int rL = k - 64 * (T - 1);
+  const int nbL = (rL + 15) / 16;
+  const int I1 = p;
+  const int I2 = T - 1 - p;
+  const bool two = I1 != I2;
+  TV* Y = Yp + ypos[s] + (int64_t)p * 64 * T;
+  const double out_sc = static_cast<double>(xsc[s]) *
+                        (scale != nullptr ? static_cast<double>(scale[s]) : 1.0);
+  const i32x4_t rm = buffer_rsrc(mats + mat_off[s], (uint32_t)(symm_store_elems(k) * 2));
+  const i32x4_t rx = buffer_rsrc(xt + xt_off[s], (uint32_t)(((k + 31) / 32) * NT * 64));
+  const uint32_t stage0 = lds_addr(stages);
+
+  if (lane < 4) reinterpret_cast<float*>(zero16)[lane] = 0.f;
+  // x terms of blocks 0..I2 into LDS (zeros past the subdomain), then the first tiles
+  const int xdw = (I2 + 1) * XB / 4;  // dwords
+  for (int d0 = 0; d0 < xdw; d0 += 64) dma4(lds_addr(xtl + 4 * d0), (uint32_t)(d0 + lane) * 4u, rx);
+  // tasks: for J = 0..I2, tile (I2, J), then tile (I1, J) if two and J <= I1
+  const int ntasks = (I2 + 1) + (two ? I1 + 1 : 0);
+#pragma unroll
+  for (int t = 0; t < NS - 1; ++t)
+    symm_issue_task<NS>(t, ntasks, I1, I2, two, T, rL, nbL, stage0, rm, lane);
+  wait_vm<8 * (NS - 1)>();  // the x terms are in
+  wave_lds_fence();
+
+  // operand of x rows 64 J + 32 h + 8 (l/16) + e, term l % 16 (zero past the NT terms): the
+  // B operand of a row product and the A' operand of a transposed product
+  const int tm = lane & 15;
+  auto x_op = [&](int J, int h) __attribute__((always_inline)) -> op_t {
+    const unsigned char* src =
+        tm < NT ? xtl + J * XB + (h * NT + tm) * 64 + 16 * (lane >> 4) : zero16;
+    op_t o;
+    const uint4 v = *reinterpret_cast<const uint4*>(src);
+    __builtin_memcpy(&o, &v, 16);
+    return o;
+  };
+  const op_t a2_0 = x_op(I2, 0), a2_1 = x_op(I2, 1);
+  const op_t a1_0 = x_op(I1, 0), a1_1 = x_op(I1, 1);
+
+  const f32x4_t z4 = {0.f, 0.f, 0.f, 0.f};
+  f32x4_t d2[4] = {z4, z4, z4, z4};  // row product of tile row I2, row blocks 0..3
+  f32x4_t d1[4] = {z4, z4, z4, z4};  // row product of tile row I1
+  f32x4_t dt[4];                     // transposed products into block J, column blocks 0..3
+  float hold1 = 0.f, hold2 = 0.f;    // transposed parts of blocks I1, I2 (lane = row)
+
+  const int g = lane >> 4, r = (lane >> 2) & 3, q4 = lane & 3;
+  // one tile (stage of task t): row product into d, transposed product into dt
+  auto tile = [&](int t, f32x4_t (&d)[4], const op_t& ai0, const op_t& ai1, const op_t& b0,
+                  const op_t& b1) __attribute__((always_inline)) {
+    const unsigned char* sb = stages + (t % NS) * 8192;
+#pragma unroll
+    for (int Rb = 0; Rb < 4; ++Rb) {
+      const uint4 c0 = *reinterpret_cast<const uint4*>(sb + (2 * Rb) * 1024 + lane * 16);
+      const uint4 c1 = *reinterpret_cast<const uint4*>(sb + (2 * Rb + 1) * 1024 + lane * 16);
+      op_t o0, o1;
+      __builtin_memcpy(&o0, &c0, 16);
+      __builtin_memcpy(&o1, &c1, 16);
+      d[Rb] = S::mfma(o0, b0, d[Rb]);
+      d[Rb] = S::mfma(o1, b1, d[Rb]);
+    }
+    // rows 32 kr2 + 8 g + r (+4 h), columns 16 nb + 4 q4, read column-major
+    // (ds_read_tr16_b64) from the lane-ordered blocks
+#pragma unroll
+    for (int kr2 = 0; kr2 < 2; ++kr2)
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        i16x4_t hv[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int rho = 32 * kr2 + 8 * g + r + 4 * h;
+          const int blk = 2 * (rho >> 4) + (nb >> 1);
+          const int ln = (rho & 15) + 16 * (2 * (nb & 1) + (q4 >> 1));
+          hv[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+              (__attribute__((address_space(3))) i16x4_t*)(sb + blk * 1024 + ln * 16 + 8 * (q4 & 1)));
+        }
+        const i16x8_t v = {hv[0][0], hv[0][1], hv[0][2], hv[0][3],
+                           hv[1][0], hv[1][1], hv[1][2], hv[1][3]};
+        op_t bt;
+        __builtin_memcpy(&bt, &v, 16);
+        dt[nb] = S::mfma(kr2 == 0 ? ai0 : ai1, bt, dt[nb]);
+      }
+  };
+  // block J of the result complete: its transposed part, lane = row
+  auto finish_block = [&](int J) __attribute__((always_inline)) {
+    if (lane < 16) {
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        float v = dt[nb][0];
+#pragma unroll
+        for (int t = 1; t < NT; ++t) v += dt[nb][t];
+        yl[16 * nb + lane] = v;
+      }
+    }
+    wave_lds_fence();
+    const float v = yl[lane];
+    wave_lds_fence();
+    if (J == I2) {
+      hold2 = v;
+    } else if (two && J == I1) {
+      hold1 = v;
+    } else if (64 * J + lane < k) {
+      Y[64 * J + lane] = static_cast<TV>(static_cast<double>(v) * out_sc);
+    }
+  };
+
+  int t = 0;
+  for (int J = 0; J <= I2; ++J) {
+    const op_t b0 = x_op(J, 0), b1 = x_op(J, 1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dt[i] = z4;
+    // into the stage of task t - 1, read completely before
+    symm_issue_task<NS>(t + NS - 1, ntasks, I1, I2, two, T, rL, nbL, stage0, rm, lane);
+    wait_vm<8 * (NS - 1)>();  // task t's tile is in
+    tile(t, d2, a2_0, a2_1, b0, b1);
+    ++t;
+    if (two && J <= I1) {
+      symm_issue_task<NS>(t + NS - 1, ntasks, I1, I2, two, T, rL, nbL, stage0, rm, lane);
+      wait_vm<8 * (NS - 1)>();
+      tile(t, d1, a1_0, a1_1, b0, b1);
+      ++t;
+    }
+    finish_block(J);
+  }
+
+  // row products: the NT term columns of D (lanes 16 g + t, t < NT) added; lane 16 g holds
+  // rows 16 Rb + 4 g + i
+  auto store_row = [&](f32x4_t (&d)[4], float hold, int I) __attribute__((always_inline)) {
+#pragma unroll
+    for (int Rb = 0; Rb < 4; ++Rb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float v = d[Rb][i];
+        float v1 = __shfl_down(v, 1, 64);
+        float v2 = NT > 2 ? __shfl_down(v, 2, 64) : 0.f;
+        if ((lane & 15) == 0) yl[16 * Rb + 4 * (lane >> 4) + i] = v + v1 + v2;
+      }
+    }
+    wave_lds_fence();
+    const float v = yl[lane] + hold;
+    wave_lds_fence();
+    if (64 * I + lane < k) Y[64 * I + lane] = static_cast<TV>(static_cast<double>(v) * out_sc);
+  };
+  if (two) store_row(d1, hold1, I1);
+  store_row(d2, hold2, I2);
+  wait_vm<0>();  // no load into LDS outlives the wavefront
+}
+
 template <typename TM, typename TV, typename TA>
 int launch_sym(int64_t n_items, const int32_t* items, const int64_t* sub_ptr,
                const int32_t* sub_idx, const int64_t* mat_off, const int64_t* ypos,
@@ -361,6 +791,47 @@ extern "C" int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, 
 #undef DDS_ARGS
   set_error("dds_local_apply_sym: unsupported dtypes (mat %d, vec %d, acc %d)", mat_dtype,
             vec_dtype, acc_dtype);
+  return DDS_EUNSUPPORTED;
+}
+
+extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s,
+                                        int64_t n_items, int64_t max_k, const int32_t* items,
+                                        const int64_t* sub_ptr, const int32_t* sub_idx,
+                                        const int64_t* mat_off, const int64_t* ypos,
+                                        const int64_t* xt_off, const float* scale,
+                                        const void* mats, const void* x, float* xsc, void* xt,
+                                        void* Yp, const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(n_s >= 0 && n_items >= 0 && n_items <= INT32_MAX && ceil_div(n_s, 4) <= INT32_MAX,
+              "dds_local_apply_sym_mfma: bad sizes");
+  DDS_REQUIRE(max_k >= 0 && max_k <= kSymMfmaMaxK,
+              "dds_local_apply_sym_mfma: subdomains of at most %d unknowns (got %lld)",
+              kSymMfmaMaxK, (long long)max_k);
+  if (n_items == 0 || n_s == 0) return DDS_OK;
+  DDS_REQUIRE(items && sub_ptr && mat_off && ypos && xt_off && mats && x && xsc && xt && Yp,
+              "dds_local_apply_sym_mfma: null pointer");
+  hipStream_t st = as_stream(stream);
+  const dim3 g1((unsigned)ceil_div(n_s, 4)), g2((unsigned)n_items);
+  const int Tmax = (int)ceil_div(max_k, 64);
+#define DDS_SYMM(TM, TV)                                                                       \
+  do {                                                                                         \
+    const size_t lds = (size_t)kSymStages * 8192 + 16 + 256 +                                  \
+                       (size_t)Tmax * SymMfma<TM>::NT * 128 + 256;                             \
+    hipLaunchKernelGGL((sym_pack_x_kernel<TM, TV>), g1, dim3(256), 0, st, n_s, sub_ptr,         \
+                       sub_idx, xt_off, static_cast<const TV*>(x), xsc,                        \
+                       static_cast<uint16_t*>(xt), gate);                                      \
+    DDS_LAUNCHED();                                                                            \
+    hipLaunchKernelGGL((sym_mfma_apply_kernel<TM, TV>), g2, dim3(64), lds, st, items, sub_ptr,  \
+                       mat_off, ypos, xt_off, scale, xsc, static_cast<const uint16_t*>(xt),     \
+                       static_cast<const TM*>(mats), static_cast<TV*>(Yp), gate);               \
+    DDS_LAUNCHED();                                                                            \
+    return DDS_OK;                                                                             \
+  } while (0)
+  if (mat_dtype == DDS_F16 && vec_dtype == DDS_F64) DDS_SYMM(_Float16, double);
+  if (mat_dtype == DDS_F16 && vec_dtype == DDS_F32) DDS_SYMM(_Float16, float);
+  if (mat_dtype == DDS_BF16 && vec_dtype == DDS_F64) DDS_SYMM(bf16_t, double);
+  if (mat_dtype == DDS_BF16 && vec_dtype == DDS_F32) DDS_SYMM(bf16_t, float);
+#undef DDS_SYMM
+  set_error("dds_local_apply_sym_mfma: unsupported dtypes (mat %d, vec %d)", mat_dtype, vec_dtype);
   return DDS_EUNSUPPORTED;
 }
 

@@ -77,8 +77,9 @@ class Q1DiffSolve:
         s.setup(self.matrix(theta), decomposition=self.decomposition)
         self.solver = s
 
-    def _solve_natural(self, b: torch.Tensor) -> tuple[torch.Tensor, dict]:
-        x, meta = self.solver.solve(b.to(torch.float64).contiguous()[self.perm].contiguous())
+    def _solve_natural(self, b: torch.Tensor, solver: CG | None = None) -> tuple[torch.Tensor, dict]:
+        solver = solver or self.solver
+        x, meta = solver.solve(b.to(torch.float64).contiguous()[self.perm].contiguous())
         return x[self.inv_perm], meta
 
     def element_gradient(self, theta, lam, u) -> torch.Tensor:
@@ -101,6 +102,9 @@ class _Q1Solve(torch.autograd.Function):
         op.setup(theta)
         u, meta = op._solve_natural(f)
         ctx.op = op
+        # the solver of this theta: a later forward call replaces op.solver, and this node's
+        # adjoint must still be solved with A(theta) of its own forward
+        ctx.solver = op.solver
         ctx.forward_iterations = meta["iterations"]
         ctx.save_for_backward(theta.detach(), u)
         return u
@@ -110,7 +114,8 @@ class _Q1Solve(torch.autograd.Function):
         op = ctx.op
         theta, u = ctx.saved_tensors
         # adjoint solve with the same operator and preconditioner (A is symmetric)
-        lam, meta = op._solve_natural(grad_u)
+        lam, meta = op._solve_natural(grad_u, ctx.solver)
+        ctx.solver = None
         op.last_adjoint_iterations = meta["iterations"]
         g_theta = op.element_gradient(theta, lam[op.perm].contiguous(), u[op.perm].contiguous())
         return g_theta.to(theta.dtype), lam.to(grad_u.dtype), None

@@ -37,12 +37,7 @@ def segment_sum(vals: torch.Tensor, lengths: torch.Tensor,
     21-million-segment input (the merge of R_C A for the reference's p = 3 S9 / S9 / S9
     runs), so no setup step relies on it there."""
     n = int(lengths.shape[0])
-    if vals.device.type != "cuda":
-        # host tensors (gather_csr on the CPU): torch's plain sums; the compensated merges
-        # belong to the solvers' set-up, which runs on the device
-        if compensated:
-            raise ValueError("compensated segment sums run on the GPU")
-        return torch.segment_reduce(vals, "sum", lengths=lengths)
+    _lib.require_device(vals)
     ptr = torch.zeros(n + 1, dtype=torch.int64, device=vals.device)
     ptr[1:] = torch.cumsum(lengths.to(torch.int64), 0)
     vals = vals.contiguous()
@@ -71,6 +66,7 @@ def csr_to_bsr(matrix: torch.Tensor, block_size: int) -> torch.Tensor:
     if b < 1:
         raise ValueError("block_size must be positive")
     crow, col, val = matrix.crow_indices(), matrix.col_indices(), matrix.values()
+    _lib.require_device(val)
     nrows, ncols = matrix.shape
     dev = val.device
     nbr, nbc = -(-nrows // b), -(-ncols // b)
@@ -97,6 +93,7 @@ def sort_csr(matrix: torch.Tensor, num_parts: int | None = None) -> torch.Tensor
     (cusparse.py:25-71); the result does not depend on it. The input is left intact.
     """
     rows, cols = matrix.shape
+    _lib.require_device(matrix.values())
     crow = matrix.crow_indices().to(torch.int64)
     col = matrix.col_indices().to(torch.int64)
     val = matrix.values()
@@ -124,6 +121,7 @@ def gather_csr(matrix: torch.Tensor) -> torch.Tensor:
     """Merge runs of equal consecutive column indices in each row, summing their values
     (gather_csr_{count,sum}_kernel, cusparse.py:78-158). Index dtypes are preserved."""
     crow, col, val = matrix.crow_indices(), matrix.col_indices(), matrix.values()
+    _lib.require_device(val)
     row = csr_rows(crow)
     nnz = col.shape[0]
     dev = val.device
@@ -152,6 +150,7 @@ def permute_csr_tensor(mat: torch.Tensor, perm: torch.Tensor) -> torch.Tensor:
     """CSR matrix Ap with Ap[perm][:, perm] == mat: entry (i, j) moves to (perm[i], perm[j]).
     Columns come back sorted, indices int32."""
     crow, col, val = mat.crow_indices(), mat.col_indices(), mat.values()
+    _lib.require_device(val)
     perm = perm.to(device=val.device, dtype=torch.int64)
     n = mat.shape[0]
     new_row = perm[csr_rows(crow)]

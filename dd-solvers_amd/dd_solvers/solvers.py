@@ -477,8 +477,11 @@ class CG(SparseSolver):
             self.matrix = _SELLOperator(matrix) if self.spmv_format == "sell" else _CSROperator(matrix)
         else:
             self.matrix = _TorchOperator(matrix)
+        # captured graphs hold the addresses of the previous operator's arrays and buffers
         self._buf = None
         self._graph = None
+        self._bbuf = None
+        self._bgraph = None
         return meta
 
     # -- helpers ----------------------------------------------------------------------
@@ -820,6 +823,8 @@ class CG(SparseSolver):
         self.matrix = None
         self._buf = None
         self._graph = None
+        self._bbuf = None
+        self._bgraph = None
 
 
 # =======================================================================================
@@ -903,7 +908,11 @@ class Inv(DenseBatchSolver):
                   column-major array, applied by the gemv_strided_batched kernels
                   (dds_local_apply_batched). "sym": the lower triangle in 64 x 64 tiles,
                   each off-diagonal tile read once and applied twice
-                  (dds_local_apply_sym); "full": the whole matrix column-major with a
+                  (dds_local_apply_sym); "sym_mfma" (FP16/BF16 storage, chosen by "auto"
+                  above 64 unknowns with an FP32 accumulator): the same triangle in MFMA
+                  operand order, both products on the matrix cores with FP32 accumulation
+                  and x split into 16-bit terms (dds_local_apply_sym_mfma); "full": the
+                  whole matrix column-major with a
                   16-byte leading dimension (dds_local_apply); "mfma" (FP16/BF16): the
                   whole matrix in MFMA operand order, applied to blocks of 16 right-hand
                   sides as a batched GEMM on the matrix cores (dds_mfma_apply; a single
@@ -914,10 +923,10 @@ class Inv(DenseBatchSolver):
 
     def __init__(self, precision: torch.dtype | None = None, accumulate: torch.dtype | None = None,
                  scaled: bool = False, layout: str = "auto", autotune: bool = False):
-        if layout not in ("auto", "batched", "sym", "full", "mfma"):
+        if layout not in ("auto", "batched", "sym", "sym_mfma", "full", "mfma"):
             raise ValueError(f"unknown layout {layout!r}")
-        if layout == "mfma" and precision not in (torch.float16, torch.bfloat16):
-            raise ValueError("layout='mfma' stores the inverses in float16 or bfloat16")
+        if layout in ("mfma", "sym_mfma") and precision not in (torch.float16, torch.bfloat16):
+            raise ValueError(f"layout={layout!r} stores the inverses in float16 or bfloat16")
         self.precision = precision
         self.accumulate = accumulate
         self.scaled = scaled
@@ -1031,6 +1040,46 @@ def _sym_index(k: int, V: int) -> np.ndarray:
     return idx
 
 
+_SYMM_INDEX_CACHE: dict = {}
+
+
+def _symm_index(k: int) -> np.ndarray:
+    """Source of every element of the matrix-core symmetric layout of a k x k matrix
+    (csrc/local_sym.hip, sym_mfma_apply_kernel), with the encoding of _sym_index. Tiles
+    (I, J), J <= I, in order; each keeps its non-zero 16 x 32 blocks (Rb, K2) in that order,
+    lane l of a block holding T[16 Rb + l%16][32 K2 + 8(l/16) + e], e < 8. Diagonal tiles
+    hold their lower triangle with the diagonal halved and skip the blocks above it."""
+    if k in _SYMM_INDEX_CACHE:
+        return _SYMM_INDEX_CACHE[k]
+    T = -(-k // 64)
+    rL = k - 64 * (T - 1)
+    lane = np.arange(64)[:, None]
+    e = np.arange(8)[None, :]
+    parts = []
+    for I in range(T):
+        R = 64 if I < T - 1 else rL
+        nb = -(-R // 16)
+        for J in range(I + 1):
+            diag = J == I
+            nk = -(-R // 32) if diag else 2
+            for Rb in range(nb):
+                for K2 in range(nk):
+                    if diag and K2 == 1 and Rb < 2:
+                        continue
+                    r = 16 * Rb + lane % 16 + 0 * e
+                    c = 32 * K2 + 8 * (lane // 16) + e
+                    row, col = 64 * I + r, 64 * J + c
+                    flat = row * k + col
+                    ok = (row < k) & (col < k) & (~diag | (c <= r))
+                    out = np.where(ok, flat, -1)
+                    if diag:
+                        out = np.where(ok & (c == r), -2 - flat, out)
+                    parts.append(out.reshape(-1))
+    idx = np.concatenate(parts).astype(np.int64)
+    _SYMM_INDEX_CACHE[k] = idx
+    return idx
+
+
 def _sym_plan(sizes: np.ndarray):
     """Work items (subdomain, tile-row pair) and the offsets of their partial vectors for
     the symmetric layout: subdomain s has P_s = ceil(T_s / 2) items, item p covering tile
@@ -1130,6 +1179,13 @@ class SchwarzOperator(SparseSolver):
         self.waves = waves or self.waves
         n = matrix.shape[0]
         self.n = n
+        # scratch vectors sized for the previous operator (n, n_coarse) are dropped; the
+        # apply allocates them again at the new sizes on first use
+        self._xc = self._yc = self._ybuf = None
+        self._mbuf = None
+        self._cast_buf = None
+        self._mfma = None
+        self.Yp = None
         if self.collect_timings:
             events = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
             events[0].record()
@@ -1329,11 +1385,16 @@ class SchwarzOperator(SparseSolver):
         rows_per_tile = _lib._lib.dds_local_tile_rows(_lib.dtype_code(store_dtype))
         layout = getattr(ls, "layout", "full")
         reference_case = sub_idx is None and bool(np.all(sizes == sizes[0]))
+        low16 = store_dtype in (torch.float16, torch.bfloat16)
         if layout == "auto":
             if reference_case and k_max <= 64 and acc == pi:
                 layout = "batched"
+            elif k_max <= 64:
+                layout = "full"
             else:
-                layout = "full" if k_max <= 64 else "sym"
+                layout = "sym_mfma" if low16 and acc == torch.float32 and k_max <= 4096 else "sym"
+        if layout == "sym_mfma" and not (low16 and acc == torch.float32):
+            raise ValueError("layout='sym_mfma' needs float16/bfloat16 storage and a float32 accumulator")
         if layout == "batched" and not reference_case:
             raise ValueError("layout='batched' needs equal, contiguous, non-overlapping subdomains")
         if layout == "batched" and acc != pi:
@@ -1346,8 +1407,9 @@ class SchwarzOperator(SparseSolver):
             mat_off = np.concatenate([[0], np.cumsum([mf_idx[int(k)].shape[0] for k in sizes])]).astype(np.int64)
             mf_idx_dev = {}
             acc = torch.float32
-        elif layout == "sym":
-            sym_idx = {int(k): _sym_index(int(k), width) for k in np.unique(sizes)}
+        elif layout in ("sym", "sym_mfma"):
+            sym_idx = {int(k): (_sym_index(int(k), width) if layout == "sym" else _symm_index(int(k)))
+                       for k in np.unique(sizes)}
             sym_len = np.array([sym_idx[int(k)].shape[0] for k in sizes], dtype=np.int64)
             ld = sizes.copy()
             mat_off = np.concatenate([[0], np.cumsum(sym_len)]).astype(np.int64)
@@ -1409,7 +1471,7 @@ class SchwarzOperator(SparseSolver):
                     src = mf_idx_dev[k]
                     packed = torch.where(src >= 0, m.reshape(len(js), -1)[:, src.clamp(min=0)],
                                          torch.zeros((), dtype=pi, device=dev))
-                elif layout == "sym":
+                elif layout in ("sym", "sym_mfma"):
                     if k not in sym_idx_dev:
                         sym_idx_dev[k] = torch.as_tensor(sym_idx[k], device=dev)
                     src = sym_idx_dev[k]
@@ -1454,10 +1516,17 @@ class SchwarzOperator(SparseSolver):
                 Ym=torch.empty((int(sizes.sum()), _lib.NRHS), dtype=torch.float64, device=dev),
                 grid=0, waves=4)
             alg = int((sizes.astype(np.int64) ** 2).sum()) * esz
-        elif layout == "sym":
+        elif layout in ("sym", "sym_mfma"):
             items, ypos = _sym_plan(sizes)
             tiles = np.zeros((0, 2), dtype=np.int32)
             self.Yp = torch.empty(int(ypos[-1]), dtype=pi, device=dev)
+            if layout == "sym_mfma":  # per apply: power-of-two scales of x, x split in terms
+                nt = 2 if store_dtype == torch.float16 else 3
+                words = 32 * nt * (-(-sizes.astype(np.int64) // 32))
+                self._xsc = torch.ones(n_s, dtype=torch.float32, device=dev)
+                self._xt_off = torch.as_tensor(np.concatenate([[0], np.cumsum(words)[:-1]]).astype(np.int64),
+                                               device=dev)
+                self._xt = torch.zeros(int(words.sum()), dtype=torch.int16, device=dev)
             alg = int((sizes.astype(np.int64) * (sizes + 1) // 2).sum()) * esz
         else:
             tiles = _tiles_for(sizes, rows_per_tile)
@@ -1495,7 +1564,7 @@ class SchwarzOperator(SparseSolver):
         if L.layout == "mfma":
             raise RuntimeError("layout='mfma' applies blocks of right-hand sides (2-D); "
                                "CG routes single vectors through a block of one")
-        if L.layout == "sym":
+        if L.layout in ("sym", "sym_mfma"):
             self._enqueue_sym_apply(x, gate)
             self._enqueue_sym_combine(Y, gate)
             return
@@ -1517,6 +1586,14 @@ class SchwarzOperator(SparseSolver):
     def _enqueue_sym_apply(self, x, gate) -> None:
         """Partial products of every (subdomain, tile-row pair) item into Yp."""
         L = self.local
+        if L.layout == "sym_mfma":
+            _lib.call("dds_local_apply_sym_mfma", _lib.dtype_code(L.mat_dtype),
+                      _lib.dtype_code(self.pi), self.n_solvers, L.n_items, L.max_k,
+                      L.items.data_ptr(), self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx),
+                      L.mat_off.data_ptr(), L.ypos.data_ptr(), self._xt_off.data_ptr(),
+                      _lib.ptr(L.scale), L.mats.data_ptr(), x.data_ptr(), self._xsc.data_ptr(),
+                      self._xt.data_ptr(), self.Yp.data_ptr(), _lib.ptr(gate), _lib.stream())
+            return
         _lib.call("dds_local_apply_sym", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
                   _lib.dtype_code(L.acc_dtype), L.n_items, L.items.data_ptr(),
                   self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), L.mat_off.data_ptr(),
@@ -1655,6 +1732,45 @@ class SchwarzOperator(SparseSolver):
             y = torch.empty_like(x)
             self._enqueue_pi(x, y, None, None, None)
         return y.to(rhs.dtype), {}
+
+    def stored_local_inverses(self) -> list:
+        """The local inverses as the device holds them: for each subdomain the k_i x k_i
+        FP64 matrix of the stored values (storage rounding included) times the subdomain's
+        scale, read back from the packed store. For checks against an exact product."""
+        L = self.local
+        if L is None:
+            raise ValueError("no explicit local inverses (local solver is not Inv)")
+        sizes = self._host["sizes"].astype(np.int64)
+        mats = L.mats.to(torch.float64).cpu().numpy()
+        off = L.mat_off.cpu().numpy()
+        ld = L.ld.cpu().numpy()
+        scale = L.scale.cpu().numpy().astype(np.float64) if L.scale is not None else None
+        width = _lib._lib.dds_local_vec_width(_lib.dtype_code(L.mat_dtype))
+        out = []
+        for i, k in enumerate(sizes):
+            k = int(k)
+            o = int(off[i])
+            if L.layout in ("sym", "sym_mfma"):
+                idx = _sym_index(k, width) if L.layout == "sym" else _symm_index(k)
+                vals = mats[o:o + idx.shape[0]]
+                m = np.zeros(k * k)
+                keep = idx >= 0
+                m[idx[keep]] = vals[keep]
+                half = idx <= -2
+                m[-2 - idx[half]] = 2.0 * vals[half]
+                m = m.reshape(k, k)
+                m = np.tril(m) + np.tril(m, -1).T
+            elif L.layout == "mfma":
+                idx = _mfma_index(k)
+                m = np.zeros(k * k)
+                keep = idx >= 0
+                m[idx[keep]] = mats[o:o + idx.shape[0]][keep]
+                m = m.reshape(k, k)
+            else:  # "full" / "batched": column-major, leading dimension ld
+                l_ = int(ld[i])
+                m = mats[o:o + l_ * k].reshape(k, l_)[:, :k].T.copy()
+            out.append(m * (scale[i] if scale is not None else 1.0))
+        return out
 
     def destroy(self) -> None:
         self.coarse_solver.destroy()

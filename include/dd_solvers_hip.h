@@ -18,6 +18,7 @@
  *   dds_local_apply           Inv.solve -> gemv_strided_batched (solvers.py:355-359),
  *                             generalised to uneven and overlapping subdomains
  *   dds_local_apply_sym       the same on symmetric tile-packed inverses
+ *   dds_local_apply_sym_mfma  the same for FP16/BF16 inverses on the matrix cores
  *   dds_local_apply_batched   the same on the reference's (n, k, k) layout (equal subdomains)
  *   dds_overlap_sum           y += repeat_interleave(...) prolongation (solvers.py:821-825,
  *                             :882-887) fused with the overlapped sum_i R_i^T y_i
@@ -178,6 +179,24 @@ int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_i
                         const int64_t* mat_off, const int64_t* ypos, const float* scale,
                         const void* mats, const void* x, void* Yp, const dds_pcg_state* gate,
                         void* stream);
+
+/* The same partial vectors as dds_local_apply_sym for inverses stored in FP16 or BF16, on the
+ * matrix cores with FP32 accumulation. Storage: the lower triangle of 64 x 64 tiles, each
+ * tile as its non-zero 16 x 32 blocks in MFMA A-operand order (lane l of block (Rb, K2)
+ * holds T[16Rb + l%16][32K2 + 8(l/16) + e], e < 8; diagonal tiles hold L + D/2). Per apply
+ * two kernels run: the first writes, as device scratch, xsc[s] (n_s floats: the power of
+ * two that bounds |x| on subdomain s) and xt (16-bit words: x / xsc[s] split into 2 (FP16)
+ * or 3 (BF16) terms, for each 32-row block c of subdomain s the NT x 32 words at
+ * xt_off[s] + 32 NT c, words past k_s zero and left untouched); the second streams the tiles
+ * into LDS. vec_dtype F64 or F32; max_k (the largest subdomain) at most 4096. Replaces the
+ * same Inv.solve -> gemv_strided_batched step (solvers.py:355-359) as dds_local_apply,
+ * accumulating in FP32 as gemv_kernels.cu:47-54 does for FP32 vectors. */
+int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s, int64_t n_items,
+                             int64_t max_k, const int32_t* items, const int64_t* sub_ptr,
+                             const int32_t* sub_idx, const int64_t* mat_off,
+                             const int64_t* ypos, const int64_t* xt_off, const float* scale,
+                             const void* mats, const void* x, float* xsc, void* xt, void* Yp,
+                             const dds_pcg_state* gate, void* stream);
 
 /* y[sub_ptr[s] + a] = sum of the partial vectors of dds_local_apply_sym that cover local
  * row a of subdomain s (items p = 0 .. T_s-1-a/64), in item order: the result indexed like

@@ -54,7 +54,14 @@ _NP = {"float64": np.float64, "float32": np.float32}
 class SchwarzOracle:
     def __init__(self, A: sps.csr_matrix, owner_ptr, sub_ptr, sub_idx, solvers_per_coarse=None,
                  kind: str = "additive", precision: str = "float64", storage: str | None = None,
-                 accumulate: str | None = None, scaled: bool = False, coarse: bool = True):
+                 accumulate: str | None = None, scaled: bool = False, coarse: bool = True,
+                 local_subset=None, inverses=None):
+        """local_subset: indices of the subdomains whose inverses are formed (default all).
+        With a subset, local() sums only those subdomains: a timing sample of the local
+        apply on a workload too large to invert whole on a host, never a preconditioner.
+        inverses: the k_i x k_i matrices to apply as the local inverses, as given (no
+        inversion, storage rounding or scaling here): the exact product of a solver's own
+        stored inverses, against which its kernels' arithmetic alone is measured."""
         self.kind = kind
         self.pi = _NP[precision]
         self.storage = storage or precision
@@ -75,8 +82,13 @@ class SchwarzOracle:
         # Local inverses: inverse in the preconditioner precision, then the storage cast.
         self.inv = []
         self.scale = []
-        for i in range(n_s):
+        self.local_subset = None if local_subset is None else np.asarray(local_subset, np.int64)
+        for i in (range(n_s) if local_subset is None else self.local_subset):
             idx = self.sub_idx[self.sub_ptr[i]:self.sub_ptr[i + 1]]
+            if inverses is not None:
+                self.inv.append((idx, np.asarray(inverses[i], dtype=np.float64).astype(self.acc)))
+                self.scale.append(1.0)
+                continue
             Ai = A[idx][:, idx].toarray().astype(self.pi)
             Ainv = np.linalg.inv(Ai).astype(np.float64)
             s = 1.0

@@ -95,3 +95,26 @@ def test_adjoint_48_against_direct_gradient_and_finite_differences():
     ad = dirs @ g
     fd = gd["fd"]
     assert np.all(np.abs(ad - fd) <= 1e-6 * np.abs(fd).max()), np.abs(ad - fd).max() / np.abs(fd).max()
+
+
+def test_two_forwards_then_one_backward():
+    """Two solves at different theta before one backward (the gradient of J1 + J2): each
+    node's adjoint runs with the operator of its own forward, checked against the sum of
+    the two FP64 direct-solve gradients."""
+    N = 12
+    theta1, theta_obs, f = _inputs(N, seed=0)
+    theta2, _, _ = _inputs(N, seed=5)
+    u_obs = spla.spsolve(P.assemble_q1_3d(N, np.exp(theta_obs).reshape(N, N, N)).tocsc(), f)
+    _, g1, _, _ = objective_and_gradient(N, theta1, f, u_obs)
+    _, g2, _, _ = objective_and_gradient(N, theta2, f, u_obs)
+    op = Q1DiffSolve(N, box=4, overlap=1, storage=torch.float64, rtol=1e-13)
+    t1 = torch.as_tensor(theta1, device=dev).requires_grad_(True)
+    t2 = torch.as_tensor(theta2, device=dev).requires_grad_(True)
+    uo = torch.as_tensor(u_obs, device=dev)
+    ft = torch.as_tensor(f, device=dev)
+    u1 = op(t1, ft)
+    u2 = op(t2, ft)  # replaces op.solver before either backward runs
+    J = 0.5 * ((u1 - uo) ** 2).sum() + 0.5 * ((u2 - uo) ** 2).sum()
+    J.backward()
+    assert np.linalg.norm(t1.grad.cpu().numpy() - g1) <= 1e-8 * np.linalg.norm(g1)
+    assert np.linalg.norm(t2.grad.cpu().numpy() - g2) <= 1e-8 * np.linalg.norm(g2)

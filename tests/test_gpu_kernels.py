@@ -170,22 +170,35 @@ def _torch_csr(A):
 
 @pytest.mark.parametrize("storage", [torch.float64, torch.float32, torch.float16, torch.bfloat16])
 @pytest.mark.parametrize("overlap", [0, 1])
-@pytest.mark.parametrize("layout,waves", [("full", 1), ("full", 4), ("full", 8), ("sym", None)])
+@pytest.mark.parametrize("layout,waves", [("full", 1), ("full", 4), ("full", 8), ("sym", None),
+                                          ("sym_mfma", None)])
 @pytest.mark.parametrize("N,box", [(14, 4), (18, 8)])
 def test_local_apply_matches_oracle(storage, overlap, layout, waves, N, box):
     # (18, 8): subdomains of 8^3 + overlap -> k up to 1000, several 64-row tiles
+    if layout == "sym_mfma" and storage not in (torch.float16, torch.bfloat16):
+        pytest.skip("the matrix-core layout stores FP16 or BF16")
+    print(f"local apply: storage {storage} overlap {overlap} layout {layout} N {N} box {box}", flush=True)
     pb = P.make_problem("q1_3d", N, box=box, overlap=overlap, seed=2, lognormal=True)
     d = pb.decomposition
     scaled = storage == torch.float16
-    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(storage, scaled=scaled, layout=layout),
-                            dds.DenseCholesky())
+    acc = torch.float32 if layout == "sym_mfma" else None
+    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(storage, scaled=scaled, layout=layout,
+                                                   accumulate=acc), dds.DenseCholesky())
     M.setup(_torch_csr(pb.A), decomposition=d, waves=waves)
     assert M.local.layout == layout
     sname = {torch.float64: "float64", torch.float32: "float32", torch.float16: "float16",
              torch.bfloat16: "bfloat16"}[storage]
-    O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, storage=sname, scaled=scaled)
     r = np.random.default_rng(0).standard_normal(pb.A.shape[0])
     y = M._local_only(torch.as_tensor(r, device=dev)).cpu().numpy()
+    # 1. the kernel's arithmetic: the exact FP64 product of the inverses the device stores.
+    # FP64 accumulation leaves rounding only; the matrix-core apply keeps ~22 (FP16) or ~24
+    # (BF16) bits of x / sigma_s and accumulates in FP32.
+    Oe = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, inverses=M.stored_local_inverses())
+    ye = Oe.local(r)
+    rel_exact = np.linalg.norm(y - ye) / np.linalg.norm(ye)
+    assert rel_exact < (2e-6 if layout == "sym_mfma" else 1e-13), rel_exact
+    # 2. the whole set-up: inversion and storage rounding against the oracle's own inverses
+    O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, storage=sname, scaled=scaled)
     yo = O.local(r)
     rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
     tol = {torch.float64: 1e-12, torch.float32: 1e-6, torch.float16: 2e-3, torch.bfloat16: 1e-2}[storage]

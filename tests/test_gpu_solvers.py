@@ -167,11 +167,70 @@ def test_q1_3d_full_layout_matches_oracle():
 
 @pytest.mark.parametrize("b", [2, 4, 8])
 def test_q1_3d_bsr_block_sizes(b):
-    # 24 elements -> 23 unknowns per axis: BSR pads the last block row
+    # n = 24^3: every b divides n, no padding
     pb = P.make_problem("q1_3d", 25, box=8, overlap=1, seed=3, lognormal=True)
-    if pb.A.shape[0] % b:
-        pytest.skip("BSR needs n divisible by b")
+    assert pb.A.shape[0] % b == 0
     _parity(pb, "hybrid", bsr=b)
+
+
+@pytest.mark.parametrize("b", [2, 4, 8])
+def test_q1_3d_bsr_padded_tail(b):
+    """n = 23^3 = 12167, which none of b = 2, 4, 8 divides (as 63^3 at 64^3): the operator is
+    padded to a whole number of block rows and the PCG vectors carry a zero tail. Iterates,
+    iteration count and the residual history against the oracle."""
+    pb = P.make_problem("q1_3d", 24, box=8, overlap=1, seed=3, lognormal=True)
+    n = pb.A.shape[0]
+    assert n % b != 0
+    d = pb.decomposition
+    O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid")
+    xo, mo = oracle_cg(lambda v: pb.A @ v, pb.b, precond=O.apply, T0=O.T0, rtol=1e-8)
+    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(), dds.DenseCholesky()), rtol=1e-8)
+    s.setup(_torch_csr(pb.A), decomposition=d, block_size=b)
+    assert s.matrix.shape[0] == -(-n // b) * b > n
+    x, m = s.solve(torch.as_tensor(pb.b, device=dev))
+    assert x.shape[0] == n
+    assert abs(m["iterations"] - mo["iterations"]) <= 1
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
+    k_ = min(len(m["residual norms"]), len(mo["residual norms"]))
+    assert np.allclose(m["residual norms"][:k_], mo["residual norms"][:k_], rtol=1e-8, atol=0)
+    # the padded tail of the working vectors stays zero
+    assert not s._buf.x[n:].any() and not s._buf.r[n:].any()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_setup_again_with_larger_problem(use_graph):
+    """One CG + HybridSchwarz object set up on a small problem, solved, then set up on a
+    larger one (larger n and n_c) and solved again, single right-hand side and blocks of
+    16: every buffer and captured graph follows the new sizes (results against the oracle)."""
+    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(torch.float16, scaled=True, layout="mfma"),
+                                 dds.DenseCholesky()), rtol=1e-8, bsr_matmul=False,
+               check_every=4, use_graph=use_graph)
+    for N, box in ((10, 3), (16, 4)):
+        pb = P.make_problem("q1_3d", N, box=box, overlap=1, seed=N, lognormal=True)
+        d = pb.decomposition
+        s.setup(_torch_csr(pb.A), decomposition=d)
+        O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid",
+                          inverses=s.preconditioner.stored_local_inverses())
+        xo, mo = oracle_cg(lambda v: pb.A @ v, pb.b, precond=O.apply, T0=O.T0, rtol=1e-8)
+        x, m = s.solve(torch.as_tensor(pb.b, device=dev))
+        assert abs(m["iterations"] - mo["iterations"]) <= 2
+        assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-6 * np.linalg.norm(xo)
+        B = np.random.default_rng(N).uniform(-1, 1, (pb.A.shape[0], 16))
+        X, mb = s.solve(torch.as_tensor(B, device=dev))
+        res = np.linalg.norm(B - pb.A @ X.cpu().numpy(), axis=0) / np.linalg.norm(B, axis=0)
+        assert res.max() < 1.01e-8
+    # and the plain FP64 preconditioner on a new object graph, set up twice the same way
+    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(), dds.DenseCholesky()), rtol=1e-8,
+               bsr_matmul=False, check_every=4, use_graph=use_graph)
+    for N, box in ((10, 3), (16, 4)):
+        pb = P.make_problem("q1_3d", N, box=box, overlap=1, seed=N, lognormal=True)
+        d = pb.decomposition
+        s.setup(_torch_csr(pb.A), decomposition=d)
+        O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid")
+        xo, mo = oracle_cg(lambda v: pb.A @ v, pb.b, precond=O.apply, T0=O.T0, rtol=1e-8)
+        x, m = s.solve(torch.as_tensor(pb.b, device=dev))
+        assert abs(m["iterations"] - mo["iterations"]) <= 1
+        assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
 
 
 def test_graph_replay_matches_eager():
@@ -185,5 +244,30 @@ def test_graph_replay_matches_eager():
 @pytest.mark.parametrize("storage", ["float32", "float16", "bfloat16"])
 def test_low_precision_inverses(storage):
     pb = P.make_problem("q1_3d", 16, box=4, overlap=1, seed=4, lognormal=True)
-    # iterations may differ more with quantised inverses; the solution still meets rtol
+    # iterations may differ more with inverses rounded independently on each side; the
+    # solution still meets rtol
     _parity(pb, "hybrid", storage=storage, it_tol=2, x_tol=1e-6)
+
+
+@pytest.mark.parametrize("storage,accumulate,layout", [
+    (torch.float32, None, "sym"), (torch.float16, None, "sym"), (torch.bfloat16, None, "sym"),
+    (torch.float16, torch.float32, "sym_mfma"), (torch.bfloat16, torch.float32, "sym_mfma")])
+def test_low_precision_against_stored_inverses(storage, accumulate, layout):
+    """The oracle applies exactly the inverses the device stores (rounding and scale
+    included). With FP64 accumulation the two preconditioners differ by rounding only, so
+    the FP64 bar holds (iterates 1e-10, iterations +-1); the matrix-core apply keeps ~22-24
+    bits of x and accumulates in FP32 (iterates 1e-6, iterations +-1)."""
+    pb = P.make_problem("q1_3d", 18, box=8, overlap=1, seed=4, lognormal=True)
+    d = pb.decomposition
+    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(storage, scaled=storage == torch.float16,
+                                                        accumulate=accumulate, layout=layout),
+                                 dds.DenseCholesky()), rtol=1e-8, bsr_matmul=False)
+    s.setup(_torch_csr(pb.A), decomposition=d)
+    assert s.preconditioner.local.layout == layout
+    O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid",
+                      inverses=s.preconditioner.stored_local_inverses())
+    xo, mo = oracle_cg(lambda v: pb.A @ v, pb.b, precond=O.apply, T0=O.T0, rtol=1e-8)
+    x, m = s.solve(torch.as_tensor(pb.b, device=dev))
+    assert abs(m["iterations"] - mo["iterations"]) <= 1
+    err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+    assert err <= (1e-6 if layout == "sym_mfma" else 1e-10), err

@@ -101,44 +101,19 @@ def test_cpu_tensors_rejected():
 
 # ---- setup utilities (known answers of src/dd_solvers/tests/test_bsr.py / test_cusparse.py)
 
-@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-def test_csr_to_bsr_known_answer(dtype):
-    A = torch.sparse_csr_tensor(torch.tensor([0, 2, 3, 5, 9]), torch.tensor([0, 1, 1, 0, 2, 0, 1, 2, 3]),
-                                torch.tensor([1, 2, 3, 4, 5, 6, 7, 8, 9], dtype=dtype))
-    B = csr_to_bsr(A, 2)
-    assert B.layout == torch.sparse_bsr and B.dtype == dtype
-    assert B.crow_indices().tolist() == [0, 1, 3]
-    assert B.col_indices().tolist() == [0, 0, 1]
-    assert B.values().tolist() == [[[1, 2], [0, 3]], [[4, 0], [6, 7]], [[5, 0], [8, 9]]]
-    assert B.values().stride() == (4, 1, 2)
-
-
-@pytest.mark.parametrize("ind_dtype", [torch.int32, torch.int64])
-@pytest.mark.parametrize("num_parts", [1, 2, 3, 4, None])
-def test_sort_csr_known_answer(ind_dtype, num_parts):
-    M = torch.sparse_csr_tensor(torch.tensor([0, 2, 3, 5, 9], dtype=ind_dtype),
-                                torch.tensor([1, 0, 1, 0, 2, 2, 3, 1, 0], dtype=ind_dtype),
+def test_csr_setup_helpers_reject_host_tensors():
+    """csr_to_bsr, sort_csr, gather_csr and permute_csr_tensor run on the device, as the
+    reference's cuSPARSE / numba kernels do: host tensors are refused, not computed on the
+    CPU. Their known answers run on the GPU (tests/test_gpu_setup_helpers.py)."""
+    M = torch.sparse_csr_tensor(torch.tensor([0, 2, 3, 5, 9]), torch.tensor([1, 0, 1, 0, 2, 2, 3, 1, 0]),
                                 torch.tensor([2, 1, 3, 4, 5, 8, 9, 7, 6], dtype=torch.float64))
-    S = sort_csr(M, num_parts=num_parts)
-    assert torch.equal(S.crow_indices(), torch.tensor([0, 2, 3, 5, 9], dtype=torch.int32))
-    assert torch.equal(S.col_indices(), torch.tensor([0, 1, 1, 0, 2, 0, 1, 2, 3], dtype=torch.int32))
-    assert S.values().tolist() == [1, 2, 3, 4, 5, 6, 7, 8, 9]
+    for fn in (lambda: csr_to_bsr(M, 2), lambda: sort_csr(M), lambda: gather_csr(M),
+               lambda: permute_csr_tensor(M, torch.arange(4))):
+        with pytest.raises(ValueError, match="HIP device"):
+            fn()
 
 
-def test_gather_csr_known_answers():
-    G = gather_csr(torch.sparse_csr_tensor(torch.tensor([0, 3, 6], dtype=torch.int32),
-                                           torch.tensor([0, 2, 2, 0, 0, 1], dtype=torch.int32),
-                                           torch.tensor([1.0, 2, 3, 4, 5, 6])))
-    assert G.crow_indices().tolist() == [0, 2, 4] and G.col_indices().tolist() == [0, 2, 0, 1]
-    assert G.values().tolist() == [1.0, 5.0, 9.0, 6.0]
-    G = gather_csr(torch.sparse_csr_tensor(torch.tensor([0, 2, 2, 5], dtype=torch.int32),
-                                           torch.tensor([0, 1, 0, 0, 1], dtype=torch.int32),
-                                           torch.tensor([1.0, 2, 3, 4, 5])))
-    assert G.crow_indices().tolist() == [0, 2, 2, 4] and G.col_indices().tolist() == [0, 1, 0, 1]
-    assert G.values().tolist() == [1.0, 2.0, 7.0, 5.0]
-
-
-def test_asm_permutation_and_permute_dg24():
+def test_asm_permutation_dg24():
     A, b, f2s, s2c, k = dg24()
     f2d = torch.arange(24).reshape(8, 3)
     perm = asm_permutation(torch.as_tensor(f2s), torch.as_tensor(s2c), f2d)
@@ -148,10 +123,7 @@ def test_asm_permutation_and_permute_dg24():
     ref = sd[np.argsort(s2c, kind="stable")].reshape(-1)
     assert perm.tolist() == ref.tolist()
     inv = inverse_permutation(perm)
-    At = torch.sparse_csr_tensor(torch.as_tensor(A.indptr), torch.as_tensor(A.indices),
-                                 torch.as_tensor(A.data), A.shape)
-    Ap = permute_csr_tensor(At, inv)
-    assert np.allclose(Ap.to_dense().numpy(), A.toarray()[ref][:, ref])
+    assert inv[perm].tolist() == list(range(24))
 
 
 # ---- problem generators ------------------------------------------------------------------
@@ -337,6 +309,131 @@ def test_sym_layout_and_item_plan(k, V):
         y[64 * I2:64 * I2 + 64] = rows[I2]
         Yp[ypos[s_] + p * 64 * T: ypos[s_] + (p + 1) * 64 * T] = y
     # combine (sym_combine_kernel): row a sums items p = 0 .. min(P-1, T-1-a//64)
+    P = -(-T // 2)
+    z = np.array([sum(Yp[ypos[0] + p * 64 * T + a] for p in range(min(P - 1, T - 1 - a // 64) + 1))
+                  for a in range(k)])
+    assert not np.isnan(z).any()
+    assert np.allclose(z, A @ x, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("k", [1, 8, 20, 33, 64, 65, 100, 130, 200, 1000])
+def test_symm_layout_model(k):
+    """The matrix-core symmetric layout (_symm_index) read the way sym_mfma_apply_kernel reads
+    it: tile offsets and block indices from the kernel's closed forms, the 16x16x32 MFMA lane
+    rules (A: row l%16, k = 8(l/16)+e; B: column l%16, k = 8(l/16)+e; D: rows 4(l/16)+i,
+    column l%16), the LDS copy with its row pitch and the ds_read_tr16_b64 lane rule (in each
+    group of 16 lanes, lane (r, q) names 4 elements of a row; lane c receives element c%4 of
+    lane (i, c//4) for i < 4). Items and the combine then reproduce A x exactly."""
+    from dd_solvers.solvers import _sym_plan, _symm_index
+
+    rng = np.random.default_rng(k)
+    B = rng.standard_normal((k, k))
+    A = B + B.T
+    idx = _symm_index(k)
+    flat = A.reshape(-1)
+    packed = np.where(idx >= 0, flat[np.maximum(idx, 0)],
+                      np.where(idx <= -2, 0.5 * flat[np.maximum(-2 - idx, 0)], 0.0))
+    T = -(-k // 64)
+    rL = k - 64 * (T - 1)
+    nbL = -(-rL // 16)
+    last = 512 * (8 * ((T - 1) * (T - 2) // 2) + 6 * (T - 1) + 2 * nbL * (T - 1))
+    nkL = -(-rL // 32)
+    n_diag_last = sum(1 for Rb in range(nbL) for K2 in range(nkL) if not (K2 == 1 and Rb < 2))
+    assert idx.shape[0] == last + 512 * n_diag_last
+
+    def tile_off(I, J):
+        if I < T - 1:
+            return 512 * (8 * (I * (I - 1) // 2) + 6 * I + 8 * J)
+        return 512 * (8 * ((T - 1) * (T - 2) // 2) + 6 * (T - 1) + 2 * nbL * J)
+
+    def blocks(I, J):  # the kernel's issue(): 8 operand registers per lane, absent ones zero
+        diag = J == I
+        nb = 4 if I < T - 1 else nbL
+        nk = -(-rL // 32) if diag and I == T - 1 else 2
+        c = np.zeros((8, 64, 8))
+        for q in range(8):
+            Rb, K2 = q >> 1, q & 1
+            if Rb < nb and K2 < nk and not (diag and K2 == 1 and Rb < 2):
+                bi = Rb + (max(Rb - 2, 0) if nk == 2 else 0) + K2 if diag else q
+                o = tile_off(I, J) + bi * 512
+                c[q] = packed[o:o + 512].reshape(64, 8)
+        return c
+
+    P_ = 72  # kSymLdsPitch
+    lanes = np.arange(64)
+    xp = np.zeros(64 * T)
+    x = rng.standard_normal(k)
+    xp[:k] = x
+
+    def row_product(c, xJ):
+        D = np.zeros((4, 16, 16))
+        for Rb in range(4):
+            for K2 in range(2):
+                Am = np.zeros((16, 32))
+                Bm = np.zeros((32, 16))
+                for l in lanes:
+                    Am[l % 16, 8 * (l // 16):8 * (l // 16) + 8] = c[2 * Rb + K2][l]
+                    if l % 16 == 0:
+                        Bm[8 * (l // 16):8 * (l // 16) + 8, 0] = xJ[32 * K2 + 8 * (l // 16):32 * K2 + 8 * (l // 16) + 8]
+                D[Rb] += Am @ Bm
+        # D lane rule: rows 4(l/16)+i of column 0 sit in lane 16 g, i.e. D[Rb][:, 0]
+        return np.concatenate([D[Rb][:, 0] for Rb in range(4)])
+
+    def transposed_product(c, xI):
+        lds = np.full(64 * P_, np.nan)
+        for q in range(8):
+            for l in lanes:
+                row = 16 * (q >> 1) + l % 16
+                col = 32 * (q & 1) + 8 * (l // 16)
+                lds[row * P_ + col:row * P_ + col + 8] = c[q][l]
+        out = np.zeros(64)
+        for nb in range(4):
+            D = np.zeros((16, 16))
+            for kr2 in range(2):
+                def tr(extra):
+                    addr = np.array([(32 * kr2 + 8 * (l // 16) + (l >> 2 & 3)) * P_ + 16 * nb
+                                     + 4 * (l & 3) + extra for l in lanes])
+                    got = np.zeros((64, 4))
+                    for l in lanes:
+                        g, cc = l // 16, l % 16
+                        for i in range(4):
+                            got[l, i] = lds[addr[16 * g + 4 * i + cc // 4] + cc % 4]
+                    return got
+                bt = np.concatenate([tr(0), tr(4 * P_)], axis=1)
+                Am = np.zeros((16, 32))
+                Bm = np.zeros((32, 16))
+                for l in lanes:
+                    if l % 16 == 0:
+                        Am[0, 8 * (l // 16):8 * (l // 16) + 8] = xI[32 * kr2 + 8 * (l // 16):32 * kr2 + 8 * (l // 16) + 8]
+                    Bm[8 * (l // 16):8 * (l // 16) + 8, l % 16] = bt[l]
+                D += Am @ Bm
+            out[16 * nb:16 * nb + 16] = D[0]
+        return out
+
+    items, ypos = _sym_plan(np.array([k]))
+    Yp = np.full(int(ypos[-1]), np.nan)
+    for s_, p in items:
+        I1, I2 = p, T - 1 - p
+        two = I1 != I2
+        y = np.full(64 * T, np.nan)
+        rows = {I1: np.zeros(64), I2: np.zeros(64)}
+        hold = {I1: np.zeros(64), I2: np.zeros(64)}
+        for J in range(I2 + 1):
+            dt = np.zeros(64)
+            for w, I in ((0, I2), (1, I1)):
+                if w == 1 and not (two and J <= I1):
+                    continue
+                c = blocks(I, J)
+                rows[I] += row_product(c, xp[64 * J:64 * J + 64])
+                dt += transposed_product(c, xp[64 * I:64 * I + 64])
+            if J == I2 or (two and J == I1):
+                hold[J] = dt
+            else:
+                y[64 * J:64 * J + 64] = dt
+        if two:
+            y[64 * I1:64 * I1 + 64] = rows[I1] + hold[I1]
+        y[64 * I2:64 * I2 + 64] = rows[I2] + hold[I2]
+        Yp[ypos[s_] + p * 64 * T: ypos[s_] + (p + 1) * 64 * T] = y
     P = -(-T // 2)
     z = np.array([sum(Yp[ypos[0] + p * 64 * T + a] for p in range(min(P - 1, T - 1 - a // 64) + 1))
                   for a in range(k)])

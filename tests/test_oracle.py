@@ -273,6 +273,57 @@ def test_overlap_apply_brute_force():
     assert np.allclose(S.apply(r), y, atol=1e-12)
 
 
+def test_scaled_storage():
+    """SchwarzOracle(scaled=True) stores A_i^-1 / s_i with s_i = float32(max |A_i^-1|) and
+    multiplies the product by s_i. FP64 storage: the same apply as unscaled to rounding.
+    FP16 storage of an operator whose inverses exceed the FP16 range (A scaled by 1e-6, so
+    |A_i^-1| ~ 1e6 > 65504): unscaled storage overflows, scaled storage keeps FP16's
+    relative precision. A scale that is stored but not applied back is caught."""
+    pb = _small_problem(1)
+    d = pb.decomposition
+    r = np.random.default_rng(4).standard_normal(pb.A.shape[0])
+    plain = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, coarse=False)
+    sc = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, coarse=False, scaled=True)
+    assert np.allclose(sc.local(r), plain.local(r), rtol=1e-14, atol=1e-14 * np.abs(plain.local(r)).max())
+    for i, (idx, Aq) in enumerate(sc.inv):
+        assert np.abs(Aq).max() == pytest.approx(1.0, rel=1e-7)  # max |A^-1 / s| is 1
+        inv_i = np.linalg.inv(pb.A[idx][:, idx].toarray())
+        assert sc.scale[i] == float(np.float32(np.abs(inv_i).max()))
+    small = pb.A * 1e-6
+    exact = SchwarzOracle(small, d.owner_ptr, d.sub_ptr, d.sub_idx, coarse=False).local(r)
+    with np.errstate(over="ignore", invalid="ignore"):
+        over = SchwarzOracle(small, d.owner_ptr, d.sub_ptr, d.sub_idx, coarse=False,
+                             storage="float16").local(r)
+    assert not np.all(np.isfinite(over))
+    h = SchwarzOracle(small, d.owner_ptr, d.sub_ptr, d.sub_idx, coarse=False, storage="float16",
+                      scaled=True)
+    y = h.local(r)
+    assert np.linalg.norm(y - exact) <= 2e-3 * np.linalg.norm(exact)
+    h.scale = [1.0] * len(h.scale)  # planted: the scale not applied back
+    assert np.linalg.norm(h.local(r) - exact) > 0.5 * np.linalg.norm(exact)
+
+
+def test_given_inverses():
+    """SchwarzOracle(inverses=...) applies the given matrices as they are: with the
+    oracle's own FP64 inverses it reproduces the default apply; with any other matrices
+    it computes sum_i R_i^T M_i R_i r by brute force."""
+    pb = _small_problem(1)
+    d = pb.decomposition
+    r = np.random.default_rng(5).standard_normal(pb.A.shape[0])
+    ref = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, coarse=False)
+    given = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, coarse=False,
+                          inverses=[m for _, m in ref.inv])
+    assert np.array_equal(given.local(r), ref.local(r))
+    rng = np.random.default_rng(6)
+    Ms = [rng.standard_normal((int(k), int(k))) for k in np.diff(d.sub_ptr)]
+    g2 = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, coarse=False, inverses=Ms)
+    y = np.zeros_like(r)
+    for i, M in enumerate(Ms):
+        idx = d.subdomain(i)
+        y[idx] += M @ r[idx]
+    assert np.allclose(g2.local(r), y, rtol=1e-13, atol=1e-13)
+
+
 def test_cho_solve_matches_dense_inverse():
     rng = np.random.default_rng(4)
     B = rng.standard_normal((40, 40))
