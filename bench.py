@@ -250,14 +250,18 @@ def local_kernel_stats(M):
     Y = M.Y if M.Y is not None else torch.empty_like(xv)
     if L.layout in ("sym", "sym_mfma"):
         from dd_solvers.solvers import _sym_written_rows
-        t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None))
-        t_comb = kernel_time_ms(lambda: M._enqueue_sym_combine(Y, None))
-        yp_rows = _sym_written_rows(sizes)
+        t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None, Y))
+        if L.direct:  # one item per subdomain writes the result: no partial vectors
+            t_comb, comb = 0.0, 0
+            yp_rows = int(sizes.sum())
+        else:
+            t_comb = kernel_time_ms(lambda: M._enqueue_sym_combine(Y, None))
+            yp_rows = _sym_written_rows(sizes, L.group)
+            comb = int(yp_rows * 8 + sizes.sum() * 8)
         alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + yp_rows * 8)
-        comb = int(yp_rows * 8 + sizes.sum() * 8)
         kname = (f"sym_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
                  if L.layout == "sym" else
-                 f"sym_mfma_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}> (+ sym_xscale_kernel)")
+                 f"sym_mfma_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}> (+ sym_pack_x_kernel)")
     else:
         t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
         t_comb, comb = 0.0, 0

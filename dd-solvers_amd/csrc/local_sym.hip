@@ -283,11 +283,11 @@ __global__ __launch_bounds__(64) void sym_apply_kernel(
   store(I2, row2);
 }
 
-// y[sub_ptr[s] + a] = sum over the items p of subdomain s that cover local row a (blocks
-// 0..T-1-p) of Yp[ypos[s] + p*64*T + a], in item order. Grid (n_s, ceil(max_k/256)):
-// subdomains on x, which has no 65535 limit.
+// y[sub_ptr[s] + a] = sum over the items q of subdomain s that cover local row a (item q:
+// pairs q G .. q G + G - 1, blocks 0..T-1-qG) of Yp[ypos[s] + q*64*T + a], in item order.
+// Grid (n_s, ceil(max_k/256)): subdomains on x, which has no 65535 limit.
 template <typename TV>
-__global__ __launch_bounds__(256) void sym_combine_kernel(const int64_t* __restrict__ sub_ptr,
+__global__ __launch_bounds__(256) void sym_combine_kernel(int G, const int64_t* __restrict__ sub_ptr,
                                                           const int64_t* __restrict__ ypos,
                                                           const TV* __restrict__ Yp,
                                                           TV* __restrict__ y,
@@ -299,11 +299,11 @@ __global__ __launch_bounds__(256) void sym_combine_kernel(const int64_t* __restr
   const int k = (int)(sub_ptr[s + 1] - base);
   if (a >= k) return;
   const int T = (k + 63) / 64;
-  const int P = (T + 1) / 2;
-  const int last = min(P - 1, T - 1 - a / 64);
+  const int Q = ((T + 1) / 2 + G - 1) / G;
+  const int last = min(Q - 1, (T - 1 - a / 64) / G);
   const TV* src = Yp + ypos[s] + a;
   TV acc = 0;
-  for (int p = 0; p <= last; ++p) acc += src[(int64_t)p * 64 * T];
+  for (int q = 0; q <= last; ++q) acc += src[(int64_t)q * 64 * T];
   y[base + a] = acc;
 }
 
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void sym_combine_kernel(const int64_t* __restr
 //                  = 512 (8 (T-1)(T-2)/2 + 6 (T-1) + 2 nb(rL) J)            for I = T-1
 // Padding is zero.
 //
-// x is taken per subdomain relative to sigma_s = 2^e >= max |x| over S_s (sym_xscale_kernel)
+// x is taken per subdomain relative to sigma_s = 2^e >= max |x| over S_s (sym_pack_x_kernel)
 // and split into NT 16-bit terms (FP16: hi + lo; BF16: hi + mid + lo), which keeps about 22
 // (FP16) or 24 (BF16) bits of x / sigma_s.
 //   Row product y_I += T x_J: D = A(T block) x B, B column n < NT holding term n of x_J.
@@ -459,31 +459,52 @@ __device__ __forceinline__ i32x4_t sgpr4(i32x4_t v) {  // wave-uniform: SGPR ope
   return r;
 }
 
-// Loads straight into LDS (buffer_load ... lds): lane l's bytes land at m0 + l * width. The
-// compiler does not track these loads, so their completion is awaited explicitly with
-// s_waitcnt vmcnt (vector loads complete in issue order). The LDS reads issued before are
-// awaited first (lgkmcnt(0)), so a stage is never overwritten while it is being read. The
-// 16-byte form streams matrix tiles, read once per apply: non-temporal.
-__device__ __forceinline__ void dma16(uint32_t m0, uint32_t voff, i32x4_t rsrc) {
+// Loads straight into LDS (buffer_load ... lds): lane l's bytes land at m0 + offset + l *
+// width, read from base + soffset + voffset + offset (the instruction offset applies to
+// both addresses). The compiler does not track these loads, so their completion is awaited
+// explicitly with s_waitcnt vmcnt (vector loads complete in issue order). The LDS reads
+// issued before are awaited first (lgkmcnt(0)), so a stage is never overwritten while it is
+// being read. The 16-byte forms stream matrix tiles, read once per apply: non-temporal.
+// Four consecutive 1 KiB blocks: LDS m0 + 1024 q, memory soffset + voff + 1024 q, q < 4.
+__device__ __forceinline__ void dma16x4(uint32_t m0, uint32_t voff, i32x4_t rsrc, uint32_t soff) {
   m0 = __builtin_amdgcn_readfirstlane(m0);
+  soff = __builtin_amdgcn_readfirstlane(soff);
   rsrc = sgpr4(rsrc);
   asm volatile(
       "s_waitcnt lgkmcnt(0)\n\t"
       "s_mov_b32 m0, %0\n\t"
       "s_nop 0\n\t"
-      "buffer_load_dwordx4 %1, %2, 0 offen nt lds" ::"s"(m0),
-      "v"(voff), "s"(rsrc)
+      "buffer_load_dwordx4 %1, %2, %3 offen nt lds\n\t"
+      "buffer_load_dwordx4 %1, %2, %3 offen offset:1024 nt lds\n\t"
+      "buffer_load_dwordx4 %1, %2, %3 offen offset:2048 nt lds\n\t"
+      "buffer_load_dwordx4 %1, %2, %3 offen offset:3072 nt lds" ::"s"(m0),
+      "v"(voff), "s"(rsrc), "s"(soff)
       : "memory");
 }
-__device__ __forceinline__ void dma4(uint32_t m0, uint32_t voff, i32x4_t rsrc) {
+// One 1 KiB block: LDS m0, memory soffset + voff.
+__device__ __forceinline__ void dma16(uint32_t m0, uint32_t voff, i32x4_t rsrc, uint32_t soff) {
   m0 = __builtin_amdgcn_readfirstlane(m0);
+  soff = __builtin_amdgcn_readfirstlane(soff);
   rsrc = sgpr4(rsrc);
   asm volatile(
       "s_waitcnt lgkmcnt(0)\n\t"
       "s_mov_b32 m0, %0\n\t"
       "s_nop 0\n\t"
-      "buffer_load_dword %1, %2, 0 offen lds" ::"s"(m0),
-      "v"(voff), "s"(rsrc)
+      "buffer_load_dwordx4 %1, %2, %3 offen nt lds" ::"s"(m0),
+      "v"(voff), "s"(rsrc), "s"(soff)
+      : "memory");
+}
+// 4 bytes per lane: LDS m0 + 4 l, memory soffset + voff.
+__device__ __forceinline__ void dma4(uint32_t m0, uint32_t voff, i32x4_t rsrc, uint32_t soff) {
+  m0 = __builtin_amdgcn_readfirstlane(m0);
+  soff = __builtin_amdgcn_readfirstlane(soff);
+  rsrc = sgpr4(rsrc);
+  asm volatile(
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, %0\n\t"
+      "s_nop 0\n\t"
+      "buffer_load_dword %1, %2, %3 offen lds" ::"s"(m0),
+      "v"(voff), "s"(rsrc), "s"(soff)
       : "memory");
 }
 template <int N>
@@ -491,20 +512,6 @@ __device__ __forceinline__ void wait_vm() {
   asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
-// Workgroups are dispatched round-robin over the XCDs: this numbers the items so that each
-// XCD takes a contiguous range of them (any grid size).
-__device__ __forceinline__ int64_t xcd_item() {
-  const int64_t G = gridDim.x, b = blockIdx.x;
-  const int64_t q = G / kXcds, r = G % kXcds, x = b % kXcds;
-  return x * q + (x < r ? x : r) + b / kXcds;
-}
-
-// Stages of the tile pipeline per wavefront (8 KiB each): NS - 1 tiles in flight while one
-// is computed.
-#ifndef DDS_SYM_STAGES
-#define DDS_SYM_STAGES 4
-#endif
-constexpr int kSymStages = DDS_SYM_STAGES;
 constexpr uint32_t kOob = 0x80000000u;  // a buffer offset past every range: loads zeros
 // Largest subdomain of the matrix-core apply: its x terms (NT x 128 B per 64-row block) sit
 // in LDS beside the tile stages.
@@ -526,47 +533,80 @@ __device__ __forceinline__ int64_t symm_tile_off(int I, int J, int T, int nbL) {
   return 512 * (8 * ((int64_t)(T - 1) * (T - 2) / 2) + 6 * (int64_t)(T - 1) + 2 * (int64_t)nbL * J);
 }
 
+// LDS arrangement of a 1 KiB block: slot s (16 bytes) holds chunk s ^ (4 (s/16 mod 2)) of the
+// block (chunk l = lane l's operand), so that both the row-product reads (lane l reads its
+// chunk) and the transposed reads (16 chunks r + 4h + 8a + 16b + 32n at a time) cover 16
+// distinct 16-byte bank groups. The map is its own inverse.
+__device__ __forceinline__ int symm_slot(int c) { return c ^ (((c >> 4) & 1) << 2); }
+
 // The 8 block loads of tile (I, J) into LDS stage `st` (zeros for absent blocks; every block
-// zero when `real` is false, so every path issues the same count of loads).
+// zero when `real` is false, so every path issues the same count of loads). voff: this
+// lane's chunk of a block in bytes (16 symm_slot(lane)).
 __device__ __forceinline__ void symm_issue(uint32_t st, bool real, int I, int J, int T, int rL,
-                                           int nbL, i32x4_t rm, int lane) {
+                                           int nbL, i32x4_t rm, uint32_t voff) {
   const bool diag = J == I;
   const int nb = I < T - 1 ? 4 : nbL;
-  const int nk = diag && I == T - 1 ? (rL + 31) / 32 : 2;
   const uint32_t toff = real ? (uint32_t)(symm_tile_off(I, J, T, nbL) * 2) : 0u;
+  if (real && !diag && nb == 4) {  // a whole off-diagonal tile: 8 consecutive blocks
+    dma16x4(st, voff, rm, toff);
+    dma16x4(st + 4096u, voff, rm, toff + 4096u);
+    return;
+  }
+  const int nk = diag && I == T - 1 ? (rL + 31) / 32 : 2;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int Rb = q >> 1, K2 = q & 1;
     const bool ok = real && Rb < nb && K2 < nk && !(diag && K2 == 1 && Rb < 2);
     const int idx = diag ? Rb + (nk == 2 ? (Rb > 2 ? Rb - 2 : 0) : 0) + K2 : q;
-    dma16(st + (uint32_t)q * 1024u,
-          ok ? toff + (uint32_t)idx * 1024u + (uint32_t)lane * 16u : kOob, rm);
+    dma16(st + (uint32_t)q * 1024u, voff, rm, ok ? toff + (uint32_t)idx * 1024u : kOob);
   }
 }
 
-// Task t of an item (tiles in order: for J = 0..I2, tile (I2, J), then tile (I1, J) when
-// I1 != I2 and J <= I1) into LDS stage t % NS; past the last task, zeros.
-template <int NS>
-__device__ __forceinline__ void symm_issue_task(int t, int ntasks, int I1, int I2, bool two,
-                                                int T, int rL, int nbL, uint32_t stage0,
-                                                i32x4_t rm, int lane) {
-  int J = 0, w = 0;
-  const bool real = t < ntasks;
-  if (real) {
-    const int pairs = two ? I1 + 1 : 0;
-    if (t < 2 * pairs) {
-      J = t >> 1;
-      w = t & 1;
-    } else {
-      J = t - pairs;
-    }
-  }
-  symm_issue(stage0 + (uint32_t)(t % NS) * 8192u, real, w == 0 ? I2 : I1, J, T, rL, nbL, rm, lane);
+// The tile rows of an item: the pairs p0 .. p_end-1 cover rows p and T-1-p, taken in
+// increasing order (rows p0 .. p_end-1, then max(T - p_end, p_end) .. T-1-p0; the middle row
+// of an odd T once), so a whole subdomain (p0 = 0, p_end = ceil(T/2)) reads its tiles in
+// storage order. Row i of the item:
+__device__ __forceinline__ int symm_row(int T, int p0, int p_end, int i) {
+  const int nlow = p_end - p0;
+  return i < nlow ? p0 + i : max(T - p_end, p_end) + (i - nlow);
 }
 
+// A cursor over the item's tiles: row index i (tile row I), tile J <= I.
+struct SymmCursor {
+  int i, I, J;
+};
+__device__ __forceinline__ SymmCursor symm_advance(SymmCursor c, int T, int p0, int p_end) {
+  if (++c.J > c.I) {
+    c.J = 0;
+    c.I = symm_row(T, p0, p_end, ++c.i);
+  }
+  return c;
+}
+
+// The tile under the cursor into LDS stage `st`; zeros when the cursor is past the item.
+__device__ __forceinline__ void symm_issue_at(uint32_t st, SymmCursor c, int nrows, int T,
+                                              int rL, int nbL, i32x4_t rm, uint32_t voff) {
+  const bool real = c.i < nrows;
+  symm_issue(st, real, real ? c.I : 0, real ? c.J : 0, T, rL, nbL, rm, voff);
+}
+
+#ifndef DDS_SYMM_STAGES
+#define DDS_SYMM_STAGES 3
+#endif
+// LDS stages of the tile pipeline (8 KiB each): NS - 1 tiles in flight per wavefront.
+constexpr int kSymmStages = DDS_SYMM_STAGES;
+static_assert(kSymmStages >= 2 && 8 * (kSymmStages - 1) <= 63, "vmcnt range");
+
+// One workgroup (one wavefront) per item (s, q): the tile-row pairs p = q G .. q G + G - 1
+// of subdomain s (G = pairs_per_item, the last group of a subdomain possibly shorter), tile
+// row by tile row. The tiles stream through NS LDS stages (NS - 1 in flight while one is
+// computed). The item's result accumulates in LDS in FP32 (yv: each tile's transposed part
+// T_IJ^T x_I into block J, each row's product sum_J T_IJ x_J into block I), and every row the
+// item touches is written once, at its end: Yp[ypos[s] + q 64 T + a] for the local rows
+// a < min(k, 64 (T - q G)).
 template <typename TM, typename TV>
 __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
-    const int32_t* __restrict__ items, const int64_t* __restrict__ sub_ptr,
+    int G, int Tmax, const int32_t* __restrict__ items, const int64_t* __restrict__ sub_ptr,
     const int64_t* __restrict__ mat_off, const int64_t* __restrict__ ypos,
     const int64_t* __restrict__ xt_off, const float* __restrict__ scale,
     const float* __restrict__ xsc, const uint16_t* __restrict__ xt, const TM* __restrict__ mats,
@@ -574,83 +614,82 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   using S = SymMfma<TM>;
   using op_t = typename S::op_t;
   constexpr int NT = S::NT;
-  constexpr int NS = kSymStages;
+  constexpr int NS = kSymmStages;
   constexpr int XB = NT * 128;  // bytes of x terms per 64-row block
+  constexpr int NF = NS - 1;    // tiles in flight
   if (gate != nullptr && gate->done) return;
   extern __shared__ __align__(16) unsigned char sym_smem[];
-  // [NS stages x 8 blocks x 64 lanes x 16 B][16 zero bytes][yl: 64 floats][x terms]
+  // [NS stages x 8 blocks x 64 lanes x 16 B][16 zero bytes][yl: 64 floats]
+  // [x terms: Tmax XB + 256][yv: 64 Tmax floats]
   unsigned char* stages = sym_smem;
   unsigned char* zero16 = sym_smem + NS * 8192;
   float* yl = reinterpret_cast<float*>(zero16 + 16);
   unsigned char* xtl = zero16 + 16 + 256;
+  float* yv = reinterpret_cast<float*>(xtl + Tmax * XB + 256);
 
   const int lane = threadIdx.x;
-  const int64_t item = xcd_item();
-  const int s = items[2 * item];
-  const int p = items[2 * item + 1];
+  const int s = items[2 * blockIdx.x];
+  const int q = items[2 * blockIdx.x + 1];
   const int64_t base = sub_ptr[s];
   const int k = (int)(sub_ptr[s + 1] - base);
-  const int T = (k + 63) / 64;
-  const int rL = k - 64 * (T - 1);
-  const int nbL = (rL + 15) / 16;
-  const int I1 = p;
-  const int I2 = T - 1 - p;
-  const bool two = I1 != I2;
-  TV* Y = Yp + ypos[s] + (int64_t)p * 64 * T;
+  const int T = (k + 63) / 64, rL = k - 64 * (T - 1), nbL = (rL + 15) / 16;
+  const int P = (T + 1) / 2;
+  const int p0 = q * G, p_end = min(P, p0 + G);
+  const int I2max = T - 1 - p0;  // the item touches blocks 0..I2max
+  const int nrows = (p_end - p0) + (T - p0) - max(T - p_end, p_end);
+  int ntot = 0;
+  for (int i = 0; i < nrows; ++i) ntot += symm_row(T, p0, p_end, i) + 1;
   const double out_sc = static_cast<double>(xsc[s]) *
                         (scale != nullptr ? static_cast<double>(scale[s]) : 1.0);
   const i32x4_t rm = buffer_rsrc(mats + mat_off[s], (uint32_t)(symm_store_elems(k) * 2));
   const i32x4_t rx = buffer_rsrc(xt + xt_off[s], (uint32_t)(((k + 31) / 32) * NT * 64));
   const uint32_t stage0 = lds_addr(stages);
+  const uint32_t voff = 16u * (uint32_t)symm_slot(lane);  // the chunk this lane's slot receives
 
   if (lane < 4) reinterpret_cast<float*>(zero16)[lane] = 0.f;
-  // x terms of blocks 0..I2 into LDS (zeros past the subdomain), then the first tiles
-  const int xdw = (I2 + 1) * XB / 4;  // dwords
-  for (int d0 = 0; d0 < xdw; d0 += 64) dma4(lds_addr(xtl + 4 * d0), (uint32_t)(d0 + lane) * 4u, rx);
-  // tasks: for J = 0..I2, tile (I2, J), then tile (I1, J) if two and J <= I1
-  const int ntasks = (I2 + 1) + (two ? I1 + 1 : 0);
+  for (int a = lane; a < 64 * (I2max + 1); a += 64) yv[a] = 0.f;
+  // x terms of blocks 0..I2max into LDS (zeros past the subdomain), then the first tiles
+  const int xdw = (I2max + 1) * XB / 4;  // dwords
+  for (int d0 = 0; d0 < xdw; d0 += 64)
+    dma4(lds_addr(xtl + 4 * d0), (uint32_t)lane * 4u, rx, 4u * (uint32_t)d0);
+  SymmCursor ic = {0, symm_row(T, p0, p_end, 0), 0};  // issue cursor
 #pragma unroll
-  for (int t = 0; t < NS - 1; ++t)
-    symm_issue_task<NS>(t, ntasks, I1, I2, two, T, rL, nbL, stage0, rm, lane);
-  wait_vm<8 * (NS - 1)>();  // the x terms are in
+  for (int t = 0; t < NF; ++t) {
+    symm_issue_at(stage0 + (uint32_t)(t % NS) * 8192u, ic, nrows, T, rL, nbL, rm, voff);
+    if (ic.i < nrows) ic = symm_advance(ic, T, p0, p_end);
+  }
+  wait_vm<8 * NF>();  // the x terms (issued before every tile) are in
   wave_lds_fence();
 
+  const int tm = lane & 15;
   // operand of x rows 64 J + 32 h + 8 (l/16) + e, term l % 16 (zero past the NT terms): the
   // B operand of a row product and the A' operand of a transposed product
-  const int tm = lane & 15;
   auto x_op = [&](int J, int h) __attribute__((always_inline)) -> op_t {
-    const unsigned char* src =
-        tm < NT ? xtl + J * XB + (h * NT + tm) * 64 + 16 * (lane >> 4) : zero16;
+    const unsigned char* src = tm < NT ? xtl + J * XB + (h * NT + tm) * 64 + 16 * (lane >> 4) : zero16;
     op_t o;
     const uint4 v = *reinterpret_cast<const uint4*>(src);
     __builtin_memcpy(&o, &v, 16);
     return o;
   };
-  const op_t a2_0 = x_op(I2, 0), a2_1 = x_op(I2, 1);
-  const op_t a1_0 = x_op(I1, 0), a1_1 = x_op(I1, 1);
-
   const f32x4_t z4 = {0.f, 0.f, 0.f, 0.f};
-  f32x4_t d2[4] = {z4, z4, z4, z4};  // row product of tile row I2, row blocks 0..3
-  f32x4_t d1[4] = {z4, z4, z4, z4};  // row product of tile row I1
-  f32x4_t dt[4];                     // transposed products into block J, column blocks 0..3
-  float hold1 = 0.f, hold2 = 0.f;    // transposed parts of blocks I1, I2 (lane = row)
+  f32x4_t d[4];   // row product of the current tile row, row blocks 0..3
+  f32x4_t dt[4];  // transposed product of the current tile, column blocks 0..3
+  op_t a0, a1, b0, b1;
 
   const int g = lane >> 4, r = (lane >> 2) & 3, q4 = lane & 3;
-  // one tile (stage of task t): row product into d, transposed product into dt
-  auto tile = [&](int t, f32x4_t (&d)[4], const op_t& ai0, const op_t& ai1, const op_t& b0,
-                  const op_t& b1) __attribute__((always_inline)) {
+  auto tile = [&](int t) __attribute__((always_inline)) {
     const unsigned char* sb = stages + (t % NS) * 8192;
 #pragma unroll
     for (int Rb = 0; Rb < 4; ++Rb) {
-      const uint4 c0 = *reinterpret_cast<const uint4*>(sb + (2 * Rb) * 1024 + lane * 16);
-      const uint4 c1 = *reinterpret_cast<const uint4*>(sb + (2 * Rb + 1) * 1024 + lane * 16);
+      const uint4 c0 = *reinterpret_cast<const uint4*>(sb + (2 * Rb) * 1024 + voff);
+      const uint4 c1 = *reinterpret_cast<const uint4*>(sb + (2 * Rb + 1) * 1024 + voff);
       op_t o0, o1;
       __builtin_memcpy(&o0, &c0, 16);
       __builtin_memcpy(&o1, &c1, 16);
       d[Rb] = S::mfma(o0, b0, d[Rb]);
       d[Rb] = S::mfma(o1, b1, d[Rb]);
     }
-    // rows 32 kr2 + 8 g + r (+4 h), columns 16 nb + 4 q4, read column-major
+    // rows 32 kr2 + 8 g + r (+4 h), columns 16 nb + 4 q4 of the tile, read column-major
     // (ds_read_tr16_b64) from the lane-ordered blocks
 #pragma unroll
     for (int kr2 = 0; kr2 < 2; ++kr2)
@@ -662,18 +701,18 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
           const int rho = 32 * kr2 + 8 * g + r + 4 * h;
           const int blk = 2 * (rho >> 4) + (nb >> 1);
           const int ln = (rho & 15) + 16 * (2 * (nb & 1) + (q4 >> 1));
-          hv[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-              (__attribute__((address_space(3))) i16x4_t*)(sb + blk * 1024 + ln * 16 + 8 * (q4 & 1)));
+          hv[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) i16x4_t*)(
+              sb + blk * 1024 + symm_slot(ln) * 16 + 8 * (q4 & 1)));
         }
         const i16x8_t v = {hv[0][0], hv[0][1], hv[0][2], hv[0][3],
                            hv[1][0], hv[1][1], hv[1][2], hv[1][3]};
         op_t bt;
         __builtin_memcpy(&bt, &v, 16);
-        dt[nb] = S::mfma(kr2 == 0 ? ai0 : ai1, bt, dt[nb]);
+        dt[nb] = S::mfma(kr2 == 0 ? a0 : a1, bt, dt[nb]);
       }
   };
-  // block J of the result complete: its transposed part, lane = row
-  auto finish_block = [&](int J) __attribute__((always_inline)) {
+  // yv[64 J + lane] += the transposed part (rows 16 nb + lane of D, the NT term rows added)
+  auto add_transposed = [&](int J) __attribute__((always_inline)) {
     if (lane < 16) {
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) {
@@ -684,39 +723,12 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
       }
     }
     wave_lds_fence();
-    const float v = yl[lane];
+    yv[64 * J + lane] += yl[lane];
     wave_lds_fence();
-    if (J == I2) {
-      hold2 = v;
-    } else if (two && J == I1) {
-      hold1 = v;
-    } else if (64 * J + lane < k) {
-      Y[64 * J + lane] = static_cast<TV>(static_cast<double>(v) * out_sc);
-    }
   };
-
-  int t = 0;
-  for (int J = 0; J <= I2; ++J) {
-    const op_t b0 = x_op(J, 0), b1 = x_op(J, 1);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) dt[i] = z4;
-    // into the stage of task t - 1, read completely before
-    symm_issue_task<NS>(t + NS - 1, ntasks, I1, I2, two, T, rL, nbL, stage0, rm, lane);
-    wait_vm<8 * (NS - 1)>();  // task t's tile is in
-    tile(t, d2, a2_0, a2_1, b0, b1);
-    ++t;
-    if (two && J <= I1) {
-      symm_issue_task<NS>(t + NS - 1, ntasks, I1, I2, two, T, rL, nbL, stage0, rm, lane);
-      wait_vm<8 * (NS - 1)>();
-      tile(t, d1, a1_0, a1_1, b0, b1);
-      ++t;
-    }
-    finish_block(J);
-  }
-
-  // row products: the NT term columns of D (lanes 16 g + t, t < NT) added; lane 16 g holds
-  // rows 16 Rb + 4 g + i
-  auto store_row = [&](f32x4_t (&d)[4], float hold, int I) __attribute__((always_inline)) {
+  // yv[64 I + lane] += the row product (the NT term columns of D, lanes 16 g + t, added;
+  // lane 16 g holds rows 16 Rb + 4 g + i)
+  auto add_row = [&](int I) __attribute__((always_inline)) {
 #pragma unroll
     for (int Rb = 0; Rb < 4; ++Rb) {
 #pragma unroll
@@ -728,13 +740,36 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
       }
     }
     wave_lds_fence();
-    const float v = yl[lane] + hold;
+    yv[64 * I + lane] += yl[lane];
     wave_lds_fence();
-    if (64 * I + lane < k) Y[64 * I + lane] = static_cast<TV>(static_cast<double>(v) * out_sc);
   };
-  if (two) store_row(d1, hold1, I1);
-  store_row(d2, hold2, I2);
+
+  SymmCursor cc = {0, symm_row(T, p0, p_end, 0), 0};  // compute cursor
+  for (int t = 0; t < ntot; ++t) {
+    // into the stage of task t - 1, read completely before
+    symm_issue_at(stage0 + (uint32_t)((t + NF) % NS) * 8192u, ic, nrows, T, rL, nbL, rm, voff);
+    if (ic.i < nrows) ic = symm_advance(ic, T, p0, p_end);
+    wait_vm<8 * NF>();  // task t's tile is in (vector loads complete in issue order)
+    if (cc.J == 0) {  // a new tile row
+      a0 = x_op(cc.I, 0);
+      a1 = x_op(cc.I, 1);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d[i] = z4;
+    }
+    b0 = x_op(cc.J, 0);
+    b1 = x_op(cc.J, 1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dt[i] = z4;
+    tile(t);
+    add_transposed(cc.J);
+    if (cc.J == cc.I) add_row(cc.I);  // the row is complete
+    cc = symm_advance(cc, T, p0, p_end);
+  }
   wait_vm<0>();  // no load into LDS outlives the wavefront
+
+  TV* Y = Yp + ypos[s] + (int64_t)q * 64 * T;
+  const int rows = min(k, 64 * (I2max + 1));
+  for (int a = lane; a < rows; a += 64) Y[a] = static_cast<TV>(static_cast<double>(yv[a]) * out_sc);
 }
 
 template <typename TM, typename TV, typename TA>
@@ -794,35 +829,51 @@ extern "C" int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, 
   return DDS_EUNSUPPORTED;
 }
 
+// Launch: the x pack, then the apply. pairs_per_item: G as above.
 extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s,
-                                        int64_t n_items, int64_t max_k, const int32_t* items,
-                                        const int64_t* sub_ptr, const int32_t* sub_idx,
-                                        const int64_t* mat_off, const int64_t* ypos,
-                                        const int64_t* xt_off, const float* scale,
-                                        const void* mats, const void* x, float* xsc, void* xt,
-                                        void* Yp, const dds_pcg_state* gate, void* stream) {
+                                        int64_t n_items, int64_t max_k, int64_t pairs_per_item,
+                                        const int32_t* items, const int64_t* sub_ptr,
+                                        const int32_t* sub_idx, const int64_t* mat_off,
+                                        const int64_t* ypos, const int64_t* xt_off,
+                                        const float* scale, const void* mats, const void* x,
+                                        float* xsc, void* xt, void* Yp,
+                                        const dds_pcg_state* gate, void* stream) {
   DDS_REQUIRE(n_s >= 0 && n_items >= 0 && n_items <= INT32_MAX && ceil_div(n_s, 4) <= INT32_MAX,
               "dds_local_apply_sym_mfma: bad sizes");
   DDS_REQUIRE(max_k >= 0 && max_k <= kSymMfmaMaxK,
               "dds_local_apply_sym_mfma: subdomains of at most %d unknowns (got %lld)",
               kSymMfmaMaxK, (long long)max_k);
+  DDS_REQUIRE(pairs_per_item >= 1 && pairs_per_item <= kSymMfmaMaxK,
+              "dds_local_apply_sym_mfma: pairs_per_item %lld out of range",
+              (long long)pairs_per_item);
   if (n_items == 0 || n_s == 0) return DDS_OK;
   DDS_REQUIRE(items && sub_ptr && mat_off && ypos && xt_off && mats && x && xsc && xt && Yp,
               "dds_local_apply_sym_mfma: null pointer");
   hipStream_t st = as_stream(stream);
   const dim3 g1((unsigned)ceil_div(n_s, 4)), g2((unsigned)n_items);
   const int Tmax = (int)ceil_div(max_k, 64);
+  const int G = (int)pairs_per_item;
+  int dev = 0, max_lds = 0;
+  DDS_HIP(hipGetDevice(&dev));
+  DDS_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
 #define DDS_SYMM(TM, TV)                                                                       \
   do {                                                                                         \
-    const size_t lds = (size_t)kSymStages * 8192 + 16 + 256 +                                  \
-                       (size_t)Tmax * SymMfma<TM>::NT * 128 + 256;                             \
+    const size_t lds = (size_t)kSymmStages * 8192 + 16 + 256 +                                 \
+                       (size_t)Tmax * SymMfma<TM>::NT * 128 + 256 + (size_t)Tmax * 256;        \
+    DDS_REQUIRE(lds <= (size_t)max_lds,                                                        \
+                "dds_local_apply_sym_mfma: %zu bytes of LDS for max_k %lld exceed the %d "     \
+                "per workgroup", lds, (long long)max_k, max_lds);                              \
+    if (lds > 65536)                                                                           \
+      DDS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sym_mfma_apply_kernel<TM, TV>), \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
     hipLaunchKernelGGL((sym_pack_x_kernel<TM, TV>), g1, dim3(256), 0, st, n_s, sub_ptr,         \
                        sub_idx, xt_off, static_cast<const TV*>(x), xsc,                        \
                        static_cast<uint16_t*>(xt), gate);                                      \
     DDS_LAUNCHED();                                                                            \
-    hipLaunchKernelGGL((sym_mfma_apply_kernel<TM, TV>), g2, dim3(64), lds, st, items, sub_ptr,  \
-                       mat_off, ypos, xt_off, scale, xsc, static_cast<const uint16_t*>(xt),     \
-                       static_cast<const TM*>(mats), static_cast<TV*>(Yp), gate);               \
+    hipLaunchKernelGGL((sym_mfma_apply_kernel<TM, TV>), g2, dim3(64), lds, st, G, Tmax, items, \
+                       sub_ptr, mat_off, ypos, xt_off, scale, xsc,                             \
+                       static_cast<const uint16_t*>(xt), static_cast<const TM*>(mats),         \
+                       static_cast<TV*>(Yp), gate);                                            \
     DDS_LAUNCHED();                                                                            \
     return DDS_OK;                                                                             \
   } while (0)
@@ -835,20 +886,23 @@ extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_
   return DDS_EUNSUPPORTED;
 }
 
-extern "C" int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, const int64_t* sub_ptr,
-                               const int64_t* ypos, const void* Yp, void* y,
-                               const dds_pcg_state* gate, void* stream) {
+extern "C" int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, int64_t pairs_per_item,
+                               const int64_t* sub_ptr, const int64_t* ypos, const void* Yp,
+                               void* y, const dds_pcg_state* gate, void* stream) {
   DDS_REQUIRE(n_s >= 0 && n_s <= INT32_MAX && max_k >= 0 && ceil_div(max_k, 256) <= 65535,
               "dds_sym_combine: bad sizes");
+  DDS_REQUIRE(pairs_per_item >= 1 && pairs_per_item <= INT32_MAX,
+              "dds_sym_combine: pairs_per_item %lld out of range", (long long)pairs_per_item);
+  const int G = (int)pairs_per_item;
   if (n_s == 0 || max_k == 0) return DDS_OK;
   DDS_REQUIRE(sub_ptr && ypos && Yp && y, "dds_sym_combine: null pointer");
   const dim3 grid((unsigned)n_s, (unsigned)ceil_div(max_k, 256));
   hipStream_t s = as_stream(stream);
   if (dtype == DDS_F64) {
-    hipLaunchKernelGGL(sym_combine_kernel<double>, grid, dim3(256), 0, s, sub_ptr, ypos,
+    hipLaunchKernelGGL(sym_combine_kernel<double>, grid, dim3(256), 0, s, G, sub_ptr, ypos,
                        static_cast<const double*>(Yp), static_cast<double*>(y), gate);
   } else if (dtype == DDS_F32) {
-    hipLaunchKernelGGL(sym_combine_kernel<float>, grid, dim3(256), 0, s, sub_ptr, ypos,
+    hipLaunchKernelGGL(sym_combine_kernel<float>, grid, dim3(256), 0, s, G, sub_ptr, ypos,
                        static_cast<const float*>(Yp), static_cast<float*>(y), gate);
   } else {
     set_error("dds_sym_combine: unsupported dtype %d", dtype);

@@ -994,9 +994,11 @@ class _LocalStore:
     acc_dtype: torch.dtype
     bytes_matrices: int
     layout: str = "full"
-    items: torch.Tensor | None = None   # sym: (n_items, 2) int32 (subdomain, tile-row pair)
+    items: torch.Tensor | None = None   # sym: (n_items, 2) int32 (subdomain, pair group)
     ypos: torch.Tensor | None = None    # sym: (n_s,) int64 offsets of the partial vectors
     n_items: int = 0
+    group: int = 1                      # sym: tile-row pairs per item
+    direct: bool = False                # sym_mfma: one item per subdomain, written into Y
     bytes_algorithmic: int = 0          # lower triangle (sym) or full matrix, storage dtype
 
 
@@ -1080,29 +1082,54 @@ def _symm_index(k: int) -> np.ndarray:
     return idx
 
 
-def _sym_plan(sizes: np.ndarray):
-    """Work items (subdomain, tile-row pair) and the offsets of their partial vectors for
-    the symmetric layout: subdomain s has P_s = ceil(T_s / 2) items, item p covering tile
-    rows p and T_s-1-p and writing a 64*T_s-long partial vector."""
-    T = -(-sizes // 64)
+def _sym_pair_tasks(T: np.ndarray, p: np.ndarray) -> np.ndarray:
+    """Tiles that tile-row pair p (rows p and T-1-p) of a subdomain of T tile rows reads."""
+    return np.where(p == T - 1 - p, T - p, T + 1)
+
+
+def _sym_plan(sizes: np.ndarray, group: int = 1):
+    """Work items (subdomain, pair group q) and the offsets of their partial vectors for the
+    symmetric layouts: subdomain s has P_s = ceil(T_s / 2) tile-row pairs (pair p covers
+    tile rows p and T_s-1-p); item q covers pairs q*group .. q*group + group - 1 and writes
+    a 64*T_s-long partial vector (its first 64 (T_s - q*group) rows)."""
+    T = -(-np.asarray(sizes, dtype=np.int64) // 64)
     P = -(-T // 2)
-    ypos = np.concatenate([[0], np.cumsum(P * 64 * T)]).astype(np.int64)
-    items = _enumerate(P)
-    # longest items first (tiles touched: T + 1, or T - p for the middle row of an odd T),
-    # so the last workgroups to start are the short ones
+    Q = -(-P // group)
+    ypos = np.concatenate([[0], np.cumsum(Q * 64 * T)]).astype(np.int64)
+    items = _enumerate(Q)
+    # longest items first (tiles read), so the last workgroups to start are the short ones
     Ts = T[items[:, 0]]
-    p = items[:, 1]
-    work = np.where(p == Ts - 1 - p, Ts - p, Ts + 1)
+    work = np.zeros(items.shape[0], dtype=np.int64)
+    for j in range(group):
+        p = items[:, 1].astype(np.int64) * group + j
+        work += np.where(p < -(-Ts // 2), _sym_pair_tasks(Ts, p), 0)
     return items[np.argsort(-work, kind="stable")], ypos
 
 
-def _sym_written_rows(sizes: np.ndarray) -> int:
+def _symm_group(sizes: np.ndarray, n_cu: int) -> int:
+    """Tile-row pairs per item of the matrix-core symmetric apply. An item sums its products
+    in LDS and writes each row it touches once, so larger items write less (whole subdomains:
+    exactly the result, no combine) and start less often, while more items balance the CUs
+    better: whole subdomains when every CU gets at least 8 of them, otherwise the largest
+    power of two that keeps 32 items per CU (items of one pair at most)."""
+    T = -(-np.asarray(sizes, dtype=np.int64) // 64)
+    P = -(-T // 2)
+    if sizes.shape[0] >= 8 * n_cu:
+        return int(P.max())
+    g = 1
+    while 2 * g <= P.max() and int((-(-P // (2 * g))).sum()) >= 32 * n_cu:
+        g *= 2
+    return g
+
+
+def _sym_written_rows(sizes: np.ndarray, group: int = 1) -> int:
     """Entries of the partial vectors that the symmetric apply writes (and its combine
-    reads): item p writes the valid rows of blocks 0 .. T-1-p."""
+    reads): item q writes the valid rows of blocks 0 .. T-1-q*group."""
     total = 0
     for k, cnt in zip(*np.unique(np.asarray(sizes, dtype=np.int64), return_counts=True)):
         k, T = int(k), -(-int(k) // 64)
-        total += int(cnt) * sum(min(64 * (T - p), k) for p in range(-(-T // 2)))
+        P = -(-T // 2)
+        total += int(cnt) * sum(min(64 * (T - q * group), k) for q in range(-(-P // group)))
     return total
 
 
@@ -1517,9 +1544,14 @@ class SchwarzOperator(SparseSolver):
                 grid=0, waves=4)
             alg = int((sizes.astype(np.int64) ** 2).sum()) * esz
         elif layout in ("sym", "sym_mfma"):
-            items, ypos = _sym_plan(sizes)
+            group = 1
+            if layout == "sym_mfma":
+                group = _symm_group(sizes, torch.cuda.get_device_properties(dev).multi_processor_count)
+            items, ypos = _sym_plan(sizes, group)
             tiles = np.zeros((0, 2), dtype=np.int32)
-            self.Yp = torch.empty(int(ypos[-1]), dtype=pi, device=dev)
+            direct = layout == "sym_mfma" and items.shape[0] == n_s
+            # one item per subdomain writes the result itself (ypos = sub_ptr)
+            self.Yp = None if direct else torch.empty(int(ypos[-1]), dtype=pi, device=dev)
             if layout == "sym_mfma":  # per apply: power-of-two scales of x, x split in terms
                 nt = 2 if store_dtype == torch.float16 else 3
                 words = 32 * nt * (-(-sizes.astype(np.int64) // 32))
@@ -1542,7 +1574,9 @@ class SchwarzOperator(SparseSolver):
             bytes_matrices=int(total * esz), layout=layout,
             items=None if items is None else torch.as_tensor(items, device=dev),
             ypos=None if ypos is None else torch.as_tensor(ypos[:-1], device=dev),
-            n_items=0 if items is None else int(items.shape[0]), bytes_algorithmic=alg)
+            n_items=0 if items is None else int(items.shape[0]), bytes_algorithmic=alg,
+            group=group if layout in ("sym", "sym_mfma") else 1,
+            direct=layout == "sym_mfma" and direct)
         collect()
 
     # -- apply pieces ---------------------------------------------------------------------
@@ -1565,8 +1599,9 @@ class SchwarzOperator(SparseSolver):
             raise RuntimeError("layout='mfma' applies blocks of right-hand sides (2-D); "
                                "CG routes single vectors through a block of one")
         if L.layout in ("sym", "sym_mfma"):
-            self._enqueue_sym_apply(x, gate)
-            self._enqueue_sym_combine(Y, gate)
+            self._enqueue_sym_apply(x, gate, Y)
+            if not L.direct:
+                self._enqueue_sym_combine(Y, gate)
             return
         if L.layout == "batched":
             k = L.max_k
@@ -1583,16 +1618,20 @@ class SchwarzOperator(SparseSolver):
                   L.mat_off.data_ptr(), L.ld.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(),
                   x.data_ptr(), Y.data_ptr(), self.waves or 4, _lib.ptr(gate), _lib.stream())
 
-    def _enqueue_sym_apply(self, x, gate) -> None:
-        """Partial products of every (subdomain, tile-row pair) item into Yp."""
+    def _enqueue_sym_apply(self, x, gate, Y=None) -> None:
+        """Partial products of every (subdomain, pair group) item into Yp; for a direct
+        layout (one item per subdomain) the result itself into Y."""
         L = self.local
         if L.layout == "sym_mfma":
+            if L.direct and Y is None:
+                raise ValueError("a direct sym_mfma apply writes the result: pass Y")
+            out, pos = (Y, self.sub_ptr) if L.direct else (self.Yp, L.ypos)
             _lib.call("dds_local_apply_sym_mfma", _lib.dtype_code(L.mat_dtype),
-                      _lib.dtype_code(self.pi), self.n_solvers, L.n_items, L.max_k,
+                      _lib.dtype_code(self.pi), self.n_solvers, L.n_items, L.max_k, L.group,
                       L.items.data_ptr(), self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx),
-                      L.mat_off.data_ptr(), L.ypos.data_ptr(), self._xt_off.data_ptr(),
+                      L.mat_off.data_ptr(), pos.data_ptr(), self._xt_off.data_ptr(),
                       _lib.ptr(L.scale), L.mats.data_ptr(), x.data_ptr(), self._xsc.data_ptr(),
-                      self._xt.data_ptr(), self.Yp.data_ptr(), _lib.ptr(gate), _lib.stream())
+                      self._xt.data_ptr(), out.data_ptr(), _lib.ptr(gate), _lib.stream())
             return
         _lib.call("dds_local_apply_sym", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
                   _lib.dtype_code(L.acc_dtype), L.n_items, L.items.data_ptr(),
@@ -1603,7 +1642,7 @@ class SchwarzOperator(SparseSolver):
     def _enqueue_sym_combine(self, Y, gate) -> None:
         """Y[sub_ptr[s] + a] = sum of the partial vectors of subdomain s at row a."""
         L = self.local
-        _lib.call("dds_sym_combine", _lib.dtype_code(self.pi), self.n_solvers, L.max_k,
+        _lib.call("dds_sym_combine", _lib.dtype_code(self.pi), self.n_solvers, L.max_k, L.group,
                   self.sub_ptr.data_ptr(), L.ypos.data_ptr(), self.Yp.data_ptr(),
                   Y.data_ptr(), _lib.ptr(gate), _lib.stream())
 

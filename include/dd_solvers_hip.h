@@ -180,30 +180,36 @@ int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_i
                         const void* mats, const void* x, void* Yp, const dds_pcg_state* gate,
                         void* stream);
 
-/* The same partial vectors as dds_local_apply_sym for inverses stored in FP16 or BF16, on the
- * matrix cores with FP32 accumulation. Storage: the lower triangle of 64 x 64 tiles, each
- * tile as its non-zero 16 x 32 blocks in MFMA A-operand order (lane l of block (Rb, K2)
- * holds T[16Rb + l%16][32K2 + 8(l/16) + e], e < 8; diagonal tiles hold L + D/2). Per apply
- * two kernels run: the first writes, as device scratch, xsc[s] (n_s floats: the power of
- * two that bounds |x| on subdomain s) and xt (16-bit words: x / xsc[s] split into 2 (FP16)
- * or 3 (BF16) terms, for each 32-row block c of subdomain s the NT x 32 words at
+/* The symmetric local apply for inverses stored in FP16 or BF16, on the matrix cores with
+ * FP32 accumulation. Storage: the lower triangle of 64 x 64 tiles, each tile as its non-zero
+ * 16 x 32 blocks in MFMA A-operand order (lane l of block (Rb, K2) holds
+ * T[16Rb + l%16][32K2 + 8(l/16) + e], e < 8; diagonal tiles hold L + D/2). Items (s, q)
+ * group pairs_per_item = G tile-row pairs: item q of subdomain s covers pairs
+ * p = qG .. min(P_s, qG + G) - 1 (P_s = ceil(T_s / 2)), sums its products in FP32 and writes
+ * Yp[ypos[s] + q*64*T_s + a] for the local rows a < min(k_s, 64 (T_s - qG)) once. When
+ * G >= P_s for every subdomain each subdomain has one item and Yp = y, ypos = sub_ptr gives
+ * the result directly; otherwise dds_sym_combine sums the items with the same G. Per apply
+ * two kernels run: the first writes, as device scratch, xsc[s] (n_s floats: the power of two
+ * that bounds |x| on subdomain s) and xt (16-bit words: x / xsc[s] split into 2 (FP16) or 3
+ * (BF16) terms, for each 32-row block c of subdomain s the NT x 32 words at
  * xt_off[s] + 32 NT c, words past k_s zero and left untouched); the second streams the tiles
  * into LDS. vec_dtype F64 or F32; max_k (the largest subdomain) at most 4096. Replaces the
  * same Inv.solve -> gemv_strided_batched step (solvers.py:355-359) as dds_local_apply,
  * accumulating in FP32 as gemv_kernels.cu:47-54 does for FP32 vectors. */
 int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s, int64_t n_items,
-                             int64_t max_k, const int32_t* items, const int64_t* sub_ptr,
-                             const int32_t* sub_idx, const int64_t* mat_off,
-                             const int64_t* ypos, const int64_t* xt_off, const float* scale,
-                             const void* mats, const void* x, float* xsc, void* xt, void* Yp,
-                             const dds_pcg_state* gate, void* stream);
+                             int64_t max_k, int64_t pairs_per_item, const int32_t* items,
+                             const int64_t* sub_ptr, const int32_t* sub_idx,
+                             const int64_t* mat_off, const int64_t* ypos, const int64_t* xt_off,
+                             const float* scale, const void* mats, const void* x, float* xsc,
+                             void* xt, void* Yp, const dds_pcg_state* gate, void* stream);
 
-/* y[sub_ptr[s] + a] = sum of the partial vectors of dds_local_apply_sym that cover local
- * row a of subdomain s (items p = 0 .. T_s-1-a/64), in item order: the result indexed like
- * the output of dds_local_apply. One lane per (s, a), coalesced; n_s <= 65535. */
-int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, const int64_t* sub_ptr,
-                    const int64_t* ypos, const void* Yp, void* y, const dds_pcg_state* gate,
-                    void* stream);
+/* y[sub_ptr[s] + a] = sum of the partial vectors that cover local row a of subdomain s, in
+ * item order: items q = 0 .. min(Q_s - 1, (T_s-1-a/64) / G), Q_s = ceil(P_s / G), with
+ * G = pairs_per_item (1 for dds_local_apply_sym): the result indexed like the output of
+ * dds_local_apply. One lane per (s, a), coalesced; max_k <= 65535 * 256. */
+int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, int64_t pairs_per_item,
+                    const int64_t* sub_ptr, const int64_t* ypos, const void* Yp, void* y,
+                    const dds_pcg_state* gate, void* stream);
 
 /* z[j] = sum_{p in [dof_ptr[j], dof_ptr[j+1])} Y[dof_pos[p]]   (dof_ptr == NULL: z[j] = Y[j])
  *        + coarse_sign * yc[cell[j]]                           (yc != NULL)

@@ -321,9 +321,10 @@ def test_symm_layout_model(k):
     """The matrix-core symmetric layout (_symm_index) read the way sym_mfma_apply_kernel reads
     it: tile offsets and block indices from the kernel's closed forms, the 16x16x32 MFMA lane
     rules (A: row l%16, k = 8(l/16)+e; B: column l%16, k = 8(l/16)+e; D: rows 4(l/16)+i,
-    column l%16), the LDS copy with its row pitch and the ds_read_tr16_b64 lane rule (in each
-    group of 16 lanes, lane (r, q) names 4 elements of a row; lane c receives element c%4 of
-    lane (i, c//4) for i < 4). Items and the combine then reproduce A x exactly."""
+    column l%16), the tile's blocks as the loads into LDS leave them (chunks in swizzled
+    slots) and the ds_read_tr16_b64 lane rule (in each group of 16 lanes, lane (r, q) names 4
+    elements of a row; lane c receives element c%4 of lane (i, c//4) for i < 4). Items and
+    the combine then reproduce A x exactly; every transposed read hits written LDS."""
     from dd_solvers.solvers import _sym_plan, _symm_index
 
     rng = np.random.default_rng(k)
@@ -359,7 +360,6 @@ def test_symm_layout_model(k):
                 c[q] = packed[o:o + 512].reshape(64, 8)
         return c
 
-    P_ = 72  # kSymLdsPitch
     lanes = np.arange(64)
     xp = np.zeros(64 * T)
     x = rng.standard_normal(k)
@@ -379,27 +379,34 @@ def test_symm_layout_model(k):
         # D lane rule: rows 4(l/16)+i of column 0 sit in lane 16 g, i.e. D[Rb][:, 0]
         return np.concatenate([D[Rb][:, 0] for Rb in range(4)])
 
+    def slot(c):  # symm_slot: LDS slot of chunk c in a block (its own inverse)
+        return c ^ (((c >> 4) & 1) << 2)
+
     def transposed_product(c, xI):
-        lds = np.full(64 * P_, np.nan)
+        # the tile as the loads into LDS leave it: block q in 1 KiB, slot s holding chunk
+        # slot(s) (16 bytes = 8 elements)
+        lds = np.full((8, 64, 8), np.nan)
         for q in range(8):
-            for l in lanes:
-                row = 16 * (q >> 1) + l % 16
-                col = 32 * (q & 1) + 8 * (l // 16)
-                lds[row * P_ + col:row * P_ + col + 8] = c[q][l]
+            for s_ in lanes:
+                lds[q, s_] = c[q][slot(s_)]
+        flat = lds.reshape(-1)  # element (2-byte) addresses
+        g, r, q4 = lanes // 16, (lanes >> 2) & 3, lanes & 3
         out = np.zeros(64)
         for nb in range(4):
             D = np.zeros((16, 16))
             for kr2 in range(2):
-                def tr(extra):
-                    addr = np.array([(32 * kr2 + 8 * (l // 16) + (l >> 2 & 3)) * P_ + 16 * nb
-                                     + 4 * (l & 3) + extra for l in lanes])
+                def tr(h):
+                    rho = 32 * kr2 + 8 * g + r + 4 * h
+                    blk = 2 * (rho >> 4) + (nb >> 1)
+                    ln = (rho & 15) + 16 * (2 * (nb & 1) + (q4 >> 1))
+                    addr = blk * 512 + np.array([slot(int(v)) for v in ln]) * 8 + 4 * (q4 & 1)
                     got = np.zeros((64, 4))
-                    for l in lanes:
-                        g, cc = l // 16, l % 16
+                    for l in lanes:  # lane c of a 16-lane group: element c % 4 of lane (i, c // 4)
+                        gg, cc = l // 16, l % 16
                         for i in range(4):
-                            got[l, i] = lds[addr[16 * g + 4 * i + cc // 4] + cc % 4]
+                            got[l, i] = flat[addr[16 * gg + 4 * i + cc // 4] + cc % 4]
                     return got
-                bt = np.concatenate([tr(0), tr(4 * P_)], axis=1)
+                bt = np.concatenate([tr(0), tr(1)], axis=1)
                 Am = np.zeros((16, 32))
                 Bm = np.zeros((32, 16))
                 for l in lanes:
@@ -410,35 +417,41 @@ def test_symm_layout_model(k):
             out[16 * nb:16 * nb + 16] = D[0]
         return out
 
-    items, ypos = _sym_plan(np.array([k]))
-    Yp = np.full(int(ypos[-1]), np.nan)
-    for s_, p in items:
-        I1, I2 = p, T - 1 - p
-        two = I1 != I2
-        y = np.full(64 * T, np.nan)
-        rows = {I1: np.zeros(64), I2: np.zeros(64)}
-        hold = {I1: np.zeros(64), I2: np.zeros(64)}
-        for J in range(I2 + 1):
-            dt = np.zeros(64)
-            for w, I in ((0, I2), (1, I1)):
-                if w == 1 and not (two and J <= I1):
-                    continue
-                c = blocks(I, J)
-                rows[I] += row_product(c, xp[64 * J:64 * J + 64])
-                dt += transposed_product(c, xp[64 * I:64 * I + 64])
-            if J == I2 or (two and J == I1):
-                hold[J] = dt
-            else:
-                y[64 * J:64 * J + 64] = dt
-        if two:
-            y[64 * I1:64 * I1 + 64] = rows[I1] + hold[I1]
-        y[64 * I2:64 * I2 + 64] = rows[I2] + hold[I2]
-        Yp[ypos[s_] + p * 64 * T: ypos[s_] + (p + 1) * 64 * T] = y
     P = -(-T // 2)
-    z = np.array([sum(Yp[ypos[0] + p * 64 * T + a] for p in range(min(P - 1, T - 1 - a // 64) + 1))
-                  for a in range(k)])
-    assert not np.isnan(z).any()
-    assert np.allclose(z, A @ x, rtol=1e-12, atol=1e-12)
+    for G in sorted({1, 2, P}):
+        # items of G pairs: the sums accumulate per item (yv), each touched row written once
+        items, ypos = _sym_plan(np.array([k]), G)
+        Q = -(-P // G)
+        assert items.shape[0] == Q
+        Yp = np.full(int(ypos[-1]), np.nan)
+        for s_, q in items:
+            I2max = T - 1 - q * G
+            yv = np.zeros(64 * (I2max + 1))
+            for p in range(q * G, min(P, q * G + G)):
+                I1, I2 = p, T - 1 - p
+                two = I1 != I2
+                rows = {I1: np.zeros(64), I2: np.zeros(64)}
+                for J in range(I2 + 1):
+                    dt = np.zeros(64)
+                    for w, I in ((0, I2), (1, I1)):
+                        if w == 1 and not (two and J <= I1):
+                            continue
+                        c = blocks(I, J)
+                        rows[I] += row_product(c, xp[64 * J:64 * J + 64])
+                        dt += transposed_product(c, xp[64 * I:64 * I + 64])
+                    yv[64 * J:64 * J + 64] += dt
+                if two:
+                    yv[64 * I1:64 * I1 + 64] += rows[I1]
+                yv[64 * I2:64 * I2 + 64] += rows[I2]
+            n_rows = min(k, 64 * (I2max + 1))
+            o = ypos[s_] + q * 64 * T
+            Yp[o:o + n_rows] = yv[:n_rows]
+        # the combine (sym_combine_kernel): row a sums items q = 0 .. min(Q-1, (T-1-a//64)//G);
+        # with one item (G >= P) that item's vector is the result
+        z = np.array([sum(Yp[ypos[0] + q * 64 * T + a] for q in range(min(Q - 1, (T - 1 - a // 64) // G) + 1))
+                      for a in range(k)])
+        assert not np.isnan(z).any()
+        assert np.allclose(z, A @ x, rtol=1e-12, atol=1e-12)
 
 
 @pytest.mark.parametrize("k", [3, 16, 17, 33, 100])
@@ -517,3 +530,96 @@ def test_sell_compact_column_format():
                 c32[~cp[q] + 64 * j + np.arange(64)]
             assert np.array_equal(got, col[slot]), (q, j)
     assert d16.shape[0] == 64 * (n_slots - 4) and c32.shape[0] == 64 * 4
+
+
+def _symm_schedule(k, G, q, NS, NT):
+    """Restates the control flow of sym_mfma_apply_kernel for one item (subdomain of k
+    unknowns, pairs q G .. q G + G - 1, tile rows p and T-1-p of each in increasing order)
+    and checks, with vector memory operations completing in issue order: the x terms are in
+    before the loop; each wait count inside the loop (8 (NS - 1)) equals the loads issued
+    after the awaited tile's; the awaited tile is still in its stage and is the tile computed;
+    a stage is reloaded only after its task was computed; the item reads every tile of its
+    rows once, a whole subdomain in storage order."""
+    T = -(-k // 64)
+    P = -(-T // 2)
+    p0, p_end = q * G, min(P, q * G + G)
+    I2max = T - 1 - p0
+    nlow = p_end - p0
+    nrows = nlow + (T - p0) - max(T - p_end, p_end)
+
+    def row(i):
+        return p0 + i if i < nlow else max(T - p_end, p_end) + (i - nlow)
+
+    rows = [row(i) for i in range(nrows)]
+    assert sorted(rows) == sorted({p for p in range(p0, p_end)} | {T - 1 - p for p in range(p0, p_end)})
+    assert rows == sorted(rows)
+
+    def advance(c):
+        i, I, J = c
+        J += 1
+        if J > I:
+            i, J = i + 1, 0
+            I = row(i)
+        return (i, I, J)
+
+    NF = NS - 1
+    ntot = sum(I + 1 for I in rows)
+    xdw = (I2max + 1) * NT * 128 // 4
+    queue = [("x",)] * ((xdw + 63) // 64)
+    stage_task = {}
+
+    def issue(t, c):
+        prev = stage_task.get(t % NS)
+        real = c[0] < nrows
+        queue.extend([("tile", t, (c[1], c[2]) if real else None)] * 8)  # 8 loads, always
+        stage_task[t % NS] = t
+        return prev, (advance(c) if real else c)
+
+    ic = (0, row(0), 0)
+    for t in range(NF):
+        _, ic = issue(t, ic)
+    assert max(i for i, e in enumerate(queue) if e == ("x",)) <= len(queue) - 1 - 8 * NF
+    cc = (0, row(0), 0)
+    seen = []
+    for t in range(ntot):
+        prev, ic = issue(t + NF, ic)
+        assert prev is None or prev < t             # the stage's task was computed
+        last = max(i for i, e in enumerate(queue) if e[0] == "tile" and e[1] == t)
+        assert len(queue) - 1 - last == 8 * NF      # the exact count after t's loads
+        assert stage_task[t % NS] == t
+        assert queue[last][2] == (cc[1], cc[2])     # the tile loaded is the task computed
+        assert cc[2] <= cc[1] <= I2max
+        seen.append((cc[1], cc[2]))
+        cc = advance(cc)
+    assert cc[0] == nrows
+    assert seen == [(I, J) for I in rows for J in range(I + 1)]
+    if G >= P:
+        assert seen == [(I, J) for I in range(T) for J in range(I + 1)]  # storage order
+
+
+@pytest.mark.parametrize("NS,NT", [(3, 2), (3, 3), (2, 2), (4, 3)])
+def test_symm_schedule(NS, NT):
+    for k in (1, 63, 64, 65, 127, 128, 129, 700, 1000, 2049, 4096):
+        T = -(-k // 64)
+        P = -(-T // 2)
+        for G in sorted({1, 2, 3, P}):
+            for q in range(-(-P // G)):
+                _symm_schedule(k, G, q, NS, NT)
+
+
+def test_symm_group_choice():
+    from dd_solvers.solvers import _sym_plan, _symm_group, _sym_written_rows
+    many = np.full(4096, 1000)
+    assert _symm_group(many, 256) == 8          # whole subdomains: one item each
+    items, ypos = _sym_plan(many, 8)
+    assert items.shape[0] == 4096
+    few = np.full(64, 4000)                     # T = 63, P = 32 pairs
+    g = _symm_group(few, 256)
+    assert g == 1 or 64 * -(-32 // g) >= 32 * 256
+    items, ypos = _sym_plan(few, g)
+    assert _sym_written_rows(few, g) <= int(ypos[-1])
+    # items longest first
+    T = 63
+    work = [sum((T + 1) if p != T - 1 - p else (T - p) for p in range(q * g, min(32, q * g + g)))
+            for _, q in items]
+    assert work == sorted(work, reverse=True)
