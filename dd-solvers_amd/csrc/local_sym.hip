@@ -323,16 +323,18 @@ __global__ __launch_bounds__(256) void sym_combine_kernel(int G, const int64_t* 
 //                  = 512 (8 (T-1)(T-2)/2 + 6 (T-1) + 2 nb(rL) J)            for I = T-1
 // Padding is zero.
 //
-// x is taken per subdomain relative to sigma_s = 2^e >= max |x| over S_s (sym_pack_x_kernel)
-// and split into NT 16-bit terms (FP16: hi + lo; BF16: hi + mid + lo), which keeps about 22
-// (FP16) or 24 (BF16) bits of x / sigma_s.
-//   Row product y_I += T x_J: D = A(T block) x B, B column n < NT holding term n of x_J.
-//   Transposed product y_J += T^T x_I: D = A' x B', A' row m < NT holding term m of x_I, B'
-//   the tile block in column-major order, read back from an LDS copy of the tile with
-//   ds_read_tr16_b64 (in each group of 16 lanes, lane (r, q) names 4 consecutive elements of
-//   row r; lane c receives element c of the 4 rows).
-// Both products accumulate in FP32 in the MFMA; the NT term columns (rows) are added and
-// scaled back by sigma_s * scale[s] when a block of the result is written.
+// x is packed once per apply (sym_pack_x_kernel): each 64-row block J of a subdomain's x is
+// taken relative to sig_J = 2^e > max |x| over the block and split into NT 16-bit terms
+// (FP16: hi + lo; BF16: hi + mid + lo), which keeps about 22 (FP16) or 24 (BF16) bits of
+// x / sig_J; the apply loads the terms and scales of its blocks into LDS with its first tiles.
+//   Row product y_I += T x_J: D = A(T block) x B, B column n < NT holding term n of x_J / sig_J;
+//   y_I += sig_J D.
+//   Transposed product y_J += T^T x_I: D = A' x B', A' row m < NT holding term m of
+//   x_I / sig_I, B' the tile block in column-major order, read back from the LDS copy of the
+//   tile with ds_read_tr16_b64 (in each group of 16 lanes, lane (r, q) names 4 consecutive
+//   elements of row r; lane c receives element c of the 4 rows); y_J += sig_I D.
+// The products accumulate in FP32 (MFMA per tile, then the scaled sums); the NT term columns
+// (rows) are added, and scale[s] applied when the result is written.
 // ---------------------------------------------------------------------------------------
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -388,50 +390,89 @@ __device__ __forceinline__ typename SymMfma<TM>::op_t as_op(const Vec16<TM>& v) 
   return o;
 }
 
-// Pack of x per subdomain, once per apply. sigma_s = 2^e with max_{a in S_s} |x[g(a)]| < 2^e
-// (1 when x is zero on S_s; e clamped to the normal float range) goes to xsc[s]; x / sigma_s
-// is split into NT 16-bit terms stored in MFMA operand order: for each 32-row block c of
-// the subdomain, term t of rows 32 c + j (j < 32) at xt[xt_off[s] + (c NT + t) 32 + j]. A
-// lane l with t = l % 16 < NT then loads its operand (rows 32 c + 8 (l/16) + e, e < 8) as
-// 16 contiguous bytes. Rows past k_s keep the zeros written at setup. One wavefront per
-// subdomain.
-template <typename TM, typename TV>
-__global__ __launch_bounds__(256) void sym_pack_x_kernel(int64_t n_s,
-                                                         const int64_t* __restrict__ sub_ptr,
-                                                         const int32_t* __restrict__ sub_idx,
-                                                         const int64_t* __restrict__ xt_off,
-                                                         const TV* __restrict__ x,
-                                                         float* __restrict__ xsc,
-                                                         uint16_t* __restrict__ xt,
-                                                         const dds_pcg_state* __restrict__ gate) {
+// Pack of x, once per apply, per 64-row block b (blk_owner[b] = s, the block's index in its
+// subdomain J = b - blk_ptr[s]; lane = row 64 J + lane): sig[b] = 2^e with max |x| over the
+// block's rows < 2^e (1 when they are zero; e clamped to the normal float range) and
+// x / sig[b] split into NT 16-bit terms at xt[b NT 64 + (2 (lane/32) NT + t) 32 + lane % 32]
+// (rows past k zero). A wavefront packs kPackBlocks consecutive blocks; four wavefronts per
+// workgroup.
+constexpr int kPackBlocks = 8;
+
+// The kPackBlocks blocks from b0 of one wavefront (IDX: rows gathered through sub_idx). Every
+// load is unconditional, at clamped in-range indices (a block's subdomain has k >= 1), so
+// the blocks' dependent loads are in flight together; out-of-range rows are selected away.
+template <typename TM, typename TV, bool IDX>
+__device__ __forceinline__ void sym_pack_blocks(int64_t b0, int64_t n_blk, int lane,
+                                                const int32_t* __restrict__ blk_owner,
+                                                const int64_t* __restrict__ blk_ptr,
+                                                const int64_t* __restrict__ sub_ptr,
+                                                const int32_t* __restrict__ sub_idx,
+                                                const TV* __restrict__ x, float* __restrict__ sig,
+                                                uint16_t* __restrict__ xt) {
   using S = SymMfma<TM>;
   constexpr int NT = S::NT;
-  if (gate != nullptr && gate->done) return;
-  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (s >= n_s) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t base = sub_ptr[s];
-  const int k = (int)(sub_ptr[s + 1] - base);
-  auto at = [&](int a) -> double {
-    const int64_t g = sub_idx != nullptr ? (int64_t)sub_idx[base + a] : base + a;
-    return static_cast<double>(x[g]);
-  };
-  double m = 0.0;
-  for (int a = lane; a < k; a += 64) m = fmax(m, fabs(at(a)));
+  int own[kPackBlocks];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-  int e = 0;
-  if (m > 0.0) frexp(m, &e);
-  e = e < -126 ? -126 : (e > 127 ? 127 : e);
-  const double inv = ldexp(1.0, -e);  // exact
-  if (lane == 0) xsc[s] = ldexpf(1.0f, e);
-  uint16_t* out = xt + xt_off[s];
-  for (int a = lane; a < k; a += 64) {
-    const float v = static_cast<float>(at(a) * inv);
-    const int c = a >> 5, j = a & 31;
+  for (int u = 0; u < kPackBlocks; ++u)
+    own[u] = __builtin_amdgcn_readfirstlane(blk_owner[b0 + u < n_blk ? b0 + u : n_blk - 1]);
+  int64_t row[kPackBlocks];
+  int ok[kPackBlocks];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) out[(c * NT + t) * 32 + j] = S::term(v, t);
+  for (int u = 0; u < kPackBlocks; ++u) {
+    const int64_t b = b0 + u < n_blk ? b0 + u : n_blk - 1;
+    const int64_t base = sub_ptr[own[u]];
+    const int k = (int)(sub_ptr[own[u] + 1] - base);
+    const int a = 64 * (int)(b - blk_ptr[own[u]]) + lane;
+    ok[u] = a < k;
+    row[u] = base + (a < k ? a : k - 1);
   }
+  if constexpr (IDX) {
+#pragma unroll
+    for (int u = 0; u < kPackBlocks; ++u) row[u] = sub_idx[row[u]];
+  }
+  double v[kPackBlocks];
+#pragma unroll
+  for (int u = 0; u < kPackBlocks; ++u) {
+    const double xv = static_cast<double>(x[row[u]]);
+    v[u] = ok[u] ? xv : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kPackBlocks; ++u) {
+    const int64_t b = b0 + u;
+    if (b < n_blk) {
+      double m = fabs(v[u]);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+      int e = 0;
+      if (m > 0.0) frexp(m, &e);
+      e = e < -126 ? -126 : (e > 127 ? 127 : e);
+      const float vs = static_cast<float>(v[u] * ldexp(1.0, -e));  // exact scaling
+      uint16_t* out = xt + b * NT * 64;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) out[((lane >> 5) * NT + t) * 32 + (lane & 31)] = S::term(vs, t);
+      if (lane == 0) sig[b] = ldexpf(1.0f, e);
+    }
+  }
+}
+
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void sym_pack_x_kernel(int64_t n_blk,
+                                                         const int32_t* __restrict__ blk_owner,
+                                                         const int64_t* __restrict__ blk_ptr,
+                                                         const int64_t* __restrict__ sub_ptr,
+                                                         const int32_t* __restrict__ sub_idx,
+                                                         const TV* __restrict__ x,
+                                                         float* __restrict__ sig,
+                                                         uint16_t* __restrict__ xt,
+                                                         const dds_pcg_state* __restrict__ gate) {
+  if (gate != nullptr && gate->done) return;
+  const int64_t b0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kPackBlocks;
+  if (b0 >= n_blk) return;
+  const int lane = threadIdx.x & 63;
+  if (sub_idx != nullptr)
+    sym_pack_blocks<TM, TV, true>(b0, n_blk, lane, blk_owner, blk_ptr, sub_ptr, sub_idx, x, sig, xt);
+  else
+    sym_pack_blocks<TM, TV, false>(b0, n_blk, lane, blk_owner, blk_ptr, sub_ptr, sub_idx, x, sig, xt);
 }
 
 // Buffer resource (gfx9 family): 64-bit base, byte range; loads past the range return zero.
@@ -607,10 +648,11 @@ static_assert(kSymmStages >= 2 && 8 * (kSymmStages - 1) <= 63, "vmcnt range");
 template <typename TM, typename TV>
 __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
     int G, int Tmax, const int32_t* __restrict__ items, const int64_t* __restrict__ sub_ptr,
-    const int64_t* __restrict__ mat_off, const int64_t* __restrict__ ypos,
-    const int64_t* __restrict__ xt_off, const float* __restrict__ scale,
-    const float* __restrict__ xsc, const uint16_t* __restrict__ xt, const TM* __restrict__ mats,
-    TV* __restrict__ Yp, const dds_pcg_state* __restrict__ gate) {
+    const int64_t* __restrict__ blk_ptr, const int64_t* __restrict__ mat_off,
+    const int64_t* __restrict__ ypos, const float* __restrict__ scale,
+    const TM* __restrict__ mats, const float* __restrict__ sigx,
+    const uint16_t* __restrict__ xt, TV* __restrict__ Yp,
+    const dds_pcg_state* __restrict__ gate) {
   using S = SymMfma<TM>;
   using op_t = typename S::op_t;
   constexpr int NT = S::NT;
@@ -620,12 +662,13 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   if (gate != nullptr && gate->done) return;
   extern __shared__ __align__(16) unsigned char sym_smem[];
   // [NS stages x 8 blocks x 64 lanes x 16 B][16 zero bytes][yl: 64 floats]
-  // [x terms: Tmax XB + 256][yv: 64 Tmax floats]
+  // [x terms: Tmax XB + 256][yv: 64 Tmax floats][sig: 64 ceil(Tmax / 64) floats]
   unsigned char* stages = sym_smem;
   unsigned char* zero16 = sym_smem + NS * 8192;
   float* yl = reinterpret_cast<float*>(zero16 + 16);
   unsigned char* xtl = zero16 + 16 + 256;
   float* yv = reinterpret_cast<float*>(xtl + Tmax * XB + 256);
+  float* sig = yv + 64 * Tmax;
 
   const int lane = threadIdx.x;
   const int s = items[2 * blockIdx.x];
@@ -639,26 +682,30 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   const int nrows = (p_end - p0) + (T - p0) - max(T - p_end, p_end);
   int ntot = 0;
   for (int i = 0; i < nrows; ++i) ntot += symm_row(T, p0, p_end, i) + 1;
-  const double out_sc = static_cast<double>(xsc[s]) *
-                        (scale != nullptr ? static_cast<double>(scale[s]) : 1.0);
+  const double out_sc = scale != nullptr ? static_cast<double>(scale[s]) : 1.0;
   const i32x4_t rm = buffer_rsrc(mats + mat_off[s], (uint32_t)(symm_store_elems(k) * 2));
-  const i32x4_t rx = buffer_rsrc(xt + xt_off[s], (uint32_t)(((k + 31) / 32) * NT * 64));
   const uint32_t stage0 = lds_addr(stages);
   const uint32_t voff = 16u * (uint32_t)symm_slot(lane);  // the chunk this lane's slot receives
 
   if (lane < 4) reinterpret_cast<float*>(zero16)[lane] = 0.f;
   for (int a = lane; a < 64 * (I2max + 1); a += 64) yv[a] = 0.f;
-  // x terms of blocks 0..I2max into LDS (zeros past the subdomain), then the first tiles
+  // the x terms and scales of blocks 0..I2max (zeros past the subdomain), then the first
+  // tiles
+  const int64_t b0x = blk_ptr[s];
+  const i32x4_t rx = buffer_rsrc(xt + b0x * NT * 64, (uint32_t)(T * NT * 128));
+  const i32x4_t rs = buffer_rsrc(sigx + b0x, (uint32_t)(T * 4));
   const int xdw = (I2max + 1) * XB / 4;  // dwords
   for (int d0 = 0; d0 < xdw; d0 += 64)
     dma4(lds_addr(xtl + 4 * d0), (uint32_t)lane * 4u, rx, 4u * (uint32_t)d0);
+  for (int d0 = 0; d0 <= I2max; d0 += 64)
+    dma4(lds_addr(sig + d0), (uint32_t)lane * 4u, rs, 4u * (uint32_t)d0);
   SymmCursor ic = {0, symm_row(T, p0, p_end, 0), 0};  // issue cursor
 #pragma unroll
   for (int t = 0; t < NF; ++t) {
     symm_issue_at(stage0 + (uint32_t)(t % NS) * 8192u, ic, nrows, T, rL, nbL, rm, voff);
     if (ic.i < nrows) ic = symm_advance(ic, T, p0, p_end);
   }
-  wait_vm<8 * NF>();  // the x terms (issued before every tile) are in
+  wait_vm<8 * NF>();  // the x terms and scales (issued before every tile) are in
   wave_lds_fence();
 
   const int tm = lane & 15;
@@ -675,9 +722,10 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   f32x4_t d[4];   // row product of the current tile row, row blocks 0..3
   f32x4_t dt[4];  // transposed product of the current tile, column blocks 0..3
   op_t a0, a1, b0, b1;
+  float sI = 1.f;  // sig of the current tile row
 
   const int g = lane >> 4, r = (lane >> 2) & 3, q4 = lane & 3;
-  auto tile = [&](int t) __attribute__((always_inline)) {
+  auto tile = [&](int t, float sJ) __attribute__((always_inline)) {
     const unsigned char* sb = stages + (t % NS) * 8192;
 #pragma unroll
     for (int Rb = 0; Rb < 4; ++Rb) {
@@ -686,8 +734,9 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
       op_t o0, o1;
       __builtin_memcpy(&o0, &c0, 16);
       __builtin_memcpy(&o1, &c1, 16);
-      d[Rb] = S::mfma(o0, b0, d[Rb]);
-      d[Rb] = S::mfma(o1, b1, d[Rb]);
+      f32x4_t p = S::mfma(o0, b0, z4);
+      p = S::mfma(o1, b1, p);
+      d[Rb] += sJ * p;  // T x_J = sig[J] (T (x_J / sig[J]))
     }
     // rows 32 kr2 + 8 g + r (+4 h), columns 16 nb + 4 q4 of the tile, read column-major
     // (ds_read_tr16_b64) from the lane-ordered blocks
@@ -711,15 +760,16 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
         dt[nb] = S::mfma(kr2 == 0 ? a0 : a1, bt, dt[nb]);
       }
   };
-  // yv[64 J + lane] += the transposed part (rows 16 nb + lane of D, the NT term rows added)
-  auto add_transposed = [&](int J) __attribute__((always_inline)) {
+  // yv[64 J + lane] += sI x the transposed part (rows 16 nb + lane of D, the NT term rows
+  // added)
+  auto add_transposed = [&](int J, float sI) __attribute__((always_inline)) {
     if (lane < 16) {
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) {
         float v = dt[nb][0];
 #pragma unroll
         for (int t = 1; t < NT; ++t) v += dt[nb][t];
-        yl[16 * nb + lane] = v;
+        yl[16 * nb + lane] = sI * v;
       }
     }
     wave_lds_fence();
@@ -753,6 +803,7 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
     if (cc.J == 0) {  // a new tile row
       a0 = x_op(cc.I, 0);
       a1 = x_op(cc.I, 1);
+      sI = sig[cc.I];
 #pragma unroll
       for (int i = 0; i < 4; ++i) d[i] = z4;
     }
@@ -760,8 +811,8 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
     b1 = x_op(cc.J, 1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) dt[i] = z4;
-    tile(t);
-    add_transposed(cc.J);
+    tile(t, sig[cc.J]);
+    add_transposed(cc.J, sI);
     if (cc.J == cc.I) add_row(cc.I);  // the row is complete
     cc = symm_advance(cc, T, p0, p_end);
   }
@@ -829,16 +880,19 @@ extern "C" int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, 
   return DDS_EUNSUPPORTED;
 }
 
-// Launch: the x pack, then the apply. pairs_per_item: G as above.
+// Launch: the x pack (sym_pack_x_kernel, into sig and xt), then the apply. pairs_per_item:
+// G as above.
 extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s,
                                         int64_t n_items, int64_t max_k, int64_t pairs_per_item,
                                         const int32_t* items, const int64_t* sub_ptr,
                                         const int32_t* sub_idx, const int64_t* mat_off,
-                                        const int64_t* ypos, const int64_t* xt_off,
-                                        const float* scale, const void* mats, const void* x,
-                                        float* xsc, void* xt, void* Yp,
+                                        const int64_t* ypos, const float* scale,
+                                        const void* mats, const void* x, int64_t n_blk,
+                                        const int32_t* blk_owner, const int64_t* blk_ptr,
+                                        float* sig, void* xt, void* Yp,
                                         const dds_pcg_state* gate, void* stream) {
-  DDS_REQUIRE(n_s >= 0 && n_items >= 0 && n_items <= INT32_MAX && ceil_div(n_s, 4) <= INT32_MAX,
+  DDS_REQUIRE(n_s >= 0 && n_items >= 0 && n_items <= INT32_MAX && n_blk >= 0 &&
+                  ceil_div(n_blk, 4 * kPackBlocks) <= INT32_MAX,
               "dds_local_apply_sym_mfma: bad sizes");
   DDS_REQUIRE(max_k >= 0 && max_k <= kSymMfmaMaxK,
               "dds_local_apply_sym_mfma: subdomains of at most %d unknowns (got %lld)",
@@ -847,10 +901,10 @@ extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_
               "dds_local_apply_sym_mfma: pairs_per_item %lld out of range",
               (long long)pairs_per_item);
   if (n_items == 0 || n_s == 0) return DDS_OK;
-  DDS_REQUIRE(items && sub_ptr && mat_off && ypos && xt_off && mats && x && xsc && xt && Yp,
+  DDS_REQUIRE(items && sub_ptr && mat_off && ypos && mats && x && blk_owner && blk_ptr && sig &&
+                  xt && Yp,
               "dds_local_apply_sym_mfma: null pointer");
   hipStream_t st = as_stream(stream);
-  const dim3 g1((unsigned)ceil_div(n_s, 4)), g2((unsigned)n_items);
   const int Tmax = (int)ceil_div(max_k, 64);
   const int G = (int)pairs_per_item;
   int dev = 0, max_lds = 0;
@@ -859,21 +913,22 @@ extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_
 #define DDS_SYMM(TM, TV)                                                                       \
   do {                                                                                         \
     const size_t lds = (size_t)kSymmStages * 8192 + 16 + 256 +                                 \
-                       (size_t)Tmax * SymMfma<TM>::NT * 128 + 256 + (size_t)Tmax * 256;        \
+                       (size_t)Tmax * (SymMfma<TM>::NT * 128 + 256) + 256 +                    \
+                       (size_t)256 * ceil_div(Tmax, 64);                                       \
     DDS_REQUIRE(lds <= (size_t)max_lds,                                                        \
                 "dds_local_apply_sym_mfma: %zu bytes of LDS for max_k %lld exceed the %d "     \
                 "per workgroup", lds, (long long)max_k, max_lds);                              \
     if (lds > 65536)                                                                           \
       DDS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sym_mfma_apply_kernel<TM, TV>), \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
-    hipLaunchKernelGGL((sym_pack_x_kernel<TM, TV>), g1, dim3(256), 0, st, n_s, sub_ptr,         \
-                       sub_idx, xt_off, static_cast<const TV*>(x), xsc,                        \
-                       static_cast<uint16_t*>(xt), gate);                                      \
+    hipLaunchKernelGGL((sym_pack_x_kernel<TM, TV>), dim3((unsigned)ceil_div(n_blk, 4 * kPackBlocks)), \
+                       dim3(256), 0, st, n_blk, blk_owner, blk_ptr, sub_ptr, sub_idx,          \
+                       static_cast<const TV*>(x), sig, static_cast<uint16_t*>(xt), gate);      \
     DDS_LAUNCHED();                                                                            \
-    hipLaunchKernelGGL((sym_mfma_apply_kernel<TM, TV>), g2, dim3(64), lds, st, G, Tmax, items, \
-                       sub_ptr, mat_off, ypos, xt_off, scale, xsc,                             \
-                       static_cast<const uint16_t*>(xt), static_cast<const TM*>(mats),         \
-                       static_cast<TV*>(Yp), gate);                                            \
+    hipLaunchKernelGGL((sym_mfma_apply_kernel<TM, TV>), dim3((unsigned)n_items), dim3(64), lds,  \
+                       st, G, Tmax, items, sub_ptr, blk_ptr, mat_off, ypos, scale,             \
+                       static_cast<const TM*>(mats), sig,                                      \
+                       static_cast<const uint16_t*>(xt), static_cast<TV*>(Yp), gate);          \
     DDS_LAUNCHED();                                                                            \
     return DDS_OK;                                                                             \
   } while (0)

@@ -1213,6 +1213,7 @@ class SchwarzOperator(SparseSolver):
         self._cast_buf = None
         self._mfma = None
         self.Yp = None
+        self._xpack = None
         if self.collect_timings:
             events = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
             events[0].record()
@@ -1552,13 +1553,16 @@ class SchwarzOperator(SparseSolver):
             direct = layout == "sym_mfma" and items.shape[0] == n_s
             # one item per subdomain writes the result itself (ypos = sub_ptr)
             self.Yp = None if direct else torch.empty(int(ypos[-1]), dtype=pi, device=dev)
-            if layout == "sym_mfma":  # per apply: power-of-two scales of x, x split in terms
+            if layout == "sym_mfma":  # per apply: x packed per 64-row block (scales, terms)
                 nt = 2 if store_dtype == torch.float16 else 3
-                words = 32 * nt * (-(-sizes.astype(np.int64) // 32))
-                self._xsc = torch.ones(n_s, dtype=torch.float32, device=dev)
-                self._xt_off = torch.as_tensor(np.concatenate([[0], np.cumsum(words)[:-1]]).astype(np.int64),
-                                               device=dev)
-                self._xt = torch.zeros(int(words.sum()), dtype=torch.int16, device=dev)
+                T = -(-sizes.astype(np.int64) // 64)
+                blk_ptr = np.concatenate([[0], np.cumsum(T)]).astype(np.int64)
+                n_blk = int(blk_ptr[-1])
+                self._xpack = dict(
+                    n_blk=n_blk, blk_ptr=torch.as_tensor(blk_ptr, device=dev),
+                    blk_owner=torch.as_tensor(np.repeat(np.arange(n_s, dtype=np.int32), T), device=dev),
+                    sig=torch.ones(n_blk, dtype=torch.float32, device=dev),
+                    xt=torch.zeros(n_blk * nt * 64, dtype=torch.int16, device=dev))
             alg = int((sizes.astype(np.int64) * (sizes + 1) // 2).sum()) * esz
         else:
             tiles = _tiles_for(sizes, rows_per_tile)
@@ -1626,12 +1630,14 @@ class SchwarzOperator(SparseSolver):
             if L.direct and Y is None:
                 raise ValueError("a direct sym_mfma apply writes the result: pass Y")
             out, pos = (Y, self.sub_ptr) if L.direct else (self.Yp, L.ypos)
+            xp = self._xpack
             _lib.call("dds_local_apply_sym_mfma", _lib.dtype_code(L.mat_dtype),
                       _lib.dtype_code(self.pi), self.n_solvers, L.n_items, L.max_k, L.group,
                       L.items.data_ptr(), self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx),
-                      L.mat_off.data_ptr(), pos.data_ptr(), self._xt_off.data_ptr(),
-                      _lib.ptr(L.scale), L.mats.data_ptr(), x.data_ptr(), self._xsc.data_ptr(),
-                      self._xt.data_ptr(), out.data_ptr(), _lib.ptr(gate), _lib.stream())
+                      L.mat_off.data_ptr(), pos.data_ptr(), _lib.ptr(L.scale),
+                      L.mats.data_ptr(), x.data_ptr(), xp["n_blk"], xp["blk_owner"].data_ptr(),
+                      xp["blk_ptr"].data_ptr(), xp["sig"].data_ptr(), xp["xt"].data_ptr(),
+                      out.data_ptr(), _lib.ptr(gate), _lib.stream())
             return
         _lib.call("dds_local_apply_sym", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
                   _lib.dtype_code(L.acc_dtype), L.n_items, L.items.data_ptr(),

@@ -189,18 +189,21 @@ int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, int64_t n_i
  * Yp[ypos[s] + q*64*T_s + a] for the local rows a < min(k_s, 64 (T_s - qG)) once. When
  * G >= P_s for every subdomain each subdomain has one item and Yp = y, ypos = sub_ptr gives
  * the result directly; otherwise dds_sym_combine sums the items with the same G. Per apply
- * two kernels run: the first writes, as device scratch, xsc[s] (n_s floats: the power of two
- * that bounds |x| on subdomain s) and xt (16-bit words: x / xsc[s] split into 2 (FP16) or 3
- * (BF16) terms, for each 32-row block c of subdomain s the NT x 32 words at
- * xt_off[s] + 32 NT c, words past k_s zero and left untouched); the second streams the tiles
- * into LDS. vec_dtype F64 or F32; max_k (the largest subdomain) at most 4096. Replaces the
- * same Inv.solve -> gemv_strided_batched step (solvers.py:355-359) as dds_local_apply,
+ * two kernels run: the first packs x (local row a of subdomain s is x[sub_idx[sub_ptr[s] + a]],
+ * or x[sub_ptr[s] + a] when sub_idx is NULL) per 64-row block into device scratch: block b
+ * (n_blk = blk_ptr[n_s] = sum_s ceil(k_s / 64) of them, blocks of subdomain s at
+ * blk_ptr[s] .., blk_owner[b] = s) gets sig[b] (n_blk floats, the power of two that bounds
+ * |x| on the block) and x / sig[b] split into 2 (FP16) or 3 (BF16) 16-bit terms in xt
+ * (NT x 64 words per block); the second streams the tiles into LDS. vec_dtype F64 or F32;
+ * max_k (the largest subdomain) at most 4096. Replaces the same
+ * Inv.solve -> gemv_strided_batched step (solvers.py:355-359) as dds_local_apply,
  * accumulating in FP32 as gemv_kernels.cu:47-54 does for FP32 vectors. */
 int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s, int64_t n_items,
                              int64_t max_k, int64_t pairs_per_item, const int32_t* items,
                              const int64_t* sub_ptr, const int32_t* sub_idx,
-                             const int64_t* mat_off, const int64_t* ypos, const int64_t* xt_off,
-                             const float* scale, const void* mats, const void* x, float* xsc,
+                             const int64_t* mat_off, const int64_t* ypos, const float* scale,
+                             const void* mats, const void* x, int64_t n_blk,
+                             const int32_t* blk_owner, const int64_t* blk_ptr, float* sig,
                              void* xt, void* Yp, const dds_pcg_state* gate, void* stream);
 
 /* y[sub_ptr[s] + a] = sum of the partial vectors that cover local row a of subdomain s, in

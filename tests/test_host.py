@@ -535,11 +535,11 @@ def test_sell_compact_column_format():
 def _symm_schedule(k, G, q, NS, NT):
     """Restates the control flow of sym_mfma_apply_kernel for one item (subdomain of k
     unknowns, pairs q G .. q G + G - 1, tile rows p and T-1-p of each in increasing order)
-    and checks, with vector memory operations completing in issue order: the x terms are in
-    before the loop; each wait count inside the loop (8 (NS - 1)) equals the loads issued
-    after the awaited tile's; the awaited tile is still in its stage and is the tile computed;
-    a stage is reloaded only after its task was computed; the item reads every tile of its
-    rows once, a whole subdomain in storage order."""
+    and checks, with vector memory operations completing in issue order: the x terms and
+    scales are in before the loop; each wait count inside the loop (8 (NS - 1)) equals the
+    loads issued after the awaited tile's; the awaited tile is still in its stage and is the
+    tile computed; a stage is reloaded only after its task was computed; the item reads every
+    tile of its rows once, a whole subdomain in storage order."""
     T = -(-k // 64)
     P = -(-T // 2)
     p0, p_end = q * G, min(P, q * G + G)
@@ -565,7 +565,7 @@ def _symm_schedule(k, G, q, NS, NT):
     NF = NS - 1
     ntot = sum(I + 1 for I in rows)
     xdw = (I2max + 1) * NT * 128 // 4
-    queue = [("x",)] * ((xdw + 63) // 64)
+    queue = [("x",)] * ((xdw + 63) // 64 + (I2max + 1 + 63) // 64)  # terms, then scales
     stage_task = {}
 
     def issue(t, c):
