@@ -248,9 +248,9 @@ def local_kernel_stats(M):
     sizes = M._host["sizes"].astype(np.int64)
     xv = torch.randn(M.n, dtype=torch.float64, device=M.device)
     Y = M.Y if M.Y is not None else torch.empty_like(xv)
+    pack = None
     if L.layout in ("sym", "sym_mfma"):
         from dd_solvers.solvers import _sym_written_rows
-        t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None, Y))
         if L.direct:  # one item per subdomain writes the result: no partial vectors
             t_comb, comb = 0.0, 0
             yp_rows = int(sizes.sum())
@@ -258,10 +258,26 @@ def local_kernel_stats(M):
             t_comb = kernel_time_ms(lambda: M._enqueue_sym_combine(Y, None))
             yp_rows = _sym_written_rows(sizes, L.group)
             comb = int(yp_rows * 8 + sizes.sum() * 8)
-        alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + yp_rows * 8)
-        kname = (f"sym_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
-                 if L.layout == "sym" else
-                 f"sym_mfma_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}> (+ sym_pack_x_kernel)")
+        if L.layout == "sym":
+            t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None, Y))
+            alg = int(L.bytes_algorithmic + sizes.sum() * (8 + 4) + yp_rows * 8)
+            kname = f"sym_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}, {_TNAME[L.acc_dtype]}>"
+        else:
+            # the two kernels apart: the pack of x (gathered x and indices in, terms and block
+            # scales out), then the apply alone on that pack (lower triangles, terms and
+            # scales in, the result out)
+            xp = M._xpack
+            nt = 2 if L.mat_dtype == torch.float16 else 3
+            xbytes = xp["n_blk"] * (nt * 128 + 4)
+            t_pack = kernel_time_ms(lambda: M._enqueue_sym_pack(xv, None))
+            M._enqueue_sym_pack(xv, None)
+            t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply_packed(None, Y))
+            alg = int(L.bytes_algorithmic + xbytes + yp_rows * 8)
+            pack_alg = int(sizes.sum() * (8 + (4 if M.sub_idx is not None else 0)) + xbytes)
+            pack = {"kernel": f"sym_pack_x_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}>",
+                    "avg_launch_us": round(t_pack * 1e3, 2), "alg_bytes_per_launch": pack_alg,
+                    "GBs": round(pack_alg / (t_pack * 1e-3) / 1e9, 1)}
+            kname = f"sym_mfma_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}>"
     else:
         t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
         t_comb, comb = 0.0, 0
@@ -271,7 +287,7 @@ def local_kernel_stats(M):
     return {"kernel": kname, "avg_launch_us": round(t_loc * 1e3, 2), "alg_bytes_per_launch": alg,
             "stored_matrix_bytes": int(L.bytes_matrices), "GBs": round(gbs, 1),
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "combine_kernel_us": round(t_comb * 1e3, 2),
-            "combine_alg_bytes": comb}
+            "combine_alg_bytes": comb, **({"pack": pack} if pack else {})}
 
 
 def spmv_stats(op, n, dev):
@@ -1020,6 +1036,8 @@ def main():
             "stored_matrix_bytes": kern["stored_matrix_bytes"],
             "combine_kernel_us": kern["combine_kernel_us"],
             "combine_alg_bytes": kern["combine_alg_bytes"],
+            "pack": kern.get("pack"),
+            "pack_traffic": measured_traffic(wl, "sym_pack"),
         },
         "solve_roofline": {"alg_bytes": int(solve_bytes),
                            "achieved_GBs": round(solve_bytes / mixed["tts_s"] / 1e9, 1),

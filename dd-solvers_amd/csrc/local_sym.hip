@@ -880,30 +880,51 @@ extern "C" int dds_local_apply_sym(int mat_dtype, int vec_dtype, int acc_dtype, 
   return DDS_EUNSUPPORTED;
 }
 
-// Launch: the x pack (sym_pack_x_kernel, into sig and xt), then the apply. pairs_per_item:
-// G as above.
-extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s,
-                                        int64_t n_items, int64_t max_k, int64_t pairs_per_item,
-                                        const int32_t* items, const int64_t* sub_ptr,
-                                        const int32_t* sub_idx, const int64_t* mat_off,
-                                        const int64_t* ypos, const float* scale,
-                                        const void* mats, const void* x, int64_t n_blk,
-                                        const int32_t* blk_owner, const int64_t* blk_ptr,
-                                        float* sig, void* xt, void* Yp,
-                                        const dds_pcg_state* gate, void* stream) {
-  DDS_REQUIRE(n_s >= 0 && n_items >= 0 && n_items <= INT32_MAX && n_blk >= 0 &&
-                  ceil_div(n_blk, 4 * kPackBlocks) <= INT32_MAX,
-              "dds_local_apply_sym_mfma: bad sizes");
+extern "C" int dds_sym_mfma_pack(int mat_dtype, int vec_dtype, int64_t n_s, int64_t n_blk,
+                                 const int32_t* blk_owner, const int64_t* blk_ptr,
+                                 const int64_t* sub_ptr, const int32_t* sub_idx, const void* x,
+                                 float* sig, void* xt, const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(n_s >= 0 && n_blk >= 0 && ceil_div(n_blk, 4 * kPackBlocks) <= INT32_MAX,
+              "dds_sym_mfma_pack: bad sizes");
+  if (n_blk == 0 || n_s == 0) return DDS_OK;
+  DDS_REQUIRE(blk_owner && blk_ptr && sub_ptr && x && sig && xt, "dds_sym_mfma_pack: null pointer");
+  hipStream_t st = as_stream(stream);
+  const dim3 grid((unsigned)ceil_div(n_blk, 4 * kPackBlocks));
+#define DDS_PACK(TM, TV)                                                                      \
+  do {                                                                                        \
+    hipLaunchKernelGGL((sym_pack_x_kernel<TM, TV>), grid, dim3(256), 0, st, n_blk, blk_owner,  \
+                       blk_ptr, sub_ptr, sub_idx, static_cast<const TV*>(x), sig,             \
+                       static_cast<uint16_t*>(xt), gate);                                     \
+    DDS_LAUNCHED();                                                                           \
+    return DDS_OK;                                                                            \
+  } while (0)
+  if (mat_dtype == DDS_F16 && vec_dtype == DDS_F64) DDS_PACK(_Float16, double);
+  if (mat_dtype == DDS_F16 && vec_dtype == DDS_F32) DDS_PACK(_Float16, float);
+  if (mat_dtype == DDS_BF16 && vec_dtype == DDS_F64) DDS_PACK(bf16_t, double);
+  if (mat_dtype == DDS_BF16 && vec_dtype == DDS_F32) DDS_PACK(bf16_t, float);
+#undef DDS_PACK
+  set_error("dds_sym_mfma_pack: unsupported dtypes (mat %d, vec %d)", mat_dtype, vec_dtype);
+  return DDS_EUNSUPPORTED;
+}
+
+extern "C" int dds_sym_mfma_apply_packed(int mat_dtype, int vec_dtype, int64_t n_s,
+                                         int64_t n_items, int64_t max_k, int64_t pairs_per_item,
+                                         const int32_t* items, const int64_t* sub_ptr,
+                                         const int64_t* blk_ptr, const int64_t* mat_off,
+                                         const int64_t* ypos, const float* scale,
+                                         const void* mats, const float* sig, const void* xt,
+                                         void* Yp, const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(n_s >= 0 && n_items >= 0 && n_items <= INT32_MAX,
+              "dds_sym_mfma_apply_packed: bad sizes");
   DDS_REQUIRE(max_k >= 0 && max_k <= kSymMfmaMaxK,
-              "dds_local_apply_sym_mfma: subdomains of at most %d unknowns (got %lld)",
+              "dds_sym_mfma_apply_packed: subdomains of at most %d unknowns (got %lld)",
               kSymMfmaMaxK, (long long)max_k);
   DDS_REQUIRE(pairs_per_item >= 1 && pairs_per_item <= kSymMfmaMaxK,
-              "dds_local_apply_sym_mfma: pairs_per_item %lld out of range",
+              "dds_sym_mfma_apply_packed: pairs_per_item %lld out of range",
               (long long)pairs_per_item);
   if (n_items == 0 || n_s == 0) return DDS_OK;
-  DDS_REQUIRE(items && sub_ptr && mat_off && ypos && mats && x && blk_owner && blk_ptr && sig &&
-                  xt && Yp,
-              "dds_local_apply_sym_mfma: null pointer");
+  DDS_REQUIRE(items && sub_ptr && blk_ptr && mat_off && ypos && mats && sig && xt && Yp,
+              "dds_sym_mfma_apply_packed: null pointer");
   hipStream_t st = as_stream(stream);
   const int Tmax = (int)ceil_div(max_k, 64);
   const int G = (int)pairs_per_item;
@@ -916,19 +937,15 @@ extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_
                        (size_t)Tmax * (SymMfma<TM>::NT * 128 + 256) + 256 +                    \
                        (size_t)256 * ceil_div(Tmax, 64);                                       \
     DDS_REQUIRE(lds <= (size_t)max_lds,                                                        \
-                "dds_local_apply_sym_mfma: %zu bytes of LDS for max_k %lld exceed the %d "     \
+                "dds_sym_mfma_apply_packed: %zu bytes of LDS for max_k %lld exceed the %d "    \
                 "per workgroup", lds, (long long)max_k, max_lds);                              \
     if (lds > 65536)                                                                           \
       DDS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sym_mfma_apply_kernel<TM, TV>), \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
-    hipLaunchKernelGGL((sym_pack_x_kernel<TM, TV>), dim3((unsigned)ceil_div(n_blk, 4 * kPackBlocks)), \
-                       dim3(256), 0, st, n_blk, blk_owner, blk_ptr, sub_ptr, sub_idx,          \
-                       static_cast<const TV*>(x), sig, static_cast<uint16_t*>(xt), gate);      \
-    DDS_LAUNCHED();                                                                            \
     hipLaunchKernelGGL((sym_mfma_apply_kernel<TM, TV>), dim3((unsigned)n_items), dim3(64), lds,  \
                        st, G, Tmax, items, sub_ptr, blk_ptr, mat_off, ypos, scale,             \
-                       static_cast<const TM*>(mats), sig,                                      \
-                       static_cast<const uint16_t*>(xt), static_cast<TV*>(Yp), gate);          \
+                       static_cast<const TM*>(mats), sig, static_cast<const uint16_t*>(xt),    \
+                       static_cast<TV*>(Yp), gate);                                            \
     DDS_LAUNCHED();                                                                            \
     return DDS_OK;                                                                             \
   } while (0)
@@ -937,8 +954,28 @@ extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_
   if (mat_dtype == DDS_BF16 && vec_dtype == DDS_F64) DDS_SYMM(bf16_t, double);
   if (mat_dtype == DDS_BF16 && vec_dtype == DDS_F32) DDS_SYMM(bf16_t, float);
 #undef DDS_SYMM
-  set_error("dds_local_apply_sym_mfma: unsupported dtypes (mat %d, vec %d)", mat_dtype, vec_dtype);
+  set_error("dds_sym_mfma_apply_packed: unsupported dtypes (mat %d, vec %d)", mat_dtype,
+            vec_dtype);
   return DDS_EUNSUPPORTED;
+}
+
+// The pack, then the apply. pairs_per_item: G as above.
+extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s,
+                                        int64_t n_items, int64_t max_k, int64_t pairs_per_item,
+                                        const int32_t* items, const int64_t* sub_ptr,
+                                        const int32_t* sub_idx, const int64_t* mat_off,
+                                        const int64_t* ypos, const float* scale,
+                                        const void* mats, const void* x, int64_t n_blk,
+                                        const int32_t* blk_owner, const int64_t* blk_ptr,
+                                        float* sig, void* xt, void* Yp,
+                                        const dds_pcg_state* gate, void* stream) {
+  if (n_items == 0 || n_s == 0) return DDS_OK;
+  const int rc = dds_sym_mfma_pack(mat_dtype, vec_dtype, n_s, n_blk, blk_owner, blk_ptr,
+                                   sub_ptr, sub_idx, x, sig, xt, gate, stream);
+  if (rc != DDS_OK) return rc;
+  return dds_sym_mfma_apply_packed(mat_dtype, vec_dtype, n_s, n_items, max_k, pairs_per_item,
+                                   items, sub_ptr, blk_ptr, mat_off, ypos, scale, mats, sig, xt,
+                                   Yp, gate, stream);
 }
 
 extern "C" int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, int64_t pairs_per_item,

@@ -1622,6 +1622,25 @@ class SchwarzOperator(SparseSolver):
                   L.mat_off.data_ptr(), L.ld.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(),
                   x.data_ptr(), Y.data_ptr(), self.waves or 4, _lib.ptr(gate), _lib.stream())
 
+    def _enqueue_sym_pack(self, x, gate) -> None:
+        """The x pack of the matrix-core symmetric apply (block scales and 16-bit terms)."""
+        L, xp = self.local, self._xpack
+        _lib.call("dds_sym_mfma_pack", _lib.dtype_code(L.mat_dtype), _lib.dtype_code(self.pi),
+                  self.n_solvers, xp["n_blk"], xp["blk_owner"].data_ptr(), xp["blk_ptr"].data_ptr(),
+                  self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx), x.data_ptr(),
+                  xp["sig"].data_ptr(), xp["xt"].data_ptr(), _lib.ptr(gate), _lib.stream())
+
+    def _enqueue_sym_apply_packed(self, gate, Y=None) -> None:
+        """The matrix-core symmetric apply from the last pack of x."""
+        L, xp = self.local, self._xpack
+        out, pos = (Y, self.sub_ptr) if L.direct else (self.Yp, L.ypos)
+        _lib.call("dds_sym_mfma_apply_packed", _lib.dtype_code(L.mat_dtype),
+                  _lib.dtype_code(self.pi), self.n_solvers, L.n_items, L.max_k, L.group,
+                  L.items.data_ptr(), self.sub_ptr.data_ptr(), xp["blk_ptr"].data_ptr(),
+                  L.mat_off.data_ptr(), pos.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(),
+                  xp["sig"].data_ptr(), xp["xt"].data_ptr(), out.data_ptr(), _lib.ptr(gate),
+                  _lib.stream())
+
     def _enqueue_sym_apply(self, x, gate, Y=None) -> None:
         """Partial products of every (subdomain, pair group) item into Yp; for a direct
         layout (one item per subdomain) the result itself into Y."""

@@ -19,6 +19,8 @@
  *                             generalised to uneven and overlapping subdomains
  *   dds_local_apply_sym       the same on symmetric tile-packed inverses
  *   dds_local_apply_sym_mfma  the same for FP16/BF16 inverses on the matrix cores
+ *   dds_sym_mfma_pack, dds_sym_mfma_apply_packed
+ *                             its two kernels, launched apart
  *   dds_local_apply_batched   the same on the reference's (n, k, k) layout (equal subdomains)
  *   dds_overlap_sum           y += repeat_interleave(...) prolongation (solvers.py:821-825,
  *                             :882-887) fused with the overlapped sum_i R_i^T y_i
@@ -205,6 +207,20 @@ int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_s, int64_t 
                              const void* mats, const void* x, int64_t n_blk,
                              const int32_t* blk_owner, const int64_t* blk_ptr, float* sig,
                              void* xt, void* Yp, const dds_pcg_state* gate, void* stream);
+
+/* The two kernels of dds_local_apply_sym_mfma as separate launches (same arguments): the x
+ * pack into sig / xt, and the apply from a packed x. A packed x stays valid until x changes,
+ * so the apply can be launched (and timed) alone on one pack. */
+int dds_sym_mfma_pack(int mat_dtype, int vec_dtype, int64_t n_s, int64_t n_blk,
+                      const int32_t* blk_owner, const int64_t* blk_ptr, const int64_t* sub_ptr,
+                      const int32_t* sub_idx, const void* x, float* sig, void* xt,
+                      const dds_pcg_state* gate, void* stream);
+int dds_sym_mfma_apply_packed(int mat_dtype, int vec_dtype, int64_t n_s, int64_t n_items,
+                              int64_t max_k, int64_t pairs_per_item, const int32_t* items,
+                              const int64_t* sub_ptr, const int64_t* blk_ptr,
+                              const int64_t* mat_off, const int64_t* ypos, const float* scale,
+                              const void* mats, const float* sig, const void* xt, void* Yp,
+                              const dds_pcg_state* gate, void* stream);
 
 /* y[sub_ptr[s] + a] = sum of the partial vectors that cover local row a of subdomain s, in
  * item order: items q = 0 .. min(Q_s - 1, (T_s-1-a/64) / G), Q_s = ceil(P_s / G), with

@@ -205,6 +205,51 @@ def test_local_apply_matches_oracle(storage, overlap, layout, waves, N, box):
     assert rel < tol, rel
 
 
+@pytest.mark.parametrize("storage", [torch.float16, torch.bfloat16])
+@pytest.mark.parametrize("group", [1, 2, 3, "all"])
+@pytest.mark.parametrize("N,box,overlap", [(18, 8, 1), (14, 4, 0), (20, 8, 0)])
+def test_sym_mfma_pair_groups(monkeypatch, storage, group, N, box, overlap):
+    """Items of G tile-row pairs (partial vectors summed by the combine), and one item per
+    subdomain writing the result directly: the exact product of the stored inverses; the
+    pack and the apply launched apart give the same bits as the entry point that runs both."""
+    pb = P.make_problem("q1_3d", N, box=box, overlap=overlap, seed=3, lognormal=True)
+    d = pb.decomposition
+    sizes = np.diff(d.sub_ptr)
+    P_max = -(-(-(-int(sizes.max()) // 64)) // 2)
+    G = P_max if group == "all" else group
+    print(f"sym_mfma groups: storage {storage} G {G} (P_max {P_max}) N {N} box {box} overlap {overlap}",
+          flush=True)
+    monkeypatch.setattr(dds.solvers, "_symm_group", lambda sizes, n_cu: G)
+    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(storage, scaled=storage == torch.float16,
+                                                   accumulate=torch.float32, layout="sym_mfma"),
+                            dds.DenseCholesky())
+    M.setup(_torch_csr(pb.A), decomposition=d)
+    L = M.local
+    assert L.layout == "sym_mfma" and L.group == G and L.direct == (G >= P_max)
+    r = np.random.default_rng(1).standard_normal(pb.A.shape[0])
+    rt = torch.as_tensor(r, device=dev)
+    y = M._local_only(rt).cpu().numpy()
+    ye = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx,
+                       inverses=M.stored_local_inverses()).local(r)
+    rel = np.linalg.norm(y - ye) / np.linalg.norm(ye)
+    assert rel < 2e-6, rel
+    # the two halves: the pack, then the apply from it (and again from the same pack)
+    Yst = M.Y if M.Y is not None else torch.empty_like(rt)
+    Y1 = torch.full_like(Yst, float("nan"))
+    M._enqueue_sym_apply(rt, None, Y1)
+    if not L.direct:
+        M._enqueue_sym_combine(Y1, None)
+    M._enqueue_sym_pack(rt, None)
+    for _ in range(2):
+        Y2 = torch.full_like(Yst, float("nan"))
+        M._enqueue_sym_apply_packed(None, Y2)
+        if not L.direct:
+            M._enqueue_sym_combine(Y2, None)
+        torch.cuda.synchronize()
+        assert not torch.isnan(Y2).any()
+        assert torch.equal(Y1, Y2)
+
+
 @pytest.mark.parametrize("kind,N", [("q1_3d", 24), ("p1_2d", 64), ("q1_3d", 5)])
 @pytest.mark.parametrize("fmt", ["sell", "csr"])
 def test_spmv_formats_with_dot(kind, N, fmt):
