@@ -22,6 +22,8 @@
 // subdomain in item order, which keeps the result deterministic without atomics.
 #include "common.hpp"
 
+#include <mutex>
+
 namespace dds {
 namespace {
 
@@ -1003,3 +1005,63 @@ extern "C" int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, int64_t pa
   DDS_LAUNCHED();
   return DDS_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// The matrix-core symmetric apply without device allocation: its workspace (the block
+// scales and 16-bit terms of the x pack) lives in `scratch`, a file-scope device array of
+// kSymmScratchBytes that every GPU the library runs on holds until the process ends. Its
+// device address is looked up once per device (hipGetSymbolAddress, that device current) on
+// the first call there; kernels get that address, never the array's host-side name.
+// ---------------------------------------------------------------------------------------
+static constexpr size_t kSymmScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kSymmScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+int scratch_for_current_device(unsigned char** out) {
+  int dev = 0;
+  DDS_HIP(hipGetDevice(&dev));
+  DDS_REQUIRE(dev >= 0 && dev < kScratchDevices, "device %d beyond the %d supported", dev,
+              kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    DDS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DDS_REQUIRE(p != nullptr, "scratch: no device address on device %d", dev);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return DDS_OK;
+}
+}  // namespace
+
+extern "C" int dds_local_apply_sym_mfma_noalloc(
+    int mat_dtype, int vec_dtype, int64_t n_s, int64_t n_items, int64_t max_k,
+    int64_t pairs_per_item, const int32_t* items, const int64_t* sub_ptr, const int32_t* sub_idx,
+    const int64_t* mat_off, const int64_t* ypos, const float* scale, const void* mats,
+    const void* x, int64_t n_blk, const int32_t* blk_owner, const int64_t* blk_ptr, void* Yp,
+    const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(mat_dtype == DDS_F16 || mat_dtype == DDS_BF16,
+              "dds_local_apply_sym_mfma_noalloc: unsupported matrix dtype %d", mat_dtype);
+  DDS_REQUIRE(n_blk >= 0, "dds_local_apply_sym_mfma_noalloc: bad block count");
+  const int nt = mat_dtype == DDS_F16 ? 2 : 3;
+  const size_t sig_bytes = (size_t)ceil_div(n_blk * 4, 256) * 256;
+  const size_t need = sig_bytes + (size_t)n_blk * nt * 128;
+  DDS_REQUIRE(need <= kSymmScratchBytes,
+              "dds_local_apply_sym_mfma_noalloc: %lld blocks of 64 rows need %zu bytes of "
+              "workspace, beyond the %zu of the static scratch (use dds_local_apply_sym_mfma "
+              "with a caller workspace)",
+              (long long)n_blk, need, kSymmScratchBytes);
+  if (n_items == 0 || n_s == 0) return DDS_OK;
+  unsigned char* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc != DDS_OK) return rc;
+  return dds_local_apply_sym_mfma(mat_dtype, vec_dtype, n_s, n_items, max_k, pairs_per_item,
+                                  items, sub_ptr, sub_idx, mat_off, ypos, scale, mats, x, n_blk,
+                                  blk_owner, blk_ptr, reinterpret_cast<float*>(ws),
+                                  ws + sig_bytes, Yp, gate, stream);
+}
+

@@ -21,6 +21,9 @@
  *   dds_local_apply_sym_mfma  the same for FP16/BF16 inverses on the matrix cores
  *   dds_sym_mfma_pack, dds_sym_mfma_apply_packed
  *                             its two kernels, launched apart
+ *   dds_local_apply_sym_mfma_noalloc
+ *                             the same with its workspace in a static device array (no
+ *                             counterpart in the reference)
  *   dds_local_apply_batched   the same on the reference's (n, k, k) layout (equal subdomains)
  *   dds_overlap_sum           y += repeat_interleave(...) prolongation (solvers.py:821-825,
  *                             :882-887) fused with the overlapped sum_i R_i^T y_i
@@ -221,6 +224,25 @@ int dds_sym_mfma_apply_packed(int mat_dtype, int vec_dtype, int64_t n_s, int64_t
                               const int64_t* mat_off, const int64_t* ypos, const float* scale,
                               const void* mats, const float* sig, const void* xt, void* Yp,
                               const dds_pcg_state* gate, void* stream);
+
+/* dds_local_apply_sym_mfma without a caller workspace: sig and xt live in a static device
+ * array of the library (32 MiB per GPU, held until the process ends; its address looked up
+ * once per device on the first call there). No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It allocates nothing. Limit: the workspace,
+ * ceil(4 n_blk / 256) * 256 + n_blk * NT * 128 bytes (NT = 2 for FP16, 3 for BF16), must fit
+ * in the 32 MiB (about 86,000 blocks of 64 rows in BF16, 129,000 in FP16); beyond it the call
+ * fails with an error and launches nothing. Nothing in the array needs to start at zero (the
+ * pack writes every word the apply reads). Every call on one GPU shares its array, so calls
+ * that overlap on one GPU (different streams, or other threads) race. */
+int dds_local_apply_sym_mfma_noalloc(int mat_dtype, int vec_dtype, int64_t n_s, int64_t n_items,
+                                     int64_t max_k, int64_t pairs_per_item, const int32_t* items,
+                                     const int64_t* sub_ptr, const int32_t* sub_idx,
+                                     const int64_t* mat_off, const int64_t* ypos,
+                                     const float* scale, const void* mats, const void* x,
+                                     int64_t n_blk, const int32_t* blk_owner,
+                                     const int64_t* blk_ptr, void* Yp,
+                                     const dds_pcg_state* gate, void* stream);
 
 /* y[sub_ptr[s] + a] = sum of the partial vectors that cover local row a of subdomain s, in
  * item order: items q = 0 .. min(Q_s - 1, (T_s-1-a/64) / G), Q_s = ceil(P_s / G), with
