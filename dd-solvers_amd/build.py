@@ -1,8 +1,12 @@
-"""Build the HIP kernel library `dd_solvers/libdd_solvers_hip.so` for gfx950 with hipcc.
+"""Build the HIP kernel library `dd_solvers/libdd_solvers_hip.so` for gfx950 with hipcc,
+then the op module `dd_solvers/_C` on top of it.
 
 Each csrc/*.hip translation unit is compiled to an object in parallel and the objects are
 linked into one shared library exporting the C ABI of include/dd_solvers_hip.h. No
-PyTorch headers are involved: the library sees only plain pointers and streams.
+PyTorch headers are involved there: the library sees only plain pointers and streams.
+csrc/torch_op.cpp (host code, PyTorch headers) registers the reference's native op
+`torch.ops.dd_solvers_gemv.gemv_strided_batched` over that C ABI and is linked as the
+Python extension module `dd_solvers._C`.
 
     python dd-solvers_amd/build.py [--jobs N] [--arch gfx950]
 """
@@ -79,7 +83,34 @@ def build(arch: str = "gfx950", jobs: int | None = None, verbose: bool = False) 
     tmp = LIB + ".tmp"
     _run([cc, f"--offload-arch={arch}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_op_module()
     return LIB
+
+
+def build_op_module() -> str:
+    """dd_solvers/_C<EXT_SUFFIX>: csrc/torch_op.cpp against PyTorch and the kernel library,
+    both found at run time through the rpath (the package directory, torch's lib)."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension
+
+    out = os.path.join(HERE, "dd_solvers", "_C" + sysconfig.get_config_var("EXT_SUFFIX"))
+    torch_lib = cpp_extension.library_paths()[0]
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    inc = [f"-I{p}" for p in cpp_extension.include_paths()]
+    rocm_inc = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "include")
+    cmd = [hipcc(), "-x", "c++", "-O2", "-std=c++17", "-fPIC", "-shared",
+           f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-DTORCH_EXTENSION_NAME=_C",
+           "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", *inc, f"-I{rocm_inc}",
+           f"-I{sysconfig.get_paths()['include']}", "-I", INCLUDE,
+           os.path.join(CSRC, "torch_op.cpp"), "-o", out + ".tmp",
+           f"-L{torch_lib}", "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip",
+           "-ltorch_python", f"-L{os.path.dirname(LIB)}", "-ldd_solvers_hip",
+           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{torch_lib}"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

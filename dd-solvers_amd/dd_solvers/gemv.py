@@ -21,6 +21,10 @@ def _check(matrices: torch.Tensor, vectors: torch.Tensor) -> None:
             "Input tensors must have compatible dimensions: (n, k, k) and (n, k)")
     if matrices.numel() > 0 and not (matrices.stride(1) == 1 and matrices.stride(2) == matrices.size(1)):
         raise RuntimeError("Matrices must be in column-major format")
+    k = matrices.size(1)
+    if matrices.size(0) > 1 and matrices.numel() > 0 and matrices.stride(0) != k * k:
+        # the kernels address matrix b at b*k*k
+        raise RuntimeError("gemv_strided_batched: the matrices must follow each other (batch stride k*k)")
 
 
 @torch.library.custom_op("dd_solvers::gemv_strided_batched", mutates_args=())

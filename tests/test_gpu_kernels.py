@@ -31,18 +31,37 @@ DTYPE_PAIRS = [
 @pytest.mark.parametrize("mat_dtype,vec_dtype", DTYPE_PAIRS)
 @pytest.mark.parametrize("k", [1, 2, 5, 10, 20, 50, 100, 200])
 @pytest.mark.parametrize("use_shared_memory", [True, False])
-def test_gemv_strided_batched(mat_dtype, vec_dtype, k, use_shared_memory):
-    # the 112 cases of src/dd_solvers/tests/test_gemv.py, checked against the oracle
+@pytest.mark.parametrize("via", ["custom_op", "native_op"])
+def test_gemv_strided_batched(mat_dtype, vec_dtype, k, use_shared_memory, via):
+    # the 112 cases of src/dd_solvers/tests/test_gemv.py, checked against the oracle, through
+    # the Python custom op (gemv.py) and through the native op that dd_solvers._C registers
+    # (torch.ops.dd_solvers_gemv, the reference's gemv.cpp:55-57)
     n = 7
     g = torch.Generator().manual_seed(42)
     A = torch.randn((n, k, k), generator=g, dtype=torch.float32).to(mat_dtype)
     A = A.permute(0, 2, 1).contiguous().permute(0, 2, 1).to(dev)  # column-major
     b = torch.randn((n, k), generator=g, dtype=torch.float32).to(vec_dtype).to(dev)
-    out = gemv_strided_batched(A, b, use_shared_memory=use_shared_memory)
+    if via == "custom_op":
+        out = gemv_strided_batched(A, b, use_shared_memory=use_shared_memory)
+    else:
+        out = torch.ops.dd_solvers_gemv.gemv_strided_batched(A, b, use_shared_memory)
+        assert torch.equal(out, gemv_strided_batched(A, b, use_shared_memory=use_shared_memory))
     ref = K.gemv_strided_batched(A.to(torch.float64).cpu().numpy(), b.cpu().numpy())
     assert out.dtype == vec_dtype
     tol = 1e-12 if vec_dtype == torch.float64 else 1e-5
     assert np.allclose(out.cpu().numpy(), ref, rtol=tol, atol=tol)
+
+
+def test_native_op_rejects_bad_inputs():
+    op = torch.ops.dd_solvers_gemv.gemv_strided_batched
+    A = torch.randn(3, 4, 4, dtype=torch.float64, device=dev)  # row-major
+    with pytest.raises(RuntimeError, match="column-major"):
+        op(A, torch.randn(3, 4, dtype=torch.float64, device=dev), False)
+    Ac = A.transpose(1, 2).contiguous().transpose(1, 2)
+    with pytest.raises(RuntimeError, match="Double-Float"):
+        op(Ac, torch.randn(3, 4, dtype=torch.float32, device=dev), True)
+    with pytest.raises(RuntimeError, match="compatible"):
+        op(Ac, torch.randn(3, 5, dtype=torch.float64, device=dev), False)
 
 
 def test_gemv_rejects_bad_inputs():

@@ -31,6 +31,22 @@ def _declared_symbols():
     return names
 
 
+def test_native_op_registered_with_reference_schema():
+    """dd_solvers._C registers dd_solvers_gemv::gemv_strided_batched with the schema the
+    reference's TORCH_LIBRARY infers from its function (gemv.cpp:9-12, 55-57); its checks
+    run before anything touches a device."""
+    import dd_solvers._C  # noqa: F401
+    op = torch.ops.dd_solvers_gemv.gemv_strided_batched
+    schema = str(op.default._schema)
+    assert schema == "dd_solvers_gemv::gemv_strided_batched(Tensor _0, Tensor _1, bool _2) -> Tensor _0"
+    with pytest.raises(RuntimeError, match="compatible dimensions"):
+        op(torch.zeros(2, 3, 3), torch.zeros(2, 4), False)
+    with pytest.raises(RuntimeError, match="column-major"):
+        op(torch.zeros(2, 3, 3), torch.zeros(2, 3), False)
+    with pytest.raises(RuntimeError, match="same GPU"):
+        op(torch.zeros(2, 3, 3).transpose(1, 2), torch.zeros(2, 3), False)
+
+
 def test_library_exports_every_declared_symbol():
     declared = _declared_symbols()
     assert "dds_local_apply" in declared and "dds_chol_factor" in declared
