@@ -178,6 +178,96 @@ __global__ __launch_bounds__(256) void segment_sum_kernel(int64_t n_seg,
   out[i] = kCompensated ? acc + comp : acc;
 }
 
+// Batched inverse of general (nonsingular) matrices by Gauss-Jordan elimination with
+// partial pivoting, in place: one workgroup per matrix (row-major k x k at a, leading
+// dimension ld). Step q takes the row p >= q of largest |a_pq| (the smallest such p on
+// ties), swaps rows p and q, scales row q by 1/a_qq and eliminates column q from every
+// other row: a_ij <- (j == q ? 0 : a_ij) - a_iq a_qj, which leaves the inverse's columns in
+// the eliminated positions. The row swaps are undone at the end as column swaps in reverse
+// order. A zero pivot column sets info[b] = q + 1 (the matrix is singular) and stops.
+// LDS (dynamic): the pivot rows chosen (k ints), column q and row q of the step (k values
+// each), and the argmax reduction.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void pivoted_inverse_kernel(T* __restrict__ A,
+                                                                   int64_t stride, int64_t ld,
+                                                                   const int32_t* __restrict__ ks,
+                                                                   int32_t* __restrict__ info) {
+  extern __shared__ __align__(16) unsigned char piv_smem[];
+  const int k = ks[blockIdx.x];
+  const int kmax = (int)ld;
+  T* colq = reinterpret_cast<T*>(piv_smem);
+  T* rowq = colq + kmax;
+  T* rv = rowq + kmax;
+  int* ri = reinterpret_cast<int*>(rv + kThreads);
+  int* pr = ri + kThreads;
+  T* a = A + (int64_t)blockIdx.x * stride;
+  const int tid = threadIdx.x;
+  for (int q = 0; q < k; ++q) {
+    T best = T(-1);
+    int bi = q;
+    for (int i = q + tid; i < k; i += kThreads) {
+      const T v = fabs(a[(int64_t)i * ld + q]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    rv[tid] = best;
+    ri[tid] = bi;
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1) {
+      if (tid < h && (rv[tid + h] > rv[tid] || (rv[tid + h] == rv[tid] && ri[tid + h] < ri[tid]))) {
+        rv[tid] = rv[tid + h];
+        ri[tid] = ri[tid + h];
+      }
+      __syncthreads();
+    }
+    const int p = ri[0];
+    const bool zero = !(rv[0] > T(0));
+    __syncthreads();  // everyone has read the result before rv / ri are reused
+    if (zero) {
+      if (tid == 0 && info != nullptr) info[blockIdx.x] = q + 1;
+      return;  // uniform: every thread read the same rv[0]
+    }
+    if (tid == 0) pr[q] = p;
+    if (p != q) {
+      for (int j = tid; j < k; j += kThreads) {
+        const T t = a[(int64_t)p * ld + j];
+        a[(int64_t)p * ld + j] = a[(int64_t)q * ld + j];
+        a[(int64_t)q * ld + j] = t;
+      }
+    }
+    __syncthreads();
+    const T ipiv = T(1) / a[(int64_t)q * ld + q];
+    for (int i = tid; i < k; i += kThreads) colq[i] = a[(int64_t)i * ld + q];
+    __syncthreads();  // the pivot and column q are read before row q is overwritten
+    for (int j = tid; j < k; j += kThreads) {
+      const T v = j == q ? ipiv : a[(int64_t)q * ld + j] * ipiv;
+      rowq[j] = v;
+      a[(int64_t)q * ld + j] = v;
+    }
+    __syncthreads();
+    for (int i = 0; i < k; ++i) {
+      if (i == q) continue;
+      const T f = colq[i];
+      T* ar = a + (int64_t)i * ld;
+      for (int j = tid; j < k; j += kThreads) ar[j] = (j == q ? T(0) : ar[j]) - f * rowq[j];
+    }
+    __syncthreads();
+  }
+  for (int q = k - 1; q >= 0; --q) {
+    const int p = pr[q];
+    if (p != q) {
+      for (int i = tid; i < k; i += kThreads) {
+        const T t = a[(int64_t)i * ld + p];
+        a[(int64_t)i * ld + p] = a[(int64_t)i * ld + q];
+        a[(int64_t)i * ld + q] = t;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 }  // namespace dds
 
@@ -232,3 +322,35 @@ extern "C" int dds_spd_inverse_batched(int dtype, int64_t n_mats, const int32_t*
   DDS_LAUNCHED();
   return DDS_OK;
 }
+
+extern "C" int dds_inverse_batched(int dtype, int64_t n_mats, const int32_t* k, int64_t ld,
+                                   int64_t stride, void* A, int32_t* info, void* stream) {
+  DDS_REQUIRE(n_mats >= 0 && n_mats <= INT32_MAX, "dds_inverse_batched: bad batch size");
+  DDS_REQUIRE(ld >= 1 && ld <= 4096 && stride >= ld * ld,
+              "dds_inverse_batched: bad ld/stride (ld <= 4096, stride >= ld * ld)");
+  if (n_mats == 0) return DDS_OK;
+  DDS_REQUIRE(k && A, "dds_inverse_batched: null pointer");
+  hipStream_t s = as_stream(stream);
+  const size_t esz = dtype == DDS_F64 ? 8 : 4;
+  const size_t lds = (2 * (size_t)ld + kThreads) * esz + (kThreads + (size_t)ld) * 4;
+#define DDS_PIVINV(T)                                                                       \
+  do {                                                                                      \
+    if (lds > 65536)                                                                        \
+      DDS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&pivoted_inverse_kernel<T>), \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));   \
+    hipLaunchKernelGGL(pivoted_inverse_kernel<T>, dim3((unsigned)n_mats), dim3(kThreads), lds, \
+                       s, static_cast<T*>(A), stride, ld, k, info);                         \
+  } while (0)
+  if (dtype == DDS_F64) {
+    DDS_PIVINV(double);
+  } else if (dtype == DDS_F32) {
+    DDS_PIVINV(float);
+  } else {
+    set_error("dds_inverse_batched: dtype must be F64 or F32 (got %d)", dtype);
+    return DDS_EUNSUPPORTED;
+  }
+#undef DDS_PIVINV
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+

@@ -91,6 +91,7 @@ _SIGNATURES = {
     "dds_cast": [_i, _i, _i64, _p, _p, _p, _p],
     "dds_q1_element_grad": [_i64, _p, _p, _p, _p, _p, _p],
     "dds_spd_inverse_batched": [_i, _i64, _p, _i64, _i64, _p, _p, _p],
+    "dds_inverse_batched": [_i, _i64, _p, _i64, _i64, _p, _p, _p],
     "dds_segment_sum": [_i, _i, _i64, _p, _p, _p, _p],
     "dds_stream": [_i, _i64, _p, _p, _d, _p, _i64, _p],
     "dds_pcg_residual_m": [_i64, _p, _p, _p, _p, _ip, _p],

@@ -937,13 +937,42 @@ class Inv(DenseBatchSolver):
         return f"Inv({self.precision or ''})"
 
     def setup(self, matrix_batch: torch.Tensor):
-        inv = torch.linalg.inv(matrix_batch).to(self.precision or matrix_batch.dtype)
+        """Inverts the (n, k, k) batch on the GPU (torch.linalg.inv in the reference,
+        solvers.py:350-352): Gauss-Jordan with partial pivoting, dds_inverse_batched. A
+        singular matrix raises torch.linalg.LinAlgError, as torch.linalg.inv does."""
+        inv = inverse_batched(matrix_batch).to(self.precision or matrix_batch.dtype)
         # column-major storage, as gemv_strided_batched requires
         self.matrix_inv = inv.transpose(1, 2).contiguous().transpose(1, 2)
 
     def solve(self, rhs_batch: torch.Tensor) -> torch.Tensor:
         return gemv_strided_batched(self.matrix_inv,
                                     rhs_batch.reshape(self.matrix_inv.shape[0], -1))
+
+
+def inverse_batched(matrix_batch: torch.Tensor) -> torch.Tensor:
+    """Inverse of every matrix of a (n, k, k) float64/float32 GPU batch (k <= 4096), by
+    Gauss-Jordan with partial pivoting (dds_inverse_batched); a singular matrix raises
+    torch.linalg.LinAlgError."""
+    _lib.require_device(matrix_batch)
+    if matrix_batch.dim() != 3 or matrix_batch.shape[1] != matrix_batch.shape[2]:
+        raise ValueError("inverse_batched: expected a (n, k, k) batch")
+    if matrix_batch.dtype not in (torch.float64, torch.float32):
+        raise TypeError(f"inverse_batched: float64 or float32 matrices (got {matrix_batch.dtype})")
+    n, k, _ = matrix_batch.shape
+    a = matrix_batch.contiguous().clone()
+    if n == 0 or k == 0:
+        return a
+    ks = torch.full((n,), k, dtype=torch.int32, device=a.device)
+    info = torch.zeros(n, dtype=torch.int32, device=a.device)
+    with torch.cuda.device(a.device):
+        _lib.call("dds_inverse_batched", _lib.dtype_code(a.dtype), n, ks.data_ptr(), k, k * k,
+                  a.data_ptr(), info.data_ptr(), _lib.stream())
+    bad = torch.nonzero(info).flatten()
+    if bad.numel():
+        b = int(bad[0])
+        raise torch.linalg.LinAlgError(
+            f"inverse_batched: matrix {b} is singular (no pivot in column {int(info[b]) - 1})")
+    return a
 
 
 class LU(DenseBatchSolver):
@@ -1472,16 +1501,25 @@ class SchwarzOperator(SparseSolver):
             blk, kk = build_chunk(c0, c1)
             # local matrices of an SPD operator are SPD: batched Gauss-Jordan inverse in
             # place; a chunk holding a matrix with a non-positive pivot is rebuilt and
-            # inverted with pivoting (torch.linalg.inv, as the reference's Inv.setup)
+            # inverted with partial pivoting (dds_inverse_batched; the reference's Inv.setup
+            # uses torch.linalg.inv for every matrix)
             info = torch.zeros(c1 - c0, dtype=torch.int32, device=dev)
             _lib.call("dds_spd_inverse_batched", _lib.dtype_code(pi), c1 - c0,
                       kk.to(torch.int32).contiguous().data_ptr(), k_max, k_max * k_max,
                       blk.data_ptr(), info.data_ptr(), _lib.stream())
             if int(info.max()) == 0:
                 inv = blk
-            else:
+            else:  # a local matrix that is not SPD: again from the start, with pivoting
                 blk, kk = build_chunk(c0, c1)
-                inv = torch.linalg.inv(blk)
+                info.zero_()
+                _lib.call("dds_inverse_batched", _lib.dtype_code(pi), c1 - c0,
+                          kk.to(torch.int32).contiguous().data_ptr(), k_max, k_max * k_max,
+                          blk.data_ptr(), info.data_ptr(), _lib.stream())
+                bad = torch.nonzero(info).flatten()
+                if bad.numel():
+                    raise torch.linalg.LinAlgError(
+                        f"local matrix of subdomain {c0 + int(bad[0])} is singular")
+                inv = blk
             # pack every subdomain of one size at once
             ks = sizes[c0:c1]
             for k in np.unique(ks):

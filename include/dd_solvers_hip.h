@@ -28,6 +28,9 @@
  *   dds_overlap_sum           y += repeat_interleave(...) prolongation (solvers.py:821-825,
  *                             :882-887) fused with the overlapped sum_i R_i^T y_i
  *   dds_restrict              x.reshape(n_s,-1).sum(1) + segment_reduce (solvers.py:817-820)
+ *   dds_spd_inverse_batched, dds_inverse_batched
+ *                             torch.linalg.inv in Inv.setup (solvers.py:350-352): SPD
+ *                             (no pivoting) and general (partial pivoting) batches
  *   dds_chol_factor           cudssExecute(ANALYSIS|FACTORIZATION) of the coarse matrix
  *                             (src/dd_solvers/cpp/cudss.cpp:80-105), as a dense FP64 Cholesky
  *   dds_chol_solve            cudssExecute(SOLVE) (cudss.cpp:64-78)
@@ -360,6 +363,14 @@ int dds_sum_partials(const double* partials, int count, double* out, void* strea
  * for the local matrices of the Schwarz preconditioners, which are SPD. */
 int dds_spd_inverse_batched(int dtype, int64_t n_mats, const int32_t* k, int64_t ld,
                             int64_t stride, void* A, int32_t* info, void* stream);
+/* In-place inverse of n_mats general matrices, laid out as for dds_spd_inverse_batched
+ * (ld <= 4096, stride >= ld * ld): Gauss-Jordan elimination with partial pivoting by rows,
+ * one workgroup per matrix. info[b] (optional) is set to q + 1 when column q has no nonzero
+ * pivot (the matrix is singular; the result is then unusable). Replaces
+ * torch.linalg.inv(matrix_batch) in Inv.setup (solvers.py:350-352) for any nonsingular
+ * batch, and serves the Schwarz setup's local matrices that are not SPD. */
+int dds_inverse_batched(int dtype, int64_t n_mats, const int32_t* k, int64_t ld, int64_t stride,
+                        void* A, int32_t* info, void* stream);
 /* Segmented sum (dtype DDS_F64 or DDS_F32): out[i] = sum of vals[seg_ptr[i] ..
  * seg_ptr[i+1]) added in index order in dtype, one thread per segment; compensated != 0
  * carries each addition's rounding error (Neumaier), for long runs that cancel. The

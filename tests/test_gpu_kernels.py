@@ -374,6 +374,91 @@ def test_spd_inverse_batched_matches_torch(dtype, tol):
         assert torch.equal(At[b, k:, :], orig[b, k:, :]) and torch.equal(At[b, :k, k:], orig[b, :k, k:])
 
 
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-3)])
+def test_inverse_batched_pivoting(dtype, tol):
+    """dds_inverse_batched (partial pivoting) against numpy on nonsymmetric, zero-diagonal
+    and indefinite matrices of several sizes; the padding outside each k x k stays."""
+    rng = np.random.default_rng(11)
+    ks = [1, 2, 7, 33, 64, 65, 130, 300]
+    kmax = max(ks)
+    mats = []
+    for k in ks:
+        B = rng.standard_normal((k, k))
+        Z = B.copy()
+        np.fill_diagonal(Z, 0.0)
+        mats += [B + 3 * np.eye(k), Z if k > 1 else B, B + B.T]
+    A = np.zeros((len(mats), kmax, kmax))
+    kk = []
+    for b, M in enumerate(mats):
+        k = M.shape[0]
+        kk.append(k)
+        A[b, :k, :k] = M
+        A[b, k:, k:] = np.eye(kmax - k) * 7.0
+    At = torch.as_tensor(A, dtype=dtype, device=dev).contiguous()
+    orig = At.clone()
+    info = torch.zeros(len(mats), dtype=torch.int32, device=dev)
+    kt = torch.as_tensor(kk, dtype=torch.int32, device=dev)
+    _lib.call("dds_inverse_batched", _lib.dtype_code(dtype), len(mats), kt.data_ptr(), kmax,
+              kmax * kmax, At.data_ptr(), info.data_ptr(), _lib.stream())
+    assert info.cpu().tolist() == [0] * len(mats)
+    for b, M in enumerate(mats):
+        k = M.shape[0]
+        ref = np.linalg.inv(M)
+        got = At[b, :k, :k].double().cpu().numpy()
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert err <= tol * max(1.0, np.linalg.cond(M) / 100), (b, k, err)
+        assert torch.equal(At[b, k:, :], orig[b, k:, :]) and torch.equal(At[b, :k, k:], orig[b, :k, k:])
+
+
+def test_inverse_batched_singular_and_inv_setup():
+    """A singular matrix raises torch.linalg.LinAlgError (as torch.linalg.inv does); Inv.setup
+    inverts any nonsingular batch and Inv.solve applies it."""
+    from dd_solvers.solvers import inverse_batched
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((3, 6, 6))
+    A[1, :, 2] = 0.0
+    with pytest.raises(torch.linalg.LinAlgError, match="matrix 1 is singular"):
+        inverse_batched(torch.as_tensor(A, device=dev))
+    A[1] = rng.standard_normal((6, 6))
+    np.fill_diagonal(A[2], 0.0)
+    inv = dds.Inv(torch.float64)
+    inv.setup(torch.as_tensor(A, device=dev))
+    rhs = rng.standard_normal((3, 6))
+    out = inv.solve(torch.as_tensor(rhs, device=dev)).cpu().numpy()
+    assert np.allclose(out, np.linalg.solve(A, rhs[..., None])[..., 0], rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("layout", ["full", "sym"])
+def test_schwarz_setup_inverts_indefinite_local_matrices(layout):
+    """Local matrices that are not SPD (the SPD Gauss-Jordan flags them) are inverted again
+    with pivoting, end to end through the preconditioner setup: A - c v v^T with
+    v = e_i - e_j for two DoFs of one coarse cell leaves the coarse matrix as it is and makes
+    every local matrix holding i or j indefinite."""
+    import scipy.sparse as sp
+    pb = P.make_problem("q1_3d", 12, box=4, overlap=1, seed=4, lognormal=True)
+    d = pb.decomposition
+    A = pb.A.tocsr()
+    i, j = int(d.owner_ptr[0]), int(d.owner_ptr[0]) + 1
+    c = 10.0 * A.diagonal().max()
+    vvT = sp.coo_matrix(([1.0, -1.0, -1.0, 1.0], ([i, i, j, j], [i, j, i, j])), shape=A.shape)
+    Ap = (A - c * vvT).tocsr()
+    Ap.sort_indices()
+    M = dds.AdditiveSchwarz(torch.float64, dds.Inv(torch.float64, layout=layout), dds.DenseCholesky())
+    M.setup(_torch_csr(Ap), decomposition=d)
+    assert M.local.layout == layout
+    got = M.stored_local_inverses()
+    sub_ptr, sub_idx = d.sub_ptr, d.sub_idx
+    n_indef = 0
+    for s_ in range(len(sub_ptr) - 1):
+        idx = sub_idx[sub_ptr[s_]:sub_ptr[s_ + 1]] if sub_idx is not None else np.arange(sub_ptr[s_], sub_ptr[s_ + 1])
+        Al = Ap[idx][:, idx].toarray()
+        n_indef += int(np.linalg.eigvalsh(Al).min() < 0)
+        ref = np.linalg.inv(Al)
+        err = np.linalg.norm(got[s_] - ref) / np.linalg.norm(ref)
+        assert err < 1e-13 * np.linalg.cond(Al), (s_, err)
+    assert n_indef >= 1
+
+
 def test_spd_inverse_batched_flags_indefinite():
     from dd_solvers import _lib
     A = torch.tensor([[[2.0, 0.0], [0.0, -1.0]], [[4.0, 1.0], [1.0, 3.0]]], dtype=torch.float64, device=dev)

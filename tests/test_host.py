@@ -639,3 +639,58 @@ def test_symm_group_choice():
     work = [sum((T + 1) if p != T - 1 - p else (T - p) for p in range(q * g, min(32, q * g + g)))
             for _, q in items]
     assert work == sorted(work, reverse=True)
+
+
+def _pivoted_gj_model(A):
+    """The steps of pivoted_inverse_kernel (csrc/setup.hip) on one matrix."""
+    a = np.array(A, dtype=np.float64)
+    k = a.shape[0]
+    pr = np.zeros(k, dtype=np.int64)
+    for q in range(k):
+        col = np.abs(a[q:, q])
+        if not (col.max() > 0):
+            return None, q + 1
+        p = q + int(np.argmax(col))  # the first of equal maxima
+        pr[q] = p
+        a[[p, q]] = a[[q, p]]
+        ipiv = 1.0 / a[q, q]
+        colq = a[:, q].copy()
+        a[q] = a[q] * ipiv
+        a[q, q] = ipiv
+        for i in range(k):
+            if i != q:
+                rowi = a[i].copy()
+                rowi[q] = 0.0
+                a[i] = rowi - colq[i] * a[q]
+    for q in range(k - 1, -1, -1):
+        p = pr[q]
+        a[:, [p, q]] = a[:, [q, p]]
+    return a, 0
+
+
+@pytest.mark.parametrize("kind", ["random", "zero_diagonal", "permutation", "indefinite", "singular"])
+def test_pivoted_inverse_model(kind):
+    """Gauss-Jordan with partial pivoting as the kernel runs it gives the inverse (numpy),
+    including matrices that cannot be inverted without row exchanges, and flags a singular
+    one at the first column without a pivot."""
+    rng = np.random.default_rng(7)
+    k = 9
+    if kind == "random":
+        A = rng.standard_normal((k, k))
+    elif kind == "zero_diagonal":
+        A = rng.standard_normal((k, k))
+        np.fill_diagonal(A, 0.0)
+    elif kind == "permutation":
+        A = np.eye(k)[rng.permutation(k)] * rng.uniform(1, 2, k)
+    elif kind == "indefinite":
+        B = rng.standard_normal((k, k))
+        A = B + B.T
+    else:
+        A = rng.standard_normal((k, k))
+        A[:, 4] = 0.0
+    inv, info = _pivoted_gj_model(A)
+    if kind == "singular":
+        assert info == 5 and inv is None
+    else:
+        assert info == 0
+        assert np.allclose(inv, np.linalg.inv(A), rtol=1e-10, atol=1e-12)
