@@ -25,7 +25,7 @@ class ConvergenceNotAchieved(Exception):
 
 
 def cg(matvec, b, precond=None, T0=None, rtol=1e-7, atol=0.0, maxiter=1000,
-       estimate_cond=False, x0=None, record_iterates=False):
+       estimate_cond=False, x0=None, record_iterates=False, dot=None):
     """Preconditioned CG exactly as solvers.py:228-297.
 
     * atol_eff = max(atol, rtol * ||b||) (:229); maxiter None/0 -> 10 n (:231);
@@ -36,8 +36,12 @@ def cg(matvec, b, precond=None, T0=None, rtol=1e-7, atol=0.0, maxiter=1000,
     * alpha = rho / (p . q); x += alpha p; r -= alpha q (:264-268);
     * iterations = len(norms) - 1 (:276); raise iff i + 1 == maxiter (:294-295).
     Returns (x, metadata); metadata gains "iterates" when record_iterates is set.
+    dot (default numpy's): the inner product for rho, p . q and the residual norm, so that
+    the same recurrence can be run with its reductions summed in another order.
     """
     b = np.asarray(b, dtype=np.float64)
+    norm = np.linalg.norm if dot is None else (lambda v: float(np.sqrt(dot(v, v))))
+    dot = dot or np.dot
     atol_eff = max(atol, rtol * float(np.linalg.norm(b)))
     n = b.shape[0]
     maxiter = maxiter or n * 10
@@ -52,7 +56,7 @@ def cg(matvec, b, precond=None, T0=None, rtol=1e-7, atol=0.0, maxiter=1000,
     alphas, betas = [], []
     i = -1
     for i in range(maxiter):
-        res = float(np.linalg.norm(r))
+        res = float(norm(r))
         norms.append(res)
         if res < atol_eff:
             break

@@ -50,15 +50,20 @@ def test_dg_solve_matches_oracle(p, fine, solvers, coarse, kind):
     xo, mo = oracle_cg(lambda v: A @ v, b, precond=O.apply,
                        T0=O.T0 if kind == "hybrid" else None, rtol=1e-9, maxiter=2000)
     x, md = _gpu_solve(A, b, k, spc, pr.block_size, kind)
-    # The first iterates agree to rounding. Past ~100 iterations the CG recurrences of any
-    # two FP64 implementations drift apart (loss of A-orthogonality: the p = 3 Additive
-    # case takes ~260 iterations and its residual histories differ by a few per cent from
-    # iteration ~100 on, with local inverses equal to 3e-15), so the iteration count may
-    # differ by 1% and the solutions agree to the solver tolerance, not to rounding.
+    # The first iterates agree to rounding. Past ~60-100 iterations the CG recurrences of
+    # any two FP64 implementations drift apart: the oracle run twice on the CPU with only
+    # its summation orders changed (tests/test_oracle.py::test_dg_cg_reordered_reduction_drift)
+    # takes 259 vs 256 iterations on the p = 3 Additive case, its residual histories
+    # separate from iteration 60 on, and its solutions differ by 7.5e-11. The bars below
+    # sit on that envelope: iteration counts within max(1, 1%), solutions within 1e-9
+    # (measured here: at most 1.4e-11).
     n = min(10, len(md["residual norms"]), len(mo["residual norms"]))
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"  iterations {md['iterations']} (oracle {mo['iterations']}), x rel diff {err:.1e}",
+          flush=True)
     assert np.allclose(md["residual norms"][:n], mo["residual norms"][:n], rtol=1e-10, atol=0)
     assert abs(md["iterations"] - mo["iterations"]) <= max(1, mo["iterations"] // 100)
-    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert err <= 1e-9
     # the reference's acceptance check, in FP64 on the host
     assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b) * 1.01
 
@@ -107,7 +112,10 @@ def test_dg_full_size_s10(s10_system, kind):
     assert np.allclose(got[:10], pub[:10], rtol=1e-10, atol=0)
     # (after ~100 iterations the two recurrences drift apart, as in test_dg_solve_matches_oracle)
     xs = x[:: max(1, x.shape[0] // 64)]
-    assert np.abs(xs - np.array(ref["x_sample"])).max() <= 1e-6 * np.abs(xs).max()
+    err = np.abs(xs - np.array(ref["x_sample"])).max() / np.abs(xs).max()
+    print(f"  iterations {md['iterations']} (oracle {ref['iterations']}), x sample rel diff {err:.1e}",
+          flush=True)
+    assert err <= 1e-9
     assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b) * 1.01
 
 

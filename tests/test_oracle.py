@@ -374,3 +374,52 @@ def test_adjoint_gradient_matches_central_differences():
     fd = fd_directional(N, theta, f, u_obs, dirs, eps=1e-6)
     ad = dirs @ g
     assert np.all(np.abs(fd - ad) <= 1e-6 * np.abs(ad).max())
+
+
+def _blocked_dot(u, v):
+    """u . v summed in another order: 256 strided partial sums, added in reverse."""
+    w = np.asarray(u) * np.asarray(v)
+    m = -(-w.shape[0] // 256) * 256
+    return float(np.pad(w, (0, m - w.shape[0])).reshape(-1, 256).sum(axis=0)[::-1].sum())
+
+
+def _reversed_rows(A):
+    """The same CSR matrix with each row's entries in reverse order (the SpMV sums them in
+    reverse)."""
+    A = A.tocsr()
+    data, idx = A.data.copy(), A.indices.copy()
+    for i in range(A.shape[0]):
+        a, b = A.indptr[i], A.indptr[i + 1]
+        data[a:b], idx[a:b] = data[a:b][::-1], idx[a:b][::-1]
+    B = sps.csr_matrix((data, idx, A.indptr.copy()), shape=A.shape)
+    B.has_sorted_indices = False
+    return B
+
+
+@pytest.mark.parametrize("kind", ["additive", "hybrid"])
+def test_dg_cg_reordered_reduction_drift(kind):
+    """How far two FP64 runs of the same PCG drift apart when only their summation orders
+    differ (dots, norms, SpMV rows), on the DG p = 3 S5 / S3 / S2 system of the GPU DG
+    tests: the measured envelope that their GPU-vs-oracle tolerances rest on. The first
+    iterates agree to rounding; the residual histories separate after ~60-100 iterations;
+    the iteration counts may differ by ~1% (259 vs 256 for Additive); the solutions agree
+    far below the solver tolerance."""
+    from dd_solvers import dg
+    from oracle.schwarz import BlockSchwarzOracle
+    pr = dg.assemble_sipg(dg.UniformTriangleMesh(5), 3)
+    perm, A, b, k, spc = dg.dg_experiment(pr, "S3", "S2")
+    O = BlockSchwarzOracle(A, k, spc, kind=kind)
+    T0 = O.T0 if kind == "hybrid" else None
+    x1, m1 = cg(lambda v: A @ v, b, precond=O.apply, T0=T0, rtol=1e-9, maxiter=2000)
+    Ar = _reversed_rows(A)
+    x2, m2 = cg(lambda v: Ar @ v, b, precond=O.apply, T0=T0, rtol=1e-9, maxiter=2000,
+                dot=_blocked_dot)
+    r1, r2 = np.array(m1["residual norms"]), np.array(m2["residual norms"])
+    n = min(len(r1), len(r2))
+    rel = np.abs(r1[:n] - r2[:n]) / r1[:n]
+    assert rel[:10].max() < 1e-13
+    assert rel[:50].max() < 1e-10
+    assert abs(m1["iterations"] - m2["iterations"]) <= 0.02 * m1["iterations"]
+    assert np.linalg.norm(x1 - x2) <= 1e-9 * np.linalg.norm(x1)
+    if kind == "additive":  # the drift is real: the histories separate and the counts differ
+        assert rel.max() > 1e-3 and m1["iterations"] != m2["iterations"]
