@@ -342,7 +342,6 @@ typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef short i16x4_t __attribute__((__vector_size__(4 * sizeof(short))));
-typedef int i32x4_t __attribute__((ext_vector_type(4)));
 typedef short i16x8_t __attribute__((ext_vector_type(8)));
 
 template <typename TM>
@@ -475,84 +474,6 @@ __global__ __launch_bounds__(256) void sym_pack_x_kernel(int64_t n_blk,
     sym_pack_blocks<TM, TV, true>(b0, n_blk, lane, blk_owner, blk_ptr, sub_ptr, sub_idx, x, sig, xt);
   else
     sym_pack_blocks<TM, TV, false>(b0, n_blk, lane, blk_owner, blk_ptr, sub_ptr, sub_idx, x, sig, xt);
-}
-
-// Buffer resource (gfx9 family): 64-bit base, byte range; loads past the range return zero.
-__device__ __forceinline__ i32x4_t buffer_rsrc(const void* base, uint32_t bytes) {
-  const uint64_t b = reinterpret_cast<uint64_t>(base);
-  i32x4_t r;
-  r[0] = static_cast<int>(static_cast<uint32_t>(b));
-  r[1] = static_cast<int>(static_cast<uint32_t>(b >> 32));
-  r[2] = static_cast<int>(bytes);
-  r[3] = 0x00020000;
-  return r;
-}
-
-__device__ __forceinline__ uint32_t lds_addr(const void* p) {
-  return static_cast<uint32_t>(
-      reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) const void*)(p)));
-}
-
-__device__ __forceinline__ i32x4_t sgpr4(i32x4_t v) {  // wave-uniform: SGPR operands
-  i32x4_t r;
-  r[0] = __builtin_amdgcn_readfirstlane(v[0]);
-  r[1] = __builtin_amdgcn_readfirstlane(v[1]);
-  r[2] = __builtin_amdgcn_readfirstlane(v[2]);
-  r[3] = __builtin_amdgcn_readfirstlane(v[3]);
-  return r;
-}
-
-// Loads straight into LDS (buffer_load ... lds): lane l's bytes land at m0 + offset + l *
-// width, read from base + soffset + voffset + offset (the instruction offset applies to
-// both addresses). The compiler does not track these loads, so their completion is awaited
-// explicitly with s_waitcnt vmcnt (vector loads complete in issue order). The LDS reads
-// issued before are awaited first (lgkmcnt(0)), so a stage is never overwritten while it is
-// being read. The 16-byte forms stream matrix tiles, read once per apply: non-temporal.
-// Four consecutive 1 KiB blocks: LDS m0 + 1024 q, memory soffset + voff + 1024 q, q < 4.
-__device__ __forceinline__ void dma16x4(uint32_t m0, uint32_t voff, i32x4_t rsrc, uint32_t soff) {
-  m0 = __builtin_amdgcn_readfirstlane(m0);
-  soff = __builtin_amdgcn_readfirstlane(soff);
-  rsrc = sgpr4(rsrc);
-  asm volatile(
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_mov_b32 m0, %0\n\t"
-      "s_nop 0\n\t"
-      "buffer_load_dwordx4 %1, %2, %3 offen nt lds\n\t"
-      "buffer_load_dwordx4 %1, %2, %3 offen offset:1024 nt lds\n\t"
-      "buffer_load_dwordx4 %1, %2, %3 offen offset:2048 nt lds\n\t"
-      "buffer_load_dwordx4 %1, %2, %3 offen offset:3072 nt lds" ::"s"(m0),
-      "v"(voff), "s"(rsrc), "s"(soff)
-      : "memory");
-}
-// One 1 KiB block: LDS m0, memory soffset + voff.
-__device__ __forceinline__ void dma16(uint32_t m0, uint32_t voff, i32x4_t rsrc, uint32_t soff) {
-  m0 = __builtin_amdgcn_readfirstlane(m0);
-  soff = __builtin_amdgcn_readfirstlane(soff);
-  rsrc = sgpr4(rsrc);
-  asm volatile(
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_mov_b32 m0, %0\n\t"
-      "s_nop 0\n\t"
-      "buffer_load_dwordx4 %1, %2, %3 offen nt lds" ::"s"(m0),
-      "v"(voff), "s"(rsrc), "s"(soff)
-      : "memory");
-}
-// 4 bytes per lane: LDS m0 + 4 l, memory soffset + voff.
-__device__ __forceinline__ void dma4(uint32_t m0, uint32_t voff, i32x4_t rsrc, uint32_t soff) {
-  m0 = __builtin_amdgcn_readfirstlane(m0);
-  soff = __builtin_amdgcn_readfirstlane(soff);
-  rsrc = sgpr4(rsrc);
-  asm volatile(
-      "s_waitcnt lgkmcnt(0)\n\t"
-      "s_mov_b32 m0, %0\n\t"
-      "s_nop 0\n\t"
-      "buffer_load_dword %1, %2, %3 offen lds" ::"s"(m0),
-      "v"(voff), "s"(rsrc), "s"(soff)
-      : "memory");
-}
-template <int N>
-__device__ __forceinline__ void wait_vm() {
-  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
 constexpr uint32_t kOob = 0x80000000u;  // a buffer offset past every range: loads zeros

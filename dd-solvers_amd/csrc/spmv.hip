@@ -128,6 +128,52 @@ __global__ __launch_bounds__(kBlock) void bsr_row_kernel(int64_t n_block_rows, i
 }
 
 // ---------------------------------------------------------------------------------------
+// BSR with a block size B known at compile time (the DG sizes (p+1)(p+2)/2): one lane per
+// scalar row as above (blocks in stored order, c ascending), with the column loop unrolled
+// so that a block's B values and B entries of x are all in flight at once, and the next
+// block's column index loaded ahead.
+// ---------------------------------------------------------------------------------------
+template <typename T, typename I, int B>
+__global__ __launch_bounds__(kBlock) void bsr_rowb_kernel(int64_t n_block_rows,
+                                                          const I* __restrict__ crow,
+                                                          const int32_t* __restrict__ col,
+                                                          const T* __restrict__ val,
+                                                          const T* __restrict__ x,
+                                                          T* __restrict__ y,
+                                                          double* __restrict__ partials,
+                                                          const dds_pcg_state* __restrict__ gate) {
+  __shared__ double scratch[kBlock / 64];
+  if (gate != nullptr && gate->done) return;
+  constexpr int BB = B * B;
+  const int64_t n_rows = n_block_rows * B;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double dot = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < n_rows; row += stride) {
+    const int64_t br = row / B;
+    const int r = (int)(row - br * B);
+    double acc = 0.0;
+    const int64_t e0 = (int64_t)crow[br], e1 = (int64_t)crow[br + 1];
+    int32_t cn = e0 < e1 ? col[e0] : 0;
+    for (int64_t blk = e0; blk < e1; ++blk) {
+      const int32_t cc = cn;
+      if (blk + 1 < e1) cn = col[blk + 1];
+      const T* v = val + blk * BB + r;
+      const T* xs = x + (int64_t)cc * B;
+      T vv[B], xx[B];
+#pragma unroll
+      for (int c = 0; c < B; ++c) vv[c] = load_nt(v + c * B);
+#pragma unroll
+      for (int c = 0; c < B; ++c) xx[c] = xs[c];
+#pragma unroll
+      for (int c = 0; c < B; ++c) acc += static_cast<double>(vv[c]) * static_cast<double>(xx[c]);
+    }
+    y[row] = static_cast<T>(acc);
+    if (partials != nullptr) dot += acc * static_cast<double>(x[row]);
+  }
+  if (partials != nullptr) write_partial(dot, partials, scratch);
+}
+
+// ---------------------------------------------------------------------------------------
 // BSR with a small block size B (3, 5, 6, 7: DG elements): one lane per block. The G lanes
 // of a block row take its blocks lg, lg + G, ...; a lane reads its block's B*B values
 // (contiguous) and the B entries of x it multiplies, keeps B row sums, and the G lanes'
@@ -385,6 +431,14 @@ int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, con
     hipLaunchKernelGGL((bsr_blk_kernel<T, I, B, G>), dim3((unsigned)grid), dim3(kBlock), \
                        0, s, nbr, cr, col, v, xv, yv, partials, gate);                   \
   } while (0)
+#define DDS_ROWB(B)                                                                      \
+  do {                                                                                   \
+    grid = grid_for(nbr * B, capped);                                                    \
+    hipLaunchKernelGGL((bsr_rowb_kernel<T, I, B>), dim3((unsigned)grid), dim3(kBlock), 0, \
+                       s, nbr, cr, col, v, xv, yv, partials, gate);                      \
+  } while (0)
+  // measured on the DG shapes of the reference's BSR benchmark: one lane per block for
+  // b <= 7, one lane per row with the block size compiled in for the larger DG sizes
   switch (b) {
     case 2: DDS_SEG(2, 16); break;
     case 3: DDS_BLK(3, 4); break;
@@ -393,6 +447,10 @@ int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, con
     case 6: DDS_BLK(6, 8); break;
     case 7: DDS_BLK(7, 8); break;
     case 8: DDS_SEG(8, 64); break;
+    case 10: DDS_ROWB(10); break;
+    case 15: DDS_ROWB(15); break;
+    case 21: DDS_ROWB(21); break;
+    case 28: DDS_ROWB(28); break;
     default:
       grid = grid_for(nbr * b, capped);
       hipLaunchKernelGGL((bsr_row_kernel<T, I>), dim3((unsigned)grid), dim3(kBlock), 0, s, nbr,
@@ -400,6 +458,7 @@ int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, con
   }
 #undef DDS_SEG
 #undef DDS_BLK
+#undef DDS_ROWB
   if (n_partials) *n_partials = partials ? (int)grid : 0;
   DDS_LAUNCHED();
   return DDS_OK;

@@ -377,6 +377,16 @@ class _SELLOperator:
         e = self.values.element_size()
         return self.nnz * (e + 4) + 8 * (self.shape[0] + 1) + e * (self.shape[0] + self.shape[1])
 
+    @property
+    def bytes_physical(self) -> int:
+        """Bytes the kernel itself moves at least once: every stored slot (padding
+        included) of the values and of the compact columns (2 bytes per entry, 4 in wide
+        slices), the per-slot column bases, the slice tables, x and y."""
+        e = self.values.element_size()
+        n_slices = self.slice_ptr.shape[0] - 1
+        return (self.stored * e + 2 * self.col16.numel() + 4 * self.col32.numel()
+                + 4 * self.slot_base.numel() + 16 * n_slices + e * (self.shape[0] + self.shape[1]))
+
 
 class _TorchOperator:
     """Fallback operator for matrices the kernels do not take (dense, other layouts)."""

@@ -93,7 +93,7 @@ def test_bsr_fixture(dtype):
     assert np.allclose(y.cpu().numpy(), ref, atol=1e-12 if dtype == torch.float64 else 1e-5)
 
 
-@pytest.mark.parametrize("b", [1, 2, 3, 4, 5, 6, 7, 8, 10])
+@pytest.mark.parametrize("b", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 15, 21, 28])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_bsr_random(b, dtype):
     # test_bsr.py:161-193: 1000 block rows, 3 random blocks per row (duplicates allowed)
