@@ -505,6 +505,51 @@ REF_BSR_A100_MS = {
 }
 
 
+# The same benchmark's 3D rows: DG-SIPG on the tetrahedral S_m (6 * 8^m cells), one block
+# per tetrahedron and per interior face (each direction); RPT-I, min over repetitions.
+REF_BSR3D_A100_MS = {(7, 4): (6.7186, 38), (6, 10): (4.3284, 45), (5, 20): (1.9365, 52)}
+
+
+def dg_tet_bsr(m: int, b: int, dev, seed: int = 0):
+    """A BSR matrix with the block structure of DG on the tetrahedral mesh S_m
+    (dg.UniformTetMesh: the 2^m grid of cubes, six Kuhn tetrahedra each), random FP64
+    values; the block counts of the reference's 3D rows. Built on the device: the faces are
+    matched by the sum of their vertices' grid coordinates (unique in a Kuhn mesh)."""
+    import torch
+    from dd_solvers.dg import _KUHN_ORDERS
+
+    n = 1 << m
+    cube = torch.arange(n ** 3, device=dev)
+    base = torch.stack([cube % n, (cube // n) % n, cube // (n * n)], 1)
+    eye = torch.eye(3, dtype=torch.int64, device=dev)
+    path = torch.zeros((6, 4, 3), dtype=torch.int64, device=dev)
+    for t, (a, b_, c) in enumerate(_KUHN_ORDERS.tolist()):
+        path[t, 1] = eye[a]
+        path[t, 2] = eye[a] + eye[b_]
+        path[t, 3] = 1
+    P = (base[:, None, None, :] + path[None]).reshape(-1, 4, 3)      # (T, 4, 3) grid coords
+    T = P.shape[0]
+    lf = torch.tensor([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]], device=dev)
+    S = P[:, lf].sum(2).reshape(-1, 3)                                 # (4T, 3)
+    key = (S[:, 2] * (3 * n + 1) + S[:, 1]) * (3 * n + 1) + S[:, 0]
+    del P, S
+    ks, order = torch.sort(key, stable=True)
+    pair = ks[1:] == ks[:-1]
+    i0 = torch.nonzero(pair).flatten()
+    c1, c2 = order[i0] // 4, order[i0 + 1] // 4
+    del key, ks, order, pair, i0
+    rows = torch.cat([torch.arange(T, device=dev), c1, c2])
+    cols = torch.cat([torch.arange(T, device=dev), c2, c1])
+    del c1, c2
+    o = torch.argsort(rows * T + cols)
+    rows, cols = rows[o], cols[o]
+    crow = torch.zeros(T + 1, dtype=torch.int64, device=dev)
+    crow[1:] = torch.cumsum(torch.bincount(rows, minlength=T), 0)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    vals = torch.rand((cols.shape[0], b, b), dtype=torch.float64, device=dev, generator=g)
+    return torch.sparse_bsr_tensor(crow, cols.to(torch.int32), vals, (T * b, T * b))
+
+
 def dg_triangle_bsr(m: int, b: int, dev, seed: int = 0):
     """A BSR matrix with the block structure of DG on the uniform triangle mesh S_m (N = 2^m
     squares per side, each cut into a lower and an upper triangle; a block per triangle and
@@ -533,26 +578,28 @@ def dg_triangle_bsr(m: int, b: int, dev, seed: int = 0):
 
 
 def bsr_reference_shapes(dev):
-    """FastBSR (the drop-in class) on the reference's 2D DG benchmark shapes."""
+    """FastBSR (the drop-in class) on the reference's DG benchmark shapes, 2D and 3D."""
     import torch
     import dd_solvers as dds
 
     out = []
-    for (m, b), (ref_ms, line) in REF_BSR_A100_MS.items():
-        A = dg_triangle_bsr(m, b, dev)
+    shapes = [(2, m, b, ref) for (m, b), ref in REF_BSR_A100_MS.items()] + \
+             [(3, m, b, ref) for (m, b), ref in REF_BSR3D_A100_MS.items()]
+    for dim, m, b, (ref_ms, line) in shapes:
+        A = dg_triangle_bsr(m, b, dev) if dim == 2 else dg_tet_bsr(m, b, dev)
         op = dds.FastBSR(A)
         x = torch.rand(A.shape[1], dtype=torch.float64, device=dev)
         y = torch.empty(A.shape[0], dtype=torch.float64, device=dev)
         ms = kernel_time_ms(lambda: op._enqueue_matvec(x, y, None, None), reps=20)
         # the reference's GB/s: values + block columns + block-row pointers + x + y
         nbytes = op.bytes_per_matvec
-        out.append({"mesh": f"S{m}", "b": b, "dofs": int(A.shape[0]), "nnzb": op.nnzb,
+        out.append({"dim": dim, "mesh": f"S{m}", "b": b, "dofs": int(A.shape[0]), "nnzb": op.nnzb,
                     "ms": round(ms, 4), "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
                     "reference_a100_ms": ref_ms, "speedup_vs_a100": round(ref_ms / ms, 2),
                     "reference_csv_line": line})
         del A, op, x, y
         torch.cuda.empty_cache()
-    log("BSR vs reference A100: " + ", ".join(f"S{o['mesh'][1:]} b={o['b']} {o['ms']} ms "
+    log("BSR vs reference A100: " + ", ".join(f"{o['dim']}D S{o['mesh'][1:]} b={o['b']} {o['ms']} ms "
                                                f"({o['GBs']} GB/s, {o['speedup_vs_a100']}x)" for o in out))
     return out
 

@@ -128,10 +128,10 @@ __global__ __launch_bounds__(kBlock) void bsr_row_kernel(int64_t n_block_rows, i
 }
 
 // ---------------------------------------------------------------------------------------
-// BSR with a block size B known at compile time (the DG sizes (p+1)(p+2)/2): one lane per
-// scalar row as above (blocks in stored order, c ascending), with the column loop unrolled
-// so that a block's B values and B entries of x are all in flight at once, and the next
-// block's column index loaded ahead.
+// BSR with a block size B known at compile time (the DG sizes: (p+1)(p+2)/2 on triangles,
+// (p+1)(p+2)(p+3)/6 on tetrahedra): one lane per scalar row as above (blocks in stored
+// order, c ascending), with the column loop unrolled so that a block's B values and B
+// entries of x are all in flight at once, and the next block's column index loaded ahead.
 // ---------------------------------------------------------------------------------------
 template <typename T, typename I, int B>
 __global__ __launch_bounds__(kBlock) void bsr_rowb_kernel(int64_t n_block_rows,
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void bsr_rowb_kernel(int64_t n_block_rows,
 }
 
 // ---------------------------------------------------------------------------------------
-// BSR with a small block size B (3, 5, 6, 7: DG elements): one lane per block. The G lanes
+// BSR with a small block size B (3 .. 7: DG elements, and b = 4): one lane per block. The G lanes
 // of a block row take its blocks lg, lg + G, ...; a lane reads its block's B*B values
 // (contiguous) and the B entries of x it multiplies, keeps B row sums, and the G lanes'
 // sums are combined by a butterfly; lane r mod G writes row r. Far fewer load instructions
@@ -228,8 +228,9 @@ __global__ __launch_bounds__(kBlock) void bsr_blk_kernel(int64_t n_block_rows,
 }
 
 // ---------------------------------------------------------------------------------------
-// BSR with a power-of-two block size B: G lanes stream the block row's contiguous value
-// segment (nnzb_row * B * B elements), so consecutive lanes read consecutive values. With
+// BSR with a power-of-two block size B (used for b = 2): G lanes stream the block row's
+// contiguous value segment (nnzb_row * B * B elements), so consecutive lanes read
+// consecutive values. With
 // column-major blocks element q of the segment sits in row q % B, which is lane % B for
 // every lane because G is a multiple of B; lanes with equal lane % B are summed by a
 // butterfly over xor offsets G/2 .. B, leaving row r's result in lane r.
@@ -437,17 +438,20 @@ int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, con
     hipLaunchKernelGGL((bsr_rowb_kernel<T, I, B>), dim3((unsigned)grid), dim3(kBlock), 0, \
                        s, nbr, cr, col, v, xv, yv, partials, gate);                      \
   } while (0)
-  // measured on the DG shapes of the reference's BSR benchmark: one lane per block for
-  // b <= 7, one lane per row with the block size compiled in for the larger DG sizes
+  // measured on the DG shapes of the reference's BSR benchmark (2D and 3D) and on the Q1
+  // 64^3 operator at b = 2, 4, 8: the value segment streamed for b = 2, one lane per block
+  // for b = 3 .. 7, one lane per row with the block size compiled in for b = 8 and the
+  // larger DG sizes
   switch (b) {
     case 2: DDS_SEG(2, 16); break;
     case 3: DDS_BLK(3, 4); break;
-    case 4: DDS_SEG(4, 32); break;
+    case 4: DDS_BLK(4, 8); break;
     case 5: DDS_BLK(5, 4); break;
     case 6: DDS_BLK(6, 8); break;
     case 7: DDS_BLK(7, 8); break;
-    case 8: DDS_SEG(8, 64); break;
+    case 8: DDS_ROWB(8); break;
     case 10: DDS_ROWB(10); break;
+    case 20: DDS_ROWB(20); break;
     case 15: DDS_ROWB(15); break;
     case 21: DDS_ROWB(21); break;
     case 28: DDS_ROWB(28); break;

@@ -14,6 +14,9 @@ Reads two result files of the reference (text data, not code):
                                         over subdomain / coarse meshes S8-S11 and C8-C11
                                         ("experiment": "parallelism"; the precision runs
                                         carry "experiment": "precisions")
+  results/benchmark_bsrmv.csv        -> per (dimension, mesh, degree) of the BSR SpMV
+                                        benchmark: DoFs, DoFs per element, nnz, BSR nnz
+                                        ("bsr_shapes")
 
 Run from the repository root in a container that has the reference checked out at
 /root/reference:  python tests/golden/make_dg_published.py
@@ -80,12 +83,20 @@ def main():
                          "iterations": md["iterations"],
                          "residual_norms": md["residual norms"],
                          "solver": s}
+    bsr_shapes = {}
+    with open(os.path.join(REF, "benchmark_bsrmv.csv")) as fh:
+        for r in csv.DictReader(fh):
+            key = f"{r['dimension']}_{r['mesh']}_p{r['degree']}"
+            bsr_shapes.setdefault(key, {"dim": int(r["dimension"][0]), "mesh": r["mesh"],
+                                        "p": int(r["degree"]), "dofs": int(r["DOFs"]),
+                                        "dofs_per_element": int(r["DOFs per element"]),
+                                        "nnz": int(r["nnz"]), "bsr_nnz": int(r["bsr nnz"])})
     with open(OUT, "w") as fh:
         json.dump({"source": "mgr0dzicki/dd-solvers results/experiment_amgx_pcg.csv, "
-                             "results/experiment_precisions.csv (rtol 1e-9, FP64 runs) and "
-                             "results/experiment_parallelism.csv",
-                   "columns": columns, "levels": levels, "runs": runs, "solves": solves}, fh,
-                  indent=1)
+                             "results/experiment_precisions.csv (rtol 1e-9, FP64 runs), "
+                             "results/experiment_parallelism.csv and results/benchmark_bsrmv.csv",
+                   "columns": columns, "levels": levels, "runs": runs, "solves": solves,
+                   "bsr_shapes": bsr_shapes}, fh, indent=1)
     print(f"wrote {OUT}: {len(levels)} levels, {len(runs)} runs, {len(solves)} solves")
 
 

@@ -153,3 +153,90 @@ def test_reference_nodes_gll_on_edges_and_planted_shift():
     finally:
         dg.reference_nodes = orig
     assert abs(nb - ref) > 1e-8 * ref
+
+
+# ---------------------------------------------------------------------------------------
+# 3D: tetrahedral S_m (the reference's UniformMeshes(3, m)), DG_p for p <= 3
+# ---------------------------------------------------------------------------------------
+
+def _nloc3(p):
+    return (p + 1) * (p + 2) * (p + 3) // 6
+
+
+def test_tet_mesh_counts_match_the_published_bsr_shapes():
+    """6 * 8^m cells and 12 * 4^m boundary faces on the explicit meshes S0-S5, hence a block
+    per cell and two per interior face: nnzb = T + (4 T - 12 * 4^m). The reference's three
+    3D rows of its BSR benchmark (S7 p = 1, S6 p = 2, S5 p = 3) give exactly these DoFs,
+    block counts and nonzeros (S5 from the explicit mesh, S6 and S7 from the counts)."""
+    for m in range(6):
+        M = dg.UniformTetMesh(m)
+        (cp, _), (cm, _), (cb, _) = M.faces()
+        assert M.num_cells == 6 * 8 ** m and cb.shape[0] == 12 * 4 ** m
+        assert 2 * cp.shape[0] + cb.shape[0] == 4 * M.num_cells and np.all(cp != cm)
+    shapes = [v for v in PUB["bsr_shapes"].values() if v["dim"] == 3]
+    assert sorted((v["mesh"], v["p"]) for v in shapes) == [("S5", 3), ("S6", 2), ("S7", 1)]
+    for v in shapes:
+        m, nl = int(v["mesh"][1:]), _nloc3(v["p"])
+        T = 6 * 8 ** m
+        nnzb = T + (4 * T - 12 * 4 ** m)
+        if m <= 5:
+            (cp, _), _, _ = dg.UniformTetMesh(m).faces()
+            assert nnzb == T + 2 * cp.shape[0]
+        assert v["dofs_per_element"] == nl and v["dofs"] == T * nl
+        assert v["bsr_nnz"] == nnzb and v["nnz"] == nnzb * nl * nl
+
+
+def test_tet_mesh_hierarchy():
+    """Each cell of S_3 lies in the S_k cell that parent_cells names (barycentric
+    coordinates >= 0), every S_k cell holds 8^(3-k) of them; C_k cells hold the cells whose
+    centroids they contain."""
+    M = dg.UniformTetMesh(3)
+    X = M.vertices[M.cells]
+    for k in range(3):
+        C = dg.UniformTetMesh(k)
+        par = M.parent_cells(k)
+        Y = C.vertices[C.cells[par]]
+        Tm = np.stack([Y[:, 1] - Y[:, 0], Y[:, 2] - Y[:, 0], Y[:, 3] - Y[:, 0]], -1)
+        lam = np.linalg.solve(Tm[:, None], (X - Y[:, :1])[..., None])[..., 0]
+        bary = np.concatenate([1 - lam.sum(-1, keepdims=True), lam], -1)
+        assert bary.min() > -1e-12
+        assert np.all(np.bincount(par, minlength=C.num_cells) == 8 ** (3 - k))
+        q = M.quad_cells(k)
+        assert np.all(np.bincount(q, minlength=8 ** k) == 6 * 8 ** (3 - k))
+    with pytest.raises(ValueError):
+        dg.reference_nodes_tet(4)
+
+
+@pytest.mark.parametrize("p,levels,order", [(1, (2, 3), 1.6), (2, (1, 2, 3), 2.9), (3, (1, 2), 3.5)])
+def test_sipg_3d_symmetric_spd_and_converges(p, levels, order):
+    """The 3D SIPG system is symmetric and positive definite, and its solution converges to
+    the constant-coefficient 3D test case's exact solution at order p + 1 in L2 (the
+    observed rates on S1-S3 approach it from below: p = 1 1.39, 1.76; p = 2 3.16, 3.25;
+    p = 3 3.82)."""
+    errs = []
+    for m in levels:
+        pr = dg.assemble_sipg(dg.UniformTetMesh(m), p, dg.constant_coefficient_3d)
+        A = pr.A
+        assert pr.block_size == _nloc3(p) and A.shape[0] == 6 * 8 ** m * _nloc3(p)
+        assert abs(A - A.T).max() <= 1e-14 * abs(A).max()
+        if m == levels[0]:
+            assert np.linalg.eigvalsh(A.toarray()).min() > 0
+        u = spla.spsolve(A.tocsc(), pr.b)
+        errs.append(dg.error_norm(pr, u, 2, dg.constant_coefficient_3d))
+    rates = [np.log2(errs[i] / errs[i + 1]) for i in range(len(errs) - 1)]
+    assert rates[-1] > order, rates
+
+
+@pytest.mark.parametrize("kind", ["additive", "hybrid"])
+def test_3d_experiment_with_the_oracle(kind):
+    """The experiment set-up on a 3D system (S3, subdomains S1, coarse C0 / S0): the
+    oracle's Schwarz CG converges, and its first residual is ||b||."""
+    pr = dg.assemble_sipg(dg.UniformTetMesh(3), 1)
+    perm, A, b, k, spc = dg.dg_experiment(pr, "S1", "S0")
+    assert k == 64 * 4 and spc.sum() == 48 and spc.shape[0] == 6
+    O = BlockSchwarzOracle(A, k, spc, kind=kind)
+    x, md = cg(lambda v: A @ v, b, precond=O.apply, T0=O.T0 if kind == "hybrid" else None,
+               rtol=1e-9, maxiter=500)
+    assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b) * 1.01
+    perm2, A2, b2, k2, spc2 = dg.dg_experiment(pr, "C1", "C0")
+    assert spc2.tolist() == [8] and k2 == 3072 // 8 * 4

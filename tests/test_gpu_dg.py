@@ -68,6 +68,29 @@ def test_dg_solve_matches_oracle(p, fine, solvers, coarse, kind):
     assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b) * 1.01
 
 
+@pytest.mark.parametrize("kind", ["additive", "hybrid"])
+@pytest.mark.parametrize("p,fine,solvers,coarse", [(1, 3, "S1", "S0"), (2, 2, "S1", "S0"),
+                                                   (3, 2, "S1", "C0"), (1, 3, "C2", "C1")])
+def test_dg_3d_solve_matches_oracle(p, fine, solvers, coarse, kind):
+    """3D DG-SIPG on the tetrahedral S_m (block sizes 4, 10, 20: the BSR SpMV in the Krylov
+    loop) against the oracle, with the bars of the 2D runs."""
+    print(f"DG 3D p={p} S{fine} solvers {solvers} coarse {coarse} {kind}", flush=True)
+    pr = dg.assemble_sipg(dg.UniformTetMesh(fine), p)
+    perm, A, b, k, spc = dg.dg_experiment(pr, solvers, coarse)
+    O = BlockSchwarzOracle(A, k, spc, kind=kind)
+    xo, mo = oracle_cg(lambda v: A @ v, b, precond=O.apply,
+                       T0=O.T0 if kind == "hybrid" else None, rtol=1e-9, maxiter=2000)
+    x, md = _gpu_solve(A, b, k, spc, pr.block_size, kind)
+    n = min(10, len(md["residual norms"]), len(mo["residual norms"]))
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"  iterations {md['iterations']} (oracle {mo['iterations']}), x rel diff {err:.1e}",
+          flush=True)
+    assert np.allclose(md["residual norms"][:n], mo["residual norms"][:n], rtol=1e-10, atol=0)
+    assert abs(md["iterations"] - mo["iterations"]) <= max(1, mo["iterations"] // 100)
+    assert err <= 1e-9
+    assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b) * 1.01
+
+
 @pytest.mark.parametrize("storage", [torch.float32, torch.float16, torch.bfloat16])
 @pytest.mark.parametrize("kind", ["additive", "hybrid"])
 def test_dg_low_precision_inverses_match_oracle(storage, kind):
