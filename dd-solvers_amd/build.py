@@ -47,8 +47,14 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# host-only translation units of the kernel library (compiled as C++, no device code);
+# torch_op.cpp is the op module, built separately against PyTorch
+HOST_SOURCES = ("nd_plan.cpp",)
+
+
 def sources() -> list[str]:
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                  if f.endswith(".hip") or f in HOST_SOURCES)
 
 
 def _run(cmd: list[str]) -> None:
@@ -69,10 +75,15 @@ def build(arch: str = "gfx950", jobs: int | None = None, verbose: bool = False) 
     objs = []
     cmds = []
     for src in srcs:
-        obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
+        base = os.path.basename(src)
+        obj = os.path.join(BUILD, os.path.splitext(base)[0] + ".o")
         objs.append(obj)
+        if base in HOST_SOURCES:
+            cmds.append([cc, "-x", "c++", "-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread",
+                         "-I", INCLUDE, "-c", src, "-o", obj])
+            continue
         cmds.append([cc, f"--offload-arch={arch}", *HIP_FLAGS,
-                     *FILE_FLAGS.get(os.path.basename(src), []), "-I", INCLUDE, "-I", CSRC,
+                     *FILE_FLAGS.get(base, []), "-I", INCLUDE, "-I", CSRC,
                      "-c", src, "-o", obj])
     jobs = jobs or min(8, len(cmds))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -81,7 +92,7 @@ def build(arch: str = "gfx950", jobs: int | None = None, verbose: bool = False) 
                 print(" ".join(cmd))
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([cc, f"--offload-arch={arch}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([cc, f"--offload-arch={arch}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     build_op_module()
     return LIB

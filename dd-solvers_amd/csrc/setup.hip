@@ -24,20 +24,27 @@ constexpr int kTile = 64;
 constexpr int kThreads = 256;
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void gj_inverse_kernel(T* __restrict__ A, int64_t stride, int64_t ld,
-                                                              const int32_t* __restrict__ ks,
-                                                              int32_t* __restrict__ info) {
-  __shared__ T P[kNB][kNB + 1];
-  __shared__ T Ti[kTile][kNB + 1];
-  __shared__ T Rt[kNB][kTile + 1];
-  __shared__ int bad;
+struct GjLds {
+  T P[kNB][kNB + 1];
+  T Ti[kTile][kNB + 1];
+  T Rt[kNB][kTile + 1];
+};
+
+// Eliminates the first kp pivots (kp <= k) of the k x k matrix at a (row-major, leading
+// dimension ld) in place, by the steps above applied to every row and column of the
+// matrix: with kp = k the result is A^-1; with kp < k and A = [A_JJ A_JS; A_SJ A_SS]
+// (J the first kp), it is [A_JJ^-1, A_JJ^-1 A_JS; -A_SJ A_JJ^-1, A_SS - A_SJ A_JJ^-1 A_JS].
+// Sets bad = 1 on a pivot <= 0. Called by all kThreads threads of the workgroup; the
+// workgroup's global writes are visible to all its threads on return.
+template <typename T>
+__device__ void gj_sweep(T* __restrict__ a, int k, int kp, int64_t ld, GjLds<T>& L, int& bad) {
+  auto& P = L.P;
+  auto& Ti = L.Ti;
+  auto& Rt = L.Rt;
   const int tid = threadIdx.x;
-  T* a = A + (int64_t)blockIdx.x * stride;
-  const int k = ks[blockIdx.x];
-  if (tid == 0) bad = 0;
   const int ty = tid >> 4, tx = tid & 15;
-  for (int p0 = 0; p0 < k; p0 += kNB) {
-    const int nb = min(kNB, k - p0);
+  for (int p0 = 0; p0 < kp; p0 += kNB) {
+    const int nb = min(kNB, kp - p0);
     // 1. pivot block into LDS (identity outside nb x nb) and its inverse by scalar sweeps
     for (int e = tid; e < kNB * kNB; e += kThreads) {
       const int r = e / kNB, c = e % kNB;
@@ -147,7 +154,102 @@ __global__ __launch_bounds__(kThreads) void gj_inverse_kernel(T* __restrict__ A,
     }
     __syncthreads();
   }
-  if (tid == 0 && info != nullptr) info[blockIdx.x] = bad;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gj_inverse_kernel(T* __restrict__ A, int64_t stride, int64_t ld,
+                                                              const int32_t* __restrict__ ks,
+                                                              int32_t* __restrict__ info) {
+  __shared__ GjLds<T> lds;
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const int k = ks[blockIdx.x];
+  gj_sweep<T>(A + (int64_t)blockIdx.x * stride, k, k, ld, lds, bad);
+  if (threadIdx.x == 0 && info != nullptr) info[blockIdx.x] = bad;
+}
+
+// One front of the multifrontal block LDL^T factorisation per workgroup (fronts lo ..
+// lo + gridDim.x - 1 of one height): assembly of F (M x M, row-major at work +
+// work_off[b]) from the matrix entries in the front's columns and the children's update
+// matrices (child by child, in slot order: every entry's sum has the same order on every
+// run), then (mode 0) the partial Gauss-Jordan sweep of the nJ pivots and the packing of
+// F_JJ^-1, W = -(swept F_SJ) and U = swept F_SS.
+__global__ __launch_bounds__(kThreads) void nd_factor_kernel(
+    int mode, int64_t lo, const int64_t* __restrict__ fj0, const int64_t* __restrict__ fnj,
+    const int64_t* __restrict__ fns, const int64_t* __restrict__ foff,
+    const int64_t* __restrict__ wtoff, const int64_t* __restrict__ moff,
+    const int64_t* __restrict__ uoff2, const int64_t* __restrict__ child_ptr,
+    const int64_t* __restrict__ child_idx, const int32_t* __restrict__ maps,
+    const int64_t* __restrict__ s_idx, const int64_t* __restrict__ ap_ptr,
+    const int32_t* __restrict__ ap_idx, const double* __restrict__ ap_val,
+    const int64_t* __restrict__ work_off, double* __restrict__ work,
+    double* __restrict__ ustore, double* __restrict__ fac, int32_t* __restrict__ info) {
+  __shared__ GjLds<double> lds;
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  const int64_t t = lo + blockIdx.x;
+  const int64_t a = fj0[t], nJ = fnj[t], nS = fns[t], M = nJ + nS;
+  double* F = work + work_off[blockIdx.x];
+  if (tid == 0) bad = 0;
+  for (int64_t e = tid; e < M * M; e += kThreads) F[e] = 0.0;
+  __syncthreads();
+  // entries of the permuted matrix in the columns J: row a + r gives F[pos(col)][r] (and
+  // the mirror F[r][pos(col)] for a coupling column); columns below a were eliminated below
+  const int64_t* S = s_idx + moff[t];
+  for (int64_t r = tid; r < nJ; r += kThreads) {
+    for (int64_t q = ap_ptr[a + r]; q < ap_ptr[a + r + 1]; ++q) {
+      const int64_t col = ap_idx[q];
+      if (col < a) continue;
+      const double v = ap_val[q];
+      if (col < a + nJ) {
+        F[(col - a) * M + r] = v;
+      } else {
+        int64_t lo_s = 0, hi_s = nS;  // position of col in the ascending coupling set
+        while (lo_s < hi_s) {
+          const int64_t mid = (lo_s + hi_s) >> 1;
+          if (S[mid] < col) lo_s = mid + 1;
+          else hi_s = mid;
+        }
+        const int64_t pc = nJ + lo_s;
+        F[pc * M + r] = v;
+        F[r * M + pc] = v;
+      }
+    }
+  }
+  __syncthreads();
+  for (int64_t q = child_ptr[t]; q < child_ptr[t + 1]; ++q) {
+    const int64_t c = child_idx[q];
+    const int64_t nc = fns[c];
+    const double* U = ustore + uoff2[c];
+    const int32_t* p = maps + moff[c];
+    for (int64_t e = tid; e < nc * nc; e += kThreads) {
+      const int64_t i = e / nc, j = e - i * nc;
+      F[(int64_t)p[i] * M + p[j]] += U[e];
+    }
+    __syncthreads();
+  }
+  if (mode != 0) return;
+  gj_sweep<double>(F, (int)M, (int)nJ, M, lds, bad);
+  const int64_t m = M;
+  double* st = fac + foff[t];
+  for (int64_t e = tid; e < nJ * nJ; e += kThreads) {
+    const int64_t c = e / nJ, r = e - c * nJ;
+    st[c * m + r] = F[r * M + c];
+  }
+  double* wt = fac + wtoff[t];
+  for (int64_t e = tid; e < nS * nJ; e += kThreads) {
+    const int64_t s = e / nJ, c = e - s * nJ;
+    const double w = -F[(nJ + s) * M + c];
+    st[c * m + nJ + s] = w;
+    wt[s * nJ + c] = w;
+  }
+  double* u = ustore + uoff2[t];
+  for (int64_t e = tid; e < nS * nS; e += kThreads) {
+    const int64_t i = e / nS, j = e - i * nS;
+    u[e] = F[(nJ + i) * M + nJ + j];
+  }
+  if (tid == 0) info[blockIdx.x] = bad;
 }
 
 // out[i] = vals[seg_ptr[i]] + vals[seg_ptr[i] + 1] + ... + vals[seg_ptr[i + 1] - 1], added
@@ -319,6 +421,28 @@ extern "C" int dds_spd_inverse_batched(int dtype, int64_t n_mats, const int32_t*
     set_error("dds_spd_inverse_batched: dtype must be F64 or F32 (got %d)", dtype);
     return DDS_EUNSUPPORTED;
   }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_nd_factor(int mode, int64_t lo, int64_t hi, const int64_t* fj0,
+                             const int64_t* fnj, const int64_t* fns, const int64_t* foff,
+                             const int64_t* wtoff, const int64_t* moff, const int64_t* uoff2,
+                             const int64_t* child_ptr, const int64_t* child_idx,
+                             const int32_t* maps, const int64_t* s_idx, const int64_t* ap_ptr,
+                             const int32_t* ap_idx, const double* ap_val,
+                             const int64_t* work_off, double* work, double* ustore, double* fac,
+                             int32_t* info, void* stream) {
+  DDS_REQUIRE((mode == 0 || mode == 1) && lo >= 0 && hi >= lo && hi - lo <= INT32_MAX,
+              "dds_nd_factor: bad mode or front range");
+  if (hi == lo) return DDS_OK;
+  DDS_REQUIRE(fj0 && fnj && fns && moff && uoff2 && child_ptr && ap_ptr && work_off && work,
+              "dds_nd_factor: null table");
+  DDS_REQUIRE(mode == 1 || (foff && wtoff && ustore && fac && info), "dds_nd_factor: null output");
+  hipLaunchKernelGGL(nd_factor_kernel, dim3((unsigned)(hi - lo)), dim3(kThreads), 0,
+                     as_stream(stream), mode, lo, fj0, fnj, fns, foff, wtoff, moff, uoff2,
+                     child_ptr, child_idx, maps, s_idx, ap_ptr, ap_idx, ap_val, work_off, work,
+                     ustore, fac, info);
   DDS_LAUNCHED();
   return DDS_OK;
 }

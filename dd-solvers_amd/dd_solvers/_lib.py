@@ -82,6 +82,9 @@ _SIGNATURES = {
     "dds_nd_backward": [_i, _i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p],
     "dds_nd_top_assemble": [_i, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p],
     "dds_nd_top_push": [_i, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
+    "dds_nd_plan_build": [_i64, _p, _p, _i64, _i64, _p],
+    "dds_nd_factor": [_i, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                      _p, _p, _p, _p, _p, _p, _p],
     "dds_pcg_residual": [_i64, _p, _p, _p, _p, _ip, _p, _p],
     "dds_pcg_update_p": [_i64, _p, _p, _p, _p],
     "dds_pcg_update_xr": [_i64, _p, _p, _p, _p, _p, _ip, _p, _p],
@@ -114,8 +117,13 @@ for _name, _args in _SIGNATURES.items():
     _fn = getattr(_lib, _name)
     _fn.argtypes = _args
     _fn.restype = ctypes.c_int
+# the symbolic-plan accessors return a length and nothing
+_lib.dds_nd_plan_field.argtypes = [_p, _i, _p]
+_lib.dds_nd_plan_field.restype = ctypes.c_int64
+_lib.dds_nd_plan_free.argtypes = [_p]
+_lib.dds_nd_plan_free.restype = None
 
-EXPORTED = ["dds_last_error", *_SIGNATURES]
+EXPORTED = ["dds_last_error", "dds_nd_plan_field", "dds_nd_plan_free", *_SIGNATURES]
 
 if _lib.dds_abi_version() != ABI_VERSION:
     raise ImportError(f"{LIB_PATH}: ABI version {_lib.dds_abi_version()} != {ABI_VERSION}; rebuild")

@@ -352,6 +352,80 @@ int dds_dot_partials(int64_t n, const double* a, const double* b, double* partia
 int dds_pcg_finalize(int op, const double* partials, int count, dds_pcg_state* st,
                      double* norms, double* alphas, double* betas, void* stream);
 /* Plain sum of `count` partials into *out (device double), for dot products outside PCG. */
+/* Symbolic analysis (host code, no device work): the nested-dissection ordering of the
+ * symmetric pattern (n x n CSR, int64 row pointers, int32 column indices sorted and unique
+ * within each row; the diagonal may be present) and every table of the factorisation and of
+ * the sweeps above, held by the library until dds_nd_plan_free. Tree nodes are split down to
+ * leaf_size unknowns; nodes of depth < d_top, the largest bound whose nodes hold at most
+ * top_max unknowns together, form the dense top front. Fails (DDS_EINVAL) when a lower
+ * front exceeds the sweeps' 7680-unknown LDS capacity. Replaces the analysis phase of
+ * cudssExecute(ANALYSIS) in spdFactorize (src/dd_solvers/cpp/cudss.cpp:80-96), which orders
+ * the coarse matrix on the host as well. */
+typedef struct dds_nd_plan dds_nd_plan;
+int dds_nd_plan_build(int64_t n, const int64_t* indptr, const int32_t* indices,
+                      int64_t leaf_size, int64_t top_max, dds_nd_plan** plan);
+/* Fields of a plan (element type; length). F fronts, L heights (the top front's height
+ * last), L' = L minus the top's height. dds_nd_plan_field copies a field into dst (unless
+ * dst is NULL) and returns its length in elements, -1 for an unknown field. */
+enum dds_nd_field {
+  DDS_ND_SCALARS = 0,   /* int64; DDS_ND_N_SCALARS, indexed by dds_nd_scalar */
+  DDS_ND_PERM = 1,      /* int64; n: perm[i] = original index of unknown i */
+  DDS_ND_FJ0 = 2,       /* int64; F: first unknown of each front */
+  DDS_ND_FNJ = 3,       /* int64; F: nJ */
+  DDS_ND_FNS = 4,       /* int64; F: nS */
+  DDS_ND_LEVEL_PTR = 5, /* int64; L + 1: fronts of height h are [level_ptr[h], level_ptr[h+1]) */
+  DDS_ND_FOFF = 6,      /* int64; F + 1: stacked factor offsets */
+  DDS_ND_WTOFF = 7,     /* int64; F + 1: row-major W offsets (wtoff[F] = factor size) */
+  DDS_ND_SOFF = 8,      /* int64; F + 1: forward slot offsets */
+  DDS_ND_WBASE = 9,     /* int64; F: the front's slot in its parent's front (-1: none) */
+  DDS_ND_MOFF = 10,     /* int64; F + 1: coupling-set offsets (maps, s_idx, backward buffers) */
+  DDS_ND_MAPS = 11,     /* int32; moff[F]: positions of a coupling set in the parent front */
+  DDS_ND_PUSH_DST = 12, /* int64; push entries */
+  DDS_ND_PUSH_SRC = 13, /* int32; push entries */
+  DDS_ND_PUSH_PTR = 14, /* int64; F + 1 */
+  DDS_ND_PARENT = 15,   /* int64; F (-1: none) */
+  DDS_ND_CHILD_PTR = 16,/* int64; F + 1 */
+  DDS_ND_CHILD_IDX = 17,/* int64; children of each front, in slot order */
+  DDS_ND_S_IDX = 18,    /* int64; moff[F]: coupling sets (permuted numbering, ascending) */
+  DDS_ND_AP_PTR = 19,   /* int64; n + 1: the permuted pattern A[perm][:, perm] */
+  DDS_ND_AP_IDX = 20,   /* int32; nnz: its column indices, ascending per row */
+  DDS_ND_AP_SRC = 21,   /* int64; nnz: position of each of its entries in the input */
+  DDS_ND_FWD_PTR = 22,  /* int64; L' + 1: forward items per height */
+  DDS_ND_FWD_ITEMS = 23,/* int64; 12 per forward item (dds_nd_forward records) */
+  DDS_ND_FWD_LDS = 24,  /* int64; L': lds_elems of each forward launch */
+  DDS_ND_BWD_PTR = 25,  /* int64; L' + 1 */
+  DDS_ND_BWD_ITEMS = 26,/* int64; 12 per backward item (dds_nd_backward records) */
+  DDS_ND_BWD_LDS = 27,  /* int64; L' */
+  DDS_ND_UOFF2 = 28     /* int64; F + 1: update-matrix offsets (nS x nS per front) */
+};
+enum dds_nd_scalar {
+  DDS_ND_N = 0, DDS_ND_FRONTS = 1, DDS_ND_TOP = 2 /* top-front unknowns (0: none) */,
+  DDS_ND_DTOP = 3, DDS_ND_LEVELS = 4, DDS_ND_LOWER_LEVELS = 5, DDS_ND_FAC_SIZE = 6,
+  DDS_ND_SLOT_SIZE = 7, DDS_ND_XS_SIZE = 8, DDS_ND_NNZ = 9, DDS_ND_TREE_NODES = 10,
+  DDS_ND_N_SCALARS = 11
+};
+int64_t dds_nd_plan_field(const dds_nd_plan* plan, int field, void* dst);
+void dds_nd_plan_free(dds_nd_plan* plan);
+/* Numeric factorisation of the fronts [lo, hi) of one height (FP64, setup), one workgroup
+ * per front. Front t's matrix F (M x M, M = nJ + nS, row-major at work + work_off[t - lo]) is
+ * assembled from the entries of the permuted matrix in its columns (ap_ptr/ap_idx/ap_val;
+ * rows J, columns >= j0, coupling columns found in s_idx[moff[t] ..]) and the update
+ * matrices of its children (ustore + uoff2[c], nS_c x nS_c row-major, added at maps[moff[c]
+ * ..] child by child in slot order). mode 0: the first nJ pivots of F are eliminated by
+ * blocked Gauss-Jordan (no pivoting), which leaves F_JJ^-1, -W = -F_SJ F_JJ^-1 and the
+ * Schur complement U = F_SS - W F_JS in place; the stacked factor [F_JJ^-1; W] goes to fac +
+ * foff[t] (column-major), W row-major to fac + wtoff[t], U to ustore + uoff2[t]; info[t - lo]
+ * = 1 when a pivot is not positive (not SPD). mode 1 (the top front): assembly only, the
+ * dense front is then factored by dds_chol_factor and inverted by dds_chol_inverse into fac
+ * + foff[t]. Replaces cudssExecute(FACTORIZATION) in spdFactorize (cudss.cpp:97-105). */
+int dds_nd_factor(int mode, int64_t lo, int64_t hi, const int64_t* fj0, const int64_t* fnj,
+                  const int64_t* fns, const int64_t* foff, const int64_t* wtoff,
+                  const int64_t* moff, const int64_t* uoff2, const int64_t* child_ptr,
+                  const int64_t* child_idx, const int32_t* maps, const int64_t* s_idx,
+                  const int64_t* ap_ptr, const int32_t* ap_idx, const double* ap_val,
+                  const int64_t* work_off, double* work, double* ustore, double* fac,
+                  int32_t* info, void* stream);
+
 int dds_sum_partials(const double* partials, int count, double* out, void* stream);
 
 /* ---- setup -------------------------------------------------------------------------------

@@ -9,7 +9,8 @@ Reads two result files of the reference (text data, not code):
                                         (p, fine, solvers, coarse, Additive/Hybrid);
                                         for every run ("solves"): the solver's
                                         precisions, its iteration count (null when it
-                                        raised) and its best solve time on the A100
+                                        raised), its best solve time and its best
+                                        solver set-up time on the A100
   results/experiment_parallelism.csv -> the same "solves" records for the p = 1 S11 runs
                                         over subdomain / coarse meshes S8-S11 and C8-C11
                                         ("experiment": "parallelism"; the precision runs
@@ -55,6 +56,7 @@ def main():
         pi = s.split("Schwarz(")[1].split(",")[0].replace("torch.", "")
         inv = s.split("Inv(")[1].split(")")[0].replace("torch.", "") or None
         times = ast.literal_eval(r["solve times"]) if r["solve times"] else []
+        setup = ast.literal_eval(r["solver setup times"]) if r.get("solver setup times") else []
         md = ast.literal_eval(r["metadata"]) if r["metadata"] else {}
         return {"experiment": experiment, "p": int(r["p"]), "fine": r["fine m"],
                 "solvers": r["solvers m"], "coarse": r["coarse m"],
@@ -62,6 +64,7 @@ def main():
                 "precision": pi, "inv": inv,
                 "iterations": None if r["exception"] else md.get("iterations"),
                 "best_ms": min(times) if times and not r["exception"] else None,
+                "setup_ms": min(setup) if setup else None,
                 "solver": s}
 
     with open(os.path.join(REF, "experiment_parallelism.csv")) as fh:

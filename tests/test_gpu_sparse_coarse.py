@@ -16,6 +16,7 @@ import dd_solvers as dds
 from dd_solvers import dg
 from dd_solvers import sparse_coarse as sc
 
+from oracle import nd_plan as onp
 from test_sparse_coarse import dg_coarse, lap2d, random_spd_pieces
 
 pytestmark = pytest.mark.gpu
@@ -55,8 +56,13 @@ def test_solve_matches_superlu(name, dtype):
     tol = 1e-11 if dtype == torch.float64 else 2e-4
     assert np.linalg.norm(x - xr) <= tol * np.linalg.norm(xr)
     if dtype == torch.float64 and A.shape[0] < 5000:
-        # the host restatement of the sweeps over the same tables and factor
-        xh = sc.solve_host(S.plan, S.fac.cpu().numpy(), b)
+        # the device factor (Gauss-Jordan sweeps of the fronts, dense Cholesky of the top)
+        # against the restated one (Cholesky of every front), and the host restatement of
+        # the sweeps over the same tables and factor
+        fac = S.fac.cpu().numpy()
+        fh = onp.factorize_host(S.plan, S.plan.Ap.data)
+        assert np.abs(fac - fh).max() <= 1e-12 * np.abs(fh).max()
+        xh = onp.solve_host(S.plan, fac, b)
         assert np.abs(x - xh).max() <= 1e-13 * np.abs(xh).max()
 
 

@@ -56,6 +56,31 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.EXPORTED) >= declared
 
 
+def _declared_arity():
+    """Number of parameters of every dds_* prototype in include/*.h."""
+    out = {}
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            for name, params in re.findall(r"\b(dds_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", txt, flags=re.S):
+                params = params.strip()
+                out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_signatures_match_the_header():
+    """The ctypes binding declares as many arguments as the C prototype of each entry point
+    (an argument missing from the binding would be passed with the default int conversion,
+    truncating a 64-bit pointer)."""
+    arity = _declared_arity()
+    assert arity["dds_nd_factor"] == 23 and arity["dds_last_error"] == 0
+    for name in _lib.EXPORTED:
+        fn = getattr(_lib._lib, name)
+        assert fn.argtypes is not None, name
+        assert len(fn.argtypes) == arity[name], (name, len(fn.argtypes), arity[name])
+
+
 def test_abi_version():
     assert _lib._lib.dds_abi_version() == _lib.ABI_VERSION
 

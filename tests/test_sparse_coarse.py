@@ -1,8 +1,8 @@
-"""Sparse coarse solver (dd_solvers.sparse_coarse) on the host: the nested-dissection
-ordering, the symbolic plan and the block LDL^T factor, checked through a host restatement
-of the device sweeps (sparse_coarse.solve_host, same tables) against scipy's sparse direct
-solve (SuperLU) of the same matrix. No GPU is needed: the factorisation runs in torch on
-the CPU."""
+"""Sparse coarse solver (dd_solvers.sparse_coarse) on the host: the library's native
+symbolic analysis (dds_nd_plan_build) against its restatement (oracle.nd_plan.NDPlan) table
+by table, and the plan with the restated block LDL^T factor and sweeps
+(oracle.nd_plan.factorize_host / solve_host, same tables) against scipy's sparse direct
+solve (SuperLU) of the same matrix. No GPU is needed."""
 
 import numpy as np
 import pytest
@@ -12,6 +12,25 @@ import torch
 
 from dd_solvers import _lib, dg
 from dd_solvers import sparse_coarse as sc
+from oracle import nd_plan as onp
+
+PLAN_FIELDS = ("perm", "fj0", "fnj", "fns", "level_ptr", "foff", "wtoff", "soff", "wbase",
+               "moff", "maps", "push_dst", "push_src", "push_ptr", "uoff2", "fac_size",
+               "slot_size", "xs_size", "nT", "d_top", "n_fronts")
+
+
+def same_plan(p, q):
+    """Every table of the native plan p equals the restatement's q."""
+    for f in PLAN_FIELDS:
+        assert np.array_equal(np.asarray(getattr(p, f)), np.asarray(getattr(q, f))), f
+    assert len(p.fwd) == len(q.fwd) and len(p.bwd) == len(q.bwd)
+    for (x, lx), (y, ly) in zip(p.fwd + p.bwd, q.fwd + q.bwd):
+        assert np.array_equal(x, y) and lx == ly
+    assert all(np.array_equal(a, b) for a, b in zip(p.S, q.S))
+    assert p.children == [[int(c) for c in cs] for cs in q.children]
+    assert all((a is None and b is None) or np.array_equal(a, b) for a, b in zip(p.cmap, q.cmap))
+    assert np.array_equal(p.Ap.indptr, q.Ap.indptr) and np.array_equal(p.Ap.indices, q.Ap.indices)
+    assert np.array_equal(p.Ap.data, q.Ap.data)
 
 
 def lap2d(nx, ny):
@@ -43,9 +62,10 @@ def random_spd_pieces(seed=0):
 def _check(A, leaf, top_max, tol=1e-11):
     plan = sc.NDPlan(A, leaf_size=leaf, top_max=top_max)
     plan.validate()
-    fac = sc.factorize(plan, plan.Ap.data, "cpu").numpy()
+    same_plan(plan, onp.NDPlan(A, leaf_size=leaf, top_max=top_max))
+    fac = onp.factorize_host(plan, plan.Ap.data)
     b = np.random.default_rng(1).standard_normal(A.shape[0])
-    x = sc.solve_host(plan, fac, b)
+    x = onp.solve_host(plan, fac, b)
     xr = spla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - xr) <= tol * np.linalg.norm(xr)
     return plan
@@ -110,13 +130,13 @@ def test_planted_mistake_is_caught():
     """A factor entry off by 1e-6 (relative) is visible to the check above."""
     A = lap2d(20, 20)
     plan = sc.NDPlan(A, leaf_size=8, top_max=30)
-    fac = sc.factorize(plan, plan.Ap.data, "cpu").numpy()
+    fac = onp.factorize_host(plan, plan.Ap.data)
     t = plan.n_fronts // 2
     nj, ns, foff = int(plan.fnj[t]), int(plan.fns[t]), int(plan.foff[t])
     assert ns > 0
     fac[foff + nj] *= 1 + 1e-6               # W[0, 0] (column-major stacked factor)
     b = np.random.default_rng(1).standard_normal(A.shape[0])
-    x = sc.solve_host(plan, fac, b)
+    x = onp.solve_host(plan, fac, b)
     xr = spla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - xr) > 1e-11 * np.linalg.norm(xr)
 
@@ -125,7 +145,7 @@ def test_not_positive_definite():
     A = lap2d(10, 10) - 3.0 * sps.identity(100)
     plan = sc.NDPlan(A.tocsr(), leaf_size=8, top_max=20)
     with pytest.raises(np.linalg.LinAlgError):
-        sc.factorize(plan, plan.Ap.data, "cpu")
+        onp.factorize_host(plan, plan.Ap.data)
 
 
 def test_front_capacity_is_enforced():
@@ -292,9 +312,9 @@ def test_plan_cache_same_pattern_new_values():
     p1, v1 = sc.plan_for(A1, 16, 64)
     p2, v2 = sc.plan_for(A2, 16, 64)
     assert p2 is p1 and not np.array_equal(v1, v2)
-    fac = sc.factorize(p2, v2, "cpu").numpy()
+    fac = onp.factorize_host(p2, v2)
     b = rng.standard_normal(A2.shape[0])
-    x = sc.solve_host(p2, fac, b)
+    x = onp.solve_host(p2, fac, b)
     xr = spla.spsolve(A2.tocsc(), b)
     assert np.linalg.norm(x - xr) <= 1e-11 * np.linalg.norm(xr)
     p3, _ = sc.plan_for(lap2d(31, 24), 16, 64)
@@ -320,3 +340,30 @@ def test_item_shapes_on_wide_fronts():
     assert set(np.unique(C)) <= {16, 32, 64}
     assert (C[ns_b > 256] == 16).all() and (ns_b > 256).any()
     assert (C[(ns_b > 128) & (ns_b <= 256)] <= 32).all()
+
+
+def test_native_plan_independent_of_threads(monkeypatch):
+    """The dissection runs its pieces on a thread pool (OMP_NUM_THREADS threads, at most
+    16); the plan is the same for any thread count, and equal to the restatement's."""
+    A = sps.block_diag([lap2d(150, 130), random_spd_pieces(4), lap2d(33, 1)]).tocsr()
+    ref = onp.NDPlan(A, leaf_size=16, top_max=200)
+    for t in ("1", "3", "8"):
+        monkeypatch.setenv("OMP_NUM_THREADS", t)
+        same_plan(sc.NDPlan(A, leaf_size=16, top_max=200), ref)
+
+
+def test_native_plan_rejects_bad_patterns():
+    A = lap2d(5, 5)
+    indptr = A.indptr.astype(np.int64)
+    bad = A.indices.astype(np.int32).copy()
+    bad[[0, 1]] = bad[[1, 0]]                      # unsorted row
+    import ctypes
+    h = ctypes.c_void_p()
+    st = _lib._lib.dds_nd_plan_build(25, indptr.ctypes.data, bad.ctypes.data, 4, 8, ctypes.byref(h))
+    assert st == 1 and b"sorted" in _lib._lib.dds_last_error() and not h.value
+    st = _lib._lib.dds_nd_plan_build(25, indptr.ctypes.data, A.indices.astype(np.int32).ctypes.data,
+                                     0, 8, ctypes.byref(h))
+    assert st == 1 and b"bad arguments" in _lib._lib.dds_last_error()
+    assert _lib._lib.dds_nd_plan_field(None, 1, None) == -1
+    with pytest.raises(ValueError, match="top_max"):
+        sc.SparseCholesky(top_max=sc.TOP_CAP + 1)
