@@ -222,7 +222,7 @@ __global__ __launch_bounds__(M * 64) void finalize_m_kernel(int op, const double
 // Operators
 // ---------------------------------------------------------------------------------------
 
-// SELL-64 times 16 vectors; optional per-column partials of x . y. One wavefront per
+// SELL-64 (slot pairs, as dds_sellmv) times 16 vectors; optional per-column partials of x . y. One wavefront per
 // slice: lane l owns column pair l % 8 of the rows 8u + l / 8 (u < 8), so each gather of
 // x reads 8 whole 128-byte rows per instruction.
 constexpr int kSellWaves = 16;  // wavefronts per SpMM workgroup
@@ -250,13 +250,17 @@ __global__ __launch_bounds__(64 * kSellWaves) void sell_m_kernel(int64_t n_rows,
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc[u] = double2{0.0, 0.0};
     for (int j = 0; j < w; ++j) {
-      const double* vj = val + s0 + 64 * j;
-      const int32_t* cj = col + s0 + 64 * j;
+      // slot pairs: entry (j, r) at 128 (j / 2) + 2 r + j % 2; an odd last slot at 64 j + r
+      const bool paired = j < (w & ~1);
+      const int64_t o = paired ? s0 + 128 * (j >> 1) + (j & 1) : s0 + 64 * j;
+      const int sr = paired ? 2 : 1;
+      const double* vj = val + o;
+      const int32_t* cj = col + o;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int r = 8 * u + rg;
-        const double v = vj[r];
-        const double2 xv = ld2(x + (int64_t)cj[r] * M + 2 * cp);
+        const double v = vj[sr * r];
+        const double2 xv = ld2(x + (int64_t)cj[sr * r] * M + 2 * cp);
         acc[u].x += v * xv.x;
         acc[u].y += v * xv.y;
       }

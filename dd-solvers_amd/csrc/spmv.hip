@@ -273,51 +273,73 @@ __global__ __launch_bounds__(kBlock) void bsr_seg_kernel(int64_t n_block_rows,
 }
 
 // ---------------------------------------------------------------------------------------
-// Sliced ELLPACK with 64-row slices (SELL-64): slice q holds rows 64q .. 64q+63 as
-// width_q columns stored column by column (entry j of row 64q + l at
-// slice_ptr[q] + 64*j + l), padded with (col = own row, value 0). One wavefront per slice,
-// lane = row: every value and index load is one coalesced 512 B / 256 B access, no row
-// pointers are read, and the padding costs nothing but its bytes.
+// Sliced ELLPACK with 64-row slices (SELL-64), slots stored in pairs: slice q holds rows
+// 64q .. 64q+63 as w_q slots (entry j of each row), padded with (col = own row, value 0).
+// Slots 2p and 2p+1 of lane l are adjacent, at slice_ptr[q] + 128p + 2l (+1); an odd last
+// slot j is stored alone at slice_ptr[q] + 64j + l. One wavefront per slice, lane = row:
+// a slot pair is one coalesced 16-byte load per lane (1 KiB per wavefront for FP64) and one
+// 4-byte load of its two compact column offsets, so the loads in flight carry twice the
+// bytes of one-slot loads; no row pointers are read, the padding costs only its bytes, and
+// each row still sums its entries in slot order.
+// Compact column indices: a slice whose slots each span < 65536 columns stores 16-bit
+// offsets from a per-slot base (2 bytes per entry instead of 4); other slices keep int32
+// columns, in the same pair order. Contiguous runs of slices per wave, contiguous runs of
+// waves per XCD. Two wavefronts per workgroup: a workgroup retires when its slowest wave
+// does, and short workgroups pack the CUs more evenly.
 // ---------------------------------------------------------------------------------------
-// SELL-64 with compact column indices: one wavefront per slice (contiguous runs of slices
-// per wave, contiguous runs of waves per XCD). A slice whose slots each span < 65536
-// columns stores 16-bit offsets from a per-slot base (2 bytes per entry instead of 4).
-// Two wavefronts per workgroup: a workgroup retires when its slowest wave does, and short
-// workgroups pack the CUs more evenly (measured at 128^3: 4 waves 116 us, 2 waves 112 us).
 constexpr int kSellWaves = 2;
 
-template <typename T, typename ColF>
-__device__ __forceinline__ double sell_slice(const T* __restrict__ v, int w, ColF col_of,
-                                             const T* __restrict__ x) {
+template <typename T>
+struct Pair;
+template <>
+struct Pair<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+typedef int32_t i32x2_t __attribute__((ext_vector_type(2)));
+
+// Row sum of one lane over its w slots; pair(p) gives the two column indices of slot pair
+// p, single(j) the column of the odd last slot.
+template <typename T, typename PairF, typename SingleF>
+__device__ __forceinline__ double sell_slice(const T* __restrict__ v, int lane, int w, PairF pair,
+                                             SingleF single, const T* __restrict__ x) {
+  using T2 = typename Pair<T>::type;
+  const T2* vp = reinterpret_cast<const T2*>(v) + lane;
+  const int np = w >> 1;
   double acc = 0.0;
-  int j = 0;
-  for (; j + 7 < w; j += 8) {
-    int32_t cc[8];
-    T vv[8];
+  int p = 0;
+  for (; p + 3 < np; p += 4) {
+    i32x2_t cc[4];
+    T2 vv[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) cc[u] = col_of(j + u);
+    for (int u = 0; u < 4; ++u) cc[u] = pair(p + u);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) vv[u] = load_nt(v + 64 * (j + u));
+    for (int u = 0; u < 4; ++u) vv[u] = __builtin_nontemporal_load(vp + 64 * (p + u));
     T xx[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) xx[u] = x[cc[u]];
+    for (int u = 0; u < 4; ++u) {
+      xx[2 * u] = x[cc[u].x];
+      xx[2 * u + 1] = x[cc[u].y];
+    }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc += static_cast<double>(vv[u]) * static_cast<double>(xx[u]);
+    for (int u = 0; u < 4; ++u) {
+      acc += static_cast<double>(vv[u].x) * static_cast<double>(xx[2 * u]);
+      acc += static_cast<double>(vv[u].y) * static_cast<double>(xx[2 * u + 1]);
+    }
   }
-  for (; j + 3 < w; j += 4) {
-    int32_t cc[4];
-    T vv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) cc[u] = col_of(j + u);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) vv[u] = load_nt(v + 64 * (j + u));
-    T xx[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) xx[u] = x[cc[u]];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) acc += static_cast<double>(vv[u]) * static_cast<double>(xx[u]);
+  for (; p < np; ++p) {
+    const i32x2_t cc = pair(p);
+    const T2 vv = __builtin_nontemporal_load(vp + 64 * p);
+    acc += static_cast<double>(vv.x) * static_cast<double>(x[cc.x]);
+    acc += static_cast<double>(vv.y) * static_cast<double>(x[cc.y]);
   }
-  for (; j < w; ++j) acc += static_cast<double>(load_nt(v + 64 * j)) * static_cast<double>(x[col_of(j)]);
+  if (w & 1) {
+    const int j = w - 1;
+    acc += static_cast<double>(load_nt(v + 64 * j + lane)) * static_cast<double>(x[single(j)]);
+  }
   return acc;
 }
 
@@ -344,13 +366,24 @@ __global__ __launch_bounds__(64 * kSellWaves) void sell_kernel(
     const int64_t cp = col_ptr[q];
     double acc;
     if (cp >= 0) {
-      const uint16_t* d = col16 + cp + lane;
+      // 16-bit offsets: slot pair p of this lane is one 32-bit word (cp is a multiple of 64)
+      const uint32_t* d2 = reinterpret_cast<const uint32_t*>(col16 + cp) + lane;
+      const uint16_t* d1 = col16 + cp + lane;
       const int32_t* b = slot_base + (s0 >> 6);
-      acc = sell_slice<T>(val + s0 + lane, w,
-                          [&](int j) { return b[j] + (int32_t)load_nt(d + 64 * j); }, x);
+      acc = sell_slice<T>(
+          val + s0, lane, w,
+          [&](int p) {
+            const uint32_t o = load_nt(d2 + 64 * p);
+            return i32x2_t{b[2 * p] + (int32_t)(uint16_t)(o & 0xffffu),
+                           b[2 * p + 1] + (int32_t)(uint16_t)(o >> 16)};
+          },
+          [&](int j) { return b[j] + (int32_t)load_nt(d1 + 64 * j); }, x);
     } else {
-      const int32_t* c = col32 + (~cp) + lane;
-      acc = sell_slice<T>(val + s0 + lane, w, [&](int j) { return load_nt(c + 64 * j); }, x);
+      const int32_t* c = col32 + (~cp);
+      const i32x2_t* c2 = reinterpret_cast<const i32x2_t*>(c) + lane;
+      acc = sell_slice<T>(
+          val + s0, lane, w, [&](int p) { return __builtin_nontemporal_load(c2 + 64 * p); },
+          [&](int j) { return load_nt(c + 64 * j + lane); }, x);
     }
     const int64_t row = q * 64 + lane;
     if (row < n_rows) {

@@ -277,9 +277,9 @@ class _CSROperator:
 
 class _SELLOperator:
     """y = A x in SELL-64 (dds_sellmv), built from a CSR matrix at setup: rows in slices
-    of 64, each slice padded to its longest row and stored column by column, so a
-    wavefront reads every slot of its 64 rows with coalesced loads. CG(bsr_matmul=False)
-    uses it for square matrices."""
+    of 64, each slice padded to its longest row and stored slot by slot, two slots
+    interleaved per pair, so a wavefront reads a slot pair of its 64 rows with one
+    coalesced 16-byte load per lane. CG(bsr_matmul=False) uses it for square matrices."""
 
     def __init__(self, matrix: torch.Tensor, lanes: int | None = None):
         if matrix.layout is not torch.sparse_csr:
@@ -315,6 +315,38 @@ class _SELLOperator:
         self.nnz = int(vals.shape[0])
         self.stored = total
         self._compact_columns(width)
+        self._pair_slots(width)
+
+    def _pair_slots(self, width):
+        """Reorder every stored array from slot order (entry (j, l) of a slice at 64 j + l)
+        into slot pairs: (j, l) at 128 (j // 2) + 2 l + j % 2 for j < 2 (w // 2), an odd
+        last slot unchanged, so a lane loads its two entries of a slot pair at once."""
+        dev = self.device
+        sp = self.slice_ptr
+        q = torch.repeat_interleave(torch.arange(width.shape[0], device=dev), 64 * width)
+        e = torch.arange(self.stored, device=dev) - sp[q]
+        j, l = e // 64, e % 64
+        paired = j < 2 * (width[q] // 2)
+        new = torch.where(paired, 128 * (j // 2) + 2 * l + j % 2, e)
+
+        def move(a, base):
+            out = torch.empty_like(a)
+            out[base + new] = a[base + e]
+            return out
+
+        self.values = move(self.values, sp[q])
+        self.col = move(self.col, sp[q])
+        cp = self.col_ptr[q]
+        n16 = cp >= 0
+        if self.col16.numel():
+            c16 = torch.empty_like(self.col16)
+            c16[cp[n16] + new[n16]] = self.col16[cp[n16] + e[n16]]
+            self.col16 = c16
+        if self.col32.numel():
+            c32 = torch.empty_like(self.col32)
+            o = -1 - cp[~n16]
+            c32[o + new[~n16]] = self.col32[o + e[~n16]]
+            self.col32 = c32
 
     def _compact_columns(self, width):
         """Per slot (64 entries) an int32 base and 16-bit offsets when the slot's columns

@@ -573,6 +573,54 @@ def test_sell_compact_column_format():
     assert d16.shape[0] == 64 * (n_slots - 4) and c32.shape[0] == 64 * 4
 
 
+def test_sell_slot_pair_order():
+    """_pair_slots moves every stored array into slot pairs; read back with the kernels'
+    addressing (slot pair p of lane l at 128 p + 2 l (+1), an odd last slot j at 64 j + l;
+    a lane's two 16-bit offsets as one little-endian word) it gives each slot's values and
+    columns, for even and odd widths and for compact and wide slices."""
+    import torch
+    from dd_solvers.solvers import _SELLOperator
+    rng = np.random.default_rng(1)
+    width = torch.tensor([3, 1, 4, 2, 5, 6])
+    n_slots = int(width.sum())
+    col = rng.integers(0, 60000, size=(n_slots, 64))
+    col[5, 7] = 900_000        # slice 2 goes wide (int32 columns)
+    val = rng.standard_normal((n_slots, 64))
+    op = object.__new__(_SELLOperator)
+    op.col = torch.as_tensor(col.reshape(-1), dtype=torch.int32)
+    op.values = torch.as_tensor(val.reshape(-1))
+    op.device = torch.device("cpu")
+    op.slice_ptr = torch.as_tensor(np.concatenate([[0], np.cumsum(64 * width.numpy())]))
+    op.stored = 64 * n_slots
+    op._compact_columns(width)
+    op._pair_slots(width)
+    cp = op.col_ptr.numpy()
+    words = op.col16.numpy().view(np.uint32) if op.col16.numel() else None
+    d16 = op.col16.numpy().view(np.uint16).astype(np.int64)
+    c32, base = op.col32.numpy(), op.slot_base.numpy()
+    v, call = op.values.numpy(), op.col.numpy()
+    slot0 = np.concatenate([[0], np.cumsum(width.numpy())])
+    lanes = np.arange(64)
+    for q in range(width.shape[0]):
+        w, s0 = int(width[q]), int(op.slice_ptr[q])
+        for j in range(w):
+            slot = slot0[q] + j
+            paired = j < 2 * (w // 2)
+            pos = 128 * (j // 2) + 2 * lanes + j % 2 if paired else 64 * j + lanes
+            assert np.array_equal(v[s0 + pos], val[slot]), (q, j)
+            assert np.array_equal(call[s0 + pos], col[slot]), (q, j)
+            if cp[q] >= 0:
+                if paired:
+                    wd = words[(cp[q] + 128 * (j // 2)) // 2 + lanes].astype(np.int64)
+                    off = (wd >> 16) if j % 2 else (wd & 0xFFFF)
+                else:
+                    off = d16[cp[q] + pos]
+                got = base[slot] + off
+            else:
+                got = c32[~cp[q] + pos]
+            assert np.array_equal(got, col[slot]), (q, j)
+
+
 def _symm_schedule(k, G, q, NS, NT):
     """Restates the control flow of sym_mfma_apply_kernel for one item (subdomain of k
     unknowns, pairs q G .. q G + G - 1, tile rows p and T-1-p of each in increasing order)
