@@ -7,6 +7,19 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// two-element vectors for 8- and 16-byte loads
+template <typename T>
+struct Pair;
+template <>
+struct Pair<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+typedef int32_t i32x2_t __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ void write_partial(double dot, double* partials, double* scratch) {
   const double t = block_sum(dot, scratch);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -228,6 +241,147 @@ __global__ __launch_bounds__(kBlock) void bsr_blk_kernel(int64_t n_block_rows,
 }
 
 // ---------------------------------------------------------------------------------------
+// BSR, runs of block rows streamed by a wavefront: the values of NR consecutive block rows
+// are one contiguous run (column-major blocks: element e of the run is in block e / B^2,
+// column (e mod B^2) / B, row e mod B). G lanes read it V elements at a time (one 8- or
+// 16-byte load) at V l + S t, S = V G, t = 0, 1, ...; S is a multiple of B, so a lane's
+// elements always lie in the rows (V l + k) mod B and it keeps V running sums. Its elements
+// come in increasing order, so the block row they belong to only moves forward: when it
+// changes the lane parks its sums in LDS (slot [block row][V l + k], zero if the lane had
+// nothing there) and starts again. The x entries the run multiplies (B per block) are
+// gathered into LDS first, so the stream issues one global load per step instead of a
+// value load plus a dependent gather; a run with more than NR * MAXB blocks reads x from
+// global memory instead. U steps are loaded together. Then row r of block row j is the sum
+// of the slots [j][q], q = r, r + B, ... < S, in that order. Each wavefront takes a
+// contiguous range of runs, contiguous per XCD; no workgroup barrier (wavefronts run
+// different counts).
+// ---------------------------------------------------------------------------------------
+constexpr int kRunWaves = 4;
+
+template <typename T, int V>
+struct VecLoad;
+template <typename T>
+struct VecLoad<T, 1> {
+  __device__ static __forceinline__ void load(const T* p, T (&v)[1]) { v[0] = load_nt(p); }
+};
+template <typename T>
+struct VecLoad<T, 2> {
+  __device__ static __forceinline__ void load(const T* p, T (&v)[2]) {
+    const typename Pair<T>::type t =
+        __builtin_nontemporal_load(reinterpret_cast<const typename Pair<T>::type*>(p));
+    v[0] = t.x;
+    v[1] = t.y;
+  }
+};
+
+// LDS ordering among the lanes of one wavefront
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, typename I, int B, int V, int G, int NR, int U, int MAXB>
+__global__ __launch_bounds__(64 * kRunWaves) void bsr_run_kernel(
+    int64_t n_block_rows, const I* __restrict__ crow, const int32_t* __restrict__ col,
+    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+    double* __restrict__ partials, const dds_pcg_state* __restrict__ gate) {
+  constexpr int BB = B * B;
+  constexpr int S = V * G;
+  static_assert(G <= 64 && NR <= 64 && S % B == 0 && (V == 1 || (V == 2 && B % 2 == 0)),
+                "bad BSR run tiling");
+  __shared__ double red[kRunWaves][NR * S];
+  __shared__ int32_t bst[kRunWaves][NR + 1];
+  __shared__ T xsh[kRunWaves][NR * MAXB * B];
+  __shared__ double scratch[kRunWaves];
+  if (gate != nullptr && gate->done) return;
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t n_runs = (n_block_rows + NR - 1) / NR;
+  const int64_t wave = xcd_block_id() * kRunWaves + wid;
+  const int64_t n_waves = (int64_t)gridDim.x * kRunWaves;
+  const int64_t per = (n_runs + n_waves - 1) / n_waves;
+  const int64_t q0 = wave * per;
+  const int64_t q1 = q0 + per < n_runs ? q0 + per : n_runs;
+  double* R = red[wid];
+  int32_t* bs = bst[wid];
+  double dot = 0.0;
+  for (int64_t q = q0; q < q1; ++q) {
+    const int64_t br0 = q * NR;
+    const int nr = (int)(n_block_rows - br0 < NR ? n_block_rows - br0 : NR);
+    const int64_t b0 = (int64_t)crow[br0];
+    if (lane <= nr) bs[lane] = (int32_t)(((int64_t)crow[br0 + lane] - b0) * BB);
+    for (int i = lane; i < NR * S; i += 64) R[i] = 0.0;
+    wave_sync();
+    const int len = bs[nr];
+    const int nbk = len / BB;
+    const bool staged = nbk <= NR * MAXB;
+    T* X = xsh[wid];
+    if (staged) {
+      for (int i = lane; i < nbk * B; i += 64) {
+        const int kb = i / B, c = i - kb * B;
+        X[i] = x[(int64_t)col[b0 + kb] * B + c];
+      }
+    }
+    wave_sync();
+    if (lane < G) {
+      const T* seg = val + b0 * BB;
+      const int32_t* cb = col + b0;
+      int cur = 0;
+      int nxt = bs[1];
+      double acc[V];
+#pragma unroll
+      for (int k = 0; k < V; ++k) acc[k] = 0.0;
+      for (int e0 = V * lane; e0 < len; e0 += U * S) {
+        T vv[U][V];
+        T xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = e0 + u * S;
+          if (e < len) {
+            const int blk = e / BB;
+            const int c = (e - blk * BB) / B;
+            VecLoad<T, V>::load(seg + e, vv[u]);
+            xv[u] = staged ? X[blk * B + c] : x[(int64_t)cb[blk] * B + c];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = e0 + u * S;
+          if (e < len) {
+            while (e >= nxt) {  // the next element is in a later block row: park the sums
+#pragma unroll
+              for (int k = 0; k < V; ++k) {
+                R[cur * S + V * lane + k] = acc[k];
+                acc[k] = 0.0;
+              }
+              nxt = bs[++cur + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < V; ++k)
+              acc[k] += static_cast<double>(vv[u][k]) * static_cast<double>(xv[u]);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < V; ++k) R[cur * S + V * lane + k] = acc[k];
+    }
+    wave_sync();
+    for (int o = lane; o < nr * B; o += 64) {
+      const int j = o / B, r = o - j * B;
+      const double* Rj = R + j * S;
+      double sum = 0.0;
+#pragma unroll
+      for (int i = 0; i < S / B; ++i) sum += Rj[r + i * B];
+      const int64_t row = (br0 + j) * B + r;
+      y[row] = static_cast<T>(sum);
+      if (partials != nullptr) dot += sum * static_cast<double>(x[row]);
+    }
+    wave_sync();
+  }
+  if (partials != nullptr) write_partial(dot, partials, scratch);
+}
+
+// ---------------------------------------------------------------------------------------
 // BSR with a power-of-two block size B (used for b = 2): G lanes stream the block row's
 // contiguous value segment (nnzb_row * B * B elements), so consecutive lanes read
 // consecutive values. With
@@ -289,17 +443,6 @@ __global__ __launch_bounds__(kBlock) void bsr_seg_kernel(int64_t n_block_rows,
 // ---------------------------------------------------------------------------------------
 constexpr int kSellWaves = 2;
 
-template <typename T>
-struct Pair;
-template <>
-struct Pair<double> {
-  typedef double type __attribute__((ext_vector_type(2)));
-};
-template <>
-struct Pair<float> {
-  typedef float type __attribute__((ext_vector_type(2)));
-};
-typedef int32_t i32x2_t __attribute__((ext_vector_type(2)));
 
 // Row sum of one lane over its w slots; pair(p) gives the two column indices of slot pair
 // p, single(j) the column of the odd last slot.
@@ -471,23 +614,33 @@ int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, con
     hipLaunchKernelGGL((bsr_rowb_kernel<T, I, B>), dim3((unsigned)grid), dim3(kBlock), 0, \
                        s, nbr, cr, col, v, xv, yv, partials, gate);                      \
   } while (0)
-  // measured on the DG shapes of the reference's BSR benchmark (2D and 3D) and on the Q1
-  // 64^3 operator at b = 2, 4, 8: the value segment streamed for b = 2, one lane per block
-  // for b = 3 .. 7, one lane per row with the block size compiled in for b = 8 and the
-  // larger DG sizes
-  switch (b) {
-    case 2: DDS_SEG(2, 16); break;
-    case 3: DDS_BLK(3, 4); break;
-    case 4: DDS_BLK(4, 8); break;
-    case 5: DDS_BLK(5, 4); break;
-    case 6: DDS_BLK(6, 8); break;
-    case 7: DDS_BLK(7, 8); break;
-    case 8: DDS_ROWB(8); break;
-    case 10: DDS_ROWB(10); break;
-    case 20: DDS_ROWB(20); break;
-    case 15: DDS_ROWB(15); break;
-    case 21: DDS_ROWB(21); break;
-    case 28: DDS_ROWB(28); break;
+#define DDS_RUN(B, V, G, NR, U, MAXB)                                                      \
+  do {                                                                                     \
+    grid = std::min<int64_t>(capped ? kMaxPartials : (int64_t)1 << 20,                     \
+                             ceil_div(ceil_div(ceil_div(nbr, NR), kRunWaves), kXcds) *      \
+                                 kXcds);                                                   \
+    hipLaunchKernelGGL((bsr_run_kernel<T, I, B, V, G, NR, U, MAXB>), dim3((unsigned)grid), \
+                       dim3(64 * kRunWaves), 0, s, nbr, cr, col, v, xv, yv, partials, gate); \
+  } while (0)
+  // measured on the DG shapes of the reference's BSR benchmark (2D and 3D): runs of block
+  // rows streamed (x staged in LDS) for the DG sizes from 6 up, one lane per block for b =
+  // 3 .. 7, the value segment of a block row streamed for b = 2, one lane per row with the
+  // block size compiled in for b = 8 (the Q1 block size sweep)
+  // two-element loads need values aligned to two elements (always so for a fresh tensor)
+  const bool pairs_ok = (reinterpret_cast<uintptr_t>(values) % (2 * sizeof(T))) == 0;
+  switch (pairs_ok ? b : -b) {
+    case 2: case -2: DDS_SEG(2, 16); break;
+    case 3: case -3: DDS_BLK(3, 4); break;
+    case 4: case -4: DDS_BLK(4, 8); break;
+    case 5: case -5: DDS_BLK(5, 4); break;
+    case 6: DDS_RUN(6, 2, 63, 4, 3, 6); break;
+    case 7: case -7: DDS_BLK(7, 8); break;
+    case 8: case -8: DDS_ROWB(8); break;
+    case 10: DDS_RUN(10, 2, 60, 4, 4, 6); break;
+    case 15: case -15: DDS_RUN(15, 1, 60, 2, 8, 6); break;
+    case 20: DDS_RUN(20, 2, 60, 1, 6, 6); break;
+    case 21: case -21: DDS_RUN(21, 1, 63, 2, 4, 6); break;
+    case 28: DDS_RUN(28, 2, 56, 1, 7, 6); break;
     default:
       grid = grid_for(nbr * b, capped);
       hipLaunchKernelGGL((bsr_row_kernel<T, I>), dim3((unsigned)grid), dim3(kBlock), 0, s, nbr,
@@ -496,6 +649,7 @@ int launch_bsr(int64_t nbr, int64_t b, const void* crow, const int32_t* col, con
 #undef DDS_SEG
 #undef DDS_BLK
 #undef DDS_ROWB
+#undef DDS_RUN
   if (n_partials) *n_partials = partials ? (int)grid : 0;
   DDS_LAUNCHED();
   return DDS_OK;

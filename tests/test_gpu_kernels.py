@@ -93,7 +93,56 @@ def test_bsr_fixture(dtype):
     assert np.allclose(y.cpu().numpy(), ref, atol=1e-12 if dtype == torch.float64 else 1e-5)
 
 
-@pytest.mark.parametrize("b", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 15, 21, 28])
+@pytest.mark.parametrize("b", [2, 3, 4, 6, 8, 10, 15, 20, 21, 28])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_bsr_ragged_rows(b, dtype):
+    """Block rows of 0 to 9 blocks (empty rows included) through every streaming shape:
+    the tail steps of a lane group and groups sharing a wavefront; with x . y partials."""
+    rng = np.random.default_rng(100 + b)
+    nbr = 3001
+    cnt = rng.integers(0, 10, size=nbr)
+    cnt[::97] = 0
+    crow = np.concatenate([[0], np.cumsum(cnt)])
+    col = rng.integers(0, nbr, size=int(crow[-1]))
+    vals = rng.standard_normal((int(crow[-1]), b, b))
+    M = _bsr_tensor(crow, col, vals, dtype)
+    op = FastBSR(M)
+    x = torch.as_tensor(rng.standard_normal(nbr * b), dtype=dtype, device=dev)
+    y = torch.empty_like(x)
+    partials = torch.zeros(_lib.MAX_PARTIALS, dtype=torch.float64, device=dev)
+    n = op._enqueue_matvec(x, y, partials, None)
+    ref = K.bsrmv(crow, col, vals.astype(np.float32 if dtype == torch.float32 else np.float64),
+                  x.cpu().numpy().astype(np.float64))
+    tol = 1e-12 if dtype == torch.float64 else 1e-4
+    assert np.allclose(y.cpu().numpy(), ref, atol=tol * max(1.0, np.abs(ref).max()))
+    assert np.isclose(partials[:n].sum().item(), float(ref @ x.cpu().numpy().astype(np.float64)),
+                      rtol=1e-10 if dtype == torch.float64 else 1e-4)
+
+
+@pytest.mark.parametrize("b", [4, 6, 10, 20, 28])
+def test_bsr_values_off_the_pair_alignment(b):
+    """Values starting one element past a 16-byte boundary (a caller's view into a larger
+    buffer) take the one-element kernels, with the same result."""
+    rng = np.random.default_rng(b)
+    nbr = 500
+    crow = np.arange(0, 4 * nbr + 1, 4)
+    col = rng.integers(0, nbr, size=4 * nbr)
+    vals = rng.standard_normal((4 * nbr, b, b))
+    buf = torch.empty(vals.size + 1, dtype=torch.float64, device=dev)
+    buf[1:] = torch.as_tensor(np.ascontiguousarray(vals.transpose(0, 2, 1)).reshape(-1), device=dev)
+    x = torch.as_tensor(rng.standard_normal(nbr * b), device=dev)
+    y = torch.empty_like(x)
+    crow_t = torch.as_tensor(crow, dtype=torch.int64, device=dev)
+    col_t = torch.as_tensor(col, dtype=torch.int32, device=dev)
+    assert buf[1:].data_ptr() % 16 == 8
+    _lib.call_partials("dds_bsrmv", _lib.F64, _lib.I64, nbr, b, crow_t.data_ptr(), col_t.data_ptr(),
+                       buf[1:].data_ptr(), x.data_ptr(), y.data_ptr(), None, _lib.PARTIALS_OUT,
+                       None, _lib.stream())
+    ref = K.bsrmv(crow, col, vals, x.cpu().numpy())
+    assert np.allclose(y.cpu().numpy(), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("b", [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 15, 20, 21, 28])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_bsr_random(b, dtype):
     # test_bsr.py:161-193: 1000 block rows, 3 random blocks per row (duplicates allowed)
