@@ -92,7 +92,8 @@ def make_block_solver(pb, storage, dev, kind="hybrid", use_graph=True, autotune=
     torch.cuda.synchronize()
     tn = getattr(s.preconditioner, "tuning", None)
     log(f"setup block storage={storage}: {time.perf_counter() - t:.1f}s"
-        + (f", tuned grid={tn['grid']} waves={tn['waves']} ({tn['ms'] * 1e3:.1f} us)" if tn else ""))
+        + (f", tuned grid={tn['grid']} waves={tn['waves']} rpw={tn['rpw']} ({tn['ms'] * 1e3:.1f} us)"
+           if tn else ""))
     return s
 
 
@@ -111,22 +112,20 @@ def mfma_kernel_stats(M):
     s = _lib.stream()
 
     def launch():
-        _lib.call("dds_mfma_apply", dt, L.n_items, L.items.data_ptr(), M.sub_ptr.data_ptr(),
-                  L.mat_off.data_ptr(), mf["b_off"].data_ptr(), _lib.ptr(L.scale),
-                  mf["sigma"].data_ptr(), L.mats.data_ptr(), mf["bp"].data_ptr(),
-                  mf["Ym"].data_ptr(), int(mf["grid"]), int(mf["waves"]), None, s)
+        M._enqueue_mfma_apply(mf["Ym"], None)
 
     ms = kernel_time_ms(launch)
     nt = 2 if L.mat_dtype == torch.float16 else 3
     esz = L.mats.element_size()
     alg = int(L.bytes_algorithmic + mf["bp"].numel() * esz + sizes.sum() * 16 * 8)
     flops = float(2 * (sizes ** 2).sum() * 16 * nt)
-    return {"kernel": f"mfma_apply_kernel<{'_Float16' if nt == 2 else 'bf16_t'}>",
+    kname = "mfma_sub_kernel" if int(mf["grid"]) < 0 else "mfma_apply_kernel"
+    return {"kernel": f"{kname}<{'_Float16' if nt == 2 else 'bf16_t'}>",
             "avg_launch_us": round(ms * 1e3, 2), "alg_bytes_per_launch": alg,
             "GBs": round(alg / (ms * 1e-3) / 1e9, 1),
             "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "mfma_TFLOPs": round(flops / (ms * 1e-3) / 1e12, 1),
-            "grid": int(mf["grid"]), "waves": int(mf["waves"])}
+            "grid": int(mf["grid"]), "waves": int(mf["waves"]), "rpw": int(mf["rpw"])}
 
 
 def batched_extra(pb, storage, dev, steps, warmup, seed, kind="hybrid", fp64_tts=None):

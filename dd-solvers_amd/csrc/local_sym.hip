@@ -554,10 +554,12 @@ __device__ __forceinline__ void symm_issue_at(uint32_t st, SymmCursor c, int nro
   symm_issue(st, real, real ? c.I : 0, real ? c.J : 0, T, rL, nbL, rm, voff);
 }
 
+// LDS stages of the tile pipeline (8 KiB each): NS - 1 tiles in flight per wavefront. Two
+// stages leave LDS for six workgroups per CU at max_k = 1000 (25 KiB each); measured at Q1
+// 128^3 FP16: 2 stages 660.6 us, 3 stages 676.5 us (four workgroups per CU), 4 stages 728 us.
 #ifndef DDS_SYMM_STAGES
-#define DDS_SYMM_STAGES 3
+#define DDS_SYMM_STAGES 2
 #endif
-// LDS stages of the tile pipeline (8 KiB each): NS - 1 tiles in flight per wavefront.
 constexpr int kSymmStages = DDS_SYMM_STAGES;
 static_assert(kSymmStages >= 2 && 8 * (kSymmStages - 1) <= 63, "vmcnt range");
 

@@ -16,8 +16,6 @@ namespace {
 constexpr int M = 16;  // right-hand sides per block
 constexpr int kBlock = 256;
 
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bool all_done(const dds_pcg_state* st) {
@@ -494,27 +492,104 @@ __global__ __launch_bounds__(kPackThreads) void pack_x_kernel(const int64_t* __r
   }
 }
 
+// One v_mfma_f32_16x16x32 per 16-byte A and B operand: k runs over the lane group's 8
+// elements. Both operands hold the same columns (32K2 + 4(l/16) + i, then 16 on, i < 4) in
+// the same order, so the product sums over the right pairs; the two 16x16x16 instructions
+// this replaces summed the same 32 products in two steps.
 template <typename TM>
-__device__ __forceinline__ float4_t mfma16(const TM* a, const TM* b, float4_t c);
+__device__ __forceinline__ float4_t mfma32(const Vec16<TM>& a, const Vec16<TM>& b, float4_t c);
 template <>
-__device__ __forceinline__ float4_t mfma16<_Float16>(const _Float16* a, const _Float16* b, float4_t c) {
-  half4_t av, bv;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    av[i] = a[i];
-    bv[i] = b[i];
-  }
-  return __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, c, 0, 0, 0);
+__device__ __forceinline__ float4_t mfma32<_Float16>(const Vec16<_Float16>& a, const Vec16<_Float16>& b,
+                                                     float4_t c) {
+  typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+  h8_t av, bv;
+  __builtin_memcpy(&av, &a, 16);
+  __builtin_memcpy(&bv, &b, 16);
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, c, 0, 0, 0);
 }
 template <>
-__device__ __forceinline__ float4_t mfma16<bf16_t>(const bf16_t* a, const bf16_t* b, float4_t c) {
-  bf16x4_t av, bv;
+__device__ __forceinline__ float4_t mfma32<bf16_t>(const Vec16<bf16_t>& a, const Vec16<bf16_t>& b,
+                                                   float4_t c) {
+  typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+  b8_t av, bv;
+  __builtin_memcpy(&av, &a, 16);
+  __builtin_memcpy(&bv, &b, 16);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+}
+
+// Row blocks R0 .. R0 + RPW - 1 of subdomain s (one wavefront): for each column pair K2 in
+// increasing order, the NT B operands (packed x terms, reused across the RPW row blocks) and
+// the RPW A operands are loaded, then one MFMA per (row block, term). Every row block's
+// accumulation is the same sequence of instructions whatever RPW and whichever wavefront,
+// so all launch shapes give the same bits.
+template <typename TM, int RPW>
+__device__ __forceinline__ void mfma_rows(int s, int R0, int lane, const int64_t* __restrict__ sub_ptr,
+                                          const int64_t* __restrict__ a_off,
+                                          const int64_t* __restrict__ b_off,
+                                          const float* __restrict__ scale,
+                                          const float* __restrict__ sigma, const TM* __restrict__ A,
+                                          const TM* __restrict__ bp, double* __restrict__ Y) {
+  using Tr = MfmaTraits<TM>;
+  constexpr int NT = Tr::NT;
+  const int64_t base = sub_ptr[s];
+  const int k = (int)(sub_ptr[s + 1] - base);
+  const int RB = (k + 15) / 16;
+  const int KP = (k + 31) / 32;
+  const int nrb = RB - R0 < RPW ? RB - R0 : RPW;
+  float4_t c[RPW][NT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    av[i] = (short)a[i].bits;
-    bv[i] = (short)b[i].bits;
+  for (int r = 0; r < RPW; ++r)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) c[r][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const TM* As = A + a_off[s];
+  const TM* Bs = bp + b_off[s];
+  if (nrb == RPW) {
+    // full group: all operand loads of a step are issued before its MFMAs (no branch
+    // between a load and its use), two steps per iteration
+#pragma unroll 2
+    for (int K2 = 0; K2 < KP; ++K2) {
+      Vec16<TM> bv[NT], av[RPW];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
+#pragma unroll
+      for (int r = 0; r < RPW; ++r)
+        av[r] = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
+#pragma unroll
+      for (int r = 0; r < RPW; ++r)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) c[r][t] = mfma32<TM>(av[r], bv[t], c[r][t]);
+    }
+  } else {
+    for (int K2 = 0; K2 < KP; ++K2) {
+      Vec16<TM> bv[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        if (r < nrb) {
+          const Vec16<TM> av = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) c[r][t] = mfma32<TM>(av, bv[t], c[r][t]);
+        }
+      }
+    }
   }
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, c, 0, 0, 0);
+  // D tile: lane l holds rows 4(l/16) + i, column l % 16
+  const int col = lane % 16;
+  const double sg = (double)sigma[(int64_t)s * M + col] * (scale != nullptr ? (double)scale[s] : 1.0);
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    if (r < nrb) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 16 * (R0 + r) + 4 * (lane / 16) + i;
+        double v = 0.0;
+#pragma unroll
+        for (int t = NT - 1; t >= 0; --t) v += (double)c[r][t][i];
+        if (row < k) Y[(base + row) * M + col] = v * sg;
+      }
+    }
+  }
 }
 
 // Persistent: work item i = (s, g) covers row blocks 4g .. 4g+3 of subdomain s; wave w of
@@ -526,82 +601,34 @@ __global__ void mfma_apply_kernel(int64_t n_items, const int32_t* __restrict__ i
                                   const float* __restrict__ sigma, const TM* __restrict__ A,
                                   const TM* __restrict__ bp, double* __restrict__ Y,
                                   const dds_pcg_state* __restrict__ st) {
-  using Tr = MfmaTraits<TM>;
-  constexpr int NT = Tr::NT;
   if (all_done(st)) return;
   const int lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
   const int64_t stride = (int64_t)gridDim.x * W;
-  for (int64_t it = (int64_t)blockIdx.x * W + (threadIdx.x >> 6); it < n_items; it += stride) {
-    const int s = items[2 * it];
-    const int g = items[2 * it + 1];
-    const int64_t base = sub_ptr[s];
-    const int k = (int)(sub_ptr[s + 1] - base);
-    const int RB = (k + 15) / 16;
-    const int KP = (k + 31) / 32;
-    const int R0 = 4 * g;
-    const int nrb = RB - R0 < 4 ? RB - R0 : 4;
-    float4_t c[4][NT];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int t = 0; t < NT; ++t) c[r][t] = float4_t{0.f, 0.f, 0.f, 0.f};
-    const TM* As = A + a_off[s];
-    const TM* Bs = bp + b_off[s];
-    if (nrb == 4) {
-      // full group: all operand loads of a step are issued before its MFMAs (no branch
-      // between a load and its use), two steps per iteration
-#pragma unroll 2
-      for (int K2 = 0; K2 < KP; ++K2) {
-        Vec16<TM> bv[NT], av[4];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          av[r] = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int t = 0; t < NT; ++t) {
-            c[r][t] = mfma16<TM>(av[r].v, bv[t].v, c[r][t]);
-            c[r][t] = mfma16<TM>(av[r].v + 4, bv[t].v + 4, c[r][t]);
-          }
-      }
-    } else {
-      for (int K2 = 0; K2 < KP; ++K2) {
-        Vec16<TM> bv[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (r < nrb) {
-            const Vec16<TM> av = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-              c[r][t] = mfma16<TM>(av.v, bv[t].v, c[r][t]);
-              c[r][t] = mfma16<TM>(av.v + 4, bv[t].v + 4, c[r][t]);
-            }
-          }
-        }
-      }
-    }
-    // D tile: lane l holds rows 4(l/16) + i, column l % 16
-    const int col = lane % 16;
-    const double sg = (double)sigma[(int64_t)s * M + col] * (scale != nullptr ? (double)scale[s] : 1.0);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (r < nrb) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int row = 16 * (R0 + r) + 4 * (lane / 16) + i;
-          double v = 0.0;
-#pragma unroll
-          for (int t = NT - 1; t >= 0; --t) v += (double)c[r][t][i];
-          if (row < k) Y[(base + row) * M + col] = v * sg;
-        }
-      }
-    }
-  }
+  for (int64_t it = (int64_t)blockIdx.x * W + (threadIdx.x >> 6); it < n_items; it += stride)
+    mfma_rows<TM, 4>(items[2 * it], 4 * items[2 * it + 1], lane, sub_ptr, a_off, b_off, scale,
+                     sigma, A, bp, Y);
+}
+
+// One workgroup per subdomain (order[b]: longest first): its W wavefronts take the row-block
+// groups g = w, w + W, ... of RPW row blocks each. The waves of a subdomain run on one CU
+// together, so its packed x (B operands) comes from HBM once and the other waves' reads of
+// it hit the XCD's L2, where the persistent item grid spreads a subdomain's groups over
+// XCDs that each fetch it again.
+template <typename TM, int RPW>
+__global__ void mfma_sub_kernel(const int32_t* __restrict__ order, const int64_t* __restrict__ sub_ptr,
+                                const int64_t* __restrict__ a_off, const int64_t* __restrict__ b_off,
+                                const float* __restrict__ scale, const float* __restrict__ sigma,
+                                const TM* __restrict__ A, const TM* __restrict__ bp,
+                                double* __restrict__ Y, const dds_pcg_state* __restrict__ st) {
+  if (all_done(st)) return;
+  const int lane = threadIdx.x & 63;
+  const int W = blockDim.x >> 6;
+  const int s = order[blockIdx.x];
+  const int k = (int)(sub_ptr[s + 1] - sub_ptr[s]);
+  const int NG = ((k + 15) / 16 + RPW - 1) / RPW;
+  for (int g = threadIdx.x >> 6; g < NG; g += W)
+    mfma_rows<TM, RPW>(s, RPW * g, lane, sub_ptr, a_off, b_off, scale, sigma, A, bp, Y);
 }
 
 }  // namespace
@@ -773,6 +800,41 @@ extern "C" int dds_mfma_apply(int mat_dtype, int64_t n_items, const int32_t* ite
     set_error("dds_mfma_apply: storage must be F16 or BF16 (got %d)", mat_dtype);
     return DDS_EUNSUPPORTED;
   }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_mfma_apply_sub(int mat_dtype, int64_t n_s, const int32_t* order,
+                                  const int64_t* sub_ptr, const int64_t* a_off,
+                                  const int64_t* b_off, const float* scale, const float* sigma,
+                                  const void* A, const void* bp, double* Y, int waves, int rpw,
+                                  const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n_s >= 0 && n_s <= INT32_MAX && order && sub_ptr && a_off && b_off && sigma && A &&
+                  bp && Y,
+              "dds_mfma_apply_sub: bad arguments");
+  DDS_REQUIRE(waves == 4 || waves == 8 || waves == 16,
+              "dds_mfma_apply_sub: waves must be 4, 8 or 16 (got %d)", waves);
+  DDS_REQUIRE(rpw == 2 || rpw == 4 || rpw == 8,
+              "dds_mfma_apply_sub: rpw must be 2, 4 or 8 (got %d)", rpw);
+  if (n_s == 0) return DDS_OK;
+  hipStream_t s = as_stream(stream);
+#define DDS_SUB(TM, R)                                                                     \
+  hipLaunchKernelGGL((mfma_sub_kernel<TM, R>), dim3((unsigned)n_s), dim3(64 * waves), 0, s, \
+                     order, sub_ptr, a_off, b_off, scale, sigma, static_cast<const TM*>(A),  \
+                     static_cast<const TM*>(bp), Y, st)
+  if (mat_dtype == DDS_F16) {
+    if (rpw == 2) DDS_SUB(_Float16, 2);
+    else if (rpw == 4) DDS_SUB(_Float16, 4);
+    else DDS_SUB(_Float16, 8);
+  } else if (mat_dtype == DDS_BF16) {
+    if (rpw == 2) DDS_SUB(bf16_t, 2);
+    else if (rpw == 4) DDS_SUB(bf16_t, 4);
+    else DDS_SUB(bf16_t, 8);
+  } else {
+    set_error("dds_mfma_apply_sub: storage must be F16 or BF16 (got %d)", mat_dtype);
+    return DDS_EUNSUPPORTED;
+  }
+#undef DDS_SUB
   DDS_LAUNCHED();
   return DDS_OK;
 }

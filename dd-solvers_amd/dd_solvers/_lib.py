@@ -109,6 +109,7 @@ _SIGNATURES = {
     "dds_overlap_sum_m": [_i64, _p, _p, _p, _p, _p, _d, _p, _p, _p, _ip, _p, _p],
     "dds_mfma_pack_x": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
     "dds_mfma_apply": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p],
+    "dds_mfma_apply_sub": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p],
 }
 
 _lib.dds_last_error.restype = ctypes.c_char_p

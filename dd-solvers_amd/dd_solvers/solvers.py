@@ -1617,12 +1617,15 @@ class SchwarzOperator(SparseSolver):
             b_off = np.concatenate([[0], np.cumsum(KP * NT * 512)]).astype(np.int64)
             tiles = np.zeros((0, 2), dtype=np.int32)
             ypos = None
+            # one workgroup per subdomain (grid -1), longest first, is the default shape;
+            # the autotuner also tries the persistent item grid (grid >= 0)
+            order = np.argsort(-sizes, kind="stable").astype(np.int32)
             self._mfma = dict(
                 b_off=torch.as_tensor(b_off[:-1], device=dev),
                 bp=torch.empty(int(b_off[-1]), dtype=store_dtype, device=dev),
                 sigma=torch.empty((n_s, _lib.NRHS), dtype=torch.float32, device=dev),
                 Ym=torch.empty((int(sizes.sum()), _lib.NRHS), dtype=torch.float64, device=dev),
-                grid=0, waves=4)
+                order=torch.as_tensor(order, device=dev), grid=-1, waves=8, rpw=4)
             alg = int((sizes.astype(np.int64) ** 2).sum()) * esz
         elif layout in ("sym", "sym_mfma"):
             group = 1
@@ -1820,10 +1823,27 @@ class SchwarzOperator(SparseSolver):
         _lib.call("dds_mfma_pack_x", dt, self.n_solvers, self.sub_ptr.data_ptr(),
                   _lib.ptr(self.sub_idx), mf["b_off"].data_ptr(), X.data_ptr(),
                   mf["bp"].data_ptr(), mf["sigma"].data_ptr(), _lib.ptr(st), s)
-        _lib.call("dds_mfma_apply", dt, L.n_items, L.items.data_ptr(), self.sub_ptr.data_ptr(),
-                  L.mat_off.data_ptr(), mf["b_off"].data_ptr(), _lib.ptr(L.scale),
-                  mf["sigma"].data_ptr(), L.mats.data_ptr(), mf["bp"].data_ptr(), Y.data_ptr(),
-                  int(mf["grid"]), int(mf["waves"]), _lib.ptr(st), s)
+        self._enqueue_mfma_apply(Y, st)
+
+    def _enqueue_mfma_apply(self, Y, st) -> None:
+        """The MFMA product of the packed right-hand sides into Y, in the launch shape of
+        self._mfma: grid -1 one workgroup of `waves` wavefronts per subdomain taking `rpw`
+        row blocks at a time (dds_mfma_apply_sub); grid >= 0 the persistent item grid
+        (dds_mfma_apply, 0: one wavefront per item). Every shape computes the same bits."""
+        L, mf = self.local, self._mfma
+        dt = _lib.dtype_code(L.mat_dtype)
+        s = _lib.stream()
+        if int(mf["grid"]) < 0:
+            _lib.call("dds_mfma_apply_sub", dt, self.n_solvers, mf["order"].data_ptr(),
+                      self.sub_ptr.data_ptr(), L.mat_off.data_ptr(), mf["b_off"].data_ptr(),
+                      _lib.ptr(L.scale), mf["sigma"].data_ptr(), L.mats.data_ptr(),
+                      mf["bp"].data_ptr(), Y.data_ptr(), int(mf["waves"]), int(mf["rpw"]),
+                      _lib.ptr(st), s)
+        else:
+            _lib.call("dds_mfma_apply", dt, L.n_items, L.items.data_ptr(), self.sub_ptr.data_ptr(),
+                      L.mat_off.data_ptr(), mf["b_off"].data_ptr(), _lib.ptr(L.scale),
+                      mf["sigma"].data_ptr(), L.mats.data_ptr(), mf["bp"].data_ptr(), Y.data_ptr(),
+                      int(mf["grid"]), int(mf["waves"]), _lib.ptr(st), s)
 
     def _buffers_multi(self):
         if getattr(self, "_mbuf", None) is None:

@@ -163,15 +163,17 @@ def test_autotuner_shapes_are_bitwise_equal():
     mf = pc._mfma
     n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
     ref = None
-    for grid, waves in mfma_candidates(pc.local.n_items, n_cu):
-        mf["grid"], mf["waves"] = grid, waves
+    shapes = mfma_candidates(pc.local.n_items, n_cu)
+    assert any(g < 0 for g, _, _ in shapes) and any(g >= 0 for g, _, _ in shapes)
+    for shape in shapes:
+        mf["grid"], mf["waves"], mf["rpw"] = shape
         Y = pc._local_multi_into(X, None).clone()
         if ref is None:
             ref = Y
-        assert torch.equal(Y, ref), (grid, waves)
+        assert torch.equal(Y, ref), shape
     best = autotune_mfma(pc)
-    assert (best["grid"], best["waves"]) in best["timings"]
-    assert (mf["grid"], mf["waves"]) == (best["grid"], best["waves"])
+    assert (best["grid"], best["waves"], best["rpw"]) in best["timings"]
+    assert (mf["grid"], mf["waves"], mf["rpw"]) == (best["grid"], best["waves"], best["rpw"])
     assert torch.equal(pc._local_multi_into(X, None), ref)
 
 
@@ -179,7 +181,7 @@ def test_autotune_flag_in_setup():
     pb = _kmeans()
     s = dds.CG(_pc("hybrid", "bfloat16", autotune=True), rtol=1e-8, bsr_matmul=False)
     s.setup(_torch_csr(pb.A), decomposition=pb.decomposition)
-    assert s.preconditioner.tuning["waves"] in (1, 2, 4, 8)
+    assert s.preconditioner.tuning["waves"] in (1, 2, 4, 8, 16)
     X, meta = s.solve(torch.as_tensor(np.ascontiguousarray(pb.b[:, :4]), device=dev))
     A = pb.A
     for j in range(4):

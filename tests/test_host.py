@@ -117,6 +117,12 @@ def test_host_validation_without_gpu():
         _lib.call("dds_sellmv", _lib.F64, 64, 1, 1, None, None, 1, 1, 1, 1, None, None, None, None)
     with pytest.raises(_lib.HipError, match="waves must be"):
         _lib.call("dds_mfma_apply", _lib.F16, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 0, 3, None, None)
+    with pytest.raises(_lib.HipError, match="waves must be 4, 8 or 16"):
+        _lib.call("dds_mfma_apply_sub", _lib.F16, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 2, 4, None, None)
+    with pytest.raises(_lib.HipError, match="rpw must be"):
+        _lib.call("dds_mfma_apply_sub", _lib.F16, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 8, 3, None, None)
+    with pytest.raises(_lib.HipError, match="F16 or BF16"):
+        _lib.call("dds_mfma_apply_sub", _lib.F64, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 8, 4, None, None)
     with pytest.raises(_lib.HipError, match="F16 or BF16"):
         _lib.call("dds_mfma_apply", _lib.F64, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 0, 4, None, None)
     with pytest.raises(_lib.HipError, match="F16 or BF16"):
@@ -498,8 +504,10 @@ def test_symm_layout_model(k):
 @pytest.mark.parametrize("k", [3, 16, 17, 33, 100])
 def test_mfma_operand_layout_model(k):
     """The A-operand order written at setup (_mfma_index) and the B-operand order of the X
-    pack, combined with the 16x16x16 MFMA lane rule (A: row l%16, k = 4(l/16)+i; B: column
-    l%16, k = 4(l/16)+i; D: rows 4(l/16)+i, column l%16), reproduce A @ X."""
+    pack, combined with the 16x16x32 MFMA lane rule (A: row l%16, k = 8(l/16)+e; B: column
+    l%16, k = 8(l/16)+e; D: rows 4(l/16)+i, column l%16), reproduce A @ X: the operands'
+    slot e of lane l holds the same column of A and row of X, so the instruction's own
+    numbering of k only permutes the sum."""
     from dd_solvers.solvers import _mfma_index
     rng = np.random.default_rng(k)
     A = rng.standard_normal((k, k))
@@ -521,14 +529,13 @@ def test_mfma_operand_layout_model(k):
             a0 = (R * KP + K2) * 512
             a_op = packed[a0:a0 + 512].reshape(64, 8)
             b_op = Bp[K2 * 512:K2 * 512 + 512].reshape(64, 8)
-            for half in range(2):
-                Am = np.zeros((16, 16))
-                Bm = np.zeros((16, 16))
-                for lane in range(64):
-                    for i in range(4):
-                        Am[lane % 16, 4 * (lane // 16) + i] = a_op[lane, 4 * half + i]
-                        Bm[4 * (lane // 16) + i, lane % 16] = b_op[lane, 4 * half + i]
-                D += Am @ Bm
+            Am = np.zeros((16, 32))
+            Bm = np.zeros((32, 16))
+            for lane in range(64):
+                for e in range(8):
+                    Am[lane % 16, 8 * (lane // 16) + e] = a_op[lane, e]
+                    Bm[8 * (lane // 16) + e, lane % 16] = b_op[lane, e]
+            D += Am @ Bm
         Y[16 * R:16 * R + 16] = D
     assert np.allclose(Y[:k], A @ X, rtol=1e-12, atol=1e-12)
 
@@ -536,10 +543,13 @@ def test_mfma_operand_layout_model(k):
 def test_mfma_autotuner_candidates():
     from dd_solvers.tuning import mfma_candidates
     c = mfma_candidates(10_000, 256)
-    assert (0, 1) in c and (0, 8) in c
-    assert all(w in (1, 2, 4, 8) for _, w in c)
-    assert all(g == 0 or g * w < 10_000 for g, w in c)
-    assert mfma_candidates(10, 256) == [(0, 1), (0, 2), (0, 4), (0, 8)]
+    assert (0, 1, 4) in c and (0, 8, 4) in c and (-1, 16, 8) in c
+    assert all(w in (4, 8, 16) and r in (2, 4, 8) for g, w, r in c if g < 0)
+    assert all(w in (1, 2, 4, 8) and r == 4 for g, w, r in c if g >= 0)
+    assert all(g <= 0 or g * w < 10_000 for g, w, _ in c)
+    small = mfma_candidates(10, 256)
+    assert [x for x in small if x[0] >= 0] == [(0, 1, 4), (0, 2, 4), (0, 4, 4), (0, 8, 4)]
+    assert len([x for x in small if x[0] < 0]) == 9
 
 
 def test_sell_compact_column_format():
