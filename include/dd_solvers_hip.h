@@ -41,6 +41,14 @@
  *                             cudssExecute(SOLVE) of a large sparse coarse matrix
  *                             (cudss.cpp:64-78, called from CUDSS.solve, solvers.py:106-110):
  *                             the two sweeps of a nested-dissection block LDL^T factor
+ *   dds_nd_plan_build, dds_nd_plan_field, dds_nd_plan_free
+ *                             cudssExecute(ANALYSIS) in spdFactorize (cudss.cpp:80-96): the
+ *                             ordering and symbolic tables, host code
+ *   dds_nd_factor             cudssExecute(FACTORIZATION) (cudss.cpp:97-105): the fronts of
+ *                             one tree height
+ *   dds_mfma_apply, dds_mfma_apply_sub
+ *                             the local apply to blocks of 16 right-hand sides (no counterpart:
+ *                             the reference solves one right-hand side at a time)
  *   dds_pcg_*                 the CG.solve loop body (solvers.py:228-297): torch.linalg.norm,
  *                             torch.dot, fused_add / fused_sub / fused_mul_add (:170-189)
  *   dds_cast                  rhs.to(preconditioner_precision) / y.to(rhs.dtype)
