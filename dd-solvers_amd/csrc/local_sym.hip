@@ -557,6 +557,8 @@ __device__ __forceinline__ void symm_issue_at(uint32_t st, SymmCursor c, int nro
 // LDS stages of the tile pipeline (8 KiB each): NS - 1 tiles in flight per wavefront. Two
 // stages leave LDS for six workgroups per CU at max_k = 1000 (25 KiB each); measured at Q1
 // 128^3 FP16: 2 stages 660.6 us, 3 stages 676.5 us (four workgroups per CU), 4 stages 728 us.
+// Half-tile stages (4 KiB, twice the waits per tile) were slower at every depth: 952 us
+// with 3 stages, 991 us with 4, 1106 us with 5.
 #ifndef DDS_SYMM_STAGES
 #define DDS_SYMM_STAGES 2
 #endif
