@@ -545,8 +545,7 @@ __device__ __forceinline__ void mfma_rows(int s, int R0, int lane, const int64_t
   const TM* Bs = bp + b_off[s];
   if (nrb == RPW) {
     // full group: all operand loads of a step are issued before its MFMAs (no branch
-    // between a load and its use), two steps per iteration
-#pragma unroll 2
+    // between a load and its use)
     for (int K2 = 0; K2 < KP; ++K2) {
       Vec16<TM> bv[NT], av[RPW];
 #pragma unroll
