@@ -477,8 +477,8 @@ __global__ __launch_bounds__(256) void sym_pack_x_kernel(int64_t n_blk,
 }
 
 constexpr uint32_t kOob = 0x80000000u;  // a buffer offset past every range: loads zeros
-// Largest subdomain of the matrix-core apply: its x terms (NT x 128 B per 64-row block) sit
-// in LDS beside the tile stages.
+// Largest subdomain of the matrix-core apply: its result (64 floats per 64-row block) and
+// block scales sit in LDS beside the tile stages.
 constexpr int kSymMfmaMaxK = 4096;
 
 __host__ __device__ inline int64_t symm_store_elems(int k) {  // elements of one subdomain
@@ -503,27 +503,64 @@ __device__ __forceinline__ int64_t symm_tile_off(int I, int J, int T, int nbL) {
 // distinct 16-byte bank groups. The map is its own inverse.
 __device__ __forceinline__ int symm_slot(int c) { return c ^ (((c >> 4) & 1) << 2); }
 
-// The 8 block loads of tile (I, J) into LDS stage `st` (zeros for absent blocks; every block
-// zero when `real` is false, so every path issues the same count of loads). voff: this
-// lane's chunk of a block in bytes (16 symm_slot(lane)).
+// A stage of the tile pipeline: the tile (8 blocks of 1 KiB), then, when the x terms travel
+// with the tiles (PER_TILE), the terms of block J (the B operands of its row products) and of
+// block I (the A operands of its transposed products, loaded with the first tile of a row;
+// zeros otherwise), XL x 256 bytes each. Otherwise the item's terms are loaded into LDS once,
+// before its tiles (Tmax XB bytes beside the stages). Measured at Q1 128^3 (max_k = 1000, two
+// stages): FP16 (2 terms) 659.8 us with the item's terms in LDS (25.2 KiB per workgroup, six
+// per CU), 670.8 us with the terms per tile (21.5 KiB, seven per CU); BF16 (3 terms) 686.3 us
+// with the item's terms (27.3 KiB, five per CU), 661.7 us per tile (22.5 KiB, six per CU).
+template <int NT>
+struct SymmStage {
+  static constexpr bool PER_TILE = NT > 2;
+  static constexpr int XB = NT * 128;            // bytes of x terms per 64-row block
+  static constexpr int XL = PER_TILE ? (XB + 255) / 256 : 0;  // 4-byte loads per block of terms
+  static constexpr int XJ = 8192;                // offset of the terms of block J
+  static constexpr int XI = XJ + 256 * XL;       // offset of the terms of block I
+  static constexpr int BYTES = XI + 256 * XL;
+  static constexpr int LOADS = 8 + 2 * XL;       // loads issued per tile, on every path
+  // LDS of one workgroup: stages, 16 zero bytes, yl (64 floats), the item's terms (when not
+  // PER_TILE), yv (64 Tmax floats), block scales (64 ceil(Tmax / 64) floats)
+  static constexpr size_t lds_bytes(int NS, int Tmax) {
+    return (size_t)NS * BYTES + 16 + 256 + (PER_TILE ? 0 : (size_t)Tmax * XB + 256) +
+           (size_t)Tmax * 256 + (size_t)256 * ((Tmax + 63) / 64);
+  }
+};
+
+// The loads of tile (I, J) into LDS stage `st`: its 8 blocks (zeros for absent blocks; every
+// block zero when `real` is false, so every path issues the same count of loads), then the
+// x terms of blocks J and I. voff: this lane's chunk of a block in bytes (16 symm_slot(lane)).
+template <int NT>
 __device__ __forceinline__ void symm_issue(uint32_t st, bool real, int I, int J, int T, int rL,
-                                           int nbL, i32x4_t rm, uint32_t voff) {
+                                           int nbL, i32x4_t rm, uint32_t voff, i32x4_t rx,
+                                           int lane) {
+  using SS = SymmStage<NT>;
   const bool diag = J == I;
   const int nb = I < T - 1 ? 4 : nbL;
   const uint32_t toff = real ? (uint32_t)(symm_tile_off(I, J, T, nbL) * 2) : 0u;
   if (real && !diag && nb == 4) {  // a whole off-diagonal tile: 8 consecutive blocks
     dma16x4(st, voff, rm, toff);
     dma16x4(st + 4096u, voff, rm, toff + 4096u);
-    return;
-  }
-  const int nk = diag && I == T - 1 ? (rL + 31) / 32 : 2;
+  } else {
+    const int nk = diag && I == T - 1 ? (rL + 31) / 32 : 2;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int Rb = q >> 1, K2 = q & 1;
-    const bool ok = real && Rb < nb && K2 < nk && !(diag && K2 == 1 && Rb < 2);
-    const int idx = diag ? Rb + (nk == 2 ? (Rb > 2 ? Rb - 2 : 0) : 0) + K2 : q;
-    dma16(st + (uint32_t)q * 1024u, voff, rm, ok ? toff + (uint32_t)idx * 1024u : kOob);
+    for (int q = 0; q < 8; ++q) {
+      const int Rb = q >> 1, K2 = q & 1;
+      const bool ok = real && Rb < nb && K2 < nk && !(diag && K2 == 1 && Rb < 2);
+      const int idx = diag ? Rb + (nk == 2 ? (Rb > 2 ? Rb - 2 : 0) : 0) + K2 : q;
+      dma16(st + (uint32_t)q * 1024u, voff, rm, ok ? toff + (uint32_t)idx * 1024u : kOob);
+    }
   }
+  const uint32_t l4 = 4u * (uint32_t)lane;
+#pragma unroll
+  for (int u = 0; u < SS::XL; ++u)
+    dma4(st + (uint32_t)(SS::XJ + 256 * u), l4, rx,
+         real ? (uint32_t)(J * SS::XB + 256 * u) : kOob);
+#pragma unroll
+  for (int u = 0; u < SS::XL; ++u)
+    dma4(st + (uint32_t)(SS::XI + 256 * u), l4, rx,
+         real && J == 0 ? (uint32_t)(I * SS::XB + 256 * u) : kOob);
 }
 
 // The tile rows of an item: the pairs p0 .. p_end-1 cover rows p and T-1-p, taken in
@@ -548,27 +585,30 @@ __device__ __forceinline__ SymmCursor symm_advance(SymmCursor c, int T, int p0, 
 }
 
 // The tile under the cursor into LDS stage `st`; zeros when the cursor is past the item.
+template <int NT>
 __device__ __forceinline__ void symm_issue_at(uint32_t st, SymmCursor c, int nrows, int T,
-                                              int rL, int nbL, i32x4_t rm, uint32_t voff) {
+                                              int rL, int nbL, i32x4_t rm, uint32_t voff,
+                                              i32x4_t rx, int lane) {
   const bool real = c.i < nrows;
-  symm_issue(st, real, real ? c.I : 0, real ? c.J : 0, T, rL, nbL, rm, voff);
+  symm_issue<NT>(st, real, real ? c.I : 0, real ? c.J : 0, T, rL, nbL, rm, voff, rx, lane);
 }
 
-// LDS stages of the tile pipeline (8 KiB each): NS - 1 tiles in flight per wavefront. Two
-// stages leave LDS for six workgroups per CU at max_k = 1000 (25 KiB each); measured at Q1
-// 128^3 FP16: 2 stages 660.6 us, 3 stages 676.5 us (four workgroups per CU), 4 stages 728 us.
-// Half-tile stages (4 KiB, twice the waits per tile) were slower at every depth: 952 us
-// with 3 stages, 991 us with 4, 1106 us with 5.
+// LDS stages of the tile pipeline (a tile and its x terms, 8.5 KiB for FP16): NS - 1 tiles
+// in flight per wavefront. Measured at Q1 128^3 FP16 with the x terms of the whole item in
+// LDS instead (25 KiB per workgroup at max_k = 1000, six per CU): 2 stages 660.6 us, 3
+// stages 676.5 us (four workgroups per CU), 4 stages 728 us. Half-tile stages (4 KiB, twice
+// the waits per tile) were slower at every depth: 952 us with 3 stages, 991 us with 4,
+// 1106 us with 5.
 #ifndef DDS_SYMM_STAGES
 #define DDS_SYMM_STAGES 2
 #endif
 constexpr int kSymmStages = DDS_SYMM_STAGES;
-static_assert(kSymmStages >= 2 && 8 * (kSymmStages - 1) <= 63, "vmcnt range");
+static_assert(kSymmStages >= 2 && SymmStage<3>::LOADS * (kSymmStages - 1) <= 63, "vmcnt range");
 
 // One workgroup (one wavefront) per item (s, q): the tile-row pairs p = q G .. q G + G - 1
 // of subdomain s (G = pairs_per_item, the last group of a subdomain possibly shorter), tile
-// row by tile row. The tiles stream through NS LDS stages (NS - 1 in flight while one is
-// computed). The item's result accumulates in LDS in FP32 (yv: each tile's transposed part
+// row by tile row. The tiles and their x terms stream through NS LDS stages (NS - 1 in
+// flight while one is computed). The item's result accumulates in LDS in FP32 (yv: each tile's transposed part
 // T_IJ^T x_I into block J, each row's product sum_J T_IJ x_J into block I), and every row the
 // item touches is written once, at its end: Yp[ypos[s] + q 64 T + a] for the local rows
 // a < min(k, 64 (T - q G)).
@@ -583,18 +623,20 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   using S = SymMfma<TM>;
   using op_t = typename S::op_t;
   constexpr int NT = S::NT;
+  using SS = SymmStage<NT>;
   constexpr int NS = kSymmStages;
-  constexpr int XB = NT * 128;  // bytes of x terms per 64-row block
-  constexpr int NF = NS - 1;    // tiles in flight
+  constexpr int SB = SS::BYTES;    // one stage
+  constexpr int LPT = SS::LOADS;   // loads per tile
+  constexpr int NF = NS - 1;       // tiles in flight
   if (gate != nullptr && gate->done) return;
   extern __shared__ __align__(16) unsigned char sym_smem[];
-  // [NS stages x 8 blocks x 64 lanes x 16 B][16 zero bytes][yl: 64 floats]
-  // [x terms: Tmax XB + 256][yv: 64 Tmax floats][sig: 64 ceil(Tmax / 64) floats]
+  // [NS stages x SB][16 zero bytes][yl: 64 floats][x terms: Tmax XB + 256, when not
+  // PER_TILE][yv: 64 Tmax floats][sig: 64 ceil(Tmax / 64) floats]
   unsigned char* stages = sym_smem;
-  unsigned char* zero16 = sym_smem + NS * 8192;
+  unsigned char* zero16 = sym_smem + NS * SB;
   float* yl = reinterpret_cast<float*>(zero16 + 16);
   unsigned char* xtl = zero16 + 16 + 256;
-  float* yv = reinterpret_cast<float*>(xtl + Tmax * XB + 256);
+  float* yv = reinterpret_cast<float*>(xtl + (SS::PER_TILE ? 0 : Tmax * SS::XB + 256));
   float* sig = yv + 64 * Tmax;
 
   const int lane = threadIdx.x;
@@ -616,30 +658,33 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
 
   if (lane < 4) reinterpret_cast<float*>(zero16)[lane] = 0.f;
   for (int a = lane; a < 64 * (I2max + 1); a += 64) yv[a] = 0.f;
-  // the x terms and scales of blocks 0..I2max (zeros past the subdomain), then the first
-  // tiles
+  // the x terms (unless they travel with the tiles) and the scales of blocks 0..I2max
+  // (zeros past the subdomain), then the first tiles
   const int64_t b0x = blk_ptr[s];
-  const i32x4_t rx = buffer_rsrc(xt + b0x * NT * 64, (uint32_t)(T * NT * 128));
+  const i32x4_t rx = buffer_rsrc(xt + b0x * NT * 64, (uint32_t)(T * SS::XB));
   const i32x4_t rs = buffer_rsrc(sigx + b0x, (uint32_t)(T * 4));
-  const int xdw = (I2max + 1) * XB / 4;  // dwords
-  for (int d0 = 0; d0 < xdw; d0 += 64)
-    dma4(lds_addr(xtl + 4 * d0), (uint32_t)lane * 4u, rx, 4u * (uint32_t)d0);
+  if (!SS::PER_TILE) {
+    const int xdw = (I2max + 1) * SS::XB / 4;  // dwords
+    for (int d0 = 0; d0 < xdw; d0 += 64)
+      dma4(lds_addr(xtl + 4 * d0), (uint32_t)lane * 4u, rx, 4u * (uint32_t)d0);
+  }
   for (int d0 = 0; d0 <= I2max; d0 += 64)
     dma4(lds_addr(sig + d0), (uint32_t)lane * 4u, rs, 4u * (uint32_t)d0);
   SymmCursor ic = {0, symm_row(T, p0, p_end, 0), 0};  // issue cursor
 #pragma unroll
   for (int t = 0; t < NF; ++t) {
-    symm_issue_at(stage0 + (uint32_t)(t % NS) * 8192u, ic, nrows, T, rL, nbL, rm, voff);
+    symm_issue_at<NT>(stage0 + (uint32_t)(t % NS) * SB, ic, nrows, T, rL, nbL, rm, voff, rx, lane);
     if (ic.i < nrows) ic = symm_advance(ic, T, p0, p_end);
   }
-  wait_vm<8 * NF>();  // the x terms and scales (issued before every tile) are in
+  wait_vm<LPT * NF>();  // the terms and scales (issued before every tile) are in
   wave_lds_fence();
 
   const int tm = lane & 15;
-  // operand of x rows 64 J + 32 h + 8 (l/16) + e, term l % 16 (zero past the NT terms): the
-  // B operand of a row product and the A' operand of a transposed product
-  auto x_op = [&](int J, int h) __attribute__((always_inline)) -> op_t {
-    const unsigned char* src = tm < NT ? xtl + J * XB + (h * NT + tm) * 64 + 16 * (lane >> 4) : zero16;
+  // operand of x rows 64 J + 32 h + 8 (l/16) + e, term l % 16 (zero past the NT terms), from
+  // the terms of block J at xb: the B operand of a row product and the A' operand of a
+  // transposed product
+  auto x_op = [&](const unsigned char* xb, int h) __attribute__((always_inline)) -> op_t {
+    const unsigned char* src = tm < NT ? xb + (h * NT + tm) * 64 + 16 * (lane >> 4) : zero16;
     op_t o;
     const uint4 v = *reinterpret_cast<const uint4*>(src);
     __builtin_memcpy(&o, &v, 16);
@@ -652,8 +697,7 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   float sI = 1.f;  // sig of the current tile row
 
   const int g = lane >> 4, r = (lane >> 2) & 3, q4 = lane & 3;
-  auto tile = [&](int t, float sJ) __attribute__((always_inline)) {
-    const unsigned char* sb = stages + (t % NS) * 8192;
+  auto tile = [&](const unsigned char* sb, float sJ) __attribute__((always_inline)) {
 #pragma unroll
     for (int Rb = 0; Rb < 4; ++Rb) {
       const uint4 c0 = *reinterpret_cast<const uint4*>(sb + (2 * Rb) * 1024 + voff);
@@ -724,21 +768,25 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   SymmCursor cc = {0, symm_row(T, p0, p_end, 0), 0};  // compute cursor
   for (int t = 0; t < ntot; ++t) {
     // into the stage of task t - 1, read completely before
-    symm_issue_at(stage0 + (uint32_t)((t + NF) % NS) * 8192u, ic, nrows, T, rL, nbL, rm, voff);
+    symm_issue_at<NT>(stage0 + (uint32_t)((t + NF) % NS) * SB, ic, nrows, T, rL, nbL, rm, voff,
+                      rx, lane);
     if (ic.i < nrows) ic = symm_advance(ic, T, p0, p_end);
-    wait_vm<8 * NF>();  // task t's tile is in (vector loads complete in issue order)
+    wait_vm<LPT * NF>();  // task t's loads are in (vector loads complete in issue order)
+    const unsigned char* sb = stages + (t % NS) * SB;
     if (cc.J == 0) {  // a new tile row
-      a0 = x_op(cc.I, 0);
-      a1 = x_op(cc.I, 1);
+      const unsigned char* xi = SS::PER_TILE ? sb + SS::XI : xtl + cc.I * SS::XB;
+      a0 = x_op(xi, 0);
+      a1 = x_op(xi, 1);
       sI = sig[cc.I];
 #pragma unroll
       for (int i = 0; i < 4; ++i) d[i] = z4;
     }
-    b0 = x_op(cc.J, 0);
-    b1 = x_op(cc.J, 1);
+    const unsigned char* xj = SS::PER_TILE ? sb + SS::XJ : xtl + cc.J * SS::XB;
+    b0 = x_op(xj, 0);
+    b1 = x_op(xj, 1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) dt[i] = z4;
-    tile(t, sig[cc.J]);
+    tile(sb, sig[cc.J]);
     add_transposed(cc.J, sI);
     if (cc.J == cc.I) add_row(cc.I);  // the row is complete
     cc = symm_advance(cc, T, p0, p_end);
@@ -860,9 +908,7 @@ extern "C" int dds_sym_mfma_apply_packed(int mat_dtype, int vec_dtype, int64_t n
   DDS_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
 #define DDS_SYMM(TM, TV)                                                                       \
   do {                                                                                         \
-    const size_t lds = (size_t)kSymmStages * 8192 + 16 + 256 +                                 \
-                       (size_t)Tmax * (SymMfma<TM>::NT * 128 + 256) + 256 +                    \
-                       (size_t)256 * ceil_div(Tmax, 64);                                       \
+    const size_t lds = SymmStage<SymMfma<TM>::NT>::lds_bytes(kSymmStages, Tmax);               \
     DDS_REQUIRE(lds <= (size_t)max_lds,                                                        \
                 "dds_sym_mfma_apply_packed: %zu bytes of LDS for max_k %lld exceed the %d "    \
                 "per workgroup", lds, (long long)max_k, max_lds);                              \

@@ -634,11 +634,14 @@ def test_sell_slot_pair_order():
 def _symm_schedule(k, G, q, NS, NT):
     """Restates the control flow of sym_mfma_apply_kernel for one item (subdomain of k
     unknowns, pairs q G .. q G + G - 1, tile rows p and T-1-p of each in increasing order)
-    and checks, with vector memory operations completing in issue order: the x terms and
-    scales are in before the loop; each wait count inside the loop (8 (NS - 1)) equals the
-    loads issued after the awaited tile's; the awaited tile is still in its stage and is the
-    tile computed; a stage is reloaded only after its task was computed; the item reads every
-    tile of its rows once, a whole subdomain in storage order."""
+    and checks, with vector memory operations completing in issue order: the scales (and the
+    item's terms) are in before the loop; each wait count inside the loop (LPT (NS - 1), LPT = 8 + 2 XL loads per
+    tile: XL = ceil(NT 128 / 256) with the x terms per tile (NT = 3), else 0 with the item's
+    terms loaded first) equals the loads issued after the awaited tile's; the awaited tile
+    is still in its stage and is the tile computed, with the x terms of its block J and, at
+    the first tile of a row, of its block I; a stage is reloaded only after
+    its task was computed; the item reads every tile of its rows once, a whole subdomain in
+    storage order."""
     T = -(-k // 64)
     P = -(-T // 2)
     p0, p_end = q * G, min(P, q * G + G)
@@ -662,31 +665,42 @@ def _symm_schedule(k, G, q, NS, NT):
         return (i, I, J)
 
     NF = NS - 1
+    XL = -(-NT * 128 // 256) if NT > 2 else 0  # terms per tile, or the item's before its tiles
+    LPT = 8 + 2 * XL
     ntot = sum(I + 1 for I in rows)
-    xdw = (I2max + 1) * NT * 128 // 4
-    queue = [("x",)] * ((xdw + 63) // 64 + (I2max + 1 + 63) // 64)  # terms, then scales
+    xdw = 0 if XL else (I2max + 1) * NT * 128 // 4
+    queue = [("sig",)] * ((xdw + 63) // 64 + (I2max + 1 + 63) // 64)  # terms, then scales
     stage_task = {}
 
     def issue(t, c):
         prev = stage_task.get(t % NS)
         real = c[0] < nrows
-        queue.extend([("tile", t, (c[1], c[2]) if real else None)] * 8)  # 8 loads, always
+        I, J = (c[1], c[2]) if real else (None, None)
+        queue.extend([("tile", t, (I, J) if real else None)] * 8)  # 8 block loads, always
+        queue.extend([("xJ", t, J)] * XL)
+        queue.extend([("xI", t, I if real and J == 0 else None)] * XL)
         stage_task[t % NS] = t
         return prev, (advance(c) if real else c)
 
     ic = (0, row(0), 0)
     for t in range(NF):
         _, ic = issue(t, ic)
-    assert max(i for i, e in enumerate(queue) if e == ("x",)) <= len(queue) - 1 - 8 * NF
+    assert max(i for i, e in enumerate(queue) if e == ("sig",)) <= len(queue) - 1 - LPT * NF
     cc = (0, row(0), 0)
     seen = []
     for t in range(ntot):
         prev, ic = issue(t + NF, ic)
         assert prev is None or prev < t             # the stage's task was computed
-        last = max(i for i, e in enumerate(queue) if e[0] == "tile" and e[1] == t)
-        assert len(queue) - 1 - last == 8 * NF      # the exact count after t's loads
+        mine = [i for i, e in enumerate(queue) if len(e) > 1 and e[1] == t]
+        assert len(mine) == LPT
+        assert len(queue) - 1 - max(mine) == LPT * NF  # the exact count after t's loads
         assert stage_task[t % NS] == t
-        assert queue[last][2] == (cc[1], cc[2])     # the tile loaded is the task computed
+        got = {queue[i][0]: queue[i][2] for i in mine}
+        assert got["tile"] == (cc[1], cc[2])        # the tile loaded is the task computed
+        if XL:
+            assert got["xJ"] == cc[2]
+            if cc[2] == 0:                          # a row starts: its x_I operands
+                assert got["xI"] == cc[1]
         assert cc[2] <= cc[1] <= I2max
         seen.append((cc[1], cc[2]))
         cc = advance(cc)
@@ -696,7 +710,7 @@ def _symm_schedule(k, G, q, NS, NT):
         assert seen == [(I, J) for I in range(T) for J in range(I + 1)]  # storage order
 
 
-@pytest.mark.parametrize("NS,NT", [(3, 2), (3, 3), (2, 2), (4, 3)])
+@pytest.mark.parametrize("NS,NT", [(3, 2), (3, 3), (2, 2), (2, 3), (4, 3)])
 def test_symm_schedule(NS, NT):
     for k in (1, 63, 64, 65, 127, 128, 129, 700, 1000, 2049, 4096):
         T = -(-k // 64)
