@@ -393,7 +393,10 @@ __global__ __launch_bounds__(kBlock) void overlap_sum_m_kernel(
 // pairs): for row block R and pair K2 the 64 lanes' 8 elements are at
 //   a_off[s] + ((R * KP + K2) * 64 + l) * 8,
 // lane l holding A[16R + l%16][32K2 + 4(l/16) + i] (i < 4) then the same 16 columns on
-// (the A operands of two 16x16x16 MFMAs). Padding is zero.
+// (the A operand of one 16x16x32 MFMA). Padding is zero. Measured at Q1 128^3 (16 right-hand
+// sides, FP16, k <= 1000), against this order: groups of 4 row blocks stored pair by pair (the
+// 4 operands of a step 4 KiB contiguous) 7% slower in this kernel, although a bare stream in
+// that order (no MFMAs) ran 13% faster than one in this order.
 // X pack of subdomain s: for pair K2 and term t (< NT) at b_off[s] + ((K2 * NT + t) * 64 + l) * 8,
 // lane l holding term t of X[32K2 + 4(l/16) + i][l%16] / sigma[s][l%16] (i < 4), then the
 // same 16 rows on (the B operands).
@@ -517,6 +520,14 @@ __device__ __forceinline__ float4_t mfma32<bf16_t>(const Vec16<bf16_t>& a, const
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
 }
 
+// Operand register buffers of the MFMA apply. Measured at Q1 128^3 (16 right-hand sides,
+// FP16, best launch shape each, one device): 1 buffer 1.532 ms, 2 buffers 1.603 ms, 3 buffers
+// 1.632 ms: more loads in flight per wavefront slow this stream down.
+#ifndef DDS_MFMA_DEPTH
+#define DDS_MFMA_DEPTH 1
+#endif
+constexpr int kMfmaDepth = DDS_MFMA_DEPTH;
+
 // Row blocks R0 .. R0 + RPW - 1 of subdomain s (one wavefront): for each column pair K2 in
 // increasing order, the NT B operands (packed x terms, reused across the RPW row blocks) and
 // the RPW A operands are loaded, then one MFMA per (row block, term). Every row block's
@@ -543,20 +554,42 @@ __device__ __forceinline__ void mfma_rows(int s, int R0, int lane, const int64_t
     for (int t = 0; t < NT; ++t) c[r][t] = float4_t{0.f, 0.f, 0.f, 0.f};
   const TM* As = A + a_off[s];
   const TM* Bs = bp + b_off[s];
+  // the operand of row block R0 + r, pair K2: Ar[r] + K2 * 512
+  const TM* Ar[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) Ar[r] = As + ((int64_t)(R0 + r) * KP) * 512 + lane * 8;
   if (nrb == RPW) {
-    // full group: all operand loads of a step are issued before its MFMAs (no branch
-    // between a load and its use)
-    for (int K2 = 0; K2 < KP; ++K2) {
-      Vec16<TM> bv[NT], av[RPW];
+    // full group, software-pipelined over D register buffers: the operands of pair K2 + D - 1
+    // are loaded before the MFMAs of pair K2, and scheduling barriers keep the compiler from
+    // sinking those loads below the MFMAs (which would leave one or two loads in flight).
+    // Every step issues its loads (past the last pair: the last pair again), so the count of
+    // loads in flight is the same on every path and the compiler's waits before the MFMAs
+    // cover exactly the operands they use.
+    constexpr int D = kMfmaDepth;
+    Vec16<TM> bv[D][NT], av[D][RPW];
+    auto load = [&](int K2, int j) __attribute__((always_inline)) {
 #pragma unroll
-      for (int t = 0; t < NT; ++t) bv[t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
+      for (int t = 0; t < NT; ++t)
+        bv[j][t] = load16(Bs + ((int64_t)(K2 * NT + t) * 64 + lane) * 8);
 #pragma unroll
-      for (int r = 0; r < RPW; ++r)
-        av[r] = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
+      for (int r = 0; r < RPW; ++r) av[j][r] = load16_nt(Ar[r] + (int64_t)K2 * 512);
+    };
 #pragma unroll
-      for (int r = 0; r < RPW; ++r)
+    for (int j = 0; j < D - 1; ++j) load(j < KP ? j : KP - 1, j);
+    for (int K2 = 0; K2 < KP; K2 += D) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) c[r][t] = mfma32<TM>(av[r], bv[t], c[r][t]);
+      for (int j = 0; j < D; ++j) {
+        const int Kn = K2 + j + D - 1;
+        load(Kn < KP ? Kn : KP - 1, (j + D - 1) % D);
+        __builtin_amdgcn_sched_barrier(0);
+        if (K2 + j < KP) {
+#pragma unroll
+          for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) c[r][t] = mfma32<TM>(av[j][r], bv[j][t], c[r][t]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   } else {
     for (int K2 = 0; K2 < KP; ++K2) {
@@ -566,7 +599,7 @@ __device__ __forceinline__ void mfma_rows(int s, int R0, int lane, const int64_t
 #pragma unroll
       for (int r = 0; r < RPW; ++r) {
         if (r < nrb) {
-          const Vec16<TM> av = load16_nt(As + ((int64_t)((R0 + r) * KP + K2) * 64 + lane) * 8);
+          const Vec16<TM> av = load16_nt(Ar[r] + (int64_t)K2 * 512);
 #pragma unroll
           for (int t = 0; t < NT; ++t) c[r][t] = mfma32<TM>(av, bv[t], c[r][t]);
         }
@@ -594,7 +627,7 @@ __device__ __forceinline__ void mfma_rows(int s, int R0, int lane, const int64_t
 // Persistent: work item i = (s, g) covers row blocks 4g .. 4g+3 of subdomain s; wave w of
 // block b takes items b*W + w, b*W + w + gridDim*W, ...
 template <typename TM>
-__global__ void mfma_apply_kernel(int64_t n_items, const int32_t* __restrict__ items,
+__global__ __launch_bounds__(512) void mfma_apply_kernel(int64_t n_items, const int32_t* __restrict__ items,
                                   const int64_t* __restrict__ sub_ptr, const int64_t* __restrict__ a_off,
                                   const int64_t* __restrict__ b_off, const float* __restrict__ scale,
                                   const float* __restrict__ sigma, const TM* __restrict__ A,
@@ -615,7 +648,7 @@ __global__ void mfma_apply_kernel(int64_t n_items, const int32_t* __restrict__ i
 // it hit the XCD's L2, where the persistent item grid spreads a subdomain's groups over
 // XCDs that each fetch it again.
 template <typename TM, int RPW>
-__global__ void mfma_sub_kernel(const int32_t* __restrict__ order, const int64_t* __restrict__ sub_ptr,
+__global__ __launch_bounds__(512) void mfma_sub_kernel(const int32_t* __restrict__ order, const int64_t* __restrict__ sub_ptr,
                                 const int64_t* __restrict__ a_off, const int64_t* __restrict__ b_off,
                                 const float* __restrict__ scale, const float* __restrict__ sigma,
                                 const TM* __restrict__ A, const TM* __restrict__ bp,
@@ -811,8 +844,8 @@ extern "C" int dds_mfma_apply_sub(int mat_dtype, int64_t n_s, const int32_t* ord
   DDS_REQUIRE(n_s >= 0 && n_s <= INT32_MAX && order && sub_ptr && a_off && b_off && sigma && A &&
                   bp && Y,
               "dds_mfma_apply_sub: bad arguments");
-  DDS_REQUIRE(waves == 4 || waves == 8 || waves == 16,
-              "dds_mfma_apply_sub: waves must be 4, 8 or 16 (got %d)", waves);
+  DDS_REQUIRE(waves == 4 || waves == 8,
+              "dds_mfma_apply_sub: waves must be 4 or 8 (got %d)", waves);
   DDS_REQUIRE(rpw == 2 || rpw == 4 || rpw == 8,
               "dds_mfma_apply_sub: rpw must be 2, 4 or 8 (got %d)", rpw);
   if (n_s == 0) return DDS_OK;

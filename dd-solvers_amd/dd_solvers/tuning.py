@@ -34,7 +34,7 @@ _CACHE: dict = {}
 def mfma_candidates(n_items: int, n_cu: int):
     """(grid, waves, rpw) shapes: grid -1 one workgroup per subdomain; grid 0 one wavefront
     per work item; grid > 0 a persistent grid (rpw 4)."""
-    out = [(-1, waves, rpw) for waves in (4, 8, 16) for rpw in (2, 4, 8)]
+    out = [(-1, waves, rpw) for waves in (4, 8) for rpw in (2, 4, 8)]
     for waves in (1, 2, 4, 8):
         out.append((0, waves, 4))
         for per_cu in (1, 2, 4, 8):

@@ -526,7 +526,7 @@ int dds_mfma_apply(int mat_dtype, int64_t n_items, const int32_t* items, const i
                    const int64_t* a_off, const int64_t* b_off, const float* scale,
                    const float* sigma, const void* A, const void* bp, double* Y, int grid,
                    int waves, const dds_pcg_state* st, void* stream);
-/* The same product, one workgroup of `waves` (4, 8, 16) wavefronts per subdomain order[b]
+/* The same product, one workgroup of `waves` (4, 8) wavefronts per subdomain order[b]
  * (b < n_s; the order of the work, longest first): its waves take the groups of rpw (2, 4, 8)
  * row blocks in turn, so a subdomain's packed right-hand sides are fetched from HBM once and
  * shared through the L2 of its CU's XCD. Same bits as dds_mfma_apply for every shape. */
