@@ -208,6 +208,20 @@ __device__ __forceinline__ void dma16(uint32_t m0, uint32_t voff, i32x4_t rsrc, 
       "v"(voff), "s"(rsrc), "s"(soff)
       : "memory");
 }
+// One 1 KiB block kept in the caches (data read again by other wavefronts): LDS m0, memory
+// soffset + voff.
+__device__ __forceinline__ void dma16_keep(uint32_t m0, uint32_t voff, i32x4_t rsrc, uint32_t soff) {
+  m0 = __builtin_amdgcn_readfirstlane(m0);
+  soff = __builtin_amdgcn_readfirstlane(soff);
+  rsrc = sgpr4(rsrc);
+  asm volatile(
+      "s_waitcnt lgkmcnt(0)\n\t"
+      "s_mov_b32 m0, %0\n\t"
+      "s_nop 0\n\t"
+      "buffer_load_dwordx4 %1, %2, %3 offen lds" ::"s"(m0),
+      "v"(voff), "s"(rsrc), "s"(soff)
+      : "memory");
+}
 // 4 bytes per lane: LDS m0 + 4 l, memory soffset + voff.
 __device__ __forceinline__ void dma4(uint32_t m0, uint32_t voff, i32x4_t rsrc, uint32_t soff) {
   m0 = __builtin_amdgcn_readfirstlane(m0);

@@ -503,6 +503,28 @@ __device__ __forceinline__ int64_t symm_tile_off(int I, int J, int T, int nbL) {
 // distinct 16-byte bank groups. The map is its own inverse.
 __device__ __forceinline__ int symm_slot(int c) { return c ^ (((c >> 4) & 1) << 2); }
 
+// The transposed operand of column block nb, rows 32 kr2 .. 32 kr2 + 31, of the tile in LDS
+// at sb (lane-ordered blocks in symm_slot order): lane l gets T[32 kr2 + 8 (l/16) + e][16 nb
+// + l % 16], e < 8 -- the B operand of x^T T and, read as an A operand, the block of T^T.
+// Each lane reads rows 32 kr2 + 8 g + r (+4 h), columns 16 nb + 4 q4 with ds_read_tr16_b64.
+template <typename op_t>
+__device__ __forceinline__ op_t symm_tr_operand(const unsigned char* sb, int kr2, int nb, int lane) {
+  const int g = lane >> 4, r = (lane >> 2) & 3, q4 = lane & 3;
+  i16x4_t hv[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int rho = 32 * kr2 + 8 * g + r + 4 * h;
+    const int blk = 2 * (rho >> 4) + (nb >> 1);
+    const int ln = (rho & 15) + 16 * (2 * (nb & 1) + (q4 >> 1));
+    hv[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) i16x4_t*)(
+        sb + blk * 1024 + symm_slot(ln) * 16 + 8 * (q4 & 1)));
+  }
+  const i16x8_t v = {hv[0][0], hv[0][1], hv[0][2], hv[0][3], hv[1][0], hv[1][1], hv[1][2], hv[1][3]};
+  op_t o;
+  __builtin_memcpy(&o, &v, 16);
+  return o;
+}
+
 // A stage of the tile pipeline: the tile (8 blocks of 1 KiB), then, when the x terms travel
 // with the tiles (PER_TILE), the terms of block J (the B operands of its row products) and of
 // block I (the A operands of its transposed products, loaded with the first tile of a row;
@@ -696,7 +718,6 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   op_t a0, a1, b0, b1;
   float sI = 1.f;  // sig of the current tile row
 
-  const int g = lane >> 4, r = (lane >> 2) & 3, q4 = lane & 3;
   auto tile = [&](const unsigned char* sb, float sJ) __attribute__((always_inline)) {
 #pragma unroll
     for (int Rb = 0; Rb < 4; ++Rb) {
@@ -709,27 +730,11 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
       p = S::mfma(o1, b1, p);
       d[Rb] += sJ * p;  // T x_J = sig[J] (T (x_J / sig[J]))
     }
-    // rows 32 kr2 + 8 g + r (+4 h), columns 16 nb + 4 q4 of the tile, read column-major
-    // (ds_read_tr16_b64) from the lane-ordered blocks
 #pragma unroll
     for (int kr2 = 0; kr2 < 2; ++kr2)
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        i16x4_t hv[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int rho = 32 * kr2 + 8 * g + r + 4 * h;
-          const int blk = 2 * (rho >> 4) + (nb >> 1);
-          const int ln = (rho & 15) + 16 * (2 * (nb & 1) + (q4 >> 1));
-          hv[h] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) i16x4_t*)(
-              sb + blk * 1024 + symm_slot(ln) * 16 + 8 * (q4 & 1)));
-        }
-        const i16x8_t v = {hv[0][0], hv[0][1], hv[0][2], hv[0][3],
-                           hv[1][0], hv[1][1], hv[1][2], hv[1][3]};
-        op_t bt;
-        __builtin_memcpy(&bt, &v, 16);
-        dt[nb] = S::mfma(kr2 == 0 ? a0 : a1, bt, dt[nb]);
-      }
+      for (int nb = 0; nb < 4; ++nb)
+        dt[nb] = S::mfma(kr2 == 0 ? a0 : a1, symm_tr_operand<op_t>(sb, kr2, nb, lane), dt[nb]);
   };
   // yv[64 J + lane] += sI x the transposed part (rows 16 nb + lane of D, the NT term rows
   // added)
@@ -796,6 +801,255 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   TV* Y = Yp + ypos[s] + (int64_t)q * 64 * T;
   const int rows = min(k, 64 * (I2max + 1));
   for (int a = lane; a < rows; a += 64) Y[a] = static_cast<TV>(static_cast<double>(yv[a]) * out_sc);
+}
+
+// ---- 16 right-hand sides on the symmetric tiles -------------------------------------------
+// Y_s = scale[s] A_s^-1 X_s for an (n, 16) block X, from the matrix-core symmetric store
+// (lower tiles read once, used for Y_I += T X_J and Y_J += T^T X_I), with X_s packed per
+// 64-row block b = blk_ptr[s] + J and column c by dds_sym_mfma_pack16: sigma[b][c] = 2^e >
+// max |X_b[:, c]| and X_b / sigma[b] (exact) in NT terms; for half h (rows 32 h ..) and term
+// t the B operand (lane l: rows 32 h + 8 (l/16) + e, column l%16) at ((2 b + h) NT + t) 512 +
+// 8 l elements.
+//
+// One workgroup of 4 wavefronts per subdomain (order[b], longest first), k <= 1024. Wavefront
+// w owns rows 16 w .. 16 w + 15 of every 64-row block of the result and keeps them in
+// registers (yacc[J], the MFMA D layout: lane l rows 4 (l/16) + i, column l % 16): from each
+// tile (I, J) it takes row block w (T[16w.., :] X_J into row block I) and column block w
+// (T[:, 16w..]^T X_I into row block J). The tiles stream through NS = NF + 2 LDS stages,
+// each with the tile (8 blocks, symm_slot order), the packed X_J (2 NT KiB) and, with the
+// first tile of a row, X_I (2 NT KiB); each wavefront issues a quarter of a stage's loads
+// and a barrier per tile publishes them. A stage is refilled two barriers after its last
+// read.
+constexpr int kSym16MaxT = 16;  // 64-row blocks of the largest subdomain (k <= 1024)
+#ifndef DDS_SYM16_AHEAD
+#define DDS_SYM16_AHEAD 1
+#endif
+constexpr int kSym16Ahead = DDS_SYM16_AHEAD;  // tiles in flight
+constexpr int kSym16Stages = kSym16Ahead + 2;
+
+template <int NT>
+struct Sym16Stage {
+  static constexpr int XJ = 8192;                // packed X of block J (2 NT chunks of 1 KiB)
+  static constexpr int XI = XJ + 2 * NT * 1024;  // packed X of block I
+  static constexpr int BYTES = XI + 2 * NT * 1024;
+  static constexpr int LOADS = 2 + NT;           // per wavefront and tile, on every path
+};
+static_assert(Sym16Stage<3>::LOADS * kSym16Ahead <= 63, "vmcnt range");
+
+// Byte offset of block q (row block q >> 1, column half q & 1) of tile (I, J) in a
+// subdomain's matrix-core symmetric store; kOob for a block the store does not hold.
+__device__ __forceinline__ uint32_t symm_block_src(int q, bool real, int I, int J, int T, int rL,
+                                                   int nbL) {
+  if (!real) return kOob;
+  const bool diag = J == I;
+  const int nb = I < T - 1 ? 4 : nbL;
+  const int Rb = q >> 1, K2 = q & 1;
+  const uint32_t toff = (uint32_t)(symm_tile_off(I, J, T, nbL) * 2);
+  if (!diag) return Rb < nb ? toff + (uint32_t)q * 1024u : kOob;
+  const int nk = I == T - 1 ? (rL + 31) / 32 : 2;
+  const bool ok = Rb < nb && K2 < nk && !(K2 == 1 && Rb < 2);
+  const int idx = Rb + (nk == 2 ? (Rb > 2 ? Rb - 2 : 0) : 0) + K2;
+  return ok ? toff + (uint32_t)idx * 1024u : kOob;
+}
+
+// The four wavefronts' LDS reads are complete and their awaited loads landed.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <int N>
+__device__ __forceinline__ void add_at(f32x4_t (&a)[N], int j, f32x4_t v) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (i == j) a[i] += v;
+}
+
+__device__ __forceinline__ bool sym16_done(const dds_pcg_state* st) {
+  if (st == nullptr) return false;
+  bool done = true;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) done = done && st[j].done != 0;
+  return done;
+}
+
+// One wavefront per 64-row block b of the (n, 16) block X (gathered through sub_idx): the
+// column scales and the NT terms of X_b / sigma[b] in B-operand order.
+template <typename TM>
+__global__ __launch_bounds__(256) void sym16_pack_kernel(
+    int64_t n_blk, const int32_t* __restrict__ blk_owner, const int64_t* __restrict__ blk_ptr,
+    const int64_t* __restrict__ sub_ptr, const int32_t* __restrict__ sub_idx,
+    const double* __restrict__ x, float* __restrict__ sigma, uint16_t* __restrict__ bp,
+    const dds_pcg_state* __restrict__ st) {
+  using S = SymMfma<TM>;
+  constexpr int NT = S::NT;
+  if (sym16_done(st)) return;
+  const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= n_blk) return;
+  const int lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+  const int s = blk_owner[b];
+  const int64_t base = sub_ptr[s];
+  const int k = (int)(sub_ptr[s + 1] - base);
+  const int J = (int)(b - blk_ptr[s]);
+  double v[2][8];
+  double m = 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int a = 64 * J + 32 * h + 8 * g + e;
+      const int ac = a < k ? a : 0;
+      const int64_t row = sub_idx != nullptr ? (int64_t)sub_idx[base + ac] : base + ac;
+      v[h][e] = a < k ? x[row * 16 + c] : 0.0;
+      m = fmax(m, fabs(v[h][e]));
+    }
+  m = fmax(m, __shfl_xor(m, 16, 64));
+  m = fmax(m, __shfl_xor(m, 32, 64));
+  int ex = 0;
+  if (m > 0.0) frexp(m, &ex);
+  ex = ex < -126 ? -126 : (ex > 127 ? 127 : ex);
+  const double inv = ldexp(1.0, -ex);  // exact scaling
+  if (g == 0) sigma[b * 16 + c] = ldexpf(1.0f, ex);
+  uint16_t* out = bp + (int64_t)b * 2 * NT * 512 + 8 * lane;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      uint16_t q[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) q[e] = S::term(static_cast<float>(v[h][e] * inv), t);
+      uint4 w;
+      __builtin_memcpy(&w, q, 16);
+      *reinterpret_cast<uint4*>(out + (h * NT + t) * 512) = w;
+    }
+}
+
+template <typename TM>
+__global__ __launch_bounds__(256) void sym16_apply_kernel(
+    const int32_t* __restrict__ order, const int64_t* __restrict__ sub_ptr,
+    const int64_t* __restrict__ blk_ptr, const int64_t* __restrict__ mat_off,
+    const float* __restrict__ scale, const float* __restrict__ sigma,
+    const TM* __restrict__ mats, const uint16_t* __restrict__ bp, double* __restrict__ Y,
+    const dds_pcg_state* __restrict__ st) {
+  using S = SymMfma<TM>;
+  using op_t = typename S::op_t;
+  constexpr int NT = S::NT;
+  using SS = Sym16Stage<NT>;
+  constexpr int NS = kSym16Stages;
+  constexpr int NF = kSym16Ahead;
+  constexpr int SB = SS::BYTES;
+  extern __shared__ __align__(16) unsigned char s16_smem[];
+  if (sym16_done(st)) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int s = order[blockIdx.x];
+  const int64_t base = sub_ptr[s];
+  const int k = (int)(sub_ptr[s + 1] - base);
+  const int T = (k + 63) / 64, rL = k - 64 * (T - 1), nbL = (rL + 15) / 16;
+  const int ntot = T * (T + 1) / 2;
+  const i32x4_t rm = buffer_rsrc(mats + mat_off[s], (uint32_t)(symm_store_elems(k) * 2));
+  const int64_t b0 = blk_ptr[s];
+  const i32x4_t rx = buffer_rsrc(bp + b0 * 2 * NT * 512, (uint32_t)(2 * T * NT * 1024));
+  const uint32_t stage0 = lds_addr(s16_smem);
+  // the column scales of the subdomain's blocks (16 floats each) after the stages, a quarter
+  // per wavefront, issued before every tile
+  const float* sgl = reinterpret_cast<const float*>(s16_smem + NS * SB);
+  dma4(stage0 + (uint32_t)(NS * SB + 256 * w), 4u * (uint32_t)lane,
+       buffer_rsrc(sigma + b0 * 16, (uint32_t)(T * 64)), 256u * (uint32_t)w);
+  const uint32_t vsw = 16u * (uint32_t)symm_slot(lane);  // tile blocks: this lane's slot
+  const uint32_t vnat = 16u * (uint32_t)lane;            // packed X: lane order
+
+  // wavefront w's quarter of the loads of tile (I, J) into stage t % NS: tile blocks 2w and
+  // 2w + 1, then chunks c = w NT .. w NT + NT - 1 of [X_J (2 NT chunks), X_I (2 NT chunks)]
+  auto issue = [&](int t, int I, int J, bool real) __attribute__((always_inline)) {
+    const uint32_t sa = stage0 + (uint32_t)(t % NS) * SB;
+#pragma unroll
+    for (int qq = 0; qq < 2; ++qq) {
+      const int q = 2 * w + qq;
+      dma16(sa + (uint32_t)q * 1024u, vsw, rm, symm_block_src(q, real, I, J, T, rL, nbL));
+    }
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      const int c = w * NT + u;
+      if (c < 2 * NT)
+        dma16_keep(sa + (uint32_t)(SS::XJ + c * 1024), vnat, rx,
+                   real ? (uint32_t)((2 * J * NT + c) * 1024) : kOob);
+      else
+        dma16_keep(sa + (uint32_t)(SS::XI + (c - 2 * NT) * 1024), vnat, rx,
+                   real && J == 0 ? (uint32_t)((2 * I * NT + c - 2 * NT) * 1024) : kOob);
+    }
+  };
+  int iI = 0, iJ = 0;  // issue cursor
+  auto advance = [](int& I, int& J) __attribute__((always_inline)) {
+    if (++J > I) {
+      J = 0;
+      ++I;
+    }
+  };
+#pragma unroll
+  for (int t = 0; t < NF; ++t) {
+    issue(t, iI, iJ, t < ntot);
+    advance(iI, iJ);
+  }
+
+  const f32x4_t z4 = {0.f, 0.f, 0.f, 0.f};
+  f32x4_t yacc[kSym16MaxT];
+#pragma unroll
+  for (int j = 0; j < kSym16MaxT; ++j) yacc[j] = z4;
+  f32x4_t racc = z4;  // row block w of the current tile row
+  op_t xi[2][NT];     // X_I operands of the current tile row
+  float sI = 1.f;     // their column scale (this lane's column)
+  const int c = lane & 15;
+  int cI = 0, cJ = 0;
+  for (int t = 0; t < ntot; ++t) {
+    issue(t + NF, iI, iJ, t + NF < ntot);  // into the stage read two barriers ago
+    advance(iI, iJ);
+    wait_vm<SS::LOADS * NF>();  // this wavefront's loads of tile t are in
+    lds_barrier();              // and every wavefront's
+    const unsigned char* sb = s16_smem + (t % NS) * SB;
+    if (cJ == 0) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+          xi[h][u] = *reinterpret_cast<const op_t*>(sb + SS::XI + (h * NT + u) * 1024 + vnat);
+      racc = z4;
+      sI = sgl[cI * 16 + c];
+    }
+    // row block w: T[16w.., 32h..] X_J[32h.., :] = sigma_J (T X_J / sigma_J)
+    f32x4_t p = z4;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const op_t a = *reinterpret_cast<const op_t*>(sb + (2 * w + h) * 1024 + vsw);
+#pragma unroll
+      for (int u = 0; u < NT; ++u)
+        p = S::mfma(a, *reinterpret_cast<const op_t*>(sb + SS::XJ + (h * NT + u) * 1024 + vnat), p);
+    }
+    racc += sgl[cJ * 16 + c] * p;
+    // column block w: T[32h.., 16w..]^T X_I[32h.., :]
+    f32x4_t d = z4;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const op_t a = symm_tr_operand<op_t>(sb, h, w, lane);
+#pragma unroll
+      for (int u = 0; u < NT; ++u) d = S::mfma(a, xi[h][u], d);
+    }
+    add_at(yacc, cJ, sI * d);
+    if (cJ == cI) add_at(yacc, cI, racc);
+    advance(cI, cJ);
+  }
+  wait_vm<0>();  // no load into LDS outlives the workgroup
+
+  const double sg = scale != nullptr ? (double)scale[s] : 1.0;
+  double* Ys = Y + base * 16;
+#pragma unroll
+  for (int J = 0; J < kSym16MaxT; ++J) {
+    if (J < T) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 64 * J + 16 * w + 4 * (lane >> 4) + i;
+        if (row < k) Ys[(int64_t)row * 16 + c] = (double)yacc[J][i] * sg;
+      }
+    }
+  }
 }
 
 template <typename TM, typename TV, typename TA>
@@ -949,6 +1203,68 @@ extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_
   return dds_sym_mfma_apply_packed(mat_dtype, vec_dtype, n_s, n_items, max_k, pairs_per_item,
                                    items, sub_ptr, blk_ptr, mat_off, ypos, scale, mats, sig, xt,
                                    Yp, gate, stream);
+}
+
+extern "C" int dds_sym_mfma_pack16(int mat_dtype, int64_t n_blk, const int32_t* blk_owner,
+                                   const int64_t* blk_ptr, const int64_t* sub_ptr,
+                                   const int32_t* sub_idx, const double* x, float* sigma,
+                                   void* bp, const dds_pcg_state* st, void* stream) {
+  DDS_REQUIRE(n_blk >= 0, "dds_sym_mfma_pack16: bad block count");
+  if (n_blk == 0) return DDS_OK;
+  DDS_REQUIRE(blk_owner && blk_ptr && sub_ptr && x && sigma && bp,
+              "dds_sym_mfma_pack16: null pointer");
+  hipStream_t s = as_stream(stream);
+  const dim3 grid((unsigned)ceil_div(n_blk, 4));
+  if (mat_dtype == DDS_F16) {
+    hipLaunchKernelGGL(sym16_pack_kernel<_Float16>, grid, dim3(256), 0, s, n_blk, blk_owner,
+                       blk_ptr, sub_ptr, sub_idx, x, sigma, static_cast<uint16_t*>(bp), st);
+  } else if (mat_dtype == DDS_BF16) {
+    hipLaunchKernelGGL(sym16_pack_kernel<bf16_t>, grid, dim3(256), 0, s, n_blk, blk_owner,
+                       blk_ptr, sub_ptr, sub_idx, x, sigma, static_cast<uint16_t*>(bp), st);
+  } else {
+    set_error("dds_sym_mfma_pack16: storage must be F16 or BF16 (got %d)", mat_dtype);
+    return DDS_EUNSUPPORTED;
+  }
+  DDS_LAUNCHED();
+  return DDS_OK;
+}
+
+extern "C" int dds_sym_mfma_apply16(int mat_dtype, int64_t n_s, int64_t max_k,
+                                    const int32_t* order, const int64_t* sub_ptr,
+                                    const int64_t* blk_ptr, const int64_t* mat_off,
+                                    const float* scale, const float* sigma, const void* mats,
+                                    const void* bp, double* Y, const dds_pcg_state* st,
+                                    void* stream) {
+  DDS_REQUIRE(n_s >= 0 && n_s <= INT32_MAX, "dds_sym_mfma_apply16: bad subdomain count");
+  DDS_REQUIRE(max_k >= 0 && max_k <= 64 * kSym16MaxT,
+              "dds_sym_mfma_apply16: max_k %lld exceeds %d", (long long)max_k, 64 * kSym16MaxT);
+  if (n_s == 0 || max_k == 0) return DDS_OK;
+  DDS_REQUIRE(order && sub_ptr && blk_ptr && mat_off && sigma && mats && bp && Y,
+              "dds_sym_mfma_apply16: null pointer");
+  hipStream_t s = as_stream(stream);
+#define DDS_S16(TM)                                                                             \
+  do {                                                                                          \
+    constexpr size_t lds = (size_t)kSym16Stages * Sym16Stage<SymMfma<TM>::NT>::BYTES +           \
+                           (size_t)kSym16MaxT * 64;                                             \
+    static_assert(lds <= 160 * 1024, "LDS");                                                    \
+    if (lds > 65536)                                                                            \
+      DDS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&sym16_apply_kernel<TM>),        \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));       \
+    hipLaunchKernelGGL((sym16_apply_kernel<TM>), dim3((unsigned)n_s), dim3(256), lds, s, order,  \
+                       sub_ptr, blk_ptr, mat_off, scale, sigma, static_cast<const TM*>(mats),   \
+                       static_cast<const uint16_t*>(bp), Y, st);                                 \
+  } while (0)
+  if (mat_dtype == DDS_F16) {
+    DDS_S16(_Float16);
+  } else if (mat_dtype == DDS_BF16) {
+    DDS_S16(bf16_t);
+  } else {
+    set_error("dds_sym_mfma_apply16: storage must be F16 or BF16 (got %d)", mat_dtype);
+    return DDS_EUNSUPPORTED;
+  }
+#undef DDS_S16
+  DDS_LAUNCHED();
+  return DDS_OK;
 }
 
 extern "C" int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, int64_t pairs_per_item,

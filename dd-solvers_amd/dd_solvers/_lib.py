@@ -110,6 +110,8 @@ _SIGNATURES = {
     "dds_mfma_pack_x": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
     "dds_mfma_apply": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p],
     "dds_mfma_apply_sub": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p],
+    "dds_sym_mfma_pack16": [_i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p],
+    "dds_sym_mfma_apply16": [_i, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
 }
 
 _lib.dds_last_error.restype = ctypes.c_char_p

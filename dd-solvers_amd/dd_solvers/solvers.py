@@ -597,7 +597,10 @@ class CG(SparseSolver):
         _lib.require_device(rhs)
         with torch.cuda.device(rhs.device):
             pc = self.preconditioner
-            block_pc = pc is not None and getattr(pc, "_block_apply", False)
+            # a single float64 vector takes the block path only where the local solver exists
+            # for blocks alone (layout 'mfma')
+            block_pc = pc is not None and getattr(pc, "_block_apply", False) and \
+                getattr(pc, "_mfma", None) is not None
             if rhs.dim() == 2 or (block_pc and rhs.dim() == 1 and rhs.dtype == torch.float64):
                 if not (self._fast_multi() and rhs.dtype == torch.float64):
                     raise ValueError("blocks of right-hand sides need float64 data, a SELL or CSR "
@@ -1205,6 +1208,7 @@ def _sym_written_rows(sizes: np.ndarray, group: int = 1) -> int:
 
 
 _MFMA_INDEX_CACHE: dict = {}
+_SYM16_MAX_K = 1024  # largest subdomain of the 16-column symmetric apply (csrc/local_sym.hip)
 
 
 def _mfma_index(k: int) -> np.ndarray:
@@ -1260,6 +1264,7 @@ class SchwarzOperator(SparseSolver):
         self.collect_timings = collect_timings
         self.waves = None
         self._mfma = None
+        self._msym = None
 
     # -- setup ------------------------------------------------------------------------
     def setup(self, matrix: torch.Tensor, dofs_per_solver: int | None = None,
@@ -1283,6 +1288,7 @@ class SchwarzOperator(SparseSolver):
         self._mbuf = None
         self._cast_buf = None
         self._mfma = None
+        self._msym = None
         self.Yp = None
         self._xpack = None
         if self.collect_timings:
@@ -1646,6 +1652,15 @@ class SchwarzOperator(SparseSolver):
                     blk_owner=torch.as_tensor(np.repeat(np.arange(n_s, dtype=np.int32), T), device=dev),
                     sig=torch.ones(n_blk, dtype=torch.float32, device=dev),
                     xt=torch.zeros(n_blk * nt * 64, dtype=torch.int16, device=dev))
+                if pi == torch.float64 and k_max <= _SYM16_MAX_K:
+                    # blocks of 16 right-hand sides on the same tiles (dds_sym_mfma_apply16):
+                    # X packed per (64-row block, column): scales and 16-bit terms
+                    self._msym = dict(
+                        bp=torch.zeros(n_blk * 2 * nt * 512, dtype=store_dtype, device=dev),
+                        sigma=torch.empty((n_blk, _lib.NRHS), dtype=torch.float32, device=dev),
+                        Ym=torch.empty((int(sizes.sum()), _lib.NRHS), dtype=torch.float64, device=dev),
+                        order=torch.as_tensor(np.argsort(-sizes, kind="stable").astype(np.int32),
+                                              device=dev))
             alg = int((sizes.astype(np.int64) * (sizes + 1) // 2).sum()) * esz
         else:
             tiles = _tiles_for(sizes, rows_per_tile)
@@ -1812,14 +1827,31 @@ class SchwarzOperator(SparseSolver):
     # -- blocks of 16 right-hand sides ---------------------------------------------------
     @property
     def _block_apply(self) -> bool:
-        return self._mfma is not None and self.pi == torch.float64 and \
-            hasattr(self.coarse_solver, "_enqueue_solve_multi")
+        """Blocks of 16 right-hand sides run on the matrix cores (layout 'mfma', or
+        'sym_mfma' with k <= 1024)."""
+        return (self._mfma is not None or self._msym is not None) and \
+            self.pi == torch.float64 and hasattr(self.coarse_solver, "_enqueue_solve_multi")
+
+    @property
+    def _block_state(self) -> dict:
+        return self._mfma if self._mfma is not None else self._msym
 
     def _enqueue_local_multi(self, X, Y, st) -> None:
         """Y[sub_ptr[s] + a][:] = (A_s^-1 X_s)[a][:] for (n, 16) X (MFMA)."""
-        L, mf = self.local, self._mfma
+        L = self.local
         s = _lib.stream()
         dt = _lib.dtype_code(L.mat_dtype)
+        if self._mfma is None:  # the symmetric tiles
+            ms, xp = self._msym, self._xpack
+            _lib.call("dds_sym_mfma_pack16", dt, xp["n_blk"], xp["blk_owner"].data_ptr(),
+                      xp["blk_ptr"].data_ptr(), self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx),
+                      X.data_ptr(), ms["sigma"].data_ptr(), ms["bp"].data_ptr(), _lib.ptr(st), s)
+            _lib.call("dds_sym_mfma_apply16", dt, self.n_solvers, L.max_k, ms["order"].data_ptr(),
+                      self.sub_ptr.data_ptr(), xp["blk_ptr"].data_ptr(), L.mat_off.data_ptr(),
+                      _lib.ptr(L.scale), ms["sigma"].data_ptr(), L.mats.data_ptr(),
+                      ms["bp"].data_ptr(), Y.data_ptr(), _lib.ptr(st), s)
+            return
+        mf = self._mfma
         _lib.call("dds_mfma_pack_x", dt, self.n_solvers, self.sub_ptr.data_ptr(),
                   _lib.ptr(self.sub_idx), mf["b_off"].data_ptr(), X.data_ptr(),
                   mf["bp"].data_ptr(), mf["sigma"].data_ptr(), _lib.ptr(st), s)
@@ -1854,7 +1886,7 @@ class SchwarzOperator(SparseSolver):
         return self._mbuf
 
     def _overlap_sum_multi(self, Y, z, yc, sign, r, partials, st) -> int:
-        overl = self.dof_ptr is not None and Y is self._mfma["Ym"]
+        overl = self.dof_ptr is not None and Y is self._block_state["Ym"]
         return _lib.call_partials(
             "dds_overlap_sum_m", self.n, _lib.ptr(self.dof_ptr) if overl else None,
             _lib.ptr(self.dof_pos) if overl else None, Y.data_ptr(),
@@ -1864,7 +1896,7 @@ class SchwarzOperator(SparseSolver):
 
     def _local_multi_into(self, X, st):
         """Local part for a block: returns the tensor holding it in unknown order."""
-        Ym = self._mfma["Ym"]
+        Ym = self._block_state["Ym"]
         self._enqueue_local_multi(X, Ym, st)
         if self.dof_ptr is None:  # non-overlapping: Ym is already in unknown order
             return Ym
@@ -1998,7 +2030,7 @@ class AdditiveSchwarz(SchwarzOperator):
 def _additive_multi(self, X, Z, R, partials, st) -> int:
     mb = self._buffers_multi()
     s = _lib.stream()
-    Ym = self._mfma["Ym"]
+    Ym = self._block_state["Ym"]
     self._enqueue_local_multi(X, Ym, st)
     _lib.call("dds_restrict_m", self.n_coarse, self.coarse_ptr.data_ptr(), X.data_ptr(),
               mb["xc"].data_ptr(), _lib.ptr(st), s)

@@ -46,7 +46,7 @@
  *                             ordering and symbolic tables, host code
  *   dds_nd_factor             cudssExecute(FACTORIZATION) (cudss.cpp:97-105): the fronts of
  *                             one tree height
- *   dds_mfma_apply, dds_mfma_apply_sub
+ *   dds_mfma_apply, dds_mfma_apply_sub, dds_sym_mfma_apply16
  *                             the local apply to blocks of 16 right-hand sides (no counterpart:
  *                             the reference solves one right-hand side at a time)
  *   dds_pcg_*                 the CG.solve loop body (solvers.py:228-297): torch.linalg.norm,
@@ -516,7 +516,7 @@ int dds_overlap_sum_m(int64_t n, const int64_t* dof_ptr, const int32_t* dof_pos,
  * (subdomain, column) by its largest magnitude (sigma, n_s x 16 floats) and splits it into
  * 2 FP16 or 3 BF16 terms in B-operand order; dds_mfma_apply computes
  * Y[sub_ptr[s] + a][:] = scale[s] * sigma[s][:] * (A_s^-1 X_s)[a][:] with
- * v_mfma_f32_16x16x16 (FP32 accumulation), one wavefront per item (s, 64-row group),
+ * v_mfma_f32_16x16x32 (FP32 accumulation), one wavefront per item (s, 64-row group),
  * `grid` persistent workgroups of `waves` wavefronts (grid 0: one per item group). Replace
  * Inv.solve -> gemv_strided_batched (solvers.py:355-359) for stacked right-hand sides. */
 int dds_mfma_pack_x(int mat_dtype, int64_t n_s, const int64_t* sub_ptr, const int32_t* sub_idx,
@@ -534,6 +534,21 @@ int dds_mfma_apply_sub(int mat_dtype, int64_t n_s, const int32_t* order, const i
                        const int64_t* a_off, const int64_t* b_off, const float* scale,
                        const float* sigma, const void* A, const void* bp, double* Y, int waves,
                        int rpw, const dds_pcg_state* st, void* stream);
+/* The same product from the matrix-core symmetric store of dds_local_apply_sym_mfma (lower
+ * 64 x 64 tiles, half the bytes of the full inverses), for max_k <= 1024: one workgroup of 4
+ * wavefronts per subdomain order[b] reads each tile once for Y_I += T X_J and
+ * Y_J += T^T X_I. dds_sym_mfma_pack16 scales X per 64-row block b (blk_ptr / blk_owner as in
+ * dds_sym_mfma_pack) and column: sigma[b][c] a power of two above max |X_b[:, c]| (n_blk x 16
+ * floats), X_b / sigma[b] split into 2 FP16 or 3 BF16 terms (bp: n_blk x 2 x NT x 512
+ * elements); dds_sym_mfma_apply16 writes Y[sub_ptr[s] + a][:] = scale[s] (A_s^-1 X_s)[a][:]. */
+int dds_sym_mfma_pack16(int mat_dtype, int64_t n_blk, const int32_t* blk_owner,
+                        const int64_t* blk_ptr, const int64_t* sub_ptr, const int32_t* sub_idx,
+                        const double* x, float* sigma, void* bp, const dds_pcg_state* st,
+                        void* stream);
+int dds_sym_mfma_apply16(int mat_dtype, int64_t n_s, int64_t max_k, const int32_t* order,
+                         const int64_t* sub_ptr, const int64_t* blk_ptr, const int64_t* mat_off,
+                         const float* scale, const float* sigma, const void* mats,
+                         const void* bp, double* Y, const dds_pcg_state* st, void* stream);
 
 /* dst = (dst_dtype) src, F64 <-> F32 <-> F64 */
 int dds_cast(int src_dtype, int dst_dtype, int64_t n, const void* src, void* dst,

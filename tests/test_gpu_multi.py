@@ -245,3 +245,101 @@ def test_block_condition_number_estimate():
     _, meta = s.solve(B)
     assert len(meta["condition number estimate"]) == 5
     assert np.allclose(meta["condition number estimate"], 42.0)
+
+
+# -- 16 right-hand sides on the symmetric matrix-core tiles (dds_sym_mfma_apply16) ---------
+@functools.lru_cache(maxsize=None)
+def _kmeans13():
+    # the same grid in 13 k-means subdomains: 720-951 unknowns (k <= 1024 for this path)
+    return P.make_problem("p1_2d_kmeans", 96, overlap=2, seed=5, nrhs=16, n_clusters=13, jitter=0.2)
+
+
+def _pc_sym(kind, storage):
+    cls = dds.AdditiveSchwarz if kind == "additive" else dds.HybridSchwarz
+    return cls(torch.float64, dds.Inv(TD[storage], scaled=True, layout="sym_mfma",
+                                      accumulate=torch.float32), dds.DenseCholesky())
+
+
+def _stored_oracle(pb_A, d, kind, pc, spc=None):
+    return SchwarzOracle(pb_A, d.owner_ptr, d.sub_ptr, d.sub_idx,
+                         d.solvers_per_coarse if spc is None else spc, kind=kind,
+                         inverses=pc.stored_local_inverses())
+
+
+@pytest.mark.parametrize("storage", ["float16", "bfloat16"])
+@pytest.mark.parametrize("kind", ["additive", "hybrid"])
+def test_sym16_block_apply_matches_stored_inverses(kind, storage):
+    """The symmetric store applied to 16 columns: within 1e-5 per column of the FP64 product
+    of the very inverses it stores (16-bit terms of x / sigma, FP32 accumulation), the local
+    part alone and the whole preconditioner."""
+    pb = _kmeans13()
+    d = pb.decomposition
+    pc = _pc_sym(kind, storage)
+    pc.setup(_torch_csr(pb.A), decomposition=d)
+    assert pc.local.layout == "sym_mfma" and pc._msym is not None and pc._mfma is None
+    print(f"sym16 apply: {kind} {storage} k {np.diff(d.sub_ptr).min()}..{np.diff(d.sub_ptr).max()}",
+          flush=True)
+    R = np.random.default_rng(11).uniform(-1, 1, (pb.A.shape[0], 16))
+    R[:, 3] *= 1e-6  # a column far below the others: per-column scaling keeps its accuracy
+    Rt = torch.as_tensor(R, device=dev)
+    O = _stored_oracle(pb.A, d, kind, pc)
+    yl = pc._local_multi_into(Rt.clone(), None).cpu().numpy()
+    ylo = np.stack([O.local(R[:, j]) for j in range(16)], axis=1)
+    assert _colrel(yl, ylo).max() <= 1e-5
+    Z, _ = pc.solve(Rt)
+    Zo = np.stack([O.apply(R[:, j]) for j in range(16)], axis=1)
+    assert _colrel(Z.cpu().numpy(), Zo).max() <= 1e-5
+
+
+@pytest.mark.parametrize("case", ["dg24", "q1_boxes"])
+def test_sym16_small_and_partial_tiles(case):
+    """Subdomains of 3 unknowns (one partial 64-row block) and of 64 and fewer (boxes of the
+    non-overlapping Q1 grid): the padding rows and columns of the tiles stay inert."""
+    if case == "dg24":
+        A, b, perm, Ap, k, spc = dg24_permuted()
+        n = Ap.shape[0]
+        ptr = np.arange(0, n + 1, k)
+        pc = _pc_sym("hybrid", "float16")
+        pc.setup(_torch_csr(Ap), dofs_per_solver=k, solvers_per_coarse=torch.as_tensor(spc, device=dev))
+        O = SchwarzOracle(Ap, ptr, ptr, np.arange(n), spc, kind="hybrid",
+                          inverses=pc.stored_local_inverses())
+    else:
+        pb = _q1_nonoverlap()
+        Ap, d = pb.A, pb.decomposition
+        n = Ap.shape[0]
+        pc = _pc_sym("hybrid", "bfloat16")
+        pc.setup(_torch_csr(Ap), decomposition=d)
+        O = _stored_oracle(Ap, d, "hybrid", pc)
+    assert pc._msym is not None
+    R = np.random.default_rng(1).uniform(-1, 1, (n, 16))
+    Z, _ = pc.solve(torch.as_tensor(R, device=dev))
+    Zo = np.stack([O.apply(R[:, j]) for j in range(16)], axis=1)
+    assert _colrel(Z.cpu().numpy(), Zo).max() <= 1e-5
+
+
+def test_sym16_block_cg_and_graph_replay():
+    """CG over a block of 13 columns through the symmetric store: per column the oracle CG's
+    iteration count (+-2) with the stored inverses, the solution within 1e-6, the true FP64
+    residual below rtol; a graph replay gives the eager bits; a single vector keeps the
+    single-vector path."""
+    pb = _kmeans13()
+    d = pb.decomposition
+    B = np.ascontiguousarray(pb.b[:, :13])
+    out = []
+    for use_graph in (False, True):
+        s = dds.CG(_pc_sym("hybrid", "float16"), rtol=1e-8, bsr_matmul=False, check_every=4,
+                   use_graph=use_graph)
+        s.setup(_torch_csr(pb.A), decomposition=d)
+        X, meta = s.solve(torch.as_tensor(B, device=dev))
+        out.append((X, meta))
+    assert torch.equal(out[0][0], out[1][0])
+    assert out[0][1]["iterations"] == out[1][1]["iterations"]
+    X, meta = out[0][0].cpu().numpy(), out[0][1]
+    O = _stored_oracle(pb.A, d, "hybrid", s.preconditioner)
+    for j in range(13):
+        xo, mo = oracle_cg(lambda v: pb.A @ v, B[:, j], precond=O.apply, T0=O.T0, rtol=1e-8)
+        assert abs(meta["iterations"][j] - mo["iterations"]) <= 2, (j, meta["iterations"][j], mo["iterations"])
+        assert np.linalg.norm(X[:, j] - xo) / np.linalg.norm(xo) <= 1e-6
+        assert np.linalg.norm(B[:, j] - pb.A @ X[:, j]) < 1.01e-8 * np.linalg.norm(B[:, j])
+    x1, m1 = s.solve(torch.as_tensor(B[:, 0].copy(), device=dev))
+    assert x1.dim() == 1 and abs(m1["iterations"] - meta["iterations"][0]) <= 2

@@ -123,6 +123,12 @@ def test_host_validation_without_gpu():
         _lib.call("dds_mfma_apply_sub", _lib.F16, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 8, 3, None, None)
     with pytest.raises(_lib.HipError, match="F16 or BF16"):
         _lib.call("dds_mfma_apply_sub", _lib.F64, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 8, 4, None, None)
+    with pytest.raises(_lib.HipError, match="max_k 1025 exceeds 1024"):
+        _lib.call("dds_sym_mfma_apply16", _lib.F16, 4, 1025, 1, 1, 1, 1, None, 1, 1, 1, 1, None, None)
+    with pytest.raises(_lib.HipError, match="F16 or BF16"):
+        _lib.call("dds_sym_mfma_apply16", _lib.F64, 4, 64, 1, 1, 1, 1, None, 1, 1, 1, 1, None, None)
+    with pytest.raises(_lib.HipError, match="F16 or BF16"):
+        _lib.call("dds_sym_mfma_pack16", _lib.F32, 4, 1, 1, 1, None, 1, 1, 1, None, None)
     with pytest.raises(_lib.HipError, match="F16 or BF16"):
         _lib.call("dds_mfma_apply", _lib.F64, 4, 1, 1, 1, 1, None, 1, 1, 1, 1, 0, 4, None, None)
     with pytest.raises(_lib.HipError, match="F16 or BF16"):
