@@ -76,56 +76,89 @@ def make_solver(pb, storage, dev, bsr=None, scaled=False, use_graph=True, accumu
     return s
 
 
-def make_block_solver(pb, storage, dev, kind="hybrid", use_graph=True, autotune=True):
-    """CG over blocks of 16 right-hand sides, local inverses as an MFMA batched GEMM."""
+def make_block_solver(pb, storage, dev, kind="hybrid", use_graph=True, autotune=True, layout=None):
+    """CG over blocks of 16 right-hand sides on the matrix cores: the symmetric tiles
+    (layout 'sym_mfma', half the bytes) where every subdomain has at most 1024 unknowns,
+    else the full inverses as an MFMA batched GEMM (layout 'mfma', autotuned)."""
+    import numpy as np
     import torch
     import dd_solvers as dds
 
     A = torch.sparse_csr_tensor(torch.as_tensor(pb.A.indptr), torch.as_tensor(pb.A.indices),
                                 torch.as_tensor(pb.A.data), pb.A.shape).to(dev)
     cls = dds.HybridSchwarz if kind == "hybrid" else dds.AdditiveSchwarz
-    s = dds.CG(cls(torch.float64, dds.Inv(storage, scaled=True, layout="mfma", autotune=autotune),
-                   dds.DenseCholesky()),
+    if layout is None:
+        layout = "sym_mfma" if int(np.diff(pb.decomposition.sub_ptr).max()) <= 1024 else "mfma"
+    inv = dds.Inv(storage, scaled=True, layout="sym_mfma", accumulate=torch.float32) \
+        if layout == "sym_mfma" else dds.Inv(storage, scaled=True, layout="mfma", autotune=autotune)
+    s = dds.CG(cls(torch.float64, inv, dds.DenseCholesky()),
                rtol=1e-8, maxiter=2000, bsr_matmul=False, check_every=8, use_graph=use_graph)
     t = time.perf_counter()
     s.setup(A, decomposition=pb.decomposition)
     torch.cuda.synchronize()
     tn = getattr(s.preconditioner, "tuning", None)
-    log(f"setup block storage={storage}: {time.perf_counter() - t:.1f}s"
+    log(f"setup block storage={storage} layout={layout}: {time.perf_counter() - t:.1f}s"
         + (f", tuned grid={tn['grid']} waves={tn['waves']} rpw={tn['rpw']} ({tn['ms'] * 1e3:.1f} us)"
            if tn else ""))
     return s
 
 
 def mfma_kernel_stats(M):
-    """Event-timed average of one dds_mfma_apply launch, with its algorithmic bytes
-    (inverses + packed right-hand sides + FP64 result) and executed MFMA flops."""
+    """Event-timed average of one launch of the block apply alone (dds_mfma_apply, or
+    dds_sym_mfma_apply16 on the symmetric tiles), with its algorithmic bytes (stored
+    inverses + packed right-hand sides and scales + FP64 result) and MFMA flops (2 k^2 16 NT
+    per subdomain: the symmetric tiles are used twice), and the pack beside it."""
     import numpy as np
     import torch
     from dd_solvers import _lib
 
-    L, mf = M.local, M._mfma
+    L = M.local
+    sym = M._mfma is None
+    mf = M._msym if sym else M._mfma
     sizes = M._host["sizes"].astype(np.int64)
     X = torch.rand((M.n, _lib.NRHS), dtype=torch.float64, device=M.device) - 0.5
     M._enqueue_local_multi(X, mf["Ym"], None)
     dt = _lib.dtype_code(L.mat_dtype)
     s = _lib.stream()
-
-    def launch():
-        M._enqueue_mfma_apply(mf["Ym"], None)
-
-    ms = kernel_time_ms(launch)
     nt = 2 if L.mat_dtype == torch.float16 else 3
     esz = L.mats.element_size()
-    alg = int(L.bytes_algorithmic + mf["bp"].numel() * esz + sizes.sum() * 16 * 8)
+    out = {}
+    if sym:
+        xp = M._xpack
+
+        def launch():
+            _lib.call("dds_sym_mfma_apply16", dt, M.n_solvers, L.max_k, mf["order"].data_ptr(),
+                      M.sub_ptr.data_ptr(), xp["blk_ptr"].data_ptr(), L.mat_off.data_ptr(),
+                      _lib.ptr(L.scale), mf["sigma"].data_ptr(), L.mats.data_ptr(),
+                      mf["bp"].data_ptr(), mf["Ym"].data_ptr(), None, s)
+
+        def pack():
+            _lib.call("dds_sym_mfma_pack16", dt, xp["n_blk"], xp["blk_owner"].data_ptr(),
+                      xp["blk_ptr"].data_ptr(), M.sub_ptr.data_ptr(), _lib.ptr(M.sub_idx),
+                      X.data_ptr(), mf["sigma"].data_ptr(), mf["bp"].data_ptr(), None, s)
+
+        kname = "sym16_apply_kernel"
+        tp = kernel_time_ms(pack)
+        pack_alg = int(sizes.sum() * 16 * 8 + mf["bp"].numel() * esz + mf["sigma"].numel() * 4
+                       + (sizes.sum() * 4 if M.sub_idx is not None else 0))
+        out["pack"] = {"kernel": f"sym16_pack_kernel<{'_Float16' if nt == 2 else 'bf16_t'}>",
+                       "avg_launch_us": round(tp * 1e3, 2), "alg_bytes_per_launch": pack_alg,
+                       "GBs": round(pack_alg / (tp * 1e-3) / 1e9, 1)}
+    else:
+        def launch():
+            M._enqueue_mfma_apply(mf["Ym"], None)
+
+        kname = "mfma_sub_kernel" if int(mf["grid"]) < 0 else "mfma_apply_kernel"
+        out.update(grid=int(mf["grid"]), waves=int(mf["waves"]), rpw=int(mf["rpw"]))
+    ms = kernel_time_ms(launch)
+    alg = int(L.bytes_algorithmic + mf["bp"].numel() * esz + mf["sigma"].numel() * 4
+              + sizes.sum() * 16 * 8)
     flops = float(2 * (sizes ** 2).sum() * 16 * nt)
-    kname = "mfma_sub_kernel" if int(mf["grid"]) < 0 else "mfma_apply_kernel"
-    return {"kernel": f"{kname}<{'_Float16' if nt == 2 else 'bf16_t'}>",
+    return {"kernel": f"{kname}<{'_Float16' if nt == 2 else 'bf16_t'}>", "layout": L.layout,
             "avg_launch_us": round(ms * 1e3, 2), "alg_bytes_per_launch": alg,
             "GBs": round(alg / (ms * 1e-3) / 1e9, 1),
             "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "mfma_TFLOPs": round(flops / (ms * 1e-3) / 1e12, 1),
-            "grid": int(mf["grid"]), "waves": int(mf["waves"]), "rpw": int(mf["rpw"])}
+            "mfma_TFLOPs": round(flops / (ms * 1e-3) / 1e12, 1), **out}
 
 
 def batched_extra(pb, storage, dev, steps, warmup, seed, kind="hybrid", fp64_tts=None):
