@@ -927,15 +927,16 @@ def profile_kernel_only(args, dev):
                "factor_bytes": 8 * int(np.sum(plan.fnj * plan.fnj + 2 * plan.fns * plan.fnj))}
         print(json.dumps(out))
         return
-    if args.layout == "mfma":
-        solver = make_block_solver(pb, st, dev, use_graph=False, autotune=False)
+    if args.layout in ("mfma", "sym16"):
+        solver = make_block_solver(pb, st, dev, use_graph=False, autotune=False,
+                                   layout="mfma" if args.layout == "mfma" else "sym_mfma")
     else:
         acc = torch.float32 if st in (torch.float16, torch.bfloat16, torch.float32) else None
         solver = make_solver(pb, st, dev, scaled=st == torch.float16, accumulate=acc)
     M = solver.preconditioner
     L = M.local
     xv = torch.randn(pb.A.shape[0], dtype=torch.float64, device=dev)
-    if args.layout == "mfma":
+    if args.layout in ("mfma", "sym16"):
         info = mfma_kernel_stats(M)  # 3 + 20 launches, event-timed
     else:
         info = local_kernel_stats(M)  # 3 + 20 launches of each kernel, event-timed
@@ -979,9 +980,11 @@ def main():
                     help="after setup only launch the local apply, the SpMV and a 1 GiB device "
                          "copy, 20 times each (for rocprofv3 --pmc byte counters)")
     ap.add_argument("--storage", default="float16", choices=["float64", "float32", "float16", "bfloat16"])
-    ap.add_argument("--layout", default="sym", choices=["sym", "mfma", "sparse"],
-                    help="--profile-kernel: the local-inverse layout to launch (sparse: the "
-                         "sweeps of the sparse local factors, BatchSparseCholesky)")
+    ap.add_argument("--layout", default="sym", choices=["sym", "mfma", "sym16", "sparse"],
+                    help="--profile-kernel: the local-inverse layout to launch (mfma: 16 "
+                         "right-hand sides on the full inverses; sym16: 16 on the symmetric "
+                         "tiles; sparse: the sweeps of the sparse local factors, "
+                         "BatchSparseCholesky)")
     ap.add_argument("--solve-only", default=None, choices=["fp64", "fp16"],
                     help="only set up and time this form of the headline workload (for "
                          "rocprofv3 --kernel-trace runs)")
