@@ -957,8 +957,23 @@ __global__ __launch_bounds__(256) void sym16_apply_kernel(
   const uint32_t vsw = 16u * (uint32_t)symm_slot(lane);  // tile blocks: this lane's slot
   const uint32_t vnat = 16u * (uint32_t)lane;            // packed X: lane order
 
+  // Tile rows in increasing order, even rows J = 0 .. I and odd rows J = I .. 0, so that a
+  // row starts with the blocks of X the previous one used last (still in L2).
+  auto row_first = [](int I, int J) { return J == ((I & 1) ? I : 0); };
+  auto row_last = [](int I, int J) { return J == ((I & 1) ? 0 : I); };
+  auto advance = [](int& I, int& J) __attribute__((always_inline)) {
+    if (I & 1) {
+      if (--J < 0) {
+        ++I;
+        J = 0;
+      }
+    } else if (++J > I) {
+      J = ++I;
+    }
+  };
   // wavefront w's quarter of the loads of tile (I, J) into stage t % NS: tile blocks 2w and
-  // 2w + 1, then chunks c = w NT .. w NT + NT - 1 of [X_J (2 NT chunks), X_I (2 NT chunks)]
+  // 2w + 1, then chunks c = w NT .. w NT + NT - 1 of [X_J (2 NT chunks), X_I (2 NT chunks,
+  // with the first tile of a row)]
   auto issue = [&](int t, int I, int J, bool real) __attribute__((always_inline)) {
     const uint32_t sa = stage0 + (uint32_t)(t % NS) * SB;
 #pragma unroll
@@ -974,16 +989,10 @@ __global__ __launch_bounds__(256) void sym16_apply_kernel(
                    real ? (uint32_t)((2 * J * NT + c) * 1024) : kOob);
       else
         dma16_keep(sa + (uint32_t)(SS::XI + (c - 2 * NT) * 1024), vnat, rx,
-                   real && J == 0 ? (uint32_t)((2 * I * NT + c - 2 * NT) * 1024) : kOob);
+                   real && row_first(I, J) ? (uint32_t)((2 * I * NT + c - 2 * NT) * 1024) : kOob);
     }
   };
   int iI = 0, iJ = 0;  // issue cursor
-  auto advance = [](int& I, int& J) __attribute__((always_inline)) {
-    if (++J > I) {
-      J = 0;
-      ++I;
-    }
-  };
 #pragma unroll
   for (int t = 0; t < NF; ++t) {
     issue(t, iI, iJ, t < ntot);
@@ -1005,7 +1014,7 @@ __global__ __launch_bounds__(256) void sym16_apply_kernel(
     wait_vm<SS::LOADS * NF>();  // this wavefront's loads of tile t are in
     lds_barrier();              // and every wavefront's
     const unsigned char* sb = s16_smem + (t % NS) * SB;
-    if (cJ == 0) {
+    if (row_first(cI, cJ)) {
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -1033,7 +1042,7 @@ __global__ __launch_bounds__(256) void sym16_apply_kernel(
       for (int u = 0; u < NT; ++u) d = S::mfma(a, xi[h][u], d);
     }
     add_at(yacc, cJ, sI * d);
-    if (cJ == cI) add_at(yacc, cI, racc);
+    if (row_last(cI, cJ)) add_at(yacc, cI, racc);
     advance(cI, cJ);
   }
   wait_vm<0>();  // no load into LDS outlives the workgroup
