@@ -726,6 +726,104 @@ def test_symm_schedule(NS, NT):
                 _symm_schedule(k, G, q, NS, NT)
 
 
+def _sym16_schedule(k, NF, NT):
+    """Restates the control flow of sym16_apply_kernel for one subdomain of k unknowns (4
+    wavefronts, NS = NF + 2 LDS stages, one barrier per tile) and checks, with each
+    wavefront's vector memory operations completing in issue order: every wavefront issues
+    the same loads per tile (2 tile blocks, NT of the 4 NT packed-X chunks) and its wait
+    count (LPT NF) leaves exactly the loads of the tiles after the awaited one in flight; the
+    scales are in before the first tile; the four quarters of a stage are the tile computed,
+    its X_J and, at the first tile of a row, its X_I; a stage is refilled only after the
+    barrier that follows its last read; the walk (odd rows right to left) reads every tile
+    (I, J <= I) once and starts and ends every row once."""
+    T = -(-k // 64)
+    NS = NF + 2
+    LPT = 2 + NT
+    ntot = T * (T + 1) // 2
+
+    def first(I, J):
+        return J == (I if I & 1 else 0)
+
+    def last(I, J):
+        return J == (0 if I & 1 else I)
+
+    def advance(I, J):
+        if I & 1:
+            J -= 1
+            if J < 0:
+                I, J = I + 1, 0
+        else:
+            J += 1
+            if J > I:
+                I, J = I + 1, I + 1
+        return I, J
+
+    queues = [[("sig",)] for _ in range(4)]  # per wavefront, in issue order
+    stage = {}          # stage -> (tile index, (I, J)) it holds
+    last_read = {}      # stage -> tile index computed from it
+    barrier = -1        # the last barrier passed: all computes before it are done
+
+    def issue(t, c):
+        real = t < ntot
+        st = t % NS
+        if st in last_read:  # refilled only after the barrier following its last read
+            assert last_read[st] < barrier, (t, st, last_read[st], barrier)
+        for w in range(4):
+            q = queues[w]
+            q += [("tile", t, (c, 2 * w + u) if real else None) for u in range(2)]
+            for u in range(NT):
+                ch = w * NT + u
+                if ch < 2 * NT:
+                    q.append(("x", t, ("J", c[1], ch) if real else None))
+                else:
+                    q.append(("x", t, ("I", c[0], ch - 2 * NT) if real and first(*c) else None))
+        stage[st] = (t, c if real else None)
+        return advance(*c) if real else c
+
+    ic = (0, 0)
+    for t in range(NF):
+        ic = issue(t, ic)
+    cc = (0, 0)
+    seen, firsts, lasts = [], [], []
+    for t in range(ntot):
+        ic = issue(t + NF, ic)
+        got = {}
+        for w in range(4):
+            q = queues[w]
+            mine = [i for i, e in enumerate(q) if len(e) > 1 and e[1] == t]
+            assert len(mine) == LPT
+            assert len(q) - 1 - max(mine) == LPT * NF  # the wait leaves only later tiles' loads
+            if t == 0:
+                assert max(i for i, e in enumerate(q) if e == ("sig",)) < min(mine)
+            for i in mine:
+                got.setdefault(q[i][0], []).append(q[i][2])
+        barrier = t  # every wavefront's loads of t are in, every compute of t - 1 is done
+        assert stage[t % NS] == (t, cc)
+        assert sorted(b for _, b in got["tile"]) == list(range(8))
+        assert all(c == cc for c, _ in got["tile"])
+        xs = [x for x in got["x"] if x is not None]
+        assert sorted(ch for kind, _, ch in xs if kind == "J") == list(range(2 * NT))
+        assert all(blk == cc[1] for kind, blk, _ in xs if kind == "J")
+        xi = sorted(ch for kind, blk, ch in xs if kind == "I" and blk == cc[0])
+        assert xi == (list(range(2 * NT)) if first(*cc) else [])
+        last_read[t % NS] = t
+        seen.append(cc)
+        if first(*cc):
+            firsts.append(cc[0])
+        if last(*cc):
+            lasts.append(cc[0])
+        cc = advance(*cc)
+    assert sorted(seen) == sorted((I, J) for I in range(T) for J in range(I + 1))
+    assert firsts == list(range(T)) and lasts == list(range(T))
+    assert [I for I, _ in seen] == sorted(I for I, _ in seen)  # rows in increasing order
+
+
+@pytest.mark.parametrize("NF,NT", [(1, 2), (1, 3), (2, 2), (2, 3)])
+def test_sym16_schedule(NF, NT):
+    for k in (1, 63, 64, 65, 128, 129, 700, 1000, 1024):
+        _sym16_schedule(k, NF, NT)
+
+
 def test_symm_group_choice():
     from dd_solvers.solvers import _sym_plan, _symm_group, _sym_written_rows
     many = np.full(4096, 1000)
