@@ -201,7 +201,8 @@ def test_q1_3d_bsr_padded_tail(b):
 def test_setup_again_with_larger_problem(use_graph):
     """One CG + HybridSchwarz object set up on a small problem, solved, then set up on a
     larger one (larger n and n_c) and solved again, single right-hand side and blocks of
-    16: every buffer and captured graph follows the new sizes (results against the oracle)."""
+    16 (full-inverse MFMA store, then the symmetric matrix-core tiles): every buffer and
+    captured graph follows the new sizes (results against the oracle)."""
     s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(torch.float16, scaled=True, layout="mfma"),
                                  dds.DenseCholesky()), rtol=1e-8, bsr_matmul=False,
                check_every=4, use_graph=use_graph)
@@ -209,6 +210,26 @@ def test_setup_again_with_larger_problem(use_graph):
         pb = P.make_problem("q1_3d", N, box=box, overlap=1, seed=N, lognormal=True)
         d = pb.decomposition
         s.setup(_torch_csr(pb.A), decomposition=d)
+        O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid",
+                          inverses=s.preconditioner.stored_local_inverses())
+        xo, mo = oracle_cg(lambda v: pb.A @ v, pb.b, precond=O.apply, T0=O.T0, rtol=1e-8)
+        x, m = s.solve(torch.as_tensor(pb.b, device=dev))
+        assert abs(m["iterations"] - mo["iterations"]) <= 2
+        assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-6 * np.linalg.norm(xo)
+        B = np.random.default_rng(N).uniform(-1, 1, (pb.A.shape[0], 16))
+        X, mb = s.solve(torch.as_tensor(B, device=dev))
+        res = np.linalg.norm(B - pb.A @ X.cpu().numpy(), axis=0) / np.linalg.norm(B, axis=0)
+        assert res.max() < 1.01e-8
+    # the symmetric matrix-core tiles (one vector and blocks of 16 on the same store)
+    s = dds.CG(dds.HybridSchwarz(torch.float64, dds.Inv(torch.float16, scaled=True, layout="sym_mfma",
+                                                        accumulate=torch.float32),
+                                 dds.DenseCholesky()), rtol=1e-8, bsr_matmul=False,
+               check_every=4, use_graph=use_graph)
+    for N, box in ((10, 3), (16, 4)):
+        pb = P.make_problem("q1_3d", N, box=box, overlap=1, seed=N, lognormal=True)
+        d = pb.decomposition
+        s.setup(_torch_csr(pb.A), decomposition=d)
+        assert s.preconditioner._msym is not None
         O = SchwarzOracle(pb.A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse, kind="hybrid",
                           inverses=s.preconditioner.stored_local_inverses())
         xo, mo = oracle_cg(lambda v: pb.A @ v, pb.b, precond=O.apply, T0=O.T0, rtol=1e-8)
