@@ -343,3 +343,37 @@ def test_sym16_block_cg_and_graph_replay():
         assert np.linalg.norm(B[:, j] - pb.A @ X[:, j]) < 1.01e-8 * np.linalg.norm(B[:, j])
     x1, m1 = s.solve(torch.as_tensor(B[:, 0].copy(), device=dev))
     assert x1.dim() == 1 and abs(m1["iterations"] - meta["iterations"][0]) <= 2
+
+
+def test_sym16_uneven_sizes_up_to_1024():
+    """Subdomains of 1, 7, 63, 64, 65, 129, 1000 and 1024 unknowns (one node of overlap on
+    each side) of a 1-D SPD operator: every last-tile shape (rows and column pairs of the
+    partial 64-row block) and the largest subdomain the path takes, against the oracle's
+    FP64 product with the stored inverses."""
+    sizes = [1024, 65, 1, 1000, 7, 64, 129, 63]
+    n = int(sum(sizes))
+    main = 2.0 + np.random.default_rng(5).uniform(0.1, 1.0, n)
+    A = sps.diags([-np.ones(n - 1), main, -np.ones(n - 1)], [-1, 0, 1], format="csr")
+    owner_ptr = np.concatenate([[0], np.cumsum(sizes)])
+    subs = [np.arange(max(0, a - 1), min(n, b + 1)) for a, b in zip(owner_ptr[:-1], owner_ptr[1:])]
+    sub_ptr = np.concatenate([[0], np.cumsum([len(x) for x in subs])])
+    d = P.Decomposition(perm=np.arange(n), owner_ptr=owner_ptr, sub_ptr=sub_ptr,
+                        sub_idx=np.concatenate(subs), solvers_per_coarse=np.array([4, 4]))
+    k_max = int(np.diff(sub_ptr).max())
+    assert k_max > 1024  # the overlap makes the largest 1026: the block path is off
+    pc = _pc_sym("additive", "float16")
+    pc.setup(_torch_csr(A), decomposition=d)
+    assert pc._msym is None
+    # the same sizes without overlap reach exactly 1024
+    d = P.Decomposition(perm=np.arange(n), owner_ptr=owner_ptr, sub_ptr=owner_ptr,
+                        sub_idx=np.arange(n), solvers_per_coarse=np.array([4, 4]))
+    for storage in ("float16", "bfloat16"):
+        pc = _pc_sym("additive", storage)
+        pc.setup(_torch_csr(A), decomposition=d)
+        assert pc._msym is not None and pc.local.max_k == 1024
+        R = np.random.default_rng(2).uniform(-1, 1, (n, 16))
+        yl = pc._local_multi_into(torch.as_tensor(R, device=dev), None).cpu().numpy()
+        O = SchwarzOracle(A, d.owner_ptr, d.sub_ptr, d.sub_idx, d.solvers_per_coarse,
+                          kind="additive", inverses=pc.stored_local_inverses())
+        ylo = np.stack([O.local(R[:, j]) for j in range(16)], axis=1)
+        assert _colrel(yl, ylo).max() <= 1e-5, storage
