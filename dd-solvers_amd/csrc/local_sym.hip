@@ -1059,7 +1059,7 @@ __global__ __launch_bounds__(256) void sym16_apply_kernel(
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = 64 * J + 16 * w + 4 * (lane >> 4) + i;
-        if (row < k) Ys[(int64_t)row * 16 + c] = (double)yacc[J][i] * sg;
+        if (row < k) __builtin_nontemporal_store((double)yacc[J][i] * sg, Ys + (int64_t)row * 16 + c);
       }
     }
   }

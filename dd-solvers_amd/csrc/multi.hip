@@ -618,7 +618,7 @@ __device__ __forceinline__ void mfma_rows(int s, int R0, int lane, const int64_t
         double v = 0.0;
 #pragma unroll
         for (int t = NT - 1; t >= 0; --t) v += (double)c[r][t][i];
-        if (row < k) Y[(base + row) * M + col] = v * sg;
+        if (row < k) __builtin_nontemporal_store(v * sg, Y + (base + row) * M + col);
       }
     }
   }
