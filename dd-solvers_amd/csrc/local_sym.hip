@@ -736,21 +736,23 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
       for (int nb = 0; nb < 4; ++nb)
         dt[nb] = S::mfma(kr2 == 0 ? a0 : a1, symm_tr_operand<op_t>(sb, kr2, nb, lane), dt[nb]);
   };
-  // yv[64 J + lane] += sI x the transposed part (rows 16 nb + lane of D, the NT term rows
-  // added)
+  // yv[64 J + 16 nb + lane] += sI x the transposed part (rows 16 nb + lane of D, the NT term
+  // rows added), by lanes 0..15 themselves: the four reads, then the four writes (a wavefront's
+  // LDS operations run in order, so the row sums of add_row see these)
   auto add_transposed = [&](int J, float sI) __attribute__((always_inline)) {
     if (lane < 16) {
+      float* yJ = yv + 64 * J + lane;
+      float o[4];
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) o[nb] = yJ[16 * nb];
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) {
         float v = dt[nb][0];
 #pragma unroll
         for (int t = 1; t < NT; ++t) v += dt[nb][t];
-        yl[16 * nb + lane] = sI * v;
+        yJ[16 * nb] = o[nb] + sI * v;
       }
     }
-    wave_lds_fence();
-    yv[64 * J + lane] += yl[lane];
-    wave_lds_fence();
   };
   // yv[64 I + lane] += the row product (the NT term columns of D, lanes 16 g + t, added;
   // lane 16 g holds rows 16 Rb + 4 g + i)
