@@ -304,7 +304,7 @@ def local_kernel_stats(M):
             t_pack = kernel_time_ms(lambda: M._enqueue_sym_pack(xv, None))
             M._enqueue_sym_pack(xv, None)
             t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply_packed(None, Y))
-            alg = int(L.bytes_algorithmic + xbytes + yp_rows * 8)
+            alg = int(L.bytes_algorithmic + xbytes + yp_rows * Y.element_size())
             pack_alg = int(sizes.sum() * (8 + (4 if M.sub_idx is not None else 0)) + xbytes)
             pack = {"kernel": f"sym_pack_x_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}>",
                     "avg_launch_us": round(t_pack * 1e3, 2), "alg_bytes_per_launch": pack_alg,

@@ -144,10 +144,12 @@ __global__ __launch_bounds__(kBlock) void sum_kernel(const double* __restrict__ 
   if (threadIdx.x == 0) *out = sum;
 }
 
-template <typename T>
+// TY: the type of the local results Y (T, or float under FP64 vectors: the matrix-core apply
+// accumulates in FP32 and writes half the bytes); the sums run in T.
+template <typename T, typename TY>
 __global__ __launch_bounds__(kBlock) void overlap_sum_kernel(
     int64_t n, const int64_t* __restrict__ dof_ptr, const int32_t* __restrict__ dof_pos,
-    const T* Y, const int32_t* __restrict__ cell, const T* __restrict__ yc, T sign, T* z,
+    const TY* Y, const int32_t* __restrict__ cell, const T* __restrict__ yc, T sign, T* z,
     const double* __restrict__ r, double* __restrict__ partials,
     const dds_pcg_state* __restrict__ gate) {
   __shared__ double scratch[kBlock / 64];
@@ -158,9 +160,9 @@ __global__ __launch_bounds__(kBlock) void overlap_sum_kernel(
     T v;
     if (dof_ptr != nullptr) {
       v = 0;
-      for (int64_t p = dof_ptr[j]; p < dof_ptr[j + 1]; ++p) v += Y[dof_pos[p]];
+      for (int64_t p = dof_ptr[j]; p < dof_ptr[j + 1]; ++p) v += static_cast<T>(Y[dof_pos[p]]);
     } else {
-      v = Y[j];
+      v = static_cast<T>(Y[j]);
     }
     if (yc != nullptr) v += sign * yc[cell[j]];
     z[j] = v;
@@ -317,7 +319,8 @@ extern "C" int dds_sum_partials(const double* partials, int count, double* out, 
 }
 
 extern "C" int dds_overlap_sum(int dtype, int64_t n, const int64_t* dof_ptr,
-                               const int32_t* dof_pos, const void* Y, const int32_t* cell,
+                               const int32_t* dof_pos, const void* Y, int y_dtype,
+                               const int32_t* cell,
                                const void* yc, double coarse_sign, void* z, const double* r,
                                double* dot_partials, int* n_partials,
                                const dds_pcg_state* gate, void* stream) {
@@ -327,18 +330,23 @@ extern "C" int dds_overlap_sum(int dtype, int64_t n, const int64_t* dof_ptr,
   DDS_REQUIRE(!dot_partials || r, "dds_overlap_sum: partials need r");
   const int64_t g = vec_grid(n);
   hipStream_t s = as_stream(stream);
-  if (dtype == DDS_F64) {
-    hipLaunchKernelGGL(overlap_sum_kernel<double>, dim3((unsigned)g), dim3(kBlock), 0, s, n,
-                       dof_ptr, dof_pos, static_cast<const double*>(Y), cell,
+  if (dtype == DDS_F64 && y_dtype == DDS_F64) {
+    hipLaunchKernelGGL((overlap_sum_kernel<double, double>), dim3((unsigned)g), dim3(kBlock), 0, s,
+                       n, dof_ptr, dof_pos, static_cast<const double*>(Y), cell,
                        static_cast<const double*>(yc), coarse_sign, static_cast<double*>(z), r,
                        dot_partials, gate);
-  } else if (dtype == DDS_F32) {
-    hipLaunchKernelGGL(overlap_sum_kernel<float>, dim3((unsigned)g), dim3(kBlock), 0, s, n,
-                       dof_ptr, dof_pos, static_cast<const float*>(Y), cell,
+  } else if (dtype == DDS_F64 && y_dtype == DDS_F32) {
+    hipLaunchKernelGGL((overlap_sum_kernel<double, float>), dim3((unsigned)g), dim3(kBlock), 0, s,
+                       n, dof_ptr, dof_pos, static_cast<const float*>(Y), cell,
+                       static_cast<const double*>(yc), coarse_sign, static_cast<double*>(z), r,
+                       dot_partials, gate);
+  } else if (dtype == DDS_F32 && y_dtype == DDS_F32) {
+    hipLaunchKernelGGL((overlap_sum_kernel<float, float>), dim3((unsigned)g), dim3(kBlock), 0, s,
+                       n, dof_ptr, dof_pos, static_cast<const float*>(Y), cell,
                        static_cast<const float*>(yc), (float)coarse_sign,
                        static_cast<float*>(z), r, dot_partials, gate);
   } else {
-    set_error("dds_overlap_sum: unsupported dtype %d", dtype);
+    set_error("dds_overlap_sum: unsupported dtypes (vectors %d, local results %d)", dtype, y_dtype);
     return DDS_EUNSUPPORTED;
   }
   DDS_LAUNCHED();

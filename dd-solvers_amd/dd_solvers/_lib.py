@@ -72,7 +72,7 @@ _SIGNATURES = {
     "dds_sym_combine": [_i, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p],
     "dds_local_apply_sym_mfma_noalloc": [_i, _i, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p,
                                          _p, _p, _i64, _p, _p, _p, _p, _p],
-    "dds_overlap_sum": [_i, _i64, _p, _p, _p, _p, _p, _d, _p, _p, _p, _ip, _p, _p],
+    "dds_overlap_sum": [_i, _i64, _p, _p, _p, _i, _p, _p, _d, _p, _p, _p, _ip, _p, _p],
     "dds_restrict": [_i, _i64, _p, _p, _p, _i, _p, _p],
     "dds_chol_factor": [_i, _i64, _p, _i64, _p, _p],
     "dds_chol_solve": [_i, _i64, _p, _i64, _p, _p, _p, _p],

@@ -1642,6 +1642,10 @@ class SchwarzOperator(SparseSolver):
             direct = layout == "sym_mfma" and items.shape[0] == n_s
             # one item per subdomain writes the result itself (ypos = sub_ptr)
             self.Yp = None if direct else torch.empty(int(ypos[-1]), dtype=pi, device=dev)
+            if direct and self.Y is not None and pi == torch.float64:
+                # the overlapped local results in FP32 (the precision they are accumulated
+                # in): half the bytes written while the tiles stream in, summed in FP64
+                self.Y = torch.empty(self.Y.shape[0], dtype=torch.float32, device=dev)
             if layout == "sym_mfma":  # per apply: x packed per 64-row block (scales, terms)
                 nt = 2 if store_dtype == torch.float16 else 3
                 T = -(-sizes.astype(np.int64) // 64)
@@ -1733,7 +1737,7 @@ class SchwarzOperator(SparseSolver):
         L, xp = self.local, self._xpack
         out, pos = (Y, self.sub_ptr) if L.direct else (self.Yp, L.ypos)
         _lib.call("dds_sym_mfma_apply_packed", _lib.dtype_code(L.mat_dtype),
-                  _lib.dtype_code(self.pi), self.n_solvers, L.n_items, L.max_k, L.group,
+                  _lib.dtype_code(out.dtype), self.n_solvers, L.n_items, L.max_k, L.group,
                   L.items.data_ptr(), self.sub_ptr.data_ptr(), xp["blk_ptr"].data_ptr(),
                   L.mat_off.data_ptr(), pos.data_ptr(), _lib.ptr(L.scale), L.mats.data_ptr(),
                   xp["sig"].data_ptr(), xp["xt"].data_ptr(), out.data_ptr(), _lib.ptr(gate),
@@ -1747,6 +1751,10 @@ class SchwarzOperator(SparseSolver):
             if L.direct and Y is None:
                 raise ValueError("a direct sym_mfma apply writes the result: pass Y")
             out, pos = (Y, self.sub_ptr) if L.direct else (self.Yp, L.ypos)
+            if out.dtype != self.pi:  # FP32 results under FP64 vectors: the two halves
+                self._enqueue_sym_pack(x, gate)
+                self._enqueue_sym_apply_packed(gate, Y)
+                return
             xp = self._xpack
             _lib.call("dds_local_apply_sym_mfma", _lib.dtype_code(L.mat_dtype),
                       _lib.dtype_code(self.pi), self.n_solvers, L.n_items, L.max_k, L.group,
@@ -1774,7 +1782,8 @@ class SchwarzOperator(SparseSolver):
         return _lib.call_partials(
             "dds_overlap_sum", _lib.dtype_code(self.pi), self.n,
             _lib.ptr(self.dof_ptr) if overl else None, _lib.ptr(self.dof_pos) if overl else None,
-            Y.data_ptr(), _lib.ptr(self.cell) if yc is not None else None, _lib.ptr(yc),
+            Y.data_ptr(), _lib.dtype_code(Y.dtype),
+            _lib.ptr(self.cell) if yc is not None else None, _lib.ptr(yc),
             float(sign), z.data_ptr(), _lib.ptr(r), _lib.ptr(partials), _lib.PARTIALS_OUT,
             _lib.ptr(gate), _lib.stream())
 

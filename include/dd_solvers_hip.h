@@ -265,12 +265,14 @@ int dds_sym_combine(int dtype, int64_t n_s, int64_t max_k, int64_t pairs_per_ite
 
 /* z[j] = sum_{p in [dof_ptr[j], dof_ptr[j+1])} Y[dof_pos[p]]   (dof_ptr == NULL: z[j] = Y[j])
  *        + coarse_sign * yc[cell[j]]                           (yc != NULL)
- * in dtype (F64 or F32). If r != NULL (F64 only) per-block partials of r . z are written.
- * z may alias Y when dof_ptr == NULL. */
+ * in dtype (F64 or F32). Y is of y_dtype: dtype, or F32 under F64 vectors (the local results
+ * of the matrix-core apply, accumulated in FP32), summed in dtype. If r != NULL (F64 only)
+ * per-block partials of r . z are written. z may alias Y when dof_ptr == NULL and the two
+ * dtypes agree. */
 int dds_overlap_sum(int dtype, int64_t n, const int64_t* dof_ptr, const int32_t* dof_pos,
-                    const void* Y, const int32_t* cell, const void* yc, double coarse_sign,
-                    void* z, const double* r, double* dot_partials, int* n_partials,
-                    const dds_pcg_state* gate, void* stream);
+                    const void* Y, int y_dtype, const int32_t* cell, const void* yc,
+                    double coarse_sign, void* z, const double* r, double* dot_partials,
+                    int* n_partials, const dds_pcg_state* gate, void* stream);
 
 /* xc[c] = sum_{j in [coarse_ptr[c], coarse_ptr[c+1])} x[j]: lanes = 0, one workgroup per
  * cell (large cells); lanes = 8, 16, 32 or 64, that many lanes per cell (small cells). */

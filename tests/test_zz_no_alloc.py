@@ -28,8 +28,10 @@ def _setup(monkeypatch, N, box, overlap, storage, direct):
     M.setup(A, decomposition=pb.decomposition)
     assert M.local.layout == "sym_mfma" and M.local.direct == direct
     x = torch.as_tensor(np.random.default_rng(N).standard_normal(pb.A.shape[0]), device=dev)
-    Y = M.Y if M.Y is not None else torch.empty_like(x)
-    return M, x, torch.empty_like(Y)
+    # the entry points write the vectors' dtype (FP64), whatever the dtype the preconditioner
+    # keeps its own local results in
+    Y = torch.empty(M.Y.shape[0] if M.Y is not None else x.shape[0], dtype=torch.float64, device=dev)
+    return M, x, Y
 
 
 def _noalloc(M, x, Y, n_blk=None):
