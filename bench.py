@@ -301,15 +301,28 @@ def local_kernel_stats(M):
             xp = M._xpack
             nt = 2 if L.mat_dtype == torch.float16 else 3
             xbytes = xp["n_blk"] * (nt * 128 + 4)
+            gx = int(sizes.sum() * (8 + (4 if M.sub_idx is not None else 0)))  # gathered x
             t_pack = kernel_time_ms(lambda: M._enqueue_sym_pack(xv, None))
             M._enqueue_sym_pack(xv, None)
-            t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply_packed(None, Y))
-            alg = int(L.bytes_algorithmic + xbytes + yp_rows * Y.element_size())
-            pack_alg = int(sizes.sum() * (8 + (4 if M.sub_idx is not None else 0)) + xbytes)
+            t_ap = kernel_time_ms(lambda: M._enqueue_sym_apply_packed(None, Y))
+            ap_alg = int(L.bytes_algorithmic + xbytes + yp_rows * Y.element_size())
             pack = {"kernel": f"sym_pack_x_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}>",
-                    "avg_launch_us": round(t_pack * 1e3, 2), "alg_bytes_per_launch": pack_alg,
-                    "GBs": round(pack_alg / (t_pack * 1e-3) / 1e9, 1)}
-            kname = f"sym_mfma_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[M.pi]}>"
+                    "avg_launch_us": round(t_pack * 1e3, 2), "alg_bytes_per_launch": gx + int(xbytes),
+                    "GBs": round((gx + xbytes) / (t_pack * 1e-3) / 1e9, 1)}
+            kname = f"sym_mfma_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[Y.dtype]}>"
+            if L.mat_dtype == torch.float16 and M.fused_pack:
+                # what the solve runs: one kernel packing x in its LDS (gathered x and the
+                # lower triangles in, the result out); the pack and the apply from a packed
+                # x, launched apart, are reported beside it
+                pack["apply_packed_us"] = round(t_ap * 1e3, 2)
+                pack["apply_packed_alg_bytes"] = ap_alg
+                pack["apply_packed_GBs"] = round(ap_alg / (t_ap * 1e-3) / 1e9, 1)
+                t_loc = kernel_time_ms(lambda: M._enqueue_sym_apply(xv, None, Y))
+                alg = int(L.bytes_algorithmic + gx + yp_rows * Y.element_size())
+                kname = (f"sym_mfma_apply_kernel<{_TNAME[L.mat_dtype]}, {_TNAME[Y.dtype]}, "
+                         f"true, {_TNAME[M.pi]}>")
+            else:
+                t_loc, alg = t_ap, ap_alg
     else:
         t_loc = kernel_time_ms(lambda: M._enqueue_local(xv, Y, None))
         t_comb, comb = 0.0, 0

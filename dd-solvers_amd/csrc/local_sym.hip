@@ -503,6 +503,49 @@ __device__ __forceinline__ int64_t symm_tile_off(int I, int J, int T, int nbL) {
 // distinct 16-byte bank groups. The map is its own inverse.
 __device__ __forceinline__ int symm_slot(int c) { return c ^ (((c >> 4) & 1) << 2); }
 
+// The x terms and scales of blocks 0 .. nb-1 of a subdomain (k unknowns from base, gathered
+// through sub_idx when it is given) computed in place into LDS, in the layout and with the
+// arithmetic of sym_pack_blocks (the same bits): per block, the power-of-two scale above its
+// largest magnitude, and the NT 16-bit terms of each scaled value.
+template <typename TM, typename TX>
+__device__ __forceinline__ void symm_pack_lds(int nb, int k, int64_t base,
+                                              const int32_t* __restrict__ sub_idx,
+                                              const TX* __restrict__ x, int lane,
+                                              uint16_t* xtl, float* sig) {
+  using S = SymMfma<TM>;
+  constexpr int NT = S::NT;
+  constexpr int U = 8;  // blocks whose loads are in flight together
+  for (int b0 = 0; b0 < nb; b0 += U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int a = 64 * (b0 + u) + lane;
+      const bool ok = b0 + u < nb && a < k;
+      int64_t row = base + (ok ? a : 0);
+      if (sub_idx != nullptr) row = sub_idx[row];
+      const double xv = static_cast<double>(x[row]);
+      v[u] = ok ? xv : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u;
+      if (b < nb) {
+        double m = fabs(v[u]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+        int e = 0;
+        if (m > 0.0) frexp(m, &e);
+        e = e < -126 ? -126 : (e > 127 ? 127 : e);
+        const float vs = static_cast<float>(v[u] * ldexp(1.0, -e));  // exact scaling
+        uint16_t* out = xtl + b * NT * 64;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) out[((lane >> 5) * NT + t) * 32 + (lane & 31)] = S::term(vs, t);
+        if (lane == 0) sig[b] = ldexpf(1.0f, e);
+      }
+    }
+  }
+}
+
 // The transposed operand of column block nb, rows 32 kr2 .. 32 kr2 + 31, of the tile in LDS
 // at sb (lane-ordered blocks in symm_slot order): lane l gets T[32 kr2 + 8 (l/16) + e][16 nb
 // + l % 16], e < 8 -- the B operand of x^T T and, read as an A operand, the block of T^T.
@@ -634,14 +677,17 @@ static_assert(kSymmStages >= 2 && SymmStage<3>::LOADS * (kSymmStages - 1) <= 63,
 // T_IJ^T x_I into block J, each row's product sum_J T_IJ x_J into block I), and every row the
 // item touches is written once, at its end: Yp[ypos[s] + q 64 T + a] for the local rows
 // a < min(k, 64 (T - q G)).
-template <typename TM, typename TV>
+// FUSE: the x terms and scales are computed by each item from x itself (TX, gathered through
+// sub_idx) into LDS (symm_pack_lds), instead of read from the pack (sigx, xt).
+template <typename TM, typename TV, bool FUSE = false, typename TX = double>
 __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
     int G, int Tmax, const int32_t* __restrict__ items, const int64_t* __restrict__ sub_ptr,
     const int64_t* __restrict__ blk_ptr, const int64_t* __restrict__ mat_off,
     const int64_t* __restrict__ ypos, const float* __restrict__ scale,
     const TM* __restrict__ mats, const float* __restrict__ sigx,
     const uint16_t* __restrict__ xt, TV* __restrict__ Yp,
-    const dds_pcg_state* __restrict__ gate) {
+    const dds_pcg_state* __restrict__ gate, const TX* __restrict__ xin = nullptr,
+    const int32_t* __restrict__ sub_idx = nullptr) {
   using S = SymMfma<TM>;
   using op_t = typename S::op_t;
   constexpr int NT = S::NT;
@@ -681,23 +727,29 @@ __global__ __launch_bounds__(64) void sym_mfma_apply_kernel(
   if (lane < 4) reinterpret_cast<float*>(zero16)[lane] = 0.f;
   for (int a = lane; a < 64 * (I2max + 1); a += 64) yv[a] = 0.f;
   // the x terms (unless they travel with the tiles) and the scales of blocks 0..I2max
-  // (zeros past the subdomain), then the first tiles
-  const int64_t b0x = blk_ptr[s];
+  // (zeros past the subdomain), then the first tiles; fused, the first tiles, then the terms
+  // and scales computed from x
+  static_assert(!(FUSE && SS::PER_TILE), "fused x terms are kept in LDS for the whole item");
+  const int64_t b0x = FUSE ? 0 : blk_ptr[s];
   const i32x4_t rx = buffer_rsrc(xt + b0x * NT * 64, (uint32_t)(T * SS::XB));
-  const i32x4_t rs = buffer_rsrc(sigx + b0x, (uint32_t)(T * 4));
-  if (!SS::PER_TILE) {
-    const int xdw = (I2max + 1) * SS::XB / 4;  // dwords
-    for (int d0 = 0; d0 < xdw; d0 += 64)
-      dma4(lds_addr(xtl + 4 * d0), (uint32_t)lane * 4u, rx, 4u * (uint32_t)d0);
+  if constexpr (!FUSE) {
+    const i32x4_t rs = buffer_rsrc(sigx + b0x, (uint32_t)(T * 4));
+    if (!SS::PER_TILE) {
+      const int xdw = (I2max + 1) * SS::XB / 4;  // dwords
+      for (int d0 = 0; d0 < xdw; d0 += 64)
+        dma4(lds_addr(xtl + 4 * d0), (uint32_t)lane * 4u, rx, 4u * (uint32_t)d0);
+    }
+    for (int d0 = 0; d0 <= I2max; d0 += 64)
+      dma4(lds_addr(sig + d0), (uint32_t)lane * 4u, rs, 4u * (uint32_t)d0);
   }
-  for (int d0 = 0; d0 <= I2max; d0 += 64)
-    dma4(lds_addr(sig + d0), (uint32_t)lane * 4u, rs, 4u * (uint32_t)d0);
   SymmCursor ic = {0, symm_row(T, p0, p_end, 0), 0};  // issue cursor
 #pragma unroll
   for (int t = 0; t < NF; ++t) {
     symm_issue_at<NT>(stage0 + (uint32_t)(t % NS) * SB, ic, nrows, T, rL, nbL, rm, voff, rx, lane);
     if (ic.i < nrows) ic = symm_advance(ic, T, p0, p_end);
   }
+  if constexpr (FUSE)
+    symm_pack_lds<TM, TX>(I2max + 1, k, base, sub_idx, xin, lane, reinterpret_cast<uint16_t*>(xtl), sig);
   wait_vm<LPT * NF>();  // the terms and scales (issued before every tile) are in
   wave_lds_fence();
 
@@ -1218,6 +1270,61 @@ extern "C" int dds_local_apply_sym_mfma(int mat_dtype, int vec_dtype, int64_t n_
   return dds_sym_mfma_apply_packed(mat_dtype, vec_dtype, n_s, n_items, max_k, pairs_per_item,
                                    items, sub_ptr, blk_ptr, mat_off, ypos, scale, mats, sig, xt,
                                    Yp, gate, stream);
+}
+
+// The apply with the x pack done by each item in its own LDS (FP16 tiles only: their two
+// terms per block fit beside the stages), so no pack pass and no packed x in HBM. The same
+// result bits as the pack followed by the apply.
+extern "C" int dds_local_apply_sym_mfma_fused(int mat_dtype, int vec_dtype, int y_dtype,
+                                              int64_t n_s, int64_t n_items, int64_t max_k,
+                                              int64_t pairs_per_item, const int32_t* items,
+                                              const int64_t* sub_ptr, const int32_t* sub_idx,
+                                              const int64_t* mat_off, const int64_t* ypos,
+                                              const float* scale, const void* mats,
+                                              const void* x, void* Yp,
+                                              const dds_pcg_state* gate, void* stream) {
+  DDS_REQUIRE(n_s >= 0 && n_items >= 0 && n_items <= INT32_MAX,
+              "dds_local_apply_sym_mfma_fused: bad sizes");
+  DDS_REQUIRE(max_k >= 0 && max_k <= kSymMfmaMaxK,
+              "dds_local_apply_sym_mfma_fused: subdomains of at most %d unknowns (got %lld)",
+              kSymMfmaMaxK, (long long)max_k);
+  DDS_REQUIRE(pairs_per_item >= 1 && pairs_per_item <= kSymMfmaMaxK,
+              "dds_local_apply_sym_mfma_fused: pairs_per_item %lld out of range",
+              (long long)pairs_per_item);
+  if (n_items == 0 || n_s == 0) return DDS_OK;
+  DDS_REQUIRE(items && sub_ptr && mat_off && ypos && mats && x && Yp,
+              "dds_local_apply_sym_mfma_fused: null pointer");
+  hipStream_t st = as_stream(stream);
+  const int Tmax = (int)ceil_div(max_k, 64);
+  const int G = (int)pairs_per_item;
+  int dev = 0, max_lds = 0;
+  DDS_HIP(hipGetDevice(&dev));
+  DDS_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+#define DDS_SYMF(TV, TX)                                                                       \
+  do {                                                                                         \
+    const size_t lds = SymmStage<SymMfma<_Float16>::NT>::lds_bytes(kSymmStages, Tmax);         \
+    DDS_REQUIRE(lds <= (size_t)max_lds,                                                        \
+                "dds_local_apply_sym_mfma_fused: %zu bytes of LDS for max_k %lld exceed the "  \
+                "%d per workgroup", lds, (long long)max_k, max_lds);                           \
+    const void* fn = reinterpret_cast<const void*>(&sym_mfma_apply_kernel<_Float16, TV, true, TX>); \
+    if (lds > 65536)                                                                           \
+      DDS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));  \
+    hipLaunchKernelGGL((sym_mfma_apply_kernel<_Float16, TV, true, TX>), dim3((unsigned)n_items), \
+                       dim3(64), lds, st, G, Tmax, items, sub_ptr, nullptr, mat_off, ypos,     \
+                       scale, static_cast<const _Float16*>(mats), nullptr, nullptr,            \
+                       static_cast<TV*>(Yp), gate, static_cast<const TX*>(x), sub_idx);        \
+    DDS_LAUNCHED();                                                                            \
+    return DDS_OK;                                                                             \
+  } while (0)
+  if (mat_dtype == DDS_F16) {
+    if (vec_dtype == DDS_F64 && y_dtype == DDS_F64) DDS_SYMF(double, double);
+    if (vec_dtype == DDS_F64 && y_dtype == DDS_F32) DDS_SYMF(float, double);
+    if (vec_dtype == DDS_F32 && y_dtype == DDS_F32) DDS_SYMF(float, float);
+  }
+#undef DDS_SYMF
+  set_error("dds_local_apply_sym_mfma_fused: unsupported dtypes (mat %d, vec %d, y %d)",
+            mat_dtype, vec_dtype, y_dtype);
+  return DDS_EUNSUPPORTED;
 }
 
 extern "C" int dds_sym_mfma_pack16(int mat_dtype, int64_t n_blk, const int32_t* blk_owner,

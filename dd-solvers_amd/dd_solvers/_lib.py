@@ -66,6 +66,8 @@ _SIGNATURES = {
     "dds_local_apply_sym": [_i, _i, _i, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "dds_local_apply_sym_mfma": [_i, _i, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p,
                                  _i64, _p, _p, _p, _p, _p, _p, _p],
+    "dds_local_apply_sym_mfma_fused": [_i, _i, _i, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p,
+                                       _p, _p, _p, _p, _p, _p],
     "dds_sym_mfma_pack": [_i, _i, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "dds_sym_mfma_apply_packed": [_i, _i, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p,
                                   _p, _p, _p, _p],

@@ -1263,6 +1263,7 @@ class SchwarzOperator(SparseSolver):
         self.coarse_solver = coarse_solver
         self.collect_timings = collect_timings
         self.waves = None
+        self.fused_pack = True  # FP16 symmetric matrix-core apply: the x pack inside its kernel
         self._mfma = None
         self._msym = None
 
@@ -1751,6 +1752,15 @@ class SchwarzOperator(SparseSolver):
             if L.direct and Y is None:
                 raise ValueError("a direct sym_mfma apply writes the result: pass Y")
             out, pos = (Y, self.sub_ptr) if L.direct else (self.Yp, L.ypos)
+            if L.mat_dtype == torch.float16 and self.fused_pack:  # one kernel, pack in LDS
+                _lib.call("dds_local_apply_sym_mfma_fused", _lib.dtype_code(L.mat_dtype),
+                          _lib.dtype_code(self.pi), _lib.dtype_code(out.dtype), self.n_solvers,
+                          L.n_items, L.max_k, L.group, L.items.data_ptr(),
+                          self.sub_ptr.data_ptr(), _lib.ptr(self.sub_idx),
+                          L.mat_off.data_ptr(), pos.data_ptr(), _lib.ptr(L.scale),
+                          L.mats.data_ptr(), x.data_ptr(), out.data_ptr(), _lib.ptr(gate),
+                          _lib.stream())
+                return
             if out.dtype != self.pi:  # FP32 results under FP64 vectors: the two halves
                 self._enqueue_sym_pack(x, gate)
                 self._enqueue_sym_apply_packed(gate, Y)

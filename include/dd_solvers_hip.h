@@ -21,6 +21,8 @@
  *   dds_local_apply_sym_mfma  the same for FP16/BF16 inverses on the matrix cores
  *   dds_sym_mfma_pack, dds_sym_mfma_apply_packed
  *                             its two kernels, launched apart
+ *   dds_local_apply_sym_mfma_fused
+ *                             the same for FP16 in one kernel (the pack in LDS)
  *   dds_local_apply_sym_mfma_noalloc
  *                             the same with its workspace in a static device array (no
  *                             counterpart in the reference)
@@ -235,6 +237,18 @@ int dds_sym_mfma_apply_packed(int mat_dtype, int vec_dtype, int64_t n_s, int64_t
                               const int64_t* mat_off, const int64_t* ypos, const float* scale,
                               const void* mats, const float* sig, const void* xt, void* Yp,
                               const dds_pcg_state* gate, void* stream);
+
+/* dds_local_apply_sym_mfma for FP16 inverses with the x pack done by each workgroup in its
+ * own LDS: one launch, no sig / xt workspace, the same result bits. x is vec_dtype (FP64 or
+ * FP32), Yp is y_dtype (FP64, or FP32 under either x); sub_idx as above. Replaces the same
+ * Inv.solve -> gemv_strided_batched step (solvers.py:355-359). */
+int dds_local_apply_sym_mfma_fused(int mat_dtype, int vec_dtype, int y_dtype, int64_t n_s,
+                                   int64_t n_items, int64_t max_k, int64_t pairs_per_item,
+                                   const int32_t* items, const int64_t* sub_ptr,
+                                   const int32_t* sub_idx, const int64_t* mat_off,
+                                   const int64_t* ypos, const float* scale, const void* mats,
+                                   const void* x, void* Yp, const dds_pcg_state* gate,
+                                   void* stream);
 
 /* dds_local_apply_sym_mfma without a caller workspace: sig and xt live in a static device
  * array of the library (32 MiB per GPU, held until the process ends; its address looked up

@@ -318,6 +318,39 @@ def test_sym_mfma_pair_groups(monkeypatch, storage, group, N, box, overlap):
         assert torch.equal(Y1, Y2)
 
 
+@pytest.mark.parametrize("pi", [torch.float64, torch.float32])
+@pytest.mark.parametrize("group", [1, "all"])
+@pytest.mark.parametrize("N,box,overlap", [(18, 8, 1), (20, 8, 0)])
+def test_sym_mfma_fused_pack_same_bits(monkeypatch, pi, group, N, box, overlap):
+    """FP16 tiles: the apply that packs x in its own LDS gives the bits of the pack kernel
+    followed by the apply, for FP64 and FP32 vectors, with and without overlap (gathered x)."""
+    pb = P.make_problem("q1_3d", N, box=box, overlap=overlap, seed=5, lognormal=True)
+    d = pb.decomposition
+    sizes = np.diff(d.sub_ptr)
+    P_max = -(-(-(-int(sizes.max()) // 64)) // 2)
+    G = P_max if group == "all" else group
+    print(f"sym_mfma fused pack: pi {pi} G {G} N {N} box {box} overlap {overlap}", flush=True)
+    monkeypatch.setattr(dds.solvers, "_symm_group", lambda sizes, n_cu: G)
+    M = dds.AdditiveSchwarz(pi, dds.Inv(torch.float16, scaled=True, accumulate=torch.float32,
+                                        layout="sym_mfma"), dds.DenseCholesky())
+    M.setup(_torch_csr(pb.A), decomposition=d)
+    L = M.local
+    assert L.layout == "sym_mfma" and L.direct == (G >= P_max)
+    rt = torch.as_tensor(np.random.default_rng(2).standard_normal(pb.A.shape[0]), device=dev).to(pi)
+    Yst = M.Y if M.Y is not None else torch.empty_like(rt)
+    out = []
+    for fused in (True, False):
+        M.fused_pack = fused
+        Y = torch.full_like(Yst, float("nan"))
+        M._enqueue_sym_apply(rt, None, Y)
+        if not L.direct:
+            M._enqueue_sym_combine(Y, None)
+        torch.cuda.synchronize()
+        assert not torch.isnan(Y).any()
+        out.append(Y)
+    assert torch.equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("kind,N", [("q1_3d", 24), ("p1_2d", 64), ("q1_3d", 5)])
 @pytest.mark.parametrize("fmt", ["sell", "csr"])
 def test_spmv_formats_with_dot(kind, N, fmt):
