@@ -777,11 +777,30 @@ class CG(SparseSolver):
         if self.use_graph:
             if self._graph is None:
                 self._capture(x, buf)
+            # Blocks of check_every iterations; once the last block's convergence rate puts
+            # the end within half a block, blocks of check_every / 4 (fewer gated launches
+            # after convergence, at one more state read per short block).
+            tail = self._graph_tail is not None
+            short, prev = False, None
             while enqueued < maxiter:
-                self._graph.replay()
-                enqueued += self.check_every
-                if int(done_flag.item()):
+                if short:
+                    self._graph_tail.replay()
+                    enqueued += self.check_every // 4
+                else:
+                    self._graph.replay()
+                    enqueued += self.check_every
+                if not tail:
+                    if int(done_flag.item()):
+                        break
+                    continue
+                stv = _read_state(buf.state)
+                if int(stv["done"]):
                     break
+                it, rr = int(stv["it"]), float(stv["rr"])
+                if prev is not None and it > prev[0] and 0.0 < rr < prev[1]:
+                    per_it = 0.5 * np.log(rr / prev[1]) / (it - prev[0])  # log norm ratio
+                    short = np.log(atol / np.sqrt(rr)) / per_it <= self.check_every / 2
+                prev = (it, rr)
         else:
             while enqueued < maxiter:
                 k = min(self.check_every, maxiter - enqueued)
@@ -808,18 +827,22 @@ class CG(SparseSolver):
         return out, meta
 
     def _capture(self, x, buf):
-        """Capture `check_every` iterations on a side stream into a hipGraph."""
-        g = torch.cuda.CUDAGraph()
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        # Capture with the solve already converged would record nothing different: every
-        # kernel checks the gate at run time, not at capture time.
-        with torch.cuda.stream(side):
-            with torch.cuda.graph(g, stream=side):
-                for _ in range(self.check_every):
-                    self._enqueue_iteration(x, buf)
-        torch.cuda.current_stream().wait_stream(side)
-        self._graph = g
+        """Capture `check_every` iterations on a side stream into a hipGraph, and (for
+        check_every >= 4) check_every / 4 iterations into a second one for the last blocks."""
+        def capture(k):
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            # Capture with the solve already converged would record nothing different: every
+            # kernel checks the gate at run time, not at capture time.
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(g, stream=side):
+                    for _ in range(k):
+                        self._enqueue_iteration(x, buf)
+            torch.cuda.current_stream().wait_stream(side)
+            return g
+        self._graph = capture(self.check_every)
+        self._graph_tail = capture(self.check_every // 4) if self.check_every >= 4 else None
 
     def _solve_generic(self, rhs, x0):
         """Reference control flow with a host check per iteration, for preconditioners and

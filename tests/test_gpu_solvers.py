@@ -254,11 +254,15 @@ def test_setup_again_with_larger_problem(use_graph):
         assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("check_every", [4, 8, 3])
+def test_graph_replay_matches_eager(check_every):
+    """Captured blocks (with the short blocks near convergence when check_every >= 4) give
+    the eager solve's iterations, residual norms and bits."""
     pb = P.make_problem("q1_3d", 16, box=4, overlap=1, seed=4, lognormal=True)
-    x1, m1 = _parity(pb, "hybrid", check_every=4)
-    x2, m2 = _parity(pb, "hybrid", check_every=4, use_graph=True)
+    x1, m1 = _parity(pb, "hybrid", check_every=check_every)
+    x2, m2 = _parity(pb, "hybrid", check_every=check_every, use_graph=True)
     assert m1["iterations"] == m2["iterations"]
+    assert m1["residual norms"] == m2["residual norms"]
     assert torch.equal(x1, x2)
 
 
