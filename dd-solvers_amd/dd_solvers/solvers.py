@@ -500,6 +500,7 @@ class CG(SparseSolver):
         self.matrix = None
         self._buf = None
         self._graph = None
+        self._graph_tail = None
 
     def __str__(self):
         return f"CG({self.preconditioner or ''}, bsr_matmul={self.bsr_matmul})"
@@ -522,6 +523,7 @@ class CG(SparseSolver):
         # captured graphs hold the addresses of the previous operator's arrays and buffers
         self._buf = None
         self._graph = None
+        self._graph_tail = None
         self._bbuf = None
         self._bgraph = None
         return meta
@@ -891,6 +893,7 @@ class CG(SparseSolver):
         self.matrix = None
         self._buf = None
         self._graph = None
+        self._graph_tail = None
         self._bbuf = None
         self._bgraph = None
 
